@@ -1,0 +1,214 @@
+"""Throughput benchmark of the SPH step on one MI355X.
+
+Workload: Evrard collapse, N = 1,048,576 equal-mass gas particles (M = 1,
+R = 1, G = 1, rho ~ 1/r from r = R sqrt(U), seeded isotropic directions,
+u = 0.05, gamma = 5/3), cubic spline, N_ngb = 64 (FP64 Newton h solve),
+theta = 0.5, Plummer eps = 1e-3, alpha = 1 with the Balsara switch,
+dt = 1e-3.  One "step" is the full KDK step of sph_serial_v2.py on the
+device: kick/drift, Morton sort, tree, h solve, CSR neighbours, density+EOS,
+tree gravity, div/curl, acceleration, kick, du/dt, u update.
+
+Prints ONE JSON line: particle-steps/s (whole job, inputs resident in HBM),
+the roofline of the dominant kernel from HIP events on the library's stream,
+the all-FP64 form of the same kernels, and the CPU baseline (the oracle's
+FP64 all-pairs restatement of the reference step, row-sampled).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FLOP_PER_INTERACTION = 20
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def per_particle_bytes(kbar):
+    """Algorithmic (compulsory) HBM bytes per particle of each CSR pass with
+    this build's layout: int32 CSR index stream (4 B per neighbour), int64 row
+    offsets, FP64 x,y,z / v / m / h, FP32 hydro scalars (mixed form)."""
+    idx = 4.0 * kbar + 8.0
+    return {
+        "density": idx + 24 + 8 + 8 + 8 + 4 * 4,            # x,m,h,u in; rho,P,c,P/rho^2 out
+        "divcurl": idx + 24 + 24 + 8 + 8 + 4 + 4 + 3 * 4,    # x,v,m,h,rho,c in; divV,curlV,f out
+        "accsph": idx + 24 + 24 + 8 + 8 + 4 * 4 + 24 + 24,   # x,v,m,h,rho,P/rho^2,c,f,a_grav in; a out
+        "dudt": idx + 24 + 24 + 8 + 8 + 4 * 4 + 4,           # ... in; du out
+    }
+
+
+def cpu_baseline(sim_state, n, m, rows, seed=0):
+    """Time the oracle's FP64 all-pairs restatement of the reference step
+    (libsx.h_smooth_fast, getDensity, getAcc_g_smth, shear.div_curlVel,
+    getAcc_sph_shear, get_dU_shear) on `rows` sampled target rows against all
+    n sources; extrapolate linearly in rows (rows are independent)."""
+    import oracle as O
+
+    O.lib()
+    rng = np.random.default_rng(seed)
+    pos = sim_state["pos"]; vel = sim_state["vel"]; h = sim_state["h"]; u = sim_state["u"]
+    rho = sim_state["rho"]
+    P = (2.0 / 3.0) * rho * u
+    c = np.sqrt(10.0 / 9.0 * u)
+    eps = np.full(n, 1e-3)
+    # sampled rows are contiguous after a random shuffle of the particle order,
+    # which keeps the reference's row-range interface
+    perm = rng.permutation(n)
+    pos_p, vel_p, h_p, m_p = pos[perm], vel[perm], h[perm], m[perm]
+    rho_p, P_p, c_p = rho[perm], P[perm], c[perm]
+    dv = np.zeros(n); cv = np.zeros(n)
+    t = {}
+    t0 = time.perf_counter()
+    O.h_smooth_fast(pos_p, h_p * (50.0 / 64.0) ** (1 / 3), nbeg=0, nend=rows)
+    t["h_smooth_fast"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.density_allpairs(pos_p, m_p, h_p, 0, rows)
+    t["getDensity"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.grav_direct(pos_p, m_p, eps, 1.0, O.SOFT_SPLINE, 0.0, 0, rows)
+    # the reference's symmetric j > i loop does half the pairs per row on average
+    t["getAcc_g_smth"] = 0.5 * (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    O.divcurl_allpairs(pos_p, vel_p, rho_p, m_p, h_p, 0, rows)
+    t["div_curlVel"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.accsph_allpairs(pos_p, vel_p, rho_p, P_p, c_p, h_p, m_p, 1.0, dv, cv, 0, rows)
+    t["getAcc_sph_shear"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.dudt_allpairs(pos_p, vel_p, rho_p, P_p, c_p, h_p, m_p, 1.0, dv, cv, 0, rows)
+    t["get_dU_shear"] = time.perf_counter() - t0
+    sample_s = sum(t.values())
+    step_s = sample_s * n / rows
+    return step_s, sample_s, t
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1048576)
+    ap.add_argument("--cpu-rows", type=int, default=512)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64-form", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding is a "
+              "design in DESIGN.md, not a measured path", file=sys.stderr)
+        sys.exit(2)
+
+    from sph_3_amd import ic
+    from sph_3_amd.step import DeviceSPH, StepConfig
+
+    n = args.n
+    d = ic.evrard(n, seed=42)
+    cfg = StepConfig(dt=1e-3, gamma=5 / 3, alpha=1.0, G=1.0, theta=0.5, softening="plummer", eps_plummer=1e-3,
+                     n_ngb=64.0, h_tol=1e-10, precision="mixed", balsara=True)
+
+    def run(precision, steps, warmup):
+        cfg.precision = precision
+        sim = DeviceSPH(d["pos"], d["vel"], d["m"], d["u"], d["h0"], cfg)
+        log(f"[bench] {precision}: prime")
+        sim.prime()
+        for w in range(warmup):
+            sim.step()
+            log(f"[bench] {precision}: warmup step {w + 1}/{warmup} done")
+        sim.reset_stats()
+        try:
+            import torch
+
+            torch.cuda.synchronize()
+        except Exception:
+            pass
+        t0 = time.perf_counter()
+        for k in range(steps):
+            sim.step(timing=True)
+        try:
+            import torch
+
+            torch.cuda.synchronize()
+        except Exception:
+            pass
+        dt = time.perf_counter() - t0
+        log(f"[bench] {precision}: {steps} timed steps in {dt:.3f} s")
+        return sim, dt
+
+    sim, dt = run("mixed", args.steps, args.warmup)
+    st = sim.stats()
+    ms_step = 1e3 * dt / args.steps
+    value = n * args.steps / dt
+    phases = {k: v / max(1, st["phase_calls"][k]) for k, v in st["phase_ms"].items() if st["phase_calls"][k]}
+    kbar = st["ngb_total"] / n
+    bytes_pp = per_particle_bytes(kbar)
+    # dominant kernel among the measured phases
+    dom = max(phases, key=phases.get)
+    if dom == "gravity":
+        flops = FLOP_PER_INTERACTION * (st["grav_pp"] + st["grav_pn"])
+        ach = flops / (phases[dom] * 1e-3) / 1e12
+        roof = {"kernel": "k_gravity<float>", "bound": "valu", "achieved": ach, "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": ach / FP32_PEAK_TFLOPS, "traffic": None,
+                "algorithmic": f"{FLOP_PER_INTERACTION} flop x {st['grav_pp'] + st['grav_pn']} interactions"}
+    else:
+        b = bytes_pp.get(dom)
+        if b is None:
+            b = 0.0
+        ach = b * n / (phases[dom] * 1e-3) / 1e9
+        roof = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ach / HBM_PEAK_GBS, "traffic": None}
+    # HBM fractions of the CSR passes (reported beside the dominant kernel)
+    hbm = {k: {"ms": phases[k], "GB/s": bytes_pp[k] * n / (phases[k] * 1e-3) / 1e9}
+           for k in bytes_pp if k in phases}
+    grav_tflops = None
+    if "gravity" in phases:
+        grav_tflops = FLOP_PER_INTERACTION * (st["grav_pp"] + st["grav_pn"]) / (phases["gravity"] * 1e-3) / 1e12
+    out = {
+        "metric": "SPH particle-steps/sec on 1 GPU, mixed-precision vs FP64, at <=1e-3 accel error",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32",
+        "precision": "mixed: f32 pair/gravity arithmetic with Kahan f32 sums; f64 positions, velocities, "
+                     "centres of mass, h solve, reductions",
+        "data": "synthetic (seeded Evrard sphere)",
+        "config": {"workload": "evrard_collapse", "n": n, "M": 1.0, "R": 1.0, "G": 1.0, "u0": 0.05,
+                   "gamma": 5 / 3, "n_ngb": 64, "theta": 0.5, "softening": "plummer", "eps": 1e-3,
+                   "alpha": 1.0, "balsara": True, "dt": 1e-3},
+        "phases_ms": phases,
+        "neighbours_per_particle": kbar,
+        "gravity_interactions_per_particle": (st["grav_pp"] + st["grav_pn"]) / n,
+        "gravity_tflops": grav_tflops,
+        "csr_pass_bandwidth": hbm,
+        "roofline": roof,
+    }
+    if not args.no_fp64_form:
+        sim64, dt64 = run("fp64", max(2, args.steps // 2), 1)
+        out["fp64_form"] = {"ms_per_step": 1e3 * dt64 / max(2, args.steps // 2),
+                            "mixed_speedup": (1e3 * dt64 / max(2, args.steps // 2)) / ms_step}
+        del sim64
+    if not args.no_cpu_baseline:
+        g = sim.get()
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        log(f"[bench] cpu baseline: {args.cpu_rows} rows x {n} sources on {threads} threads")
+        step_s, sample_s, parts = cpu_baseline(g, n, d["m"], args.cpu_rows)
+        out["cpu_baseline"] = {
+            "value": n / step_s, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{args.cpu_rows} random target rows x all {n} sources of the FP64 all-pairs restatement "
+                      f"of sph_serial_v2.py's step (h_smooth_fast, getDensity, getAcc_g_smth, div_curlVel, "
+                      f"getAcc_sph_shear, get_dU_shear); {sample_s:.1f} s measured, extrapolated x{n / args.cpu_rows:.0f}",
+            "step_seconds_extrapolated": step_s,
+            "function_seconds_sample": parts,
+        }
+        out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

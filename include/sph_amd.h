@@ -1,0 +1,162 @@
+/*
+ * sph_amd.h -- C ABI of the MI355X (gfx950) SPH step library libsph_amd.so.
+ *
+ * Drop-in boundary for the per-step hot path of hassanfv/SPH_3: the Python
+ * time loop (sph_code/sph_serial_v2.py:148-218) calls functions of
+ * sph_code/libsx.py and sph_code/shear.py; the package sph_3_amd exposes the
+ * same Python names and binds these entry points with ctypes.
+ *
+ * Conventions
+ *   - plain C types only; every function returns an int status (SPH_OK = 0);
+ *     on failure sph_last_error(ctx) describes it.  Nothing here exits or
+ *     aborts the process.
+ *   - pos/vel/acc arrays are N x 3 row-major FP64 (numpy C order), scalar
+ *     arrays are FP64 of length N, all in the caller's particle order.  The
+ *     library never keeps or frees caller memory; results are written into
+ *     caller buffers before the call returns.
+ *   - one context per host thread; a context owns one HIP stream and is not
+ *     re-entrant.  Device data are cached inside the context between calls
+ *     (positions, smoothing lengths, masses are compared with the previous
+ *     call's), so a time step that calls density, div/curl, acceleration and
+ *     du/dt with the same positions sorts, builds the tree and the neighbour
+ *     lists once.
+ *   - precision: SPH_PREC_MIXED = FP32 pair arithmetic with Kahan-compensated
+ *     FP32 sums, FP64 positions/velocities/centres of mass/reductions/h solve;
+ *     SPH_PREC_FP64 = the same kernels with FP64 pair arithmetic.
+ */
+#ifndef SPH_AMD_H
+#define SPH_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sph_ctx sph_ctx;
+
+enum sph_status {
+    SPH_OK = 0,
+    SPH_ERR_ARG = 1,      /* invalid argument (null pointer, n < 1, bad enum) */
+    SPH_ERR_HIP = 2,      /* HIP runtime error */
+    SPH_ERR_NOCONV = 3,   /* an iterative solve did not converge */
+    SPH_ERR_CAPACITY = 4, /* caller buffer too small (required size reported) */
+    SPH_ERR_STATE = 5     /* call out of sequence (e.g. sph_step before sph_state_set) */
+};
+
+enum sph_precision { SPH_PREC_MIXED = 0, SPH_PREC_FP64 = 1 };
+enum sph_softening { SPH_SOFT_SPLINE = 0, SPH_SOFT_PLUMMER = 1 };
+enum sph_hmode {
+    SPH_H_NEWTON = 0,    /* FP64 Newton-Raphson: (4pi/3)(2h)^3 sum_j W(r_ij,h) = n_ngb */
+    SPH_H_REF_COUNT = 1, /* sph_code/libsx.py:804-845 h_smooth_fast: +-0.3 % steps to n_low..n_up */
+    SPH_H_KTH = 2        /* sph_code/libsx.py:752-774 do_smoothingX: h = 0.5 * (kth+1)-th distance */
+};
+
+/* ---- context ------------------------------------------------------------ */
+int sph_ctx_create(int device, sph_ctx **out);
+int sph_ctx_destroy(sph_ctx *ctx);
+const char *sph_last_error(const sph_ctx *ctx);
+int sph_version(void);
+
+/* ---- per-function entry points (the reference's Python call surface) ---- */
+
+/* Smoothing lengths.  Replaces sph_code/libsx.py:804 h_smooth_fast(pos, h)
+ * (mode SPH_H_REF_COUNT, n_low/n_up = 45/55), libsx.py:754 do_smoothingX
+ * ((pos, targets)) (mode SPH_H_KTH, param = 50), and the device h update
+ * EV/lib.h:497 smoothing_h_ngb (mode SPH_H_NEWTON, param = n_ngb).
+ * targets may be NULL (= pos, m = n); h_in may be NULL for SPH_H_KTH.
+ * *n_unconverged counts targets that hit max_steps (mode 1) or did not reach
+ * tol (mode 0). */
+int sph_smoothing_length(sph_ctx *ctx, int64_t n, const double *pos, int64_t m, const double *targets,
+                         const double *h_in, int mode, double param, int n_low, int n_up, int max_steps,
+                         double tol, double *h_out, int64_t *n_unconverged);
+
+/* Neighbour lists S_i = { j : |r_i - r_j|^2 < (h_i + h_j)^2 } (self included,
+ * FP64, rows ascending).  Replaces EV/ngb.h:125 ngbDB_new_v3.  offsets has
+ * n+1 entries and is always written; idx is written when idx_cap >=
+ * offsets[n], else SPH_ERR_CAPACITY is returned (idx may be NULL to query). */
+int sph_neighbours(sph_ctx *ctx, int64_t n, const double *pos, const double *h, int64_t *offsets,
+                   int32_t *idx, int64_t idx_cap);
+
+/* rho_i = sum_j m_j W(r_ij, (h_i+h_j)/2).  Replaces libsx.py:11 getDensity. */
+int sph_density(sph_ctx *ctx, int64_t n, const double *pos, const double *m, const double *h, int precision,
+                double *rho_out);
+
+/* |div v| and |curl v|.  Replaces shear.py:8 div_curlVel(pos, v, rho, m, h). */
+int sph_div_curl(sph_ctx *ctx, int64_t n, const double *pos, const double *vel, const double *rho,
+                 const double *m, const double *h, int precision, double *divV_out, double *curlV_out);
+
+/* Pressure + artificial-viscosity acceleration.  divV/curlV NULL: replaces
+ * libsx.py:102 getAcc_sph; non-NULL (Balsara switch): replaces shear.py:151
+ * getAcc_sph_shear.  acc_out is n x 3. */
+int sph_acc_hydro(sph_ctx *ctx, int64_t n, const double *pos, const double *vel, const double *rho,
+                  const double *P, const double *c, const double *h, const double *m, const double *divV,
+                  const double *curlV, double alpha, int precision, double *acc_out);
+
+/* du/dt.  Replaces libsx.py:355 get_dU (divV/curlV NULL) and shear.py:334
+ * get_dU_shear. */
+int sph_dudt(sph_ctx *ctx, int64_t n, const double *pos, const double *vel, const double *rho, const double *P,
+             const double *c, const double *h, const double *m, const double *divV, const double *curlV,
+             double alpha, int precision, double *dudt_out);
+
+/* Self-gravity by tree walk.  Replaces libsx.py:625 getAcc_g_smth(pos, mass,
+ * G, epsilon) (soft_kind SPH_SOFT_SPLINE, eps_ij = (eps_i+eps_j)/2, theta = 0
+ * gives its exact direct sum) and EV/bh.h:454 compute_forces_kernel.  eps may
+ * be NULL (zeros).  n_interactions[0..1] = particle-particle, particle-node
+ * (may be NULL).  acc_out is n x 3. */
+int sph_gravity(sph_ctx *ctx, int64_t n, const double *pos, const double *m, const double *eps, double G,
+                double theta, int soft_kind, double eps_plummer, int precision, double *acc_out,
+                int64_t *n_interactions);
+
+/* out5 = [KE, px, py, pz, M] (FP64).  Replaces libsx.py:571 getKE. */
+int sph_kinetics(sph_ctx *ctx, int64_t n, const double *vel, const double *m, double *out5);
+
+/* ---- device-resident time loop (sph_serial_v2.py:148-218 on the GPU) ---- */
+typedef struct sph_step_params {
+    double dt;          /* fixed step */
+    double gamma;       /* adiabatic index */
+    double alpha;       /* AV coefficient */
+    double G;           /* gravitational constant (0: no gravity) */
+    double theta;       /* opening angle */
+    double eps_plummer; /* Plummer softening (soft_kind = SPH_SOFT_PLUMMER) */
+    double n_ngb;       /* Newton h solve target */
+    double h_tol;       /* Newton relative tolerance */
+    int soft_kind;      /* SPH_SOFT_SPLINE uses eps_i = h_i (EV/lib.h:584) */
+    int precision;      /* SPH_PREC_MIXED / SPH_PREC_FP64 */
+    int balsara;        /* 1: shear.py forms, 0: libsx.py forms */
+    int timing;         /* 1: record per-phase HIP-event times */
+} sph_step_params;
+
+typedef struct sph_stats {
+    int64_t n, steps;
+    int64_t ngb_total;       /* CSR entries of the last neighbour build */
+    int64_t grav_pp, grav_pn; /* interactions of the last gravity pass */
+    int64_t h_unconverged;   /* cumulative */
+    int64_t h_rounds;        /* rounds of the last h solve */
+    double phase_ms[16];     /* accumulated HIP-event milliseconds per phase */
+    int64_t phase_calls[16];
+} sph_stats;
+
+/* phase indices of sph_stats.phase_ms */
+enum sph_phase {
+    SPH_PH_KICKDRIFT = 0, SPH_PH_ORDER, SPH_PH_TOPOLOGY, SPH_PH_HSOLVE, SPH_PH_MOMENTS, SPH_PH_NGB,
+    SPH_PH_DENSITY, SPH_PH_GRAVITY, SPH_PH_DIVCURL, SPH_PH_ACCSPH, SPH_PH_KICK, SPH_PH_DUDT,
+    SPH_PH_UUPDATE, SPH_PH_COUNT
+};
+
+/* Load the resident state (caller order); u and h are per particle. */
+int sph_state_set(sph_ctx *ctx, int64_t n, const double *pos, const double *vel, const double *m,
+                  const double *u, const double *h);
+/* Read it back in caller order; any pointer may be NULL. */
+int sph_state_get(sph_ctx *ctx, double *pos, double *vel, double *u, double *h, double *rho, double *acc);
+/* Forces at the current positions and the first u update (sph_serial_v2.py:94-143). */
+int sph_prime(sph_ctx *ctx, const sph_step_params *p);
+/* One KDK step (sph_serial_v2.py:148-203), entirely on the device. */
+int sph_step(sph_ctx *ctx, const sph_step_params *p);
+int sph_get_stats(sph_ctx *ctx, sph_stats *out);
+int sph_reset_stats(sph_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPH_AMD_H */

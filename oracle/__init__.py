@@ -1,0 +1,302 @@
+"""FP64 CPU oracle for the SPH hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this package.  It is the checker, never the thing that
+is measured or shipped: the product path (``sph_3_amd``) does not import it and
+fails loudly when its HIP library is missing.
+
+The C restatement lives in ``sph_oracle.c`` (built by ``oracle/Makefile`` into
+``liboracle.so``); this module wraps it with numpy.  Functions in two groups:
+
+* literal restatements of ``sph_code/libsx.py`` and ``sph_code/shear.py``
+  (all-pairs, FP64, row range ``[nbeg, nend)`` as in the ``*_mpi`` twins);
+* the build's definitions the device path must reproduce: Morton order and
+  binary radix tree, symmetric CSR neighbour sets, Newton-Raphson smoothing
+  length, tree gravity with a per-particle opening test, CSR-restricted sums.
+
+Parity pins for every function are listed in DESIGN.md's oracle table and
+live in ``tests/test_oracle_*.py``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_d = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_i64 = ctypes.c_int64
+
+
+def build() -> str:
+    """Compile liboracle.so in place (gcc + OpenMP)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return os.path.join(_HERE, "liboracle.so")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.or_ke.restype = ctypes.c_double
+        L.or_pe.restype = ctypes.c_double
+        L.or_h_newton.restype = ctypes.c_int64
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    if a is None:
+        return None
+    if a.dtype == np.float64:
+        return a.ctypes.data_as(_d)
+    if a.dtype == np.int64:
+        return a.ctypes.data_as(_i64p)
+    if a.dtype == np.int32:
+        return a.ctypes.data_as(_i32p)
+    if a.dtype == np.uint64:
+        return a.ctypes.data_as(_u64p)
+    raise TypeError(a.dtype)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+def _rows(N, nbeg, nend):
+    nbeg = 0 if nbeg is None else int(nbeg)
+    nend = N if nend is None else int(nend)
+    return nbeg, nend
+
+
+# ---------------------------------------------------------------- all-pairs
+def density_allpairs(pos, m, h, nbeg=None, nend=None):
+    pos = _f64(pos, (-1, 3)); N = len(pos)
+    nbeg, nend = _rows(N, nbeg, nend)
+    out = np.zeros(nend - nbeg)
+    lib().or_density_allpairs(_i64(N), _i64(nbeg), _i64(nend), _p(pos), _p(_f64(m)), _p(_f64(h)), _p(out))
+    return out
+
+
+def divcurl_allpairs(pos, v, rho, m, h, nbeg=None, nend=None):
+    pos = _f64(pos, (-1, 3)); N = len(pos)
+    nbeg, nend = _rows(N, nbeg, nend)
+    dv = np.zeros(nend - nbeg); cv = np.zeros(nend - nbeg)
+    lib().or_divcurl_allpairs(_i64(N), _i64(nbeg), _i64(nend), _p(pos), _p(_f64(v, (-1, 3))),
+                              _p(_f64(rho)), _p(_f64(m)), _p(_f64(h)), _p(dv), _p(cv))
+    return dv, cv
+
+
+def accsph_allpairs(pos, v, rho, P, c, h, m, alpha, divV=None, curlV=None, nbeg=None, nend=None):
+    pos = _f64(pos, (-1, 3)); N = len(pos)
+    nbeg, nend = _rows(N, nbeg, nend)
+    bal = divV is not None
+    dv = _f64(divV) if bal else np.zeros(N)
+    cv = _f64(curlV) if bal else np.zeros(N)
+    out = np.zeros((nend - nbeg, 3))
+    lib().or_accsph_allpairs(_i64(N), _i64(nbeg), _i64(nend), _p(pos), _p(_f64(v, (-1, 3))), _p(_f64(rho)),
+                             _p(_f64(P)), _p(_f64(c)), _p(_f64(h)), _p(_f64(m)), _p(dv), _p(cv),
+                             ctypes.c_double(alpha), ctypes.c_int(int(bal)), _p(out))
+    return out
+
+
+def dudt_allpairs(pos, v, rho, P, c, h, m, alpha, divV=None, curlV=None, nbeg=None, nend=None):
+    pos = _f64(pos, (-1, 3)); N = len(pos)
+    nbeg, nend = _rows(N, nbeg, nend)
+    bal = divV is not None
+    dv = _f64(divV) if bal else np.zeros(N)
+    cv = _f64(curlV) if bal else np.zeros(N)
+    out = np.zeros(nend - nbeg)
+    lib().or_dudt_allpairs(_i64(N), _i64(nbeg), _i64(nend), _p(pos), _p(_f64(v, (-1, 3))), _p(_f64(rho)),
+                           _p(_f64(P)), _p(_f64(c)), _p(_f64(h)), _p(_f64(m)), _p(dv), _p(cv),
+                           ctypes.c_double(alpha), ctypes.c_int(int(bal)), _p(out))
+    return out
+
+
+SOFT_SPLINE, SOFT_PLUMMER = 0, 1
+
+
+def grav_direct(pos, m, eps, G=1.0, soft_kind=SOFT_SPLINE, eps_plummer=0.0, nbeg=None, nend=None):
+    pos = _f64(pos, (-1, 3)); N = len(pos)
+    nbeg, nend = _rows(N, nbeg, nend)
+    eps = np.zeros(N) if eps is None else np.broadcast_to(_f64(eps), (N,)).copy()
+    out = np.zeros((nend - nbeg, 3))
+    lib().or_grav_direct(_i64(N), _i64(nbeg), _i64(nend), _p(pos), _p(_f64(m)), _p(eps), ctypes.c_double(G),
+                         ctypes.c_int(soft_kind), ctypes.c_double(eps_plummer), _p(out))
+    return out
+
+
+def ke(v, m):
+    v = _f64(v, (-1, 3))
+    return lib().or_ke(_i64(len(v)), _p(v), _p(_f64(m)))
+
+
+def pe(pos, m, G, eps):
+    pos = _f64(pos, (-1, 3))
+    return lib().or_pe(_i64(len(pos)), _p(pos), _p(_f64(m)), ctypes.c_double(G), _p(_f64(eps)))
+
+
+def h_initial(pos, sub=None, k=50):
+    pos = _f64(pos, (-1, 3))
+    sub = pos if sub is None else _f64(sub, (-1, 3))
+    out = np.zeros(len(sub))
+    lib().or_h_initial(_i64(len(pos)), _p(pos), _i64(len(sub)), _p(sub), _i64(k), _p(out))
+    return out
+
+
+def h_smooth_fast(pos, h, n_low=45, n_up=55, max_iter=100000, nbeg=None, nend=None):
+    pos = _f64(pos, (-1, 3)); N = len(pos)
+    nbeg, nend = _rows(N, nbeg, nend)
+    out = np.zeros(nend - nbeg)
+    bad = np.zeros(1, np.int64)
+    lib().or_h_smooth_fast(_i64(N), _i64(nbeg), _i64(nend), _p(pos), _p(_f64(h)), _i64(n_low), _i64(n_up),
+                           _i64(max_iter), _p(out), _p(bad))
+    return out, int(bad[0])
+
+
+# ------------------------------------------------------------- build's defs
+def neighbours_brute(pos, h):
+    """CSR (offsets int64 [N+1], idx int32) of S_i = {j : r_ij^2 < (h_i+h_j)^2}."""
+    pos = _f64(pos, (-1, 3)); N = len(pos); h = _f64(h)
+    off = np.zeros(N + 1, np.int64)
+    lib().or_neighbours_brute(_i64(N), _p(pos), _p(h), _p(off), None)
+    idx = np.zeros(int(off[-1]), np.int32)
+    lib().or_neighbours_brute(_i64(N), _p(pos), _p(h), _p(off), _p(idx))
+    return off, idx
+
+
+def morton_keys(pos):
+    pos = _f64(pos, (-1, 3))
+    keys = np.zeros(len(pos), np.uint64)
+    box = np.zeros(6)
+    lib().or_morton_keys(_i64(len(pos)), _p(pos), _p(keys), _p(box))
+    return keys, box
+
+
+def sort_keys(keys):
+    keys = np.ascontiguousarray(keys, np.uint64)
+    perm = np.zeros(len(keys), np.int32)
+    lib().or_sort_keys(_i64(len(keys)), _p(keys), _p(perm))
+    return perm
+
+
+class Tree:
+    """Binary radix tree over Morton-sorted particles (host FP64 twin of the
+    device build).  ``perm[s]`` is the input index of sorted particle s."""
+
+    def __init__(self, pos, m=None, h=None, eps=None, bucket=16):
+        pos = _f64(pos, (-1, 3)); N = len(pos)
+        self.N, self.bucket = N, int(bucket)
+        keys, self.box6 = morton_keys(pos)
+        self.perm = sort_keys(keys)
+        self.keys = keys[self.perm]
+        nn = max(2 * N - 1, 1)
+        self.first = np.zeros(nn, np.int32); self.count = np.zeros(nn, np.int32)
+        self.left = np.zeros(max(N - 1, 1), np.int32); self.right = np.zeros(max(N - 1, 1), np.int32)
+        self.escape = np.zeros(nn, np.int32)
+        lib().or_tree_topology(_i64(N), _p(self.keys), _p(self.first), _p(self.count), _p(self.left),
+                               _p(self.right), _p(self.escape))
+        self.spos = np.ascontiguousarray(pos[self.perm])
+        self.sm = np.ascontiguousarray(_f64(m)[self.perm]) if m is not None else np.ones(N) / N
+        self.sh = np.ascontiguousarray(_f64(h)[self.perm]) if h is not None else np.zeros(N)
+        self.seps = (np.ascontiguousarray(np.broadcast_to(_f64(eps), (N,))[self.perm])
+                     if eps is not None else np.zeros(N))
+        self.nbox = np.zeros((nn, 6)); self.com = np.zeros((nn, 3)); self.mass = np.zeros(nn)
+        self.hmax = np.zeros(nn); self.epsmax = np.zeros(nn); self.size = np.zeros(nn)
+        lib().or_tree_moments(_i64(N), _p(self.spos), _p(self.sm), _p(self.sh), _p(self.seps), _p(self.left),
+                              _p(self.right), _p(self.nbox), _p(self.com), _p(self.mass), _p(self.hmax),
+                              _p(self.epsmax), _p(self.size))
+
+    def gravity(self, theta=0.5, G=1.0, soft_kind=SOFT_SPLINE, eps_plummer=0.0):
+        """Returns (acc in INPUT order, (n_pp, n_pn))."""
+        acc = np.zeros((self.N, 3)); n = np.zeros(2, np.int64)
+        lib().or_grav_tree(_i64(self.N), _p(self.spos), _p(self.sm), _p(self.seps), _p(self.first),
+                           _p(self.count), _p(self.left), _p(self.escape), _p(self.nbox), _p(self.com),
+                           _p(self.mass), _p(self.epsmax), _p(self.size), ctypes.c_int(self.bucket),
+                           ctypes.c_double(theta), ctypes.c_double(G), ctypes.c_int(soft_kind),
+                           ctypes.c_double(eps_plummer), _p(acc), _p(n))
+        out = np.zeros_like(acc); out[self.perm] = acc
+        return out, (int(n[0]), int(n[1]))
+
+    def neighbours(self):
+        """CSR of S_i in INPUT numbering, rows ascending (uses the tree's h)."""
+        N = self.N
+        off = np.zeros(N + 1, np.int64)
+        args = (_i64(N), _p(self.spos), _p(self.sh), _p(self.first), _p(self.count), _p(self.left),
+                _p(self.escape), _p(self.nbox), _p(self.hmax), ctypes.c_int(self.bucket))
+        lib().or_neighbours_tree(*args, _p(off), None)
+        idx = np.zeros(int(off[-1]), np.int32)
+        lib().or_neighbours_tree(*args, _p(off), _p(idx))
+        return csr_permute(off, idx, self.perm)
+
+
+def csr_permute(off_s, idx_s, perm):
+    """Sorted-numbering CSR -> input-numbering CSR with ascending rows."""
+    N = len(perm)
+    cnt_s = np.diff(off_s)
+    cnt = np.zeros(N, np.int64); cnt[perm] = cnt_s
+    off = np.zeros(N + 1, np.int64); np.cumsum(cnt, out=off[1:])
+    rows_s = np.repeat(np.arange(N), cnt_s)
+    rows = perm[rows_s].astype(np.int64)
+    cols = perm[idx_s].astype(np.int64)
+    order = np.lexsort((cols, rows))
+    return off, cols[order].astype(np.int32)
+
+
+def h_newton(pos, h0, n_ngb=64.0, tol=1e-12, max_iter=200):
+    pos = _f64(pos, (-1, 3))
+    out = np.zeros(len(pos))
+    bad = lib().or_h_newton(_i64(len(pos)), _p(pos), _p(_f64(h0)), ctypes.c_double(n_ngb), ctypes.c_double(tol),
+                            _i64(max_iter), _p(out))
+    return out, int(bad)
+
+
+def density_csr(pos, m, h, off, idx):
+    pos = _f64(pos, (-1, 3)); out = np.zeros(len(pos))
+    lib().or_density_csr(_i64(len(pos)), _p(pos), _p(_f64(m)), _p(_f64(h)), _p(off), _p(idx), _p(out))
+    return out
+
+
+def divcurl_csr(pos, v, rho, m, h, off, idx):
+    pos = _f64(pos, (-1, 3)); N = len(pos)
+    dv = np.zeros(N); cv = np.zeros(N)
+    lib().or_divcurl_csr(_i64(N), _p(pos), _p(_f64(v, (-1, 3))), _p(_f64(rho)), _p(_f64(m)), _p(_f64(h)),
+                         _p(off), _p(idx), _p(dv), _p(cv))
+    return dv, cv
+
+
+def accsph_csr(pos, v, rho, P, c, h, m, alpha, off, idx, divV=None, curlV=None):
+    pos = _f64(pos, (-1, 3)); N = len(pos)
+    bal = divV is not None
+    dv = _f64(divV) if bal else np.zeros(N)
+    cv = _f64(curlV) if bal else np.zeros(N)
+    out = np.zeros((N, 3))
+    lib().or_accsph_csr(_i64(N), _p(pos), _p(_f64(v, (-1, 3))), _p(_f64(rho)), _p(_f64(P)), _p(_f64(c)),
+                        _p(_f64(h)), _p(_f64(m)), _p(dv), _p(cv), ctypes.c_double(alpha), ctypes.c_int(int(bal)),
+                        _p(off), _p(idx), _p(out))
+    return out
+
+
+def dudt_csr(pos, v, rho, P, c, h, m, alpha, off, idx, divV=None, curlV=None):
+    pos = _f64(pos, (-1, 3)); N = len(pos)
+    bal = divV is not None
+    dv = _f64(divV) if bal else np.zeros(N)
+    cv = _f64(curlV) if bal else np.zeros(N)
+    out = np.zeros(N)
+    lib().or_dudt_csr(_i64(N), _p(pos), _p(_f64(v, (-1, 3))), _p(_f64(rho)), _p(_f64(P)), _p(_f64(c)),
+                      _p(_f64(h)), _p(_f64(m)), _p(dv), _p(cv), ctypes.c_double(alpha), ctypes.c_int(int(bal)),
+                      _p(off), _p(idx), _p(out))
+    return out

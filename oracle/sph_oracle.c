@@ -1,0 +1,851 @@
+/*
+ * sph_oracle.c -- FP64 CPU restatement of the SPH per-step hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP path: only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ * The product library (sph_3_amd/csrc) never links or calls it.
+ *
+ * Build: oracle/Makefile  (gcc -O2 -fopenmp -ffp-contract=off).  Contraction
+ * is off so that every FP64 decision that must be bit-exact against the GPU
+ * (neighbour membership r^2 < (h_i+h_j)^2, Morton keys, node centres of mass,
+ * the opening test) is evaluated as written, one rounding per operation.
+ *
+ * Two families of functions:
+ *   (1) literal restatements of the reference's all-pairs numba loops
+ *       (sph_code/libsx.py, sph_code/shear.py), with the row range
+ *       [nbeg, nend) of the reference's *_mpi twins;
+ *   (2) the build's own definitions that the GPU path implements and that the
+ *       reference leaves to its GPU code: Morton order + binary radix tree,
+ *       symmetric CSR neighbour sets, Newton-Raphson smoothing length, tree
+ *       gravity with a per-particle opening test, CSR-restricted hydro sums.
+ * Each function cites the reference lines it follows.
+ *
+ * Arrays: pos/vel are N x 3 row-major (numpy C order), everything FP64.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define SIG (1.0 / M_PI)
+
+/* ---------------------------------------------------------------------------
+ * Cubic-spline kernel, libsx.py:29-39 (W) and libsx.py:130-144 (grad W).
+ * -------------------------------------------------------------------------*/
+static inline double kernel_W(double rr, double hij)
+{
+    double q = rr / hij, W = 0.0;
+    if (q <= 1.0) W = SIG / (hij * hij * hij) * (1.0 - 1.5 * q * q + 0.75 * q * q * q);
+    if (q > 1.0 && q <= 2.0) W = SIG / (hij * hij * hij) * 0.25 * (2.0 - q) * (2.0 - q) * (2.0 - q);
+    return W;
+}
+
+/* returns nW with gradW = nW * d (d = r_i - r_j), libsx.py:130-144 */
+static inline double kernel_nW(double rr, double hij)
+{
+    double q = rr / hij, h5 = hij * hij * hij * hij * hij, nW = 0.0;
+    if (q <= 1.0) nW = SIG / h5 * (-3.0 + 2.25 * q);
+    if (q > 1.0 && q <= 2.0) nW = -3.0 * SIG / 4.0 / h5 * (2.0 - q) * (2.0 - q) / (q + 1e-20);
+    return nW;
+}
+
+/* ---------------------------------------------------------------------------
+ * (1) All-pairs restatements
+ * -------------------------------------------------------------------------*/
+
+/* getDensity / getDensity_mpi, libsx.py:10-44, 50-86 */
+void or_density_allpairs(int64_t N, int64_t nbeg, int64_t nend, const double *pos,
+                         const double *m, const double *h, double *rho)
+{
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = nbeg; i < nend; i++) {
+        double s = 0.0;
+        for (int64_t j = 0; j < N; j++) {
+            double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
+                   dz = pos[3 * i + 2] - pos[3 * j + 2];
+            double rr = sqrt(dx * dx + dy * dy + dz * dz);
+            double hij = 0.5 * (h[i] + h[j]);
+            s += m[j] * kernel_W(rr, hij);
+        }
+        rho[i - nbeg] = s;
+    }
+}
+
+/* div_curlVel, shear.py:6-69: divV stored as abs(), curl magnitude */
+void or_divcurl_allpairs(int64_t N, int64_t nbeg, int64_t nend, const double *pos,
+                         const double *v, const double *rho, const double *m,
+                         const double *h, double *divV, double *curlV)
+{
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = nbeg; i < nend; i++) {
+        double s = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+        for (int64_t j = 0; j < N; j++) {
+            double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
+                   dz = pos[3 * i + 2] - pos[3 * j + 2];
+            double rr = sqrt(dx * dx + dy * dy + dz * dz);
+            double hij = 0.5 * (h[i] + h[j]);
+            double nW = kernel_nW(rr, hij);
+            double gx = nW * dx, gy = nW * dy, gz = nW * dz;
+            double vxij = v[3 * i] - v[3 * j], vyij = v[3 * i + 1] - v[3 * j + 1],
+                   vzij = v[3 * i + 2] - v[3 * j + 2];
+            double f = m[j] / rho[i];
+            s += f * (-vxij * gx + -vyij * gy + -vzij * gz);
+            cx += f * (vyij * gz - vzij * gy);
+            cy += f * (vzij * gx - vxij * gz);
+            cz += f * (vxij * gy - vyij * gx);
+        }
+        divV[i - nbeg] = fabs(s);
+        curlV[i - nbeg] = sqrt(cx * cx + cy * cy + cz * cz);
+    }
+}
+
+/* Pairwise artificial viscosity Pi_ij, libsx.py:146-160 (Gadget-2 form) with the
+ * Balsara factor of shear.py:205-209 when balsara != 0. */
+static inline double av_pi(double vij_rij, double rr, double ci, double cj, double rhoi,
+                           double rhoj, double alpha)
+{
+    double wij = vij_rij / (rr + 1e-5);
+    double vsig = ci + cj - 3.0 * wij;
+    double rhoij = 0.5 * (rhoi + rhoj);
+    return (vij_rij <= 0.0) ? -0.5 * alpha * vsig * wij / rhoij : 0.0;
+}
+
+static inline double balsara_f(double divV, double curlV, double c, double h)
+{
+    return divV / (divV + curlV + 0.0001 * c / h);
+}
+
+/* getAcc_sph (libsx.py:100-177) when balsara==0, getAcc_sph_shear
+ * (shear.py:149-235) when balsara!=0.  acc is (nend-nbeg) x 3. */
+void or_accsph_allpairs(int64_t N, int64_t nbeg, int64_t nend, const double *pos,
+                        const double *v, const double *rho, const double *P,
+                        const double *c, const double *h, const double *m,
+                        const double *divV, const double *curlV, double alpha,
+                        int balsara, double *acc)
+{
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = nbeg; i < nend; i++) {
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        double fi = balsara ? balsara_f(divV[i], curlV[i], c[i], h[i]) : 1.0;
+        for (int64_t j = 0; j < N; j++) {
+            double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
+                   dz = pos[3 * i + 2] - pos[3 * j + 2];
+            double rr = sqrt(dx * dx + dy * dy + dz * dz);
+            double hij = 0.5 * (h[i] + h[j]);
+            double nW = kernel_nW(rr, hij);
+            double gx = nW * dx, gy = nW * dy, gz = nW * dz;
+            double vxij = v[3 * i] - v[3 * j], vyij = v[3 * i + 1] - v[3 * j + 1],
+                   vzij = v[3 * i + 2] - v[3 * j + 2];
+            double vr = vxij * dx + vyij * dy + vzij * dz;
+            double PI = av_pi(vr, rr, c[i], c[j], rho[i], rho[j], alpha);
+            if (balsara) {
+                double fj = balsara_f(divV[j], curlV[j], c[j], h[j]);
+                PI = 0.5 * (fi + fj) * PI;
+            }
+            double t = m[j] * (P[i] / (rho[i] * rho[i]) + P[j] / (rho[j] * rho[j]) + PI);
+            ax -= t * gx;
+            ay -= t * gy;
+            az -= t * gz;
+        }
+        acc[3 * (i - nbeg)] = ax;
+        acc[3 * (i - nbeg) + 1] = ay;
+        acc[3 * (i - nbeg) + 2] = az;
+    }
+}
+
+/* get_dU (libsx.py:353-421) / get_dU_shear (shear.py:332-406) */
+void or_dudt_allpairs(int64_t N, int64_t nbeg, int64_t nend, const double *pos,
+                      const double *v, const double *rho, const double *P,
+                      const double *c, const double *h, const double *m,
+                      const double *divV, const double *curlV, double alpha, int balsara,
+                      double *dudt)
+{
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = nbeg; i < nend; i++) {
+        double s = 0.0;
+        double fi = balsara ? balsara_f(divV[i], curlV[i], c[i], h[i]) : 1.0;
+        for (int64_t j = 0; j < N; j++) {
+            double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
+                   dz = pos[3 * i + 2] - pos[3 * j + 2];
+            double rr = sqrt(dx * dx + dy * dy + dz * dz);
+            double hij = 0.5 * (h[i] + h[j]);
+            double nW = kernel_nW(rr, hij);
+            double gx = nW * dx, gy = nW * dy, gz = nW * dz;
+            double vxij = v[3 * i] - v[3 * j], vyij = v[3 * i + 1] - v[3 * j + 1],
+                   vzij = v[3 * i + 2] - v[3 * j + 2];
+            double vg = vxij * gx + vyij * gy + vzij * gz;
+            double vr = vxij * dx + vyij * dy + vzij * dz;
+            double PI = av_pi(vr, rr, c[i], c[j], rho[i], rho[j], alpha);
+            if (balsara) {
+                double fj = balsara_f(divV[j], curlV[j], c[j], h[j]);
+                PI = 0.5 * (fi + fj) * PI;
+            }
+            s += m[j] * (P[i] / (rho[i] * rho[i]) + PI / 2.0) * vg;
+        }
+        dudt[i - nbeg] = s;
+    }
+}
+
+/* Spline-softened pair factor, libsx.py:640-652 (fk with inv_r3). */
+static inline double grav_fk_spline(double rr, double epsij)
+{
+    double q = rr / epsij;
+    if (q <= 1.0) return (1.0 / (epsij * epsij * epsij)) * (4.0 / 3.0 - 1.2 * q * q + 0.5 * q * q * q);
+    double inv_r3 = 1.0 / (rr * rr * rr);
+    if (q <= 2.0)
+        return inv_r3 * (-1.0 / 15.0 + (8.0 / 3.0) * q * q * q - 3.0 * q * q * q * q +
+                         1.2 * q * q * q * q * q - (1.0 / 6.0) * q * q * q * q * q * q);
+    return inv_r3;
+}
+
+/* Plummer pair factor 1/(r^2+eps^2)^(3/2), P3/archive/Evrard_collapse_old/acc_test.py:16-29 */
+static inline double grav_fk_plummer(double r2, double eps2)
+{
+    double s = r2 + eps2;
+    return 1.0 / (s * sqrt(s));
+}
+
+/* Rows [nbeg,nend) of the full direct-sum field: for soft_kind 0 the spline of
+ * getAcc_g_smth (libsx.py:623-662) with eps_ij = (eps_i+eps_j)/2; for
+ * soft_kind 1 Plummer with the scalar eps_plummer.  Row form of the symmetric
+ * loop; the self pair is skipped (it contributes d = 0 in the reference). */
+void or_grav_direct(int64_t N, int64_t nbeg, int64_t nend, const double *pos,
+                    const double *m, const double *eps, double G, int soft_kind,
+                    double eps_plummer, double *acc)
+{
+    double e2 = eps_plummer * eps_plummer;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = nbeg; i < nend; i++) {
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        for (int64_t j = 0; j < N; j++) {
+            if (j == i) continue;
+            double dx = pos[3 * j] - pos[3 * i], dy = pos[3 * j + 1] - pos[3 * i + 1],
+                   dz = pos[3 * j + 2] - pos[3 * i + 2];
+            double r2 = dx * dx + dy * dy + dz * dz, fk;
+            if (soft_kind == 0)
+                fk = grav_fk_spline(sqrt(r2), 0.5 * (eps[i] + eps[j]));
+            else
+                fk = grav_fk_plummer(r2, e2);
+            ax += G * fk * dx * m[j];
+            ay += G * fk * dy * m[j];
+            az += G * fk * dz * m[j];
+        }
+        acc[3 * (i - nbeg)] = ax;
+        acc[3 * (i - nbeg) + 1] = ay;
+        acc[3 * (i - nbeg) + 2] = az;
+    }
+}
+
+/* getKE, libsx.py:569-579 */
+double or_ke(int64_t N, const double *v, const double *m)
+{
+    double ke = 0.0;
+    for (int64_t i = 0; i < N; i++)
+        ke += 0.5 * m[i] * (v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
+    return ke;
+}
+
+/* getPE, libsx.py:583-620 (spline potential, eps_ij = (eps_i+eps_j)/2) */
+double or_pe(int64_t N, const double *pos, const double *m, double G, const double *eps)
+{
+    double pe = 0.0;
+#pragma omp parallel for reduction(+ : pe) schedule(dynamic, 16)
+    for (int64_t i = 0; i < N; i++) {
+        for (int64_t j = i + 1; j < N; j++) {
+            double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
+                   dz = pos[3 * i + 2] - pos[3 * j + 2];
+            double rr = sqrt(dx * dx + dy * dy + dz * dz), fk = 0.0, inv_r = 0.0;
+            if (rr != 0.0) inv_r = 1.0 / rr;
+            double e = 0.5 * (eps[i] + eps[j]), q = rr / e;
+            if (q <= 1.0 && q != 0.0)
+                fk = m[j] * ((-2.0 / e) * ((1.0 / 3.0) * q * q - 0.15 * q * q * q * q + 0.05 * q * q * q * q * q) + 1.4 / e);
+            if (q > 1.0 && q <= 2.0)
+                fk = m[j] * ((-1.0 / 15.0) * inv_r -
+                             (1.0 / e) * ((4.0 / 3.0) * q * q - q * q * q + 0.3 * q * q * q * q - (1.0 / 30.0) * q * q * q * q * q) +
+                             1.6 / e);
+            if (q > 2.0) fk = m[j] * inv_r;
+            pe -= G * m[i] * fk;
+        }
+    }
+    return pe;
+}
+
+static int cmp_double(const void *a, const void *b)
+{
+    double x = *(const double *)a, y = *(const double *)b;
+    return (x > y) - (x < y);
+}
+
+/* do_smoothingX / smoothing_length_mpi, libsx.py:752-800:
+ * h = 0.5 * (the kth smallest distance to all N particles, self included),
+ * k = 50 (0-based) in the reference. */
+void or_h_initial(int64_t N, const double *pos, int64_t M, const double *sub, int64_t k,
+                  double *h)
+{
+#pragma omp parallel
+    {
+        double *dist = (double *)malloc(sizeof(double) * N);
+#pragma omp for schedule(dynamic, 4)
+        for (int64_t i = 0; i < M; i++) {
+            for (int64_t j = 0; j < N; j++) {
+                double dx = pos[3 * j] - sub[3 * i], dy = pos[3 * j + 1] - sub[3 * i + 1],
+                       dz = pos[3 * j + 2] - sub[3 * i + 2];
+                dist[j] = sqrt(dx * dx + dy * dy + dz * dz);
+            }
+            qsort(dist, N, sizeof(double), cmp_double);
+            h[i] = dist[k] * 0.5;
+        }
+        free(dist);
+    }
+}
+
+/* h_smooth_fast / h_smooth_fast_mpi, libsx.py:804-900: step h by -/+0.3 %
+ * until 45 <= #(dist < 2h) <= 55.  The reference loop has no cap; here it
+ * stops after max_iter steps and counts such rows in *n_unconverged. */
+void or_h_smooth_fast(int64_t N, int64_t nbeg, int64_t nend, const double *pos,
+                      const double *h_in, int64_t n_low, int64_t n_up, int64_t max_iter,
+                      double *h_out, int64_t *n_unconverged)
+{
+    int64_t bad = 0;
+#pragma omp parallel reduction(+ : bad)
+    {
+        double *dist = (double *)malloc(sizeof(double) * N);
+#pragma omp for schedule(dynamic, 4)
+        for (int64_t i = nbeg; i < nend; i++) {
+            double hi = h_in[i];
+            for (int64_t j = 0; j < N; j++) {
+                double dx = pos[3 * j] - pos[3 * i], dy = pos[3 * j + 1] - pos[3 * i + 1],
+                       dz = pos[3 * j + 2] - pos[3 * i + 2];
+                dist[j] = sqrt(dx * dx + dy * dy + dz * dz);
+            }
+            int64_t cnt = 0;
+            for (int64_t j = 0; j < N; j++) cnt += dist[j] < 2.0 * hi;
+            int64_t it = 0;
+            while ((cnt > n_up || cnt < n_low) && it < max_iter) {
+                if (cnt > n_up) hi -= 0.003 * hi;
+                if (cnt < n_low) hi += 0.003 * hi;
+                cnt = 0;
+                for (int64_t j = 0; j < N; j++) cnt += dist[j] < 2.0 * hi;
+                it++;
+            }
+            if (cnt > n_up || cnt < n_low) bad++;
+            h_out[i - nbeg] = hi;
+        }
+        free(dist);
+    }
+    if (n_unconverged) *n_unconverged = bad;
+}
+
+/* ---------------------------------------------------------------------------
+ * (2) The build's definitions
+ * -------------------------------------------------------------------------*/
+
+/* Symmetric neighbour set S_i = { j : |r_i - r_j|^2 < (h_i + h_j)^2 } (self
+ * included), i.e. every j inside the support 2*hbar_ij of the reference's
+ * symmetrised kernel (libsx.py:27-37; EV/lib.h:625-627).  FP64, one rounding
+ * per operation, association ((dx*dx + dy*dy) + dz*dz).
+ * Brute force, rows ascending.  Pass idx == NULL to get counts only
+ * (offsets[i+1]-offsets[i] filled, offsets[0] = 0, prefix summed). */
+void or_neighbours_brute(int64_t N, const double *pos, const double *h, int64_t *offsets,
+                         int32_t *idx)
+{
+    if (!idx) {
+        offsets[0] = 0;
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int64_t i = 0; i < N; i++) {
+            int64_t c = 0;
+            for (int64_t j = 0; j < N; j++) {
+                double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
+                       dz = pos[3 * i + 2] - pos[3 * j + 2];
+                double r2 = dx * dx + dy * dy + dz * dz, t = h[i] + h[j];
+                c += r2 < t * t;
+            }
+            offsets[i + 1] = c;
+        }
+        for (int64_t i = 0; i < N; i++) offsets[i + 1] += offsets[i];
+        return;
+    }
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < N; i++) {
+        int64_t k = offsets[i];
+        for (int64_t j = 0; j < N; j++) {
+            double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
+                   dz = pos[3 * i + 2] - pos[3 * j + 2];
+            double r2 = dx * dx + dy * dy + dz * dz, t = h[i] + h[j];
+            if (r2 < t * t) idx[k++] = (int32_t)j;
+        }
+    }
+}
+
+/* ---- Morton order + binary radix tree (Karras 2012) ----------------------
+ * Keys: 16 bits per axis over the bounding cube of side L = max extent,
+ * c = min((uint)((x - lo) * (65536 / L)), 65535), interleaved x-major
+ * (bit 3k+2 = x bit k).  Particles are ordered by a stable sort on the key;
+ * ties keep the input order.  Node ids: internal k -> k (0 <= k < N-1),
+ * leaf of sorted particle s -> N-1+s.  Root is id 0 for every N >= 1. */
+static inline uint64_t spread16(uint64_t v)
+{
+    v &= 0xFFFFull;
+    v = (v | (v << 16)) & 0x0000FF0000FFull;
+    v = (v | (v << 8)) & 0x00F00F00F00Full;
+    v = (v | (v << 4)) & 0x0C30C30C30C3ull;
+    v = (v | (v << 2)) & 0x249249249249ull;
+    return v;
+}
+
+static inline uint32_t cell16(double x, double lo, double scale)
+{
+    double t = (x - lo) * scale;
+    if (!(t > 0.0)) return 0;
+    if (t >= 65535.0) return 65535;
+    return (uint32_t)t;
+}
+
+void or_morton_keys(int64_t N, const double *pos, uint64_t *keys, double *box6)
+{
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = 0; i < N; i++)
+        for (int d = 0; d < 3; d++) {
+            double x = pos[3 * i + d];
+            if (x < lo[d]) lo[d] = x;
+            if (x > hi[d]) hi[d] = x;
+        }
+    double L = hi[0] - lo[0];
+    if (hi[1] - lo[1] > L) L = hi[1] - lo[1];
+    if (hi[2] - lo[2] > L) L = hi[2] - lo[2];
+    double scale = (L > 0.0) ? 65536.0 / L : 0.0;
+    for (int64_t i = 0; i < N; i++) {
+        uint64_t cx = cell16(pos[3 * i], lo[0], scale), cy = cell16(pos[3 * i + 1], lo[1], scale),
+                 cz = cell16(pos[3 * i + 2], lo[2], scale);
+        keys[i] = (spread16(cx) << 2) | (spread16(cy) << 1) | spread16(cz);
+    }
+    if (box6) {
+        for (int d = 0; d < 3; d++) {
+            box6[d] = lo[d];
+            box6[3 + d] = hi[d];
+        }
+    }
+}
+
+/* stable LSD sort of (key, index) -> perm, 16-bit digits */
+void or_sort_keys(int64_t N, const uint64_t *keys, int32_t *perm)
+{
+    int32_t *a = (int32_t *)malloc(sizeof(int32_t) * N), *b = (int32_t *)malloc(sizeof(int32_t) * N);
+    int64_t *cnt = (int64_t *)malloc(sizeof(int64_t) * 65537);
+    for (int64_t i = 0; i < N; i++) a[i] = (int32_t)i;
+    for (int pass = 0; pass < 3; pass++) {
+        int sh = 16 * pass;
+        memset(cnt, 0, sizeof(int64_t) * 65537);
+        for (int64_t i = 0; i < N; i++) cnt[((keys[a[i]] >> sh) & 0xFFFF) + 1]++;
+        for (int d = 0; d < 65536; d++) cnt[d + 1] += cnt[d];
+        for (int64_t i = 0; i < N; i++) b[cnt[(keys[a[i]] >> sh) & 0xFFFF]++] = a[i];
+        int32_t *t = a;
+        a = b;
+        b = t;
+    }
+    memcpy(perm, a, sizeof(int32_t) * N);
+    free(a);
+    free(b);
+    free(cnt);
+}
+
+static inline int clz64(uint64_t x) { return x ? __builtin_clzll(x) : 64; }
+static inline int clz32(uint32_t x) { return x ? __builtin_clz(x) : 32; }
+
+static inline int delta(const uint64_t *k, int64_t n, int64_t i, int64_t j)
+{
+    if (j < 0 || j >= n) return -1;
+    if (k[i] != k[j]) return clz64(k[i] ^ k[j]);
+    return 64 + clz32((uint32_t)i ^ (uint32_t)j);
+}
+
+/* Tree arrays (nn = 2N-1 node ids, see above):
+ *   first[nn], count[nn], left[N-1] (right child = its sibling), escape[nn]
+ *   box[nn*6] (lo xyz, hi xyz), com[nn*3], mass[nn], hmax[nn], epsmax[nn],
+ *   size[nn] = max extent of box.
+ * skeys: sorted keys. */
+void or_tree_topology(int64_t N, const uint64_t *skeys, int32_t *first, int32_t *count,
+                      int32_t *left, int32_t *right, int32_t *escape)
+{
+    int64_t nl = N - 1;
+    for (int64_t s = 0; s < N; s++) {
+        first[nl + s] = (int32_t)s;
+        count[nl + s] = 1;
+    }
+    for (int64_t i = 0; i < N - 1; i++) {
+        int d = (delta(skeys, N, i, i + 1) - delta(skeys, N, i, i - 1)) > 0 ? 1 : -1;
+        int dmin = delta(skeys, N, i, i - d);
+        int64_t lmax = 2;
+        while (delta(skeys, N, i, i + lmax * d) > dmin) lmax *= 2;
+        int64_t l = 0;
+        for (int64_t t = lmax / 2; t >= 1; t /= 2)
+            if (delta(skeys, N, i, i + (l + t) * d) > dmin) l += t;
+        int64_t j = i + l * d;
+        int dnode = delta(skeys, N, i, j);
+        int64_t s = 0, t = l;
+        do {
+            t = (t + 1) >> 1;
+            if (delta(skeys, N, i, i + (s + t) * d) > dnode) s += t;
+        } while (t > 1);
+        int64_t gamma = i + s * d + (d < 0 ? -1 : 0);
+        int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+        first[i] = (int32_t)lo;
+        count[i] = (int32_t)(hi - lo + 1);
+        left[i] = (int32_t)(lo == gamma ? nl + gamma : gamma);
+        right[i] = (int32_t)(hi == gamma + 1 ? nl + gamma + 1 : gamma + 1);
+    }
+    /* escape = first node after the subtree in pre-order: the topmost node
+     * whose range starts at last+1 (internal last+1 if it starts there). */
+    for (int64_t id = 0; id < 2 * N - 1; id++) {
+        int64_t b = (int64_t)first[id] + count[id] - 1;
+        int32_t e;
+        if (b == N - 1) e = -1;
+        else if (b + 1 < N - 1 && first[b + 1] == b + 1) e = (int32_t)(b + 1);
+        else e = (int32_t)(nl + b + 1);
+        escape[id] = e;
+    }
+}
+
+static void combine(int64_t p, int64_t a, int64_t b, double *box, double *com, double *mass,
+                    double *hmax, double *epsmax, double *size)
+{
+    for (int d = 0; d < 3; d++) {
+        box[6 * p + d] = box[6 * a + d] < box[6 * b + d] ? box[6 * a + d] : box[6 * b + d];
+        box[6 * p + 3 + d] = box[6 * a + 3 + d] > box[6 * b + 3 + d] ? box[6 * a + 3 + d] : box[6 * b + 3 + d];
+    }
+    double ma = mass[a], mb = mass[b], mt = ma + mb;
+    mass[p] = mt;
+    for (int d = 0; d < 3; d++) {
+        if (mt > 0.0) com[3 * p + d] = (ma * com[3 * a + d] + mb * com[3 * b + d]) / mt;
+        else com[3 * p + d] = 0.5 * (box[6 * p + d] + box[6 * p + 3 + d]);
+    }
+    hmax[p] = hmax[a] > hmax[b] ? hmax[a] : hmax[b];
+    epsmax[p] = epsmax[a] > epsmax[b] ? epsmax[a] : epsmax[b];
+    double s = box[6 * p + 3] - box[6 * p];
+    if (box[6 * p + 4] - box[6 * p + 1] > s) s = box[6 * p + 4] - box[6 * p + 1];
+    if (box[6 * p + 5] - box[6 * p + 2] > s) s = box[6 * p + 5] - box[6 * p + 2];
+    size[p] = s;
+}
+
+/* Node moments from sorted particle data (spos = sorted positions).  Post-order
+ * via an explicit stack; same combination formula as the device build. */
+void or_tree_moments(int64_t N, const double *spos, const double *sm, const double *sh,
+                     const double *seps, const int32_t *left, const int32_t *right,
+                     double *box, double *com, double *mass, double *hmax, double *epsmax,
+                     double *size)
+{
+    int64_t nl = N - 1;
+    for (int64_t s = 0; s < N; s++) {
+        int64_t id = nl + s;
+        for (int d = 0; d < 3; d++) {
+            box[6 * id + d] = box[6 * id + 3 + d] = spos[3 * s + d];
+            com[3 * id + d] = spos[3 * s + d];
+        }
+        mass[id] = sm[s];
+        hmax[id] = sh ? sh[s] : 0.0;
+        epsmax[id] = seps ? seps[s] : 0.0;
+        size[id] = 0.0;
+    }
+    if (N < 2) return;
+    int64_t *stack = (int64_t *)malloc(sizeof(int64_t) * 2 * N);
+    char *done = (char *)calloc(N, 1);
+    int64_t sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        int64_t k = stack[sp - 1];
+        int64_t a = left[k], b = right[k];
+        int pend = 0;
+        if (b < nl && !done[b]) { stack[sp++] = b; pend = 1; }
+        if (a < nl && !done[a]) { stack[sp++] = a; pend = 1; }
+        if (pend) continue;
+        sp--;
+        combine(k, a, b, box, com, mass, hmax, epsmax, size);
+        done[k] = 1;
+    }
+    free(stack);
+    free(done);
+}
+
+/* Opening test, per target particle (FP64): a node is used as a monopole iff
+ * size^2 < theta^2 * |com - x|^2 and x lies outside the node's box.  theta = 0
+ * never accepts, which reduces the walk to the exact direct sum. */
+static inline int mac_accept(const double *x, const double *box, const double *com, double size,
+                             double theta2)
+{
+    double dx = com[0] - x[0], dy = com[1] - x[1], dz = com[2] - x[2];
+    double d2 = dx * dx + dy * dy + dz * dz;
+    int outside = x[0] < box[0] || x[0] > box[3] || x[1] < box[1] || x[1] > box[4] ||
+                  x[2] < box[2] || x[2] > box[5];
+    return (size * size < theta2 * d2) && outside;
+}
+
+/* Tree gravity in sorted order (pre-order walk with escape links, buckets of
+ * <= bucket particles opened directly).  Node softening for the spline uses
+ * eps_ij = (eps_i + epsmax_node)/2.  acc is sorted-order N x 3.
+ * n_int[0] += particle-particle, n_int[1] += particle-node interactions. */
+void or_grav_tree(int64_t N, const double *spos, const double *sm, const double *seps,
+                  const int32_t *first, const int32_t *count, const int32_t *left,
+                  const int32_t *escape, const double *box, const double *com,
+                  const double *mass, const double *epsmax, const double *size, int bucket,
+                  double theta, double G, int soft_kind, double eps_plummer, double *acc,
+                  int64_t *n_int)
+{
+    double theta2 = theta * theta, e2 = eps_plummer * eps_plummer;
+    int64_t npp = 0, npn = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : npp, npn)
+    for (int64_t i = 0; i < N; i++) {
+        const double *x = spos + 3 * i;
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        int64_t node = 0;
+        while (node >= 0) {
+            if (mac_accept(x, box + 6 * node, com + 3 * node, size[node], theta2)) {
+                double dx = com[3 * node] - x[0], dy = com[3 * node + 1] - x[1], dz = com[3 * node + 2] - x[2];
+                double r2 = dx * dx + dy * dy + dz * dz;
+                double fk = soft_kind == 0 ? grav_fk_spline(sqrt(r2), 0.5 * (seps[i] + epsmax[node]))
+                                           : grav_fk_plummer(r2, e2);
+                ax += G * fk * dx * mass[node];
+                ay += G * fk * dy * mass[node];
+                az += G * fk * dz * mass[node];
+                npn++;
+                node = escape[node];
+            } else if (count[node] <= bucket) {
+                for (int64_t j = first[node]; j < first[node] + count[node]; j++) {
+                    if (j == i) continue;
+                    double dx = spos[3 * j] - x[0], dy = spos[3 * j + 1] - x[1], dz = spos[3 * j + 2] - x[2];
+                    double r2 = dx * dx + dy * dy + dz * dz;
+                    double fk = soft_kind == 0 ? grav_fk_spline(sqrt(r2), 0.5 * (seps[i] + seps[j]))
+                                               : grav_fk_plummer(r2, e2);
+                    ax += G * fk * dx * sm[j];
+                    ay += G * fk * dy * sm[j];
+                    az += G * fk * dz * sm[j];
+                    npp++;
+                }
+                node = escape[node];
+            } else {
+                node = left[node];
+            }
+        }
+        acc[3 * i] = ax;
+        acc[3 * i + 1] = ay;
+        acc[3 * i + 2] = az;
+    }
+    n_int[0] += npp;
+    n_int[1] += npn;
+}
+
+/* squared distance from x to a box (0 inside), association as for r^2 */
+static inline double box_dist2(const double *x, const double *b)
+{
+    double e[3];
+    for (int d = 0; d < 3; d++) {
+        double t = 0.0;
+        if (b[d] - x[d] > 0.0) t = b[d] - x[d];
+        if (x[d] - b[3 + d] > 0.0) t = x[d] - b[3 + d];
+        e[d] = t;
+    }
+    return e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+}
+
+/* Symmetric CSR neighbour sets in sorted order via the tree (same set as
+ * or_neighbours_brute).  Node pruned iff boxdist^2 >= (h_i + hmax)^2, which is
+ * exact under round-to-nearest monotonicity.  idx == NULL: counts only into
+ * offsets[1..N] then prefix sum. Rows come out ascending in sorted index. */
+void or_neighbours_tree(int64_t N, const double *spos, const double *sh, const int32_t *first,
+                        const int32_t *count, const int32_t *left, const int32_t *escape,
+                        const double *box, const double *hmax, int bucket, int64_t *offsets,
+                        int32_t *idx)
+{
+    if (!idx) offsets[0] = 0;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < N; i++) {
+        const double *x = spos + 3 * i;
+        int64_t c = 0, k = idx ? offsets[i] : 0;
+        int64_t node = 0;
+        while (node >= 0) {
+            double t = sh[i] + hmax[node];
+            if (box_dist2(x, box + 6 * node) >= t * t) {
+                node = escape[node];
+            } else if (count[node] <= bucket) {
+                for (int64_t j = first[node]; j < first[node] + count[node]; j++) {
+                    double dx = x[0] - spos[3 * j], dy = x[1] - spos[3 * j + 1], dz = x[2] - spos[3 * j + 2];
+                    double r2 = dx * dx + dy * dy + dz * dz, tt = sh[i] + sh[j];
+                    if (r2 < tt * tt) {
+                        if (idx) idx[k] = (int32_t)j;
+                        k++;
+                        c++;
+                    }
+                }
+                node = escape[node];
+            } else {
+                node = left[node];
+            }
+        }
+        if (!idx) offsets[i + 1] = c;
+    }
+    if (!idx)
+        for (int64_t i = 0; i < N; i++) offsets[i + 1] += offsets[i];
+}
+
+/* Kernel-weighted neighbour number and its h-derivative at smoothing length h
+ * from a list of squared distances: f(h) = (32/3) * sum_j w(r_j/h) - n_ngb,
+ * i.e. (4 pi/3)(2h)^3 sum_j W(r_j, h) - n_ngb with W = w(q)/(pi h^3). */
+static void nr_f(const double *r2, int64_t n, double h, double n_ngb, double *f, double *df)
+{
+    double s = 0.0, ds = 0.0, ih = 1.0 / h;
+    for (int64_t k = 0; k < n; k++) {
+        double q = sqrt(r2[k]) * ih;
+        if (q < 1.0) {
+            s += 1.0 - 1.5 * q * q + 0.75 * q * q * q;
+            ds += (-3.0 * q + 2.25 * q * q) * (-q * ih);
+        } else if (q < 2.0) {
+            double t = 2.0 - q;
+            s += 0.25 * t * t * t;
+            ds += (-0.75 * t * t) * (-q * ih);
+        }
+    }
+    *f = (32.0 / 3.0) * s - n_ngb;
+    *df = (32.0 / 3.0) * ds;
+}
+
+/* Newton-Raphson smoothing length (the build's FP64 h solve): root of
+ * f(h) = 0 above, bracketed, |dh| <= tol*h.  Brute-force distances; this is
+ * the checker of the device solve.  Returns number of rows that failed. */
+int64_t or_h_newton(int64_t N, const double *pos, const double *h_in, double n_ngb, double tol,
+                    int64_t max_iter, double *h_out)
+{
+    int64_t bad = 0;
+#pragma omp parallel reduction(+ : bad)
+    {
+        double *r2 = (double *)malloc(sizeof(double) * N);
+#pragma omp for schedule(dynamic, 8)
+        for (int64_t i = 0; i < N; i++) {
+            for (int64_t j = 0; j < N; j++) {
+                double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
+                       dz = pos[3 * i + 2] - pos[3 * j + 2];
+                r2[j] = dx * dx + dy * dy + dz * dz;
+            }
+            double h = h_in[i], lo = 0.0, hi = INFINITY, f, df;
+            int ok = 0;
+            for (int64_t it = 0; it < max_iter; it++) {
+                nr_f(r2, N, h, n_ngb, &f, &df);
+                if (f < 0.0) lo = h; else hi = h;
+                double hn = (df > 0.0) ? h - f / df : NAN;
+                if (!(hn > lo && hn < hi)) hn = isinf(hi) ? 2.0 * h : 0.5 * (lo + hi);
+                double dh = fabs(hn - h);
+                h = hn;
+                if (dh <= tol * h) { ok = 1; break; }
+            }
+            if (!ok) bad++;
+            h_out[i] = h;
+        }
+        free(r2);
+    }
+    return bad;
+}
+
+/* CSR-restricted sums (same formulas as the all-pairs forms, over S_i). CSR
+ * indices refer to the same particle numbering as pos.  Used at sizes where
+ * the all-pairs forms are too slow. */
+void or_density_csr(int64_t N, const double *pos, const double *m, const double *h,
+                    const int64_t *off, const int32_t *idx, double *rho)
+{
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < N; i++) {
+        double s = 0.0;
+        for (int64_t k = off[i]; k < off[i + 1]; k++) {
+            int64_t j = idx[k];
+            double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
+                   dz = pos[3 * i + 2] - pos[3 * j + 2];
+            double rr = sqrt(dx * dx + dy * dy + dz * dz);
+            s += m[j] * kernel_W(rr, 0.5 * (h[i] + h[j]));
+        }
+        rho[i] = s;
+    }
+}
+
+void or_divcurl_csr(int64_t N, const double *pos, const double *v, const double *rho,
+                    const double *m, const double *h, const int64_t *off, const int32_t *idx,
+                    double *divV, double *curlV)
+{
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < N; i++) {
+        double s = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+        for (int64_t k = off[i]; k < off[i + 1]; k++) {
+            int64_t j = idx[k];
+            double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
+                   dz = pos[3 * i + 2] - pos[3 * j + 2];
+            double rr = sqrt(dx * dx + dy * dy + dz * dz);
+            double nW = kernel_nW(rr, 0.5 * (h[i] + h[j]));
+            double gx = nW * dx, gy = nW * dy, gz = nW * dz;
+            double vxij = v[3 * i] - v[3 * j], vyij = v[3 * i + 1] - v[3 * j + 1],
+                   vzij = v[3 * i + 2] - v[3 * j + 2];
+            double f = m[j] / rho[i];
+            s += f * (-vxij * gx + -vyij * gy + -vzij * gz);
+            cx += f * (vyij * gz - vzij * gy);
+            cy += f * (vzij * gx - vxij * gz);
+            cz += f * (vxij * gy - vyij * gx);
+        }
+        divV[i] = fabs(s);
+        curlV[i] = sqrt(cx * cx + cy * cy + cz * cz);
+    }
+}
+
+void or_accsph_csr(int64_t N, const double *pos, const double *v, const double *rho,
+                   const double *P, const double *c, const double *h, const double *m,
+                   const double *divV, const double *curlV, double alpha, int balsara,
+                   const int64_t *off, const int32_t *idx, double *acc)
+{
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < N; i++) {
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        double fi = balsara ? balsara_f(divV[i], curlV[i], c[i], h[i]) : 1.0;
+        for (int64_t k = off[i]; k < off[i + 1]; k++) {
+            int64_t j = idx[k];
+            double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
+                   dz = pos[3 * i + 2] - pos[3 * j + 2];
+            double rr = sqrt(dx * dx + dy * dy + dz * dz);
+            double nW = kernel_nW(rr, 0.5 * (h[i] + h[j]));
+            double gx = nW * dx, gy = nW * dy, gz = nW * dz;
+            double vxij = v[3 * i] - v[3 * j], vyij = v[3 * i + 1] - v[3 * j + 1],
+                   vzij = v[3 * i + 2] - v[3 * j + 2];
+            double vr = vxij * dx + vyij * dy + vzij * dz;
+            double PI = av_pi(vr, rr, c[i], c[j], rho[i], rho[j], alpha);
+            if (balsara) PI = 0.5 * (fi + balsara_f(divV[j], curlV[j], c[j], h[j])) * PI;
+            double t = m[j] * (P[i] / (rho[i] * rho[i]) + P[j] / (rho[j] * rho[j]) + PI);
+            ax -= t * gx;
+            ay -= t * gy;
+            az -= t * gz;
+        }
+        acc[3 * i] = ax;
+        acc[3 * i + 1] = ay;
+        acc[3 * i + 2] = az;
+    }
+}
+
+void or_dudt_csr(int64_t N, const double *pos, const double *v, const double *rho,
+                 const double *P, const double *c, const double *h, const double *m,
+                 const double *divV, const double *curlV, double alpha, int balsara,
+                 const int64_t *off, const int32_t *idx, double *dudt)
+{
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < N; i++) {
+        double s = 0.0;
+        double fi = balsara ? balsara_f(divV[i], curlV[i], c[i], h[i]) : 1.0;
+        for (int64_t k = off[i]; k < off[i + 1]; k++) {
+            int64_t j = idx[k];
+            double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
+                   dz = pos[3 * i + 2] - pos[3 * j + 2];
+            double rr = sqrt(dx * dx + dy * dy + dz * dz);
+            double nW = kernel_nW(rr, 0.5 * (h[i] + h[j]));
+            double gx = nW * dx, gy = nW * dy, gz = nW * dz;
+            double vxij = v[3 * i] - v[3 * j], vyij = v[3 * i + 1] - v[3 * j + 1],
+                   vzij = v[3 * i + 2] - v[3 * j + 2];
+            double vg = vxij * gx + vyij * gy + vzij * gz;
+            double vr = vxij * dx + vyij * dy + vzij * dz;
+            double PI = av_pi(vr, rr, c[i], c[j], rho[i], rho[j], alpha);
+            if (balsara) PI = 0.5 * (fi + balsara_f(divV[j], curlV[j], c[j], h[j])) * PI;
+            s += m[j] * (P[i] / (rho[i] * rho[i]) + PI / 2.0) * vg;
+        }
+        dudt[i] = s;
+    }
+}

@@ -1,0 +1,906 @@
+// sph_capi.hip -- the C ABI (include/sph_amd.h): context, device caches of
+// the particle state in Morton order, the per-function entry points that
+// mirror the reference's Python call surface, and the device-resident step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sph_amd.h"
+#include "sph_common.h"
+#include "sph_grav.h"
+#include "sph_hydro.h"
+#include "sph_integrate.h"
+#include "sph_ngb.h"
+#include "sph_order.h"
+#include "sph_tree.h"
+
+#define SPH_VERSION 1
+#define SPH_BUCKET 16      // particles per tree bucket (walk leaves)
+#define SPH_HCAP 256       // candidate slots per target in the h solves
+#define SPH_HROUNDS 64     // maximum radius-adjustment rounds of an h solve
+#define SPH_KEYBITS 48     // Morton key bits (16 per axis)
+
+namespace {
+
+struct Buf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+__global__ void k_deinterleave(int64_t n, const double *__restrict__ aos, double *__restrict__ x,
+                               double *__restrict__ y, double *__restrict__ z)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) {
+        x[i] = aos[3 * i];
+        y[i] = aos[3 * i + 1];
+        z[i] = aos[3 * i + 2];
+    }
+}
+
+__global__ void k_iota(int64_t n, int32_t *__restrict__ a)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) a[i] = (int32_t)i;
+}
+
+__global__ void k_copy_d(int64_t n, const double *__restrict__ s, double *__restrict__ d)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) d[i] = s[i];
+}
+
+}  // namespace
+
+struct sph_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    std::string err;
+    int64_t n = 0;
+
+    // sorted particle state
+    Buf x, y, z, vx, vy, vz, m, h, eps, u, duprev, ax, ay, az, gx, gy, gz, tmpd;
+    Buf orig, tmpi;
+    // ordering
+    Buf keys0, keys1, vals0, vals1, hist, bpart, box;
+    // tree
+    Buf nodes, right, parent, flags;
+    // neighbour lists
+    Buf cnt, off, scantmp, idx;
+    // hydro scalars (8 bytes per particle, used as float or double)
+    Buf rho, P, c, prho2, divV, curlV, bal, du;
+    // h solve scratch
+    Buf cand, hR, hsteps, hstate, hcount, hout, hin, tx, ty, tz;
+    // staging and reductions
+    Buf stage, kpart, kout;
+
+    // caches of the stateless entry points (host copies of the last inputs)
+    std::vector<double> cpos, ch, cm, ceps;
+    bool order_ok = false, h_ok = false, m_ok = false, eps_ok = false, moments_ok = false, csr_ok = false;
+    bool resident = false, primed = false;
+    int64_t csr_total = 0;
+
+    // statistics
+    sph_stats stats;
+    hipEvent_t ev[2 * SPH_PH_COUNT];
+    bool timing = false;
+    int rho_prec = SPH_PREC_MIXED;  // arithmetic type of the resident rho buffer
+};
+
+namespace {
+
+int fail(sph_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCK(...)                                                                                   \
+    do {                                                                                             \
+        hipError_t e_ = (__VA_ARGS__);                                                               \
+        if (e_ != hipSuccess)                                                                        \
+            return fail(ctx, SPH_ERR_HIP, std::string(#__VA_ARGS__) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define CK(...)                                                                                      \
+    do {                                                                                             \
+        int r_ = (__VA_ARGS__);                                                                      \
+        if (r_ != SPH_OK) return r_;                                                                 \
+    } while (0)
+
+int ensure(sph_ctx *ctx, Buf &b, size_t bytes)
+{
+    if (bytes == 0) bytes = 8;
+    if (b.bytes >= bytes) return SPH_OK;
+    if (b.p) HIPCK(hipFree(b.p));
+    b.p = nullptr;
+    b.bytes = 0;
+    size_t nb = bytes + bytes / 8;
+    HIPCK(hipMalloc(&b.p, nb));
+    b.bytes = nb;
+    return SPH_OK;
+}
+
+template <typename T> T *P_(Buf &b) { return (T *)b.p; }
+#define D(b) P_<double>(ctx->b)
+#define I32(b) P_<int32_t>(ctx->b)
+
+int capacity(sph_ctx *ctx, int64_t n)
+{
+    size_t nd = (size_t)n * 8, ni = (size_t)n * 4;
+    Buf *dbl[] = {&ctx->x, &ctx->y, &ctx->z, &ctx->vx, &ctx->vy, &ctx->vz, &ctx->m, &ctx->h, &ctx->eps,
+                  &ctx->u, &ctx->duprev, &ctx->ax, &ctx->ay, &ctx->az, &ctx->gx, &ctx->gy, &ctx->gz,
+                  &ctx->tmpd, &ctx->rho, &ctx->P, &ctx->c, &ctx->prho2, &ctx->divV, &ctx->curlV,
+                  &ctx->bal, &ctx->du, &ctx->hR, &ctx->hout, &ctx->hin, &ctx->tx, &ctx->ty, &ctx->tz};
+    for (Buf *b : dbl) CK(ensure(ctx, *b, nd));
+    Buf *i32[] = {&ctx->orig, &ctx->tmpi, &ctx->vals0, &ctx->vals1, &ctx->cnt, &ctx->hsteps, &ctx->hstate};
+    for (Buf *b : i32) CK(ensure(ctx, *b, ni));
+    CK(ensure(ctx, ctx->keys0, nd));
+    CK(ensure(ctx, ctx->keys1, nd));
+    CK(ensure(ctx, ctx->hist, (size_t)sph::radix_hist_entries(n) * 4));
+    CK(ensure(ctx, ctx->bpart, 6 * 1024 * 8));
+    CK(ensure(ctx, ctx->box, 8 * 8));
+    CK(ensure(ctx, ctx->nodes, (size_t)(2 * n) * sizeof(WNode)));
+    CK(ensure(ctx, ctx->right, ni));
+    CK(ensure(ctx, ctx->parent, 2 * ni));
+    CK(ensure(ctx, ctx->flags, ni));
+    CK(ensure(ctx, ctx->off, (size_t)(n + 1) * 8));
+    CK(ensure(ctx, ctx->scantmp, (size_t)sph::scan_tmp_entries(n) * 8));
+    CK(ensure(ctx, ctx->hcount, 4 * 8));
+    CK(ensure(ctx, ctx->stage, 3 * nd));
+    CK(ensure(ctx, ctx->kpart, 5 * 1024 * 8));
+    CK(ensure(ctx, ctx->kout, 8 * 8));
+    return SPH_OK;
+}
+
+void invalidate_all(sph_ctx *ctx)
+{
+    ctx->order_ok = ctx->h_ok = ctx->m_ok = ctx->eps_ok = ctx->moments_ok = ctx->csr_ok = false;
+    ctx->cpos.clear();
+    ctx->ch.clear();
+    ctx->cm.clear();
+    ctx->ceps.clear();
+}
+
+// Sort the particles currently in (x, y, z) by Morton key; the permutation
+// (sorted -> current index) lands in vals0; keys0 holds the sorted keys.
+int sort_current(sph_ctx *ctx, int64_t n)
+{
+    hipStream_t st = ctx->st;
+    HIPCK(sph::bbox(st, n, D(x), D(y), D(z), D(bpart), D(box)));
+    HIPCK(sph::morton(st, n, D(x), D(y), D(z), D(box), P_<uint64_t>(ctx->keys0), I32(vals0)));
+    HIPCK(sph::radix_sort(st, n, SPH_KEYBITS, P_<uint64_t>(ctx->keys0), I32(vals0), P_<uint64_t>(ctx->keys1),
+                          I32(vals1), P_<uint32_t>(ctx->hist)));
+    if ((SPH_KEYBITS / 8) % 2) {  // result in the alternate buffers
+        std::swap(ctx->keys0, ctx->keys1);
+        std::swap(ctx->vals0, ctx->vals1);
+    }
+    return SPH_OK;
+}
+
+// Stateless entry: establish the Morton order of the caller's positions.
+int set_pos(sph_ctx *ctx, int64_t n, const double *pos)
+{
+    if (ctx->resident) {
+        ctx->resident = false;
+        ctx->primed = false;
+        invalidate_all(ctx);
+    }
+    if (ctx->order_ok && n == ctx->n && (int64_t)ctx->cpos.size() == 3 * n &&
+        std::memcmp(ctx->cpos.data(), pos, sizeof(double) * 3 * n) == 0)
+        return SPH_OK;
+    invalidate_all(ctx);
+    CK(capacity(ctx, n));
+    ctx->n = n;
+    hipStream_t st = ctx->st;
+    HIPCK(hipMemcpyAsync(ctx->stage.p, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
+    k_deinterleave<<<sph_blocks(n, 256), 256, 0, st>>>(n, D(stage), D(x), D(y), D(z));
+    CK(sort_current(ctx, n));
+    HIPCK(hipMemcpyAsync(ctx->orig.p, ctx->vals0.p, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, st));
+    for (int d = 0; d < 3; d++) {
+        double *dst = d == 0 ? D(x) : d == 1 ? D(y) : D(z);
+        HIPCK(sph::gather<double, double>(st, n, I32(orig), D(stage), 3, d, dst));
+    }
+    HIPCK(sph::tree_topology(st, n, P_<uint64_t>(ctx->keys0), P_<WNode>(ctx->nodes), I32(right), I32(parent)));
+    ctx->cpos.assign(pos, pos + 3 * n);
+    ctx->order_ok = true;
+    return SPH_OK;
+}
+
+// Upload a caller-order scalar array into sorted buffer dst (double).  With a
+// cache: returns 1 in *changed when the data differ from the last upload.
+int put_scalar(sph_ctx *ctx, const double *src, Buf &dst, std::vector<double> *cache, bool *ok, bool *changed)
+{
+    int64_t n = ctx->n;
+    if (changed) *changed = false;
+    if (cache && *ok && (int64_t)cache->size() == n && std::memcmp(cache->data(), src, sizeof(double) * n) == 0)
+        return SPH_OK;
+    HIPCK(hipMemcpyAsync(ctx->stage.p, src, sizeof(double) * n, hipMemcpyHostToDevice, ctx->st));
+    HIPCK(sph::gather<double, double>(ctx->st, n, I32(orig), D(stage), 1, 0, (double *)dst.p));
+    if (cache) {
+        cache->assign(src, src + n);
+        *ok = true;
+    }
+    if (changed) *changed = true;
+    return SPH_OK;
+}
+
+template <typename R>
+int put_real(sph_ctx *ctx, const double *src, Buf &dst)
+{
+    int64_t n = ctx->n;
+    HIPCK(hipMemcpyAsync(ctx->stage.p, src, sizeof(double) * n, hipMemcpyHostToDevice, ctx->st));
+    HIPCK(sph::gather<double, R>(ctx->st, n, I32(orig), D(stage), 1, 0, (R *)dst.p));
+    return SPH_OK;
+}
+
+int put_vec3(sph_ctx *ctx, const double *src, Buf &a, Buf &b, Buf &c)
+{
+    int64_t n = ctx->n;
+    HIPCK(hipMemcpyAsync(ctx->stage.p, src, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->st));
+    Buf *t[3] = {&a, &b, &c};
+    for (int d = 0; d < 3; d++) HIPCK(sph::gather<double, double>(ctx->st, n, I32(orig), D(stage), 3, d, (double *)t[d]->p));
+    return SPH_OK;
+}
+
+template <typename T>
+int get_scalar(sph_ctx *ctx, Buf &src, double *out)
+{
+    int64_t n = ctx->n;
+    HIPCK(sph::scatter<T>(ctx->st, n, I32(orig), (const T *)src.p, 1, 0, D(stage)));
+    HIPCK(hipMemcpyAsync(out, ctx->stage.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->st));
+    HIPCK(hipStreamSynchronize(ctx->st));
+    return SPH_OK;
+}
+
+int get_vec3(sph_ctx *ctx, Buf &a, Buf &b, Buf &c, double *out)
+{
+    int64_t n = ctx->n;
+    Buf *t[3] = {&a, &b, &c};
+    for (int d = 0; d < 3; d++)
+        HIPCK(sph::scatter<double>(ctx->st, n, I32(orig), (const double *)t[d]->p, 3, d, D(stage)));
+    HIPCK(hipMemcpyAsync(out, ctx->stage.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, ctx->st));
+    HIPCK(hipStreamSynchronize(ctx->st));
+    return SPH_OK;
+}
+
+int ensure_moments(sph_ctx *ctx)
+{
+    if (ctx->moments_ok) return SPH_OK;
+    HIPCK(sph::tree_moments(ctx->st, ctx->n, D(x), D(y), D(z), ctx->m_ok ? D(m) : nullptr,
+                            ctx->h_ok ? D(h) : nullptr, ctx->eps_ok ? D(eps) : nullptr, P_<WNode>(ctx->nodes),
+                            I32(right), I32(parent), P_<uint32_t>(ctx->flags)));
+    ctx->moments_ok = true;
+    return SPH_OK;
+}
+
+int build_csr(sph_ctx *ctx)
+{
+    int64_t n = ctx->n, total = 0;
+    int64_t cap = ctx->idx.p ? (int64_t)(ctx->idx.bytes / 4) : 0;
+    HIPCK(sph::neighbours(ctx->st, n, D(x), D(y), D(z), D(h), P_<WNode>(ctx->nodes), SPH_BUCKET, I32(cnt),
+                          P_<int64_t>(ctx->off), P_<int64_t>(ctx->scantmp), I32(idx), cap, &total));
+    if (total > cap) {
+        CK(ensure(ctx, ctx->idx, (size_t)total * 4));
+        HIPCK(sph::neighbours_fill(ctx->st, n, D(x), D(y), D(z), D(h), P_<WNode>(ctx->nodes), SPH_BUCKET,
+                                   P_<int64_t>(ctx->off), I32(idx)));
+    }
+    ctx->csr_total = total;
+    ctx->stats.ngb_total = total;
+    return SPH_OK;
+}
+
+int ensure_csr(sph_ctx *ctx)
+{
+    if (ctx->csr_ok) return SPH_OK;
+    CK(ensure_moments(ctx));
+    CK(build_csr(ctx));
+    ctx->csr_ok = true;
+    return SPH_OK;
+}
+
+// Common prologue of the hydro entry points.
+int geom(sph_ctx *ctx, int64_t n, const double *pos, const double *h, const double *m)
+{
+    CK(set_pos(ctx, n, pos));
+    bool ch = false, cm = false;
+    CK(put_scalar(ctx, h, ctx->h, &ctx->ch, &ctx->h_ok, &ch));
+    if (m) CK(put_scalar(ctx, m, ctx->m, &ctx->cm, &ctx->m_ok, &cm));
+    if (ch) ctx->csr_ok = false;
+    if (ch || cm) ctx->moments_ok = false;
+    return ensure_csr(ctx);
+}
+
+int check_common(sph_ctx *ctx, int64_t n, int precision)
+{
+    if (!ctx) return SPH_ERR_ARG;
+    if (n < 1) return fail(ctx, SPH_ERR_ARG, "n must be >= 1");
+    if (n > (int64_t)0x3fffffff) return fail(ctx, SPH_ERR_ARG, "n too large for 32-bit particle indices");
+    if (precision != SPH_PREC_MIXED && precision != SPH_PREC_FP64)
+        return fail(ctx, SPH_ERR_ARG, "precision must be SPH_PREC_MIXED or SPH_PREC_FP64");
+    HIPCK(hipSetDevice(ctx->device));
+    return SPH_OK;
+}
+
+// ---- phase timing for the resident step
+void ph_begin(sph_ctx *ctx, int ph)
+{
+    if (ctx->timing) (void)hipEventRecord(ctx->ev[2 * ph], ctx->st);
+}
+void ph_end(sph_ctx *ctx, int ph)
+{
+    if (ctx->timing) (void)hipEventRecord(ctx->ev[2 * ph + 1], ctx->st);
+}
+
+}  // namespace
+
+// =========================================================== context
+extern "C" int sph_version(void) { return SPH_VERSION; }
+
+extern "C" int sph_ctx_create(int device, sph_ctx **out)
+{
+    if (!out) return SPH_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return SPH_ERR_HIP;
+    if (device < 0 || device >= ndev) return SPH_ERR_ARG;
+    sph_ctx *ctx = new sph_ctx();
+    ctx->device = device;
+    std::memset(&ctx->stats, 0, sizeof(ctx->stats));
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return SPH_ERR_HIP;
+    }
+    for (int k = 0; k < 2 * SPH_PH_COUNT; k++)
+        if (hipEventCreate(&ctx->ev[k]) != hipSuccess) {
+            delete ctx;
+            return SPH_ERR_HIP;
+        }
+    *out = ctx;
+    return SPH_OK;
+}
+
+extern "C" int sph_ctx_destroy(sph_ctx *ctx)
+{
+    if (!ctx) return SPH_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->st);
+    Buf *all[] = {&ctx->x, &ctx->y, &ctx->z, &ctx->vx, &ctx->vy, &ctx->vz, &ctx->m, &ctx->h, &ctx->eps,
+                  &ctx->u, &ctx->duprev, &ctx->ax, &ctx->ay, &ctx->az, &ctx->gx, &ctx->gy, &ctx->gz,
+                  &ctx->tmpd, &ctx->orig, &ctx->tmpi, &ctx->keys0, &ctx->keys1, &ctx->vals0, &ctx->vals1,
+                  &ctx->hist, &ctx->bpart, &ctx->box, &ctx->nodes, &ctx->right, &ctx->parent, &ctx->flags,
+                  &ctx->cnt, &ctx->off, &ctx->scantmp, &ctx->idx, &ctx->rho, &ctx->P, &ctx->c, &ctx->prho2,
+                  &ctx->divV, &ctx->curlV, &ctx->bal, &ctx->du, &ctx->cand, &ctx->hR, &ctx->hsteps,
+                  &ctx->hstate, &ctx->hcount, &ctx->hout, &ctx->hin, &ctx->tx, &ctx->ty, &ctx->tz,
+                  &ctx->stage, &ctx->kpart, &ctx->kout};
+    for (Buf *b : all)
+        if (b->p) (void)hipFree(b->p);
+    for (int k = 0; k < 2 * SPH_PH_COUNT; k++) (void)hipEventDestroy(ctx->ev[k]);
+    (void)hipStreamDestroy(ctx->st);
+    delete ctx;
+    return SPH_OK;
+}
+
+extern "C" const char *sph_last_error(const sph_ctx *ctx)
+{
+    if (!ctx) return "null context";
+    return ctx->err.c_str();
+}
+
+// =========================================================== h solves
+namespace {
+
+int run_hsolve(sph_ctx *ctx, int64_t m, const double *tx, const double *ty, const double *tz, int mode,
+               double param, int n_low, int n_up, int max_steps, double tol, int64_t *n_unconv, int64_t *rounds_out)
+{
+    CK(ensure(ctx, ctx->cand, (size_t)m * SPH_HCAP * 8));
+    sph::HSolveParams p;
+    p.m = m;
+    p.tx = tx; p.ty = ty; p.tz = tz;
+    p.x = D(x); p.y = D(y); p.z = D(z);
+    p.nodes = P_<WNode>(ctx->nodes);
+    p.bucket = SPH_BUCKET;
+    p.mode = mode;
+    p.n_ngb = param;
+    p.tol = tol;
+    p.n_low = n_low; p.n_up = n_up; p.max_steps = max_steps;
+    p.kth = (int)param;
+    p.cap = SPH_HCAP;
+    p.r_fac = 1.15;
+    p.h_in = D(hin);
+    p.h_out = D(hout);
+    p.cand = D(cand);
+    p.R = D(hR);
+    p.steps = I32(hsteps);
+    p.state = I32(hstate);
+    p.counters = P_<unsigned long long>(ctx->hcount);
+    int64_t unc = 0;
+    p.unconverged_total = &unc;
+    int64_t pending = 0;
+    HIPCK(sph::hsolve(ctx->st, p, SPH_HROUNDS, &pending));
+    if (rounds_out) *rounds_out = 0;
+    if (pending) return fail(ctx, SPH_ERR_NOCONV, "smoothing-length solve: " + std::to_string(pending) +
+                                                      " targets undecided after the maximum number of radius rounds");
+    *n_unconv = unc;
+    return SPH_OK;
+}
+
+}  // namespace
+
+extern "C" int sph_smoothing_length(sph_ctx *ctx, int64_t n, const double *pos, int64_t m, const double *targets,
+                                    const double *h_in, int mode, double param, int n_low, int n_up, int max_steps,
+                                    double tol, double *h_out, int64_t *n_unconverged)
+{
+    CK(check_common(ctx, n, SPH_PREC_FP64));
+    if (!pos || !h_out || (mode != SPH_H_KTH && !h_in))
+        return fail(ctx, SPH_ERR_ARG, "pos, h_out (and h_in unless mode = SPH_H_KTH) are required");
+    if (mode < 0 || mode > 2) return fail(ctx, SPH_ERR_ARG, "unknown h mode");
+    if (mode == SPH_H_KTH && (param < 0 || param >= (double)n))
+        return fail(ctx, SPH_ERR_ARG, "kth must satisfy 0 <= kth < n");
+    if (mode == SPH_H_KTH && (int)param + 1 > SPH_HCAP)
+        return fail(ctx, SPH_ERR_ARG, "kth + 1 exceeds the candidate capacity (256)");
+    if (targets && m < 1) return fail(ctx, SPH_ERR_ARG, "m must be >= 1");
+    CK(set_pos(ctx, n, pos));
+    CK(ensure_moments(ctx));
+    hipStream_t st = ctx->st;
+    int64_t mm = targets ? m : n;
+    if (targets) {
+        if (mm > n) {
+            CK(ensure(ctx, ctx->tx, (size_t)mm * 8));
+            CK(ensure(ctx, ctx->ty, (size_t)mm * 8));
+            CK(ensure(ctx, ctx->tz, (size_t)mm * 8));
+            CK(ensure(ctx, ctx->hin, (size_t)mm * 8));
+            CK(ensure(ctx, ctx->hout, (size_t)mm * 8));
+            CK(ensure(ctx, ctx->hR, (size_t)mm * 8));
+            CK(ensure(ctx, ctx->hsteps, (size_t)mm * 4));
+            CK(ensure(ctx, ctx->hstate, (size_t)mm * 4));
+            CK(ensure(ctx, ctx->stage, (size_t)mm * 24));
+        }
+        HIPCK(hipMemcpyAsync(ctx->stage.p, targets, sizeof(double) * 3 * mm, hipMemcpyHostToDevice, st));
+        k_deinterleave<<<sph_blocks(mm, 256), 256, 0, st>>>(mm, D(stage), D(tx), D(ty), D(tz));
+    }
+    const double *tx = targets ? D(tx) : D(x), *ty = targets ? D(ty) : D(y), *tz = targets ? D(tz) : D(z);
+    if (h_in) {
+        if (targets) HIPCK(hipMemcpyAsync(ctx->hin.p, h_in, sizeof(double) * mm, hipMemcpyHostToDevice, st));
+        else CK(put_scalar(ctx, h_in, ctx->hin, nullptr, nullptr, nullptr));
+    } else {
+        // k-th distance guess from the mean density of the bounding box
+        double box[8];
+        HIPCK(hipMemcpyAsync(box, ctx->box.p, sizeof(double) * 7, hipMemcpyDeviceToHost, st));
+        HIPCK(hipStreamSynchronize(st));
+        double L = box[6] > 0 ? 65536.0 / box[6] : 1.0;
+        double g = 0.5 * L * std::cbrt((param + 1.0) / (double)n);
+        HIPCK(sph::fill<double>(st, mm, D(hin), g));
+    }
+    int64_t unc = 0;
+    CK(run_hsolve(ctx, mm, tx, ty, tz, mode, param, n_low, n_up, max_steps, tol, &unc, nullptr));
+    if (n_unconverged) *n_unconverged = unc;
+    if (targets) {
+        HIPCK(hipMemcpyAsync(h_out, ctx->hout.p, sizeof(double) * mm, hipMemcpyDeviceToHost, st));
+        HIPCK(hipStreamSynchronize(st));
+    } else {
+        CK(get_scalar<double>(ctx, ctx->hout, h_out));
+    }
+    return SPH_OK;
+}
+
+// =========================================================== neighbours
+extern "C" int sph_neighbours(sph_ctx *ctx, int64_t n, const double *pos, const double *h, int64_t *offsets,
+                              int32_t *idx, int64_t idx_cap)
+{
+    CK(check_common(ctx, n, SPH_PREC_FP64));
+    if (!pos || !h || !offsets) return fail(ctx, SPH_ERR_ARG, "pos, h and offsets are required");
+    CK(geom(ctx, n, pos, h, nullptr));
+    int64_t total = ctx->csr_total;
+    std::vector<int64_t> soff(n + 1);
+    std::vector<int32_t> orig(n);
+    HIPCK(hipMemcpyAsync(soff.data(), ctx->off.p, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, ctx->st));
+    HIPCK(hipMemcpyAsync(orig.data(), ctx->orig.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->st));
+    HIPCK(hipStreamSynchronize(ctx->st));
+    // offsets in caller order
+    std::vector<int64_t> cnt(n);
+    for (int64_t s = 0; s < n; s++) cnt[orig[s]] = soff[s + 1] - soff[s];
+    offsets[0] = 0;
+    for (int64_t i = 0; i < n; i++) offsets[i + 1] = offsets[i] + cnt[i];
+    if (!idx || idx_cap < total)
+        return idx ? fail(ctx, SPH_ERR_CAPACITY, "idx capacity " + std::to_string(idx_cap) + " < " +
+                                                     std::to_string(total))
+                   : SPH_OK;
+    std::vector<int32_t> sidx(total);
+    HIPCK(hipMemcpyAsync(sidx.data(), ctx->idx.p, sizeof(int32_t) * total, hipMemcpyDeviceToHost, ctx->st));
+    HIPCK(hipStreamSynchronize(ctx->st));
+    for (int64_t s = 0; s < n; s++) {
+        int64_t r = orig[s], o = offsets[r];
+        for (int64_t k = soff[s]; k < soff[s + 1]; k++) idx[o++] = orig[sidx[k]];
+        std::sort(idx + offsets[r], idx + offsets[r + 1]);
+    }
+    return SPH_OK;
+}
+
+// =========================================================== hydro
+extern "C" int sph_density(sph_ctx *ctx, int64_t n, const double *pos, const double *m, const double *h,
+                           int precision, double *rho_out)
+{
+    CK(check_common(ctx, n, precision));
+    if (!pos || !m || !h || !rho_out) return fail(ctx, SPH_ERR_ARG, "null argument");
+    CK(geom(ctx, n, pos, h, m));
+    const int64_t *off = P_<int64_t>(ctx->off);
+    if (precision == SPH_PREC_MIXED) {
+        HIPCK(sph::density<float>(ctx->st, n, D(x), D(y), D(z), D(m), D(h), nullptr, 0.0, off, I32(idx),
+                                  P_<float>(ctx->rho), nullptr, nullptr, nullptr));
+        return get_scalar<float>(ctx, ctx->rho, rho_out);
+    }
+    HIPCK(sph::density<double>(ctx->st, n, D(x), D(y), D(z), D(m), D(h), nullptr, 0.0, off, I32(idx), D(rho),
+                               nullptr, nullptr, nullptr));
+    return get_scalar<double>(ctx, ctx->rho, rho_out);
+}
+
+namespace {
+template <typename R>
+int divcurl_impl(sph_ctx *ctx, const double *vel, const double *rho, double *dv, double *cv)
+{
+    int64_t n = ctx->n;
+    CK(put_vec3(ctx, vel, ctx->vx, ctx->vy, ctx->vz));
+    CK(put_real<R>(ctx, rho, ctx->rho));
+    HIPCK(sph::divcurl<R>(ctx->st, n, D(x), D(y), D(z), D(vx), D(vy), D(vz), D(m), D(h), P_<R>(ctx->rho), nullptr,
+                          P_<int64_t>(ctx->off), I32(idx), P_<R>(ctx->divV), P_<R>(ctx->curlV), nullptr));
+    CK(get_scalar<R>(ctx, ctx->divV, dv));
+    return get_scalar<R>(ctx, ctx->curlV, cv);
+}
+
+template <typename R>
+int hydro_inputs(sph_ctx *ctx, const double *vel, const double *rho, const double *P, const double *c,
+                 const double *divV, const double *curlV)
+{
+    int64_t n = ctx->n;
+    CK(put_vec3(ctx, vel, ctx->vx, ctx->vy, ctx->vz));
+    CK(put_real<R>(ctx, rho, ctx->rho));
+    CK(put_real<R>(ctx, P, ctx->P));
+    CK(put_real<R>(ctx, c, ctx->c));
+    bool bal = divV && curlV;
+    if (bal) {
+        CK(put_real<R>(ctx, divV, ctx->divV));
+        CK(put_real<R>(ctx, curlV, ctx->curlV));
+    }
+    HIPCK(sph::prep<R>(ctx->st, n, P_<R>(ctx->rho), P_<R>(ctx->P), P_<R>(ctx->c), P_<R>(ctx->divV),
+                       P_<R>(ctx->curlV), D(h), P_<R>(ctx->prho2), bal ? P_<R>(ctx->bal) : nullptr));
+    return SPH_OK;
+}
+
+template <typename R>
+int acc_impl(sph_ctx *ctx, const double *vel, const double *rho, const double *P, const double *c,
+             const double *divV, const double *curlV, double alpha, double *acc)
+{
+    CK(hydro_inputs<R>(ctx, vel, rho, P, c, divV, curlV));
+    bool bal = divV && curlV;
+    HIPCK(sph::accsph<R>(ctx->st, ctx->n, D(x), D(y), D(z), D(vx), D(vy), D(vz), D(m), D(h), P_<R>(ctx->rho),
+                         P_<R>(ctx->prho2), P_<R>(ctx->c), bal ? P_<R>(ctx->bal) : nullptr, alpha,
+                         P_<int64_t>(ctx->off), I32(idx), nullptr, nullptr, nullptr, D(ax), D(ay), D(az)));
+    return get_vec3(ctx, ctx->ax, ctx->ay, ctx->az, acc);
+}
+
+template <typename R>
+int dudt_impl(sph_ctx *ctx, const double *vel, const double *rho, const double *P, const double *c,
+              const double *divV, const double *curlV, double alpha, double *out)
+{
+    CK(hydro_inputs<R>(ctx, vel, rho, P, c, divV, curlV));
+    bool bal = divV && curlV;
+    HIPCK(sph::dudt<R>(ctx->st, ctx->n, D(x), D(y), D(z), D(vx), D(vy), D(vz), D(m), D(h), P_<R>(ctx->rho),
+                       P_<R>(ctx->prho2), P_<R>(ctx->c), bal ? P_<R>(ctx->bal) : nullptr, alpha,
+                       P_<int64_t>(ctx->off), I32(idx), P_<R>(ctx->du)));
+    return get_scalar<R>(ctx, ctx->du, out);
+}
+}  // namespace
+
+extern "C" int sph_div_curl(sph_ctx *ctx, int64_t n, const double *pos, const double *vel, const double *rho,
+                            const double *m, const double *h, int precision, double *divV_out, double *curlV_out)
+{
+    CK(check_common(ctx, n, precision));
+    if (!pos || !vel || !rho || !m || !h || !divV_out || !curlV_out) return fail(ctx, SPH_ERR_ARG, "null argument");
+    CK(geom(ctx, n, pos, h, m));
+    return precision == SPH_PREC_MIXED ? divcurl_impl<float>(ctx, vel, rho, divV_out, curlV_out)
+                                       : divcurl_impl<double>(ctx, vel, rho, divV_out, curlV_out);
+}
+
+extern "C" int sph_acc_hydro(sph_ctx *ctx, int64_t n, const double *pos, const double *vel, const double *rho,
+                             const double *P, const double *c, const double *h, const double *m, const double *divV,
+                             const double *curlV, double alpha, int precision, double *acc_out)
+{
+    CK(check_common(ctx, n, precision));
+    if (!pos || !vel || !rho || !P || !c || !h || !m || !acc_out) return fail(ctx, SPH_ERR_ARG, "null argument");
+    if ((divV == nullptr) != (curlV == nullptr)) return fail(ctx, SPH_ERR_ARG, "divV and curlV go together");
+    CK(geom(ctx, n, pos, h, m));
+    return precision == SPH_PREC_MIXED ? acc_impl<float>(ctx, vel, rho, P, c, divV, curlV, alpha, acc_out)
+                                       : acc_impl<double>(ctx, vel, rho, P, c, divV, curlV, alpha, acc_out);
+}
+
+extern "C" int sph_dudt(sph_ctx *ctx, int64_t n, const double *pos, const double *vel, const double *rho,
+                        const double *P, const double *c, const double *h, const double *m, const double *divV,
+                        const double *curlV, double alpha, int precision, double *dudt_out)
+{
+    CK(check_common(ctx, n, precision));
+    if (!pos || !vel || !rho || !P || !c || !h || !m || !dudt_out) return fail(ctx, SPH_ERR_ARG, "null argument");
+    if ((divV == nullptr) != (curlV == nullptr)) return fail(ctx, SPH_ERR_ARG, "divV and curlV go together");
+    CK(geom(ctx, n, pos, h, m));
+    return precision == SPH_PREC_MIXED ? dudt_impl<float>(ctx, vel, rho, P, c, divV, curlV, alpha, dudt_out)
+                                       : dudt_impl<double>(ctx, vel, rho, P, c, divV, curlV, alpha, dudt_out);
+}
+
+// =========================================================== gravity
+extern "C" int sph_gravity(sph_ctx *ctx, int64_t n, const double *pos, const double *m, const double *eps, double G,
+                           double theta, int soft_kind, double eps_plummer, int precision, double *acc_out,
+                           int64_t *n_interactions)
+{
+    CK(check_common(ctx, n, precision));
+    if (!pos || !m || !acc_out) return fail(ctx, SPH_ERR_ARG, "null argument");
+    if (soft_kind != SPH_SOFT_SPLINE && soft_kind != SPH_SOFT_PLUMMER) return fail(ctx, SPH_ERR_ARG, "bad soft_kind");
+    if (!(theta >= 0.0)) return fail(ctx, SPH_ERR_ARG, "theta must be >= 0");
+    CK(set_pos(ctx, n, pos));
+    bool cm = false, ce = false;
+    CK(put_scalar(ctx, m, ctx->m, &ctx->cm, &ctx->m_ok, &cm));
+    if (eps) CK(put_scalar(ctx, eps, ctx->eps, &ctx->ceps, &ctx->eps_ok, &ce));
+    else if (ctx->eps_ok) {
+        ctx->eps_ok = false;
+        ctx->ceps.clear();
+        ce = true;
+    }
+    if (cm || ce) ctx->moments_ok = false;
+    CK(ensure_moments(ctx));
+    unsigned long long *cnt = P_<unsigned long long>(ctx->hcount);
+    HIPCK(sph::fill<unsigned long long>(ctx->st, 2, cnt, 0ull));
+    const double *e = ctx->eps_ok ? D(eps) : nullptr;
+    if (precision == SPH_PREC_MIXED)
+        HIPCK(sph::gravity<float>(ctx->st, n, D(x), D(y), D(z), D(m), e, P_<WNode>(ctx->nodes), SPH_BUCKET, theta, G,
+                                  soft_kind, eps_plummer, D(gx), D(gy), D(gz), cnt));
+    else
+        HIPCK(sph::gravity<double>(ctx->st, n, D(x), D(y), D(z), D(m), e, P_<WNode>(ctx->nodes), SPH_BUCKET, theta,
+                                   G, soft_kind, eps_plummer, D(gx), D(gy), D(gz), cnt));
+    if (n_interactions) {
+        unsigned long long hc[2];
+        HIPCK(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, ctx->st));
+        HIPCK(hipStreamSynchronize(ctx->st));
+        n_interactions[0] = (int64_t)hc[0];
+        n_interactions[1] = (int64_t)hc[1];
+    }
+    return get_vec3(ctx, ctx->gx, ctx->gy, ctx->gz, acc_out);
+}
+
+extern "C" int sph_kinetics(sph_ctx *ctx, int64_t n, const double *vel, const double *m, double *out5)
+{
+    CK(check_common(ctx, n, SPH_PREC_FP64));
+    if (!vel || !m || !out5) return fail(ctx, SPH_ERR_ARG, "null argument");
+    // order-independent: plain (unsorted) upload into scratch
+    CK(ensure(ctx, ctx->stage, sizeof(double) * 3 * n));
+    CK(ensure(ctx, ctx->tx, sizeof(double) * n));
+    CK(ensure(ctx, ctx->ty, sizeof(double) * n));
+    CK(ensure(ctx, ctx->tz, sizeof(double) * n));
+    CK(ensure(ctx, ctx->tmpd, sizeof(double) * n));
+    CK(ensure(ctx, ctx->kpart, 5 * 1024 * 8));
+    CK(ensure(ctx, ctx->kout, 8 * 8));
+    hipStream_t st = ctx->st;
+    HIPCK(hipMemcpyAsync(ctx->stage.p, vel, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
+    HIPCK(hipMemcpyAsync(ctx->tmpd.p, m, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    k_deinterleave<<<sph_blocks(n, 256), 256, 0, st>>>(n, D(stage), D(tx), D(ty), D(tz));
+    HIPCK(sph::kinetics(st, n, D(tx), D(ty), D(tz), D(tmpd), D(kpart), D(kout)));
+    HIPCK(hipMemcpyAsync(out5, ctx->kout.p, sizeof(double) * 5, hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+    return SPH_OK;
+}
+
+// =========================================================== resident step
+extern "C" int sph_state_set(sph_ctx *ctx, int64_t n, const double *pos, const double *vel, const double *m,
+                             const double *u, const double *h)
+{
+    CK(check_common(ctx, n, SPH_PREC_FP64));
+    if (!pos || !vel || !m || !u || !h) return fail(ctx, SPH_ERR_ARG, "null argument");
+    ctx->resident = false;
+    invalidate_all(ctx);
+    CK(set_pos(ctx, n, pos));
+    CK(put_vec3(ctx, vel, ctx->vx, ctx->vy, ctx->vz));
+    CK(put_scalar(ctx, m, ctx->m, nullptr, nullptr, nullptr));
+    CK(put_scalar(ctx, u, ctx->u, nullptr, nullptr, nullptr));
+    CK(put_scalar(ctx, h, ctx->h, nullptr, nullptr, nullptr));
+    HIPCK(sph::fill<double>(ctx->st, n, D(duprev), 0.0));
+    HIPCK(sph::fill<double>(ctx->st, n, D(ax), 0.0));
+    HIPCK(sph::fill<double>(ctx->st, n, D(ay), 0.0));
+    HIPCK(sph::fill<double>(ctx->st, n, D(az), 0.0));
+    HIPCK(hipStreamSynchronize(ctx->st));
+    invalidate_all(ctx);  // stateless caches no longer describe the device arrays
+    ctx->resident = true;
+    ctx->primed = false;
+    ctx->stats.n = n;
+    return SPH_OK;
+}
+
+extern "C" int sph_state_get(sph_ctx *ctx, double *pos, double *vel, double *u, double *h, double *rho, double *acc)
+{
+    if (!ctx) return SPH_ERR_ARG;
+    if (!ctx->resident) return fail(ctx, SPH_ERR_STATE, "no resident state (call sph_state_set first)");
+    HIPCK(hipSetDevice(ctx->device));
+    if (pos) CK(get_vec3(ctx, ctx->x, ctx->y, ctx->z, pos));
+    if (vel) CK(get_vec3(ctx, ctx->vx, ctx->vy, ctx->vz, vel));
+    if (u) CK(get_scalar<double>(ctx, ctx->u, u));
+    if (h) CK(get_scalar<double>(ctx, ctx->h, h));
+    if (acc) CK(get_vec3(ctx, ctx->ax, ctx->ay, ctx->az, acc));
+    if (rho) {
+        if (!ctx->primed) return fail(ctx, SPH_ERR_STATE, "rho is available after sph_prime/sph_step");
+        if (ctx->rho_prec == SPH_PREC_FP64) CK(get_scalar<double>(ctx, ctx->rho, rho));
+        else CK(get_scalar<float>(ctx, ctx->rho, rho));
+    }
+    return SPH_OK;
+}
+
+namespace {
+
+// Re-sort the resident state by the current positions.
+int reorder_state(sph_ctx *ctx)
+{
+    int64_t n = ctx->n;
+    hipStream_t st = ctx->st;
+    CK(sort_current(ctx, n));
+    const int32_t *perm = I32(vals0);
+    Buf *arr[] = {&ctx->x, &ctx->y, &ctx->z, &ctx->vx, &ctx->vy, &ctx->vz, &ctx->m, &ctx->h,
+                  &ctx->u, &ctx->duprev, &ctx->ax, &ctx->ay, &ctx->az};
+    for (Buf *b : arr) {
+        HIPCK(sph::gather<double, double>(st, n, perm, (const double *)b->p, 1, 0, D(tmpd)));
+        std::swap(*b, ctx->tmpd);
+    }
+    HIPCK(sph::compose(st, n, I32(orig), perm, I32(tmpi)));
+    std::swap(ctx->orig, ctx->tmpi);
+    return SPH_OK;
+}
+
+template <typename R>
+int forces(sph_ctx *ctx, const sph_step_params *p, int first)
+{
+    int64_t n = ctx->n;
+    hipStream_t st = ctx->st;
+    // --- order + tree topology
+    ph_begin(ctx, SPH_PH_ORDER);
+    CK(reorder_state(ctx));
+    ph_end(ctx, SPH_PH_ORDER);
+    ph_begin(ctx, SPH_PH_TOPOLOGY);
+    HIPCK(sph::tree_topology(st, n, P_<uint64_t>(ctx->keys0), P_<WNode>(ctx->nodes), I32(right), I32(parent)));
+    ph_end(ctx, SPH_PH_TOPOLOGY);
+    // boxes are needed by the h walk; hmax is refreshed after the solve
+    ph_begin(ctx, SPH_PH_HSOLVE);
+    HIPCK(sph::tree_moments(st, n, D(x), D(y), D(z), D(m), D(h), nullptr, P_<WNode>(ctx->nodes), I32(right),
+                            I32(parent), P_<uint32_t>(ctx->flags)));
+    HIPCK(hipMemcpyAsync(ctx->hin.p, ctx->h.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    int64_t unc = 0;
+    CK(run_hsolve(ctx, n, D(x), D(y), D(z), SPH_H_NEWTON, p->n_ngb, 0, 0, 0, p->h_tol, &unc, nullptr));
+    std::swap(ctx->h, ctx->hout);
+    ctx->stats.h_unconverged += unc;
+    ph_end(ctx, SPH_PH_HSOLVE);
+    // --- moments with the new h (and eps = h for the spline softening)
+    ph_begin(ctx, SPH_PH_MOMENTS);
+    const double *eps = p->soft_kind == SPH_SOFT_SPLINE ? D(h) : nullptr;
+    HIPCK(sph::tree_moments(st, n, D(x), D(y), D(z), D(m), D(h), eps, P_<WNode>(ctx->nodes), I32(right),
+                            I32(parent), P_<uint32_t>(ctx->flags)));
+    ph_end(ctx, SPH_PH_MOMENTS);
+    ph_begin(ctx, SPH_PH_NGB);
+    CK(build_csr(ctx));
+    ph_end(ctx, SPH_PH_NGB);
+    const int64_t *off = P_<int64_t>(ctx->off);
+    const int32_t *idx = I32(idx);
+    ph_begin(ctx, SPH_PH_DENSITY);
+    HIPCK(sph::density<R>(st, n, D(x), D(y), D(z), D(m), D(h), D(u), p->gamma, off, idx, P_<R>(ctx->rho),
+                          P_<R>(ctx->P), P_<R>(ctx->c), P_<R>(ctx->prho2)));
+    ph_end(ctx, SPH_PH_DENSITY);
+    ph_begin(ctx, SPH_PH_GRAVITY);
+    unsigned long long *cnt = P_<unsigned long long>(ctx->hcount) + 2;
+    HIPCK(sph::fill<unsigned long long>(st, 2, cnt, 0ull));
+    if (p->G != 0.0) {
+        HIPCK(sph::gravity<R>(st, n, D(x), D(y), D(z), D(m), eps, P_<WNode>(ctx->nodes), SPH_BUCKET, p->theta, p->G,
+                              p->soft_kind, p->eps_plummer, D(gx), D(gy), D(gz), cnt));
+    }
+    ph_end(ctx, SPH_PH_GRAVITY);
+    ph_begin(ctx, SPH_PH_DIVCURL);
+    R *bal = p->balsara ? P_<R>(ctx->bal) : nullptr;
+    if (p->balsara)
+        HIPCK(sph::divcurl<R>(st, n, D(x), D(y), D(z), D(vx), D(vy), D(vz), D(m), D(h), P_<R>(ctx->rho),
+                              P_<R>(ctx->c), off, idx, P_<R>(ctx->divV), P_<R>(ctx->curlV), bal));
+    ph_end(ctx, SPH_PH_DIVCURL);
+    ph_begin(ctx, SPH_PH_ACCSPH);
+    const bool g = p->G != 0.0;
+    HIPCK(sph::accsph<R>(st, n, D(x), D(y), D(z), D(vx), D(vy), D(vz), D(m), D(h), P_<R>(ctx->rho),
+                         P_<R>(ctx->prho2), P_<R>(ctx->c), bal, p->alpha, off, idx, g ? D(gx) : nullptr,
+                         g ? D(gy) : nullptr, g ? D(gz) : nullptr, D(ax), D(ay), D(az)));
+    ph_end(ctx, SPH_PH_ACCSPH);
+    if (!first) {
+        ph_begin(ctx, SPH_PH_KICK);
+        HIPCK(sph::kick_drift(st, n, D(vx), D(vy), D(vz), D(ax), D(ay), D(az), 0.5 * p->dt, 0.0, nullptr, nullptr,
+                              nullptr));
+        ph_end(ctx, SPH_PH_KICK);
+    }
+    ph_begin(ctx, SPH_PH_DUDT);
+    HIPCK(sph::dudt<R>(st, n, D(x), D(y), D(z), D(vx), D(vy), D(vz), D(m), D(h), P_<R>(ctx->rho), P_<R>(ctx->prho2),
+                       P_<R>(ctx->c), bal, p->alpha, off, idx, P_<R>(ctx->du)));
+    ph_end(ctx, SPH_PH_DUDT);
+    ph_begin(ctx, SPH_PH_UUPDATE);
+    HIPCK(sph::u_update<R>(st, n, D(u), P_<R>(ctx->du), D(duprev), p->dt, first));
+    ph_end(ctx, SPH_PH_UUPDATE);
+    return SPH_OK;
+}
+
+int check_params(sph_ctx *ctx, const sph_step_params *p)
+{
+    if (!ctx || !p) return SPH_ERR_ARG;
+    if (!ctx->resident) return fail(ctx, SPH_ERR_STATE, "no resident state (call sph_state_set first)");
+    if (p->precision != SPH_PREC_MIXED && p->precision != SPH_PREC_FP64) return fail(ctx, SPH_ERR_ARG, "bad precision");
+    if (p->soft_kind != SPH_SOFT_SPLINE && p->soft_kind != SPH_SOFT_PLUMMER) return fail(ctx, SPH_ERR_ARG, "bad soft_kind");
+    if (!(p->dt > 0.0) || !(p->n_ngb > 0.0) || !(p->h_tol > 0.0) || !(p->theta >= 0.0))
+        return fail(ctx, SPH_ERR_ARG, "dt, n_ngb, h_tol must be > 0 and theta >= 0");
+    HIPCK(hipSetDevice(ctx->device));
+    return SPH_OK;
+}
+
+int finish_step(sph_ctx *ctx, const sph_step_params *p)
+{
+    HIPCK(hipStreamSynchronize(ctx->st));
+    unsigned long long hc[2];
+    HIPCK(hipMemcpy(hc, P_<unsigned long long>(ctx->hcount) + 2, sizeof(hc), hipMemcpyDeviceToHost));
+    ctx->stats.grav_pp = (int64_t)hc[0];
+    ctx->stats.grav_pn = (int64_t)hc[1];
+    if (ctx->timing) {
+        for (int ph = 0; ph < SPH_PH_COUNT; ph++) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, ctx->ev[2 * ph], ctx->ev[2 * ph + 1]) == hipSuccess) {
+                ctx->stats.phase_ms[ph] += ms;
+                ctx->stats.phase_calls[ph] += 1;
+            }
+        }
+    }
+    ctx->rho_prec = p->precision;
+    return SPH_OK;
+}
+
+}  // namespace
+
+extern "C" int sph_prime(sph_ctx *ctx, const sph_step_params *p)
+{
+    CK(check_params(ctx, p));
+    ctx->timing = false;
+    int r = p->precision == SPH_PREC_MIXED ? forces<float>(ctx, p, 1) : forces<double>(ctx, p, 1);
+    if (r != SPH_OK) return r;
+    CK(finish_step(ctx, p));
+    ctx->primed = true;
+    return SPH_OK;
+}
+
+extern "C" int sph_step(sph_ctx *ctx, const sph_step_params *p)
+{
+    CK(check_params(ctx, p));
+    if (!ctx->primed) return fail(ctx, SPH_ERR_STATE, "call sph_prime before sph_step");
+    ctx->timing = p->timing != 0;
+    int64_t n = ctx->n;
+    ph_begin(ctx, SPH_PH_KICKDRIFT);
+    HIPCK(sph::kick_drift(ctx->st, n, D(vx), D(vy), D(vz), D(ax), D(ay), D(az), 0.5 * p->dt, p->dt, D(x), D(y), D(z)));
+    ph_end(ctx, SPH_PH_KICKDRIFT);
+    int r = p->precision == SPH_PREC_MIXED ? forces<float>(ctx, p, 0) : forces<double>(ctx, p, 0);
+    if (r != SPH_OK) return r;
+    CK(finish_step(ctx, p));
+    ctx->stats.steps += 1;
+    ctx->timing = false;
+    return SPH_OK;
+}
+
+extern "C" int sph_get_stats(sph_ctx *ctx, sph_stats *out)
+{
+    if (!ctx || !out) return SPH_ERR_ARG;
+    *out = ctx->stats;
+    return SPH_OK;
+}
+
+extern "C" int sph_reset_stats(sph_ctx *ctx)
+{
+    if (!ctx) return SPH_ERR_ARG;
+    int64_t n = ctx->stats.n;
+    std::memset(&ctx->stats, 0, sizeof(ctx->stats));
+    ctx->stats.n = n;
+    return SPH_OK;
+}

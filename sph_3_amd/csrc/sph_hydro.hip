@@ -1,0 +1,264 @@
+// sph_hydro.hip -- SPH sums over compact CSR neighbour lists: density with
+// the fused equation of state, div/curl v with the Balsara factor, pressure
+// plus artificial-viscosity acceleration, and du/dt.
+//
+// Reference formulas: sph_code/libsx.py:10-44 (density), :91-96 (pressure),
+// :100-177 and :353-421 (acceleration, du/dt), sph_code/shear.py:6-69,
+// :149-235, :332-406 (div/curl, Balsara-switched forms); GPU twins
+// EV/lib.h:598-651, 713-736, 741-824, 829-925, 949-1040.
+//
+// Each particle is handled by a group of G consecutive lanes that stride
+// through its CSR row (coalesced index reads), accumulate with Kahan
+// compensation in the pass type R and combine with a TwoSum lane reduction.
+// Pair arithmetic is in R; coordinate and velocity differences are taken in
+// FP64 and rounded once to R.
+#include "sph_hydro.h"
+
+#define HG 16  // lanes per particle
+
+template <typename R, int G>
+__global__ void __launch_bounds__(256) k_density(int64_t n, const double *__restrict__ x,
+                                                 const double *__restrict__ y, const double *__restrict__ z,
+                                                 const double *__restrict__ m, const double *__restrict__ h,
+                                                 const double *__restrict__ u, double gamma,
+                                                 const int64_t *__restrict__ off, const int32_t *__restrict__ idx,
+                                                 R *__restrict__ rho, R *__restrict__ P, R *__restrict__ c,
+                                                 R *__restrict__ prho2)
+{
+    int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t i = gid / G;
+    int l = (int)(gid % G);
+    bool valid = i < n;
+    Kahan<R> acc;
+    if (valid) {
+        const double xi = x[i], yi = y[i], zi = z[i], hi = h[i];
+        const int64_t e = off[i + 1];
+        for (int64_t k = off[i] + l; k < e; k += G) {
+            int j = idx[k];
+            R dx = (R)(xi - x[j]), dy = (R)(yi - y[j]), dz = (R)(zi - z[j]);
+            R rr = sph_sqrt<R>(dx * dx + dy * dy + dz * dz);
+            R hij = (R)(0.5 * (hi + h[j]));
+            acc.add((R)m[j] * spline_W<R>(rr, hij));
+        }
+    }
+    R s = group_sum<R, G>(acc.s, -acc.c);
+    if (valid && l == 0) {
+        rho[i] = s;
+        if (u) {
+            R ui = (R)u[i];
+            R Pi = (R)(gamma - 1.0) * s * ui;
+            P[i] = Pi;
+            c[i] = sph_sqrt<R>((R)(gamma * (gamma - 1.0)) * ui);
+            prho2[i] = Pi / (s * s);
+        }
+    }
+}
+
+template <typename R, int G>
+__global__ void __launch_bounds__(256) k_divcurl(int64_t n, const double *__restrict__ x,
+                                                 const double *__restrict__ y, const double *__restrict__ z,
+                                                 const double *__restrict__ vx, const double *__restrict__ vy,
+                                                 const double *__restrict__ vz, const double *__restrict__ m,
+                                                 const double *__restrict__ h, const R *__restrict__ rho,
+                                                 const R *__restrict__ c, const int64_t *__restrict__ off,
+                                                 const int32_t *__restrict__ idx, R *__restrict__ divV,
+                                                 R *__restrict__ curlV, R *__restrict__ bal)
+{
+    int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t i = gid / G;
+    int l = (int)(gid % G);
+    bool valid = i < n;
+    Kahan<R> sd, scx, scy, scz;
+    if (valid) {
+        const double xi = x[i], yi = y[i], zi = z[i], hi = h[i];
+        const double vxi = vx[i], vyi = vy[i], vzi = vz[i];
+        const int64_t e = off[i + 1];
+        for (int64_t k = off[i] + l; k < e; k += G) {
+            int j = idx[k];
+            R dx = (R)(xi - x[j]), dy = (R)(yi - y[j]), dz = (R)(zi - z[j]);
+            R rr = sph_sqrt<R>(dx * dx + dy * dy + dz * dz);
+            R nW = spline_nW<R>(rr, (R)(0.5 * (hi + h[j])));
+            R gx = nW * dx, gy = nW * dy, gz = nW * dz;
+            R ux = (R)(vxi - vx[j]), uy = (R)(vyi - vy[j]), uz = (R)(vzi - vz[j]);
+            R mj = (R)m[j];
+            sd.add(-mj * (ux * gx + uy * gy + uz * gz));
+            scx.add(mj * (uy * gz - uz * gy));
+            scy.add(mj * (uz * gx - ux * gz));
+            scz.add(mj * (ux * gy - uy * gx));
+        }
+    }
+    R d = group_sum<R, G>(sd.s, -sd.c);
+    R cx = group_sum<R, G>(scx.s, -scx.c);
+    R cy = group_sum<R, G>(scy.s, -scy.c);
+    R cz = group_sum<R, G>(scz.s, -scz.c);
+    if (valid && l == 0) {
+        R ir = (R)1 / rho[i];
+        R dv = d * ir;
+        dv = dv < (R)0 ? -dv : dv;
+        R cv = sph_sqrt<R>(cx * cx + cy * cy + cz * cz) * ir;
+        divV[i] = dv;
+        curlV[i] = cv;
+        if (bal) bal[i] = dv / (dv + cv + (R)0.0001 * c[i] / (R)h[i]);
+    }
+}
+
+// Artificial viscosity Pi_ij (Gadget-2 form), libsx.py:146-160.
+template <typename R>
+__device__ __forceinline__ R av_pi(R vr, R rr, R ci, R cj, R rhoi, R rhoj, R alpha)
+{
+    if (vr > (R)0) return (R)0;
+    R w = vr / (rr + (R)1e-5);
+    R vsig = ci + cj - (R)3 * w;
+    return (R)-0.5 * alpha * vsig * w / ((R)0.5 * (rhoi + rhoj));
+}
+
+template <typename R, int G>
+__global__ void __launch_bounds__(256) k_accsph(int64_t n, const double *__restrict__ x,
+                                                const double *__restrict__ y, const double *__restrict__ z,
+                                                const double *__restrict__ vx, const double *__restrict__ vy,
+                                                const double *__restrict__ vz, const double *__restrict__ m,
+                                                const double *__restrict__ h, const R *__restrict__ rho,
+                                                const R *__restrict__ prho2, const R *__restrict__ c,
+                                                const R *__restrict__ bal, double alpha,
+                                                const int64_t *__restrict__ off, const int32_t *__restrict__ idx,
+                                                const double *__restrict__ gx_in, const double *__restrict__ gy_in,
+                                                const double *__restrict__ gz_in, double *__restrict__ ax,
+                                                double *__restrict__ ay, double *__restrict__ az)
+{
+    int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t i = gid / G;
+    int l = (int)(gid % G);
+    bool valid = i < n;
+    Kahan<R> sx, sy, sz;
+    if (valid) {
+        const double xi = x[i], yi = y[i], zi = z[i], hi = h[i];
+        const double vxi = vx[i], vyi = vy[i], vzi = vz[i];
+        const R rhoi = rho[i], pi_ = prho2[i], ci = c[i], fi = bal ? bal[i] : (R)1;
+        const R al = (R)alpha;
+        const int64_t e = off[i + 1];
+        for (int64_t k = off[i] + l; k < e; k += G) {
+            int j = idx[k];
+            R dx = (R)(xi - x[j]), dy = (R)(yi - y[j]), dz = (R)(zi - z[j]);
+            R rr = sph_sqrt<R>(dx * dx + dy * dy + dz * dz);
+            R nW = spline_nW<R>(rr, (R)(0.5 * (hi + h[j])));
+            R ux = (R)(vxi - vx[j]), uy = (R)(vyi - vy[j]), uz = (R)(vzi - vz[j]);
+            R vr = ux * dx + uy * dy + uz * dz;
+            R PI = av_pi<R>(vr, rr, ci, c[j], rhoi, rho[j], al);
+            if (bal) PI = (R)0.5 * (fi + bal[j]) * PI;
+            R t = (R)m[j] * (pi_ + prho2[j] + PI) * nW;
+            sx.add(-t * dx);
+            sy.add(-t * dy);
+            sz.add(-t * dz);
+        }
+    }
+    R rx = group_sum<R, G>(sx.s, -sx.c);
+    R ry = group_sum<R, G>(sy.s, -sy.c);
+    R rz = group_sum<R, G>(sz.s, -sz.c);
+    if (valid && l == 0) {
+        double gx = gx_in ? gx_in[i] : 0.0, gy = gy_in ? gy_in[i] : 0.0, gz = gz_in ? gz_in[i] : 0.0;
+        ax[i] = (double)rx + gx;
+        ay[i] = (double)ry + gy;
+        az[i] = (double)rz + gz;
+    }
+}
+
+template <typename R, int G>
+__global__ void __launch_bounds__(256) k_dudt(int64_t n, const double *__restrict__ x,
+                                              const double *__restrict__ y, const double *__restrict__ z,
+                                              const double *__restrict__ vx, const double *__restrict__ vy,
+                                              const double *__restrict__ vz, const double *__restrict__ m,
+                                              const double *__restrict__ h, const R *__restrict__ rho,
+                                              const R *__restrict__ prho2, const R *__restrict__ c,
+                                              const R *__restrict__ bal, double alpha,
+                                              const int64_t *__restrict__ off, const int32_t *__restrict__ idx,
+                                              R *__restrict__ dudt)
+{
+    int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t i = gid / G;
+    int l = (int)(gid % G);
+    bool valid = i < n;
+    Kahan<R> s;
+    if (valid) {
+        const double xi = x[i], yi = y[i], zi = z[i], hi = h[i];
+        const double vxi = vx[i], vyi = vy[i], vzi = vz[i];
+        const R rhoi = rho[i], pi_ = prho2[i], ci = c[i], fi = bal ? bal[i] : (R)1;
+        const R al = (R)alpha;
+        const int64_t e = off[i + 1];
+        for (int64_t k = off[i] + l; k < e; k += G) {
+            int j = idx[k];
+            R dx = (R)(xi - x[j]), dy = (R)(yi - y[j]), dz = (R)(zi - z[j]);
+            R rr = sph_sqrt<R>(dx * dx + dy * dy + dz * dz);
+            R nW = spline_nW<R>(rr, (R)(0.5 * (hi + h[j])));
+            R ux = (R)(vxi - vx[j]), uy = (R)(vyi - vy[j]), uz = (R)(vzi - vz[j]);
+            R vr = ux * dx + uy * dy + uz * dz;
+            R PI = av_pi<R>(vr, rr, ci, c[j], rhoi, rho[j], al);
+            if (bal) PI = (R)0.5 * (fi + bal[j]) * PI;
+            s.add((R)m[j] * (pi_ + (R)0.5 * PI) * nW * vr);
+        }
+    }
+    R r = group_sum<R, G>(s.s, -s.c);
+    if (valid && l == 0) dudt[i] = r;
+}
+
+namespace sph {
+
+template <typename R>
+hipError_t density(hipStream_t st, int64_t n, const double *x, const double *y, const double *z, const double *m,
+                   const double *h, const double *u, double gamma, const int64_t *off, const int32_t *idx, R *rho, R *P,
+                   R *c, R *prho2)
+{
+    k_density<R, HG><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, m, h, u, gamma, off, idx, rho, P, c, prho2);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t divcurl(hipStream_t st, int64_t n, const double *x, const double *y, const double *z, const double *vx,
+                   const double *vy, const double *vz, const double *m, const double *h, const R *rho, const R *c,
+                   const int64_t *off, const int32_t *idx, R *divV, R *curlV, R *bal)
+{
+    k_divcurl<R, HG><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, rho, c, off, idx, divV,
+                                                              curlV, bal);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t accsph(hipStream_t st, int64_t n, const double *x, const double *y, const double *z, const double *vx,
+                  const double *vy, const double *vz, const double *m, const double *h, const R *rho,
+                  const R *prho2, const R *c, const R *bal, double alpha, const int64_t *off, const int32_t *idx,
+                  const double *gx, const double *gy, const double *gz, double *ax, double *ay, double *az)
+{
+    k_accsph<R, HG><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, rho, prho2, c, bal, alpha,
+                                                             off, idx, gx, gy, gz, ax, ay, az);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t dudt(hipStream_t st, int64_t n, const double *x, const double *y, const double *z, const double *vx,
+                const double *vy, const double *vz, const double *m, const double *h, const R *rho, const R *prho2,
+                const R *c, const R *bal, double alpha, const int64_t *off, const int32_t *idx, R *out)
+{
+    k_dudt<R, HG><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, rho, prho2, c, bal, alpha,
+                                                           off, idx, out);
+    return hipGetLastError();
+}
+
+#define INST(R)                                                                                                     \
+    template hipError_t density<R>(hipStream_t, int64_t, const double *, const double *, const double *,            \
+                                   const double *, const double *, const double *, double, const int64_t *,        \
+                                   const int32_t *, R *, R *, R *, R *);                                           \
+    template hipError_t divcurl<R>(hipStream_t, int64_t, const double *, const double *, const double *,            \
+                                   const double *, const double *, const double *, const double *, const double *, \
+                                   const R *, const R *, const int64_t *, const int32_t *, R *, R *, R *);         \
+    template hipError_t accsph<R>(hipStream_t, int64_t, const double *, const double *, const double *,             \
+                                  const double *, const double *, const double *, const double *, const double *,  \
+                                  const R *, const R *, const R *, const R *, double, const int64_t *,             \
+                                  const int32_t *, const double *, const double *, const double *, double *,       \
+                                  double *, double *);                                                             \
+    template hipError_t dudt<R>(hipStream_t, int64_t, const double *, const double *, const double *,               \
+                                const double *, const double *, const double *, const double *, const double *,    \
+                                const R *, const R *, const R *, const R *, double, const int64_t *,               \
+                                const int32_t *, R *);
+INST(float)
+INST(double)
+
+}  // namespace sph

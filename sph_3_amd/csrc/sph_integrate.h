@@ -1,0 +1,21 @@
+// sph_integrate.h -- host entry points of sph_integrate.hip.
+#pragma once
+#include "sph_common.h"
+
+namespace sph {
+
+// v += a*dtk; if x != null: x += v*dtd (with the kicked v).
+hipError_t kick_drift(hipStream_t st, int64_t n, double *vx, double *vy, double *vz, const double *ax,
+                      const double *ay, const double *az, double dtk, double dtd, double *x, double *y, double *z);
+// first: u += du*dt; else u += dt/2 (du + du_prev).  Then du_prev = du.
+template <typename R>
+hipError_t u_update(hipStream_t st, int64_t n, double *u, const R *du, double *du_prev, double dt, int first);
+// prho2 = P/rho^2; if bal != null: bal = divV / (divV + curlV + 1e-4 c/h).
+template <typename R>
+hipError_t prep(hipStream_t st, int64_t n, const R *rho, const R *P, const R *c, const R *divV, const R *curlV,
+                const double *h, R *prho2, R *bal);
+// out5 = [KE, px, py, pz, M]; part: 5*1024 doubles.
+hipError_t kinetics(hipStream_t st, int64_t n, const double *vx, const double *vy, const double *vz,
+                    const double *m, double *part, double *out5);
+
+}  // namespace sph

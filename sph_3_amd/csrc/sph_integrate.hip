@@ -1,0 +1,140 @@
+// sph_integrate.hip -- FP64 kick/drift, the trapezoidal u update, per-call
+// preparation of P/rho^2 and the Balsara factor, and FP64 conservation
+// reductions.
+//
+// Integrator follows sph_code/sph_serial_v2.py:148-203 (KDK leapfrog,
+// u = u_prev + dt/2 (du + du_prev)); GPU twins EV/lib.h:1045-1073,1135-1148.
+// Reductions restate getKE (sph_code/libsx.py:569-579) plus linear momentum.
+#include "sph_integrate.h"
+
+#pragma clang fp contract(off)
+
+__global__ void __launch_bounds__(256) k_kick_drift(int64_t n, double *__restrict__ vx, double *__restrict__ vy,
+                                                    double *__restrict__ vz, const double *__restrict__ ax,
+                                                    const double *__restrict__ ay, const double *__restrict__ az,
+                                                    double dtk, double dtd, double *__restrict__ x,
+                                                    double *__restrict__ y, double *__restrict__ z)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double a = vx[i] + ax[i] * dtk, b = vy[i] + ay[i] * dtk, c = vz[i] + az[i] * dtk;
+    vx[i] = a; vy[i] = b; vz[i] = c;
+    if (x) {
+        x[i] = x[i] + a * dtd;
+        y[i] = y[i] + b * dtd;
+        z[i] = z[i] + c * dtd;
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_u_update(int64_t n, double *__restrict__ u, const R *__restrict__ du,
+                                                  double *__restrict__ du_prev, double dt, int first)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double d = (double)du[i];
+    if (first) u[i] = u[i] + d * dt;
+    else u[i] = u[i] + 0.5 * dt * (d + du_prev[i]);
+    du_prev[i] = d;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_prep(int64_t n, const R *__restrict__ rho, const R *__restrict__ P,
+                                              const R *__restrict__ c, const R *__restrict__ divV,
+                                              const R *__restrict__ curlV, const double *__restrict__ h,
+                                              R *__restrict__ prho2, R *__restrict__ bal)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    R r = rho[i];
+    prho2[i] = P[i] / (r * r);
+    if (bal) {
+        R d = divV[i];
+        bal[i] = d / (d + curlV[i] + (R)0.0001 * c[i] / (R)h[i]);
+    }
+}
+
+// [KE, px, py, pz, M] per block, then folded by one block (deterministic).
+__global__ void __launch_bounds__(256) k_kin_partial(int64_t n, const double *__restrict__ vx,
+                                                     const double *__restrict__ vy, const double *__restrict__ vz,
+                                                     const double *__restrict__ m, double *__restrict__ part)
+{
+    __shared__ double s[5][256];
+    double v[5] = {0, 0, 0, 0, 0};
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double a = vx[i], b = vy[i], c = vz[i], mm = m[i];
+        v[0] += 0.5 * mm * (a * a + b * b + c * c);
+        v[1] += mm * a;
+        v[2] += mm * b;
+        v[3] += mm * c;
+        v[4] += mm;
+    }
+    for (int d = 0; d < 5; d++) s[d][threadIdx.x] = v[d];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < (unsigned)off)
+            for (int d = 0; d < 5; d++) s[d][threadIdx.x] += s[d][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) part[blockIdx.x * 5 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ void __launch_bounds__(256) k_kin_final(int nb, const double *__restrict__ part, double *__restrict__ out)
+{
+    __shared__ double s[5][256];
+    double v[5] = {0, 0, 0, 0, 0};
+    for (int p = threadIdx.x; p < nb; p += blockDim.x)
+        for (int d = 0; d < 5; d++) v[d] += part[p * 5 + d];
+    for (int d = 0; d < 5; d++) s[d][threadIdx.x] = v[d];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < (unsigned)off)
+            for (int d = 0; d < 5; d++) s[d][threadIdx.x] += s[d][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) out[threadIdx.x] = s[threadIdx.x][0];
+}
+
+namespace sph {
+
+hipError_t kick_drift(hipStream_t st, int64_t n, double *vx, double *vy, double *vz, const double *ax,
+                      const double *ay, const double *az, double dtk, double dtd, double *x, double *y, double *z)
+{
+    k_kick_drift<<<sph_blocks(n, 256), 256, 0, st>>>(n, vx, vy, vz, ax, ay, az, dtk, dtd, x, y, z);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t u_update(hipStream_t st, int64_t n, double *u, const R *du, double *du_prev, double dt, int first)
+{
+    k_u_update<R><<<sph_blocks(n, 256), 256, 0, st>>>(n, u, du, du_prev, dt, first);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t prep(hipStream_t st, int64_t n, const R *rho, const R *P, const R *c, const R *divV, const R *curlV,
+                const double *h, R *prho2, R *bal)
+{
+    k_prep<R><<<sph_blocks(n, 256), 256, 0, st>>>(n, rho, P, c, divV, curlV, h, prho2, bal);
+    return hipGetLastError();
+}
+
+hipError_t kinetics(hipStream_t st, int64_t n, const double *vx, const double *vy, const double *vz,
+                    const double *m, double *part, double *out5)
+{
+    int nb = (int)((n + 255) / 256);
+    if (nb > 1024) nb = 1024;
+    if (nb < 1) nb = 1;
+    k_kin_partial<<<nb, 256, 0, st>>>(n, vx, vy, vz, m, part);
+    k_kin_final<<<1, 256, 0, st>>>(nb, part, out5);
+    return hipGetLastError();
+}
+
+template hipError_t u_update<float>(hipStream_t, int64_t, double *, const float *, double *, double, int);
+template hipError_t u_update<double>(hipStream_t, int64_t, double *, const double *, double *, double, int);
+template hipError_t prep<float>(hipStream_t, int64_t, const float *, const float *, const float *, const float *,
+                                const float *, const double *, float *, float *);
+template hipError_t prep<double>(hipStream_t, int64_t, const double *, const double *, const double *,
+                                 const double *, const double *, const double *, double *, double *);
+
+}  // namespace sph

@@ -1,0 +1,177 @@
+// sph_tree.hip -- binary radix tree over Morton-sorted particles, built
+// bottom-up without locks, with FP64 node moments and escape links for the
+// stackless walks.
+//
+// Replaces the reference's lock-based top-down octree insertion
+// (EV/bh.h:149-367 build_tree_kernel, CAS on child slots, one block),
+// centre_of_mass_kernel (EV/bh.h:374-389) and the DFS sort_kernel
+// (EV/bh.h:394-450).  Construction follows Karras (HPG 2012): every internal
+// node is found independently from the sorted keys; moments are combined by
+// the second child to arrive (one atomic per internal node).
+#include "sph_tree.h"
+
+#pragma clang fp contract(off)
+
+__device__ __forceinline__ int clz64d(uint64_t x) { return x ? __clzll(x) : 64; }
+__device__ __forceinline__ int clz32d(uint32_t x) { return x ? __clz(x) : 32; }
+
+// Common-prefix length of sorted keys i and j; equal keys fall back to the
+// index bits so that every split is unique.  -1 outside [0, n).
+__device__ __forceinline__ int delta(const uint64_t *__restrict__ k, int64_t n, int64_t i, int64_t j)
+{
+    if (j < 0 || j >= n) return -1;
+    uint64_t a = k[i], b = k[j];
+    if (a != b) return clz64d(a ^ b);
+    return 64 + clz32d((uint32_t)i ^ (uint32_t)j);
+}
+
+__global__ void __launch_bounds__(256) k_topology(int64_t n, const uint64_t *__restrict__ k,
+                                                  WNode *__restrict__ nodes,
+                                                  int32_t *__restrict__ right,
+                                                  int32_t *__restrict__ parent)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t nl = n - 1;
+    if (i < n) {  // leaf of sorted particle i
+        WNode *L = &nodes[nl + i];
+        L->first = (int)i;
+        L->count = 1;
+        L->left = -1;
+    }
+    if (i >= n - 1) return;
+    int d = (delta(k, n, i, i + 1) - delta(k, n, i, i - 1)) > 0 ? 1 : -1;
+    int dmin = delta(k, n, i, i - d);
+    int64_t lmax = 2;
+    while (delta(k, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int64_t l = 0;
+    for (int64_t t = lmax / 2; t >= 1; t /= 2)
+        if (delta(k, n, i, i + (l + t) * d) > dmin) l += t;
+    int64_t j = i + l * d;
+    int dnode = delta(k, n, i, j);
+    int64_t s = 0, t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (delta(k, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    int64_t gamma = i + s * d + (d < 0 ? -1 : 0);
+    int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+    int32_t lc = (int32_t)(lo == gamma ? nl + gamma : gamma);
+    int32_t rc = (int32_t)(hi == gamma + 1 ? nl + gamma + 1 : gamma + 1);
+    WNode *P = &nodes[i];
+    P->first = (int)lo;
+    P->count = (int)(hi - lo + 1);
+    P->left = lc;
+    right[i] = rc;
+    parent[lc] = (int32_t)i;
+    parent[rc] = (int32_t)i;
+    if (i == 0) parent[0] = -1;
+}
+
+// Escape link: the topmost node whose range starts right after this node's
+// range (internal b+1 if its range starts at b+1, else leaf b+1); -1 past the end.
+__global__ void __launch_bounds__(256) k_escape(int64_t n, WNode *__restrict__ nodes)
+{
+    int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (id >= 2 * n - 1) return;
+    int64_t nl = n - 1;
+    int64_t b = (int64_t)nodes[id].first + nodes[id].count - 1;
+    int e;
+    if (b == n - 1) e = -1;
+    else if (b + 1 < n - 1 && nodes[b + 1].first == b + 1) e = (int)(b + 1);
+    else e = (int)(nl + b + 1);
+    nodes[id].escape = e;
+}
+
+__device__ __forceinline__ void combine(WNode *__restrict__ nodes, int64_t p, int64_t a, int64_t b)
+{
+    // children were written by other threads: read through L2 (the acquire
+    // fence before this call invalidates this CU's vector L1).
+    const WNode A = nodes[a];
+    const WNode B = nodes[b];
+    WNode &P = nodes[p];
+    double lx = A.lx < B.lx ? A.lx : B.lx, ly = A.ly < B.ly ? A.ly : B.ly, lz = A.lz < B.lz ? A.lz : B.lz;
+    double hx = A.hx > B.hx ? A.hx : B.hx, hy = A.hy > B.hy ? A.hy : B.hy, hz = A.hz > B.hz ? A.hz : B.hz;
+    double mt = A.mass + B.mass;
+    double cx, cy, cz;
+    if (mt > 0.0) {
+        cx = (A.mass * A.cx + B.mass * B.cx) / mt;
+        cy = (A.mass * A.cy + B.mass * B.cy) / mt;
+        cz = (A.mass * A.cz + B.mass * B.cz) / mt;
+    } else {
+        cx = 0.5 * (lx + hx);
+        cy = 0.5 * (ly + hy);
+        cz = 0.5 * (lz + hz);
+    }
+    double s = hx - lx;
+    if (hy - ly > s) s = hy - ly;
+    if (hz - lz > s) s = hz - lz;
+    P.lx = lx; P.ly = ly; P.lz = lz;
+    P.hx = hx; P.hy = hy; P.hz = hz;
+    P.cx = cx; P.cy = cy; P.cz = cz;
+    P.mass = mt;
+    P.size = s;
+    P.hmax = A.hmax > B.hmax ? A.hmax : B.hmax;
+    P.epsmax = A.epsmax > B.epsmax ? A.epsmax : B.epsmax;
+}
+
+// One thread per leaf: write the leaf record, then climb; the second child to
+// reach an internal node combines it and keeps climbing.
+__global__ void __launch_bounds__(256) k_moments(int64_t n, const double *__restrict__ x,
+                                                 const double *__restrict__ y,
+                                                 const double *__restrict__ z,
+                                                 const double *__restrict__ m,
+                                                 const double *__restrict__ h,
+                                                 const double *__restrict__ eps,
+                                                 WNode *__restrict__ nodes,
+                                                 const int32_t *__restrict__ left_unused,
+                                                 const int32_t *__restrict__ right,
+                                                 const int32_t *__restrict__ parent,
+                                                 uint32_t *__restrict__ flags)
+{
+    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    int64_t nl = n - 1;
+    int64_t id = nl + s;
+    {
+        WNode &L = nodes[id];
+        double px = x[s], py = y[s], pz = z[s];
+        L.lx = L.hx = L.cx = px;
+        L.ly = L.hy = L.cy = py;
+        L.lz = L.hz = L.cz = pz;
+        L.mass = m ? m[s] : 0.0;
+        L.size = 0.0;
+        L.hmax = h ? h[s] : 0.0;
+        L.epsmax = eps ? eps[s] : 0.0;
+    }
+    if (n < 2) return;
+    int64_t p = parent[id];
+    while (p >= 0) {
+        __threadfence();
+        uint32_t old = atomicAdd(&flags[p], 1u);
+        if (old == 0) return;  // first arrival: the sibling finishes this node
+        __threadfence();
+        combine(nodes, p, nodes[p].left, right[p]);
+        p = parent[p];
+    }
+}
+
+namespace sph {
+
+hipError_t tree_topology(hipStream_t st, int64_t n, const uint64_t *skeys, WNode *nodes, int32_t *right,
+                         int32_t *parent)
+{
+    k_topology<<<sph_blocks(n, 256), 256, 0, st>>>(n, skeys, nodes, right, parent);
+    k_escape<<<sph_blocks(2 * n - 1, 256), 256, 0, st>>>(n, nodes);
+    return hipGetLastError();
+}
+
+hipError_t tree_moments(hipStream_t st, int64_t n, const double *x, const double *y, const double *z,
+                        const double *m, const double *h, const double *eps, WNode *nodes,
+                        const int32_t *right, const int32_t *parent, uint32_t *flags)
+{
+    if (n > 1) fill<uint32_t>(st, n - 1, flags, 0u);
+    k_moments<<<sph_blocks(n, 256), 256, 0, st>>>(n, x, y, z, m, h, eps, nodes, nullptr, right, parent, flags);
+    return hipGetLastError();
+}
+
+}  // namespace sph

@@ -1,0 +1,93 @@
+"""Device-resident time loop: the step of ``sph_code/sph_serial_v2.py``
+(lines 94-143 before the loop, 148-203 per step) with every array kept on the
+GPU between steps.
+
+Per step (KDK leapfrog):  v += a dt/2; r += v dt; Morton re-sort; tree;
+FP64 Newton-Raphson h solve; CSR neighbour lists; density + EOS; tree gravity;
+div/curl v (Balsara); pressure + AV acceleration (a = a_grav + a_sph);
+v += a dt/2; du/dt with the kicked v; u = u_prev + dt/2 (du + du_prev).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import PHASES, PREC, SOFT, Stats, StepParams, context, f64, ptr
+
+
+@dataclass
+class StepConfig:
+    dt: float = 1e-3
+    gamma: float = 5.0 / 3.0
+    alpha: float = 1.0
+    G: float = 1.0
+    theta: float = 0.5
+    softening: str = "plummer"
+    eps_plummer: float = 1e-3
+    n_ngb: float = 64.0
+    h_tol: float = 1e-10
+    precision: str = "mixed"
+    balsara: bool = True
+
+    def params(self, timing=False) -> StepParams:
+        return StepParams(self.dt, self.gamma, self.alpha, self.G, self.theta, self.eps_plummer, self.n_ngb,
+                          self.h_tol, SOFT[self.softening], PREC[self.precision], int(self.balsara), int(timing))
+
+
+class DeviceSPH:
+    """Resident particle state on one GPU plus the step driver."""
+
+    def __init__(self, pos, vel, m, u, h, config: StepConfig | None = None, device: int = 0):
+        self.cfg = config or StepConfig()
+        self.ctx = context(device)
+        pos = f64(pos, (-1, 3))
+        self.n = len(pos)
+        n = self.n
+        u = np.broadcast_to(f64(u), (n,)).copy()
+        m = np.broadcast_to(f64(m), (n,)).copy()
+        h = np.broadcast_to(f64(h), (n,)).copy()
+        c = self.ctx
+        c.check(c.lib.sph_state_set(c.h, n, ptr(pos), ptr(f64(vel, (-1, 3))), ptr(m), ptr(u), ptr(h)),
+                "sph_state_set")
+        self.primed = False
+
+    def prime(self):
+        """Forces at the initial positions and the first u update."""
+        p = self.cfg.params()
+        c = self.ctx
+        c.check(c.lib.sph_prime(c.h, ctypes.byref(p)), "sph_prime")
+        self.primed = True
+
+    def step(self, timing=False):
+        if not self.primed:
+            self.prime()
+        p = self.cfg.params(timing)
+        c = self.ctx
+        c.check(c.lib.sph_step(c.h, ctypes.byref(p)), "sph_step")
+
+    def get(self):
+        n = self.n
+        pos = np.empty((n, 3)); vel = np.empty((n, 3)); acc = np.empty((n, 3))
+        u = np.empty(n); h = np.empty(n); rho = np.empty(n)
+        c = self.ctx
+        c.check(c.lib.sph_state_get(c.h, ptr(pos), ptr(vel), ptr(u), ptr(h), ptr(rho) if self.primed else None,
+                                    ptr(acc)), "sph_state_get")
+        out = dict(pos=pos, vel=vel, u=u, h=h, acc=acc)
+        if self.primed:
+            out["rho"] = rho
+        return out
+
+    def stats(self) -> dict:
+        s = Stats()
+        c = self.ctx
+        c.check(c.lib.sph_get_stats(c.h, ctypes.byref(s)), "sph_get_stats")
+        return dict(n=s.n, steps=s.steps, ngb_total=s.ngb_total, grav_pp=s.grav_pp, grav_pn=s.grav_pn,
+                    h_unconverged=s.h_unconverged,
+                    phase_ms={k: s.phase_ms[i] for i, k in enumerate(PHASES)},
+                    phase_calls={k: s.phase_calls[i] for i, k in enumerate(PHASES)})
+
+    def reset_stats(self):
+        c = self.ctx
+        c.check(c.lib.sph_reset_stats(c.h), "sph_reset_stats")
