@@ -34,16 +34,30 @@ def log(msg):
 
 
 def per_particle_bytes(kbar):
-    """Algorithmic (compulsory) HBM bytes per particle of each CSR pass with
-    this build's layout: int32 CSR index stream (4 B per neighbour), int64 row
-    offsets, FP64 x,y,z / v / m / h, FP32 hydro scalars (mixed form)."""
+    """Algorithmic (compulsory) HBM bytes per particle of each pass with this
+    build's layout: int32 CSR index stream (4 B per neighbour), int64 row
+    offsets, FP64 x,y,z / v / m / h, FP32 hydro scalars (mixed form), 128-B
+    tree node records (2N-1 of them)."""
     idx = 4.0 * kbar + 8.0
     return {
+        "order": 6 * (8 + 4) * 2 + 24 + 13 * 8 * 2,          # 6 radix passes of (key,val) r+w, keys, 13 arrays gathered
+        "moments": 24 + 8 + 8 + 2 * 128,                     # x, m, h in; ~2 node records out
+        "hsolve": 24 + 8 + 8 + 2 * 128,                      # x, h in, h out, tree read once
+        "ngb": 24 + 8 + 2 * 128 + idx,                       # x, h, tree in; CSR out
         "density": idx + 24 + 8 + 8 + 8 + 4 * 4,            # x,m,h,u in; rho,P,c,P/rho^2 out
         "divcurl": idx + 24 + 24 + 8 + 8 + 4 + 4 + 3 * 4,    # x,v,m,h,rho,c in; divV,curlV,f out
         "accsph": idx + 24 + 24 + 8 + 8 + 4 * 4 + 24 + 24,   # x,v,m,h,rho,P/rho^2,c,f,a_grav in; a out
         "dudt": idx + 24 + 24 + 8 + 8 + 4 * 4 + 4,           # ... in; du out
+        "kick_drift": 6 * 8 * 2 + 3 * 8 + 3 * 8,
+        "kick": 3 * 8 * 2 + 3 * 8,
+        "u_update": 8 * 2 + 4 + 8 * 2,
     }
+
+
+KERNEL_OF_PHASE = {"order": "k_rs_scatter+k_gather", "moments": "k_moments_local+k_moments_top",
+                   "hsolve": "k_hsolve", "ngb": "k_ngb<false>+k_ngb<true>", "density": "k_density<float,16>",
+                   "divcurl": "k_divcurl<float,16>", "accsph": "k_accsph<float,16>", "dudt": "k_dudt<float,16>",
+                   "gravity": "k_gravity<float>"}
 
 
 def cpu_baseline(sim_state, n, m, rows, seed=0):
@@ -154,18 +168,19 @@ def main():
     if dom == "gravity":
         flops = FLOP_PER_INTERACTION * (st["grav_pp"] + st["grav_pn"])
         ach = flops / (phases[dom] * 1e-3) / 1e12
-        roof = {"kernel": "k_gravity<float>", "bound": "valu", "achieved": ach, "peak": FP32_PEAK_TFLOPS,
+        roof = {"kernel": KERNEL_OF_PHASE[dom], "bound": "valu", "achieved": ach, "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": ach / FP32_PEAK_TFLOPS, "traffic": None,
-                "algorithmic": f"{FLOP_PER_INTERACTION} flop x {st['grav_pp'] + st['grav_pn']} interactions"}
+                "algorithmic": f"{FLOP_PER_INTERACTION} flop x {st['grav_pp'] + st['grav_pn']} interactions",
+                "ms_per_launch": phases[dom]}
     else:
-        b = bytes_pp.get(dom)
-        if b is None:
-            b = 0.0
+        b = bytes_pp.get(dom, 0.0)
         ach = b * n / (phases[dom] * 1e-3) / 1e9
-        roof = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": ach / HBM_PEAK_GBS, "traffic": None}
+        roof = {"kernel": KERNEL_OF_PHASE.get(dom, dom), "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                "algorithmic": f"{b:.1f} B/particle x {n} particles", "ms_per_launch": phases[dom]}
     # HBM fractions of the CSR passes (reported beside the dominant kernel)
-    hbm = {k: {"ms": phases[k], "GB/s": bytes_pp[k] * n / (phases[k] * 1e-3) / 1e9}
+    hbm = {k: {"ms": phases[k], "GB/s": bytes_pp[k] * n / (phases[k] * 1e-3) / 1e9,
+               "frac": bytes_pp[k] * n / (phases[k] * 1e-3) / 1e9 / HBM_PEAK_GBS}
            for k in bytes_pp if k in phases}
     grav_tflops = None
     if "gravity" in phases:
@@ -185,7 +200,7 @@ def main():
         "neighbours_per_particle": kbar,
         "gravity_interactions_per_particle": (st["grav_pp"] + st["grav_pn"]) / n,
         "gravity_tflops": grav_tflops,
-        "csr_pass_bandwidth": hbm,
+        "pass_bandwidth": hbm,
         "roofline": roof,
     }
     if not args.no_fp64_form:

@@ -68,7 +68,7 @@ struct sph_ctx {
     // ordering
     Buf keys0, keys1, vals0, vals1, hist, bpart, box;
     // tree
-    Buf nodes, right, parent, flags;
+    Buf nodes, right, parent, flags, front;
     // neighbour lists
     Buf cnt, off, scantmp, idx;
     // hydro scalars (8 bytes per particle, used as float or double)
@@ -148,6 +148,7 @@ int capacity(sph_ctx *ctx, int64_t n)
     CK(ensure(ctx, ctx->right, ni));
     CK(ensure(ctx, ctx->parent, 2 * ni));
     CK(ensure(ctx, ctx->flags, ni));
+    CK(ensure(ctx, ctx->front, 2 * ni));
     CK(ensure(ctx, ctx->off, (size_t)(n + 1) * 8));
     CK(ensure(ctx, ctx->scantmp, (size_t)sph::scan_tmp_entries(n) * 8));
     CK(ensure(ctx, ctx->hcount, 4 * 8));
@@ -273,7 +274,7 @@ int ensure_moments(sph_ctx *ctx)
     if (ctx->moments_ok) return SPH_OK;
     HIPCK(sph::tree_moments(ctx->st, ctx->n, D(x), D(y), D(z), ctx->m_ok ? D(m) : nullptr,
                             ctx->h_ok ? D(h) : nullptr, ctx->eps_ok ? D(eps) : nullptr, P_<WNode>(ctx->nodes),
-                            I32(right), I32(parent), P_<uint32_t>(ctx->flags)));
+                            I32(right), I32(parent), P_<uint32_t>(ctx->flags), I32(front)));
     ctx->moments_ok = true;
     return SPH_OK;
 }
@@ -372,7 +373,7 @@ extern "C" int sph_ctx_destroy(sph_ctx *ctx)
     Buf *all[] = {&ctx->x, &ctx->y, &ctx->z, &ctx->vx, &ctx->vy, &ctx->vz, &ctx->m, &ctx->h, &ctx->eps,
                   &ctx->u, &ctx->duprev, &ctx->ax, &ctx->ay, &ctx->az, &ctx->gx, &ctx->gy, &ctx->gz,
                   &ctx->tmpd, &ctx->orig, &ctx->tmpi, &ctx->keys0, &ctx->keys1, &ctx->vals0, &ctx->vals1,
-                  &ctx->hist, &ctx->bpart, &ctx->box, &ctx->nodes, &ctx->right, &ctx->parent, &ctx->flags,
+                  &ctx->hist, &ctx->bpart, &ctx->box, &ctx->nodes, &ctx->right, &ctx->parent, &ctx->flags, &ctx->front,
                   &ctx->cnt, &ctx->off, &ctx->scantmp, &ctx->idx, &ctx->rho, &ctx->P, &ctx->c, &ctx->prho2,
                   &ctx->divV, &ctx->curlV, &ctx->bal, &ctx->du, &ctx->cand, &ctx->hR, &ctx->hsteps,
                   &ctx->hstate, &ctx->hcount, &ctx->hout, &ctx->hin, &ctx->tx, &ctx->ty, &ctx->tz,
@@ -712,6 +713,7 @@ extern "C" int sph_state_set(sph_ctx *ctx, int64_t n, const double *pos, const d
     invalidate_all(ctx);  // stateless caches no longer describe the device arrays
     ctx->resident = true;
     ctx->primed = false;
+    std::memset(&ctx->stats, 0, sizeof(ctx->stats));  // counters describe this state only
     ctx->stats.n = n;
     return SPH_OK;
 }
@@ -769,7 +771,7 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     // boxes are needed by the h walk; hmax is refreshed after the solve
     ph_begin(ctx, SPH_PH_HSOLVE);
     HIPCK(sph::tree_moments(st, n, D(x), D(y), D(z), D(m), D(h), nullptr, P_<WNode>(ctx->nodes), I32(right),
-                            I32(parent), P_<uint32_t>(ctx->flags)));
+                            I32(parent), P_<uint32_t>(ctx->flags), I32(front)));
     HIPCK(hipMemcpyAsync(ctx->hin.p, ctx->h.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     int64_t unc = 0;
     CK(run_hsolve(ctx, n, D(x), D(y), D(z), SPH_H_NEWTON, p->n_ngb, 0, 0, 0, p->h_tol, &unc, nullptr));
@@ -780,7 +782,7 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     ph_begin(ctx, SPH_PH_MOMENTS);
     const double *eps = p->soft_kind == SPH_SOFT_SPLINE ? D(h) : nullptr;
     HIPCK(sph::tree_moments(st, n, D(x), D(y), D(z), D(m), D(h), eps, P_<WNode>(ctx->nodes), I32(right),
-                            I32(parent), P_<uint32_t>(ctx->flags)));
+                            I32(parent), P_<uint32_t>(ctx->flags), I32(front)));
     ph_end(ctx, SPH_PH_MOMENTS);
     ph_begin(ctx, SPH_PH_NGB);
     CK(build_csr(ctx));

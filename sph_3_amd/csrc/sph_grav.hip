@@ -67,38 +67,36 @@ __global__ void __launch_bounds__(256) k_gravity(int64_t n, const double *__rest
     int node = 0;
     int resume = valid ? 0 : -2;
     while (node >= 0) {
-        node = wave_uniform(node);
         const WNode *nd = nodes + node;
+        const int first = wave_uniform(nd->first), cnt = wave_uniform(nd->count);
+        const int left = wave_uniform(nd->left), esc = wave_uniform(nd->escape);
         bool open = false;
         if (resume == node) {
             if (mac_accept(xi, yi, zi, nd, theta2)) {
                 interact<R>(nd->cx, nd->cy, nd->cz, nd->mass, nd->epsmax, xi, yi, zi, epsi, Gr, soft_kind, e2, sx, sy,
                             sz);
                 npn++;
-                resume = nd->escape;
+                resume = esc;
             } else {
                 open = true;
             }
         }
-        bool anyopen = wave_ballot(open) != 0;
-        int cnt = nd->count;
+        const bool anyopen = wave_ballot(open) != 0;
+        int next = esc;
         if (anyopen && cnt <= bucket) {
-            int f = nd->first;
-            for (int j = f; j < f + cnt; j++) {
+            for (int j = first; j < first + cnt; j++) {
                 double px = x[j], py = y[j], pz = z[j], pm = m[j], pe = eps ? eps[j] : 0.0;
                 if (open && j != i) {
                     interact<R>(px, py, pz, pm, pe, xi, yi, zi, epsi, Gr, soft_kind, e2, sx, sy, sz);
                     npp++;
                 }
             }
-            if (open) resume = nd->escape;
-            node = nd->escape;
+            if (open) resume = esc;
         } else if (anyopen) {
-            if (open) resume = nd->left;
-            node = nd->left;
-        } else {
-            node = nd->escape;
+            if (open) resume = left;
+            next = left;
         }
+        node = wave_uniform(next);
     }
     if (valid) {
         ax[i] = (double)sx.value();

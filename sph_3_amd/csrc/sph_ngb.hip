@@ -39,27 +39,26 @@ __device__ __forceinline__ void coop_walk(const WNode *__restrict__ nodes, bool 
     int node = 0;
     int resume = valid ? 0 : -2;
     while (node >= 0) {
-        node = wave_uniform(node);
         const WNode *nd = nodes + node;
+        // link fields as wave-uniform scalars (scalar loads, scalar select)
+        const int first = wave_uniform(nd->first), cnt = wave_uniform(nd->count);
+        const int left = wave_uniform(nd->left), esc = wave_uniform(nd->escape);
         bool open = false;
         if (resume == node) {
-            if (prune(nd)) resume = nd->escape;
+            if (prune(nd)) resume = esc;
             else open = true;
         }
-        bool anyopen = wave_ballot(open) != 0;
-        int cnt = nd->count;
+        const bool anyopen = wave_ballot(open) != 0;
+        int next = esc;
         if (anyopen && cnt <= bucket) {
-            int f = nd->first;
-            for (int j = f; j < f + cnt; j++)
+            for (int j = first; j < first + cnt; j++)
                 if (open) leaf(j);
-            if (open) resume = nd->escape;
-            node = nd->escape;
+            if (open) resume = esc;
         } else if (anyopen) {
-            if (open) resume = nd->left;
-            node = nd->left;
-        } else {
-            node = nd->escape;
+            if (open) resume = left;
+            next = left;
         }
+        node = wave_uniform(next);
     }
 }
 

@@ -168,6 +168,49 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(int64_t n, const uint64
     }
 }
 
+// Exclusive scan of the digit-major histogram hist[d * nblk + b]: one
+// workgroup per digit scans its nblk counts and publishes the digit total;
+// a second launch adds the prefix of the digit totals.
+__global__ void __launch_bounds__(256) k_rs_scan_digit(int nblk, uint32_t *__restrict__ hist,
+                                                       uint32_t *__restrict__ dtot)
+{
+    __shared__ uint32_t part[256];
+    uint32_t *a = hist + (int64_t)blockIdx.x * nblk;
+    int per = (nblk + 255) / 256;
+    int b = threadIdx.x * per, e = b + per < nblk ? b + per : nblk;
+    uint32_t s = 0;
+    for (int i = b; i < e; i++) s += a[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        uint32_t v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0;
+    for (int i = b; i < e; i++) {
+        uint32_t v = a[i];
+        a[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == 255) dtot[blockIdx.x] = part[255];
+}
+
+__global__ void __launch_bounds__(256) k_rs_scan_base(int nblk, uint32_t *__restrict__ hist,
+                                                      const uint32_t *__restrict__ dtot)
+{
+    __shared__ uint32_t base;
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (unsigned d = 0; d < blockIdx.x; d++) s += dtot[d];
+        base = s;
+    }
+    __syncthreads();
+    uint32_t *a = hist + (int64_t)blockIdx.x * nblk;
+    for (int i = threadIdx.x; i < nblk; i += 256) a[i] += base;
+}
+
 // ---------------------------------------------------------------- scans
 // Exclusive scan of uint32 -> uint32 (radix histograms) in one block.
 __global__ void __launch_bounds__(1024) k_scan_u32_small(int64_t n, uint32_t *__restrict__ a)
@@ -336,7 +379,7 @@ hipError_t morton(hipStream_t st, int64_t n, const double *x, const double *y, c
     return hipGetLastError();
 }
 
-int64_t radix_hist_entries(int64_t n) { return 256 * ((n + RS_TILE - 1) / RS_TILE); }
+int64_t radix_hist_entries(int64_t n) { return 256 * ((n + RS_TILE - 1) / RS_TILE) + 256; }
 
 hipError_t radix_sort(hipStream_t st, int64_t n, int nbits, uint64_t *k0, int32_t *v0,
                       uint64_t *k1, int32_t *v1, uint32_t *hist)
@@ -345,7 +388,8 @@ hipError_t radix_sort(hipStream_t st, int64_t n, int nbits, uint64_t *k0, int32_
     if (nblk < 1) return hipSuccess;
     for (int shift = 0; shift < nbits; shift += 8) {
         k_rs_hist<<<nblk, RS_BLOCK, 0, st>>>(n, k0, shift, nblk, hist);
-        k_scan_u32_small<<<1, 1024, 0, st>>>((int64_t)256 * nblk, hist);
+        k_rs_scan_digit<<<256, 256, 0, st>>>(nblk, hist, hist + (int64_t)256 * nblk);
+        k_rs_scan_base<<<256, 256, 0, st>>>(nblk, hist, hist + (int64_t)256 * nblk);
         k_rs_scatter<<<nblk, RS_BLOCK, 0, st>>>(n, k0, v0, k1, v1, shift, nblk, hist);
         uint64_t *tk = k0; k0 = k1; k1 = tk;
         int32_t *tv = v0; v0 = v1; v1 = tv;

@@ -7,7 +7,7 @@
 // centre_of_mass_kernel (EV/bh.h:374-389) and the DFS sort_kernel
 // (EV/bh.h:394-450).  Construction follows Karras (HPG 2012): every internal
 // node is found independently from the sorted keys; moments are combined by
-// the second child to arrive (one atomic per internal node).
+// the second child to arrive (one atomic per internal node, workgroup scope).
 #include "sph_tree.h"
 
 #pragma clang fp contract(off)
@@ -114,21 +114,33 @@ __device__ __forceinline__ void combine(WNode *__restrict__ nodes, int64_t p, in
     P.epsmax = A.epsmax > B.epsmax ? A.epsmax : B.epsmax;
 }
 
-// One thread per leaf: write the leaf record, then climb; the second child to
-// reach an internal node combines it and keeps climbing.
-__global__ void __launch_bounds__(256) k_moments(int64_t n, const double *__restrict__ x,
-                                                 const double *__restrict__ y,
-                                                 const double *__restrict__ z,
-                                                 const double *__restrict__ m,
-                                                 const double *__restrict__ h,
-                                                 const double *__restrict__ eps,
-                                                 WNode *__restrict__ nodes,
-                                                 const int32_t *__restrict__ left_unused,
-                                                 const int32_t *__restrict__ right,
-                                                 const int32_t *__restrict__ parent,
-                                                 uint32_t *__restrict__ flags)
+// Bottom-up moments without device-scope fences.  Device-wide release
+// fences write back the L2 of every XCD and cost microseconds each, so the
+// climb is split in two:
+//  (1) k_moments_local: one thread per leaf writes the leaf record and climbs
+//      while the parent's particle range lies inside this workgroup's leaf
+//      range; the second child to arrive combines (workgroup-scope atomics).
+//      A finished node whose parent spans workgroups is appended to the
+//      frontier list.
+//  (2) k_moments_top: one workgroup climbs from every frontier node the same
+//      way; all remaining (spanning) nodes are combined inside that single
+//      workgroup, so workgroup scope suffices again.
+#define MOM_BLOCK 1024
+
+__global__ void __launch_bounds__(MOM_BLOCK) k_moments_local(int64_t n, const double *__restrict__ x,
+                                                             const double *__restrict__ y,
+                                                             const double *__restrict__ z,
+                                                             const double *__restrict__ m,
+                                                             const double *__restrict__ h,
+                                                             const double *__restrict__ eps,
+                                                             WNode *__restrict__ nodes,
+                                                             const int32_t *__restrict__ right,
+                                                             const int32_t *__restrict__ parent,
+                                                             uint32_t *__restrict__ flags,
+                                                             int32_t *__restrict__ frontier,
+                                                             uint32_t *__restrict__ nfrontier)
 {
-    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t s = blockIdx.x * (int64_t)MOM_BLOCK + threadIdx.x;
     if (s >= n) return;
     int64_t nl = n - 1;
     int64_t id = nl + s;
@@ -146,12 +158,35 @@ __global__ void __launch_bounds__(256) k_moments(int64_t n, const double *__rest
     if (n < 2) return;
     int64_t p = parent[id];
     while (p >= 0) {
-        __threadfence();
-        uint32_t old = atomicAdd(&flags[p], 1u);
+        const int64_t f = nodes[p].first, last = f + nodes[p].count - 1;
+        if (f / MOM_BLOCK != last / MOM_BLOCK) {  // parent spans workgroups: hand over
+            uint32_t k = atomicAdd(nfrontier, 1u);
+            frontier[k] = (int32_t)id;
+            return;
+        }
+        uint32_t old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
         if (old == 0) return;  // first arrival: the sibling finishes this node
-        __threadfence();
         combine(nodes, p, nodes[p].left, right[p]);
+        id = p;
         p = parent[p];
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_moments_top(WNode *__restrict__ nodes, const int32_t *__restrict__ right,
+                                                     const int32_t *__restrict__ parent,
+                                                     uint32_t *__restrict__ flags,
+                                                     const int32_t *__restrict__ frontier,
+                                                     const uint32_t *__restrict__ nfrontier)
+{
+    const uint32_t nf = *nfrontier;
+    for (uint32_t k = threadIdx.x; k < nf; k += blockDim.x) {
+        int64_t p = parent[frontier[k]];
+        while (p >= 0) {
+            uint32_t old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old == 0) break;
+            combine(nodes, p, nodes[p].left, right[p]);
+            p = parent[p];
+        }
     }
 }
 
@@ -167,10 +202,14 @@ hipError_t tree_topology(hipStream_t st, int64_t n, const uint64_t *skeys, WNode
 
 hipError_t tree_moments(hipStream_t st, int64_t n, const double *x, const double *y, const double *z,
                         const double *m, const double *h, const double *eps, WNode *nodes,
-                        const int32_t *right, const int32_t *parent, uint32_t *flags)
+                        const int32_t *right, const int32_t *parent, uint32_t *flags, int32_t *frontier)
 {
-    if (n > 1) fill<uint32_t>(st, n - 1, flags, 0u);
-    k_moments<<<sph_blocks(n, 256), 256, 0, st>>>(n, x, y, z, m, h, eps, nodes, nullptr, right, parent, flags);
+    // flags[0..n-2]: arrival counters; flags[n-1]: frontier length
+    fill<uint32_t>(st, n, flags, 0u);
+    uint32_t *nf = flags + (n - 1);
+    k_moments_local<<<sph_blocks(n, MOM_BLOCK), MOM_BLOCK, 0, st>>>(n, x, y, z, m, h, eps, nodes, right, parent,
+                                                                   flags, frontier, nf);
+    if (n > MOM_BLOCK) k_moments_top<<<1, 1024, 0, st>>>(nodes, right, parent, flags, frontier, nf);
     return hipGetLastError();
 }
 
