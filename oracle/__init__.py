@@ -220,8 +220,22 @@ class Tree:
                               _p(self.right), _p(self.nbox), _p(self.com), _p(self.mass), _p(self.hmax),
                               _p(self.epsmax), _p(self.size))
 
-    def gravity(self, theta=0.5, G=1.0, soft_kind=SOFT_SPLINE, eps_plummer=0.0):
-        """Returns (acc in INPUT order, (n_pp, n_pn))."""
+    def gravity(self, theta=0.5, G=1.0, soft_kind=SOFT_SPLINE, eps_plummer=0.0, group=64):
+        """Group-walk tree gravity (the device's interaction lists: groups of
+        ``group`` consecutive sorted particles share one opening test against
+        the group box).  Returns (acc in INPUT order, (n_pp, n_pn))."""
+        acc = np.zeros((self.N, 3)); n = np.zeros(2, np.int64)
+        lib().or_grav_tree_group(_i64(self.N), _p(self.spos), _p(self.sm), _p(self.seps), _p(self.first),
+                                 _p(self.count), _p(self.left), _p(self.escape), _p(self.nbox), _p(self.com),
+                                 _p(self.mass), _p(self.epsmax), _p(self.size), ctypes.c_int(self.bucket),
+                                 ctypes.c_int(group), ctypes.c_double(theta), ctypes.c_double(G),
+                                 ctypes.c_int(soft_kind), ctypes.c_double(eps_plummer), _p(acc), _p(n))
+        out = np.zeros_like(acc); out[self.perm] = acc
+        return out, (int(n[0]), int(n[1]))
+
+    def gravity_per_particle(self, theta=0.5, G=1.0, soft_kind=SOFT_SPLINE, eps_plummer=0.0):
+        """Per-particle opening test (size^2 < theta^2 |com - x|^2, x outside
+        the node box).  Returns (acc in INPUT order, (n_pp, n_pn))."""
         acc = np.zeros((self.N, 3)); n = np.zeros(2, np.int64)
         lib().or_grav_tree(_i64(self.N), _p(self.spos), _p(self.sm), _p(self.seps), _p(self.first),
                            _p(self.count), _p(self.left), _p(self.escape), _p(self.nbox), _p(self.com),

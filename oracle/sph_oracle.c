@@ -633,6 +633,90 @@ void or_grav_tree(int64_t N, const double *spos, const double *sm, const double 
     n_int[1] += npn;
 }
 
+/* Group opening test (FP64): targets are processed in groups of `group`
+ * consecutive sorted particles with bounding box gb; a node is used as a
+ * monopole for the whole group iff size^2 < theta^2 * d^2, with d the
+ * distance from the node's centre of mass to the group box, and the node's box
+ * is disjoint from the group box.  Every particle of a group therefore sees
+ * the same interaction list: accepted nodes plus every particle of each
+ * opened bucket (itself excluded). */
+static inline int mac_group(const double *gb, const double *box, const double *com, double size, double theta2)
+{
+    double e[3];
+    for (int d = 0; d < 3; d++) {
+        double t = 0.0;
+        if (gb[d] - com[d] > 0.0) t = gb[d] - com[d];
+        if (com[d] - gb[3 + d] > 0.0) t = com[d] - gb[3 + d];
+        e[d] = t;
+    }
+    double d2 = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+    int disjoint = box[3] < gb[0] || box[0] > gb[3] || box[4] < gb[1] || box[1] > gb[4] || box[5] < gb[2] ||
+                   box[2] > gb[5];
+    return (size * size < theta2 * d2) && disjoint;
+}
+
+void or_grav_tree_group(int64_t N, const double *spos, const double *sm, const double *seps,
+                        const int32_t *first, const int32_t *count, const int32_t *left,
+                        const int32_t *escape, const double *box, const double *com,
+                        const double *mass, const double *epsmax, const double *size, int bucket,
+                        int group, double theta, double G, int soft_kind, double eps_plummer,
+                        double *acc, int64_t *n_int)
+{
+    double theta2 = theta * theta, e2 = eps_plummer * eps_plummer;
+    int64_t ngroups = (N + group - 1) / group, npp = 0, npn = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : npp, npn)
+    for (int64_t g = 0; g < ngroups; g++) {
+        int64_t g0 = g * group, g1 = g0 + group < N ? g0 + group : N;
+        double gb[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        for (int64_t i = g0; i < g1; i++)
+            for (int d = 0; d < 3; d++) {
+                if (spos[3 * i + d] < gb[d]) gb[d] = spos[3 * i + d];
+                if (spos[3 * i + d] > gb[3 + d]) gb[3 + d] = spos[3 * i + d];
+            }
+        for (int64_t i = g0; i < g1; i++) acc[3 * i] = acc[3 * i + 1] = acc[3 * i + 2] = 0.0;
+        int64_t nodes_acc = 0, parts = 0;
+        int64_t node = 0;
+        while (node >= 0) {
+            if (mac_group(gb, box + 6 * node, com + 3 * node, size[node], theta2)) {
+                for (int64_t i = g0; i < g1; i++) {
+                    const double *x = spos + 3 * i;
+                    double dx = com[3 * node] - x[0], dy = com[3 * node + 1] - x[1], dz = com[3 * node + 2] - x[2];
+                    double r2 = dx * dx + dy * dy + dz * dz;
+                    double fk = soft_kind == 0 ? grav_fk_spline(sqrt(r2), 0.5 * (seps[i] + epsmax[node]))
+                                               : grav_fk_plummer(r2, e2);
+                    acc[3 * i] += G * fk * dx * mass[node];
+                    acc[3 * i + 1] += G * fk * dy * mass[node];
+                    acc[3 * i + 2] += G * fk * dz * mass[node];
+                }
+                nodes_acc++;
+                node = escape[node];
+            } else if (count[node] <= bucket) {
+                for (int64_t j = first[node]; j < first[node] + count[node]; j++) {
+                    for (int64_t i = g0; i < g1; i++) {
+                        if (j == i) continue;
+                        const double *x = spos + 3 * i;
+                        double dx = spos[3 * j] - x[0], dy = spos[3 * j + 1] - x[1], dz = spos[3 * j + 2] - x[2];
+                        double r2 = dx * dx + dy * dy + dz * dz;
+                        double fk = soft_kind == 0 ? grav_fk_spline(sqrt(r2), 0.5 * (seps[i] + seps[j]))
+                                                   : grav_fk_plummer(r2, e2);
+                        acc[3 * i] += G * fk * dx * sm[j];
+                        acc[3 * i + 1] += G * fk * dy * sm[j];
+                        acc[3 * i + 2] += G * fk * dz * sm[j];
+                    }
+                    parts++;
+                }
+                node = escape[node];
+            } else {
+                node = left[node];
+            }
+        }
+        npn += nodes_acc * (g1 - g0);
+        npp += parts * (g1 - g0) - (g1 - g0);
+    }
+    n_int[0] += npp;
+    n_int[1] += npn;
+}
+
 /* squared distance from x to a box (0 inside), association as for r^2 */
 static inline double box_dist2(const double *x, const double *b)
 {

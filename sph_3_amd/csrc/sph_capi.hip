@@ -151,7 +151,7 @@ int capacity(sph_ctx *ctx, int64_t n)
     CK(ensure(ctx, ctx->front, 2 * ni));
     CK(ensure(ctx, ctx->off, (size_t)(n + 1) * 8));
     CK(ensure(ctx, ctx->scantmp, (size_t)sph::scan_tmp_entries(n) * 8));
-    CK(ensure(ctx, ctx->hcount, 4 * 8));
+    CK(ensure(ctx, ctx->hcount, 8 * 8));
     CK(ensure(ctx, ctx->stage, 3 * nd));
     CK(ensure(ctx, ctx->kpart, 5 * 1024 * 8));
     CK(ensure(ctx, ctx->kout, 8 * 8));
@@ -652,7 +652,7 @@ extern "C" int sph_gravity(sph_ctx *ctx, int64_t n, const double *pos, const dou
     if (cm || ce) ctx->moments_ok = false;
     CK(ensure_moments(ctx));
     unsigned long long *cnt = P_<unsigned long long>(ctx->hcount);
-    HIPCK(sph::fill<unsigned long long>(ctx->st, 2, cnt, 0ull));
+    HIPCK(sph::fill<unsigned long long>(ctx->st, 3, cnt, 0ull));
     const double *e = ctx->eps_ok ? D(eps) : nullptr;
     if (precision == SPH_PREC_MIXED)
         HIPCK(sph::gravity<float>(ctx->st, n, D(x), D(y), D(z), D(m), e, P_<WNode>(ctx->nodes), SPH_BUCKET, theta, G,
@@ -660,10 +660,11 @@ extern "C" int sph_gravity(sph_ctx *ctx, int64_t n, const double *pos, const dou
     else
         HIPCK(sph::gravity<double>(ctx->st, n, D(x), D(y), D(z), D(m), e, P_<WNode>(ctx->nodes), SPH_BUCKET, theta,
                                    G, soft_kind, eps_plummer, D(gx), D(gy), D(gz), cnt));
+    unsigned long long hc[3];
+    HIPCK(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, ctx->st));
+    HIPCK(hipStreamSynchronize(ctx->st));
+    if (hc[2]) return fail(ctx, SPH_ERR_CAPACITY, "gravity walk stack overflow");
     if (n_interactions) {
-        unsigned long long hc[2];
-        HIPCK(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, ctx->st));
-        HIPCK(hipStreamSynchronize(ctx->st));
         n_interactions[0] = (int64_t)hc[0];
         n_interactions[1] = (int64_t)hc[1];
     }
@@ -794,8 +795,8 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
                           P_<R>(ctx->P), P_<R>(ctx->c), P_<R>(ctx->prho2)));
     ph_end(ctx, SPH_PH_DENSITY);
     ph_begin(ctx, SPH_PH_GRAVITY);
-    unsigned long long *cnt = P_<unsigned long long>(ctx->hcount) + 2;
-    HIPCK(sph::fill<unsigned long long>(st, 2, cnt, 0ull));
+    unsigned long long *cnt = P_<unsigned long long>(ctx->hcount) + 4;
+    HIPCK(sph::fill<unsigned long long>(st, 3, cnt, 0ull));
     if (p->G != 0.0) {
         HIPCK(sph::gravity<R>(st, n, D(x), D(y), D(z), D(m), eps, P_<WNode>(ctx->nodes), SPH_BUCKET, p->theta, p->G,
                               p->soft_kind, p->eps_plummer, D(gx), D(gy), D(gz), cnt));
@@ -844,8 +845,9 @@ int check_params(sph_ctx *ctx, const sph_step_params *p)
 int finish_step(sph_ctx *ctx, const sph_step_params *p)
 {
     HIPCK(hipStreamSynchronize(ctx->st));
-    unsigned long long hc[2];
-    HIPCK(hipMemcpy(hc, P_<unsigned long long>(ctx->hcount) + 2, sizeof(hc), hipMemcpyDeviceToHost));
+    unsigned long long hc[3];
+    HIPCK(hipMemcpy(hc, P_<unsigned long long>(ctx->hcount) + 4, sizeof(hc), hipMemcpyDeviceToHost));
+    if (hc[2]) return fail(ctx, SPH_ERR_CAPACITY, "gravity walk stack overflow");
     ctx->stats.grav_pp = (int64_t)hc[0];
     ctx->stats.grav_pn = (int64_t)hc[1];
     if (ctx->timing) {

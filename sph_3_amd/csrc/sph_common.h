@@ -26,7 +26,7 @@ struct __align__(16) WNode {
     double epsmax;             // max softening length
     int first, count;          // sorted-particle range [first, first+count)
     int left, escape;          // left child id (internal), pre-order successor
-    int pad[2];
+    int right, pad;            // right child id (internal)
 };
 static_assert(sizeof(WNode) == 128, "WNode must be 128 bytes");
 

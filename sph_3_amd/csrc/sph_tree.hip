@@ -37,6 +37,7 @@ __global__ void __launch_bounds__(256) k_topology(int64_t n, const uint64_t *__r
         L->first = (int)i;
         L->count = 1;
         L->left = -1;
+        L->right = -1;
     }
     if (i >= n - 1) return;
     int d = (delta(k, n, i, i + 1) - delta(k, n, i, i - 1)) > 0 ? 1 : -1;
@@ -61,6 +62,7 @@ __global__ void __launch_bounds__(256) k_topology(int64_t n, const uint64_t *__r
     P->first = (int)lo;
     P->count = (int)(hi - lo + 1);
     P->left = lc;
+    P->right = rc;
     right[i] = rc;
     parent[lc] = (int32_t)i;
     parent[rc] = (int32_t)i;

@@ -176,12 +176,22 @@ def test_tree_theta0_is_direct_sum_and_counts():
     eps = np.full(n, 0.03)
     T = O.Tree(pos, m, eps=eps)
     a0 = O.grav_direct(pos, m, eps)
-    a1, (npp, npn) = T.gravity(theta=0.0)
-    assert npp == n * (n - 1) and npn == 0
-    assert np.abs(a1 - a0).max() < 1e-12 * np.abs(a0).max()
-    a2, (npp, npn) = T.gravity(theta=0.5)
-    err = np.linalg.norm(a2 - a0, axis=1) / np.linalg.norm(a0, axis=1)
-    assert np.sqrt(np.mean(err ** 2)) < 1e-2 and npp + npn < n * (n - 1) / 2
+    for walk in (T.gravity, T.gravity_per_particle):
+        a1, (npp, npn) = walk(theta=0.0)
+        assert npp == n * (n - 1) and npn == 0
+        assert np.abs(a1 - a0).max() < 1e-12 * np.abs(a0).max()
+    # opening angle 0.5: both walks are approximations of the direct sum; the
+    # group test (one decision per 64-particle group box) is the stricter one
+    ag, ng = T.gravity(theta=0.5)
+    ap, npp_ = T.gravity_per_particle(theta=0.5)
+    eg = np.linalg.norm(ag - a0, axis=1) / np.linalg.norm(a0, axis=1)
+    ep = np.linalg.norm(ap - a0, axis=1) / np.linalg.norm(a0, axis=1)
+    assert np.sqrt(np.mean(ep ** 2)) < 1e-2 and np.sqrt(np.mean(eg ** 2)) < np.sqrt(np.mean(ep ** 2))
+    assert sum(npp_) < sum(ng) < n * (n - 1)
+    # monopole error scales with the opening angle
+    a3, _ = T.gravity(theta=0.3)
+    e3 = np.linalg.norm(a3 - a0, axis=1) / np.linalg.norm(a0, axis=1)
+    assert np.sqrt(np.mean(e3 ** 2)) < np.sqrt(np.mean(eg ** 2))
 
 
 def test_tree_structure():
