@@ -198,6 +198,7 @@ def main():
                    "alpha": 1.0, "balsara": True, "dt": 1e-3},
         "phases_ms": phases,
         "neighbours_per_particle": kbar,
+        "h_solve_rounds_last_step": st["h_rounds"],
         "gravity_interactions_per_particle": (st["grav_pp"] + st["grav_pn"]) / n,
         "gravity_tflops": grav_tflops,
         "pass_bandwidth": hbm,

@@ -216,17 +216,24 @@ class Tree:
                      if eps is not None else np.zeros(N))
         self.nbox = np.zeros((nn, 6)); self.com = np.zeros((nn, 3)); self.mass = np.zeros(nn)
         self.hmax = np.zeros(nn); self.epsmax = np.zeros(nn); self.size = np.zeros(nn)
+        self.parent = np.zeros(nn, np.int32)
+        if N > 1:
+            lib().or_tree_parents(_i64(N), _p(self.left), _p(self.right), _p(self.parent))
+        else:
+            self.parent[0] = -1
         lib().or_tree_moments(_i64(N), _p(self.spos), _p(self.sm), _p(self.sh), _p(self.seps), _p(self.left),
                               _p(self.right), _p(self.nbox), _p(self.com), _p(self.mass), _p(self.hmax),
                               _p(self.epsmax), _p(self.size))
 
     def gravity(self, theta=0.5, G=1.0, soft_kind=SOFT_SPLINE, eps_plummer=0.0, group=64):
-        """Group-walk tree gravity (the device's interaction lists: groups of
-        ``group`` consecutive sorted particles share one opening test against
-        the group box).  Returns (acc in INPUT order, (n_pp, n_pn))."""
+        """Group-walk tree gravity (the device's interaction lists: the particles
+        of each group -- a maximal tree node holding at most ``group``
+        particles -- share one opening test against the group box).
+        Returns (acc in INPUT order, (n_pp, n_pn))."""
         acc = np.zeros((self.N, 3)); n = np.zeros(2, np.int64)
         lib().or_grav_tree_group(_i64(self.N), _p(self.spos), _p(self.sm), _p(self.seps), _p(self.first),
-                                 _p(self.count), _p(self.left), _p(self.escape), _p(self.nbox), _p(self.com),
+                                 _p(self.count), _p(self.left), _p(self.escape), _p(self.parent), _p(self.nbox),
+                                 _p(self.com),
                                  _p(self.mass), _p(self.epsmax), _p(self.size), ctypes.c_int(self.bucket),
                                  ctypes.c_int(group), ctypes.c_double(theta), ctypes.c_double(G),
                                  ctypes.c_int(soft_kind), ctypes.c_double(eps_plummer), _p(acc), _p(n))

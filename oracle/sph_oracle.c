@@ -655,18 +655,33 @@ static inline int mac_group(const double *gb, const double *box, const double *c
     return (size * size < theta2 * d2) && disjoint;
 }
 
+/* Groups are the maximal tree nodes holding at most `group` particles: node k
+ * is a group iff count[k] <= group and (k is the root or its parent holds more
+ * than `group`).  Groups partition the particles and each is a contiguous
+ * Morton range with a compact box.  parent[] as built by or_tree_parents. */
+void or_tree_parents(int64_t N, const int32_t *left, const int32_t *right, int32_t *parent)
+{
+    parent[0] = -1;
+    for (int64_t k = 0; k < N - 1; k++) {
+        parent[left[k]] = (int32_t)k;
+        parent[right[k]] = (int32_t)k;
+    }
+}
+
 void or_grav_tree_group(int64_t N, const double *spos, const double *sm, const double *seps,
                         const int32_t *first, const int32_t *count, const int32_t *left,
-                        const int32_t *escape, const double *box, const double *com,
-                        const double *mass, const double *epsmax, const double *size, int bucket,
-                        int group, double theta, double G, int soft_kind, double eps_plummer,
+                        const int32_t *escape, const int32_t *parent, const double *box,
+                        const double *com, const double *mass, const double *epsmax, const double *size,
+                        int bucket, int group, double theta, double G, int soft_kind, double eps_plummer,
                         double *acc, int64_t *n_int)
 {
     double theta2 = theta * theta, e2 = eps_plummer * eps_plummer;
-    int64_t ngroups = (N + group - 1) / group, npp = 0, npn = 0;
-#pragma omp parallel for schedule(dynamic, 4) reduction(+ : npp, npn)
-    for (int64_t g = 0; g < ngroups; g++) {
-        int64_t g0 = g * group, g1 = g0 + group < N ? g0 + group : N;
+    int64_t npp = 0, npn = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : npp, npn)
+    for (int64_t gk = 0; gk < 2 * N - 1; gk++) {
+        if (count[gk] > group) continue;
+        if (gk != 0 && count[parent[gk]] <= group) continue;
+        int64_t g0 = first[gk], g1 = g0 + count[gk];
         double gb[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
         for (int64_t i = g0; i < g1; i++)
             for (int d = 0; d < 3; d++) {

@@ -68,7 +68,7 @@ struct sph_ctx {
     // ordering
     Buf keys0, keys1, vals0, vals1, hist, bpart, box;
     // tree
-    Buf nodes, right, parent, flags, front;
+    Buf nodes, right, parent, flags, front, groups;
     // neighbour lists
     Buf cnt, off, scantmp, idx;
     // hydro scalars (8 bytes per particle, used as float or double)
@@ -83,6 +83,7 @@ struct sph_ctx {
     bool order_ok = false, h_ok = false, m_ok = false, eps_ok = false, moments_ok = false, csr_ok = false;
     bool resident = false, primed = false;
     int64_t csr_total = 0;
+    int64_t ngroups = 0;  // tree-node target groups of the current tree
 
     // statistics
     sph_stats stats;
@@ -149,9 +150,10 @@ int capacity(sph_ctx *ctx, int64_t n)
     CK(ensure(ctx, ctx->parent, 2 * ni));
     CK(ensure(ctx, ctx->flags, ni));
     CK(ensure(ctx, ctx->front, 2 * ni));
+    CK(ensure(ctx, ctx->groups, 2 * ni));
     CK(ensure(ctx, ctx->off, (size_t)(n + 1) * 8));
     CK(ensure(ctx, ctx->scantmp, (size_t)sph::scan_tmp_entries(n) * 8));
-    CK(ensure(ctx, ctx->hcount, 8 * 8));
+    CK(ensure(ctx, ctx->hcount, 16 * 8));
     CK(ensure(ctx, ctx->stage, 3 * nd));
     CK(ensure(ctx, ctx->kpart, 5 * 1024 * 8));
     CK(ensure(ctx, ctx->kout, 8 * 8));
@@ -183,6 +185,17 @@ int sort_current(sph_ctx *ctx, int64_t n)
     return SPH_OK;
 }
 
+// Tree topology of the sorted keys and the target groups (maximal nodes of
+// at most 64 particles) used by every walk.
+int topology(sph_ctx *ctx)
+{
+    HIPCK(sph::tree_topology(ctx->st, ctx->n, P_<uint64_t>(ctx->keys0), P_<WNode>(ctx->nodes), I32(right),
+                             I32(parent)));
+    HIPCK(sph::gravity_groups(ctx->st, ctx->n, P_<WNode>(ctx->nodes), I32(parent), I32(groups),
+                              (unsigned int *)(P_<unsigned long long>(ctx->hcount) + 9), &ctx->ngroups));
+    return SPH_OK;
+}
+
 // Stateless entry: establish the Morton order of the caller's positions.
 int set_pos(sph_ctx *ctx, int64_t n, const double *pos)
 {
@@ -206,7 +219,7 @@ int set_pos(sph_ctx *ctx, int64_t n, const double *pos)
         double *dst = d == 0 ? D(x) : d == 1 ? D(y) : D(z);
         HIPCK(sph::gather<double, double>(st, n, I32(orig), D(stage), 3, d, dst));
     }
-    HIPCK(sph::tree_topology(st, n, P_<uint64_t>(ctx->keys0), P_<WNode>(ctx->nodes), I32(right), I32(parent)));
+    CK(topology(ctx));
     ctx->cpos.assign(pos, pos + 3 * n);
     ctx->order_ok = true;
     return SPH_OK;
@@ -283,12 +296,14 @@ int build_csr(sph_ctx *ctx)
 {
     int64_t n = ctx->n, total = 0;
     int64_t cap = ctx->idx.p ? (int64_t)(ctx->idx.bytes / 4) : 0;
-    HIPCK(sph::neighbours(ctx->st, n, D(x), D(y), D(z), D(h), P_<WNode>(ctx->nodes), SPH_BUCKET, I32(cnt),
-                          P_<int64_t>(ctx->off), P_<int64_t>(ctx->scantmp), I32(idx), cap, &total));
+    unsigned long long *ovf = P_<unsigned long long>(ctx->hcount) + 8;
+    HIPCK(sph::neighbours(ctx->st, n, ctx->ngroups, I32(groups), D(x), D(y), D(z), D(h), P_<WNode>(ctx->nodes),
+                          SPH_BUCKET, I32(cnt), P_<int64_t>(ctx->off), P_<int64_t>(ctx->scantmp), ovf, I32(idx), cap,
+                          &total));
     if (total > cap) {
         CK(ensure(ctx, ctx->idx, (size_t)total * 4));
-        HIPCK(sph::neighbours_fill(ctx->st, n, D(x), D(y), D(z), D(h), P_<WNode>(ctx->nodes), SPH_BUCKET,
-                                   P_<int64_t>(ctx->off), I32(idx)));
+        HIPCK(sph::neighbours_fill(ctx->st, ctx->ngroups, I32(groups), D(x), D(y), D(z), D(h), P_<WNode>(ctx->nodes),
+                                   SPH_BUCKET, P_<int64_t>(ctx->off), I32(idx), ovf));
     }
     ctx->csr_total = total;
     ctx->stats.ngb_total = total;
@@ -373,7 +388,7 @@ extern "C" int sph_ctx_destroy(sph_ctx *ctx)
     Buf *all[] = {&ctx->x, &ctx->y, &ctx->z, &ctx->vx, &ctx->vy, &ctx->vz, &ctx->m, &ctx->h, &ctx->eps,
                   &ctx->u, &ctx->duprev, &ctx->ax, &ctx->ay, &ctx->az, &ctx->gx, &ctx->gy, &ctx->gz,
                   &ctx->tmpd, &ctx->orig, &ctx->tmpi, &ctx->keys0, &ctx->keys1, &ctx->vals0, &ctx->vals1,
-                  &ctx->hist, &ctx->bpart, &ctx->box, &ctx->nodes, &ctx->right, &ctx->parent, &ctx->flags, &ctx->front,
+                  &ctx->hist, &ctx->bpart, &ctx->box, &ctx->nodes, &ctx->right, &ctx->parent, &ctx->flags, &ctx->front, &ctx->groups,
                   &ctx->cnt, &ctx->off, &ctx->scantmp, &ctx->idx, &ctx->rho, &ctx->P, &ctx->c, &ctx->prho2,
                   &ctx->divV, &ctx->curlV, &ctx->bal, &ctx->du, &ctx->cand, &ctx->hR, &ctx->hsteps,
                   &ctx->hstate, &ctx->hcount, &ctx->hout, &ctx->hin, &ctx->tx, &ctx->ty, &ctx->tz,
@@ -421,9 +436,11 @@ int run_hsolve(sph_ctx *ctx, int64_t m, const double *tx, const double *ty, cons
     p.counters = P_<unsigned long long>(ctx->hcount);
     int64_t unc = 0;
     p.unconverged_total = &unc;
+    p.groups = (tx == D(x)) ? I32(groups) : nullptr;  // targets are the sorted particles: tree-node groups
+    p.ngroups = ctx->ngroups;
+    p.rounds = rounds_out;
     int64_t pending = 0;
     HIPCK(sph::hsolve(ctx->st, p, SPH_HROUNDS, &pending));
-    if (rounds_out) *rounds_out = 0;
     if (pending) return fail(ctx, SPH_ERR_NOCONV, "smoothing-length solve: " + std::to_string(pending) +
                                                       " targets undecided after the maximum number of radius rounds");
     *n_unconv = unc;
@@ -654,12 +671,13 @@ extern "C" int sph_gravity(sph_ctx *ctx, int64_t n, const double *pos, const dou
     unsigned long long *cnt = P_<unsigned long long>(ctx->hcount);
     HIPCK(sph::fill<unsigned long long>(ctx->st, 3, cnt, 0ull));
     const double *e = ctx->eps_ok ? D(eps) : nullptr;
+    const int64_t ng = ctx->ngroups;
     if (precision == SPH_PREC_MIXED)
-        HIPCK(sph::gravity<float>(ctx->st, n, D(x), D(y), D(z), D(m), e, P_<WNode>(ctx->nodes), SPH_BUCKET, theta, G,
-                                  soft_kind, eps_plummer, D(gx), D(gy), D(gz), cnt));
+        HIPCK(sph::gravity<float>(ctx->st, ng, I32(groups), D(x), D(y), D(z), D(m), e, P_<WNode>(ctx->nodes),
+                                  SPH_BUCKET, theta, G, soft_kind, eps_plummer, D(gx), D(gy), D(gz), cnt));
     else
-        HIPCK(sph::gravity<double>(ctx->st, n, D(x), D(y), D(z), D(m), e, P_<WNode>(ctx->nodes), SPH_BUCKET, theta,
-                                   G, soft_kind, eps_plummer, D(gx), D(gy), D(gz), cnt));
+        HIPCK(sph::gravity<double>(ctx->st, ng, I32(groups), D(x), D(y), D(z), D(m), e, P_<WNode>(ctx->nodes),
+                                   SPH_BUCKET, theta, G, soft_kind, eps_plummer, D(gx), D(gy), D(gz), cnt));
     unsigned long long hc[3];
     HIPCK(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, ctx->st));
     HIPCK(hipStreamSynchronize(ctx->st));
@@ -767,7 +785,7 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     CK(reorder_state(ctx));
     ph_end(ctx, SPH_PH_ORDER);
     ph_begin(ctx, SPH_PH_TOPOLOGY);
-    HIPCK(sph::tree_topology(st, n, P_<uint64_t>(ctx->keys0), P_<WNode>(ctx->nodes), I32(right), I32(parent)));
+    CK(topology(ctx));
     ph_end(ctx, SPH_PH_TOPOLOGY);
     // boxes are needed by the h walk; hmax is refreshed after the solve
     ph_begin(ctx, SPH_PH_HSOLVE);
@@ -775,7 +793,7 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
                             I32(parent), P_<uint32_t>(ctx->flags), I32(front)));
     HIPCK(hipMemcpyAsync(ctx->hin.p, ctx->h.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     int64_t unc = 0;
-    CK(run_hsolve(ctx, n, D(x), D(y), D(z), SPH_H_NEWTON, p->n_ngb, 0, 0, 0, p->h_tol, &unc, nullptr));
+    CK(run_hsolve(ctx, n, D(x), D(y), D(z), SPH_H_NEWTON, p->n_ngb, 0, 0, 0, p->h_tol, &unc, &ctx->stats.h_rounds));
     std::swap(ctx->h, ctx->hout);
     ctx->stats.h_unconverged += unc;
     ph_end(ctx, SPH_PH_HSOLVE);
@@ -798,8 +816,8 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     unsigned long long *cnt = P_<unsigned long long>(ctx->hcount) + 4;
     HIPCK(sph::fill<unsigned long long>(st, 3, cnt, 0ull));
     if (p->G != 0.0) {
-        HIPCK(sph::gravity<R>(st, n, D(x), D(y), D(z), D(m), eps, P_<WNode>(ctx->nodes), SPH_BUCKET, p->theta, p->G,
-                              p->soft_kind, p->eps_plummer, D(gx), D(gy), D(gz), cnt));
+        HIPCK(sph::gravity<R>(st, ctx->ngroups, I32(groups), D(x), D(y), D(z), D(m), eps, P_<WNode>(ctx->nodes), SPH_BUCKET,
+                              p->theta, p->G, p->soft_kind, p->eps_plummer, D(gx), D(gy), D(gz), cnt));
     }
     ph_end(ctx, SPH_PH_GRAVITY);
     ph_begin(ctx, SPH_PH_DIVCURL);

@@ -1,18 +1,26 @@
-// sph_grav.h -- host entry point of the tree gravity walk (sph_grav.hip).
+// sph_grav.h -- host entry points of the tree gravity walk (sph_grav.hip).
 #pragma once
 #include "sph_common.h"
+#include "sph_order.h"
 
 namespace sph {
 
-// a_i = G sum m_j f(r_ij) (r_j - r_i) over the walk's interaction set (self
+// Target groups: the maximal tree nodes with at most 64 particles.  Writes
+// their node ids to groups[] (capacity 2N-1) and the count to *ngroups
+// (synchronises the stream).
+hipError_t gravity_groups(hipStream_t st, int64_t n, const WNode *nodes, const int32_t *parent, int32_t *groups,
+                          unsigned int *ngroups_dev, int64_t *ngroups);
+
+// a_i = G sum m_j f(r_ij) (r_j - r_i) over the group's interaction list (self
 // excluded).  soft_kind 0: spline with eps_ij = (eps_i + eps_j)/2 (node:
 // epsmax), eps may be null (= 0); 1: Plummer with eps_plummer.  theta = 0
 // gives the exact direct sum.  counters[0] += particle-particle,
 // counters[1] += particle-node interactions, counters[2] += stack overflows
 // (must stay 0).
 template <typename R>
-hipError_t gravity(hipStream_t st, int64_t n, const double *x, const double *y, const double *z, const double *m,
-                   const double *eps, const WNode *nodes, int bucket, double theta, double G, int soft_kind,
-                   double eps_plummer, double *ax, double *ay, double *az, unsigned long long *counters);
+hipError_t gravity(hipStream_t st, int64_t ngroups, const int32_t *groups, const double *x, const double *y,
+                   const double *z, const double *m, const double *eps, const WNode *nodes, int bucket, double theta,
+                   double G, int soft_kind, double eps_plummer, double *ax, double *ay, double *az,
+                   unsigned long long *counters);
 
 }  // namespace sph

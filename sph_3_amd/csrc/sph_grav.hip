@@ -4,21 +4,25 @@
 // warp-shared stack, one opening decision per warp via __all_sync) and the CPU
 // direct sums sph_code/libsx.py:623-662 / :908-947 (getAcc_g_smth[_mimj_mpi]).
 //
-// One wavefront per group of 64 consecutive Morton-ordered targets.  The walk
-// is breadth-first and lane-parallel: each step pops up to 64 nodes from an
-// LDS stack, every lane tests one node against the group's bounding box
-// (FP64 opening test, identical to the CPU oracle's), and the wave appends in
-// one ballot-compacted step: accepted nodes to the source buffer, opened
-// buckets to a bucket list, children of opened internal nodes to the stack.
-// Sources (node monopoles and bucket particles) are staged in LDS relative to
-// the group centre; whenever 64 or more are staged all 64 lanes evaluate them
-// against their own target (LDS broadcast reads, no divergence), with Kahan
-// compensated sums in the pass type R.  Every particle of a group sees the
-// same list, so the interaction count is n_nodes*g + n_parts*g - g per group.
+// Targets are processed in groups: the maximal tree nodes holding at most 64
+// particles (a contiguous Morton range with a compact box), one wavefront per
+// group.  The walk is breadth-first and lane-parallel: each step pops up to
+// 64 nodes from an LDS stack, every lane tests one node against the group's
+// box (FP64 opening test, identical to the CPU oracle's), and the wave
+// appends in one ballot-compacted step: accepted nodes to the source buffer,
+// opened buckets to a bucket list, children of opened internal nodes to the
+// stack.  Sources (node monopoles and bucket particles) are staged in LDS
+// relative to the group centre; whenever 64 or more are staged all lanes
+// evaluate them against their own target (LDS broadcast reads, no
+// divergence) with Kahan-compensated sums in the pass type R.  In the FP32
+// form each relative coordinate is an unevaluated pair hi + lo of floats and
+// a pair difference is (s_hi - t_hi) + (s_lo - t_lo), so near pairs keep full
+// FP32 accuracy.  Every particle of a group sees the same list: the counted
+// interactions are n_nodes*g + n_parts*g - g per group of g particles.
 #include "sph_grav.h"
 
-#define GG 64        // targets per group (one wavefront)
-#define GSTK 4096    // LDS stack capacity (node ids)
+#define GG 64        // maximum targets per group (one wavefront)
+#define GSTK 2048    // LDS stack capacity (node ids)
 #define GSRC 128     // staged sources (< 64 before a flush + 64 appended)
 
 // Group opening test (FP64, no contraction): monopole for the whole group iff
@@ -40,31 +44,60 @@ __device__ __forceinline__ bool mac_group(const double *gb, double cx, double cy
     return (size * size < theta2 * d2) && disjoint;
 }
 
-template <typename R, int SOFT>
-__device__ __forceinline__ void eval_sources(int cnt, const R *__restrict__ sx, const R *__restrict__ sy,
-                                             const R *__restrict__ sz, const R *__restrict__ sm,
-                                             const R *__restrict__ se, R tx, R ty, R tz, R epsi, R e2,
-                                             Kahan<R> &ax, Kahan<R> &ay, Kahan<R> &az)
+// A staged source: hi part of the offset + mass, lo part of the offset + softening.
+template <typename R>
+struct __align__(4 * sizeof(R)) Src4 {
+    R x, y, z, w;
+};
+
+template <typename R>
+__device__ __forceinline__ void split(double d, R &hi, R &lo)
 {
-    for (int k = 0; k < cnt; k++) {
-        R dx = sx[k] - tx, dy = sy[k] - ty, dz = sz[k] - tz;
-        R r2 = dx * dx + dy * dy + dz * dz;
-        R w;
-        if (SOFT == 1) {
-            R inv = sph_rsqrt<R>(r2 + e2);
-            w = sm[k] * (inv * inv * inv);
-        } else {
-            w = sm[k] * grav_fk_spline<R>(sph_sqrt<R>(r2), (R)0.5 * (epsi + se[k]));
-        }
-        w = r2 > (R)0 ? w : (R)0;  // the target itself (and exact coincidences) add nothing
-        ax.add(w * dx);
-        ay.add(w * dy);
-        az.add(w * dz);
-    }
+    hi = (R)d;
+    lo = (R)(d - (double)hi);
 }
 
 template <typename R, int SOFT>
-__global__ void __launch_bounds__(GG) k_gravity_group(int64_t n, const double *__restrict__ x,
+__device__ __forceinline__ void interact(const Src4<R> &a, const Src4<R> &b, R tx, R ty, R tz, R tlx, R tly, R tlz,
+                                         R epsi, R e2, Kahan<R> &ax, Kahan<R> &ay, Kahan<R> &az)
+{
+    constexpr bool SPLIT = sizeof(R) == 4;
+    R dx = a.x - tx, dy = a.y - ty, dz = a.z - tz;
+    if (SPLIT) {
+        dx += b.x - tlx;
+        dy += b.y - tly;
+        dz += b.z - tlz;
+    }
+    R r2 = dx * dx + dy * dy + dz * dz;
+    R w;
+    if (SOFT == 1) {
+        R inv = sph_rsqrt<R>(r2 + e2);
+        w = a.w * (inv * inv * inv);
+    } else {
+        w = a.w * grav_fk_spline<R>(sph_sqrt<R>(r2), (R)0.5 * (epsi + b.w));
+    }
+    w = r2 > (R)0 ? w : (R)0;  // the target itself (and exact coincidences) add nothing
+    ax.add(w * dx);
+    ay.add(w * dy);
+    az.add(w * dz);
+}
+
+// Evaluate cnt staged sources; two independent accumulator sets for ILP.
+template <typename R, int SOFT>
+__device__ __forceinline__ void eval_sources(int cnt, const Src4<R> *__restrict__ A, const Src4<R> *__restrict__ B,
+                                             R tx, R ty, R tz, R tlx, R tly, R tlz, R epsi, R e2, Kahan<R> *k)
+{
+    int q = 0;
+    for (; q + 1 < cnt; q += 2) {
+        const Src4<R> a0 = A[q], b0 = B[q], a1 = A[q + 1], b1 = B[q + 1];
+        interact<R, SOFT>(a0, b0, tx, ty, tz, tlx, tly, tlz, epsi, e2, k[0], k[1], k[2]);
+        interact<R, SOFT>(a1, b1, tx, ty, tz, tlx, tly, tlz, epsi, e2, k[3], k[4], k[5]);
+    }
+    if (q < cnt) interact<R, SOFT>(A[q], B[q], tx, ty, tz, tlx, tly, tlz, epsi, e2, k[0], k[1], k[2]);
+}
+
+template <typename R, int SOFT>
+__global__ void __launch_bounds__(GG) k_gravity_group(const int32_t *__restrict__ groups, const double *__restrict__ x,
                                                       const double *__restrict__ y, const double *__restrict__ z,
                                                       const double *__restrict__ m, const double *__restrict__ eps,
                                                       const WNode *__restrict__ nodes, int bucket, double theta2,
@@ -73,37 +106,38 @@ __global__ void __launch_bounds__(GG) k_gravity_group(int64_t n, const double *_
                                                       unsigned long long *__restrict__ counters)
 {
     __shared__ int stk[GSTK];
-    __shared__ R sx[GSRC], sy[GSRC], sz[GSRC], sm[GSRC], se[GSRC];
+    __shared__ Src4<R> A[GSRC], B[GSRC];
     __shared__ int bfirst[GG], bcount[GG];
+    constexpr bool SPLIT = sizeof(R) == 4;
     const int lane = threadIdx.x;
-    const int64_t i = blockIdx.x * (int64_t)GG + lane;
-    const bool valid = i < n;
+    const int gnode = groups[blockIdx.x];
+    const int g0 = nodes[gnode].first, ng = nodes[gnode].count;
+    const bool valid = lane < ng;
+    const int64_t i = (int64_t)g0 + lane;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     double xi = 0, yi = 0, zi = 0, epsi = 0;
     if (valid) {
         xi = x[i]; yi = y[i]; zi = z[i];
         epsi = eps ? eps[i] : 0.0;
     }
-    // group box (FP64 wave reductions)
-    double gb[6];
-    gb[0] = valid ? xi : INFINITY; gb[1] = valid ? yi : INFINITY; gb[2] = valid ? zi : INFINITY;
-    gb[3] = valid ? xi : -INFINITY; gb[4] = valid ? yi : -INFINITY; gb[5] = valid ? zi : -INFINITY;
-    for (int off = 32; off >= 1; off >>= 1)
-        for (int d = 0; d < 3; d++) {
-            gb[d] = fmin(gb[d], __shfl_xor(gb[d], off));
-            gb[3 + d] = fmax(gb[3 + d], __shfl_xor(gb[3 + d], off));
-        }
+    // the group box is the group node's box
+    const WNode *gn = nodes + gnode;
+    const double gb[6] = {gn->lx, gn->ly, gn->lz, gn->hx, gn->hy, gn->hz};
     const double gcx = 0.5 * (gb[0] + gb[3]), gcy = 0.5 * (gb[1] + gb[4]), gcz = 0.5 * (gb[2] + gb[5]);
-    const R tx = (R)(xi - gcx), ty = (R)(yi - gcy), tz = (R)(zi - gcz), te = (R)epsi;
+    R tx, ty, tz, tlx, tly, tlz;
+    split<R>(xi - gcx, tx, tlx);
+    split<R>(yi - gcy, ty, tly);
+    split<R>(zi - gcz, tz, tlz);
+    const R te = (R)epsi;
     const R e2 = (R)(eps_plummer * eps_plummer);
-    const int ng = (int)__popcll(wave_ballot(valid));
-    Kahan<R> kx, ky, kz;
+    Kahan<R> k[6];
     long long n_nodes = 0, n_parts = 0;
     int nsrc = 0;
     int sp = 1;
     if (lane == 0) stk[0] = 0;
     __syncthreads();
     while (sp > 0) {
+        // pop up to 64 nodes; one at a time near capacity (growth then <= 1 per level)
         const int nt = (sp > GSTK - 2 * GG) ? 1 : (sp < GG ? sp : GG);
         const int base = sp - nt;
         const int node = lane < nt ? stk[base + lane] : -1;
@@ -135,9 +169,15 @@ __global__ void __launch_bounds__(GG) k_gravity_group(int64_t n, const double *_
         sp += 2 * (int)__popcll(mo);
         if (sp > GSTK) sp = GSTK;
         if (acc) {
-            const int k = nsrc + (int)__popcll(ma & lt);
-            sx[k] = (R)(ncx - gcx); sy[k] = (R)(ncy - gcy); sz[k] = (R)(ncz - gcz);
-            sm[k] = (R)nmass; se[k] = (R)neps;
+            const int q = nsrc + (int)__popcll(ma & lt);
+            Src4<R> a, b;
+            split<R>(ncx - gcx, a.x, b.x);
+            split<R>(ncy - gcy, a.y, b.y);
+            split<R>(ncz - gcz, a.z, b.z);
+            a.w = (R)nmass;
+            b.w = (R)neps;
+            A[q] = a;
+            B[q] = b;
         }
         nsrc += (int)__popcll(ma);
         n_nodes += (long long)__popcll(ma);
@@ -149,32 +189,38 @@ __global__ void __launch_bounds__(GG) k_gravity_group(int64_t n, const double *_
         const int nb = (int)__popcll(mb);
         __syncthreads();
         if (nsrc >= GG) {
-            eval_sources<R, SOFT>(nsrc, sx, sy, sz, sm, se, tx, ty, tz, te, e2, kx, ky, kz);
+            eval_sources<R, SOFT>(nsrc, A, B, tx, ty, tz, tlx, tly, tlz, te, e2, k);
             nsrc = 0;
             __syncthreads();
         }
-        for (int b = 0; b < nb; b++) {
-            const int f = bfirst[b], c = bcount[b];
+        for (int bb = 0; bb < nb; bb++) {
+            const int f = bfirst[bb], c = bcount[bb];
             if (lane < c) {
-                const int j = f + lane, k = nsrc + lane;
-                sx[k] = (R)(x[j] - gcx); sy[k] = (R)(y[j] - gcy); sz[k] = (R)(z[j] - gcz);
-                sm[k] = (R)m[j]; se[k] = eps ? (R)eps[j] : (R)0;
+                const int j = f + lane, q = nsrc + lane;
+                Src4<R> a, b;
+                split<R>(x[j] - gcx, a.x, b.x);
+                split<R>(y[j] - gcy, a.y, b.y);
+                split<R>(z[j] - gcz, a.z, b.z);
+                a.w = (R)m[j];
+                b.w = eps ? (R)eps[j] : (R)0;
+                A[q] = a;
+                B[q] = b;
             }
             nsrc += c;
             n_parts += c;
             __syncthreads();
             if (nsrc >= GG) {
-                eval_sources<R, SOFT>(nsrc, sx, sy, sz, sm, se, tx, ty, tz, te, e2, kx, ky, kz);
+                eval_sources<R, SOFT>(nsrc, A, B, tx, ty, tz, tlx, tly, tlz, te, e2, k);
                 nsrc = 0;
                 __syncthreads();
             }
         }
     }
-    if (nsrc > 0) eval_sources<R, SOFT>(nsrc, sx, sy, sz, sm, se, tx, ty, tz, te, e2, kx, ky, kz);
+    if (nsrc > 0) eval_sources<R, SOFT>(nsrc, A, B, tx, ty, tz, tlx, tly, tlz, te, e2, k);
     if (valid) {
-        ax[i] = G * (double)kx.value();
-        ay[i] = G * (double)ky.value();
-        az[i] = G * (double)kz.value();
+        ax[i] = G * ((double)k[0].value() + (double)k[3].value());
+        ay[i] = G * ((double)k[1].value() + (double)k[4].value());
+        az[i] = G * ((double)k[2].value() + (double)k[5].value());
     }
     if (lane == 0) {
         atomicAdd(&counters[0], (unsigned long long)(n_parts * ng - ng));
@@ -182,30 +228,57 @@ __global__ void __launch_bounds__(GG) k_gravity_group(int64_t n, const double *_
     }
 }
 
+// Groups: node k with count <= gmax whose parent holds more (or the root).
+__global__ void __launch_bounds__(256) k_find_groups(int64_t nn, const WNode *__restrict__ nodes,
+                                                     const int32_t *__restrict__ parent, int gmax,
+                                                     int32_t *__restrict__ groups, unsigned int *__restrict__ ngroups)
+{
+    int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= nn) return;
+    if (nodes[k].count > gmax) return;
+    if (k != 0 && nodes[parent[k]].count <= gmax) return;
+    unsigned int q = atomicAdd(ngroups, 1u);
+    groups[q] = (int32_t)k;
+}
+
 namespace sph {
 
+hipError_t gravity_groups(hipStream_t st, int64_t n, const WNode *nodes, const int32_t *parent, int32_t *groups,
+                          unsigned int *ngroups_dev, int64_t *ngroups)
+{
+    fill<unsigned int>(st, 1, ngroups_dev, 0u);
+    k_find_groups<<<sph_blocks(2 * n - 1, 256), 256, 0, st>>>(2 * n - 1, nodes, parent, GG, groups, ngroups_dev);
+    unsigned int h = 0;
+    hipError_t e = hipMemcpyAsync(&h, ngroups_dev, sizeof(h), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(st);
+    *ngroups = h;
+    return e;
+}
+
 template <typename R>
-hipError_t gravity(hipStream_t st, int64_t n, const double *x, const double *y, const double *z, const double *m,
-                   const double *eps, const WNode *nodes, int bucket, double theta, double G, int soft_kind,
-                   double eps_plummer, double *ax, double *ay, double *az, unsigned long long *counters)
+hipError_t gravity(hipStream_t st, int64_t ngroups, const int32_t *groups, const double *x, const double *y,
+                   const double *z, const double *m, const double *eps, const WNode *nodes, int bucket, double theta,
+                   double G, int soft_kind, double eps_plummer, double *ax, double *ay, double *az,
+                   unsigned long long *counters)
 {
     if (bucket > GG) return hipErrorInvalidValue;  // a bucket is expanded by one lane per particle
+    if (ngroups <= 0) return hipSuccess;
     double theta2 = theta * theta;
-    unsigned nb = sph_blocks(n, GG);
     if (soft_kind == 0)
-        k_gravity_group<R, 0><<<nb, GG, 0, st>>>(n, x, y, z, m, eps, nodes, bucket, theta2, G, eps_plummer, ax, ay,
-                                                 az, counters);
+        k_gravity_group<R, 0><<<(unsigned)ngroups, GG, 0, st>>>(groups, x, y, z, m, eps, nodes, bucket, theta2, G,
+                                                               eps_plummer, ax, ay, az, counters);
     else
-        k_gravity_group<R, 1><<<nb, GG, 0, st>>>(n, x, y, z, m, eps, nodes, bucket, theta2, G, eps_plummer, ax, ay,
-                                                 az, counters);
+        k_gravity_group<R, 1><<<(unsigned)ngroups, GG, 0, st>>>(groups, x, y, z, m, eps, nodes, bucket, theta2, G,
+                                                               eps_plummer, ax, ay, az, counters);
     return hipGetLastError();
 }
 
-template hipError_t gravity<float>(hipStream_t, int64_t, const double *, const double *, const double *,
-                                   const double *, const double *, const WNode *, int, double, double, int, double,
-                                   double *, double *, double *, unsigned long long *);
-template hipError_t gravity<double>(hipStream_t, int64_t, const double *, const double *, const double *,
-                                    const double *, const double *, const WNode *, int, double, double, int, double,
-                                    double *, double *, double *, unsigned long long *);
+template hipError_t gravity<float>(hipStream_t, int64_t, const int32_t *, const double *, const double *,
+                                   const double *, const double *, const double *, const WNode *, int, double, double,
+                                   int, double, double *, double *, double *, unsigned long long *);
+template hipError_t gravity<double>(hipStream_t, int64_t, const int32_t *, const double *, const double *,
+                                    const double *, const double *, const double *, const WNode *, int, double,
+                                    double, int, double, double *, double *, double *, unsigned long long *);
 
 }  // namespace sph

@@ -21,20 +21,27 @@ struct HSolveParams {
     double *h_out;               // result (device)
     double *cand, *R;            // scratch: cap*m, m
     int32_t *steps, *state;      // scratch: m, m
-    unsigned long long *counters;  // scratch: 2
+    unsigned long long *counters;  // scratch: 3
     int64_t *unconverged_total;  // host accumulator (may be null)
+    const int32_t *groups;       // tree-node groups when the targets are the sorted particles, else null
+    int64_t ngroups;
+    int64_t *rounds;             // host: rounds used (may be null)
 };
 
 // Runs rounds until every target is decided; *n_unconverged = targets still
-// pending after max_rounds (0 on success).  Synchronises the stream.
+// pending after max_rounds (0 on success).  Synchronises the stream.  Returns
+// hipErrorIllegalState if a walk stack overflowed.
 hipError_t hsolve(hipStream_t st, const HSolveParams &p, int max_rounds, int64_t *n_unconverged);
 
-// CSR of S_i = { j : r_ij^2 < (h_i+h_j)^2 } over sorted particles.  Counts,
-// scans (off[0..n]), reads back *total; fills idx if idx_cap >= *total.
-hipError_t neighbours(hipStream_t st, int64_t n, const double *x, const double *y, const double *z,
-                      const double *h, const WNode *nodes, int bucket, int32_t *cnt, int64_t *off,
-                      int64_t *scan_tmp, int32_t *idx, int64_t idx_cap, int64_t *total);
-hipError_t neighbours_fill(hipStream_t st, int64_t n, const double *x, const double *y, const double *z,
-                           const double *h, const WNode *nodes, int bucket, const int64_t *off, int32_t *idx);
+// CSR of S_i = { j : r_ij^2 < (h_i+h_j)^2 } over sorted particles, one
+// wavefront per tree-node group.  Counts, scans (off[0..n]), reads back
+// *total; fills idx if idx_cap >= *total.  overflow: 1 scratch counter.
+hipError_t neighbours(hipStream_t st, int64_t n, int64_t ngroups, const int32_t *groups, const double *x,
+                      const double *y, const double *z, const double *h, const WNode *nodes, int bucket, int32_t *cnt,
+                      int64_t *off, int64_t *scan_tmp, unsigned long long *overflow, int32_t *idx, int64_t idx_cap,
+                      int64_t *total);
+hipError_t neighbours_fill(hipStream_t st, int64_t ngroups, const int32_t *groups, const double *x, const double *y,
+                           const double *z, const double *h, const WNode *nodes, int bucket, const int64_t *off,
+                           int32_t *idx, unsigned long long *overflow);
 
 }  // namespace sph

@@ -6,15 +6,18 @@
 // (EV/lib.h:485-579 smoothing_h_ngb) and the CPU h searches
 // (sph_code/libsx.py:752-900 do_smoothingX, h_smooth_fast).
 //
-// All walks are wavefront-cooperative and stackless: one lane per target,
-// targets in Morton order, the wave follows one pre-order path through the
-// tree (a node is opened when any active lane needs it), and each lane keeps
-// a single "resume" node id at which it rejoins the walk after a subtree it
-// did not need.  Node records are read with wave-uniform (scalar) loads.
+// The CSR build and the Newton / k-th-distance h solves use the group walk of
+// sph_walk.h: one wavefront per group of <= 64 targets, a conservative
+// group-level box test per node (64 nodes tested per step), and the particles
+// of every surviving bucket staged in LDS where each lane applies its own
+// exact per-target test.  The reference's count stepping (k_hstep) keeps a
+// per-lane cooperative stackless walk: one lane per target, the wave follows
+// one pre-order path, each lane keeps a single "resume" node id.
 //
 // FP64 throughout, contraction off: membership r^2 < (h_i+h_j)^2 is decided
 // exactly as the CPU oracle decides it.
 #include "sph_ngb.h"
+#include "sph_walk.h"
 
 #pragma clang fp contract(off)
 
@@ -66,7 +69,9 @@ __device__ __forceinline__ void coop_walk(const WNode *__restrict__ nodes, bool 
 // mode 0: Newton-Raphson root of (32/3) sum_j w(r_j/h) - n_ngb (FP64),
 // mode 1: the reference's +-0.3 % count stepping (k_hstep below),
 // mode 2: h = 0.5 * (k+1)-th smallest distance (libsx.py:752-774, k = 50).
-// Candidates (r^2 < R_t^2) go to cand[c*M + t], c < cap.  A target whose
+// Candidates (r^2 < R_t^2) of target t go to its row cand[t*cap + c], c < cap,
+// through an 8-entry LDS buffer per lane flushed as whole 64-byte pieces.  The
+// solve then takes the group's targets one at a time with the whole wave.  A target whose
 // candidate set cannot decide its answer grows or shrinks R_t and stays
 // pending (state[t] = 0); the host relaunches until every target is done.
 struct HSolveArgs {
@@ -80,83 +85,154 @@ struct HSolveArgs {
     int n_low, n_up, max_steps;  // mode 1
     int kth;                   // mode 2
     int cap;
-    double *cand;              // cap * m doubles
+    double *cand;              // m rows of cap doubles
     double *R;                 // per target search radius (in/out)
     double *h;                 // per target h (in: start, out: result)
     int32_t *steps;            // mode 1 step counter (persists over rounds)
     int32_t *state;            // 0 pending, 1 done, 2 done-unconverged
-    unsigned long long *pending;  // [0] pending count, [1] unconverged count
+    unsigned long long *pending;  // [0] pending, [1] unconverged, [2] walk overflows
+    const int32_t *groups;     // tree-node groups of the targets (null: blocks of 64 targets)
+    const WNode *gnodes;       // node records the group ids refer to
 };
 
-__device__ __forceinline__ void nr_eval(const double *__restrict__ cand, int64_t m, int64_t t, int c, double h,
-                                        double n_ngb, double &f, double &df)
+// Group range: tree node groups[blk] or the block [64 blk, 64 blk + 64) of targets.
+__device__ __forceinline__ void group_range(const int32_t *groups, const WNode *gnodes, int64_t m, int64_t &first,
+                                            int &count)
 {
-    double s = 0.0, ds = 0.0, ih = 1.0 / h;
-    for (int k = 0; k < c; k++) {
-        double q = __builtin_sqrt(cand[(int64_t)k * m + t]) * ih;
-        if (q < 1.0) {
-            s += 1.0 - 1.5 * q * q + 0.75 * q * q * q;
-            ds += (-3.0 * q + 2.25 * q * q) * (-q * ih);
-        } else if (q < 2.0) {
-            double u = 2.0 - q;
-            s += 0.25 * u * u * u;
-            ds += (-0.75 * u * u) * (-q * ih);
-        }
+    if (groups) {
+        const WNode *g = gnodes + groups[blockIdx.x];
+        first = g->first;
+        count = g->count;
+    } else {
+        first = (int64_t)blockIdx.x * 64;
+        count = (int)((m - first) < 64 ? (m - first) : 64);
     }
-    f = (32.0 / 3.0) * s - n_ngb;
-    df = (32.0 / 3.0) * ds;
 }
 
-__device__ __forceinline__ int count_below(const double *__restrict__ cand, int64_t m, int64_t t, int c, double lim)
+__global__ void __launch_bounds__(64) k_hsolve(HSolveArgs a)
 {
-    int n = 0;
-    for (int k = 0; k < c; k++) n += __builtin_sqrt(cand[(int64_t)k * m + t]) < lim;
-    return n;
-}
-
-__global__ void __launch_bounds__(256) k_hsolve(HSolveArgs a)
-{
-    int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    bool valid = t < a.m && a.state[t] == 0;
+    __shared__ WalkLDS L;
+    __shared__ double sx[WSRC], sy[WSRC], sz[WSRC];
+    __shared__ double buf[64][9];  // per-lane staging of candidate r^2 (padded row)
+    int64_t first;
+    int ng;
+    group_range(a.groups, a.gnodes, a.m, first, ng);
+    const int lane = threadIdx.x;
+    const int64_t t = first + lane;
+    const bool valid = lane < ng && a.state[t] == 0;
+    if (__ballot(valid) == 0) return;  // whole group decided (uniform)
     double xi = 0, yi = 0, zi = 0, R = 0;
     if (valid) {
         xi = a.tx[t]; yi = a.ty[t]; zi = a.tz[t];
         R = a.R[t];
     }
-    const double R2 = R * R;
-    int c = 0;
-    const int64_t m = a.m;
-    double *cand = a.cand;
+    double gb[6];
+    group_box(valid, xi, yi, zi, gb);
+    double Rg = R;
+    for (int off = 32; off >= 1; off >>= 1) Rg = fmax(Rg, __shfl_xor(Rg, off));
+    const double R2 = R * R, Rg2 = Rg * Rg;
     const int cap = a.cap;
-    coop_walk(
-        a.nodes, valid, a.bucket, [&](const WNode *nd) { return box_dist2(xi, yi, zi, nd) >= R2; },
-        [&](int j) {
-            double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = a.z[j] - zi;
-            double r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 < R2) {
-                if (c < cap) cand[(int64_t)c * m + t] = r2;
-                c++;
-            }
+    double *row = a.cand + (valid ? t : 0) * (int64_t)cap;
+    int c = 0;
+    const bool ovf = group_walk(
+        a.nodes, a.bucket, L, [&](const WNode *nd) { return box_box_dist2(gb, nd) >= Rg2; },
+        [&](int q, int j) {
+            sx[q] = a.x[j];
+            sy[q] = a.y[j];
+            sz[q] = a.z[j];
+        },
+        [&](int cnt) {
+            if (valid)
+                for (int k = 0; k < cnt; k++) {
+                    double dx = sx[k] - xi, dy = sy[k] - yi, dz = sz[k] - zi;
+                    double r2 = dx * dx + dy * dy + dz * dz;
+                    if (r2 < R2) {
+                        const int b = c & 7;
+                        buf[lane][b] = r2;
+                        if (b == 7 && c < cap) {
+#pragma unroll
+                            for (int u = 0; u < 8; u++) row[c - 7 + u] = buf[lane][u];
+                        }
+                        c++;
+                    }
+                }
         });
-    if (!valid) return;
-    const bool over = c > cap;
-    const int cc = over ? cap : c;
-    int st = 0;  // stays pending unless decided below
-    double hnew = a.h[t], Rnew = R;
-    if (a.mode == 0) {
-        if (over) {
-            Rnew = 0.75 * R;
-        } else {
-            double hmaxv = 0.5 * R, f, df;
-            nr_eval(cand, m, t, cc, hmaxv, a.n_ngb, f, df);
+    if (ovf && lane == 0) atomicAdd(&a.pending[2], 1ull);
+    const bool over = valid && c > cap;
+    if (valid && !over)
+        for (int k = c & ~7; k < c; k++) row[k] = buf[lane][k & 7];
+    __syncthreads();  // rows written by each lane are read by the whole wave below
+    // Solve four targets at a time, 16 lanes per target: lane l16 of the
+    // sub-group holds candidates l16, l16+16, ... (up to 16, in registers) and
+    // the sub-group reduces with butterflies, which give its 16 lanes bitwise
+    // the same sums, so each sub-group's iteration is uniform.
+    double h_res = valid ? a.h[t] : 0.0, R_res = R;
+    int st_res = 0;
+    if (over) R_res = 0.75 * R;
+    uint64_t todo = __ballot(valid && !over);
+    const int sub = lane >> 4, l16 = lane & 15;
+    while (todo) {
+        // targets of this batch: the next (up to) four set bits
+        int js[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            js[q] = todo ? __builtin_ctzll(todo) : -1;
+            if (todo) todo &= todo - 1;
+        }
+        const int j = js[sub];
+        const int jj = j < 0 ? 0 : j;
+        // every lane executes the cross-lane reads (a lane that is switched off
+        // at a permute would not supply its value)
+        const int cjs = __shfl(c, jj);
+        const double Rj = __shfl(R, jj), hj0 = __shfl(h_res, jj);
+        const int cj = j < 0 ? 0 : cjs;
+        const double *rowj = a.cand + (first + jj) * (int64_t)cap;
+        const int nsl = (cj + 15) >> 4;  // uniform within the sub-group
+        double r[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int k = l16 + 16 * u;
+            r[u] = (k < cj) ? rowj[k] : INFINITY;  // squared distances
+        }
+        double hnew = hj0, Rnew = Rj;
+        int st = 0;
+        if (j >= 0 && a.mode == 0) {
+#pragma unroll
+            for (int u = 0; u < 16; u++)
+                if (u < nsl) r[u] = (r[u] < INFINITY) ? __builtin_sqrt(r[u]) : INFINITY;
+            auto feval = [&](double hh, double &f, double &df) {
+                double sl = 0.0, dsl = 0.0;
+                const double ih = 1.0 / hh;
+#pragma unroll
+                for (int u = 0; u < 16; u++) {
+                    if (u >= nsl) break;
+                    double q = r[u] * ih;
+                    if (q < 1.0) {
+                        sl += 1.0 - 1.5 * q * q + 0.75 * q * q * q;
+                        dsl += (-3.0 * q + 2.25 * q * q) * (-q * ih);
+                    } else if (q < 2.0) {
+                        double w = 2.0 - q;
+                        sl += 0.25 * w * w * w;
+                        dsl += (-0.75 * w * w) * (-q * ih);
+                    }
+                }
+                for (int off = 8; off >= 1; off >>= 1) {
+                    sl += __shfl_xor(sl, off, 16);
+                    dsl += __shfl_xor(dsl, off, 16);
+                }
+                f = (32.0 / 3.0) * sl - a.n_ngb;
+                df = (32.0 / 3.0) * dsl;
+            };
+            double hmaxv = 0.5 * Rj, f, df;
+            feval(hmaxv, f, df);
             if (f < 0.0) {
-                Rnew = 1.5 * R;
+                Rnew = 1.5 * Rj;
             } else {
-                double lo = 0.0, hi = hmaxv, hh = hnew;
+                double lo = 0.0, hi = hmaxv, hh = hj0;
                 if (!(hh > 0.0 && hh < hi)) hh = 0.5 * hi;
                 st = 2;
                 for (int it = 0; it < 100; it++) {
-                    nr_eval(cand, m, t, cc, hh, a.n_ngb, f, df);
+                    feval(hh, f, df);
                     if (f < 0.0) lo = hh; else hi = hh;
                     double hn = (df > 0.0) ? hh - f / df : -1.0;
                     if (!(hn > lo && hn < hi)) hn = 0.5 * (lo + hi);
@@ -166,39 +242,50 @@ __global__ void __launch_bounds__(256) k_hsolve(HSolveArgs a)
                 }
                 hnew = hh;
             }
-        }
-    } else {
-        const int need = a.kth + 1;
-        if (over) Rnew = 0.75 * R;
-        else if (c < need) Rnew = 1.5 * R;
-        else {
-            // (k+1)-th smallest r^2 by ascending distinct values with multiplicity
-            double cur = -1.0;
-            int taken = 0;
-            double v = 0.0;
-            for (;;) {
-                double nxt = INFINITY;
-                int mult = 0;
-                for (int k = 0; k < cc; k++) {
-                    double r2 = cand[(int64_t)k * m + t];
-                    if (r2 > cur) {
-                        if (r2 < nxt) { nxt = r2; mult = 1; }
-                        else if (r2 == nxt) mult++;
-                    }
+        } else if (j >= 0) {
+            // (kth+1)-th smallest squared distance: ascending distinct values with multiplicity
+            const int need = a.kth + 1;
+            if (cj < need) {
+                Rnew = 1.5 * Rj;
+            } else {
+                double cur = -1.0, v = 0.0;
+                int taken = 0;
+                for (;;) {
+                    double mn = INFINITY;
+#pragma unroll
+                    for (int u = 0; u < 16; u++)
+                        if (r[u] > cur && r[u] < mn) mn = r[u];
+                    for (int off = 8; off >= 1; off >>= 1) mn = fmin(mn, __shfl_xor(mn, off, 16));
+                    int mult = 0;
+#pragma unroll
+                    for (int u = 0; u < 16; u++) mult += (r[u] == mn);
+                    for (int off = 8; off >= 1; off >>= 1) mult += __shfl_xor(mult, off, 16);
+                    if (taken + mult >= need) { v = mn; break; }
+                    taken += mult;
+                    cur = mn;
                 }
-                if (taken + mult >= need) { v = nxt; break; }
-                taken += mult;
-                cur = nxt;
+                hnew = __builtin_sqrt(v) * 0.5;
+                st = 1;
             }
-            hnew = __builtin_sqrt(v) * 0.5;
-            st = 1;
+        }
+        // hand each sub-group's result to its target lane
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const double hq = __shfl(hnew, 16 * q), Rq = __shfl(Rnew, 16 * q);
+            const int sq = __shfl(st, 16 * q);
+            if (js[q] >= 0 && lane == js[q]) {
+                h_res = hq;
+                R_res = Rq;
+                st_res = sq;
+            }
         }
     }
-    a.h[t] = hnew;
-    a.R[t] = Rnew;
-    a.state[t] = st;
-    if (st == 0) atomicAdd(&a.pending[0], 1ull);
-    if (st == 2) atomicAdd(&a.pending[1], 1ull);
+    if (!valid) return;
+    a.h[t] = h_res;
+    a.R[t] = R_res;
+    a.state[t] = st_res;
+    if (st_res == 0) atomicAdd(&a.pending[0], 1ull);
+    if (st_res == 2) atomicAdd(&a.pending[1], 1ull);
 }
 
 // mode 1: the reference's count stepping, exactly (libsx.py:804-845): each
@@ -255,35 +342,57 @@ __global__ void __launch_bounds__(256) k_hsolve_init(int64_t m, const double *__
 }
 
 // ---------------------------------------------------------------- CSR lists
-// S_i = { j : r_ij^2 < (h_i + h_j)^2 }, self included, rows ascending in
-// sorted index (the pre-order walk visits buckets in index order).
+// S_i = { j : r_ij^2 < (h_i + h_j)^2 }, self included.  A node is skipped for
+// the whole group iff boxdist(group box, node box)^2 >= (hmax_group +
+// hmax_node)^2, which by monotone rounding cannot drop a member.  Row order is
+// the (deterministic) order in which the walk stages buckets.
 template <bool FILL>
-__global__ void __launch_bounds__(256) k_ngb(int64_t n, const double *__restrict__ x, const double *__restrict__ y,
-                                             const double *__restrict__ z, const double *__restrict__ h,
-                                             const WNode *__restrict__ nodes, int bucket,
-                                             int32_t *__restrict__ cnt, const int64_t *__restrict__ off,
-                                             int32_t *__restrict__ idx)
+__global__ void __launch_bounds__(64) k_ngb(const int32_t *__restrict__ groups, const double *__restrict__ x,
+                                            const double *__restrict__ y, const double *__restrict__ z,
+                                            const double *__restrict__ h, const WNode *__restrict__ nodes, int bucket,
+                                            int32_t *__restrict__ cnt, const int64_t *__restrict__ off,
+                                            int32_t *__restrict__ idx, unsigned long long *__restrict__ overflow)
 {
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    bool valid = i < n;
+    __shared__ WalkLDS L;
+    __shared__ double sx[WSRC], sy[WSRC], sz[WSRC], sh[WSRC];
+    __shared__ int sj[WSRC];
+    const WNode *g = nodes + groups[blockIdx.x];
+    const int64_t first = g->first;
+    const int ng = g->count;
+    const int lane = threadIdx.x;
+    const int64_t i = first + lane;
+    const bool valid = lane < ng;
     double xi = 0, yi = 0, zi = 0, hi = 0;
     if (valid) { xi = x[i]; yi = y[i]; zi = z[i]; hi = h[i]; }
-    int64_t k = (FILL && valid) ? off[i] : 0;
+    const double gb[6] = {g->lx, g->ly, g->lz, g->hx, g->hy, g->hz};
+    const double hg = g->hmax;
+    int64_t o = (FILL && valid) ? off[i] : 0;
     int c = 0;
-    coop_walk(
-        nodes, valid, bucket,
+    const bool ovf = group_walk(
+        nodes, bucket, L,
         [&](const WNode *nd) {
-            double t = hi + nd->hmax;
-            return box_dist2(xi, yi, zi, nd) >= t * t;
+            double t = hg + nd->hmax;
+            return box_box_dist2(gb, nd) >= t * t;
         },
-        [&](int j) {
-            double dx = xi - x[j], dy = yi - y[j], dz = zi - z[j];
-            double r2 = dx * dx + dy * dy + dz * dz, t = hi + h[j];
-            if (r2 < t * t) {
-                if (FILL) idx[k + c] = j;
-                c++;
-            }
+        [&](int q, int j) {
+            sx[q] = x[j];
+            sy[q] = y[j];
+            sz[q] = z[j];
+            sh[q] = h[j];
+            sj[q] = j;
+        },
+        [&](int n) {
+            if (valid)
+                for (int k = 0; k < n; k++) {
+                    double dx = xi - sx[k], dy = yi - sy[k], dz = zi - sz[k];
+                    double r2 = dx * dx + dy * dy + dz * dz, t = hi + sh[k];
+                    if (r2 < t * t) {
+                        if (FILL) idx[o + c] = sj[k];
+                        c++;
+                    }
+                }
         });
+    if (ovf && lane == 0) atomicAdd(overflow, 1ull);
     if (valid && !FILL) cnt[i] = c;
 }
 
@@ -298,11 +407,14 @@ hipError_t hsolve(hipStream_t st, const HSolveParams &p, int max_rounds, int64_t
     a.n_low = p.n_low; a.n_up = p.n_up; a.max_steps = p.max_steps; a.kth = p.kth;
     a.cap = p.cap; a.cand = p.cand; a.R = p.R; a.h = p.h_out; a.steps = p.steps; a.state = p.state;
     a.pending = p.counters;
+    a.groups = p.groups;
+    a.gnodes = p.nodes;
     if (p.m <= 0) { *n_unconverged = 0; return hipSuccess; }
+    if (p.bucket > 64) return hipErrorInvalidValue;
     k_hsolve_init<<<sph_blocks(p.m, 256), 256, 0, st>>>(p.m, p.h_in, p.r_fac, p.R, p.h_out, p.steps, p.state);
-    unsigned long long hc[2] = {0, 0};
+    unsigned long long hc[3] = {0, 0, 0};
     if (p.mode == 1) {
-        fill<unsigned long long>(st, 2, p.counters, 0ull);
+        fill<unsigned long long>(st, 3, p.counters, 0ull);
         k_hstep<<<sph_blocks(p.m, 256), 256, 0, st>>>(a);
         hipError_t e = hipMemcpyAsync(hc, p.counters, sizeof(hc), hipMemcpyDeviceToHost, st);
         if (e != hipSuccess) return e;
@@ -312,40 +424,51 @@ hipError_t hsolve(hipStream_t st, const HSolveParams &p, int max_rounds, int64_t
         *n_unconverged = 0;
         return hipGetLastError();
     }
+    const int64_t nblk = p.groups ? p.ngroups : (p.m + 63) / 64;
     for (int r = 0; r < max_rounds; r++) {
-        fill<unsigned long long>(st, 2, p.counters, 0ull);
-        k_hsolve<<<sph_blocks(p.m, 256), 256, 0, st>>>(a);
+        fill<unsigned long long>(st, 3, p.counters, 0ull);
+        k_hsolve<<<(unsigned)nblk, 64, 0, st>>>(a);
         hipError_t e = hipMemcpyAsync(hc, p.counters, sizeof(hc), hipMemcpyDeviceToHost, st);
         if (e != hipSuccess) return e;
         e = hipStreamSynchronize(st);
         if (e != hipSuccess) return e;
+        if (hc[2]) return hipErrorIllegalState;  // walk stack overflow (cannot happen for valid trees)
         if (p.unconverged_total) *p.unconverged_total += (int64_t)hc[1];
+        if (p.rounds) *p.rounds = r + 1;
         if (hc[0] == 0) { *n_unconverged = 0; return hipGetLastError(); }
     }
     *n_unconverged = (int64_t)hc[0];
     return hipGetLastError();
 }
 
-hipError_t neighbours(hipStream_t st, int64_t n, const double *x, const double *y, const double *z,
-                      const double *h, const WNode *nodes, int bucket, int32_t *cnt, int64_t *off,
-                      int64_t *scan_tmp, int32_t *idx, int64_t idx_cap, int64_t *total)
+hipError_t neighbours(hipStream_t st, int64_t n, int64_t ngroups, const int32_t *groups, const double *x,
+                      const double *y, const double *z, const double *h, const WNode *nodes, int bucket, int32_t *cnt,
+                      int64_t *off, int64_t *scan_tmp, unsigned long long *overflow, int32_t *idx, int64_t idx_cap,
+                      int64_t *total)
 {
-    k_ngb<false><<<sph_blocks(n, 256), 256, 0, st>>>(n, x, y, z, h, nodes, bucket, cnt, nullptr, nullptr);
+    if (bucket > 64) return hipErrorInvalidValue;
+    fill<unsigned long long>(st, 1, overflow, 0ull);
+    k_ngb<false><<<(unsigned)ngroups, 64, 0, st>>>(groups, x, y, z, h, nodes, bucket, cnt, nullptr, nullptr, overflow);
     hipError_t e = scan_counts(st, n, cnt, off, scan_tmp);
     if (e != hipSuccess) return e;
+    unsigned long long hov = 0;
     e = hipMemcpyAsync(total, off + n, sizeof(int64_t), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(&hov, overflow, sizeof(hov), hipMemcpyDeviceToHost, st);
     if (e != hipSuccess) return e;
     e = hipStreamSynchronize(st);
     if (e != hipSuccess) return e;
+    if (hov) return hipErrorIllegalState;
     if (*total > idx_cap || idx == nullptr) return hipSuccess;  // caller grows idx and calls neighbours_fill
-    k_ngb<true><<<sph_blocks(n, 256), 256, 0, st>>>(n, x, y, z, h, nodes, bucket, nullptr, off, idx);
+    k_ngb<true><<<(unsigned)ngroups, 64, 0, st>>>(groups, x, y, z, h, nodes, bucket, nullptr, off, idx, overflow);
     return hipGetLastError();
 }
 
-hipError_t neighbours_fill(hipStream_t st, int64_t n, const double *x, const double *y, const double *z,
-                           const double *h, const WNode *nodes, int bucket, const int64_t *off, int32_t *idx)
+hipError_t neighbours_fill(hipStream_t st, int64_t ngroups, const int32_t *groups, const double *x, const double *y,
+                           const double *z, const double *h, const WNode *nodes, int bucket, const int64_t *off,
+                           int32_t *idx, unsigned long long *overflow)
 {
-    k_ngb<true><<<sph_blocks(n, 256), 256, 0, st>>>(n, x, y, z, h, nodes, bucket, nullptr, off, idx);
+    k_ngb<true><<<(unsigned)ngroups, 64, 0, st>>>(groups, x, y, z, h, nodes, bucket, nullptr, off, idx, overflow);
     return hipGetLastError();
 }
 
