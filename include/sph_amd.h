@@ -111,6 +111,32 @@ int sph_gravity(sph_ctx *ctx, int64_t n, const double *pos, const double *m, con
 /* out5 = [KE, px, py, pz, M] (FP64).  Replaces libsx.py:571 getKE. */
 int sph_kinetics(sph_ctx *ctx, int64_t n, const double *vel, const double *m, double *out5);
 
+/* ---- cooling (Particles_in_BH_Tree_III/hcool_with_unit_tests_Here/photolibs3.py and
+ *      CR/v4t.h:1768-1884 doCloudyHCooling) ----------------------------------
+ * Arrays are elementwise in caller order (cgs units).  J21: amplitude of the
+ * J_nu ~ nu^-1 UV field in 1e-21 erg/s/cm^2/Hz/sr (the reference's fixed value
+ * is 1e-7); Z: metallicity in solar units (0 = the reference's primordial
+ * gas).  A NULL array argument takes the scalar after it. */
+
+/* coolingHeatingRates(T, nHcgs), photolibs3.py:332: heating and cooling per nH^2. */
+int sph_cooling_rates(sph_ctx *ctx, int64_t n, const double *T, const double *nH, const double *J21, double J21c,
+                      const double *Z, double Zc, double *heat_out, double *lambda_out);
+/* what = 0: convert_u_to_temp_h(u, nHcgs, XH), photolibs3.py:379;
+ * what = 1: coolingRateFromU_h(u, nHcgs, XH), photolibs3.py:470 (Gamma - Lambda). */
+int sph_cooling_from_u(sph_ctx *ctx, int64_t n, int what, const double *u, const double *nH, double XH,
+                       const double *J21, double J21c, const double *Z, double Zc, double *out);
+/* DoCooling_h(rho, u_old, dt, XH), photolibs3.py:489: implicit update; u_out = -1 where the
+ * downward bracket reached u_floor (the reference has no floor). */
+int sph_do_cooling(sph_ctx *ctx, int64_t n, const double *rho, const double *u_old, const double *dt, double dtc,
+                   double XH, const double *J21, double J21c, const double *Z, double Zc, double u_floor,
+                   double *u_out);
+/* Build the 4D table of log10 Lambda, log10 Gamma over (log10 nH, log10 T, log10 Z, log10 J21):
+ * lo[a], hi[a], npts[a] per axis.  Replaces the absent CR table file HCoolMu.bin (CR/v3t.cu:45). */
+int sph_cooling_table_build(sph_ctx *ctx, const double *lo, const double *hi, const int *npts);
+/* 4D multilinear lookup (inputs log10, clamped to the table), CR/v4t.h:1681-1884. */
+int sph_cooling_table_lookup(sph_ctx *ctx, int64_t n, const double *lognH, const double *logT, const double *logZ,
+                             const double *logJ21, double *logL_out, double *logG_out);
+
 /* ---- device-resident time loop (sph_serial_v2.py:148-218 on the GPU) ---- */
 typedef struct sph_step_params {
     double dt;          /* fixed step */

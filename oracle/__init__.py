@@ -51,6 +51,13 @@ def lib():
         L.or_ke.restype = ctypes.c_double
         L.or_pe.restype = ctypes.c_double
         L.or_h_newton.restype = ctypes.c_int64
+        for f in ("or_cool_rb95", "or_cool_u_to_T", "or_cool_rate_from_u", "or_do_cooling"):
+            getattr(L, f).restype = ctypes.c_double
+        for f in ("or_cool_rb95",):
+            getattr(L, f).argtypes = [ctypes.c_double]
+        L.or_cool_u_to_T.argtypes = [ctypes.c_double] * 4
+        L.or_cool_rate_from_u.argtypes = [ctypes.c_double] * 5
+        L.or_do_cooling.argtypes = [ctypes.c_double] * 7
         _LIB = L
     return _LIB
 
@@ -321,3 +328,48 @@ def dudt_csr(pos, v, rho, P, c, h, m, alpha, off, idx, divV=None, curlV=None):
                       _p(_f64(h)), _p(_f64(m)), _p(dv), _p(cv), ctypes.c_double(alpha), ctypes.c_int(int(bal)),
                       _p(off), _p(idx), _p(out))
     return out
+
+
+# ------------------------------------------------------------------ cooling
+J21_REFERENCE = 1e-7  # the reference's fixed radiation-field amplitude (photolibs3.py:233)
+
+
+def cool_rates(T):
+    out = np.zeros(7)
+    lib().or_cool_rates(ctypes.c_double(T), _p(out))
+    return out
+
+
+def cool_radfield(J21=J21_REFERENCE):
+    out = np.zeros(6)
+    lib().or_cool_radfield(ctypes.c_double(J21), _p(out))
+    return out
+
+
+def cool_abund(T, nH, gJ):
+    out = np.zeros(6)
+    lib().or_cool_abund(ctypes.c_double(T), ctypes.c_double(nH), _p(_f64(gJ)), _p(out))
+    return out
+
+
+def cool_heat_lambda(T, nH, J21=J21_REFERENCE, Z=0.0):
+    out = np.zeros(2)
+    lib().or_cool_heat_lambda(ctypes.c_double(T), ctypes.c_double(nH), ctypes.c_double(J21), ctypes.c_double(Z),
+                              _p(out))
+    return out
+
+
+def cool_rb95(T):
+    return lib().or_cool_rb95(float(T))
+
+
+def cool_u_to_T(u, nH, XH=0.76, J21=J21_REFERENCE):
+    return lib().or_cool_u_to_T(float(u), float(nH), float(XH), float(J21))
+
+
+def cool_rate_from_u(u, nH, XH=0.76, J21=J21_REFERENCE, Z=0.0):
+    return lib().or_cool_rate_from_u(float(u), float(nH), float(XH), float(J21), float(Z))
+
+
+def do_cooling(rho, u_old, dt, XH=0.76, J21=J21_REFERENCE, Z=0.0, u_floor=1e8):
+    return lib().or_do_cooling(float(rho), float(u_old), float(dt), float(XH), float(J21), float(Z), float(u_floor))

@@ -59,6 +59,13 @@ _SIGS = {
     "sph_gravity": ([_vp, _i64, _d, _d, _d, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                      ctypes.c_int, _d, _i64p], ctypes.c_int),
     "sph_kinetics": ([_vp, _i64, _d, _d, _d], ctypes.c_int),
+    "sph_cooling_rates": ([_vp, _i64, _d, _d, _d, ctypes.c_double, _d, ctypes.c_double, _d, _d], ctypes.c_int),
+    "sph_cooling_from_u": ([_vp, _i64, ctypes.c_int, _d, _d, ctypes.c_double, _d, ctypes.c_double, _d,
+                            ctypes.c_double, _d], ctypes.c_int),
+    "sph_do_cooling": ([_vp, _i64, _d, _d, _d, ctypes.c_double, ctypes.c_double, _d, ctypes.c_double, _d,
+                        ctypes.c_double, ctypes.c_double, _d], ctypes.c_int),
+    "sph_cooling_table_build": ([_vp, _d, _d, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "sph_cooling_table_lookup": ([_vp, _i64, _d, _d, _d, _d, _d, _d], ctypes.c_int),
     "sph_state_set": ([_vp, _i64, _d, _d, _d, _d, _d], ctypes.c_int),
     "sph_state_get": ([_vp, _d, _d, _d, _d, _d, _d], ctypes.c_int),
     "sph_prime": ([_vp, ctypes.POINTER(StepParams)], ctypes.c_int),

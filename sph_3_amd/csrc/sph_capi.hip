@@ -11,6 +11,7 @@
 
 #include "../../include/sph_amd.h"
 #include "sph_common.h"
+#include "sph_cool.h"
 #include "sph_grav.h"
 #include "sph_hydro.h"
 #include "sph_integrate.h"
@@ -77,6 +78,11 @@ struct sph_ctx {
     Buf cand, hR, hsteps, hstate, hcount, hout, hin, tx, ty, tz;
     // staging and reductions
     Buf stage, kpart, kout;
+    // cooling: field integrals, table, elementwise scratch
+    Buf rf1, ctabL, ctabG, cin[6], cout[2];
+    bool rf1_ok = false;
+    CoolAxes cax{};
+    bool ctab_ok = false;
 
     // caches of the stateless entry points (host copies of the last inputs)
     std::vector<double> cpos, ch, cm, ceps;
@@ -394,6 +400,10 @@ extern "C" int sph_ctx_destroy(sph_ctx *ctx)
                   &ctx->hstate, &ctx->hcount, &ctx->hout, &ctx->hin, &ctx->tx, &ctx->ty, &ctx->tz,
                   &ctx->stage, &ctx->kpart, &ctx->kout};
     for (Buf *b : all)
+        if (b->p) (void)hipFree(b->p);
+    Buf *cool[] = {&ctx->rf1, &ctx->ctabL, &ctx->ctabG, &ctx->cin[0], &ctx->cin[1], &ctx->cin[2], &ctx->cin[3],
+                   &ctx->cin[4], &ctx->cin[5], &ctx->cout[0], &ctx->cout[1]};
+    for (Buf *b : cool)
         if (b->p) (void)hipFree(b->p);
     for (int k = 0; k < 2 * SPH_PH_COUNT; k++) (void)hipEventDestroy(ctx->ev[k]);
     (void)hipStreamDestroy(ctx->st);
@@ -924,5 +934,154 @@ extern "C" int sph_reset_stats(sph_ctx *ctx)
     int64_t n = ctx->stats.n;
     std::memset(&ctx->stats, 0, sizeof(ctx->stats));
     ctx->stats.n = n;
+    return SPH_OK;
+}
+
+// =========================================================== cooling
+namespace {
+
+int cool_prep(sph_ctx *ctx, int64_t n)
+{
+    if (!ctx) return SPH_ERR_ARG;
+    if (n < 1) return fail(ctx, SPH_ERR_ARG, "n must be >= 1");
+    HIPCK(hipSetDevice(ctx->device));
+    if (!ctx->rf1_ok) {
+        CK(ensure(ctx, ctx->rf1, 8 * 8));
+        HIPCK(sph::cool_radfield(ctx->st, D(rf1)));
+        ctx->rf1_ok = true;
+    }
+    for (int k = 0; k < 6; k++) CK(ensure(ctx, ctx->cin[k], (size_t)n * 8));
+    for (int k = 0; k < 2; k++) CK(ensure(ctx, ctx->cout[k], (size_t)n * 8));
+    return SPH_OK;
+}
+
+// upload an optional array into cin[k]; returns the device pointer or null
+int up(sph_ctx *ctx, int k, const double *src, int64_t n, const double **dptr)
+{
+    *dptr = nullptr;
+    if (!src) return SPH_OK;
+    HIPCK(hipMemcpyAsync(ctx->cin[k].p, src, sizeof(double) * n, hipMemcpyHostToDevice, ctx->st));
+    *dptr = (const double *)ctx->cin[k].p;
+    return SPH_OK;
+}
+
+int down(sph_ctx *ctx, int k, double *dst, int64_t n)
+{
+    HIPCK(hipMemcpyAsync(dst, ctx->cout[k].p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->st));
+    return SPH_OK;
+}
+
+}  // namespace
+
+extern "C" int sph_cooling_rates(sph_ctx *ctx, int64_t n, const double *T, const double *nH, const double *J21,
+                                 double J21c, const double *Z, double Zc, double *heat_out, double *lambda_out)
+{
+    CK(cool_prep(ctx, n));
+    if (!T || !nH || !heat_out || !lambda_out) return fail(ctx, SPH_ERR_ARG, "null argument");
+    const double *dT, *dn, *dJ, *dZ;
+    CK(up(ctx, 0, T, n, &dT));
+    CK(up(ctx, 1, nH, n, &dn));
+    CK(up(ctx, 2, J21, n, &dJ));
+    CK(up(ctx, 3, Z, n, &dZ));
+    HIPCK(sph::cool_rates_exact(ctx->st, n, dT, dn, dJ, J21c, dZ, Zc, D(rf1), D(cout[0]), D(cout[1])));
+    CK(down(ctx, 0, heat_out, n));
+    CK(down(ctx, 1, lambda_out, n));
+    HIPCK(hipStreamSynchronize(ctx->st));
+    return SPH_OK;
+}
+
+extern "C" int sph_cooling_from_u(sph_ctx *ctx, int64_t n, int what, const double *u, const double *nH, double XH,
+                                  const double *J21, double J21c, const double *Z, double Zc, double *out)
+{
+    CK(cool_prep(ctx, n));
+    if (!u || !nH || !out) return fail(ctx, SPH_ERR_ARG, "null argument");
+    if (what != 0 && what != 1) return fail(ctx, SPH_ERR_ARG, "what must be 0 (T) or 1 (net rate)");
+    const double *du, *dn, *dJ, *dZ;
+    CK(up(ctx, 0, u, n, &du));
+    CK(up(ctx, 1, nH, n, &dn));
+    CK(up(ctx, 2, J21, n, &dJ));
+    CK(up(ctx, 3, Z, n, &dZ));
+    HIPCK(sph::cool_u_exact(ctx->st, n, what, du, dn, XH, dJ, J21c, dZ, Zc, D(rf1), D(cout[0])));
+    CK(down(ctx, 0, out, n));
+    HIPCK(hipStreamSynchronize(ctx->st));
+    return SPH_OK;
+}
+
+extern "C" int sph_do_cooling(sph_ctx *ctx, int64_t n, const double *rho, const double *u_old, const double *dt,
+                              double dtc, double XH, const double *J21, double J21c, const double *Z, double Zc,
+                              double u_floor, double *u_out)
+{
+    CK(cool_prep(ctx, n));
+    if (!rho || !u_old || !u_out) return fail(ctx, SPH_ERR_ARG, "null argument");
+    const double *dr, *du, *dd, *dJ, *dZ;
+    CK(up(ctx, 0, rho, n, &dr));
+    CK(up(ctx, 1, u_old, n, &du));
+    CK(up(ctx, 2, dt, n, &dd));
+    CK(up(ctx, 3, J21, n, &dJ));
+    CK(up(ctx, 4, Z, n, &dZ));
+    HIPCK(sph::do_cooling_exact(ctx->st, n, dr, du, dd, dtc, XH, dJ, J21c, dZ, Zc, u_floor, D(rf1), D(cout[0])));
+    CK(down(ctx, 0, u_out, n));
+    HIPCK(hipStreamSynchronize(ctx->st));
+    return SPH_OK;
+}
+
+extern "C" int sph_cooling_table_build(sph_ctx *ctx, const double *lo, const double *hi, const int *npts)
+{
+    CK(cool_prep(ctx, 1));
+    if (!lo || !hi || !npts) return fail(ctx, SPH_ERR_ARG, "null argument");
+    CoolAxes ax{};
+    int64_t total = 1;
+    for (int a = 0; a < 4; a++) {
+        if (npts[a] < 2 || !(hi[a] > lo[a])) return fail(ctx, SPH_ERR_ARG, "each axis needs npts >= 2 and hi > lo");
+        ax.lo[a] = lo[a];
+        ax.n[a] = npts[a];
+        ax.d[a] = (hi[a] - lo[a]) / (npts[a] - 1);
+        total *= npts[a];
+    }
+    if (total > ((int64_t)1 << 28)) return fail(ctx, SPH_ERR_ARG, "cooling table too large");
+    CK(ensure(ctx, ctx->ctabL, (size_t)total * 4));
+    CK(ensure(ctx, ctx->ctabG, (size_t)total * 4));
+    HIPCK(sph::cool_table_build(ctx->st, ax, D(rf1), P_<float>(ctx->ctabL), P_<float>(ctx->ctabG)));
+    HIPCK(hipStreamSynchronize(ctx->st));
+    ctx->cax = ax;
+    ctx->ctab_ok = true;
+    return SPH_OK;
+}
+
+namespace {
+__global__ void k_d2f(int64_t n, const double *__restrict__ a, float *__restrict__ b)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+__global__ void k_f2d(int64_t n, const float *__restrict__ a, double *__restrict__ b)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (double)a[i];
+}
+}  // namespace
+
+extern "C" int sph_cooling_table_lookup(sph_ctx *ctx, int64_t n, const double *lognH, const double *logT,
+                                        const double *logZ, const double *logJ21, double *logL_out, double *logG_out)
+{
+    CK(cool_prep(ctx, n));
+    if (!ctx->ctab_ok) return fail(ctx, SPH_ERR_STATE, "no cooling table (call sph_cooling_table_build)");
+    if (!lognH || !logT || !logZ || !logJ21 || !logL_out || !logG_out) return fail(ctx, SPH_ERR_ARG, "null argument");
+    const double *src[4] = {lognH, logT, logZ, logJ21};
+    for (int k = 0; k < 4; k++) {
+        HIPCK(hipMemcpyAsync(ctx->cin[k].p, src[k], sizeof(double) * n, hipMemcpyHostToDevice, ctx->st));
+        // float copy in the upper half of the same buffer
+        k_d2f<<<sph_blocks(n, 256), 256, 0, ctx->st>>>(n, D(cin[k]), (float *)ctx->cin[4 + (k >> 1)].p + (k & 1) * n);
+    }
+    float *lf = (float *)ctx->cin[4].p, *lg = (float *)ctx->cin[5].p;
+    float *oL = (float *)ctx->cout[0].p, *oG = oL + n;
+    (void)lg;
+    HIPCK(sph::cool_table_lookup(ctx->st, n, ctx->cax, P_<float>(ctx->ctabL), P_<float>(ctx->ctabG), lf, lf + n,
+                                 (float *)ctx->cin[5].p, (float *)ctx->cin[5].p + n, oL, oG));
+    k_f2d<<<sph_blocks(n, 256), 256, 0, ctx->st>>>(n, oL, D(cin[0]));
+    k_f2d<<<sph_blocks(n, 256), 256, 0, ctx->st>>>(n, oG, D(cin[1]));
+    HIPCK(hipMemcpyAsync(logL_out, ctx->cin[0].p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->st));
+    HIPCK(hipMemcpyAsync(logG_out, ctx->cin[1].p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->st));
+    HIPCK(hipStreamSynchronize(ctx->st));
     return SPH_OK;
 }
