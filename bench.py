@@ -116,8 +116,8 @@ def main():
     ap.add_argument("--no-fp64-form", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding is a "
-              "design in DESIGN.md, not a measured path", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); sharding the particle "
+              "set over several GPUs is designed but not built or measured", file=sys.stderr)
         sys.exit(2)
 
     from sph_3_amd import ic

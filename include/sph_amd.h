@@ -137,6 +137,59 @@ int sph_cooling_table_build(sph_ctx *ctx, const double *lo, const double *hi, co
 int sph_cooling_table_lookup(sph_ctx *ctx, int64_t n, const double *lognH, const double *logT, const double *logZ,
                              const double *logJ21, double *logL_out, double *logG_out);
 
+/* ---- learned cooling surrogate ----------------------------------------------
+ * An MLP 4 -> 128 -> 128 -> 128 -> 1 (SiLU) from the normalised
+ * (log10 nH, log10 T, log10 Z, log10 J21) to the normalised log10 Lambda, BF16
+ * weights and activations with FP32 accumulation on the gfx950 matrix cores.
+ * Its design point is the reference's offline Keras surrogate
+ * Particles_in_BH_Tree_III/CLOUDY_replace_CHIMES/HCooling_Here/ML/prog2.py:24-52
+ * (4 -> 128 -> 64 -> 1, Adam, MSE); the reference has no device inference path,
+ * so these entry points have no one-to-one counterpart there.  The network's
+ * box is the cooling table's box; the table (sph_cooling_table_build) is what
+ * its output falls back to and what its error map is measured against.
+ *
+ * Parameters: SPH_MLP_NPARAM floats, flat W1[128][4], b1[128], W2[128][128],
+ * b2, W3[128][128], b3, W4[128], b4 (W[out][in]).  norm[10] = box lo[4],
+ * hi[4], then mu, sigma of log10 Lambda: x_norm = 2 (v - lo)/(hi - lo) - 1,
+ * log10 Lambda = mu + sigma * y_norm. */
+#define SPH_MLP_NPARAM 33793
+
+/* Seeded initialisation over the current table's box (mu, sigma from the table). */
+int sph_mlp_init(sph_ctx *ctx, uint64_t seed);
+/* Load / read parameters and normalisation (loading clears the optimiser state and the error map). */
+int sph_mlp_set_params(sph_ctx *ctx, const float *params, const double *norm);
+int sph_mlp_get_params(sph_ctx *ctx, float *params, double *norm);
+/* Network output y_norm[n] for normalised inputs x_norm[n][4]. */
+int sph_mlp_forward(sph_ctx *ctx, int64_t n, const float *x_norm, float *y_norm);
+/* Gradient (SPH_MLP_NPARAM floats) of mean_i (y_i - t_i)^2 over the n samples, and that mean. */
+int sph_mlp_gradient(sph_ctx *ctx, int64_t n, const float *x_norm, const float *t_norm, float *grad, double *loss);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected) on the gradient of the n samples. */
+int sph_mlp_train_batch(sph_ctx *ctx, int64_t n, const float *x_norm, const float *t_norm, double lr, double *loss);
+/* Train on n_samples points drawn uniformly in the box with exact FP64 log10 Lambda targets:
+ * `epochs` passes of minibatches of `batch`, Adam with cosine decay from lr to lr/100.
+ * loss_out[epochs] (may be NULL) receives the mean normalised squared error of each pass. */
+int sph_mlp_train(sph_ctx *ctx, int64_t n_samples, int epochs, int batch, double lr, uint64_t seed, double *loss_out);
+/* Error map: the box split into cells[0]*..*cells[3] cells (row-major, last axis fastest),
+ * `per` stratified samples each; per cell, in dex: emap_table = max |MLP - table|,
+ * emap_exact = max |MLP - exact|, emap_table_exact = max |table - exact| (any may be NULL).
+ * Required before sph_mlp_cooling. */
+int sph_mlp_calibrate(sph_ctx *ctx, const int *cells, int per, uint64_t seed, float *emap_table, float *emap_exact,
+                      float *emap_table_exact);
+/* Gated log10 Lambda: the network where the input lies in the box and passes the cell's gate,
+ * the table's 4D interpolation elsewhere.  ref = SPH_MLP_REF_EXACT (the Python API's default):
+ * fall back where emap_exact > tol [dex] and emap_exact > emap_table_exact (the table measured
+ * better there); ref = SPH_MLP_REF_TABLE: fall back where emap_table > tol.  fallback_out[n]
+ * (may be NULL) flags the table results; *n_fallback counts them.  Same role as
+ * sph_cooling_table_lookup's logL. */
+enum sph_mlp_ref { SPH_MLP_REF_EXACT = 0, SPH_MLP_REF_TABLE = 1 };
+int sph_mlp_cooling(sph_ctx *ctx, int64_t n, const double *lognH, const double *logT, const double *logZ,
+                    const double *logJ21, double tol, int ref, double *logL_out, int32_t *fallback_out,
+                    int64_t *n_fallback);
+/* Device timing on n resident points uniform in the box, averaged over reps launches:
+ * out[0] gated inference ms (ref SPH_MLP_REF_EXACT), out[1] table lookup (log Lambda only) ms, out[2] ungated
+ * network ms, out[3] fallback fraction at tol. */
+int sph_mlp_timing(sph_ctx *ctx, int64_t n, int reps, uint64_t seed, double tol, double *out);
+
 /* ---- device-resident time loop (sph_serial_v2.py:148-218 on the GPU) ---- */
 typedef struct sph_step_params {
     double dt;          /* fixed step */

@@ -14,8 +14,8 @@ The C restatement lives in ``sph_oracle.c`` (built by ``oracle/Makefile`` into
   binary radix tree, symmetric CSR neighbour sets, Newton-Raphson smoothing
   length, tree gravity with a per-particle opening test, CSR-restricted sums.
 
-Parity pins for every function are listed in DESIGN.md's oracle table and
-live in ``tests/test_oracle_*.py``.
+Parity pins for every function live in ``tests/test_oracle*.py``; the
+cooling-surrogate network's oracle is ``oracle/mlp.py`` (numpy).
 """
 from __future__ import annotations
 

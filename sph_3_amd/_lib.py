@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsph_amd.so")
 
 SPH_OK, SPH_ERR_ARG, SPH_ERR_HIP, SPH_ERR_NOCONV, SPH_ERR_CAPACITY, SPH_ERR_STATE = range(6)
+MLP_NPARAM = 33793
 PREC = {"mixed": 0, "fp64": 1}
 SOFT = {"spline": 0, "plummer": 1}
 H_NEWTON, H_REF_COUNT, H_KTH = 0, 1, 2
@@ -23,6 +24,7 @@ PHASES = ["kick_drift", "order", "topology", "hsolve", "moments", "ngb", "densit
           "accsph", "kick", "dudt", "u_update"]
 
 _d = ctypes.POINTER(ctypes.c_double)
+_f = ctypes.POINTER(ctypes.c_float)
 _i64p = ctypes.POINTER(ctypes.c_int64)
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _i64 = ctypes.c_int64
@@ -66,6 +68,17 @@ _SIGS = {
                         ctypes.c_double, ctypes.c_double, _d], ctypes.c_int),
     "sph_cooling_table_build": ([_vp, _d, _d, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "sph_cooling_table_lookup": ([_vp, _i64, _d, _d, _d, _d, _d, _d], ctypes.c_int),
+    "sph_mlp_init": ([_vp, ctypes.c_uint64], ctypes.c_int),
+    "sph_mlp_set_params": ([_vp, _f, _d], ctypes.c_int),
+    "sph_mlp_get_params": ([_vp, _f, _d], ctypes.c_int),
+    "sph_mlp_forward": ([_vp, _i64, _f, _f], ctypes.c_int),
+    "sph_mlp_gradient": ([_vp, _i64, _f, _f, _f, _d], ctypes.c_int),
+    "sph_mlp_train_batch": ([_vp, _i64, _f, _f, ctypes.c_double, _d], ctypes.c_int),
+    "sph_mlp_train": ([_vp, _i64, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_uint64, _d], ctypes.c_int),
+    "sph_mlp_calibrate": ([_vp, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_uint64, _f, _f, _f],
+                          ctypes.c_int),
+    "sph_mlp_cooling": ([_vp, _i64, _d, _d, _d, _d, ctypes.c_double, ctypes.c_int, _d, _i32p, _i64p], ctypes.c_int),
+    "sph_mlp_timing": ([_vp, _i64, ctypes.c_int, ctypes.c_uint64, ctypes.c_double, _d], ctypes.c_int),
     "sph_state_set": ([_vp, _i64, _d, _d, _d, _d, _d], ctypes.c_int),
     "sph_state_get": ([_vp, _d, _d, _d, _d, _d, _d], ctypes.c_int),
     "sph_prime": ([_vp, ctypes.POINTER(StepParams)], ctypes.c_int),
@@ -110,6 +123,8 @@ def ptr(a):
         return None
     if a.dtype == np.float64:
         return a.ctypes.data_as(_d)
+    if a.dtype == np.float32:
+        return a.ctypes.data_as(_f)
     if a.dtype == np.int64:
         return a.ctypes.data_as(_i64p)
     if a.dtype == np.int32:

@@ -14,10 +14,10 @@ import ctypes
 
 import numpy as np
 
-from ._lib import context, f64, ptr
+from ._lib import MLP_NPARAM, Context, context, f64, ptr
 
 __all__ = ["coolingHeatingRates", "convert_u_to_temp_h", "coolingRateFromU_h", "DoCooling_h", "CoolingTable",
-           "J21_REFERENCE", "TABLE_RANGES"]
+           "CoolingSurrogate", "J21_REFERENCE", "TABLE_RANGES"]
 
 J21_REFERENCE = 1e-7
 # (lo, hi, points) per table axis: log10 nH, log10 T, log10 Z, log10 J21
@@ -99,7 +99,8 @@ class CoolingTable:
         lo = np.array([r[0] for r in ranges], np.float64)
         hi = np.array([r[1] for r in ranges], np.float64)
         npts = (ctypes.c_int * 4)(*[int(r[2]) for r in ranges])
-        self.ctx = context(device)
+        self.device = device
+        self.ctx = Context(device)  # one context (stream + buffers) per table
         c = self.ctx
         c.check(c.lib.sph_cooling_table_build(c.h, ptr(lo), ptr(hi), npts), "cooling table build")
         self.nbytes = 2 * 4 * int(np.prod([r[2] for r in ranges]))
@@ -112,3 +113,128 @@ class CoolingTable:
         c.check(c.lib.sph_cooling_table_lookup(c.h, n, ptr(a), ptr(b), ptr(z), ptr(u), ptr(L), ptr(G)),
                 "cooling table lookup")
         return L, G
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a.reshape(shape) if shape is not None else a
+
+
+class CoolingSurrogate:
+    """Learned log10 Lambda: an MLP 4 -> 128 -> 128 -> 128 -> 1 (SiLU) over
+    (log10 nH, log10 T, log10 Z, log10 J21), BF16 weights/activations with FP32
+    accumulation on the matrix cores, trained on the device by its own forward
+    and backward kernels on samples of the exact cooling function.
+
+    Design point: the reference's offline Keras surrogate
+    (Particles_in_BH_Tree_III/CLOUDY_replace_CHIMES/HCooling_Here/ML/prog2.py:24-52).
+    ``rates`` gates every input: outside the table's box, or in a cell of the
+    error map where the network's measured error exceeds ``tol`` [dex] and the
+    table measured better, the result is the table's interpolation.
+    ``calibrate`` measures both errors per cell against the exact FP64 function
+    the table is built from, and also the network's deviation from the table
+    (``ref="table"`` gates on that one alone)."""
+
+    NPARAM = MLP_NPARAM
+
+    def __init__(self, table: CoolingTable, seed: int = 0):
+        self.table = table
+        self.ctx = table.ctx
+        self.cells = None
+        c = self.ctx
+        c.check(c.lib.sph_mlp_init(c.h, int(seed)), "surrogate init")
+
+    # ---- parameters
+    def get_params(self):
+        p = np.empty(MLP_NPARAM, np.float32)
+        nm = np.empty(10)
+        c = self.ctx
+        c.check(c.lib.sph_mlp_get_params(c.h, ptr(p), ptr(nm)), "surrogate get_params")
+        return p, nm
+
+    def set_params(self, params, norm):
+        p = _f32(params)
+        nm = f64(norm)
+        if p.size != MLP_NPARAM or nm.size != 10:
+            raise ValueError(f"params must have {MLP_NPARAM} entries and norm 10")
+        c = self.ctx
+        c.check(c.lib.sph_mlp_set_params(c.h, ptr(p), ptr(nm)), "surrogate set_params")
+        self.cells = None
+
+    @property
+    def nbytes(self):
+        """Device bytes the inference reads: FP32 small layers + BF16 hidden weights."""
+        return 4 * (MLP_NPARAM - 2 * 128 * 128) + 2 * 2 * 128 * 128
+
+    # ---- raw network (normalised inputs / outputs)
+    def forward(self, x_norm):
+        x = _f32(x_norm).reshape(-1, 4)
+        y = np.empty(len(x), np.float32)
+        c = self.ctx
+        c.check(c.lib.sph_mlp_forward(c.h, len(x), ptr(x), ptr(y)), "surrogate forward")
+        return y
+
+    def gradient(self, x_norm, t_norm):
+        x = _f32(x_norm).reshape(-1, 4)
+        t = _f32(t_norm)
+        g = np.empty(MLP_NPARAM, np.float32)
+        loss = ctypes.c_double()
+        c = self.ctx
+        c.check(c.lib.sph_mlp_gradient(c.h, len(x), ptr(x), ptr(t), ptr(g), ctypes.byref(loss)), "surrogate gradient")
+        return g, loss.value
+
+    def train_batch(self, x_norm, t_norm, lr):
+        x = _f32(x_norm).reshape(-1, 4)
+        t = _f32(t_norm)
+        loss = ctypes.c_double()
+        c = self.ctx
+        c.check(c.lib.sph_mlp_train_batch(c.h, len(x), ptr(x), ptr(t), float(lr), ctypes.byref(loss)),
+                "surrogate train_batch")
+        self.cells = None
+        return loss.value
+
+    # ---- training and calibration
+    def train(self, n_samples=2_000_000, epochs=200, batch=4096, lr=3e-3, seed=1):
+        """Train on n_samples uniform in the box (exact FP64 targets); returns the per-epoch losses."""
+        losses = np.empty(int(epochs))
+        c = self.ctx
+        c.check(c.lib.sph_mlp_train(c.h, int(n_samples), int(epochs), int(batch), float(lr), int(seed), ptr(losses)),
+                "surrogate train")
+        self.cells = None
+        return losses
+
+    def calibrate(self, cells=(16, 24, 6, 8), per=16, seed=2):
+        """Build the error map; returns per-cell maxima [dex] of |MLP - table|,
+        |MLP - exact| and |table - exact|, each shaped ``cells``."""
+        cl = (ctypes.c_int * 4)(*[int(v) for v in cells])
+        ncell = int(np.prod(cells))
+        et, ex, te = (np.empty(ncell, np.float32) for _ in range(3))
+        c = self.ctx
+        c.check(c.lib.sph_mlp_calibrate(c.h, cl, int(per), int(seed), ptr(et), ptr(ex), ptr(te)),
+                "surrogate calibrate")
+        self.cells = tuple(int(v) for v in cells)
+        return et.reshape(cells), ex.reshape(cells), te.reshape(cells)
+
+    # ---- gated inference
+    def rates(self, lognH, logT, logZ, logJ21, tol=0.02, ref="exact"):
+        """(log10 Lambda, fallback mask, fallback count): the table's value wherever the
+        input leaves the box or its cell fails the gate (ref="exact": network error
+        > tol and > the table's error; ref="table": |network - table| > tol)."""
+        if ref not in ("exact", "table"):
+            raise ValueError("ref must be 'exact' or 'table'")
+        _, (a, b, z, u) = _vec(lognH, logT, logZ, logJ21)
+        n = len(a)
+        L = np.empty(n)
+        fb = np.empty(n, np.int32)
+        nf = ctypes.c_int64()
+        c = self.ctx
+        c.check(c.lib.sph_mlp_cooling(c.h, n, ptr(a), ptr(b), ptr(z), ptr(u), float(tol), 0 if ref == "exact" else 1,
+                                      ptr(L), ptr(fb), ctypes.byref(nf)), "surrogate rates")
+        return L, fb.astype(bool), int(nf.value)
+
+    def timing(self, n=1 << 22, reps=20, seed=3, tol=0.02):
+        """Device ms per launch: gated network, table lookup, bare network; and the fallback fraction."""
+        out = np.empty(4)
+        c = self.ctx
+        c.check(c.lib.sph_mlp_timing(c.h, int(n), int(reps), int(seed), float(tol), ptr(out)), "surrogate timing")
+        return dict(ms_gated=out[0], ms_table=out[1], ms_network=out[2], fallback_fraction=out[3])

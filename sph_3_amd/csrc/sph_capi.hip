@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -15,6 +16,7 @@
 #include "sph_grav.h"
 #include "sph_hydro.h"
 #include "sph_integrate.h"
+#include "sph_mlp.h"
 #include "sph_ngb.h"
 #include "sph_order.h"
 #include "sph_tree.h"
@@ -83,6 +85,12 @@ struct sph_ctx {
     bool rf1_ok = false;
     CoolAxes cax{};
     bool ctab_ok = false;
+    // cooling surrogate: parameters, Adam state, BF16 weight copies, samples, error map
+    Buf mP, mG, mM, mV, mW[4], mx, mxn, mt, my, mloss, memap, memap2, memap3, mgate, mcnt;
+    MLPNorm mnorm{};
+    bool mlp_ok = false, mlp_cal = false;
+    int mlp_step = 0;
+    int mcells[4] = {0, 0, 0, 0};
 
     // caches of the stateless entry points (host copies of the last inputs)
     std::vector<double> cpos, ch, cm, ceps;
@@ -401,8 +409,11 @@ extern "C" int sph_ctx_destroy(sph_ctx *ctx)
                   &ctx->stage, &ctx->kpart, &ctx->kout};
     for (Buf *b : all)
         if (b->p) (void)hipFree(b->p);
-    Buf *cool[] = {&ctx->rf1, &ctx->ctabL, &ctx->ctabG, &ctx->cin[0], &ctx->cin[1], &ctx->cin[2], &ctx->cin[3],
-                   &ctx->cin[4], &ctx->cin[5], &ctx->cout[0], &ctx->cout[1]};
+    Buf *cool[] = {&ctx->rf1,   &ctx->ctabL,  &ctx->ctabG,  &ctx->cin[0], &ctx->cin[1], &ctx->cin[2], &ctx->cin[3],
+                   &ctx->cin[4], &ctx->cin[5], &ctx->cout[0], &ctx->cout[1], &ctx->mP,     &ctx->mG,     &ctx->mM,
+                   &ctx->mV,     &ctx->mW[0],  &ctx->mW[1],  &ctx->mW[2],  &ctx->mW[3],  &ctx->mx,     &ctx->mxn,
+                   &ctx->mt,     &ctx->my,     &ctx->mloss,  &ctx->memap,  &ctx->memap2, &ctx->memap3,
+                   &ctx->mgate,  &ctx->mcnt};
     for (Buf *b : cool)
         if (b->p) (void)hipFree(b->p);
     for (int k = 0; k < 2 * SPH_PH_COUNT; k++) (void)hipEventDestroy(ctx->ev[k]);
@@ -1083,5 +1094,403 @@ extern "C" int sph_cooling_table_lookup(sph_ctx *ctx, int64_t n, const double *l
     HIPCK(hipMemcpyAsync(logL_out, ctx->cin[0].p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->st));
     HIPCK(hipMemcpyAsync(logG_out, ctx->cin[1].p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->st));
     HIPCK(hipStreamSynchronize(ctx->st));
+    return SPH_OK;
+}
+
+// =========================================================== cooling surrogate
+namespace {
+
+uint64_t splitmix(uint64_t &s)
+{
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+double unit(uint64_t &s) { return (double)(splitmix(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+MLPDev mlp_dev(sph_ctx *ctx)
+{
+    MLPDev d;
+    d.P = P_<float>(ctx->mP);
+    d.G = P_<float>(ctx->mG);
+    d.M = P_<float>(ctx->mM);
+    d.V = P_<float>(ctx->mV);
+    d.W2b = P_<unsigned short>(ctx->mW[0]);
+    d.W3b = P_<unsigned short>(ctx->mW[1]);
+    d.W2tb = P_<unsigned short>(ctx->mW[2]);
+    d.W3tb = P_<unsigned short>(ctx->mW[3]);
+    return d;
+}
+
+int mlp_alloc(sph_ctx *ctx)
+{
+    if (!ctx) return SPH_ERR_ARG;
+    HIPCK(hipSetDevice(ctx->device));
+    Buf *ps[] = {&ctx->mP, &ctx->mG, &ctx->mM, &ctx->mV};
+    for (Buf *b : ps) CK(ensure(ctx, *b, (size_t)MLP_NPARAM * 4));
+    for (int k = 0; k < 4; k++) CK(ensure(ctx, ctx->mW[k], (size_t)MLP_HIDDEN * MLP_HIDDEN * 2));
+    CK(ensure(ctx, ctx->mcnt, 64));
+    return SPH_OK;
+}
+
+// Upload parameters, refresh the BF16 copies, clear gradient and Adam moments.
+int mlp_load(sph_ctx *ctx, const float *params)
+{
+    CK(mlp_alloc(ctx));
+    HIPCK(hipMemcpyAsync(ctx->mP.p, params, (size_t)MLP_NPARAM * 4, hipMemcpyHostToDevice, ctx->st));
+    HIPCK(hipMemsetAsync(ctx->mG.p, 0, (size_t)MLP_NPARAM * 4, ctx->st));
+    HIPCK(hipMemsetAsync(ctx->mM.p, 0, (size_t)MLP_NPARAM * 4, ctx->st));
+    HIPCK(hipMemsetAsync(ctx->mV.p, 0, (size_t)MLP_NPARAM * 4, ctx->st));
+    HIPCK(sph::mlp_refresh(ctx->st, mlp_dev(ctx)));
+    HIPCK(hipStreamSynchronize(ctx->st));
+    ctx->mlp_step = 0;
+    ctx->mlp_ok = true;
+    ctx->mlp_cal = false;
+    return SPH_OK;
+}
+
+int mlp_need(sph_ctx *ctx, bool table, bool cal)
+{
+    if (!ctx) return SPH_ERR_ARG;
+    HIPCK(hipSetDevice(ctx->device));
+    if (!ctx->mlp_ok) return fail(ctx, SPH_ERR_STATE, "no surrogate parameters (call sph_mlp_init or sph_mlp_set_params)");
+    if (table && !ctx->ctab_ok) return fail(ctx, SPH_ERR_STATE, "no cooling table (call sph_cooling_table_build)");
+    if (cal && !ctx->mlp_cal) return fail(ctx, SPH_ERR_STATE, "no error map (call sph_mlp_calibrate)");
+    return SPH_OK;
+}
+
+__global__ void k_affine(int64_t n, float *__restrict__ a, float mu, float inv_sigma)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) a[i] = (a[i] - mu) * inv_sigma;
+}
+
+__global__ void k_rows_to_cols(int64_t n, const float *__restrict__ rows, float *__restrict__ cols)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n)
+        for (int a = 0; a < 4; a++) cols[a * n + i] = rows[4 * i + a];
+}
+
+// Training samples: n points uniform (per = 0) or stratified over cells, log10
+// coordinates in mx, normalised in mxn, exact log10 Lambda (normalised if
+// `normalise`) in mt.
+int mlp_make_samples(sph_ctx *ctx, int64_t n, uint64_t seed, const int *cells, int per, bool normalise)
+{
+    CK(ensure(ctx, ctx->mx, (size_t)n * 16));
+    CK(ensure(ctx, ctx->mxn, (size_t)n * 16));
+    CK(ensure(ctx, ctx->mt, (size_t)n * 4));
+    HIPCK(sph::mlp_samples(ctx->st, n, seed, ctx->mnorm, cells, per, P_<float>(ctx->mx), P_<float>(ctx->mxn)));
+    HIPCK(sph::cool_log_lambda_points(ctx->st, n, P_<float>(ctx->mx), D(rf1), P_<float>(ctx->mt)));
+    if (normalise)
+        k_affine<<<sph_blocks(n, 256), 256, 0, ctx->st>>>(n, P_<float>(ctx->mt), ctx->mnorm.mu,
+                                                          1.0f / ctx->mnorm.sigma);
+    return SPH_OK;
+}
+
+}  // namespace
+
+extern "C" int sph_mlp_init(sph_ctx *ctx, uint64_t seed)
+{
+    CK(cool_prep(ctx, 1));
+    if (!ctx->ctab_ok) return fail(ctx, SPH_ERR_STATE, "no cooling table (call sph_cooling_table_build)");
+    const CoolAxes &ax = ctx->cax;
+    MLPNorm nm{};
+    for (int a = 0; a < 4; a++) {
+        nm.lo[a] = (float)ax.lo[a];
+        nm.hi[a] = (float)(ax.lo[a] + (ax.n[a] - 1) * ax.d[a]);
+    }
+    // target normalisation from the table's log10 Lambda values
+    const int64_t total = (int64_t)ax.n[0] * ax.n[1] * ax.n[2] * ax.n[3];
+    std::vector<float> tab(total);
+    HIPCK(hipMemcpyAsync(tab.data(), ctx->ctabL.p, (size_t)total * 4, hipMemcpyDeviceToHost, ctx->st));
+    HIPCK(hipStreamSynchronize(ctx->st));
+    double s1 = 0.0, s2 = 0.0;
+    for (float v : tab) {
+        s1 += v;
+        s2 += (double)v * v;
+    }
+    const double mu = s1 / total, var = std::max(s2 / total - mu * mu, 1e-12);
+    nm.mu = (float)mu;
+    nm.sigma = (float)std::sqrt(var);
+    // He-uniform hidden layers, Glorot-uniform output, zero biases
+    std::vector<float> P(MLP_NPARAM, 0.0f);
+    uint64_t s = seed;
+    auto fill = [&](int off, int count, int fan_in, double gain) {
+        const double a = std::sqrt(gain / fan_in);
+        for (int i = 0; i < count; i++) P[off + i] = (float)((2.0 * unit(s) - 1.0) * a);
+    };
+    const int H = MLP_HIDDEN;
+    fill(0, H * 4, 4, 6.0);
+    fill(H * 4 + H, H * H, H, 6.0);
+    fill(H * 4 + H + H * H + H, H * H, H, 6.0);
+    fill(H * 4 + H + 2 * (H * H + H), H, H, 3.0);
+    ctx->mnorm = nm;
+    return mlp_load(ctx, P.data());
+}
+
+extern "C" int sph_mlp_set_params(sph_ctx *ctx, const float *params, const double *norm)
+{
+    if (!ctx) return SPH_ERR_ARG;
+    if (!params || !norm) return fail(ctx, SPH_ERR_ARG, "null argument");
+    MLPNorm nm{};
+    for (int a = 0; a < 4; a++) {
+        nm.lo[a] = (float)norm[a];
+        nm.hi[a] = (float)norm[4 + a];
+        if (!(nm.hi[a] > nm.lo[a])) return fail(ctx, SPH_ERR_ARG, "norm: hi must exceed lo on every axis");
+    }
+    nm.mu = (float)norm[8];
+    nm.sigma = (float)norm[9];
+    if (!(nm.sigma > 0.0f)) return fail(ctx, SPH_ERR_ARG, "norm: sigma must be positive");
+    ctx->mnorm = nm;
+    return mlp_load(ctx, params);
+}
+
+extern "C" int sph_mlp_get_params(sph_ctx *ctx, float *params, double *norm)
+{
+    CK(mlp_need(ctx, false, false));
+    if (params) {
+        HIPCK(hipMemcpyAsync(params, ctx->mP.p, (size_t)MLP_NPARAM * 4, hipMemcpyDeviceToHost, ctx->st));
+        HIPCK(hipStreamSynchronize(ctx->st));
+    }
+    if (norm) {
+        for (int a = 0; a < 4; a++) {
+            norm[a] = ctx->mnorm.lo[a];
+            norm[4 + a] = ctx->mnorm.hi[a];
+        }
+        norm[8] = ctx->mnorm.mu;
+        norm[9] = ctx->mnorm.sigma;
+    }
+    return SPH_OK;
+}
+
+extern "C" int sph_mlp_forward(sph_ctx *ctx, int64_t n, const float *x, float *y)
+{
+    CK(mlp_need(ctx, false, false));
+    if (n < 1 || !x || !y) return fail(ctx, SPH_ERR_ARG, "need n >= 1 and non-null x, y");
+    CK(ensure(ctx, ctx->mxn, (size_t)n * 16));
+    CK(ensure(ctx, ctx->my, (size_t)n * 4));
+    HIPCK(hipMemcpyAsync(ctx->mxn.p, x, (size_t)n * 16, hipMemcpyHostToDevice, ctx->st));
+    HIPCK(sph::mlp_forward_raw(ctx->st, n, P_<float>(ctx->mxn), mlp_dev(ctx), P_<float>(ctx->my)));
+    HIPCK(hipMemcpyAsync(y, ctx->my.p, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->st));
+    HIPCK(hipStreamSynchronize(ctx->st));
+    return SPH_OK;
+}
+
+namespace {
+// Gradient of the mean squared error over the n given samples into the device
+// gradient buffer (cleared first); *loss gets the mean.
+int mlp_grad_on(sph_ctx *ctx, int64_t n, const float *x, const float *t, double *loss)
+{
+    CK(ensure(ctx, ctx->mxn, (size_t)n * 16));
+    CK(ensure(ctx, ctx->mt, (size_t)n * 4));
+    CK(ensure(ctx, ctx->mloss, 64));
+    HIPCK(hipMemcpyAsync(ctx->mxn.p, x, (size_t)n * 16, hipMemcpyHostToDevice, ctx->st));
+    HIPCK(hipMemcpyAsync(ctx->mt.p, t, (size_t)n * 4, hipMemcpyHostToDevice, ctx->st));
+    HIPCK(hipMemsetAsync(ctx->mG.p, 0, (size_t)MLP_NPARAM * 4, ctx->st));
+    HIPCK(hipMemsetAsync(ctx->mloss.p, 0, 4, ctx->st));
+    HIPCK(sph::mlp_train_batch(ctx->st, n, 0, n, 1, 0, P_<float>(ctx->mxn), P_<float>(ctx->mt), mlp_dev(ctx),
+                               P_<float>(ctx->mloss)));
+    if (loss) {
+        float l = 0.0f;
+        HIPCK(hipMemcpyAsync(&l, ctx->mloss.p, 4, hipMemcpyDeviceToHost, ctx->st));
+        HIPCK(hipStreamSynchronize(ctx->st));
+        *loss = l;
+    }
+    return SPH_OK;
+}
+}  // namespace
+
+extern "C" int sph_mlp_gradient(sph_ctx *ctx, int64_t n, const float *x, const float *t, float *grad, double *loss)
+{
+    CK(mlp_need(ctx, false, false));
+    if (n < 1 || !x || !t || !grad) return fail(ctx, SPH_ERR_ARG, "need n >= 1 and non-null x, t, grad");
+    CK(mlp_grad_on(ctx, n, x, t, loss));
+    HIPCK(hipMemcpyAsync(grad, ctx->mG.p, (size_t)MLP_NPARAM * 4, hipMemcpyDeviceToHost, ctx->st));
+    HIPCK(hipMemsetAsync(ctx->mG.p, 0, (size_t)MLP_NPARAM * 4, ctx->st));
+    HIPCK(hipStreamSynchronize(ctx->st));
+    return SPH_OK;
+}
+
+extern "C" int sph_mlp_train_batch(sph_ctx *ctx, int64_t n, const float *x, const float *t, double lr, double *loss)
+{
+    CK(mlp_need(ctx, false, false));
+    if (n < 1 || !x || !t || !(lr > 0.0)) return fail(ctx, SPH_ERR_ARG, "need n >= 1, non-null x, t and lr > 0");
+    CK(mlp_grad_on(ctx, n, x, t, loss));
+    HIPCK(sph::mlp_adam(ctx->st, mlp_dev(ctx), (float)lr, 0.9f, 0.999f, 1e-8f, ++ctx->mlp_step));
+    HIPCK(hipStreamSynchronize(ctx->st));
+    ctx->mlp_cal = false;
+    return SPH_OK;
+}
+
+extern "C" int sph_mlp_train(sph_ctx *ctx, int64_t n_samples, int epochs, int batch, double lr, uint64_t seed,
+                             double *loss_out)
+{
+    CK(cool_prep(ctx, 1));
+    CK(mlp_need(ctx, true, false));
+    if (n_samples < 1 || n_samples > ((int64_t)1 << 27) || epochs < 1 || batch < 32 || !(lr > 0.0))
+        return fail(ctx, SPH_ERR_ARG, "need 1 <= n_samples <= 2^27, epochs >= 1, batch >= 32, lr > 0");
+    CK(mlp_make_samples(ctx, n_samples, seed, nullptr, 0, true));
+    CK(ensure(ctx, ctx->mloss, (size_t)epochs * 4));
+    HIPCK(hipMemsetAsync(ctx->mloss.p, 0, (size_t)epochs * 4, ctx->st));
+    const int64_t nb = (n_samples + batch - 1) / batch;
+    const double total = (double)nb * epochs, lr_min = lr * 0.01;
+    uint64_t rs = seed ^ 0x5DEECE66Dull;
+    const MLPDev d = mlp_dev(ctx);
+    for (int e = 0; e < epochs; e++) {
+        // a fresh permutation j -> (mult*j + off) mod n with gcd(mult, n) = 1
+        int64_t mult = 1, off = (int64_t)(splitmix(rs) % (uint64_t)n_samples);
+        for (int tries = 0; tries < 64; tries++) {
+            const int64_t cand = 1 + (int64_t)(splitmix(rs) % (uint64_t)n_samples);
+            int64_t a = cand, b = n_samples;
+            while (b) {
+                const int64_t r = a % b;
+                a = b;
+                b = r;
+            }
+            if (a == 1) {
+                mult = cand % n_samples == 0 ? 1 : cand % n_samples;
+                break;
+            }
+        }
+        for (int64_t b = 0; b < nb; b++) {
+            const int64_t first = b * batch, bsz = std::min<int64_t>(batch, n_samples - first);
+            const double tt = (double)(e * nb + b) / total;
+            const double lr_t = lr_min + 0.5 * (lr - lr_min) * (1.0 + std::cos(SPH_PI * tt));
+            HIPCK(sph::mlp_train_batch(ctx->st, n_samples, first, bsz, mult, off, P_<float>(ctx->mxn),
+                                       P_<float>(ctx->mt), d, P_<float>(ctx->mloss) + e));
+            HIPCK(sph::mlp_adam(ctx->st, d, (float)lr_t, 0.9f, 0.999f, 1e-8f, ++ctx->mlp_step));
+        }
+    }
+    std::vector<float> l(epochs);
+    HIPCK(hipMemcpyAsync(l.data(), ctx->mloss.p, (size_t)epochs * 4, hipMemcpyDeviceToHost, ctx->st));
+    HIPCK(hipStreamSynchronize(ctx->st));
+    if (loss_out)
+        for (int e = 0; e < epochs; e++) loss_out[e] = (double)l[e] / (double)nb;
+    ctx->mlp_cal = false;
+    return SPH_OK;
+}
+
+extern "C" int sph_mlp_calibrate(sph_ctx *ctx, const int *cells, int per, uint64_t seed, float *emap_table,
+                                 float *emap_exact, float *emap_table_exact)
+{
+    CK(cool_prep(ctx, 1));
+    CK(mlp_need(ctx, true, false));
+    if (!cells || per < 1) return fail(ctx, SPH_ERR_ARG, "need cells[4] and per >= 1");
+    int64_t ncell = 1;
+    for (int a = 0; a < 4; a++) {
+        if (cells[a] < 1 || cells[a] > 4096) return fail(ctx, SPH_ERR_ARG, "cells per axis must be in [1, 4096]");
+        ncell *= cells[a];
+    }
+    const int64_t nval = ncell * per;
+    if (ncell > ((int64_t)1 << 24) || nval > ((int64_t)1 << 27))
+        return fail(ctx, SPH_ERR_ARG, "error map too large (cells <= 2^24, cells*per <= 2^27)");
+    CK(mlp_make_samples(ctx, nval, seed, cells, per, false));
+    CK(ensure(ctx, ctx->my, (size_t)nval * 4));
+    CK(ensure(ctx, ctx->memap, (size_t)ncell * 4));
+    CK(ensure(ctx, ctx->memap2, (size_t)ncell * 4));
+    CK(ensure(ctx, ctx->memap3, (size_t)ncell * 4));
+    CK(ensure(ctx, ctx->mgate, (size_t)ncell * 4));
+    HIPCK(hipMemsetAsync(ctx->memap.p, 0, (size_t)ncell * 4, ctx->st));
+    HIPCK(hipMemsetAsync(ctx->memap2.p, 0, (size_t)ncell * 4, ctx->st));
+    HIPCK(hipMemsetAsync(ctx->memap3.p, 0, (size_t)ncell * 4, ctx->st));
+    HIPCK(sph::mlp_forward_raw(ctx->st, nval, P_<float>(ctx->mxn), mlp_dev(ctx), P_<float>(ctx->my)));
+    HIPCK(sph::mlp_emap(ctx->st, nval, per, P_<float>(ctx->my), P_<float>(ctx->mx), P_<float>(ctx->mt), ctx->mnorm,
+                        ctx->cax, P_<float>(ctx->ctabL), P_<float>(ctx->memap), P_<float>(ctx->memap2),
+                        P_<float>(ctx->memap3)));
+    HIPCK(sph::mlp_gate_map(ctx->st, ncell, P_<float>(ctx->memap2), P_<float>(ctx->memap3), P_<float>(ctx->mgate)));
+    if (emap_table)
+        HIPCK(hipMemcpyAsync(emap_table, ctx->memap.p, (size_t)ncell * 4, hipMemcpyDeviceToHost, ctx->st));
+    if (emap_exact)
+        HIPCK(hipMemcpyAsync(emap_exact, ctx->memap2.p, (size_t)ncell * 4, hipMemcpyDeviceToHost, ctx->st));
+    if (emap_table_exact)
+        HIPCK(hipMemcpyAsync(emap_table_exact, ctx->memap3.p, (size_t)ncell * 4, hipMemcpyDeviceToHost, ctx->st));
+    HIPCK(hipStreamSynchronize(ctx->st));
+    for (int a = 0; a < 4; a++) ctx->mcells[a] = cells[a];
+    ctx->mlp_cal = true;
+    return SPH_OK;
+}
+
+extern "C" int sph_mlp_cooling(sph_ctx *ctx, int64_t n, const double *lognH, const double *logT, const double *logZ,
+                               const double *logJ21, double tol, int ref, double *logL_out, int32_t *fallback_out,
+                               int64_t *n_fallback)
+{
+    CK(cool_prep(ctx, n));
+    CK(mlp_need(ctx, true, true));
+    if (!lognH || !logT || !logZ || !logJ21 || !logL_out) return fail(ctx, SPH_ERR_ARG, "null argument");
+    if (ref != SPH_MLP_REF_EXACT && ref != SPH_MLP_REF_TABLE) return fail(ctx, SPH_ERR_ARG, "ref must be 0 or 1");
+    const float *emap = ref == SPH_MLP_REF_EXACT ? P_<float>(ctx->mgate) : P_<float>(ctx->memap);
+    CK(ensure(ctx, ctx->mx, (size_t)n * 16));
+    CK(ensure(ctx, ctx->my, (size_t)n * 4));
+    float *cols = P_<float>(ctx->mx);  // log10 inputs as four FP32 columns
+    const double *src[4] = {lognH, logT, logZ, logJ21};
+    for (int k = 0; k < 4; k++) {
+        HIPCK(hipMemcpyAsync(ctx->cin[k].p, src[k], sizeof(double) * n, hipMemcpyHostToDevice, ctx->st));
+        k_d2f<<<sph_blocks(n, 256), 256, 0, ctx->st>>>(n, D(cin[k]), cols + k * n);
+    }
+    float *oL = (float *)ctx->cout[0].p;
+    int *flag = (int *)ctx->cout[1].p;
+    HIPCK(hipMemsetAsync(ctx->mcnt.p, 0, 8, ctx->st));
+    HIPCK(sph::mlp_infer(ctx->st, n, cols, ctx->mnorm, mlp_dev(ctx), emap, ctx->mcells, (float)tol, ctx->cax,
+                         P_<float>(ctx->ctabL), P_<float>(ctx->my), oL, flag, P_<unsigned long long>(ctx->mcnt)));
+    k_f2d<<<sph_blocks(n, 256), 256, 0, ctx->st>>>(n, oL, D(cin[0]));
+    HIPCK(hipMemcpyAsync(logL_out, ctx->cin[0].p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->st));
+    if (fallback_out)
+        HIPCK(hipMemcpyAsync(fallback_out, flag, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->st));
+    unsigned long long nf = 0;
+    HIPCK(hipMemcpyAsync(&nf, ctx->mcnt.p, 8, hipMemcpyDeviceToHost, ctx->st));
+    HIPCK(hipStreamSynchronize(ctx->st));
+    if (n_fallback) *n_fallback = (int64_t)nf;
+    return SPH_OK;
+}
+
+extern "C" int sph_mlp_timing(sph_ctx *ctx, int64_t n, int reps, uint64_t seed, double tol, double *out)
+{
+    CK(cool_prep(ctx, 1));
+    CK(mlp_need(ctx, true, true));
+    if (n < 1 || n > ((int64_t)1 << 26) || reps < 1 || !out)
+        return fail(ctx, SPH_ERR_ARG, "need 1 <= n <= 2^26, reps >= 1, out[4]");
+    CK(mlp_make_samples(ctx, n, seed, nullptr, 0, false));
+    CK(ensure(ctx, ctx->my, (size_t)n * 16));
+    CK(ensure(ctx, ctx->cout[0], (size_t)n * 8));
+    CK(ensure(ctx, ctx->cout[1], (size_t)n * 8));
+    float *cols = P_<float>(ctx->my);  // log10 inputs as four columns
+    k_rows_to_cols<<<sph_blocks(n, 256), 256, 0, ctx->st>>>(n, P_<float>(ctx->mx), cols);
+    float *oL = (float *)ctx->cout[0].p;
+    int *flag = (int *)ctx->cout[1].p;
+    const MLPDev d = mlp_dev(ctx);
+    hipEvent_t e0, e1;
+    HIPCK(hipEventCreate(&e0));
+    HIPCK(hipEventCreate(&e1));
+    float ms[3] = {0, 0, 0};
+    for (int which = 0; which < 3; which++) {
+        for (int r = -1; r < reps; r++) {  // r = -1: warm-up launch
+            if (r == 0) HIPCK(hipEventRecord(e0, ctx->st));
+            if (which == 0) {
+                HIPCK(hipMemsetAsync(ctx->mcnt.p, 0, 8, ctx->st));
+                HIPCK(sph::mlp_infer(ctx->st, n, cols, ctx->mnorm, d, P_<float>(ctx->mgate), ctx->mcells, (float)tol,
+                                     ctx->cax, P_<float>(ctx->ctabL), P_<float>(ctx->mt), oL, flag,
+                                     P_<unsigned long long>(ctx->mcnt)));
+            } else if (which == 1) {
+                HIPCK(sph::cool_table_lookup(ctx->st, n, ctx->cax, P_<float>(ctx->ctabL), nullptr, cols, cols + n,
+                                             cols + 2 * n, cols + 3 * n, oL, nullptr));
+            } else {
+                HIPCK(sph::mlp_forward_raw(ctx->st, n, P_<float>(ctx->mxn), d, oL));
+            }
+        }
+        HIPCK(hipEventRecord(e1, ctx->st));
+        HIPCK(hipEventSynchronize(e1));
+        HIPCK(hipEventElapsedTime(&ms[which], e0, e1));
+    }
+    unsigned long long nf = 0;
+    HIPCK(hipMemcpy(&nf, ctx->mcnt.p, 8, hipMemcpyDeviceToHost));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    out[0] = ms[0] / reps;
+    out[1] = ms[1] / reps;
+    out[2] = ms[2] / reps;
+    out[3] = (double)nf / (double)n;
     return SPH_OK;
 }

@@ -261,15 +261,6 @@ __global__ void __launch_bounds__(256) k_cool_table(CoolAxes ax, const double *_
 }
 
 // 4D multilinear lookup in log space; inputs are log10 values (FP32).
-__device__ __forceinline__ void table_pos(float v, float lo, float d, int n, int &i, float &f)
-{
-    float x = (v - lo) / d;
-    x = fminf(fmaxf(x, 0.0f), (float)(n - 1) - 1e-4f);
-    i = (int)x;
-    if (i > n - 2) i = n - 2;
-    f = x - (float)i;
-}
-
 __global__ void __launch_bounds__(256) k_cool_lookup(int64_t n, CoolAxes ax, const float *__restrict__ logL,
                                                      const float *__restrict__ logG, const float *__restrict__ lnH,
                                                      const float *__restrict__ lT, const float *__restrict__ lZ,
@@ -278,25 +269,23 @@ __global__ void __launch_bounds__(256) k_cool_lookup(int64_t n, CoolAxes ax, con
 {
     int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (p >= n) return;
-    int i[4];
-    float f[4];
     const float v[4] = {lnH[p], lT[p], lZ[p], lU[p]};
-#pragma unroll
-    for (int a = 0; a < 4; a++) table_pos(v[a], (float)ax.lo[a], (float)ax.d[a], ax.n[a], i[a], f[a]);
-    const int64_t s3 = 1, s2 = ax.n[3], s1 = (int64_t)ax.n[2] * ax.n[3], s0 = (int64_t)ax.n[1] * s1;
-    const int64_t base = i[0] * s0 + i[1] * s1 + i[2] * s2 + i[3] * s3;
-    float accL = 0.0f, accG = 0.0f;
-#pragma unroll
-    for (int c = 0; c < 16; c++) {
-        const int b0 = (c >> 3) & 1, b1 = (c >> 2) & 1, b2 = (c >> 1) & 1, b3 = c & 1;
-        const float w = (b0 ? f[0] : 1.0f - f[0]) * (b1 ? f[1] : 1.0f - f[1]) * (b2 ? f[2] : 1.0f - f[2]) *
-                        (b3 ? f[3] : 1.0f - f[3]);
-        const int64_t o = base + b0 * s0 + b1 * s1 + b2 * s2 + b3 * s3;
-        accL += w * logL[o];
-        accG += w * logG[o];
-    }
-    outL[p] = accL;
-    outG[p] = accG;
+    float L, G;
+    cool_table_interp(ax, logL, logG, v, L, G);
+    outL[p] = L;
+    if (outG) outG[p] = G;
+}
+
+// Exact log10 Lambda at FP32 log10 points (training and validation samples).
+__global__ void __launch_bounds__(256) k_cool_points(int64_t n, const float *__restrict__ x4,
+                                                     const double *__restrict__ rf1, float *__restrict__ logL)
+{
+    int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const double lnH = x4[4 * p], lT = x4[4 * p + 1], lZ = x4[4 * p + 2], lU = x4[4 * p + 3];
+    double h, l;
+    cool_heat_lambda(pow(10.0, lT), pow(10.0, lnH), pow(10.0, lU), pow(10.0, lZ), rf1, h, l);
+    logL[p] = (float)log10(fmax(l, 1e-40));
 }
 
 namespace sph {
@@ -341,6 +330,12 @@ hipError_t cool_table_lookup(hipStream_t st, int64_t n, const CoolAxes &ax, cons
                              float *outG)
 {
     k_cool_lookup<<<sph_blocks(n, 256), 256, 0, st>>>(n, ax, logL, logG, lnH, lT, lZ, lU, outL, outG);
+    return hipGetLastError();
+}
+
+hipError_t cool_log_lambda_points(hipStream_t st, int64_t n, const float *x4, const double *rf1, float *logL)
+{
+    k_cool_points<<<sph_blocks(n, 256), 256, 0, st>>>(n, x4, rf1, logL);
     return hipGetLastError();
 }
 

@@ -1,0 +1,624 @@
+// sph_mlp.hip -- learned surrogate of the per-particle cooling rate: an MLP
+// 4 -> 128 -> 128 -> 128 -> 1 with SiLU activations mapping the normalised
+// (log10 n_H, log10 T, log10 Z, log10 J21) to the normalised log10 Lambda.
+//
+// Design point: the Keras surrogate of
+// Particles_in_BH_Tree_III/CLOUDY_replace_CHIMES/HCooling_Here/ML/prog2.py:24-52
+// (4 -> 128 -> 64 -> 1, Adam on the mean squared error), which the reference
+// trains and runs offline only.  Here training and inference are device kernels:
+//   * weights and activations BF16, accumulation FP32; the two 128 x 128
+//     layers run on v_mfma_f32_32x32x8_bf16, one wavefront per tile of 32
+//     samples, activations staged in LDS;
+//   * backward on MFMA as well (dh = dz W, dW = dz^T h); weight gradients are
+//     summed over tiles with FP32 atomics; Adam keeps FP32 master weights and
+//     refreshes the BF16 copies (row-major and transposed);
+//   * inference gates each particle: outside the training box, or in a cell
+//     whose measured deviation from the table exceeds the tolerance, the
+//     result is the table's 4D multilinear interpolation instead.
+//
+// Numerics of one sample (what the CPU oracle emulates):
+//   z1 = W1 x + b1 (FP32),           h1 = bf16(silu(z1))
+//   z2 = bf16(W2) h1 + b2 (FP32 acc), h2 = bf16(silu(z2))   (same for layer 3)
+//   y  = W4 . h3 + b4 (FP32)
+//   backward: dz3 = bf16(dy W4 silu'(z3)); dz2 = bf16((dz3 bf16(W3)) silu'(z2));
+//   dz1 = bf16((dz2 bf16(W2)) silu'(z1)); z_l are kept in BF16 for silu'.
+//
+// MFMA operand layout (32x32x8, wave64): lane l supplies A[l%32][4(l/32)+i]
+// and B[4(l/32)+i][l%32], i = 0..3; accumulator r holds
+// D[8(r/4) + 4(l/32) + r%4][l%32], r = 0..15.
+#include "sph_mlp.h"
+
+#include <hip/amd_detail/amd_hip_unsafe_atomics.h>
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+#define TS 32            // samples per wave tile
+#define HW MLP_HIDDEN    // hidden width
+#define LDP 136          // LDS row pitch in bf16 elements (272 B: rows 68 dwords apart)
+#define TILE (TS * LDP)  // one [32][LDP] bf16 buffer
+
+// flat parameter layout
+#define O_W1 0
+#define O_B1 512
+#define O_W2 640
+#define O_B2 (O_W2 + HW * HW)
+#define O_W3 (O_B2 + HW)
+#define O_B3 (O_W3 + HW * HW)
+#define O_W4 (O_B3 + HW)
+#define O_B4 (O_W4 + HW)
+static_assert(O_B4 + 1 == MLP_NPARAM, "parameter layout");
+
+__device__ __forceinline__ unsigned short f2bf(float f)
+{
+    unsigned int u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest, ties to even (finite inputs)
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float(((unsigned int)b) << 16); }
+__device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigm(z); }
+__device__ __forceinline__ float dsilu(float z)
+{
+    const float s = sigm(z);
+    return s * (1.0f + z * (1.0f - s));
+}
+
+__device__ __forceinline__ bf16x4 as_bf16x4(uint2 v)
+{
+    bf16x4 r;
+    r[0] = (short)(v.x & 0xFFFF);
+    r[1] = (short)(v.x >> 16);
+    r[2] = (short)(v.y & 0xFFFF);
+    r[3] = (short)(v.y >> 16);
+    return r;
+}
+
+__device__ __forceinline__ int drow(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
+
+// 32 x 32 tile of A (32 x 128, LDS rows of pitch LDP) times B (128 x 32) with
+// B(k, n) = Wrow[n*128 + k] (a row-major BF16 matrix whose rows n are the
+// tile's output columns).
+__device__ __forceinline__ floatx16 tile_gemm(const unsigned short *A, const unsigned short *__restrict__ Wrow)
+{
+    const int lane = threadIdx.x & 63, r32 = lane & 31, hk = (lane >> 5) * 4;
+    floatx16 acc = {};
+#pragma unroll 4
+    for (int kk = 0; kk < HW; kk += 8) {
+        const bf16x4 a = as_bf16x4(*reinterpret_cast<const uint2 *>(A + r32 * LDP + kk + hk));
+        const bf16x4 b = as_bf16x4(*reinterpret_cast<const uint2 *>(Wrow + (size_t)r32 * HW + kk + hk));
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// Forward pass of the wave's 32 samples.  xin: this lane's sample (lane%32).
+// h1..h3 receive the activations and, if non-null, z1..z3 the pre-activations
+// (inference passes h3 == h1 to ping-pong two buffers).  Returns the
+// normalised output (identical on both lane halves).
+__device__ __forceinline__ float mlp_forward(const float xin[4], const float *__restrict__ P,
+                                             const unsigned short *__restrict__ W2b,
+                                             const unsigned short *__restrict__ W3b, unsigned short *h1,
+                                             unsigned short *h2, unsigned short *h3, unsigned short *z1,
+                                             unsigned short *z2, unsigned short *z3)
+{
+    const int lane = threadIdx.x & 63, s = lane & 31, half = lane >> 5;
+    for (int q = 0; q < 64; q++) {
+        const int f = half * 64 + q;
+        float zz = P[O_B1 + f];
+#pragma unroll
+        for (int k = 0; k < 4; k++) zz = fmaf(P[O_W1 + f * 4 + k], xin[k], zz);
+        h1[s * LDP + f] = f2bf(silu(zz));
+        if (z1) z1[s * LDP + f] = f2bf(zz);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int L = 0; L < 2; L++) {
+        const unsigned short *W = L == 0 ? W2b : W3b;
+        const int ob = L == 0 ? O_B2 : O_B3;
+        const unsigned short *hin = L == 0 ? h1 : h2;
+        unsigned short *hout = L == 0 ? h2 : h3;
+        unsigned short *zout = L == 0 ? z2 : z3;
+        for (int nt = 0; nt < 4; nt++) {
+            const floatx16 acc = tile_gemm(hin, W + (size_t)nt * 32 * HW);
+            const int f = nt * 32 + (lane & 31);
+            const float bias = P[ob + f];
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int row = drow(r, lane);
+                const float zz = acc[r] + bias;
+                hout[row * LDP + f] = f2bf(silu(zz));
+                if (zout) zout[row * LDP + f] = f2bf(zz);
+            }
+        }
+        __syncthreads();
+    }
+    float y = 0.0f;
+    for (int q = 0; q < 64; q++) {
+        const int f = half * 64 + q;
+        y = fmaf(P[O_W4 + f], bf2f(h3[s * LDP + f]), y);
+    }
+    y += __shfl_xor(y, 32);
+    return y + P[O_B4];
+}
+
+// ---------------------------------------------------------------- inference
+// Inference works on 64 samples per wavefront and computes each hidden layer
+// transposed, Z^T = W H^T: the MFMA A operand is a 32 x 16 slice of the
+// row-major BF16 weight (16-byte global loads), the B operand 16 features of
+// 32 samples from the LDS activation rows, and each lane's accumulators hold
+// 4 consecutive features of one sample, so the BF16 activations go back to
+// LDS as 8-byte stores (in place: a layer's reads all precede its writes) and
+// the output layer is a register dot product in the layer-3 epilogue.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+#define IS 64                 // samples per wave
+#define ITILE (IS * LDP)      // activation buffer [64][LDP]
+
+__device__ __forceinline__ float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+
+__device__ __forceinline__ unsigned int pack2(float a, float b)
+{
+    bf16x2 v;
+    v[0] = (__bf16)a;  // v_cvt_pk_bf16_f32: round to nearest even
+    v[1] = (__bf16)b;
+    return __builtin_bit_cast(unsigned int, v);
+}
+
+__device__ __forceinline__ float bf16_round(float a) { return (float)(__bf16)a; }
+
+// Normalised output of sample (blockIdx.x*64 + lane) for this lane's input xin.
+__device__ __forceinline__ float mlp_forward64(const float xin[4], const float *__restrict__ P,
+                                               const unsigned short *__restrict__ W2b,
+                                               const unsigned short *__restrict__ W3b, unsigned short *act)
+{
+    const int lane = threadIdx.x & 63, l32 = lane & 31, half = lane >> 5;
+    // layer 1 on VALU: this lane's sample, all 128 features (weights are wave-uniform)
+    for (int f0 = 0; f0 < HW; f0 += 8) {
+        float h[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int f = f0 + j;
+            float z = P[O_B1 + f];
+#pragma unroll
+            for (int k = 0; k < 4; k++) z = fmaf(P[O_W1 + f * 4 + k], xin[k], z);
+            h[j] = silu_fast(z);
+        }
+        uint4 w;
+        w.x = pack2(h[0], h[1]);
+        w.y = pack2(h[2], h[3]);
+        w.z = pack2(h[4], h[5]);
+        w.w = pack2(h[6], h[7]);
+        *reinterpret_cast<uint4 *>(act + lane * LDP + f0) = w;
+    }
+    __syncthreads();
+    float y[2] = {0.0f, 0.0f};
+#pragma unroll
+    for (int L = 0; L < 2; L++) {
+        const unsigned short *W = L == 0 ? W2b : W3b;
+        floatx16 acc[4][2];
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) acc[mt][0] = acc[mt][1] = floatx16{};
+#pragma unroll 1
+        for (int kk = 0; kk < HW; kk += 16) {
+            bf16x8 bop[2];
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++)
+                bop[nt] = *reinterpret_cast<const bf16x8 *>(act + (nt * 32 + l32) * LDP + kk + 8 * half);
+#pragma unroll
+            for (int mt = 0; mt < 4; mt++) {
+                const bf16x8 aop =
+                    *reinterpret_cast<const bf16x8 *>(W + (size_t)(mt * 32 + l32) * HW + kk + 8 * half);
+#pragma unroll
+                for (int nt = 0; nt < 2; nt++)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aop, bop[nt], acc[mt][nt], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+        const int ob = L == 0 ? O_B2 : O_B3;
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                // accumulators 4q..4q+3: features f..f+3 of samples nt*32 + l32
+                const int f = mt * 32 + 8 * q + 4 * half;
+                const float4 bias = *reinterpret_cast<const float4 *>(P + ob + f);
+                if (L == 0) {
+#pragma unroll
+                    for (int nt = 0; nt < 2; nt++) {
+                        const float h0 = silu_fast(acc[mt][nt][4 * q] + bias.x);
+                        const float h1 = silu_fast(acc[mt][nt][4 * q + 1] + bias.y);
+                        const float h2 = silu_fast(acc[mt][nt][4 * q + 2] + bias.z);
+                        const float h3 = silu_fast(acc[mt][nt][4 * q + 3] + bias.w);
+                        uint2 w;
+                        w.x = pack2(h0, h1);
+                        w.y = pack2(h2, h3);
+                        *reinterpret_cast<uint2 *>(act + (nt * 32 + l32) * LDP + f) = w;
+                    }
+                } else {
+                    const float4 w4 = *reinterpret_cast<const float4 *>(P + O_W4 + f);
+#pragma unroll
+                    for (int nt = 0; nt < 2; nt++) {
+                        y[nt] = fmaf(w4.x, bf16_round(silu_fast(acc[mt][nt][4 * q] + bias.x)), y[nt]);
+                        y[nt] = fmaf(w4.y, bf16_round(silu_fast(acc[mt][nt][4 * q + 1] + bias.y)), y[nt]);
+                        y[nt] = fmaf(w4.z, bf16_round(silu_fast(acc[mt][nt][4 * q + 2] + bias.z)), y[nt]);
+                        y[nt] = fmaf(w4.w, bf16_round(silu_fast(acc[mt][nt][4 * q + 3] + bias.w)), y[nt]);
+                    }
+                }
+            }
+        __syncthreads();
+    }
+    // lanes l and l^32 hold the two feature halves of samples l32 and 32 + l32
+    y[0] += __shfl_xor(y[0], 32);
+    y[1] += __shfl_xor(y[1], 32);
+    return (half ? y[1] : y[0]) + P[O_B4];
+}
+
+// Network output for rows of normalised inputs (COLS = false: x[n][4]) or for
+// log10 columns x[a*n + i] normalised here with nm (COLS = true).
+template <bool COLS>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+k_mlp_forward(int64_t n, const float *__restrict__ x, MLPNorm nm, const float *__restrict__ P,
+              const unsigned short *__restrict__ W2b, const unsigned short *__restrict__ W3b, float *__restrict__ y)
+{
+    __shared__ __align__(16) unsigned short act[ITILE];
+    const int64_t s = blockIdx.x * (int64_t)IS + threadIdx.x;
+    float xin[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (s < n) {
+        if (COLS) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) xin[k] = 2.0f * (x[k * n + s] - nm.lo[k]) / (nm.hi[k] - nm.lo[k]) - 1.0f;
+        } else {
+            const float4 v = *reinterpret_cast<const float4 *>(x + 4 * s);
+            xin[0] = v.x; xin[1] = v.y; xin[2] = v.z; xin[3] = v.w;
+        }
+    }
+    const float out = mlp_forward64(xin, P, W2b, W3b, act);
+    if (s < n) y[s] = out;
+}
+
+__device__ __forceinline__ int mlp_cell(const float v[4], const MLPNorm &nm, const int nc[4], bool &inside)
+{
+    int id = 0;
+    inside = true;
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        const float t = (v[a] - nm.lo[a]) / (nm.hi[a] - nm.lo[a]);
+        if (!(t >= 0.0f && t <= 1.0f)) inside = false;
+        const int c = (int)(fminf(fmaxf(t, 0.0f), 0.999999f) * nc[a]);
+        id = id * nc[a] + c;
+    }
+    return id;
+}
+
+// Gate of the inference at log10 columns x[a*n + i]: the network's value
+// (from yn) where the input is in the box and its cell's map value <= tol,
+// the table's interpolation elsewhere.
+__global__ void __launch_bounds__(256) k_mlp_gate(int64_t n, const float *__restrict__ x, const float *__restrict__ yn,
+                                                  MLPNorm nm, const float *__restrict__ emap, int ec0, int ec1, int ec2,
+                                                  int ec3, float tol, CoolAxes ax, const float *__restrict__ tabL,
+                                                  float *__restrict__ outL, int *__restrict__ flag,
+                                                  unsigned long long *__restrict__ nfall)
+{
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const bool valid = s < n;
+    bool fb = false;
+    if (valid) {
+        const float v[4] = {x[s], x[n + s], x[2 * n + s], x[3 * n + s]};
+        const int nc[4] = {ec0, ec1, ec2, ec3};
+        bool inside;
+        const int cell = mlp_cell(v, nm, nc, inside);
+        fb = !inside || !(emap[cell] <= tol);
+        float res = fmaf(yn[s], nm.sigma, nm.mu);
+        if (fb) {
+            float g;
+            cool_table_interp(ax, tabL, nullptr, v, res, g);
+        }
+        outL[s] = res;
+        if (flag) flag[s] = fb ? 1 : 0;
+    }
+    const uint64_t nb = __ballot(fb);
+    if ((threadIdx.x & 63) == 0 && nb) atomicAdd(nfall, (unsigned long long)__popcll(nb));
+}
+
+// ---------------------------------------------------------------- training
+// Forward and backward of one tile of 32 batch samples; batch sample j uses
+// training sample (mult*j + off) mod n.  Gradients of the batch-mean squared
+// error are added to G, the batch-mean loss to *loss.
+__global__ void __launch_bounds__(64) k_mlp_train(int64_t n, int64_t first, int64_t bsz, int64_t mult, int64_t off,
+                                                  const float *__restrict__ x, const float *__restrict__ t,
+                                                  const float *__restrict__ P, const unsigned short *__restrict__ W2b,
+                                                  const unsigned short *__restrict__ W3b,
+                                                  const unsigned short *__restrict__ W2tb,
+                                                  const unsigned short *__restrict__ W3tb, float *__restrict__ G,
+                                                  float *__restrict__ loss)
+{
+    __shared__ __align__(16) unsigned short buf[6 * TILE];
+    __shared__ float xs[TS][4], dys[TS];
+    unsigned short *h1 = buf, *h2 = buf + TILE, *h3 = buf + 2 * TILE;
+    unsigned short *z1 = buf + 3 * TILE, *z2 = buf + 4 * TILE, *z3 = buf + 5 * TILE;
+    const int lane = threadIdx.x, sl = lane & 31, half = lane >> 5;
+    const int64_t j = blockIdx.x * (int64_t)TS + sl;
+    const bool valid = j < bsz;
+    const int64_t src = valid ? (int64_t)(((unsigned long long)mult * (unsigned long long)(first + j) +
+                                           (unsigned long long)off) % (unsigned long long)n)
+                              : 0;
+    float xin[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) xin[k] = valid ? x[src * 4 + k] : 0.0f;
+    if (lane < 32)
+#pragma unroll
+        for (int k = 0; k < 4; k++) xs[sl][k] = xin[k];
+    const float y = mlp_forward(xin, P, W2b, W3b, h1, h2, h3, z1, z2, z3);
+    const float inv_b = 1.0f / (float)bsz;
+    const float err = valid ? (y - t[src]) : 0.0f;
+    if (lane < 32) dys[sl] = 2.0f * err * inv_b;
+    float l2 = lane < 32 ? err * err : 0.0f;
+    for (int o = 16; o >= 1; o >>= 1) l2 += __shfl_xor(l2, o);
+    if (lane == 0) unsafeAtomicAdd(loss, l2 * inv_b);
+    __syncthreads();
+
+    // output layer: gW4 = sum_s dy_s h3[s], gb4 = sum_s dy_s
+    for (int q = 0; q < 2; q++) {
+        const int f = lane + 64 * q;
+        float gw = 0.0f;
+        for (int s = 0; s < TS; s++) gw = fmaf(dys[s], bf2f(h3[s * LDP + f]), gw);
+        unsafeAtomicAdd(&G[O_W4 + f], gw);
+    }
+    if (lane == 0) {
+        float gb = 0.0f;
+        for (int s = 0; s < TS; s++) gb += dys[s];
+        unsafeAtomicAdd(&G[O_B4], gb);
+    }
+    // dz3 = dy W4 silu'(z3), in place over z3
+    for (int q = 0; q < 64; q++) {
+        const int f = half * 64 + q;
+        const float zz = bf2f(z3[sl * LDP + f]);
+        z3[sl * LDP + f] = f2bf(dys[sl] * P[O_W4 + f] * dsilu(zz));
+    }
+    __syncthreads();
+
+    // hidden layers 3 then 2: dz (32 x 128) in LDS
+    // dz3 lives over z3, dz2 over h3, dz1 over z3 again (dz3 is dead by then)
+    for (int L = 0; L < 2; L++) {
+        const unsigned short *dz = L == 0 ? z3 : h3;
+        const unsigned short *hp = L == 0 ? h2 : h1;
+        const unsigned short *zp = L == 0 ? z2 : z1;
+        const unsigned short *Wt = L == 0 ? W3tb : W2tb;
+        const int ow = L == 0 ? O_W3 : O_W2, obb = L == 0 ? O_B3 : O_B2;
+        // dW[o][i] = sum_s dz[s][o] hp[s][i]: A(o, s) = dz[s][o], B(s, i) = hp[s][i]
+        for (int mt = 0; mt < 4; mt++)
+            for (int nt = 0; nt < 4; nt++) {
+                floatx16 acc = {};
+#pragma unroll
+                for (int kk = 0; kk < TS; kk += 8) {
+                    const int k0 = kk + 4 * half;
+                    bf16x4 a, b;
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        a[i] = (short)dz[(k0 + i) * LDP + mt * 32 + sl];
+                        b[i] = (short)hp[(k0 + i) * LDP + nt * 32 + sl];
+                    }
+                    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+                }
+#pragma unroll
+                for (int r = 0; r < 16; r++)
+                    unsafeAtomicAdd(&G[ow + (mt * 32 + drow(r, lane)) * HW + nt * 32 + sl], acc[r]);
+            }
+        for (int q = 0; q < 2; q++) {
+            const int f = lane + 64 * q;
+            float gb = 0.0f;
+            for (int s = 0; s < TS; s++) gb += bf2f(dz[s * LDP + f]);
+            unsafeAtomicAdd(&G[obb + f], gb);
+        }
+        // dh_prev = dz W (B(o, i) = W[o][i] = Wt[i][o]); dz_prev = dh_prev silu'(z_prev)
+        unsigned short *nx = L == 0 ? h3 : z3;
+        for (int nt = 0; nt < 4; nt++) {
+            const floatx16 acc = tile_gemm(dz, Wt + (size_t)nt * 32 * HW);
+            const int i = nt * 32 + sl;
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int s = drow(r, lane);
+                nx[s * LDP + i] = f2bf(acc[r] * dsilu(bf2f(zp[s * LDP + i])));
+            }
+        }
+        __syncthreads();
+    }
+    // input layer: gW1[o][k] = sum_s dz1[s][o] x[s][k], gb1[o] = sum_s dz1[s][o]
+    const unsigned short *dz1 = z3;
+    for (int q = 0; q < 2; q++) {
+        const int o = lane + 64 * q;
+        float g[4] = {0.0f, 0.0f, 0.0f, 0.0f}, gb = 0.0f;
+        for (int s = 0; s < TS; s++) {
+            const float d = bf2f(dz1[s * LDP + o]);
+            gb += d;
+#pragma unroll
+            for (int k = 0; k < 4; k++) g[k] = fmaf(d, xs[s][k], g[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) unsafeAtomicAdd(&G[O_W1 + o * 4 + k], g[k]);
+        unsafeAtomicAdd(&G[O_B1 + o], gb);
+    }
+}
+
+__device__ __forceinline__ void store_bf16_copies(int i, float w, unsigned short *__restrict__ W2b,
+                                                  unsigned short *__restrict__ W3b, unsigned short *__restrict__ W2tb,
+                                                  unsigned short *__restrict__ W3tb)
+{
+    if (i >= O_W2 && i < O_W2 + HW * HW) {
+        const int k = i - O_W2, o = k / HW, in = k % HW;
+        W2b[k] = f2bf(w);
+        W2tb[in * HW + o] = f2bf(w);
+    } else if (i >= O_W3 && i < O_W3 + HW * HW) {
+        const int k = i - O_W3, o = k / HW, in = k % HW;
+        W3b[k] = f2bf(w);
+        W3tb[in * HW + o] = f2bf(w);
+    }
+}
+
+// Adam (Kingma & Ba 2015) with bias correction c1 = 1 - b1^t, c2 = 1 - b2^t.
+__global__ void __launch_bounds__(256) k_adam(int np, float *__restrict__ P, float *__restrict__ G,
+                                              float *__restrict__ M, float *__restrict__ V, float lr, float b1,
+                                              float b2, float eps, float c1, float c2,
+                                              unsigned short *__restrict__ W2b, unsigned short *__restrict__ W3b,
+                                              unsigned short *__restrict__ W2tb, unsigned short *__restrict__ W3tb)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const float g = G[i];
+    const float m = b1 * M[i] + (1.0f - b1) * g;
+    const float v = b2 * V[i] + (1.0f - b2) * g * g;
+    M[i] = m;
+    V[i] = v;
+    const float w = P[i] - lr * (m / c1) / (sqrtf(v / c2) + eps);
+    P[i] = w;
+    G[i] = 0.0f;
+    store_bf16_copies(i, w, W2b, W3b, W2tb, W3tb);
+}
+
+__global__ void __launch_bounds__(256) k_refresh_bf16(int np, const float *__restrict__ P,
+                                                      unsigned short *__restrict__ W2b,
+                                                      unsigned short *__restrict__ W3b,
+                                                      unsigned short *__restrict__ W2tb,
+                                                      unsigned short *__restrict__ W3tb)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    store_bf16_copies(i, P[i], W2b, W3b, W2tb, W3tb);
+}
+
+// ---------------------------------------------------------------- samples
+__device__ __forceinline__ uint64_t mix64(uint64_t v)
+{
+    // SplitMix64 finaliser (Steele, Lea & Flood 2014)
+    v += 0x9E3779B97F4A7C15ull;
+    v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ull;
+    v = (v ^ (v >> 27)) * 0x94D049BB133111EBull;
+    return v ^ (v >> 31);
+}
+__device__ __forceinline__ double unit_draw(uint64_t seed, uint64_t i, int k)
+{
+    return (double)(mix64(seed ^ mix64(i * 4 + (uint64_t)k)) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+__global__ void __launch_bounds__(256) k_mlp_samples(int64_t n, uint64_t seed, MLPNorm nm, int nc0, int nc1, int nc2,
+                                                     int nc3, int per, float *__restrict__ xraw,
+                                                     float *__restrict__ xn)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double u[4];
+    if (per > 0) {
+        int64_t c = i / per;
+        const int nc[4] = {nc0, nc1, nc2, nc3};
+        int ci[4];
+        for (int a = 3; a >= 0; a--) {
+            ci[a] = (int)(c % nc[a]);
+            c /= nc[a];
+        }
+        for (int a = 0; a < 4; a++) u[a] = (ci[a] + unit_draw(seed, i, a)) / nc[a];
+    } else {
+        for (int a = 0; a < 4; a++) u[a] = unit_draw(seed, i, a);
+    }
+    for (int a = 0; a < 4; a++) {
+        const float v = (float)(nm.lo[a] + (double)(nm.hi[a] - nm.lo[a]) * u[a]);
+        xraw[i * 4 + a] = v;
+        xn[i * 4 + a] = 2.0f * (v - nm.lo[a]) / (nm.hi[a] - nm.lo[a]) - 1.0f;
+    }
+}
+
+// Per-cell maxima over groups of `per` consecutive samples: |network - table|,
+// |network - exact| and |table - exact|.  Maps hold float bits compared as
+// unsigned ints (valid for non-negative values) and must start at zero.
+__global__ void __launch_bounds__(256) k_emap(int64_t n, int per, const float *__restrict__ yn,
+                                              const float *__restrict__ xraw, const float *__restrict__ exact,
+                                              MLPNorm nm, CoolAxes ax, const float *__restrict__ tabL,
+                                              unsigned int *__restrict__ e_nt, unsigned int *__restrict__ e_ne,
+                                              unsigned int *__restrict__ e_te)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float pred = fmaf(yn[i], nm.sigma, nm.mu);
+    const float v[4] = {xraw[4 * i], xraw[4 * i + 1], xraw[4 * i + 2], xraw[4 * i + 3]};
+    float tl, g;
+    cool_table_interp(ax, tabL, nullptr, v, tl, g);
+    const int64_t c = i / per;
+    atomicMax(&e_nt[c], __float_as_uint(fabsf(pred - tl)));
+    atomicMax(&e_ne[c], __float_as_uint(fabsf(pred - exact[i])));
+    atomicMax(&e_te[c], __float_as_uint(fabsf(tl - exact[i])));
+}
+
+// Gate maps: exact mode keeps the network's error only where the table measured
+// better in the same cell (elsewhere the network is never worse than its fallback).
+__global__ void __launch_bounds__(256) k_gate_map(int64_t ncell, const float *__restrict__ e_ne,
+                                                  const float *__restrict__ e_te, float *__restrict__ gate)
+{
+    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    gate[c] = e_ne[c] > e_te[c] ? e_ne[c] : 0.0f;
+}
+
+namespace sph {
+
+hipError_t mlp_forward_raw(hipStream_t st, int64_t n, const float *x, const MLPDev &d, float *y)
+{
+    k_mlp_forward<false><<<(unsigned)((n + IS - 1) / IS), 64, 0, st>>>(n, x, MLPNorm{}, d.P, d.W2b, d.W3b, y);
+    return hipGetLastError();
+}
+
+hipError_t mlp_infer(hipStream_t st, int64_t n, const float *x, const MLPNorm &nm, const MLPDev &d,
+                     const float *emap, const int *ec, float tol, const CoolAxes &ax, const float *tabL, float *yn,
+                     float *outL, int *flag, unsigned long long *nfall)
+{
+    k_mlp_forward<true><<<(unsigned)((n + IS - 1) / IS), 64, 0, st>>>(n, x, nm, d.P, d.W2b, d.W3b, yn);
+    k_mlp_gate<<<sph_blocks(n, 256), 256, 0, st>>>(n, x, yn, nm, emap, ec[0], ec[1], ec[2], ec[3], tol, ax, tabL, outL,
+                                                   flag, nfall);
+    return hipGetLastError();
+}
+
+hipError_t mlp_train_batch(hipStream_t st, int64_t n, int64_t first, int64_t bsz, int64_t mult, int64_t off,
+                           const float *x, const float *t, const MLPDev &d, float *loss)
+{
+    k_mlp_train<<<(unsigned)((bsz + TS - 1) / TS), 64, 0, st>>>(n, first, bsz, mult, off, x, t, d.P, d.W2b, d.W3b,
+                                                               d.W2tb, d.W3tb, d.G, loss);
+    return hipGetLastError();
+}
+
+hipError_t mlp_adam(hipStream_t st, const MLPDev &d, float lr, float b1, float b2, float eps, int step)
+{
+    const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+    k_adam<<<sph_blocks(MLP_NPARAM, 256), 256, 0, st>>>(MLP_NPARAM, d.P, d.G, d.M, d.V, lr, b1, b2, eps, c1, c2, d.W2b,
+                                                        d.W3b, d.W2tb, d.W3tb);
+    return hipGetLastError();
+}
+
+hipError_t mlp_refresh(hipStream_t st, const MLPDev &d)
+{
+    k_refresh_bf16<<<sph_blocks(MLP_NPARAM, 256), 256, 0, st>>>(MLP_NPARAM, d.P, d.W2b, d.W3b, d.W2tb, d.W3tb);
+    return hipGetLastError();
+}
+
+hipError_t mlp_samples(hipStream_t st, int64_t n, uint64_t seed, const MLPNorm &nm, const int *nc, int per,
+                       float *xraw, float *xn)
+{
+    k_mlp_samples<<<sph_blocks(n, 256), 256, 0, st>>>(n, seed, nm, nc ? nc[0] : 1, nc ? nc[1] : 1, nc ? nc[2] : 1,
+                                                      nc ? nc[3] : 1, per, xraw, xn);
+    return hipGetLastError();
+}
+
+hipError_t mlp_emap(hipStream_t st, int64_t n, int per, const float *yn, const float *xraw, const float *exact,
+                    const MLPNorm &nm, const CoolAxes &ax, const float *tabL, float *e_nt, float *e_ne, float *e_te)
+{
+    k_emap<<<sph_blocks(n, 256), 256, 0, st>>>(n, per, yn, xraw, exact, nm, ax, tabL, (unsigned int *)e_nt,
+                                               (unsigned int *)e_ne, (unsigned int *)e_te);
+    return hipGetLastError();
+}
+
+hipError_t mlp_gate_map(hipStream_t st, int64_t ncell, const float *e_ne, const float *e_te, float *gate)
+{
+    k_gate_map<<<sph_blocks(ncell, 256), 256, 0, st>>>(ncell, e_ne, e_te, gate);
+    return hipGetLastError();
+}
+
+}  // namespace sph

@@ -5,7 +5,7 @@ independent routine fixes -- closed forms, normalisation, symmetry/conservation
 identities, derivative consistency, library routines (np.partition, numpy
 brute force) -- chosen so that a dropped term, a wrong sign, index or
 coefficient fails.  The reference keeps no per-function golden vectors (its
-outputs are git-ignored); these are the pins listed in DESIGN.md.
+outputs are git-ignored), so these checks are what pins each oracle function.
 """
 import numpy as np
 import pytest
