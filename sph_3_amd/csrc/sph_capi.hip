@@ -23,7 +23,7 @@
 
 #define SPH_VERSION 1
 #define SPH_BUCKET 16      // particles per tree bucket (walk leaves)
-#define SPH_HCAP 256       // candidate slots per target in the h solves
+#define SPH_HCAP SPH_HCAP_SLOTS  // candidate slots per target in the h solves
 #define SPH_HROUNDS 64     // maximum radius-adjustment rounds of an h solve
 #define SPH_KEYBITS 48     // Morton key bits (16 per axis)
 
@@ -80,6 +80,8 @@ struct sph_ctx {
     Buf cand, hR, hsteps, hstate, hcount, hout, hin, tx, ty, tz;
     // staging and reductions
     Buf stage, kpart, kout;
+    // spread event counters (sph_common.h): 0-2 h solve, 3-5 gravity
+    Buf spread;
     // cooling: field integrals, table, elementwise scratch
     Buf rf1, ctabL, ctabG, cin[6], cout[2];
     bool rf1_ok = false;
@@ -163,11 +165,12 @@ int capacity(sph_ctx *ctx, int64_t n)
     CK(ensure(ctx, ctx->right, ni));
     CK(ensure(ctx, ctx->parent, 2 * ni));
     CK(ensure(ctx, ctx->flags, ni));
-    CK(ensure(ctx, ctx->front, 2 * ni));
+    CK(ensure(ctx, ctx->front, std::max(2 * ni, (size_t)sph::moments_scratch_entries(n) * 4)));
     CK(ensure(ctx, ctx->groups, 2 * ni));
     CK(ensure(ctx, ctx->off, (size_t)(n + 1) * 8));
     CK(ensure(ctx, ctx->scantmp, (size_t)sph::scan_tmp_entries(n) * 8));
     CK(ensure(ctx, ctx->hcount, 16 * 8));
+    CK(ensure(ctx, ctx->spread, (size_t)6 * SPH_SPREAD_WORDS * 8));
     CK(ensure(ctx, ctx->stage, 3 * nd));
     CK(ensure(ctx, ctx->kpart, 5 * 1024 * 8));
     CK(ensure(ctx, ctx->kout, 8 * 8));
@@ -406,7 +409,7 @@ extern "C" int sph_ctx_destroy(sph_ctx *ctx)
                   &ctx->cnt, &ctx->off, &ctx->scantmp, &ctx->idx, &ctx->rho, &ctx->P, &ctx->c, &ctx->prho2,
                   &ctx->divV, &ctx->curlV, &ctx->bal, &ctx->du, &ctx->cand, &ctx->hR, &ctx->hsteps,
                   &ctx->hstate, &ctx->hcount, &ctx->hout, &ctx->hin, &ctx->tx, &ctx->ty, &ctx->tz,
-                  &ctx->stage, &ctx->kpart, &ctx->kout};
+                  &ctx->stage, &ctx->kpart, &ctx->kout, &ctx->spread};
     for (Buf *b : all)
         if (b->p) (void)hipFree(b->p);
     Buf *cool[] = {&ctx->rf1,   &ctx->ctabL,  &ctx->ctabG,  &ctx->cin[0], &ctx->cin[1], &ctx->cin[2], &ctx->cin[3],
@@ -454,7 +457,8 @@ int run_hsolve(sph_ctx *ctx, int64_t m, const double *tx, const double *ty, cons
     p.R = D(hR);
     p.steps = I32(hsteps);
     p.state = I32(hstate);
-    p.counters = P_<unsigned long long>(ctx->hcount);
+    p.counters = P_<unsigned long long>(ctx->spread);
+    p.totals = P_<unsigned long long>(ctx->hcount);
     int64_t unc = 0;
     p.unconverged_total = &unc;
     p.groups = (tx == D(x)) ? I32(groups) : nullptr;  // targets are the sorted particles: tree-node groups
@@ -689,8 +693,8 @@ extern "C" int sph_gravity(sph_ctx *ctx, int64_t n, const double *pos, const dou
     }
     if (cm || ce) ctx->moments_ok = false;
     CK(ensure_moments(ctx));
-    unsigned long long *cnt = P_<unsigned long long>(ctx->hcount);
-    HIPCK(sph::fill<unsigned long long>(ctx->st, 3, cnt, 0ull));
+    unsigned long long *cnt = P_<unsigned long long>(ctx->spread) + 3 * SPH_SPREAD_WORDS;
+    HIPCK(sph::fill<unsigned long long>(ctx->st, 3 * SPH_SPREAD_WORDS, cnt, 0ull));
     const double *e = ctx->eps_ok ? D(eps) : nullptr;
     const int64_t ng = ctx->ngroups;
     if (precision == SPH_PREC_MIXED)
@@ -700,7 +704,8 @@ extern "C" int sph_gravity(sph_ctx *ctx, int64_t n, const double *pos, const dou
         HIPCK(sph::gravity<double>(ctx->st, ng, I32(groups), D(x), D(y), D(z), D(m), e, P_<WNode>(ctx->nodes),
                                    SPH_BUCKET, theta, G, soft_kind, eps_plummer, D(gx), D(gy), D(gz), cnt));
     unsigned long long hc[3];
-    HIPCK(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, ctx->st));
+    HIPCK(sph::spread_total(ctx->st, 3, cnt, P_<unsigned long long>(ctx->hcount)));
+    HIPCK(hipMemcpyAsync(hc, ctx->hcount.p, sizeof(hc), hipMemcpyDeviceToHost, ctx->st));
     HIPCK(hipStreamSynchronize(ctx->st));
     if (hc[2]) return fail(ctx, SPH_ERR_CAPACITY, "gravity walk stack overflow");
     if (n_interactions) {
@@ -834,12 +839,13 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
                           P_<R>(ctx->P), P_<R>(ctx->c), P_<R>(ctx->prho2)));
     ph_end(ctx, SPH_PH_DENSITY);
     ph_begin(ctx, SPH_PH_GRAVITY);
-    unsigned long long *cnt = P_<unsigned long long>(ctx->hcount) + 4;
-    HIPCK(sph::fill<unsigned long long>(st, 3, cnt, 0ull));
+    unsigned long long *cnt = P_<unsigned long long>(ctx->spread) + 3 * SPH_SPREAD_WORDS;
+    HIPCK(sph::fill<unsigned long long>(st, 3 * SPH_SPREAD_WORDS, cnt, 0ull));
     if (p->G != 0.0) {
         HIPCK(sph::gravity<R>(st, ctx->ngroups, I32(groups), D(x), D(y), D(z), D(m), eps, P_<WNode>(ctx->nodes), SPH_BUCKET,
                               p->theta, p->G, p->soft_kind, p->eps_plummer, D(gx), D(gy), D(gz), cnt));
     }
+    HIPCK(sph::spread_total(st, 3, cnt, P_<unsigned long long>(ctx->hcount) + 4));
     ph_end(ctx, SPH_PH_GRAVITY);
     ph_begin(ctx, SPH_PH_DIVCURL);
     R *bal = p->balsara ? P_<R>(ctx->bal) : nullptr;

@@ -121,3 +121,22 @@ template <> __device__ __forceinline__ double sph_rsqrt<double>(double x) { retu
 
 // Host-side launch helper.
 static inline unsigned sph_blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// Event counters bumped by many wavefronts.  Same-address device atomics
+// serialise (about 10 ns each), so each logical counter k owns SPH_SPREAD
+// slots 64 bytes apart and a workgroup adds into slot blockIdx.x % SPH_SPREAD;
+// sph::spread_total folds the slots.  A spread array of c counters holds
+// c * SPH_SPREAD_WORDS 64-bit words and must be cleared before use.
+#define SPH_SPREAD 64
+#define SPH_SPREAD_PAD 8
+#define SPH_SPREAD_WORDS (SPH_SPREAD * SPH_SPREAD_PAD)
+
+__device__ __forceinline__ void spread_add(unsigned long long *c, int k, unsigned long long v)
+{
+    atomicAdd(c + (size_t)k * SPH_SPREAD_WORDS + (blockIdx.x & (SPH_SPREAD - 1)) * SPH_SPREAD_PAD, v);
+}
+
+namespace sph {
+// out[k] = sum of the slots of counter k, k < ncnt (device pointers, stream-ordered).
+hipError_t spread_total(hipStream_t st, int ncnt, const unsigned long long *c, unsigned long long *out);
+}  // namespace sph

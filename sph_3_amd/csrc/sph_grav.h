@@ -14,8 +14,9 @@ hipError_t gravity_groups(hipStream_t st, int64_t n, const WNode *nodes, const i
 // a_i = G sum m_j f(r_ij) (r_j - r_i) over the group's interaction list (self
 // excluded).  soft_kind 0: spline with eps_ij = (eps_i + eps_j)/2 (node:
 // epsmax), eps may be null (= 0); 1: Plummer with eps_plummer.  theta = 0
-// gives the exact direct sum.  counters[0] += particle-particle,
-// counters[1] += particle-node interactions, counters[2] += stack overflows
+// gives the exact direct sum.  counters: spread array (sph_common.h) of three
+// counters, 0 += particle-particle, 1 += particle-node interactions, 2 += stack
+// overflows
 // (must stay 0).
 template <typename R>
 hipError_t gravity(hipStream_t st, int64_t ngroups, const int32_t *groups, const double *x, const double *y,

@@ -14,7 +14,8 @@
 // stack.  Sources (node monopoles and bucket particles) are staged in LDS
 // relative to the group centre; whenever 64 or more are staged all lanes
 // evaluate them against their own target (LDS broadcast reads, no
-// divergence) with Kahan-compensated sums in the pass type R.  In the FP32
+// divergence): plain FMA sums over each flush of <= 128 staged sources, added
+// to Kahan-compensated totals, in the pass type R.  In the FP32
 // form each relative coordinate is an unevaluated pair hi + lo of floats and
 // a pair difference is (s_hi - t_hi) + (s_lo - t_lo), so near pairs keep full
 // FP32 accuracy.  Every particle of a group sees the same list: the counted
@@ -59,7 +60,7 @@ __device__ __forceinline__ void split(double d, R &hi, R &lo)
 
 template <typename R, int SOFT>
 __device__ __forceinline__ void interact(const Src4<R> &a, const Src4<R> &b, R tx, R ty, R tz, R tlx, R tly, R tlz,
-                                         R epsi, R e2, Kahan<R> &ax, Kahan<R> &ay, Kahan<R> &az)
+                                         R epsi, R e2, R &sx, R &sy, R &sz)
 {
     constexpr bool SPLIT = sizeof(R) == 4;
     R dx = a.x - tx, dy = a.y - ty, dz = a.z - tz;
@@ -77,23 +78,29 @@ __device__ __forceinline__ void interact(const Src4<R> &a, const Src4<R> &b, R t
         w = a.w * grav_fk_spline<R>(sph_sqrt<R>(r2), (R)0.5 * (epsi + b.w));
     }
     w = r2 > (R)0 ? w : (R)0;  // the target itself (and exact coincidences) add nothing
-    ax.add(w * dx);
-    ay.add(w * dy);
-    az.add(w * dz);
+    sx = fma(w, dx, sx);
+    sy = fma(w, dy, sy);
+    sz = fma(w, dz, sz);
 }
 
-// Evaluate cnt staged sources; two independent accumulator sets for ILP.
+// Evaluate cnt (<= GSRC) staged sources.  Two independent chains of plain
+// sums (ILP); each chain's partial over this flush (<= GSRC/2 terms) is then
+// added to its Kahan accumulator, so the list total is compensated at the
+// granularity of one flush.
 template <typename R, int SOFT>
 __device__ __forceinline__ void eval_sources(int cnt, const Src4<R> *__restrict__ A, const Src4<R> *__restrict__ B,
                                              R tx, R ty, R tz, R tlx, R tly, R tlz, R epsi, R e2, Kahan<R> *k)
 {
+    R s[6] = {(R)0, (R)0, (R)0, (R)0, (R)0, (R)0};
     int q = 0;
     for (; q + 1 < cnt; q += 2) {
         const Src4<R> a0 = A[q], b0 = B[q], a1 = A[q + 1], b1 = B[q + 1];
-        interact<R, SOFT>(a0, b0, tx, ty, tz, tlx, tly, tlz, epsi, e2, k[0], k[1], k[2]);
-        interact<R, SOFT>(a1, b1, tx, ty, tz, tlx, tly, tlz, epsi, e2, k[3], k[4], k[5]);
+        interact<R, SOFT>(a0, b0, tx, ty, tz, tlx, tly, tlz, epsi, e2, s[0], s[1], s[2]);
+        interact<R, SOFT>(a1, b1, tx, ty, tz, tlx, tly, tlz, epsi, e2, s[3], s[4], s[5]);
     }
-    if (q < cnt) interact<R, SOFT>(A[q], B[q], tx, ty, tz, tlx, tly, tlz, epsi, e2, k[0], k[1], k[2]);
+    if (q < cnt) interact<R, SOFT>(A[q], B[q], tx, ty, tz, tlx, tly, tlz, epsi, e2, s[0], s[1], s[2]);
+#pragma unroll
+    for (int c = 0; c < 6; c++) k[c].add(s[c]);
 }
 
 template <typename R, int SOFT>
@@ -163,7 +170,7 @@ __global__ void __launch_bounds__(GG) k_gravity_group(const int32_t *__restrict_
                 stk[po] = nright;
                 stk[po + 1] = nleft;
             } else {
-                atomicAdd(&counters[2], 1ull);  // cannot happen for trees shallower than 2*GG levels
+                spread_add(counters, 2, 1ull);  // cannot happen for trees shallower than 2*GG levels
             }
         }
         sp += 2 * (int)__popcll(mo);
@@ -223,22 +230,40 @@ __global__ void __launch_bounds__(GG) k_gravity_group(const int32_t *__restrict_
         az[i] = G * ((double)k[2].value() + (double)k[5].value());
     }
     if (lane == 0) {
-        atomicAdd(&counters[0], (unsigned long long)(n_parts * ng - ng));
-        atomicAdd(&counters[1], (unsigned long long)(n_nodes * ng));
+        spread_add(counters, 0, (unsigned long long)(n_parts * ng - ng));
+        spread_add(counters, 1, (unsigned long long)(n_nodes * ng));
     }
 }
 
 // Groups: node k with count <= gmax whose parent holds more (or the root).
-__global__ void __launch_bounds__(256) k_find_groups(int64_t nn, const WNode *__restrict__ nodes,
-                                                     const int32_t *__restrict__ parent, int gmax,
-                                                     int32_t *__restrict__ groups, unsigned int *__restrict__ ngroups)
+// Compaction with one global atomic per 1024-thread block (wave offsets by
+// ballot, block offsets in LDS): same-address atomics serialise.
+__global__ void __launch_bounds__(1024) k_find_groups(int64_t nn, const WNode *__restrict__ nodes,
+                                                      const int32_t *__restrict__ parent, int gmax,
+                                                      int32_t *__restrict__ groups, unsigned int *__restrict__ ngroups)
 {
-    int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (k >= nn) return;
-    if (nodes[k].count > gmax) return;
-    if (k != 0 && nodes[parent[k]].count <= gmax) return;
-    unsigned int q = atomicAdd(ngroups, 1u);
-    groups[q] = (int32_t)k;
+    __shared__ unsigned int wcount[16], base;
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    bool g = false;
+    if (k < nn && nodes[k].count <= gmax) g = (k == 0) || nodes[parent[k]].count > gmax;
+    const uint64_t m = __ballot(g);
+    if (lane == 0) wcount[wave] = (unsigned int)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+            const unsigned int c = wcount[w];
+            wcount[w] = t;
+            t += c;
+        }
+        base = t ? atomicAdd(ngroups, t) : 0u;
+    }
+    __syncthreads();
+    if (g) {
+        const uint64_t below = lane ? (m & (~0ull >> (64 - lane))) : 0ull;
+        groups[base + wcount[wave] + (unsigned int)__popcll(below)] = (int32_t)k;
+    }
 }
 
 namespace sph {
@@ -247,7 +272,7 @@ hipError_t gravity_groups(hipStream_t st, int64_t n, const WNode *nodes, const i
                           unsigned int *ngroups_dev, int64_t *ngroups)
 {
     fill<unsigned int>(st, 1, ngroups_dev, 0u);
-    k_find_groups<<<sph_blocks(2 * n - 1, 256), 256, 0, st>>>(2 * n - 1, nodes, parent, GG, groups, ngroups_dev);
+    k_find_groups<<<sph_blocks(2 * n - 1, 1024), 1024, 0, st>>>(2 * n - 1, nodes, parent, GG, groups, ngroups_dev);
     unsigned int h = 0;
     hipError_t e = hipMemcpyAsync(&h, ngroups_dev, sizeof(h), hipMemcpyDeviceToHost, st);
     if (e != hipSuccess) return e;

@@ -3,6 +3,9 @@
 #include "sph_common.h"
 #include "sph_order.h"
 
+// Largest candidate capacity per target of the h solves (register slots of the solve).
+#define SPH_HCAP_SLOTS 256
+
 namespace sph {
 
 struct HSolveParams {
@@ -15,13 +18,14 @@ struct HSolveParams {
     double n_ngb, tol;
     int n_low, n_up, max_steps;
     int kth;
-    int cap;                     // candidate slots per target
+    int cap;                     // candidate slots per target (<= SPH_HCAP_SLOTS)
     double r_fac;                // initial radius = 2 * h_in * r_fac
     const double *h_in;          // start values (device)
     double *h_out;               // result (device)
     double *cand, *R;            // scratch: cap*m, m
     int32_t *steps, *state;      // scratch: m, m
-    unsigned long long *counters;  // scratch: 3
+    unsigned long long *counters;  // scratch: spread array of 3 counters (3 * SPH_SPREAD_WORDS words)
+    unsigned long long *totals;    // scratch: 3 words
     int64_t *unconverged_total;  // host accumulator (may be null)
     const int32_t *groups;       // tree-node groups when the targets are the sorted particles, else null
     int64_t ngroups;

@@ -90,7 +90,7 @@ struct HSolveArgs {
     double *h;                 // per target h (in: start, out: result)
     int32_t *steps;            // mode 1 step counter (persists over rounds)
     int32_t *state;            // 0 pending, 1 done, 2 done-unconverged
-    unsigned long long *pending;  // [0] pending, [1] unconverged, [2] walk overflows
+    unsigned long long *pending;  // spread counters: 0 pending, 1 unconverged, 2 walk overflows
     const int32_t *groups;     // tree-node groups of the targets (null: blocks of 64 targets)
     const WNode *gnodes;       // node records the group ids refer to
 };
@@ -157,29 +157,33 @@ __global__ void __launch_bounds__(64) k_hsolve(HSolveArgs a)
                     }
                 }
         });
-    if (ovf && lane == 0) atomicAdd(&a.pending[2], 1ull);
+    if (ovf && lane == 0) spread_add(a.pending, 2, 1ull);
     const bool over = valid && c > cap;
     if (valid && !over)
         for (int k = c & ~7; k < c; k++) row[k] = buf[lane][k & 7];
     __syncthreads();  // rows written by each lane are read by the whole wave below
-    // Solve four targets at a time, 16 lanes per target: lane l16 of the
-    // sub-group holds candidates l16, l16+16, ... (up to 16, in registers) and
-    // the sub-group reduces with butterflies, which give its 16 lanes bitwise
-    // the same sums, so each sub-group's iteration is uniform.
+    // Solve HB targets at a time, HL lanes per target: lane l of the sub-group
+    // holds candidates l, l+HL, ... (up to HS, in registers) and the sub-group
+    // reduces with butterflies, which give its HL lanes bitwise the same sums,
+    // so each sub-group's iteration is uniform.
+    constexpr int HL = 8, HB = 64 / HL, HS = SPH_HCAP_SLOTS / HL;
     double h_res = valid ? a.h[t] : 0.0, R_res = R;
     int st_res = 0;
     if (over) R_res = 0.75 * R;
     uint64_t todo = __ballot(valid && !over);
-    const int sub = lane >> 4, l16 = lane & 15;
+    const int sub = lane / HL, lsub = lane % HL;
     while (todo) {
-        // targets of this batch: the next (up to) four set bits
-        int js[4];
+        // targets of this batch: the next (up to) HB set bits
+        int js[HB];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
+        for (int q = 0; q < HB; q++) {
             js[q] = todo ? __builtin_ctzll(todo) : -1;
             if (todo) todo &= todo - 1;
         }
-        const int j = js[sub];
+        int j = -1;
+#pragma unroll
+        for (int q = 0; q < HB; q++)
+            if (q == sub) j = js[q];
         const int jj = j < 0 ? 0 : j;
         // every lane executes the cross-lane reads (a lane that is switched off
         // at a permute would not supply its value)
@@ -187,59 +191,90 @@ __global__ void __launch_bounds__(64) k_hsolve(HSolveArgs a)
         const double Rj = __shfl(R, jj), hj0 = __shfl(h_res, jj);
         const int cj = j < 0 ? 0 : cjs;
         const double *rowj = a.cand + (first + jj) * (int64_t)cap;
-        const int nsl = (cj + 15) >> 4;  // uniform within the sub-group
-        double r[16];
+        const int nsl = (cj + HL - 1) / HL;  // uniform within the sub-group
+        double r[HS];
 #pragma unroll
-        for (int u = 0; u < 16; u++) {
-            const int k = l16 + 16 * u;
+        for (int u = 0; u < HS; u++) {
+            const int k = lsub + HL * u;
             r[u] = (k < cj) ? rowj[k] : INFINITY;  // squared distances
         }
         double hnew = hj0, Rnew = Rj;
         int st = 0;
         if (j >= 0 && a.mode == 0) {
+            // distances; empty slots get 4 R, beyond the support of every h <= R/2
 #pragma unroll
-            for (int u = 0; u < 16; u++)
-                if (u < nsl) r[u] = (r[u] < INFINITY) ? __builtin_sqrt(r[u]) : INFINITY;
+            for (int u = 0; u < HS; u++)
+                if (u < nsl) r[u] = (r[u] < INFINITY) ? __builtin_sqrt(r[u]) : 4.0 * Rj;
+            // f(h) = (32/3) sum_j w(r_j/h) - n_ngb and f'(h), with the branch-free
+            // cubic spline w(q) = (2-q)_+^3/4 - (1-q)_+^3 and q w'(q) = q (3 (1-q)_+^2 - 3/4 (2-q)_+^2)
             auto feval = [&](double hh, double &f, double &df) {
-                double sl = 0.0, dsl = 0.0;
+                double sl = 0.0, sq = 0.0;
                 const double ih = 1.0 / hh;
 #pragma unroll
-                for (int u = 0; u < 16; u++) {
+                for (int u = 0; u < HS; u++) {
                     if (u >= nsl) break;
-                    double q = r[u] * ih;
-                    if (q < 1.0) {
-                        sl += 1.0 - 1.5 * q * q + 0.75 * q * q * q;
-                        dsl += (-3.0 * q + 2.25 * q * q) * (-q * ih);
-                    } else if (q < 2.0) {
-                        double w = 2.0 - q;
-                        sl += 0.25 * w * w * w;
-                        dsl += (-0.75 * w * w) * (-q * ih);
-                    }
+                    const double q = r[u] * ih;
+                    const double aa = fmax(2.0 - q, 0.0), bb = fmax(1.0 - q, 0.0);
+                    const double a2 = aa * aa, b2 = bb * bb;
+                    sl = fma(0.25 * a2, aa, fma(-b2, bb, sl));
+                    sq = fma(fma(3.0, b2, -0.75 * a2), q, sq);
                 }
-                for (int off = 8; off >= 1; off >>= 1) {
-                    sl += __shfl_xor(sl, off, 16);
-                    dsl += __shfl_xor(dsl, off, 16);
+                for (int off = HL / 2; off >= 1; off >>= 1) {
+                    sl += __shfl_xor(sl, off, HL);
+                    sq += __shfl_xor(sq, off, HL);
                 }
                 f = (32.0 / 3.0) * sl - a.n_ngb;
-                df = (32.0 / 3.0) * dsl;
+                df = -(32.0 / 3.0) * sq * ih;
             };
-            double hmaxv = 0.5 * Rj, f, df;
-            feval(hmaxv, f, df);
-            if (f < 0.0) {
-                Rnew = 1.5 * Rj;
-            } else {
-                double lo = 0.0, hi = hmaxv, hh = hj0;
-                if (!(hh > 0.0 && hh < hi)) hh = 0.5 * hi;
-                st = 2;
-                for (int it = 0; it < 100; it++) {
-                    feval(hh, f, df);
-                    if (f < 0.0) lo = hh; else hi = hh;
-                    double hn = (df > 0.0) ? hh - f / df : -1.0;
-                    if (!(hn > lo && hn < hi)) hn = 0.5 * (lo + hi);
-                    double dh = fabs(hn - hh);
-                    hh = hn;
-                    if (dh <= a.tol * hh) { st = 1; break; }
+            // Safeguarded Newton from the previous h on the bracket (0, R/2]: the
+            // candidates (r < R) are complete for every h <= R/2.  f(R/2) is only
+            // evaluated when no iterate has shown f >= 0 and Newton leaves the
+            // bracket upwards; f(R/2) < 0 means the root lies beyond: grow R.
+            // Stop after a Newton step dh <= sqrt(tol/10) h: in the quadratic
+            // regime the error after it is K dh^2 <= tol h for curvature
+            // K = f''/(2 f') <= 10/h (K = 1/h for a uniform density), or after
+            // any step dh <= tol h.
+            const double hmaxv = 0.5 * Rj, qtol = __builtin_sqrt(0.1 * a.tol);
+            double lo = 0.0, hi = hmaxv, hh = hj0, f, df;
+            if (!(hh > 0.0 && hh < hi)) hh = 0.5 * hi;
+            bool hik = false, grow = false;
+            st = 2;
+            for (int it = 0; it < 100; it++) {
+                feval(hh, f, df);
+                if (f < 0.0) lo = hh;
+                else {
+                    hi = hh;
+                    hik = true;
                 }
+                double hn = (df > 0.0) ? hh - f / df : -1.0;
+                bool newton = true;
+                if (!(hn > lo && hn < hi)) {
+                    if (!hik) {
+                        double fm, dfm;
+                        feval(hmaxv, fm, dfm);
+                        if (fm < 0.0) {
+                            grow = true;
+                            break;
+                        }
+                        hik = true;
+                        hn = (dfm > 0.0) ? hmaxv - fm / dfm : -1.0;
+                        if (!(hn > lo && hn < hi)) hn = 0.5 * (lo + hi);
+                    } else {
+                        hn = 0.5 * (lo + hi);
+                    }
+                    newton = false;
+                }
+                const double dh = fabs(hn - hh);
+                hh = hn;
+                if (dh <= a.tol * hh || (newton && dh <= qtol * hh)) {
+                    st = 1;
+                    break;
+                }
+            }
+            if (grow) {
+                Rnew = 1.5 * Rj;
+                st = 0;
+            } else {
                 hnew = hh;
             }
         } else if (j >= 0) {
@@ -253,13 +288,13 @@ __global__ void __launch_bounds__(64) k_hsolve(HSolveArgs a)
                 for (;;) {
                     double mn = INFINITY;
 #pragma unroll
-                    for (int u = 0; u < 16; u++)
+                    for (int u = 0; u < HS; u++)
                         if (r[u] > cur && r[u] < mn) mn = r[u];
-                    for (int off = 8; off >= 1; off >>= 1) mn = fmin(mn, __shfl_xor(mn, off, 16));
+                    for (int off = HL / 2; off >= 1; off >>= 1) mn = fmin(mn, __shfl_xor(mn, off, HL));
                     int mult = 0;
 #pragma unroll
-                    for (int u = 0; u < 16; u++) mult += (r[u] == mn);
-                    for (int off = 8; off >= 1; off >>= 1) mult += __shfl_xor(mult, off, 16);
+                    for (int u = 0; u < HS; u++) mult += (r[u] == mn);
+                    for (int off = HL / 2; off >= 1; off >>= 1) mult += __shfl_xor(mult, off, HL);
                     if (taken + mult >= need) { v = mn; break; }
                     taken += mult;
                     cur = mn;
@@ -270,9 +305,9 @@ __global__ void __launch_bounds__(64) k_hsolve(HSolveArgs a)
         }
         // hand each sub-group's result to its target lane
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const double hq = __shfl(hnew, 16 * q), Rq = __shfl(Rnew, 16 * q);
-            const int sq = __shfl(st, 16 * q);
+        for (int q = 0; q < HB; q++) {
+            const double hq = __shfl(hnew, HL * q), Rq = __shfl(Rnew, HL * q);
+            const int sq = __shfl(st, HL * q);
             if (js[q] >= 0 && lane == js[q]) {
                 h_res = hq;
                 R_res = Rq;
@@ -284,8 +319,8 @@ __global__ void __launch_bounds__(64) k_hsolve(HSolveArgs a)
     a.h[t] = h_res;
     a.R[t] = R_res;
     a.state[t] = st_res;
-    if (st_res == 0) atomicAdd(&a.pending[0], 1ull);
-    if (st_res == 2) atomicAdd(&a.pending[1], 1ull);
+    if (st_res == 0) spread_add(a.pending, 0, 1ull);
+    if (st_res == 2) spread_add(a.pending, 1, 1ull);
 }
 
 // mode 1: the reference's count stepping, exactly (libsx.py:804-845): each
@@ -325,7 +360,7 @@ __global__ void __launch_bounds__(256) k_hstep(HSolveArgs a)
     a.h[t] = hh;
     a.steps[t] = steps;
     a.state[t] = st;
-    if (st == 2) atomicAdd(&a.pending[1], 1ull);
+    if (st == 2) spread_add(a.pending, 1, 1ull);
 }
 
 __global__ void __launch_bounds__(256) k_hsolve_init(int64_t m, const double *__restrict__ h0, double fac,
@@ -410,13 +445,14 @@ hipError_t hsolve(hipStream_t st, const HSolveParams &p, int max_rounds, int64_t
     a.groups = p.groups;
     a.gnodes = p.nodes;
     if (p.m <= 0) { *n_unconverged = 0; return hipSuccess; }
-    if (p.bucket > 64) return hipErrorInvalidValue;
+    if (p.bucket > 64 || p.cap > SPH_HCAP_SLOTS || p.cap < 1) return hipErrorInvalidValue;
     k_hsolve_init<<<sph_blocks(p.m, 256), 256, 0, st>>>(p.m, p.h_in, p.r_fac, p.R, p.h_out, p.steps, p.state);
     unsigned long long hc[3] = {0, 0, 0};
     if (p.mode == 1) {
-        fill<unsigned long long>(st, 3, p.counters, 0ull);
+        fill<unsigned long long>(st, 3 * SPH_SPREAD_WORDS, p.counters, 0ull);
         k_hstep<<<sph_blocks(p.m, 256), 256, 0, st>>>(a);
-        hipError_t e = hipMemcpyAsync(hc, p.counters, sizeof(hc), hipMemcpyDeviceToHost, st);
+        spread_total(st, 3, p.counters, p.totals);
+        hipError_t e = hipMemcpyAsync(hc, p.totals, sizeof(hc), hipMemcpyDeviceToHost, st);
         if (e != hipSuccess) return e;
         e = hipStreamSynchronize(st);
         if (e != hipSuccess) return e;
@@ -426,9 +462,10 @@ hipError_t hsolve(hipStream_t st, const HSolveParams &p, int max_rounds, int64_t
     }
     const int64_t nblk = p.groups ? p.ngroups : (p.m + 63) / 64;
     for (int r = 0; r < max_rounds; r++) {
-        fill<unsigned long long>(st, 3, p.counters, 0ull);
+        fill<unsigned long long>(st, 3 * SPH_SPREAD_WORDS, p.counters, 0ull);
         k_hsolve<<<(unsigned)nblk, 64, 0, st>>>(a);
-        hipError_t e = hipMemcpyAsync(hc, p.counters, sizeof(hc), hipMemcpyDeviceToHost, st);
+        spread_total(st, 3, p.counters, p.totals);
+        hipError_t e = hipMemcpyAsync(hc, p.totals, sizeof(hc), hipMemcpyDeviceToHost, st);
         if (e != hipSuccess) return e;
         e = hipStreamSynchronize(st);
         if (e != hipSuccess) return e;

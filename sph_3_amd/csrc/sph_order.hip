@@ -432,6 +432,23 @@ hipError_t compose(hipStream_t st, int64_t n, const int32_t *a, const int32_t *b
     return hipGetLastError();
 }
 
+// One wavefront per counter: lane l sums slot l (SPH_SPREAD == 64).
+__global__ void __launch_bounds__(64) k_spread_total(const unsigned long long *__restrict__ c,
+                                                     unsigned long long *__restrict__ out)
+{
+    static_assert(SPH_SPREAD == 64, "one lane per slot");
+    unsigned long long v = c[(size_t)blockIdx.x * SPH_SPREAD_WORDS + threadIdx.x * SPH_SPREAD_PAD];
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    if (threadIdx.x == 0) out[blockIdx.x] = v;
+}
+
+hipError_t spread_total(hipStream_t st, int ncnt, const unsigned long long *c, unsigned long long *out)
+{
+    if (ncnt <= 0) return hipSuccess;
+    k_spread_total<<<ncnt, 64, 0, st>>>(c, out);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t fill(hipStream_t st, int64_t n, T *a, T v)
 {

@@ -117,17 +117,38 @@ __device__ __forceinline__ void combine(WNode *__restrict__ nodes, int64_t p, in
 }
 
 // Bottom-up moments without device-scope fences.  Device-wide release
-// fences write back the L2 of every XCD and cost microseconds each, so the
-// climb is split in two:
+// fences write back the L2 of every XCD and cost microseconds each, and
+// same-address device atomics serialise, so the climb runs in levels whose
+// workgroups each own a contiguous particle range:
 //  (1) k_moments_local: one thread per leaf writes the leaf record and climbs
-//      while the parent's particle range lies inside this workgroup's leaf
-//      range; the second child to arrive combines (workgroup-scope atomics).
-//      A finished node whose parent spans workgroups is appended to the
-//      frontier list.
-//  (2) k_moments_top: one workgroup climbs from every frontier node the same
-//      way; all remaining (spanning) nodes are combined inside that single
-//      workgroup, so workgroup scope suffices again.
+//      while the parent's particle range lies inside this workgroup's
+//      MOM_BLOCK particles; the second child to arrive combines
+//      (workgroup-scope atomics on the arrival counters).  A finished node
+//      whose parent leaves the range joins the workgroup's frontier, kept in
+//      LDS and written to the block's own region of MOM_FMAX slots.
+//  (2) k_moments_up: one workgroup per MOM_FAN level-1 blocks climbs from
+//      their frontier nodes the same way inside MOM_BLOCK*MOM_FAN particles,
+//      and a last single workgroup climbs from the remaining frontier to the
+//      root.
+// A block's frontier nodes are the maximal subtrees inside its range: at
+// most two per tree level, and the radix tree has at most 97 levels (64 key
+// plus 32 index bits of common prefix), so 194 < MOM_FMAX slots suffice.
 #define MOM_BLOCK 1024
+#define MOM_FAN 64
+#define MOM_FMAX 256
+
+__device__ __forceinline__ void front_push(int32_t *flist, unsigned int *fn, int32_t node)
+{
+    const unsigned int k = atomicAdd(fn, 1u);
+    if (k < MOM_FMAX) flist[k] = node;
+}
+
+__device__ __forceinline__ void front_flush(int32_t *flist, unsigned int fn, int32_t *reg, int32_t *cnt, int64_t blk)
+{
+    const int c = fn < MOM_FMAX ? (int)fn : MOM_FMAX;
+    for (int t = threadIdx.x; t < c; t += blockDim.x) reg[blk * MOM_FMAX + t] = flist[t];
+    if (threadIdx.x == 0) cnt[blk] = c;
+}
 
 __global__ void __launch_bounds__(MOM_BLOCK) k_moments_local(int64_t n, const double *__restrict__ x,
                                                              const double *__restrict__ y,
@@ -139,56 +160,100 @@ __global__ void __launch_bounds__(MOM_BLOCK) k_moments_local(int64_t n, const do
                                                              const int32_t *__restrict__ right,
                                                              const int32_t *__restrict__ parent,
                                                              uint32_t *__restrict__ flags,
-                                                             int32_t *__restrict__ frontier,
-                                                             uint32_t *__restrict__ nfrontier)
+                                                             int32_t *__restrict__ reg, int32_t *__restrict__ cnt)
 {
-    int64_t s = blockIdx.x * (int64_t)MOM_BLOCK + threadIdx.x;
-    if (s >= n) return;
-    int64_t nl = n - 1;
-    int64_t id = nl + s;
-    {
-        WNode &L = nodes[id];
-        double px = x[s], py = y[s], pz = z[s];
-        L.lx = L.hx = L.cx = px;
-        L.ly = L.hy = L.cy = py;
-        L.lz = L.hz = L.cz = pz;
-        L.mass = m ? m[s] : 0.0;
-        L.size = 0.0;
-        L.hmax = h ? h[s] : 0.0;
-        L.epsmax = eps ? eps[s] : 0.0;
-    }
-    if (n < 2) return;
-    int64_t p = parent[id];
-    while (p >= 0) {
-        const int64_t f = nodes[p].first, last = f + nodes[p].count - 1;
-        if (f / MOM_BLOCK != last / MOM_BLOCK) {  // parent spans workgroups: hand over
-            uint32_t k = atomicAdd(nfrontier, 1u);
-            frontier[k] = (int32_t)id;
-            return;
+    __shared__ int32_t flist[MOM_FMAX];
+    __shared__ unsigned int fn;
+    if (threadIdx.x == 0) fn = 0;
+    __syncthreads();
+    const int64_t s = blockIdx.x * (int64_t)MOM_BLOCK + threadIdx.x;
+    if (s < n) {
+        const int64_t nl = n - 1;
+        int64_t id = nl + s;
+        {
+            WNode &L = nodes[id];
+            const double px = x[s], py = y[s], pz = z[s];
+            L.lx = L.hx = L.cx = px;
+            L.ly = L.hy = L.cy = py;
+            L.lz = L.hz = L.cz = pz;
+            L.mass = m ? m[s] : 0.0;
+            L.size = 0.0;
+            L.hmax = h ? h[s] : 0.0;
+            L.epsmax = eps ? eps[s] : 0.0;
         }
-        uint32_t old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (old == 0) return;  // first arrival: the sibling finishes this node
-        combine(nodes, p, nodes[p].left, right[p]);
-        id = p;
-        p = parent[p];
-    }
-}
-
-__global__ void __launch_bounds__(1024) k_moments_top(WNode *__restrict__ nodes, const int32_t *__restrict__ right,
-                                                     const int32_t *__restrict__ parent,
-                                                     uint32_t *__restrict__ flags,
-                                                     const int32_t *__restrict__ frontier,
-                                                     const uint32_t *__restrict__ nfrontier)
-{
-    const uint32_t nf = *nfrontier;
-    for (uint32_t k = threadIdx.x; k < nf; k += blockDim.x) {
-        int64_t p = parent[frontier[k]];
+        int64_t p = n < 2 ? -1 : parent[id];
         while (p >= 0) {
-            uint32_t old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (old == 0) break;
+            const int64_t f = nodes[p].first, last = f + nodes[p].count - 1;
+            if (f / MOM_BLOCK != last / MOM_BLOCK) {  // parent leaves the block: hand over
+                front_push(flist, &fn, (int32_t)id);
+                break;
+            }
+            const uint32_t old =
+                __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old == 0) break;  // first arrival: the sibling finishes this node
             combine(nodes, p, nodes[p].left, right[p]);
+            id = p;
             p = parent[p];
         }
+    }
+    __syncthreads();
+    front_flush(flist, fn, reg, cnt, blockIdx.x);
+}
+
+// Climb from the frontier regions of `fan` consecutive blocks of the level
+// below (region b: reg_in[b*MOM_FMAX + i], i < cnt_in[b]).  span > 0: stay
+// inside this workgroup's span particles and hand the rest to its region of
+// reg_out/cnt_out; span <= 0: climb to the root (single workgroup).
+__global__ void __launch_bounds__(1024) k_moments_up(int64_t nblk_in, int fan, int64_t span,
+                                                     WNode *__restrict__ nodes, const int32_t *__restrict__ right,
+                                                     const int32_t *__restrict__ parent,
+                                                     uint32_t *__restrict__ flags, const int32_t *__restrict__ reg_in,
+                                                     const int32_t *__restrict__ cnt_in, int32_t *__restrict__ reg_out,
+                                                     int32_t *__restrict__ cnt_out)
+{
+    __shared__ int32_t pre[1025];
+    __shared__ int32_t flist[MOM_FMAX];
+    __shared__ unsigned int fn;
+    const int64_t b0 = blockIdx.x * (int64_t)fan;
+    const int nb = (int)((nblk_in - b0) < fan ? (nblk_in - b0) : fan);
+    if (threadIdx.x == 0) {
+        fn = 0;
+        pre[0] = 0;
+    }
+    if ((int)threadIdx.x < nb) pre[threadIdx.x + 1] = cnt_in[b0 + threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int j = 0; j < nb; j++) pre[j + 1] += pre[j];
+    __syncthreads();
+    const int T = pre[nb];
+    for (int e = threadIdx.x; e < T; e += blockDim.x) {
+        int lo = 0, hi = nb;  // pre[lo] <= e < pre[lo + 1]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (pre[mid] <= e) lo = mid;
+            else hi = mid;
+        }
+        int64_t id = reg_in[(b0 + lo) * MOM_FMAX + (e - pre[lo])];
+        int64_t p = parent[id];
+        while (p >= 0) {
+            if (span > 0) {
+                const int64_t f = nodes[p].first, last = f + nodes[p].count - 1;
+                if (f / span != last / span) {
+                    front_push(flist, &fn, (int32_t)id);
+                    break;
+                }
+            }
+            const uint32_t old =
+                __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old == 0) break;
+            combine(nodes, p, nodes[p].left, right[p]);
+            id = p;
+            p = parent[p];
+        }
+    }
+    if (span > 0) {
+        __syncthreads();
+        front_flush(flist, fn, reg_out, cnt_out, blockIdx.x);
     }
 }
 
@@ -202,16 +267,30 @@ hipError_t tree_topology(hipStream_t st, int64_t n, const uint64_t *skeys, WNode
     return hipGetLastError();
 }
 
+int64_t moments_scratch_entries(int64_t n)
+{
+    const int64_t b1 = (n + MOM_BLOCK - 1) / MOM_BLOCK, b2 = (b1 + MOM_FAN - 1) / MOM_FAN;
+    return (b1 + b2) * (MOM_FMAX + 1);
+}
+
 hipError_t tree_moments(hipStream_t st, int64_t n, const double *x, const double *y, const double *z,
                         const double *m, const double *h, const double *eps, WNode *nodes,
                         const int32_t *right, const int32_t *parent, uint32_t *flags, int32_t *frontier)
 {
-    // flags[0..n-2]: arrival counters; flags[n-1]: frontier length
-    fill<uint32_t>(st, n, flags, 0u);
-    uint32_t *nf = flags + (n - 1);
-    k_moments_local<<<sph_blocks(n, MOM_BLOCK), MOM_BLOCK, 0, st>>>(n, x, y, z, m, h, eps, nodes, right, parent,
-                                                                   flags, frontier, nf);
-    if (n > MOM_BLOCK) k_moments_top<<<1, 1024, 0, st>>>(nodes, right, parent, flags, frontier, nf);
+    // flags[0..n-2]: arrival counters of the internal nodes
+    const int64_t b1 = (n + MOM_BLOCK - 1) / MOM_BLOCK, b2 = (b1 + MOM_FAN - 1) / MOM_FAN;
+    if (b2 > 1024) return hipErrorInvalidValue;  // the last level's prefix table holds 1024 regions
+    int32_t *reg1 = frontier, *reg2 = reg1 + b1 * MOM_FMAX, *cnt1 = reg2 + b2 * MOM_FMAX, *cnt2 = cnt1 + b1;
+    if (n > 1) fill<uint32_t>(st, n - 1, flags, 0u);
+    k_moments_local<<<(unsigned)b1, MOM_BLOCK, 0, st>>>(n, x, y, z, m, h, eps, nodes, right, parent, flags, reg1,
+                                                        cnt1);
+    if (b1 > 1) {
+        k_moments_up<<<(unsigned)b2, 1024, 0, st>>>(b1, MOM_FAN, (int64_t)MOM_BLOCK * MOM_FAN, nodes, right, parent,
+                                                    flags, reg1, cnt1, reg2, cnt2);
+        if (b2 > 1)
+            k_moments_up<<<1, 1024, 0, st>>>(b2, (int)b2, 0, nodes, right, parent, flags, reg2, cnt2, nullptr,
+                                             nullptr);
+    }
     return hipGetLastError();
 }
 
