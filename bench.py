@@ -26,7 +26,13 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
 FLOP_PER_INTERACTION = 20
+# h solve (FP64): a walk distance test is 3 sub + 3 mul + 2 add; a Newton kernel
+# term is q = r/h (1), (2-q)+, (1-q)+ (4), squares (2), w (1 mul + 2 fma) and
+# q w'(q) (1 mul + 2 fma) = 16 flop (fma = 2)
+H_FLOP_PER_TEST = 8
+H_FLOP_PER_TERM = 16
 
 
 def log(msg):
@@ -44,8 +50,9 @@ def per_particle_bytes(kbar):
         "moments": 24 + 8 + 8 + 2 * 128,                     # x, m, h in; ~2 node records out
         "hsolve": 24 + 8 + 8 + 2 * 128,                      # x, h in, h out, tree read once
         "ngb": 24 + 8 + 2 * 128 + idx,                       # x, h, tree in; CSR out
-        "density": idx + 24 + 8 + 8 + 8 + 4 * 4,            # x,m,h,u in; rho,P,c,P/rho^2 out
-        "divcurl": idx + 24 + 24 + 8 + 8 + 4 + 4 + 3 * 4,    # x,v,m,h,rho,c in; divV,curlV,f out
+        # density fused with div/curl v (Balsara form): x,v,m,h,u in; rho,P,c,P/rho^2,divV,curlV,f out
+        "density": idx + 24 + 24 + 8 + 8 + 8 + 4 * 4 + 3 * 4,
+        "divcurl": idx + 24 + 24 + 8 + 8 + 4 + 4 + 3 * 4,    # separate pass (libsx form): x,v,m,h,rho,c in
         "accsph": idx + 24 + 24 + 8 + 8 + 4 * 4 + 24 + 24,   # x,v,m,h,rho,P/rho^2,c,f,a_grav in; a out
         "dudt": idx + 24 + 24 + 8 + 8 + 4 * 4 + 4,           # ... in; du out
         "kick_drift": 6 * 8 * 2 + 3 * 8 + 3 * 8,
@@ -54,10 +61,10 @@ def per_particle_bytes(kbar):
     }
 
 
-KERNEL_OF_PHASE = {"order": "k_rs_scatter+k_gather", "moments": "k_moments_local+k_moments_top",
-                   "hsolve": "k_hsolve", "ngb": "k_ngb<false>+k_ngb<true>", "density": "k_density<float,16>",
+KERNEL_OF_PHASE = {"order": "k_rs_scatter+k_gather", "moments": "k_moments_local+k_moments_up",
+                   "hsolve": "k_hsolve", "ngb": "k_ngb<2>", "density": "k_density_divcurl<float,16>",
                    "divcurl": "k_divcurl<float,16>", "accsph": "k_accsph<float,16>", "dudt": "k_dudt<float,16>",
-                   "gravity": "k_gravity<float>"}
+                   "gravity": "k_gravity_group<float,1>", "moments_h": "k_moments_local+k_moments_up"}
 
 
 def cpu_baseline(sim_state, n, m, rows, seed=0):
@@ -161,11 +168,21 @@ def main():
     ms_step = 1e3 * dt / args.steps
     value = n * args.steps / dt
     phases = {k: v / max(1, st["phase_calls"][k]) for k, v in st["phase_ms"].items() if st["phase_calls"][k]}
+    per_step = {k: st["phase_ms"][k] / args.steps for k in phases}
     kbar = st["ngb_total"] / n
     bytes_pp = per_particle_bytes(kbar)
-    # dominant kernel among the measured phases
-    dom = max(phases, key=phases.get)
-    if dom == "gravity":
+    bytes_pp["moments_h"] = bytes_pp["moments"]
+    # dominant kernel: the phase with the most time per step
+    dom = max(per_step, key=per_step.get)
+    h_flops = H_FLOP_PER_TEST * st["h_tests"] + H_FLOP_PER_TERM * st["h_terms"]
+    if dom == "hsolve":
+        ach = h_flops / (phases[dom] * 1e-3) / 1e12
+        roof = {"kernel": "k_hsolve", "bound": "valu_fp64", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
+                "algorithmic": f"{H_FLOP_PER_TEST} flop x {st['h_tests']} walk distance tests + "
+                               f"{H_FLOP_PER_TERM} flop x {st['h_terms']} Newton kernel terms",
+                "ms_per_launch": phases[dom]}
+    elif dom == "gravity":
         flops = FLOP_PER_INTERACTION * (st["grav_pp"] + st["grav_pn"])
         ach = flops / (phases[dom] * 1e-3) / 1e12
         roof = {"kernel": KERNEL_OF_PHASE[dom], "bound": "valu", "achieved": ach, "peak": FP32_PEAK_TFLOPS,
@@ -197,6 +214,8 @@ def main():
                    "gamma": 5 / 3, "n_ngb": 64, "theta": 0.5, "softening": "plummer", "eps": 1e-3,
                    "alpha": 1.0, "balsara": True, "dt": 1e-3},
         "phases_ms": phases,
+        "phase_ms_per_step": per_step,
+        "h_solve_fp64_tflops": h_flops / (phases["hsolve"] * 1e-3) / 1e12 if "hsolve" in phases else None,
         "neighbours_per_particle": kbar,
         "h_solve_rounds_last_step": st["h_rounds"],
         "gravity_interactions_per_particle": (st["grav_pp"] + st["grav_pn"]) / n,

@@ -214,13 +214,16 @@ typedef struct sph_stats {
     int64_t h_rounds;        /* rounds of the last h solve */
     double phase_ms[16];     /* accumulated HIP-event milliseconds per phase */
     int64_t phase_calls[16];
+    int64_t h_tests;         /* last h solve: FP64 distance tests of its walks */
+    int64_t h_terms;         /* last h solve: kernel terms summed by its Newton iterations */
 } sph_stats;
 
-/* phase indices of sph_stats.phase_ms */
+/* phase indices of sph_stats.phase_ms (SPH_PH_HMOMENTS: the node boxes built for the h walk;
+ * SPH_PH_DIVCURL stays empty in the Balsara form, whose div/curl v share the density pass) */
 enum sph_phase {
     SPH_PH_KICKDRIFT = 0, SPH_PH_ORDER, SPH_PH_TOPOLOGY, SPH_PH_HSOLVE, SPH_PH_MOMENTS, SPH_PH_NGB,
     SPH_PH_DENSITY, SPH_PH_GRAVITY, SPH_PH_DIVCURL, SPH_PH_ACCSPH, SPH_PH_KICK, SPH_PH_DUDT,
-    SPH_PH_UUPDATE, SPH_PH_COUNT
+    SPH_PH_UUPDATE, SPH_PH_HMOMENTS, SPH_PH_COUNT
 };
 
 /* Load the resident state (caller order); u and h are per particle. */

@@ -21,7 +21,7 @@ SOFT = {"spline": 0, "plummer": 1}
 H_NEWTON, H_REF_COUNT, H_KTH = 0, 1, 2
 
 PHASES = ["kick_drift", "order", "topology", "hsolve", "moments", "ngb", "density", "gravity", "divcurl",
-          "accsph", "kick", "dudt", "u_update"]
+          "accsph", "kick", "dudt", "u_update", "moments_h"]
 
 _d = ctypes.POINTER(ctypes.c_double)
 _f = ctypes.POINTER(ctypes.c_float)
@@ -42,7 +42,7 @@ class Stats(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int64), ("steps", ctypes.c_int64), ("ngb_total", ctypes.c_int64),
                 ("grav_pp", ctypes.c_int64), ("grav_pn", ctypes.c_int64), ("h_unconverged", ctypes.c_int64),
                 ("h_rounds", ctypes.c_int64), ("phase_ms", ctypes.c_double * 16),
-                ("phase_calls", ctypes.c_int64 * 16)]
+                ("phase_calls", ctypes.c_int64 * 16), ("h_tests", ctypes.c_int64), ("h_terms", ctypes.c_int64)]
 
 
 _SIGS = {

@@ -26,6 +26,11 @@
 #define SPH_HCAP SPH_HCAP_SLOTS  // candidate slots per target in the h solves
 #define SPH_HROUNDS 64     // maximum radius-adjustment rounds of an h solve
 #define SPH_KEYBITS 48     // Morton key bits (16 per axis)
+#define SPR_H 0            // spread counters of the h solve (5)
+#define SPR_G 5            // spread counters of the gravity walk (3)
+#define SPR_N 8            // spread counters of the padded neighbour rows (2)
+#define SPR_TOTAL 10
+#define SPH_NGB_STRIDE 128 // entries per padded neighbour row
 
 namespace {
 
@@ -80,7 +85,9 @@ struct sph_ctx {
     Buf cand, hR, hsteps, hstate, hcount, hout, hin, tx, ty, tz;
     // staging and reductions
     Buf stage, kpart, kout;
-    // spread event counters (sph_common.h): 0-2 h solve, 3-5 gravity
+    // spread event counters (sph_common.h): SPR_H.. h solve (5), SPR_G.. gravity (3);
+    // their totals land in hcount: [0..4] h solve, [5..7] gravity, [8] walk overflow, [9] group count,
+    // [10..11] padded neighbour rows
     Buf spread;
     // cooling: field integrals, table, elementwise scratch
     Buf rf1, ctabL, ctabG, cin[6], cout[2];
@@ -99,12 +106,14 @@ struct sph_ctx {
     bool order_ok = false, h_ok = false, m_ok = false, eps_ok = false, moments_ok = false, csr_ok = false;
     bool resident = false, primed = false;
     int64_t csr_total = 0;
+    int ngb_stride = 0;  // layout of the neighbour sets: 0 CSR, else padded rows of this width
     int64_t ngroups = 0;  // tree-node target groups of the current tree
 
     // statistics
     sph_stats stats;
     hipEvent_t ev[2 * SPH_PH_COUNT];
     bool timing = false;
+    uint32_t ph_rec = 0;  // phases recorded since the last finish_step
     int rho_prec = SPH_PREC_MIXED;  // arithmetic type of the resident rho buffer
 };
 
@@ -170,7 +179,7 @@ int capacity(sph_ctx *ctx, int64_t n)
     CK(ensure(ctx, ctx->off, (size_t)(n + 1) * 8));
     CK(ensure(ctx, ctx->scantmp, (size_t)sph::scan_tmp_entries(n) * 8));
     CK(ensure(ctx, ctx->hcount, 16 * 8));
-    CK(ensure(ctx, ctx->spread, (size_t)6 * SPH_SPREAD_WORDS * 8));
+    CK(ensure(ctx, ctx->spread, (size_t)SPR_TOTAL * SPH_SPREAD_WORDS * 8));
     CK(ensure(ctx, ctx->stage, 3 * nd));
     CK(ensure(ctx, ctx->kpart, 5 * 1024 * 8));
     CK(ensure(ctx, ctx->kout, 8 * 8));
@@ -309,11 +318,29 @@ int ensure_moments(sph_ctx *ctx)
     return SPH_OK;
 }
 
-int build_csr(sph_ctx *ctx)
+// Neighbour sets of the current positions and h: padded rows of
+// SPH_NGB_STRIDE entries built in one walk, or (csr, or a row longer than the
+// stride) the exact two-pass CSR.
+int build_ngb(sph_ctx *ctx, bool csr)
 {
-    int64_t n = ctx->n, total = 0;
-    int64_t cap = ctx->idx.p ? (int64_t)(ctx->idx.bytes / 4) : 0;
+    const int64_t n = ctx->n;
     unsigned long long *ovf = P_<unsigned long long>(ctx->hcount) + 8;
+    if (!csr) {
+        CK(ensure(ctx, ctx->idx, (size_t)n * SPH_NGB_STRIDE * 4));
+        int64_t nlong = 0, total = 0;
+        HIPCK(sph::neighbour_rows(ctx->st, ctx->ngroups, I32(groups), D(x), D(y), D(z), D(h), P_<WNode>(ctx->nodes),
+                                  SPH_BUCKET, SPH_NGB_STRIDE, I32(cnt), I32(idx), ovf,
+                                  P_<unsigned long long>(ctx->spread) + SPR_N * SPH_SPREAD_WORDS,
+                                  P_<unsigned long long>(ctx->hcount) + 10, &nlong, &total));
+        if (nlong == 0) {
+            ctx->ngb_stride = SPH_NGB_STRIDE;
+            ctx->csr_total = total;
+            ctx->stats.ngb_total = total;
+            return SPH_OK;
+        }
+    }
+    int64_t total = 0;
+    int64_t cap = ctx->idx.p ? (int64_t)(ctx->idx.bytes / 4) : 0;
     HIPCK(sph::neighbours(ctx->st, n, ctx->ngroups, I32(groups), D(x), D(y), D(z), D(h), P_<WNode>(ctx->nodes),
                           SPH_BUCKET, I32(cnt), P_<int64_t>(ctx->off), P_<int64_t>(ctx->scantmp), ovf, I32(idx), cap,
                           &total));
@@ -322,16 +349,23 @@ int build_csr(sph_ctx *ctx)
         HIPCK(sph::neighbours_fill(ctx->st, ctx->ngroups, I32(groups), D(x), D(y), D(z), D(h), P_<WNode>(ctx->nodes),
                                    SPH_BUCKET, P_<int64_t>(ctx->off), I32(idx), ovf));
     }
+    ctx->ngb_stride = 0;
     ctx->csr_total = total;
     ctx->stats.ngb_total = total;
     return SPH_OK;
 }
 
-int ensure_csr(sph_ctx *ctx)
+NgbRows ngb_rows(sph_ctx *ctx)
 {
-    if (ctx->csr_ok) return SPH_OK;
+    return NgbRows{P_<int64_t>(ctx->off), I32(cnt), I32(idx), ctx->ngb_stride};
+}
+
+// csr: the caller needs true CSR offsets (sph_neighbours).
+int ensure_csr(sph_ctx *ctx, bool csr = false)
+{
+    if (ctx->csr_ok && (!csr || ctx->ngb_stride == 0)) return SPH_OK;
     CK(ensure_moments(ctx));
-    CK(build_csr(ctx));
+    CK(build_ngb(ctx, csr));
     ctx->csr_ok = true;
     return SPH_OK;
 }
@@ -362,7 +396,10 @@ int check_common(sph_ctx *ctx, int64_t n, int precision)
 // ---- phase timing for the resident step
 void ph_begin(sph_ctx *ctx, int ph)
 {
-    if (ctx->timing) (void)hipEventRecord(ctx->ev[2 * ph], ctx->st);
+    if (ctx->timing) {
+        (void)hipEventRecord(ctx->ev[2 * ph], ctx->st);
+        ctx->ph_rec |= 1u << ph;
+    }
 }
 void ph_end(sph_ctx *ctx, int ph)
 {
@@ -435,7 +472,8 @@ extern "C" const char *sph_last_error(const sph_ctx *ctx)
 namespace {
 
 int run_hsolve(sph_ctx *ctx, int64_t m, const double *tx, const double *ty, const double *tz, int mode,
-               double param, int n_low, int n_up, int max_steps, double tol, int64_t *n_unconv, int64_t *rounds_out)
+               double param, int n_low, int n_up, int max_steps, double tol, int64_t *n_unconv, int64_t *rounds_out,
+               int64_t *work = nullptr)
 {
     CK(ensure(ctx, ctx->cand, (size_t)m * SPH_HCAP * 8));
     sph::HSolveParams p;
@@ -450,7 +488,7 @@ int run_hsolve(sph_ctx *ctx, int64_t m, const double *tx, const double *ty, cons
     p.n_low = n_low; p.n_up = n_up; p.max_steps = max_steps;
     p.kth = (int)param;
     p.cap = SPH_HCAP;
-    p.r_fac = 1.15;
+    p.r_fac = 1.06;  // candidate radius 2.12 h_prev: one round while h grows < 6 % per call
     p.h_in = D(hin);
     p.h_out = D(hout);
     p.cand = D(cand);
@@ -464,6 +502,7 @@ int run_hsolve(sph_ctx *ctx, int64_t m, const double *tx, const double *ty, cons
     p.groups = (tx == D(x)) ? I32(groups) : nullptr;  // targets are the sorted particles: tree-node groups
     p.ngroups = ctx->ngroups;
     p.rounds = rounds_out;
+    p.work = work;
     int64_t pending = 0;
     HIPCK(sph::hsolve(ctx->st, p, SPH_HROUNDS, &pending));
     if (pending) return fail(ctx, SPH_ERR_NOCONV, "smoothing-length solve: " + std::to_string(pending) +
@@ -538,6 +577,7 @@ extern "C" int sph_neighbours(sph_ctx *ctx, int64_t n, const double *pos, const 
     CK(check_common(ctx, n, SPH_PREC_FP64));
     if (!pos || !h || !offsets) return fail(ctx, SPH_ERR_ARG, "pos, h and offsets are required");
     CK(geom(ctx, n, pos, h, nullptr));
+    CK(ensure_csr(ctx, true));  // offsets need the exact CSR layout
     int64_t total = ctx->csr_total;
     std::vector<int64_t> soff(n + 1);
     std::vector<int32_t> orig(n);
@@ -571,13 +611,13 @@ extern "C" int sph_density(sph_ctx *ctx, int64_t n, const double *pos, const dou
     CK(check_common(ctx, n, precision));
     if (!pos || !m || !h || !rho_out) return fail(ctx, SPH_ERR_ARG, "null argument");
     CK(geom(ctx, n, pos, h, m));
-    const int64_t *off = P_<int64_t>(ctx->off);
+    const NgbRows rows = ngb_rows(ctx);
     if (precision == SPH_PREC_MIXED) {
-        HIPCK(sph::density<float>(ctx->st, n, D(x), D(y), D(z), D(m), D(h), nullptr, 0.0, off, I32(idx),
+        HIPCK(sph::density<float>(ctx->st, n, D(x), D(y), D(z), D(m), D(h), nullptr, 0.0, rows,
                                   P_<float>(ctx->rho), nullptr, nullptr, nullptr));
         return get_scalar<float>(ctx, ctx->rho, rho_out);
     }
-    HIPCK(sph::density<double>(ctx->st, n, D(x), D(y), D(z), D(m), D(h), nullptr, 0.0, off, I32(idx), D(rho),
+    HIPCK(sph::density<double>(ctx->st, n, D(x), D(y), D(z), D(m), D(h), nullptr, 0.0, rows, D(rho),
                                nullptr, nullptr, nullptr));
     return get_scalar<double>(ctx, ctx->rho, rho_out);
 }
@@ -590,7 +630,7 @@ int divcurl_impl(sph_ctx *ctx, const double *vel, const double *rho, double *dv,
     CK(put_vec3(ctx, vel, ctx->vx, ctx->vy, ctx->vz));
     CK(put_real<R>(ctx, rho, ctx->rho));
     HIPCK(sph::divcurl<R>(ctx->st, n, D(x), D(y), D(z), D(vx), D(vy), D(vz), D(m), D(h), P_<R>(ctx->rho), nullptr,
-                          P_<int64_t>(ctx->off), I32(idx), P_<R>(ctx->divV), P_<R>(ctx->curlV), nullptr));
+                          ngb_rows(ctx), P_<R>(ctx->divV), P_<R>(ctx->curlV), nullptr));
     CK(get_scalar<R>(ctx, ctx->divV, dv));
     return get_scalar<R>(ctx, ctx->curlV, cv);
 }
@@ -622,7 +662,7 @@ int acc_impl(sph_ctx *ctx, const double *vel, const double *rho, const double *P
     bool bal = divV && curlV;
     HIPCK(sph::accsph<R>(ctx->st, ctx->n, D(x), D(y), D(z), D(vx), D(vy), D(vz), D(m), D(h), P_<R>(ctx->rho),
                          P_<R>(ctx->prho2), P_<R>(ctx->c), bal ? P_<R>(ctx->bal) : nullptr, alpha,
-                         P_<int64_t>(ctx->off), I32(idx), nullptr, nullptr, nullptr, D(ax), D(ay), D(az)));
+                         ngb_rows(ctx), nullptr, nullptr, nullptr, D(ax), D(ay), D(az)));
     return get_vec3(ctx, ctx->ax, ctx->ay, ctx->az, acc);
 }
 
@@ -634,7 +674,7 @@ int dudt_impl(sph_ctx *ctx, const double *vel, const double *rho, const double *
     bool bal = divV && curlV;
     HIPCK(sph::dudt<R>(ctx->st, ctx->n, D(x), D(y), D(z), D(vx), D(vy), D(vz), D(m), D(h), P_<R>(ctx->rho),
                        P_<R>(ctx->prho2), P_<R>(ctx->c), bal ? P_<R>(ctx->bal) : nullptr, alpha,
-                       P_<int64_t>(ctx->off), I32(idx), P_<R>(ctx->du)));
+                       ngb_rows(ctx), P_<R>(ctx->du)));
     return get_scalar<R>(ctx, ctx->du, out);
 }
 }  // namespace
@@ -693,7 +733,7 @@ extern "C" int sph_gravity(sph_ctx *ctx, int64_t n, const double *pos, const dou
     }
     if (cm || ce) ctx->moments_ok = false;
     CK(ensure_moments(ctx));
-    unsigned long long *cnt = P_<unsigned long long>(ctx->spread) + 3 * SPH_SPREAD_WORDS;
+    unsigned long long *cnt = P_<unsigned long long>(ctx->spread) + SPR_G * SPH_SPREAD_WORDS;
     HIPCK(sph::fill<unsigned long long>(ctx->st, 3 * SPH_SPREAD_WORDS, cnt, 0ull));
     const double *e = ctx->eps_ok ? D(eps) : nullptr;
     const int64_t ng = ctx->ngroups;
@@ -704,8 +744,8 @@ extern "C" int sph_gravity(sph_ctx *ctx, int64_t n, const double *pos, const dou
         HIPCK(sph::gravity<double>(ctx->st, ng, I32(groups), D(x), D(y), D(z), D(m), e, P_<WNode>(ctx->nodes),
                                    SPH_BUCKET, theta, G, soft_kind, eps_plummer, D(gx), D(gy), D(gz), cnt));
     unsigned long long hc[3];
-    HIPCK(sph::spread_total(ctx->st, 3, cnt, P_<unsigned long long>(ctx->hcount)));
-    HIPCK(hipMemcpyAsync(hc, ctx->hcount.p, sizeof(hc), hipMemcpyDeviceToHost, ctx->st));
+    HIPCK(sph::spread_total(ctx->st, 3, cnt, P_<unsigned long long>(ctx->hcount) + 5));
+    HIPCK(hipMemcpyAsync(hc, P_<unsigned long long>(ctx->hcount) + 5, sizeof(hc), hipMemcpyDeviceToHost, ctx->st));
     HIPCK(hipStreamSynchronize(ctx->st));
     if (hc[2]) return fail(ctx, SPH_ERR_CAPACITY, "gravity walk stack overflow");
     if (n_interactions) {
@@ -814,12 +854,19 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     CK(topology(ctx));
     ph_end(ctx, SPH_PH_TOPOLOGY);
     // boxes are needed by the h walk; hmax is refreshed after the solve
-    ph_begin(ctx, SPH_PH_HSOLVE);
+    ph_begin(ctx, SPH_PH_HMOMENTS);
     HIPCK(sph::tree_moments(st, n, D(x), D(y), D(z), D(m), D(h), nullptr, P_<WNode>(ctx->nodes), I32(right),
                             I32(parent), P_<uint32_t>(ctx->flags), I32(front)));
+    ph_end(ctx, SPH_PH_HMOMENTS);
+    ph_begin(ctx, SPH_PH_HSOLVE);
     HIPCK(hipMemcpyAsync(ctx->hin.p, ctx->h.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     int64_t unc = 0;
-    CK(run_hsolve(ctx, n, D(x), D(y), D(z), SPH_H_NEWTON, p->n_ngb, 0, 0, 0, p->h_tol, &unc, &ctx->stats.h_rounds));
+    ctx->stats.h_tests = ctx->stats.h_terms = 0;
+    int64_t work[2] = {0, 0};
+    CK(run_hsolve(ctx, n, D(x), D(y), D(z), SPH_H_NEWTON, p->n_ngb, 0, 0, 0, p->h_tol, &unc, &ctx->stats.h_rounds,
+                  work));
+    ctx->stats.h_tests = work[0];
+    ctx->stats.h_terms = work[1];
     std::swap(ctx->h, ctx->hout);
     ctx->stats.h_unconverged += unc;
     ph_end(ctx, SPH_PH_HSOLVE);
@@ -830,33 +877,33 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
                             I32(parent), P_<uint32_t>(ctx->flags), I32(front)));
     ph_end(ctx, SPH_PH_MOMENTS);
     ph_begin(ctx, SPH_PH_NGB);
-    CK(build_csr(ctx));
+    CK(build_ngb(ctx, false));
     ph_end(ctx, SPH_PH_NGB);
-    const int64_t *off = P_<int64_t>(ctx->off);
-    const int32_t *idx = I32(idx);
+    const NgbRows rows = ngb_rows(ctx);
+    // density + EOS, fused with div/curl v and the Balsara factor when the switch is on
+    R *bal = p->balsara ? P_<R>(ctx->bal) : nullptr;
     ph_begin(ctx, SPH_PH_DENSITY);
-    HIPCK(sph::density<R>(st, n, D(x), D(y), D(z), D(m), D(h), D(u), p->gamma, off, idx, P_<R>(ctx->rho),
-                          P_<R>(ctx->P), P_<R>(ctx->c), P_<R>(ctx->prho2)));
+    if (p->balsara)
+        HIPCK(sph::density_divcurl<R>(st, n, D(x), D(y), D(z), D(vx), D(vy), D(vz), D(m), D(h), D(u), p->gamma, rows,
+                                      P_<R>(ctx->rho), P_<R>(ctx->P), P_<R>(ctx->c), P_<R>(ctx->prho2),
+                                      P_<R>(ctx->divV), P_<R>(ctx->curlV), bal));
+    else
+        HIPCK(sph::density<R>(st, n, D(x), D(y), D(z), D(m), D(h), D(u), p->gamma, rows, P_<R>(ctx->rho),
+                              P_<R>(ctx->P), P_<R>(ctx->c), P_<R>(ctx->prho2)));
     ph_end(ctx, SPH_PH_DENSITY);
     ph_begin(ctx, SPH_PH_GRAVITY);
-    unsigned long long *cnt = P_<unsigned long long>(ctx->spread) + 3 * SPH_SPREAD_WORDS;
+    unsigned long long *cnt = P_<unsigned long long>(ctx->spread) + SPR_G * SPH_SPREAD_WORDS;
     HIPCK(sph::fill<unsigned long long>(st, 3 * SPH_SPREAD_WORDS, cnt, 0ull));
     if (p->G != 0.0) {
         HIPCK(sph::gravity<R>(st, ctx->ngroups, I32(groups), D(x), D(y), D(z), D(m), eps, P_<WNode>(ctx->nodes), SPH_BUCKET,
                               p->theta, p->G, p->soft_kind, p->eps_plummer, D(gx), D(gy), D(gz), cnt));
     }
-    HIPCK(sph::spread_total(st, 3, cnt, P_<unsigned long long>(ctx->hcount) + 4));
+    HIPCK(sph::spread_total(st, 3, cnt, P_<unsigned long long>(ctx->hcount) + 5));
     ph_end(ctx, SPH_PH_GRAVITY);
-    ph_begin(ctx, SPH_PH_DIVCURL);
-    R *bal = p->balsara ? P_<R>(ctx->bal) : nullptr;
-    if (p->balsara)
-        HIPCK(sph::divcurl<R>(st, n, D(x), D(y), D(z), D(vx), D(vy), D(vz), D(m), D(h), P_<R>(ctx->rho),
-                              P_<R>(ctx->c), off, idx, P_<R>(ctx->divV), P_<R>(ctx->curlV), bal));
-    ph_end(ctx, SPH_PH_DIVCURL);
     ph_begin(ctx, SPH_PH_ACCSPH);
     const bool g = p->G != 0.0;
     HIPCK(sph::accsph<R>(st, n, D(x), D(y), D(z), D(vx), D(vy), D(vz), D(m), D(h), P_<R>(ctx->rho),
-                         P_<R>(ctx->prho2), P_<R>(ctx->c), bal, p->alpha, off, idx, g ? D(gx) : nullptr,
+                         P_<R>(ctx->prho2), P_<R>(ctx->c), bal, p->alpha, rows, g ? D(gx) : nullptr,
                          g ? D(gy) : nullptr, g ? D(gz) : nullptr, D(ax), D(ay), D(az)));
     ph_end(ctx, SPH_PH_ACCSPH);
     if (!first) {
@@ -867,7 +914,7 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     }
     ph_begin(ctx, SPH_PH_DUDT);
     HIPCK(sph::dudt<R>(st, n, D(x), D(y), D(z), D(vx), D(vy), D(vz), D(m), D(h), P_<R>(ctx->rho), P_<R>(ctx->prho2),
-                       P_<R>(ctx->c), bal, p->alpha, off, idx, P_<R>(ctx->du)));
+                       P_<R>(ctx->c), bal, p->alpha, rows, P_<R>(ctx->du)));
     ph_end(ctx, SPH_PH_DUDT);
     ph_begin(ctx, SPH_PH_UUPDATE);
     HIPCK(sph::u_update<R>(st, n, D(u), P_<R>(ctx->du), D(duprev), p->dt, first));
@@ -891,12 +938,15 @@ int finish_step(sph_ctx *ctx, const sph_step_params *p)
 {
     HIPCK(hipStreamSynchronize(ctx->st));
     unsigned long long hc[3];
-    HIPCK(hipMemcpy(hc, P_<unsigned long long>(ctx->hcount) + 4, sizeof(hc), hipMemcpyDeviceToHost));
+    HIPCK(hipMemcpy(hc, P_<unsigned long long>(ctx->hcount) + 5, sizeof(hc), hipMemcpyDeviceToHost));
     if (hc[2]) return fail(ctx, SPH_ERR_CAPACITY, "gravity walk stack overflow");
     ctx->stats.grav_pp = (int64_t)hc[0];
     ctx->stats.grav_pn = (int64_t)hc[1];
     if (ctx->timing) {
+        // only the phases this step recorded (an unrecorded event pair would fail and
+        // leave a sticky error behind)
         for (int ph = 0; ph < SPH_PH_COUNT; ph++) {
+            if (!(ctx->ph_rec & (1u << ph))) continue;
             float ms = 0.0f;
             if (hipEventElapsedTime(&ms, ctx->ev[2 * ph], ctx->ev[2 * ph + 1]) == hipSuccess) {
                 ctx->stats.phase_ms[ph] += ms;
@@ -904,6 +954,7 @@ int finish_step(sph_ctx *ctx, const sph_step_params *p)
             }
         }
     }
+    ctx->ph_rec = 0;
     ctx->rho_prec = p->precision;
     return SPH_OK;
 }
@@ -926,6 +977,7 @@ extern "C" int sph_step(sph_ctx *ctx, const sph_step_params *p)
     CK(check_params(ctx, p));
     if (!ctx->primed) return fail(ctx, SPH_ERR_STATE, "call sph_prime before sph_step");
     ctx->timing = p->timing != 0;
+    ctx->ph_rec = 0;
     int64_t n = ctx->n;
     ph_begin(ctx, SPH_PH_KICKDRIFT);
     HIPCK(sph::kick_drift(ctx->st, n, D(vx), D(vy), D(vz), D(ax), D(ay), D(az), 0.5 * p->dt, p->dt, D(x), D(y), D(z)));

@@ -1,8 +1,29 @@
 // sph_hydro.h -- host entry points of the CSR hydro passes (sph_hydro.hip).
 // R = float (mixed form) or double (all-FP64 form).  All arrays are sorted
-// order; off/idx is the CSR of S_i = { j : r_ij^2 < (h_i+h_j)^2 }.
+// order; rows holds the neighbour sets S_i = { j : r_ij^2 < (h_i+h_j)^2 }.
 #pragma once
 #include "sph_common.h"
+
+// Neighbour rows of the sorted particles: CSR (stride == 0: row i is
+// idx[off[i] .. off[i+1])) or padded (stride > 0: row i is
+// idx[i*stride .. i*stride + cnt[i])).
+struct NgbRows {
+    const int64_t *off;
+    const int32_t *cnt;
+    const int32_t *idx;
+    int stride;
+};
+
+__device__ __forceinline__ void row_range(const NgbRows &r, int64_t i, int64_t &b, int64_t &e)
+{
+    if (r.stride > 0) {
+        b = i * (int64_t)r.stride;
+        e = b + r.cnt[i];
+    } else {
+        b = r.off[i];
+        e = r.off[i + 1];
+    }
+}
 
 namespace sph {
 
@@ -10,22 +31,28 @@ namespace sph {
 // c = sqrt(gamma (gamma-1) u), prho2 = P / rho^2 (skipped when u is null).
 template <typename R>
 hipError_t density(hipStream_t st, int64_t n, const double *x, const double *y, const double *z, const double *m,
-                   const double *h, const double *u, double gamma, const int64_t *off, const int32_t *idx, R *rho, R *P,
+                   const double *h, const double *u, double gamma, const NgbRows &rows, R *rho, R *P,
                    R *c, R *prho2);
 // |div v|, |curl v| and (bal non-null) Balsara f = |div v| / (|div v| + |curl v| + 1e-4 c/h).
 template <typename R>
 hipError_t divcurl(hipStream_t st, int64_t n, const double *x, const double *y, const double *z, const double *vx,
                    const double *vy, const double *vz, const double *m, const double *h, const R *rho, const R *c,
-                   const int64_t *off, const int32_t *idx, R *divV, R *curlV, R *bal);
+                   const NgbRows &rows, R *divV, R *curlV, R *bal);
+// density() and divcurl() in one pass (u required; bal may be null).
+template <typename R>
+hipError_t density_divcurl(hipStream_t st, int64_t n, const double *x, const double *y, const double *z,
+                           const double *vx, const double *vy, const double *vz, const double *m, const double *h,
+                           const double *u, double gamma, const NgbRows &rows, R *rho, R *P, R *c,
+                           R *prho2, R *divV, R *curlV, R *bal);
 // a = a_grav (optional, may be null) + pressure/AV acceleration; bal null = no Balsara.
 template <typename R>
 hipError_t accsph(hipStream_t st, int64_t n, const double *x, const double *y, const double *z, const double *vx,
                   const double *vy, const double *vz, const double *m, const double *h, const R *rho,
-                  const R *prho2, const R *c, const R *bal, double alpha, const int64_t *off, const int32_t *idx,
+                  const R *prho2, const R *c, const R *bal, double alpha, const NgbRows &rows,
                   const double *gx, const double *gy, const double *gz, double *ax, double *ay, double *az);
 template <typename R>
 hipError_t dudt(hipStream_t st, int64_t n, const double *x, const double *y, const double *z, const double *vx,
                 const double *vy, const double *vz, const double *m, const double *h, const R *rho, const R *prho2,
-                const R *c, const R *bal, double alpha, const int64_t *off, const int32_t *idx, R *out);
+                const R *c, const R *bal, double alpha, const NgbRows &rows, R *out);
 
 }  // namespace sph

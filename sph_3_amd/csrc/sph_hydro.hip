@@ -21,7 +21,7 @@ __global__ void __launch_bounds__(256) k_density(int64_t n, const double *__rest
                                                  const double *__restrict__ y, const double *__restrict__ z,
                                                  const double *__restrict__ m, const double *__restrict__ h,
                                                  const double *__restrict__ u, double gamma,
-                                                 const int64_t *__restrict__ off, const int32_t *__restrict__ idx,
+                                                 NgbRows rows,
                                                  R *__restrict__ rho, R *__restrict__ P, R *__restrict__ c,
                                                  R *__restrict__ prho2)
 {
@@ -32,9 +32,10 @@ __global__ void __launch_bounds__(256) k_density(int64_t n, const double *__rest
     Kahan<R> acc;
     if (valid) {
         const double xi = x[i], yi = y[i], zi = z[i], hi = h[i];
-        const int64_t e = off[i + 1];
-        for (int64_t k = off[i] + l; k < e; k += G) {
-            int j = idx[k];
+        int64_t k0, e;
+        row_range(rows, i, k0, e);
+        for (int64_t k = k0 + l; k < e; k += G) {
+            int j = rows.idx[k];
             R dx = (R)(xi - x[j]), dy = (R)(yi - y[j]), dz = (R)(zi - z[j]);
             R rr = sph_sqrt<R>(dx * dx + dy * dy + dz * dz);
             R hij = (R)(0.5 * (hi + h[j]));
@@ -60,8 +61,7 @@ __global__ void __launch_bounds__(256) k_divcurl(int64_t n, const double *__rest
                                                  const double *__restrict__ vx, const double *__restrict__ vy,
                                                  const double *__restrict__ vz, const double *__restrict__ m,
                                                  const double *__restrict__ h, const R *__restrict__ rho,
-                                                 const R *__restrict__ c, const int64_t *__restrict__ off,
-                                                 const int32_t *__restrict__ idx, R *__restrict__ divV,
+                                                 const R *__restrict__ c, NgbRows rows, R *__restrict__ divV,
                                                  R *__restrict__ curlV, R *__restrict__ bal)
 {
     int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -72,9 +72,10 @@ __global__ void __launch_bounds__(256) k_divcurl(int64_t n, const double *__rest
     if (valid) {
         const double xi = x[i], yi = y[i], zi = z[i], hi = h[i];
         const double vxi = vx[i], vyi = vy[i], vzi = vz[i];
-        const int64_t e = off[i + 1];
-        for (int64_t k = off[i] + l; k < e; k += G) {
-            int j = idx[k];
+        int64_t k0, e;
+        row_range(rows, i, k0, e);
+        for (int64_t k = k0 + l; k < e; k += G) {
+            int j = rows.idx[k];
             R dx = (R)(xi - x[j]), dy = (R)(yi - y[j]), dz = (R)(zi - z[j]);
             R rr = sph_sqrt<R>(dx * dx + dy * dy + dz * dz);
             R nW = spline_nW<R>(rr, (R)(0.5 * (hi + h[j])));
@@ -102,6 +103,68 @@ __global__ void __launch_bounds__(256) k_divcurl(int64_t n, const double *__rest
     }
 }
 
+// Density, EOS and div/curl v in one pass over the CSR rows (the time loop's
+// getDensity + getPressure + div_curlVel, sph_serial_v2.py:166-189): div and
+// curl need only the particle's own density, so both sums share the
+// neighbour gathers and are divided by rho_i at the end.  Per-quantity
+// arithmetic and summation order are those of k_density and k_divcurl.
+template <typename R, int G>
+__global__ void __launch_bounds__(256) k_density_divcurl(
+    int64_t n, const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
+    const double *__restrict__ vx, const double *__restrict__ vy, const double *__restrict__ vz,
+    const double *__restrict__ m, const double *__restrict__ h, const double *__restrict__ u, double gamma,
+    NgbRows rows, R *__restrict__ rho, R *__restrict__ P,
+    R *__restrict__ c, R *__restrict__ prho2, R *__restrict__ divV, R *__restrict__ curlV, R *__restrict__ bal)
+{
+    int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t i = gid / G;
+    int l = (int)(gid % G);
+    bool valid = i < n;
+    Kahan<R> acc, sd, scx, scy, scz;
+    if (valid) {
+        const double xi = x[i], yi = y[i], zi = z[i], hi = h[i];
+        const double vxi = vx[i], vyi = vy[i], vzi = vz[i];
+        int64_t k0, e;
+        row_range(rows, i, k0, e);
+        for (int64_t k = k0 + l; k < e; k += G) {
+            int j = rows.idx[k];
+            R dx = (R)(xi - x[j]), dy = (R)(yi - y[j]), dz = (R)(zi - z[j]);
+            R rr = sph_sqrt<R>(dx * dx + dy * dy + dz * dz);
+            R hij = (R)(0.5 * (hi + h[j]));
+            R mj = (R)m[j];
+            acc.add(mj * spline_W<R>(rr, hij));
+            R nW = spline_nW<R>(rr, hij);
+            R gx = nW * dx, gy = nW * dy, gz = nW * dz;
+            R ux = (R)(vxi - vx[j]), uy = (R)(vyi - vy[j]), uz = (R)(vzi - vz[j]);
+            sd.add(-mj * (ux * gx + uy * gy + uz * gz));
+            scx.add(mj * (uy * gz - uz * gy));
+            scy.add(mj * (uz * gx - ux * gz));
+            scz.add(mj * (ux * gy - uy * gx));
+        }
+    }
+    R s = group_sum<R, G>(acc.s, -acc.c);
+    R d = group_sum<R, G>(sd.s, -sd.c);
+    R cx = group_sum<R, G>(scx.s, -scx.c);
+    R cy = group_sum<R, G>(scy.s, -scy.c);
+    R cz = group_sum<R, G>(scz.s, -scz.c);
+    if (valid && l == 0) {
+        rho[i] = s;
+        R ui = (R)u[i];
+        R Pi = (R)(gamma - 1.0) * s * ui;
+        R ci = sph_sqrt<R>((R)(gamma * (gamma - 1.0)) * ui);
+        P[i] = Pi;
+        c[i] = ci;
+        prho2[i] = Pi / (s * s);
+        R ir = (R)1 / s;
+        R dv = d * ir;
+        dv = dv < (R)0 ? -dv : dv;
+        R cv = sph_sqrt<R>(cx * cx + cy * cy + cz * cz) * ir;
+        divV[i] = dv;
+        curlV[i] = cv;
+        if (bal) bal[i] = dv / (dv + cv + (R)0.0001 * ci / (R)h[i]);
+    }
+}
+
 // Artificial viscosity Pi_ij (Gadget-2 form), libsx.py:146-160.
 template <typename R>
 __device__ __forceinline__ R av_pi(R vr, R rr, R ci, R cj, R rhoi, R rhoj, R alpha)
@@ -120,7 +183,7 @@ __global__ void __launch_bounds__(256) k_accsph(int64_t n, const double *__restr
                                                 const double *__restrict__ h, const R *__restrict__ rho,
                                                 const R *__restrict__ prho2, const R *__restrict__ c,
                                                 const R *__restrict__ bal, double alpha,
-                                                const int64_t *__restrict__ off, const int32_t *__restrict__ idx,
+                                                NgbRows rows,
                                                 const double *__restrict__ gx_in, const double *__restrict__ gy_in,
                                                 const double *__restrict__ gz_in, double *__restrict__ ax,
                                                 double *__restrict__ ay, double *__restrict__ az)
@@ -135,9 +198,10 @@ __global__ void __launch_bounds__(256) k_accsph(int64_t n, const double *__restr
         const double vxi = vx[i], vyi = vy[i], vzi = vz[i];
         const R rhoi = rho[i], pi_ = prho2[i], ci = c[i], fi = bal ? bal[i] : (R)1;
         const R al = (R)alpha;
-        const int64_t e = off[i + 1];
-        for (int64_t k = off[i] + l; k < e; k += G) {
-            int j = idx[k];
+        int64_t k0, e;
+        row_range(rows, i, k0, e);
+        for (int64_t k = k0 + l; k < e; k += G) {
+            int j = rows.idx[k];
             R dx = (R)(xi - x[j]), dy = (R)(yi - y[j]), dz = (R)(zi - z[j]);
             R rr = sph_sqrt<R>(dx * dx + dy * dy + dz * dz);
             R nW = spline_nW<R>(rr, (R)(0.5 * (hi + h[j])));
@@ -170,7 +234,7 @@ __global__ void __launch_bounds__(256) k_dudt(int64_t n, const double *__restric
                                               const double *__restrict__ h, const R *__restrict__ rho,
                                               const R *__restrict__ prho2, const R *__restrict__ c,
                                               const R *__restrict__ bal, double alpha,
-                                              const int64_t *__restrict__ off, const int32_t *__restrict__ idx,
+                                              NgbRows rows,
                                               R *__restrict__ dudt)
 {
     int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -183,9 +247,10 @@ __global__ void __launch_bounds__(256) k_dudt(int64_t n, const double *__restric
         const double vxi = vx[i], vyi = vy[i], vzi = vz[i];
         const R rhoi = rho[i], pi_ = prho2[i], ci = c[i], fi = bal ? bal[i] : (R)1;
         const R al = (R)alpha;
-        const int64_t e = off[i + 1];
-        for (int64_t k = off[i] + l; k < e; k += G) {
-            int j = idx[k];
+        int64_t k0, e;
+        row_range(rows, i, k0, e);
+        for (int64_t k = k0 + l; k < e; k += G) {
+            int j = rows.idx[k];
             R dx = (R)(xi - x[j]), dy = (R)(yi - y[j]), dz = (R)(zi - z[j]);
             R rr = sph_sqrt<R>(dx * dx + dy * dy + dz * dz);
             R nW = spline_nW<R>(rr, (R)(0.5 * (hi + h[j])));
@@ -204,60 +269,73 @@ namespace sph {
 
 template <typename R>
 hipError_t density(hipStream_t st, int64_t n, const double *x, const double *y, const double *z, const double *m,
-                   const double *h, const double *u, double gamma, const int64_t *off, const int32_t *idx, R *rho, R *P,
+                   const double *h, const double *u, double gamma, const NgbRows &rows, R *rho, R *P,
                    R *c, R *prho2)
 {
-    k_density<R, HG><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, m, h, u, gamma, off, idx, rho, P, c, prho2);
+    k_density<R, HG><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, m, h, u, gamma, rows, rho, P, c, prho2);
     return hipGetLastError();
 }
 
 template <typename R>
 hipError_t divcurl(hipStream_t st, int64_t n, const double *x, const double *y, const double *z, const double *vx,
                    const double *vy, const double *vz, const double *m, const double *h, const R *rho, const R *c,
-                   const int64_t *off, const int32_t *idx, R *divV, R *curlV, R *bal)
+                   const NgbRows &rows, R *divV, R *curlV, R *bal)
 {
-    k_divcurl<R, HG><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, rho, c, off, idx, divV,
+    k_divcurl<R, HG><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, rho, c, rows, divV,
                                                               curlV, bal);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t density_divcurl(hipStream_t st, int64_t n, const double *x, const double *y, const double *z,
+                           const double *vx, const double *vy, const double *vz, const double *m, const double *h,
+                           const double *u, double gamma, const NgbRows &rows, R *rho, R *P, R *c,
+                           R *prho2, R *divV, R *curlV, R *bal)
+{
+    k_density_divcurl<R, HG><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, u, gamma, rows,
+                                                                      rho, P, c, prho2, divV, curlV, bal);
     return hipGetLastError();
 }
 
 template <typename R>
 hipError_t accsph(hipStream_t st, int64_t n, const double *x, const double *y, const double *z, const double *vx,
                   const double *vy, const double *vz, const double *m, const double *h, const R *rho,
-                  const R *prho2, const R *c, const R *bal, double alpha, const int64_t *off, const int32_t *idx,
+                  const R *prho2, const R *c, const R *bal, double alpha, const NgbRows &rows,
                   const double *gx, const double *gy, const double *gz, double *ax, double *ay, double *az)
 {
     k_accsph<R, HG><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, rho, prho2, c, bal, alpha,
-                                                             off, idx, gx, gy, gz, ax, ay, az);
+                                                             rows, gx, gy, gz, ax, ay, az);
     return hipGetLastError();
 }
 
 template <typename R>
 hipError_t dudt(hipStream_t st, int64_t n, const double *x, const double *y, const double *z, const double *vx,
                 const double *vy, const double *vz, const double *m, const double *h, const R *rho, const R *prho2,
-                const R *c, const R *bal, double alpha, const int64_t *off, const int32_t *idx, R *out)
+                const R *c, const R *bal, double alpha, const NgbRows &rows, R *out)
 {
     k_dudt<R, HG><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, rho, prho2, c, bal, alpha,
-                                                           off, idx, out);
+                                                           rows, out);
     return hipGetLastError();
 }
 
-#define INST(R)                                                                                                     \
-    template hipError_t density<R>(hipStream_t, int64_t, const double *, const double *, const double *,            \
-                                   const double *, const double *, const double *, double, const int64_t *,        \
-                                   const int32_t *, R *, R *, R *, R *);                                           \
-    template hipError_t divcurl<R>(hipStream_t, int64_t, const double *, const double *, const double *,            \
+#define INST(R)                                                                                                    \
+    template hipError_t density<R>(hipStream_t, int64_t, const double *, const double *, const double *,           \
+                                   const double *, const double *, const double *, double, const NgbRows &, R *,   \
+                                   R *, R *, R *);                                                                 \
+    template hipError_t divcurl<R>(hipStream_t, int64_t, const double *, const double *, const double *,           \
                                    const double *, const double *, const double *, const double *, const double *, \
-                                   const R *, const R *, const int64_t *, const int32_t *, R *, R *, R *);         \
-    template hipError_t accsph<R>(hipStream_t, int64_t, const double *, const double *, const double *,             \
+                                   const R *, const R *, const NgbRows &, R *, R *, R *);                          \
+    template hipError_t accsph<R>(hipStream_t, int64_t, const double *, const double *, const double *,            \
                                   const double *, const double *, const double *, const double *, const double *,  \
-                                  const R *, const R *, const R *, const R *, double, const int64_t *,             \
-                                  const int32_t *, const double *, const double *, const double *, double *,       \
-                                  double *, double *);                                                             \
-    template hipError_t dudt<R>(hipStream_t, int64_t, const double *, const double *, const double *,               \
+                                  const R *, const R *, const R *, const R *, double, const NgbRows &,             \
+                                  const double *, const double *, const double *, double *, double *, double *);   \
+    template hipError_t dudt<R>(hipStream_t, int64_t, const double *, const double *, const double *,              \
                                 const double *, const double *, const double *, const double *, const double *,    \
-                                const R *, const R *, const R *, const R *, double, const int64_t *,               \
-                                const int32_t *, R *);
+                                const R *, const R *, const R *, const R *, double, const NgbRows &, R *);         \
+    template hipError_t density_divcurl<R>(hipStream_t, int64_t, const double *, const double *, const double *,   \
+                                           const double *, const double *, const double *, const double *,         \
+                                           const double *, const double *, double, const NgbRows &, R *, R *, R *, \
+                                           R *, R *, R *, R *);
 INST(float)
 INST(double)
 

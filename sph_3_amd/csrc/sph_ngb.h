@@ -24,8 +24,9 @@ struct HSolveParams {
     double *h_out;               // result (device)
     double *cand, *R;            // scratch: cap*m, m
     int32_t *steps, *state;      // scratch: m, m
-    unsigned long long *counters;  // scratch: spread array of 3 counters (3 * SPH_SPREAD_WORDS words)
-    unsigned long long *totals;    // scratch: 3 words
+    unsigned long long *counters;  // scratch: spread array of 5 counters (5 * SPH_SPREAD_WORDS words)
+    unsigned long long *totals;    // scratch: 5 words
+    int64_t *work;                 // host (may be null): += distance tests of the walks, kernel terms of Newton
     int64_t *unconverged_total;  // host accumulator (may be null)
     const int32_t *groups;       // tree-node groups when the targets are the sorted particles, else null
     int64_t ngroups;
@@ -47,5 +48,14 @@ hipError_t neighbours(hipStream_t st, int64_t n, int64_t ngroups, const int32_t 
 hipError_t neighbours_fill(hipStream_t st, int64_t ngroups, const int32_t *groups, const double *x, const double *y,
                            const double *z, const double *h, const WNode *nodes, int bucket, const int64_t *off,
                            int32_t *idx, unsigned long long *overflow);
+// The same sets in one walk as padded rows: idx[i*stride + c], c < cnt[i] (sorted
+// particles).  *n_long = rows with more than stride entries (those rows are
+// truncated: rebuild as CSR), *total = entries.  spread: 2 * SPH_SPREAD_WORDS
+// scratch words, totals: 2 scratch words, overflow: 1 scratch counter.
+// Synchronises the stream.
+hipError_t neighbour_rows(hipStream_t st, int64_t ngroups, const int32_t *groups, const double *x, const double *y,
+                          const double *z, const double *h, const WNode *nodes, int bucket, int stride, int32_t *cnt,
+                          int32_t *idx, unsigned long long *overflow, unsigned long long *spread,
+                          unsigned long long *totals, int64_t *n_long, int64_t *total);
 
 }  // namespace sph

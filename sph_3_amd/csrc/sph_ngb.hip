@@ -90,7 +90,8 @@ struct HSolveArgs {
     double *h;                 // per target h (in: start, out: result)
     int32_t *steps;            // mode 1 step counter (persists over rounds)
     int32_t *state;            // 0 pending, 1 done, 2 done-unconverged
-    unsigned long long *pending;  // spread counters: 0 pending, 1 unconverged, 2 walk overflows
+    unsigned long long *pending;  // spread counters: 0 pending, 1 unconverged, 2 walk overflows,
+                                  // 3 distance tests, 4 Newton kernel terms
     const int32_t *groups;     // tree-node groups of the targets (null: blocks of 64 targets)
     const WNode *gnodes;       // node records the group ids refer to
 };
@@ -134,6 +135,7 @@ __global__ void __launch_bounds__(64) k_hsolve(HSolveArgs a)
     const int cap = a.cap;
     double *row = a.cand + (valid ? t : 0) * (int64_t)cap;
     int c = 0;
+    long long ntest = 0, nterm = 0;  // algorithmic work (distance tests, kernel terms)
     const bool ovf = group_walk(
         a.nodes, a.bucket, L, [&](const WNode *nd) { return box_box_dist2(gb, nd) >= Rg2; },
         [&](int q, int j) {
@@ -142,6 +144,7 @@ __global__ void __launch_bounds__(64) k_hsolve(HSolveArgs a)
             sz[q] = a.z[j];
         },
         [&](int cnt) {
+            if (valid) ntest += cnt;
             if (valid)
                 for (int k = 0; k < cnt; k++) {
                     double dx = sx[k] - xi, dy = sy[k] - yi, dz = sz[k] - zi;
@@ -208,6 +211,7 @@ __global__ void __launch_bounds__(64) k_hsolve(HSolveArgs a)
             // f(h) = (32/3) sum_j w(r_j/h) - n_ngb and f'(h), with the branch-free
             // cubic spline w(q) = (2-q)_+^3/4 - (1-q)_+^3 and q w'(q) = q (3 (1-q)_+^2 - 3/4 (2-q)_+^2)
             auto feval = [&](double hh, double &f, double &df) {
+                if (lsub == 0) nterm += cj;
                 double sl = 0.0, sq = 0.0;
                 const double ih = 1.0 / hh;
 #pragma unroll
@@ -315,6 +319,15 @@ __global__ void __launch_bounds__(64) k_hsolve(HSolveArgs a)
             }
         }
     }
+    // work totals: one spread atomic per wave
+    for (int off = 32; off >= 1; off >>= 1) {
+        ntest += __shfl_xor(ntest, off);
+        nterm += __shfl_xor(nterm, off);
+    }
+    if (lane == 0) {
+        spread_add(a.pending, 3, (unsigned long long)ntest);
+        spread_add(a.pending, 4, (unsigned long long)nterm);
+    }
     if (!valid) return;
     a.h[t] = h_res;
     a.R[t] = R_res;
@@ -381,12 +394,17 @@ __global__ void __launch_bounds__(256) k_hsolve_init(int64_t m, const double *__
 // the whole group iff boxdist(group box, node box)^2 >= (hmax_group +
 // hmax_node)^2, which by monotone rounding cannot drop a member.  Row order is
 // the (deterministic) order in which the walk stages buckets.
-template <bool FILL>
+// MODE 0: counts only; 1: fill CSR rows at off[i]; 2: fill padded rows
+// idx[i*stride + c] (c < stride) and counts in one walk, with spread counters
+// 0 += rows longer than stride, 1 += entries.
+template <int MODE>
 __global__ void __launch_bounds__(64) k_ngb(const int32_t *__restrict__ groups, const double *__restrict__ x,
                                             const double *__restrict__ y, const double *__restrict__ z,
                                             const double *__restrict__ h, const WNode *__restrict__ nodes, int bucket,
                                             int32_t *__restrict__ cnt, const int64_t *__restrict__ off,
-                                            int32_t *__restrict__ idx, unsigned long long *__restrict__ overflow)
+                                            int32_t *__restrict__ idx, int stride,
+                                            unsigned long long *__restrict__ overflow,
+                                            unsigned long long *__restrict__ spread)
 {
     __shared__ WalkLDS L;
     __shared__ double sx[WSRC], sy[WSRC], sz[WSRC], sh[WSRC];
@@ -401,7 +419,7 @@ __global__ void __launch_bounds__(64) k_ngb(const int32_t *__restrict__ groups, 
     if (valid) { xi = x[i]; yi = y[i]; zi = z[i]; hi = h[i]; }
     const double gb[6] = {g->lx, g->ly, g->lz, g->hx, g->hy, g->hz};
     const double hg = g->hmax;
-    int64_t o = (FILL && valid) ? off[i] : 0;
+    const int64_t o = !valid ? 0 : MODE == 1 ? off[i] : MODE == 2 ? i * (int64_t)stride : 0;
     int c = 0;
     const bool ovf = group_walk(
         nodes, bucket, L,
@@ -422,13 +440,24 @@ __global__ void __launch_bounds__(64) k_ngb(const int32_t *__restrict__ groups, 
                     double dx = xi - sx[k], dy = yi - sy[k], dz = zi - sz[k];
                     double r2 = dx * dx + dy * dy + dz * dz, t = hi + sh[k];
                     if (r2 < t * t) {
-                        if (FILL) idx[o + c] = sj[k];
+                        if (MODE == 1 || (MODE == 2 && c < stride)) idx[o + c] = sj[k];
                         c++;
                     }
                 }
         });
     if (ovf && lane == 0) atomicAdd(overflow, 1ull);
-    if (valid && !FILL) cnt[i] = c;
+    if (valid && MODE != 1) cnt[i] = c;
+    if (MODE == 2) {
+        unsigned long long nlong = (valid && c > stride) ? 1ull : 0ull, nent = valid ? (unsigned long long)c : 0ull;
+        for (int sh2 = 32; sh2 >= 1; sh2 >>= 1) {
+            nlong += __shfl_xor(nlong, sh2);
+            nent += __shfl_xor(nent, sh2);
+        }
+        if (lane == 0) {
+            if (nlong) spread_add(spread, 0, nlong);
+            spread_add(spread, 1, nent);
+        }
+    }
 }
 
 namespace sph {
@@ -447,11 +476,11 @@ hipError_t hsolve(hipStream_t st, const HSolveParams &p, int max_rounds, int64_t
     if (p.m <= 0) { *n_unconverged = 0; return hipSuccess; }
     if (p.bucket > 64 || p.cap > SPH_HCAP_SLOTS || p.cap < 1) return hipErrorInvalidValue;
     k_hsolve_init<<<sph_blocks(p.m, 256), 256, 0, st>>>(p.m, p.h_in, p.r_fac, p.R, p.h_out, p.steps, p.state);
-    unsigned long long hc[3] = {0, 0, 0};
+    unsigned long long hc[5] = {0, 0, 0, 0, 0};
     if (p.mode == 1) {
-        fill<unsigned long long>(st, 3 * SPH_SPREAD_WORDS, p.counters, 0ull);
+        fill<unsigned long long>(st, 5 * SPH_SPREAD_WORDS, p.counters, 0ull);
         k_hstep<<<sph_blocks(p.m, 256), 256, 0, st>>>(a);
-        spread_total(st, 3, p.counters, p.totals);
+        spread_total(st, 5, p.counters, p.totals);
         hipError_t e = hipMemcpyAsync(hc, p.totals, sizeof(hc), hipMemcpyDeviceToHost, st);
         if (e != hipSuccess) return e;
         e = hipStreamSynchronize(st);
@@ -462,13 +491,17 @@ hipError_t hsolve(hipStream_t st, const HSolveParams &p, int max_rounds, int64_t
     }
     const int64_t nblk = p.groups ? p.ngroups : (p.m + 63) / 64;
     for (int r = 0; r < max_rounds; r++) {
-        fill<unsigned long long>(st, 3 * SPH_SPREAD_WORDS, p.counters, 0ull);
+        fill<unsigned long long>(st, 5 * SPH_SPREAD_WORDS, p.counters, 0ull);
         k_hsolve<<<(unsigned)nblk, 64, 0, st>>>(a);
-        spread_total(st, 3, p.counters, p.totals);
+        spread_total(st, 5, p.counters, p.totals);
         hipError_t e = hipMemcpyAsync(hc, p.totals, sizeof(hc), hipMemcpyDeviceToHost, st);
         if (e != hipSuccess) return e;
         e = hipStreamSynchronize(st);
         if (e != hipSuccess) return e;
+        if (p.work) {
+            p.work[0] += (int64_t)hc[3];
+            p.work[1] += (int64_t)hc[4];
+        }
         if (hc[2]) return hipErrorIllegalState;  // walk stack overflow (cannot happen for valid trees)
         if (p.unconverged_total) *p.unconverged_total += (int64_t)hc[1];
         if (p.rounds) *p.rounds = r + 1;
@@ -485,7 +518,8 @@ hipError_t neighbours(hipStream_t st, int64_t n, int64_t ngroups, const int32_t 
 {
     if (bucket > 64) return hipErrorInvalidValue;
     fill<unsigned long long>(st, 1, overflow, 0ull);
-    k_ngb<false><<<(unsigned)ngroups, 64, 0, st>>>(groups, x, y, z, h, nodes, bucket, cnt, nullptr, nullptr, overflow);
+    k_ngb<0><<<(unsigned)ngroups, 64, 0, st>>>(groups, x, y, z, h, nodes, bucket, cnt, nullptr, nullptr, 0, overflow,
+                                               nullptr);
     hipError_t e = scan_counts(st, n, cnt, off, scan_tmp);
     if (e != hipSuccess) return e;
     unsigned long long hov = 0;
@@ -497,7 +531,8 @@ hipError_t neighbours(hipStream_t st, int64_t n, int64_t ngroups, const int32_t 
     if (e != hipSuccess) return e;
     if (hov) return hipErrorIllegalState;
     if (*total > idx_cap || idx == nullptr) return hipSuccess;  // caller grows idx and calls neighbours_fill
-    k_ngb<true><<<(unsigned)ngroups, 64, 0, st>>>(groups, x, y, z, h, nodes, bucket, nullptr, off, idx, overflow);
+    k_ngb<1><<<(unsigned)ngroups, 64, 0, st>>>(groups, x, y, z, h, nodes, bucket, nullptr, off, idx, 0, overflow,
+                                               nullptr);
     return hipGetLastError();
 }
 
@@ -505,7 +540,32 @@ hipError_t neighbours_fill(hipStream_t st, int64_t ngroups, const int32_t *group
                            const double *z, const double *h, const WNode *nodes, int bucket, const int64_t *off,
                            int32_t *idx, unsigned long long *overflow)
 {
-    k_ngb<true><<<(unsigned)ngroups, 64, 0, st>>>(groups, x, y, z, h, nodes, bucket, nullptr, off, idx, overflow);
+    k_ngb<1><<<(unsigned)ngroups, 64, 0, st>>>(groups, x, y, z, h, nodes, bucket, nullptr, off, idx, 0, overflow,
+                                               nullptr);
+    return hipGetLastError();
+}
+
+hipError_t neighbour_rows(hipStream_t st, int64_t ngroups, const int32_t *groups, const double *x, const double *y,
+                          const double *z, const double *h, const WNode *nodes, int bucket, int stride, int32_t *cnt,
+                          int32_t *idx, unsigned long long *overflow, unsigned long long *spread,
+                          unsigned long long *totals, int64_t *n_long, int64_t *total)
+{
+    if (bucket > 64 || stride < 1) return hipErrorInvalidValue;
+    fill<unsigned long long>(st, 1, overflow, 0ull);
+    fill<unsigned long long>(st, 2 * SPH_SPREAD_WORDS, spread, 0ull);
+    k_ngb<2><<<(unsigned)ngroups, 64, 0, st>>>(groups, x, y, z, h, nodes, bucket, cnt, nullptr, idx, stride, overflow,
+                                               spread);
+    spread_total(st, 2, spread, totals);
+    unsigned long long hc[3] = {0, 0, 0};
+    hipError_t e = hipMemcpyAsync(hc, totals, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(hc + 2, overflow, sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    if (hc[2]) return hipErrorIllegalState;
+    *n_long = (int64_t)hc[0];
+    *total = (int64_t)hc[1];
     return hipGetLastError();
 }
 
