@@ -5,13 +5,15 @@ R = 1, G = 1, rho ~ 1/r from r = R sqrt(U), seeded isotropic directions,
 u = 0.05, gamma = 5/3), cubic spline, N_ngb = 64 (FP64 Newton h solve),
 theta = 0.5, Plummer eps = 1e-3, alpha = 1 with the Balsara switch,
 dt = 1e-3.  One "step" is the full KDK step of sph_serial_v2.py on the
-device: kick/drift, Morton sort, tree, h solve, CSR neighbours, density+EOS,
-tree gravity, div/curl, acceleration, kick, du/dt, u update.
+device: kick/drift, Morton sort, tree, h solve, neighbour rows,
+density+EOS+div/curl, tree gravity, acceleration, kick, du/dt, u update.
 
 Prints ONE JSON line: particle-steps/s (whole job, inputs resident in HBM),
-the roofline of the dominant kernel from HIP events on the library's stream,
-the all-FP64 form of the same kernels, and the CPU baseline (the oracle's
-FP64 all-pairs restatement of the reference step, row-sampled).
+the roofline of the dominant kernel from HIP events on the library's stream
+(HBM traffic per launch from the committed PMC summary), the all-FP64 form
+of the same kernels, the CPU baseline (the oracle's FP64 all-pairs
+restatement of the reference step, row-sampled) and the cooling surrogate at
+4,194,304 particles (training, gated inference, MFMA roofline, fallback rate).
 """
 import argparse
 import json
@@ -112,16 +114,90 @@ def cpu_baseline(sim_state, n, m, rows, seed=0):
     return step_s, sample_s, t
 
 
+MLP_FLOP_PER_SAMPLE = 2 * (4 * 128 + 2 * 128 * 128 + 128)  # forward multiply-adds x 2
+BF16_MFMA_PEAK_TFLOPS = 2500.0
+
+
+def traffic_table():
+    """HBM bytes per launch measured by rocprofv3 FETCH_SIZE/WRITE_SIZE runs (scaled by
+    the copy-kernel calibration), from the JSON block of profiles/hbm_traffic_evrard_1M.md."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_evrard_1M.md")
+    try:
+        text = open(path).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return json.loads(block)
+    except (OSError, IndexError, ValueError):
+        return {}
+
+
+def mlp_section(n_inf):
+    """Cooling surrogate at the 4M-particle cooling configuration: on-device training
+    (2M exact samples), calibration, gated inference throughput and accuracy."""
+    from sph_3_amd import cooling as C
+
+    tab = C.CoolingTable()
+    net = C.CoolingSurrogate(tab, seed=0)
+    n_samples, epochs = 2_000_000, 200
+    t0 = time.perf_counter()
+    losses = net.train(n_samples=n_samples, epochs=epochs)
+    t_train = time.perf_counter() - t0
+    net.calibrate()
+    tol = 0.05
+    tm = net.timing(n=n_inf, reps=20, tol=tol)
+    fwd = MLP_FLOP_PER_SAMPLE * n_inf / (tm["ms_network"] * 1e-3) / 1e12
+    # accuracy on random points of the box against the exact FP64 function
+    rng = np.random.default_rng(5)
+    lo = np.array([r[0] for r in C.TABLE_RANGES])
+    hi = np.array([r[1] for r in C.TABLE_RANGES])
+    v = rng.uniform(lo, hi, (200_000, 4))
+    _, lam = C.coolingHeatingRates(10 ** v[:, 1], 10 ** v[:, 0], J21=10 ** v[:, 3], Z=10 ** v[:, 2])
+    exact = np.log10(np.maximum(lam, 1e-40))
+    lg, _, _ = net.rates(*v.T, tol=tol)
+    lt, _ = tab.lookup(*v.T)
+    rms = lambda e: float(np.sqrt(np.mean(e * e)))  # noqa: E731
+    return {
+        "workload": f"cooling surrogate 4-128-128-128-1 SiLU, {n_inf} particles uniform in the table box",
+        "train": {"samples": n_samples, "epochs": epochs, "seconds": t_train, "final_loss": float(losses[-1]),
+                  "tflops": 3 * MLP_FLOP_PER_SAMPLE * n_samples * epochs / t_train / 1e12},
+        "ms_gated": tm["ms_gated"], "ms_network": tm["ms_network"], "ms_table_lookup": tm["ms_table"],
+        "fallback_fraction": tm["fallback_fraction"], "tol_dex": tol,
+        "rms_error_dex": {"gated": rms(lg - exact), "table": rms(lt - exact)},
+        "bytes": {"network": net.nbytes, "table_log_lambda": 4 * int(np.prod([r[2] for r in C.TABLE_RANGES]))},
+        "roofline": {"kernel": "k_mlp_forward<true>", "bound": "mfma", "achieved": fwd,
+                     "peak": BF16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": fwd / BF16_MFMA_PEAK_TFLOPS,
+                     "algorithmic": f"{MLP_FLOP_PER_SAMPLE} flop x {n_inf} samples",
+                     "ms_per_launch": tm["ms_network"]},
+    }
+
+
+def copy_probe(nbytes):
+    """A plain device copy of known size (the calibration run of the HBM traffic counters)."""
+    import torch
+
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "launches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1048576)
-    ap.add_argument("--cpu-rows", type=int, default=512)
+    ap.add_argument("--cpu-rows", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-form", action="store_true")
+    ap.add_argument("--no-mlp", action="store_true")
+    ap.add_argument("--mlp-n", type=int, default=4194304)
+    ap.add_argument("--copy-probe", type=int, default=0, help="only run a device copy of this many bytes")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); sharding the particle "
               "set over several GPUs is designed but not built or measured", file=sys.stderr)
@@ -223,6 +299,13 @@ def main():
         "pass_bandwidth": hbm,
         "roofline": roof,
     }
+    tr = traffic_table().get(roof["kernel"])
+    if tr:
+        roof["traffic"] = tr["fetch_bytes"] + tr["write_bytes"]
+        roof["traffic_source"] = "profiles/hbm_traffic_evrard_1M.md"
+    if not args.no_mlp:
+        log("[bench] cooling surrogate: train, calibrate, time")
+        out["mlp_cooling"] = mlp_section(args.mlp_n)
     if not args.no_fp64_form:
         sim64, dt64 = run("fp64", max(2, args.steps // 2), 1)
         out["fp64_form"] = {"ms_per_step": 1e3 * dt64 / max(2, args.steps // 2),
