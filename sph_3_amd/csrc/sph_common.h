@@ -119,6 +119,16 @@ template <typename R> __device__ __forceinline__ R sph_rsqrt(R x);
 template <> __device__ __forceinline__ float sph_rsqrt<float>(float x) { return __builtin_amdgcn_rsqf(x); }
 template <> __device__ __forceinline__ double sph_rsqrt<double>(double x) { return 1.0 / __builtin_sqrt(x); }
 
+// Workgroups are dispatched round-robin over the eight XCDs (XCD = blockIdx % 8),
+// each with its own L2.  xcd_block(b, nb) is a bijection of [0, nb) that gives
+// each XCD one contiguous range of work items, so spatially adjacent (Morton
+// consecutive) groups share an L2 instead of being spread over all eight.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb)
+{
+    const int64_t q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
+
 // Host-side launch helper.
 static inline unsigned sph_blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 

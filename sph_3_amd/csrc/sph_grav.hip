@@ -18,9 +18,12 @@
 // to Kahan-compensated totals, in the pass type R.  In the FP32
 // form each relative coordinate is an unevaluated pair hi + lo of floats and
 // a pair difference is (s_hi - t_hi) + (s_lo - t_lo), so near pairs keep full
-// FP32 accuracy.  Every particle of a group sees the same list: the counted
+// FP32 accuracy; sources whose node lies farther from the group box than a
+// quarter of the group's extent skip the lo parts (their offsets dwarf the
+// rounding).  Every particle of a group sees the same list: the counted
 // interactions are n_nodes*g + n_parts*g - g per group of g particles.
 #include "sph_grav.h"
+#include "sph_walk.h"
 
 #define GG 64        // maximum targets per group (one wavefront)
 #define GSTK 2048    // LDS stack capacity (node ids)
@@ -58,11 +61,11 @@ __device__ __forceinline__ void split(double d, R &hi, R &lo)
     lo = (R)(d - (double)hi);
 }
 
-template <typename R, int SOFT>
+template <typename R, int SOFT, bool LO>
 __device__ __forceinline__ void interact(const Src4<R> &a, const Src4<R> &b, R tx, R ty, R tz, R tlx, R tly, R tlz,
                                          R epsi, R e2, R &sx, R &sy, R &sz)
 {
-    constexpr bool SPLIT = sizeof(R) == 4;
+    constexpr bool SPLIT = LO && sizeof(R) == 4;
     R dx = a.x - tx, dy = a.y - ty, dz = a.z - tz;
     if (SPLIT) {
         dx += b.x - tlx;
@@ -83,24 +86,39 @@ __device__ __forceinline__ void interact(const Src4<R> &a, const Src4<R> &b, R t
     sz = fma(w, dz, sz);
 }
 
-// Evaluate cnt (<= GSRC) staged sources.  Two independent chains of plain
-// sums (ILP); each chain's partial over this flush (<= GSRC/2 terms) is then
-// added to its Kahan accumulator, so the list total is compensated at the
-// granularity of one flush.
-template <typename R, int SOFT>
-__device__ __forceinline__ void eval_sources(int cnt, const Src4<R> *__restrict__ A, const Src4<R> *__restrict__ B,
-                                             R tx, R ty, R tz, R tlx, R tly, R tlz, R epsi, R e2, Kahan<R> *k)
+// Evaluate cnt staged sources (LO: with the lo parts of the offsets).  Two
+// independent chains of plain sums (ILP); each chain's partial over this flush
+// (<= GSRC/2 terms) is then added to its Kahan accumulator, so the list total
+// is compensated at the granularity of one flush.
+template <typename R, int SOFT, bool LO>
+__device__ __forceinline__ void eval_run(int cnt, const Src4<R> *__restrict__ A, const Src4<R> *__restrict__ B,
+                                         R tx, R ty, R tz, R tlx, R tly, R tlz, R epsi, R e2, Kahan<R> *k)
 {
     R s[6] = {(R)0, (R)0, (R)0, (R)0, (R)0, (R)0};
     int q = 0;
     for (; q + 1 < cnt; q += 2) {
         const Src4<R> a0 = A[q], b0 = B[q], a1 = A[q + 1], b1 = B[q + 1];
-        interact<R, SOFT>(a0, b0, tx, ty, tz, tlx, tly, tlz, epsi, e2, s[0], s[1], s[2]);
-        interact<R, SOFT>(a1, b1, tx, ty, tz, tlx, tly, tlz, epsi, e2, s[3], s[4], s[5]);
+        interact<R, SOFT, LO>(a0, b0, tx, ty, tz, tlx, tly, tlz, epsi, e2, s[0], s[1], s[2]);
+        interact<R, SOFT, LO>(a1, b1, tx, ty, tz, tlx, tly, tlz, epsi, e2, s[3], s[4], s[5]);
     }
-    if (q < cnt) interact<R, SOFT>(A[q], B[q], tx, ty, tz, tlx, tly, tlz, epsi, e2, s[0], s[1], s[2]);
+    if (q < cnt) interact<R, SOFT, LO>(A[q], B[q], tx, ty, tz, tlx, tly, tlz, epsi, e2, s[0], s[1], s[2]);
 #pragma unroll
     for (int c = 0; c < 6; c++) k[c].add(s[c]);
+}
+
+// The staging buffer holds near sources at [0, nn) and far ones at
+// [GSRC - nfar, GSRC): a source is near when its node's box lies closer to the
+// group box than D/4 (D: the group's extent).  Group-relative FP32 coordinates
+// round by <= 2^-24 (|d| + D/2) per axis, so a far offset (|d| >= D/4) keeps a
+// relative error <= 3 * 2^-24 without the lo parts; near ones need them.
+template <typename R, int SOFT>
+__device__ __forceinline__ void eval_sources(int nn, int nfar, const Src4<R> *__restrict__ A,
+                                             const Src4<R> *__restrict__ B, R tx, R ty, R tz, R tlx, R tly, R tlz,
+                                             R epsi, R e2, Kahan<R> *k)
+{
+    if (nn > 0) eval_run<R, SOFT, true>(nn, A, B, tx, ty, tz, tlx, tly, tlz, epsi, e2, k);
+    if (nfar > 0)
+        eval_run<R, SOFT, false>(nfar, A + (GSRC - nfar), B + (GSRC - nfar), tx, ty, tz, tlx, tly, tlz, epsi, e2, k);
 }
 
 template <typename R, int SOFT>
@@ -114,10 +132,9 @@ __global__ void __launch_bounds__(GG) k_gravity_group(const int32_t *__restrict_
 {
     __shared__ int stk[GSTK];
     __shared__ Src4<R> A[GSRC], B[GSRC];
-    __shared__ int bfirst[GG], bcount[GG];
-    constexpr bool SPLIT = sizeof(R) == 4;
+    __shared__ int bfirst[GG], bcount[GG], bnear[GG];
     const int lane = threadIdx.x;
-    const int gnode = groups[blockIdx.x];
+    const int gnode = groups[xcd_block(blockIdx.x, gridDim.x)];
     const int g0 = nodes[gnode].first, ng = nodes[gnode].count;
     const bool valid = lane < ng;
     const int64_t i = (int64_t)g0 + lane;
@@ -131,6 +148,7 @@ __global__ void __launch_bounds__(GG) k_gravity_group(const int32_t *__restrict_
     const WNode *gn = nodes + gnode;
     const double gb[6] = {gn->lx, gn->ly, gn->lz, gn->hx, gn->hy, gn->hz};
     const double gcx = 0.5 * (gb[0] + gb[3]), gcy = 0.5 * (gb[1] + gb[4]), gcz = 0.5 * (gb[2] + gb[5]);
+    const double D2 = 0.0625 * gn->size * gn->size;  // near: node box closer than D/4
     R tx, ty, tz, tlx, tly, tlz;
     split<R>(xi - gcx, tx, tlx);
     split<R>(yi - gcy, ty, tly);
@@ -139,7 +157,7 @@ __global__ void __launch_bounds__(GG) k_gravity_group(const int32_t *__restrict_
     const R e2 = (R)(eps_plummer * eps_plummer);
     Kahan<R> k[6];
     long long n_nodes = 0, n_parts = 0;
-    int nsrc = 0;
+    int nn = 0, nfar = 0;  // staged near sources [0, nn), far ones [GSRC - nfar, GSRC)
     int sp = 1;
     if (lane == 0) stk[0] = 0;
     __syncthreads();
@@ -150,7 +168,7 @@ __global__ void __launch_bounds__(GG) k_gravity_group(const int32_t *__restrict_
         const int node = lane < nt ? stk[base + lane] : -1;
         __syncthreads();
         sp = base;
-        bool acc = false, bkt = false, opn = false;
+        bool acc = false, bkt = false, opn = false, near = false;
         double ncx = 0, ncy = 0, ncz = 0, nmass = 0, neps = 0;
         int nfirst = 0, ncount = 0, nleft = 0, nright = 0;
         if (node >= 0) {
@@ -158,6 +176,7 @@ __global__ void __launch_bounds__(GG) k_gravity_group(const int32_t *__restrict_
             ncx = nd->cx; ncy = nd->cy; ncz = nd->cz; nmass = nd->mass; neps = nd->epsmax;
             nfirst = nd->first; ncount = nd->count; nleft = nd->left; nright = nd->right;
             acc = mac_group(gb, ncx, ncy, ncz, nd->lx, nd->ly, nd->lz, nd->hx, nd->hy, nd->hz, nd->size, theta2);
+            near = box_box_dist2(gb, nd) < D2;
             if (!acc) {
                 if (ncount <= bucket) bkt = true;
                 else opn = true;
@@ -175,8 +194,9 @@ __global__ void __launch_bounds__(GG) k_gravity_group(const int32_t *__restrict_
         }
         sp += 2 * (int)__popcll(mo);
         if (sp > GSTK) sp = GSTK;
+        const uint64_t man = wave_ballot(acc && near), maf = ma & ~man;
         if (acc) {
-            const int q = nsrc + (int)__popcll(ma & lt);
+            const int q = near ? nn + (int)__popcll(man & lt) : GSRC - 1 - nfar - (int)__popcll(maf & lt);
             Src4<R> a, b;
             split<R>(ncx - gcx, a.x, b.x);
             split<R>(ncy - gcy, a.y, b.y);
@@ -186,24 +206,26 @@ __global__ void __launch_bounds__(GG) k_gravity_group(const int32_t *__restrict_
             A[q] = a;
             B[q] = b;
         }
-        nsrc += (int)__popcll(ma);
+        nn += (int)__popcll(man);
+        nfar += (int)__popcll(maf);
         n_nodes += (long long)__popcll(ma);
         if (bkt) {
             const int pb = (int)__popcll(mb & lt);
             bfirst[pb] = nfirst;
             bcount[pb] = ncount;
+            bnear[pb] = near ? 1 : 0;
         }
         const int nb = (int)__popcll(mb);
         __syncthreads();
-        if (nsrc >= GG) {
-            eval_sources<R, SOFT>(nsrc, A, B, tx, ty, tz, tlx, tly, tlz, te, e2, k);
-            nsrc = 0;
+        if (nn + nfar >= GG) {
+            eval_sources<R, SOFT>(nn, nfar, A, B, tx, ty, tz, tlx, tly, tlz, te, e2, k);
+            nn = nfar = 0;
             __syncthreads();
         }
         for (int bb = 0; bb < nb; bb++) {
-            const int f = bfirst[bb], c = bcount[bb];
+            const int f = bfirst[bb], c = bcount[bb], bn = bnear[bb];
             if (lane < c) {
-                const int j = f + lane, q = nsrc + lane;
+                const int j = f + lane, q = bn ? nn + lane : GSRC - 1 - nfar - lane;
                 Src4<R> a, b;
                 split<R>(x[j] - gcx, a.x, b.x);
                 split<R>(y[j] - gcy, a.y, b.y);
@@ -213,17 +235,18 @@ __global__ void __launch_bounds__(GG) k_gravity_group(const int32_t *__restrict_
                 A[q] = a;
                 B[q] = b;
             }
-            nsrc += c;
+            if (bn) nn += c;
+            else nfar += c;
             n_parts += c;
             __syncthreads();
-            if (nsrc >= GG) {
-                eval_sources<R, SOFT>(nsrc, A, B, tx, ty, tz, tlx, tly, tlz, te, e2, k);
-                nsrc = 0;
+            if (nn + nfar >= GG) {
+                eval_sources<R, SOFT>(nn, nfar, A, B, tx, ty, tz, tlx, tly, tlz, te, e2, k);
+                nn = nfar = 0;
                 __syncthreads();
             }
         }
     }
-    if (nsrc > 0) eval_sources<R, SOFT>(nsrc, A, B, tx, ty, tz, tlx, tly, tlz, te, e2, k);
+    if (nn + nfar > 0) eval_sources<R, SOFT>(nn, nfar, A, B, tx, ty, tz, tlx, tly, tlz, te, e2, k);
     if (valid) {
         ax[i] = G * ((double)k[0].value() + (double)k[3].value());
         ay[i] = G * ((double)k[1].value() + (double)k[4].value());

@@ -25,7 +25,7 @@ __global__ void __launch_bounds__(256) k_density(int64_t n, const double *__rest
                                                  R *__restrict__ rho, R *__restrict__ P, R *__restrict__ c,
                                                  R *__restrict__ prho2)
 {
-    int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     int64_t i = gid / G;
     int l = (int)(gid % G);
     bool valid = i < n;
@@ -64,7 +64,7 @@ __global__ void __launch_bounds__(256) k_divcurl(int64_t n, const double *__rest
                                                  const R *__restrict__ c, NgbRows rows, R *__restrict__ divV,
                                                  R *__restrict__ curlV, R *__restrict__ bal)
 {
-    int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     int64_t i = gid / G;
     int l = (int)(gid % G);
     bool valid = i < n;
@@ -116,7 +116,7 @@ __global__ void __launch_bounds__(256) k_density_divcurl(
     NgbRows rows, R *__restrict__ rho, R *__restrict__ P,
     R *__restrict__ c, R *__restrict__ prho2, R *__restrict__ divV, R *__restrict__ curlV, R *__restrict__ bal)
 {
-    int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     int64_t i = gid / G;
     int l = (int)(gid % G);
     bool valid = i < n;
@@ -188,7 +188,7 @@ __global__ void __launch_bounds__(256) k_accsph(int64_t n, const double *__restr
                                                 const double *__restrict__ gz_in, double *__restrict__ ax,
                                                 double *__restrict__ ay, double *__restrict__ az)
 {
-    int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     int64_t i = gid / G;
     int l = (int)(gid % G);
     bool valid = i < n;
@@ -237,7 +237,7 @@ __global__ void __launch_bounds__(256) k_dudt(int64_t n, const double *__restric
                                               NgbRows rows,
                                               R *__restrict__ dudt)
 {
-    int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     int64_t i = gid / G;
     int l = (int)(gid % G);
     bool valid = i < n;

@@ -101,11 +101,11 @@ __device__ __forceinline__ void group_range(const int32_t *groups, const WNode *
                                             int &count)
 {
     if (groups) {
-        const WNode *g = gnodes + groups[blockIdx.x];
+        const WNode *g = gnodes + groups[xcd_block(blockIdx.x, gridDim.x)];
         first = g->first;
         count = g->count;
     } else {
-        first = (int64_t)blockIdx.x * 64;
+        first = xcd_block(blockIdx.x, gridDim.x) * 64;
         count = (int)((m - first) < 64 ? (m - first) : 64);
     }
 }
@@ -409,7 +409,7 @@ __global__ void __launch_bounds__(64) k_ngb(const int32_t *__restrict__ groups, 
     __shared__ WalkLDS L;
     __shared__ double sx[WSRC], sy[WSRC], sz[WSRC], sh[WSRC];
     __shared__ int sj[WSRC];
-    const WNode *g = nodes + groups[blockIdx.x];
+    const WNode *g = nodes + groups[xcd_block(blockIdx.x, gridDim.x)];
     const int64_t first = g->first;
     const int ng = g->count;
     const int lane = threadIdx.x;
