@@ -109,8 +109,9 @@ __device__ __forceinline__ void eval_run(int cnt, const Src4<R> *__restrict__ A,
 // The staging buffer holds near sources at [0, nn) and far ones at
 // [GSRC - nfar, GSRC): a source is near when its node's box lies closer to the
 // group box than D/4 (D: the group's extent).  Group-relative FP32 coordinates
-// round by <= 2^-24 (|d| + D/2) per axis, so a far offset (|d| >= D/4) keeps a
-// relative error <= 3 * 2^-24 without the lo parts; near ones need them.
+// round the source by <= 2^-24 (|d| + D/2) and the target by <= 2^-24 D/2 per
+// axis, so a far offset (|d| >= D/4) keeps a relative error <= 5 * 2^-24
+// without the lo parts; near ones need them.
 template <typename R, int SOFT>
 __device__ __forceinline__ void eval_sources(int nn, int nfar, const Src4<R> *__restrict__ A,
                                              const Src4<R> *__restrict__ B, R tx, R ty, R tz, R tlx, R tly, R tlz,
