@@ -293,6 +293,7 @@ def main():
         "phase_ms_per_step": per_step,
         "h_solve_fp64_tflops": h_flops / (phases["hsolve"] * 1e-3) / 1e12 if "hsolve" in phases else None,
         "neighbours_per_particle": kbar,
+        "neighbour_pairs_from_other_side": st["ngb_extra"],
         "h_solve_rounds_last_step": st["h_rounds"],
         "gravity_interactions_per_particle": (st["grav_pp"] + st["grav_pn"]) / n,
         "gravity_tflops": grav_tflops,

@@ -216,6 +216,7 @@ typedef struct sph_stats {
     int64_t phase_calls[16];
     int64_t h_tests;         /* last h solve: FP64 distance tests of its walks */
     int64_t h_terms;         /* last h solve: kernel terms summed by its Newton iterations */
+    int64_t ngb_extra;       /* last step: neighbour pairs found only from the other particle's candidates */
 } sph_stats;
 
 /* phase indices of sph_stats.phase_ms (SPH_PH_HMOMENTS: the node boxes built for the h walk;

@@ -83,6 +83,7 @@ struct sph_ctx {
     Buf rho, P, c, prho2, divV, curlV, bal, du;
     // h solve scratch
     Buf cand, hR, hsteps, hstate, hcount, hout, hin, tx, ty, tz;
+    Buf ccidx, cccnt, cext, cnext;  // candidate indices, counts, extra pairs and their region counters
     // staging and reductions
     Buf stage, kpart, kout;
     // spread event counters (sph_common.h): SPR_H.. h solve (5), SPR_G.. gravity (3);
@@ -321,7 +322,7 @@ int ensure_moments(sph_ctx *ctx)
 // Neighbour sets of the current positions and h: padded rows of
 // SPH_NGB_STRIDE entries built in one walk, or (csr, or a row longer than the
 // stride) the exact two-pass CSR.
-int build_ngb(sph_ctx *ctx, bool csr)
+int build_ngb(sph_ctx *ctx, bool csr, const double *hscale = nullptr)
 {
     const int64_t n = ctx->n;
     unsigned long long *ovf = P_<unsigned long long>(ctx->hcount) + 8;
@@ -331,7 +332,7 @@ int build_ngb(sph_ctx *ctx, bool csr)
         HIPCK(sph::neighbour_rows(ctx->st, ctx->ngroups, I32(groups), D(x), D(y), D(z), D(h), P_<WNode>(ctx->nodes),
                                   SPH_BUCKET, SPH_NGB_STRIDE, I32(cnt), I32(idx), ovf,
                                   P_<unsigned long long>(ctx->spread) + SPR_N * SPH_SPREAD_WORDS,
-                                  P_<unsigned long long>(ctx->hcount) + 10, &nlong, &total));
+                                  P_<unsigned long long>(ctx->hcount) + 10, &nlong, &total, hscale));
         if (nlong == 0) {
             ctx->ngb_stride = SPH_NGB_STRIDE;
             ctx->csr_total = total;
@@ -343,11 +344,11 @@ int build_ngb(sph_ctx *ctx, bool csr)
     int64_t cap = ctx->idx.p ? (int64_t)(ctx->idx.bytes / 4) : 0;
     HIPCK(sph::neighbours(ctx->st, n, ctx->ngroups, I32(groups), D(x), D(y), D(z), D(h), P_<WNode>(ctx->nodes),
                           SPH_BUCKET, I32(cnt), P_<int64_t>(ctx->off), P_<int64_t>(ctx->scantmp), ovf, I32(idx), cap,
-                          &total));
+                          &total, hscale));
     if (total > cap) {
         CK(ensure(ctx, ctx->idx, (size_t)total * 4));
         HIPCK(sph::neighbours_fill(ctx->st, ctx->ngroups, I32(groups), D(x), D(y), D(z), D(h), P_<WNode>(ctx->nodes),
-                                   SPH_BUCKET, P_<int64_t>(ctx->off), I32(idx), ovf));
+                                   SPH_BUCKET, P_<int64_t>(ctx->off), I32(idx), ovf, hscale));
     }
     ctx->ngb_stride = 0;
     ctx->csr_total = total;
@@ -446,7 +447,8 @@ extern "C" int sph_ctx_destroy(sph_ctx *ctx)
                   &ctx->cnt, &ctx->off, &ctx->scantmp, &ctx->idx, &ctx->rho, &ctx->P, &ctx->c, &ctx->prho2,
                   &ctx->divV, &ctx->curlV, &ctx->bal, &ctx->du, &ctx->cand, &ctx->hR, &ctx->hsteps,
                   &ctx->hstate, &ctx->hcount, &ctx->hout, &ctx->hin, &ctx->tx, &ctx->ty, &ctx->tz,
-                  &ctx->stage, &ctx->kpart, &ctx->kout, &ctx->spread};
+                  &ctx->stage, &ctx->kpart, &ctx->kout, &ctx->spread, &ctx->ccidx, &ctx->cccnt,
+                  &ctx->cext, &ctx->cnext};
     for (Buf *b : all)
         if (b->p) (void)hipFree(b->p);
     Buf *cool[] = {&ctx->rf1,   &ctx->ctabL,  &ctx->ctabG,  &ctx->cin[0], &ctx->cin[1], &ctx->cin[2], &ctx->cin[3],
@@ -473,9 +475,13 @@ namespace {
 
 int run_hsolve(sph_ctx *ctx, int64_t m, const double *tx, const double *ty, const double *tz, int mode,
                double param, int n_low, int n_up, int max_steps, double tol, int64_t *n_unconv, int64_t *rounds_out,
-               int64_t *work = nullptr)
+               int64_t *work = nullptr, bool keep_indices = false)
 {
     CK(ensure(ctx, ctx->cand, (size_t)m * SPH_HCAP * 8));
+    if (keep_indices) {
+        CK(ensure(ctx, ctx->ccidx, (size_t)m * SPH_HCAP * 4));
+        CK(ensure(ctx, ctx->cccnt, (size_t)m * 4));
+    }
     sph::HSolveParams p;
     p.m = m;
     p.tx = tx; p.ty = ty; p.tz = tz;
@@ -503,6 +509,8 @@ int run_hsolve(sph_ctx *ctx, int64_t m, const double *tx, const double *ty, cons
     p.ngroups = ctx->ngroups;
     p.rounds = rounds_out;
     p.work = work;
+    p.cidx = keep_indices ? I32(ccidx) : nullptr;
+    p.ccnt = keep_indices ? I32(cccnt) : nullptr;
     int64_t pending = 0;
     HIPCK(sph::hsolve(ctx->st, p, SPH_HROUNDS, &pending));
     if (pending) return fail(ctx, SPH_ERR_NOCONV, "smoothing-length solve: " + std::to_string(pending) +
@@ -864,20 +872,50 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     ctx->stats.h_tests = ctx->stats.h_terms = 0;
     int64_t work[2] = {0, 0};
     CK(run_hsolve(ctx, n, D(x), D(y), D(z), SPH_H_NEWTON, p->n_ngb, 0, 0, 0, p->h_tol, &unc, &ctx->stats.h_rounds,
-                  work));
+                  work, true));
     ctx->stats.h_tests = work[0];
     ctx->stats.h_terms = work[1];
     std::swap(ctx->h, ctx->hout);
     ctx->stats.h_unconverged += unc;
     ph_end(ctx, SPH_PH_HSOLVE);
-    // --- moments with the new h (and eps = h for the spline softening)
+    // --- node h_max (and eps_max = h_max for the spline softening) of the new h.  Boxes, masses
+    // and centres of mass do not depend on h: with Plummer softening (or no gravity) the
+    // neighbour walk prunes with the h_max of the boxes' pass scaled by max_i h_i/h_old_i
+    // instead of a second bottom-up pass; the spline softening needs the exact eps_max.
+    const bool spline = p->soft_kind == SPH_SOFT_SPLINE && p->G != 0.0;
+    const double *eps = spline ? D(h) : nullptr;
     ph_begin(ctx, SPH_PH_MOMENTS);
-    const double *eps = p->soft_kind == SPH_SOFT_SPLINE ? D(h) : nullptr;
-    HIPCK(sph::tree_moments(st, n, D(x), D(y), D(z), D(m), D(h), eps, P_<WNode>(ctx->nodes), I32(right),
-                            I32(parent), P_<uint32_t>(ctx->flags), I32(front)));
+    if (spline)
+        HIPCK(sph::tree_moments(st, n, D(x), D(y), D(z), D(m), D(h), eps, P_<WNode>(ctx->nodes), I32(right),
+                                I32(parent), P_<uint32_t>(ctx->flags), I32(front)));
     ph_end(ctx, SPH_PH_MOMENTS);
+    // --- neighbour rows from the h solve's candidates; a tree walk if a row or the extras overflow
     ph_begin(ctx, SPH_PH_NGB);
-    CK(build_ngb(ctx, false));
+    {
+        CK(ensure(ctx, ctx->idx, (size_t)n * SPH_NGB_STRIDE * 4));
+        const int64_t ext_cap = std::max<int64_t>(4 * n, SPH_SPREAD * 64);
+        CK(ensure(ctx, ctx->cext, (size_t)ext_cap * 8));
+        CK(ensure(ctx, ctx->cnext, (size_t)SPH_SPREAD_WORDS * 8));
+        int64_t nlong = 0, total = 0, nextra = 0;
+        HIPCK(sph::neighbour_rows_from_candidates(
+            st, n, SPH_HCAP, D(cand), I32(ccidx), I32(cccnt), D(h), D(hR), SPH_NGB_STRIDE, I32(cnt), I32(idx),
+            P_<int2>(ctx->cext), ext_cap, P_<unsigned long long>(ctx->cnext),
+            P_<unsigned long long>(ctx->spread) + SPR_N * SPH_SPREAD_WORDS, P_<unsigned long long>(ctx->hcount) + 10,
+            &nlong, &total, &nextra));
+        ctx->stats.ngb_extra = nextra;
+        if (nlong == 0) {
+            ctx->ngb_stride = SPH_NGB_STRIDE;
+            ctx->csr_total = total;
+            ctx->stats.ngb_total = total;
+        } else {
+            const double *hscale = nullptr;
+            if (!spline) {  // node h_max are those of h_old: scale them
+                HIPCK(sph::h_ratio_max(st, n, D(h), D(hin), D(kout) + 7));  // slot 7 (kinetics uses 0..4)
+                hscale = D(kout) + 7;
+            }
+            CK(build_ngb(ctx, false, hscale));
+        }
+    }
     ph_end(ctx, SPH_PH_NGB);
     const NgbRows rows = ngb_rows(ctx);
     // density + EOS, fused with div/curl v and the Balsara factor when the switch is on

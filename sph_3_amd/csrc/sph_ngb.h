@@ -23,6 +23,8 @@ struct HSolveParams {
     const double *h_in;          // start values (device)
     double *h_out;               // result (device)
     double *cand, *R;            // scratch: cap*m, m
+    int32_t *cidx = nullptr;     // optional: cap*m candidate indices and m counts of the final rows
+    int32_t *ccnt = nullptr;     //   (targets must be the sorted sources; see neighbour_rows_from_candidates)
     int32_t *steps, *state;      // scratch: m, m
     unsigned long long *counters;  // scratch: spread array of 5 counters (5 * SPH_SPREAD_WORDS words)
     unsigned long long *totals;    // scratch: 5 words
@@ -39,15 +41,16 @@ struct HSolveParams {
 hipError_t hsolve(hipStream_t st, const HSolveParams &p, int max_rounds, int64_t *n_unconverged);
 
 // CSR of S_i = { j : r_ij^2 < (h_i+h_j)^2 } over sorted particles, one
-// wavefront per tree-node group.  Counts, scans (off[0..n]), reads back
+// wavefront per tree-node group.  hscale (device, may be null): the node h_max
+// were built from an earlier h and are scaled by *hscale in the prune.  Counts, scans (off[0..n]), reads back
 // *total; fills idx if idx_cap >= *total.  overflow: 1 scratch counter.
 hipError_t neighbours(hipStream_t st, int64_t n, int64_t ngroups, const int32_t *groups, const double *x,
                       const double *y, const double *z, const double *h, const WNode *nodes, int bucket, int32_t *cnt,
                       int64_t *off, int64_t *scan_tmp, unsigned long long *overflow, int32_t *idx, int64_t idx_cap,
-                      int64_t *total);
+                      int64_t *total, const double *hscale = nullptr);
 hipError_t neighbours_fill(hipStream_t st, int64_t ngroups, const int32_t *groups, const double *x, const double *y,
                            const double *z, const double *h, const WNode *nodes, int bucket, const int64_t *off,
-                           int32_t *idx, unsigned long long *overflow);
+                           int32_t *idx, unsigned long long *overflow, const double *hscale = nullptr);
 // The same sets in one walk as padded rows: idx[i*stride + c], c < cnt[i] (sorted
 // particles).  *n_long = rows with more than stride entries (those rows are
 // truncated: rebuild as CSR), *total = entries.  spread: 2 * SPH_SPREAD_WORDS
@@ -56,6 +59,22 @@ hipError_t neighbours_fill(hipStream_t st, int64_t ngroups, const int32_t *group
 hipError_t neighbour_rows(hipStream_t st, int64_t ngroups, const int32_t *groups, const double *x, const double *y,
                           const double *z, const double *h, const WNode *nodes, int bucket, int stride, int32_t *cnt,
                           int32_t *idx, unsigned long long *overflow, unsigned long long *spread,
-                          unsigned long long *totals, int64_t *n_long, int64_t *total);
+                          unsigned long long *totals, int64_t *n_long, int64_t *total,
+                          const double *hscale = nullptr);
+// Padded neighbour rows (as neighbour_rows) built from the h solve's final candidate rows
+// and indices (targets = the sorted particles, same positions and the solved h) instead of
+// a tree walk; exact by the containment argument at k_cand_rows.  ext: ext_cap int2 scratch
+// for the pairs found from the other side (SPH_SPREAD regions), next: SPH_SPREAD_WORDS
+// scratch words (one counter per region).  *n_long > 0: some row
+// is too long or an extra was lost -- rebuild with a walk.  Synchronises the stream.
+hipError_t neighbour_rows_from_candidates(hipStream_t st, int64_t n, int cap, const double *cand, const int32_t *cidx,
+                                         const int32_t *ccnt, const double *h, const double *R, int stride,
+                                         int32_t *cnt, int32_t *idx, int2 *ext, int64_t ext_cap,
+                                         unsigned long long *next, unsigned long long *spread,
+                                         unsigned long long *totals, int64_t *n_long, int64_t *total,
+                                         int64_t *n_extra);
+// *out (device) = max_i h_new[i] / h_old[i], rounded up by 8 ulp: the factor that
+// keeps node h_max of h_old a valid bound for the new h.
+hipError_t h_ratio_max(hipStream_t st, int64_t n, const double *h_new, const double *h_old, double *out);
 
 }  // namespace sph

@@ -86,6 +86,8 @@ struct HSolveArgs {
     int kth;                   // mode 2
     int cap;
     double *cand;              // m rows of cap doubles
+    int32_t *cidx;             // m rows of cap candidate indices (may be null)
+    int32_t *ccnt;             // candidates per target (written with cidx)
     double *R;                 // per target search radius (in/out)
     double *h;                 // per target h (in: start, out: result)
     int32_t *steps;            // mode 1 step counter (persists over rounds)
@@ -115,6 +117,8 @@ __global__ void __launch_bounds__(64) k_hsolve(HSolveArgs a)
     __shared__ WalkLDS L;
     __shared__ double sx[WSRC], sy[WSRC], sz[WSRC];
     __shared__ double buf[64][9];  // per-lane staging of candidate r^2 (padded row)
+    __shared__ int bufj[64][9];    // and of their indices
+    __shared__ int sj[WSRC];
     int64_t first;
     int ng;
     group_range(a.groups, a.gnodes, a.m, first, ng);
@@ -134,6 +138,7 @@ __global__ void __launch_bounds__(64) k_hsolve(HSolveArgs a)
     const double R2 = R * R, Rg2 = Rg * Rg;
     const int cap = a.cap;
     double *row = a.cand + (valid ? t : 0) * (int64_t)cap;
+    int32_t *rowj = a.cidx ? a.cidx + (valid ? t : 0) * (int64_t)cap : nullptr;
     int c = 0;
     long long ntest = 0, nterm = 0;  // algorithmic work (distance tests, kernel terms)
     const bool ovf = group_walk(
@@ -142,6 +147,7 @@ __global__ void __launch_bounds__(64) k_hsolve(HSolveArgs a)
             sx[q] = a.x[j];
             sy[q] = a.y[j];
             sz[q] = a.z[j];
+            sj[q] = j;
         },
         [&](int cnt) {
             if (valid) ntest += cnt;
@@ -152,9 +158,13 @@ __global__ void __launch_bounds__(64) k_hsolve(HSolveArgs a)
                     if (r2 < R2) {
                         const int b = c & 7;
                         buf[lane][b] = r2;
+                        bufj[lane][b] = sj[k];
                         if (b == 7 && c < cap) {
 #pragma unroll
                             for (int u = 0; u < 8; u++) row[c - 7 + u] = buf[lane][u];
+                            if (rowj)
+#pragma unroll
+                                for (int u = 0; u < 8; u++) rowj[c - 7 + u] = bufj[lane][u];
                         }
                         c++;
                     }
@@ -162,8 +172,13 @@ __global__ void __launch_bounds__(64) k_hsolve(HSolveArgs a)
         });
     if (ovf && lane == 0) spread_add(a.pending, 2, 1ull);
     const bool over = valid && c > cap;
-    if (valid && !over)
+    if (valid && !over) {
         for (int k = c & ~7; k < c; k++) row[k] = buf[lane][k & 7];
+        if (rowj) {
+            for (int k = c & ~7; k < c; k++) rowj[k] = bufj[lane][k & 7];
+            a.ccnt[t] = c;
+        }
+    }
     __syncthreads();  // rows written by each lane are read by the whole wave below
     // Solve HB targets at a time, HL lanes per target: lane l of the sub-group
     // holds candidates l, l+HL, ... (up to HS, in registers) and the sub-group
@@ -404,7 +419,8 @@ __global__ void __launch_bounds__(64) k_ngb(const int32_t *__restrict__ groups, 
                                             int32_t *__restrict__ cnt, const int64_t *__restrict__ off,
                                             int32_t *__restrict__ idx, int stride,
                                             unsigned long long *__restrict__ overflow,
-                                            unsigned long long *__restrict__ spread)
+                                            unsigned long long *__restrict__ spread,
+                                            const double *__restrict__ hscale)
 {
     __shared__ WalkLDS L;
     __shared__ double sx[WSRC], sy[WSRC], sz[WSRC], sh[WSRC];
@@ -418,13 +434,15 @@ __global__ void __launch_bounds__(64) k_ngb(const int32_t *__restrict__ groups, 
     double xi = 0, yi = 0, zi = 0, hi = 0;
     if (valid) { xi = x[i]; yi = y[i]; zi = z[i]; hi = h[i]; }
     const double gb[6] = {g->lx, g->ly, g->lz, g->hx, g->hy, g->hz};
-    const double hg = g->hmax;
+    // node h_max from an earlier h: *hscale >= max_i h_i / h_old_i keeps the prune conservative
+    const double hs = hscale ? *hscale : 1.0;
+    const double hg = g->hmax * hs;
     const int64_t o = !valid ? 0 : MODE == 1 ? off[i] : MODE == 2 ? i * (int64_t)stride : 0;
     int c = 0;
     const bool ovf = group_walk(
         nodes, bucket, L,
         [&](const WNode *nd) {
-            double t = hg + nd->hmax;
+            double t = hg + nd->hmax * hs;
             return box_box_dist2(gb, nd) >= t * t;
         },
         [&](int q, int j) {
@@ -460,7 +478,158 @@ __global__ void __launch_bounds__(64) k_ngb(const int32_t *__restrict__ groups, 
     }
 }
 
+// ---------------------------------------------------------------- rows from h-solve candidates
+// After the h solve every target i holds its final candidate row
+// C_i = { j : r_ij^2 < R_i^2 } (FP64 r^2 with the association of the neighbour
+// test; a - b = -(b - a) exactly) and 2 h_i <= R_i exactly (the Newton bracket
+// stops at R_i/2).  Hence for S_i = { j : r_ij^2 < (h_i+h_j)^2 }:
+//   h_j <= h_i:  (h_i+h_j)^2 <= (2 h_i)^2 <= R_i^2 by monotone rounding, so j in C_i;
+//   h_j >  h_i:  likewise i in C_j.
+// k_cand_rows keeps { j in C_i : r^2 < (h_i+h_j)^2 } in candidate order and
+// records every pair with i in C_j, h_j > h_i, j not in C_i (r^2 >= R_i^2) as an
+// extra (target i, member j) from j's row; k_ext_append appends the extras.
+// Spread counters: 0 += rows longer than stride, 1 += entries.
+template <int G>
+__global__ void __launch_bounds__(256) k_cand_rows(int64_t n, int cap, const double *__restrict__ cand,
+                                                   const int32_t *__restrict__ cidx, const int32_t *__restrict__ ccnt,
+                                                   const double *__restrict__ h, const double *__restrict__ R,
+                                                   int stride, int32_t *__restrict__ cnt, int32_t *__restrict__ idx,
+                                                   int2 *__restrict__ ext, int64_t ext_cap,
+                                                   unsigned long long *__restrict__ next,
+                                                   unsigned long long *__restrict__ spread)
+{
+    const int64_t gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int64_t i = gid / G;
+    const int l = (int)(gid % G), lane = threadIdx.x & 63;
+    const bool valid = i < n;
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const uint64_t gmask = (G == 64 ? ~0ull : ((1ull << G) - 1)) << (lane & ~(G - 1));
+    const double hi = valid ? h[i] : 0.0;
+    const int c = valid ? ccnt[i] : 0;
+    const double *rr = cand + (valid ? i : 0) * (int64_t)cap;
+    const int32_t *rj = cidx + (valid ? i : 0) * (int64_t)cap;
+    int32_t *row = idx + (valid ? i : 0) * (int64_t)stride;
+    int w = 0;
+    for (int k0 = 0; k0 < c; k0 += G) {
+        const int k = k0 + l;
+        bool in = false, extra = false;
+        int j = 0;
+        if (k < c) {
+            j = rj[k];
+            const double r2 = rr[k], hj = h[j], t = hi + hj, t2 = t * t;
+            in = r2 < t2;
+            if (in && hi > hj) {
+                const double Rj = R[j];
+                extra = r2 >= Rj * Rj;
+            }
+        }
+        const uint64_t mb = __ballot(in) & gmask;
+        if (in) {
+            const int pos = w + (int)__popcll(mb & below);
+            if (pos < stride) row[pos] = j;
+        }
+        w += (int)__popcll(mb);
+        if (extra) {  // region of this workgroup (SPH_SPREAD regions of ext_cap / SPH_SPREAD pairs)
+            const int64_t rcap = ext_cap / SPH_SPREAD, reg = blockIdx.x & (SPH_SPREAD - 1);
+            const unsigned long long e = atomicAdd(next + reg * SPH_SPREAD_PAD, 1ull);
+            if (e < (unsigned long long)rcap) ext[reg * rcap + (int64_t)e] = make_int2(j, (int)i);
+        }
+    }
+    if (valid && l == 0) cnt[i] = w;
+    unsigned long long nlong = (valid && l == 0 && w > stride) ? 1ull : 0ull;
+    unsigned long long nent = (valid && l == 0) ? (unsigned long long)w : 0ull;
+    for (int off = 32; off >= 1; off >>= 1) {
+        nlong += __shfl_xor(nlong, off);
+        nent += __shfl_xor(nent, off);
+    }
+    if (lane == 0) {
+        if (nlong) spread_add(spread, 0, nlong);
+        spread_add(spread, 1, nent);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_ext_append(int64_t ext_cap, const int2 *__restrict__ ext,
+                                                    const unsigned long long *__restrict__ next,
+                                                    int32_t *__restrict__ cnt, int32_t *__restrict__ idx, int stride,
+                                                    unsigned long long *__restrict__ spread)
+{
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t rcap = ext_cap / SPH_SPREAD, reg = e / rcap;
+    bool longrow = false, added = false;
+    if (reg < SPH_SPREAD && (unsigned long long)(e - reg * rcap) < next[reg * SPH_SPREAD_PAD]) {
+        const int2 p = ext[e];
+        const int pos = atomicAdd(&cnt[p.x], 1);
+        if (pos < stride) idx[(int64_t)p.x * stride + pos] = p.y;
+        longrow = pos == stride;  // the row just became too long (counted once)
+        added = true;
+    }
+    const uint64_t ml = __ballot(longrow), ma = __ballot(added);
+    if ((threadIdx.x & 63) == 0) {
+        if (ml) spread_add(spread, 0, (unsigned long long)__popcll(ml));
+        if (ma) spread_add(spread, 1, (unsigned long long)__popcll(ma));
+    }
+}
+
+// *out = max_i h_new[i] / h_old[i] (1 + 8 eps): positive doubles order like
+// their bit patterns, so one unsigned 64-bit atomicMax per workgroup.
+__global__ void __launch_bounds__(256) k_hratio(int64_t n, const double *__restrict__ hn,
+                                                const double *__restrict__ ho, unsigned long long *__restrict__ out)
+{
+    __shared__ double wmax[4];
+    double v = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        v = fmax(v, hn[i] / ho[i]);
+    for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        v = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3])) * (1.0 + 8.0 * 2.220446049250313e-16);
+        atomicMax(out, (unsigned long long)__double_as_longlong(v));
+    }
+}
+
 namespace sph {
+
+hipError_t neighbour_rows_from_candidates(hipStream_t st, int64_t n, int cap, const double *cand, const int32_t *cidx,
+                                         const int32_t *ccnt, const double *h, const double *R, int stride,
+                                         int32_t *cnt, int32_t *idx, int2 *ext, int64_t ext_cap,
+                                         unsigned long long *next, unsigned long long *spread,
+                                         unsigned long long *totals, int64_t *n_long, int64_t *total,
+                                         int64_t *n_extra)
+{
+    if (stride < 1 || cap < 1 || ext_cap < SPH_SPREAD) return hipErrorInvalidValue;
+    fill<unsigned long long>(st, SPH_SPREAD_WORDS, next, 0ull);
+    fill<unsigned long long>(st, 2 * SPH_SPREAD_WORDS, spread, 0ull);
+    k_cand_rows<16><<<sph_blocks(n * 16, 256), 256, 0, st>>>(n, cap, cand, cidx, ccnt, h, R, stride, cnt, idx, ext,
+                                                            ext_cap, next, spread);
+    k_ext_append<<<sph_blocks(ext_cap, 256), 256, 0, st>>>(ext_cap, ext, next, cnt, idx, stride, spread);
+    spread_total(st, 2, spread, totals);
+    unsigned long long hc[2] = {0, 0}, nx[SPH_SPREAD];
+    hipError_t e = hipMemcpyAsync(hc, totals, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    e = hipMemcpy2DAsync(nx, sizeof(unsigned long long), next, SPH_SPREAD_PAD * sizeof(unsigned long long),
+                         sizeof(unsigned long long), SPH_SPREAD, hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    int64_t nextra = 0, lost = 0;
+    for (int r = 0; r < SPH_SPREAD; r++) {
+        nextra += (int64_t)nx[r];
+        if (nx[r] > (unsigned long long)(ext_cap / SPH_SPREAD)) lost = 1;  // a lost extra voids the rows
+    }
+    *n_long = (int64_t)hc[0] + lost;
+    *total = (int64_t)hc[1];
+    *n_extra = nextra;
+    return hipGetLastError();
+}
+
+hipError_t h_ratio_max(hipStream_t st, int64_t n, const double *h_new, const double *h_old, double *out)
+{
+    fill<unsigned long long>(st, 1, (unsigned long long *)out, 0ull);
+    const int64_t nb = sph_blocks(n, 256) < 1024 ? sph_blocks(n, 256) : 1024;
+    k_hratio<<<(unsigned)nb, 256, 0, st>>>(n, h_new, h_old, (unsigned long long *)out);
+    return hipGetLastError();
+}
 
 hipError_t hsolve(hipStream_t st, const HSolveParams &p, int max_rounds, int64_t *n_unconverged)
 {
@@ -470,6 +639,7 @@ hipError_t hsolve(hipStream_t st, const HSolveParams &p, int max_rounds, int64_t
     a.mode = p.mode; a.n_ngb = p.n_ngb; a.tol = p.tol;
     a.n_low = p.n_low; a.n_up = p.n_up; a.max_steps = p.max_steps; a.kth = p.kth;
     a.cap = p.cap; a.cand = p.cand; a.R = p.R; a.h = p.h_out; a.steps = p.steps; a.state = p.state;
+    a.cidx = p.cidx; a.ccnt = p.ccnt;
     a.pending = p.counters;
     a.groups = p.groups;
     a.gnodes = p.nodes;
@@ -514,12 +684,12 @@ hipError_t hsolve(hipStream_t st, const HSolveParams &p, int max_rounds, int64_t
 hipError_t neighbours(hipStream_t st, int64_t n, int64_t ngroups, const int32_t *groups, const double *x,
                       const double *y, const double *z, const double *h, const WNode *nodes, int bucket, int32_t *cnt,
                       int64_t *off, int64_t *scan_tmp, unsigned long long *overflow, int32_t *idx, int64_t idx_cap,
-                      int64_t *total)
+                      int64_t *total, const double *hscale)
 {
     if (bucket > 64) return hipErrorInvalidValue;
     fill<unsigned long long>(st, 1, overflow, 0ull);
     k_ngb<0><<<(unsigned)ngroups, 64, 0, st>>>(groups, x, y, z, h, nodes, bucket, cnt, nullptr, nullptr, 0, overflow,
-                                               nullptr);
+                                               nullptr, hscale);
     hipError_t e = scan_counts(st, n, cnt, off, scan_tmp);
     if (e != hipSuccess) return e;
     unsigned long long hov = 0;
@@ -532,29 +702,29 @@ hipError_t neighbours(hipStream_t st, int64_t n, int64_t ngroups, const int32_t 
     if (hov) return hipErrorIllegalState;
     if (*total > idx_cap || idx == nullptr) return hipSuccess;  // caller grows idx and calls neighbours_fill
     k_ngb<1><<<(unsigned)ngroups, 64, 0, st>>>(groups, x, y, z, h, nodes, bucket, nullptr, off, idx, 0, overflow,
-                                               nullptr);
+                                               nullptr, hscale);
     return hipGetLastError();
 }
 
 hipError_t neighbours_fill(hipStream_t st, int64_t ngroups, const int32_t *groups, const double *x, const double *y,
                            const double *z, const double *h, const WNode *nodes, int bucket, const int64_t *off,
-                           int32_t *idx, unsigned long long *overflow)
+                           int32_t *idx, unsigned long long *overflow, const double *hscale)
 {
     k_ngb<1><<<(unsigned)ngroups, 64, 0, st>>>(groups, x, y, z, h, nodes, bucket, nullptr, off, idx, 0, overflow,
-                                               nullptr);
+                                               nullptr, hscale);
     return hipGetLastError();
 }
 
 hipError_t neighbour_rows(hipStream_t st, int64_t ngroups, const int32_t *groups, const double *x, const double *y,
                           const double *z, const double *h, const WNode *nodes, int bucket, int stride, int32_t *cnt,
                           int32_t *idx, unsigned long long *overflow, unsigned long long *spread,
-                          unsigned long long *totals, int64_t *n_long, int64_t *total)
+                          unsigned long long *totals, int64_t *n_long, int64_t *total, const double *hscale)
 {
     if (bucket > 64 || stride < 1) return hipErrorInvalidValue;
     fill<unsigned long long>(st, 1, overflow, 0ull);
     fill<unsigned long long>(st, 2 * SPH_SPREAD_WORDS, spread, 0ull);
     k_ngb<2><<<(unsigned)ngroups, 64, 0, st>>>(groups, x, y, z, h, nodes, bucket, cnt, nullptr, idx, stride, overflow,
-                                               spread);
+                                               spread, hscale);
     spread_total(st, 2, spread, totals);
     unsigned long long hc[3] = {0, 0, 0};
     hipError_t e = hipMemcpyAsync(hc, totals, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
