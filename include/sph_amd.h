@@ -236,6 +236,10 @@ int sph_state_get(sph_ctx *ctx, double *pos, double *vel, double *u, double *h, 
 int sph_prime(sph_ctx *ctx, const sph_step_params *p);
 /* One KDK step (sph_serial_v2.py:148-203), entirely on the device. */
 int sph_step(sph_ctx *ctx, const sph_step_params *p);
+/* Neighbour sets S_i of the resident state at its positions and h (last sph_prime or
+ * sph_step), in the caller's numbering and the layout of sph_neighbours (rows ascending);
+ * the step builds them from its smoothing-length candidates, this reads them back. */
+int sph_state_neighbours(sph_ctx *ctx, int64_t *offsets, int32_t *idx, int64_t idx_cap);
 int sph_get_stats(sph_ctx *ctx, sph_stats *out);
 int sph_reset_stats(sph_ctx *ctx);
 

@@ -84,6 +84,7 @@ _SIGS = {
     "sph_state_get": ([_vp, _d, _d, _d, _d, _d, _d], ctypes.c_int),
     "sph_prime": ([_vp, ctypes.POINTER(StepParams)], ctypes.c_int),
     "sph_step": ([_vp, ctypes.POINTER(StepParams)], ctypes.c_int),
+    "sph_state_neighbours": ([_vp, _i64p, _i32p, _i64], ctypes.c_int),
     "sph_get_stats": ([_vp, ctypes.POINTER(Stats)], ctypes.c_int),
     "sph_reset_stats": ([_vp], ctypes.c_int),
 }

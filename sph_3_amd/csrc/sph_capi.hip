@@ -612,6 +612,47 @@ extern "C" int sph_neighbours(sph_ctx *ctx, int64_t n, const double *pos, const 
     return SPH_OK;
 }
 
+// Neighbour sets of the resident state (the last sph_prime / sph_step), in the caller's
+// numbering: the same output as sph_neighbours at the state's positions and h.
+extern "C" int sph_state_neighbours(sph_ctx *ctx, int64_t *offsets, int32_t *idx, int64_t idx_cap)
+{
+    if (!ctx || !offsets) return SPH_ERR_ARG;
+    if (!ctx->resident || !ctx->primed) return fail(ctx, SPH_ERR_STATE, "no forces evaluated yet (sph_prime)");
+    HIPCK(hipSetDevice(ctx->device));
+    const int64_t n = ctx->n, K = ctx->ngb_stride;
+    std::vector<int32_t> orig(n), scnt(n);
+    std::vector<int64_t> soff(n + 1);
+    HIPCK(hipMemcpyAsync(orig.data(), ctx->orig.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->st));
+    if (K > 0) {
+        HIPCK(hipMemcpyAsync(scnt.data(), ctx->cnt.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->st));
+        HIPCK(hipStreamSynchronize(ctx->st));
+        for (int64_t s = 0; s < n; s++) soff[s] = s * K;
+    } else {
+        HIPCK(hipMemcpyAsync(soff.data(), ctx->off.p, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, ctx->st));
+        HIPCK(hipStreamSynchronize(ctx->st));
+        for (int64_t s = 0; s < n; s++) scnt[s] = (int32_t)(soff[s + 1] - soff[s]);
+    }
+    std::vector<int64_t> cnt(n);
+    for (int64_t s = 0; s < n; s++) cnt[orig[s]] = scnt[s];
+    offsets[0] = 0;
+    for (int64_t i = 0; i < n; i++) offsets[i + 1] = offsets[i] + cnt[i];
+    const int64_t total = offsets[n];
+    if (!idx || idx_cap < total)
+        return idx ? fail(ctx, SPH_ERR_CAPACITY, "idx capacity " + std::to_string(idx_cap) + " < " +
+                                                     std::to_string(total))
+                   : SPH_OK;
+    const int64_t span = K > 0 ? n * K : total;
+    std::vector<int32_t> sidx(span);
+    HIPCK(hipMemcpyAsync(sidx.data(), ctx->idx.p, sizeof(int32_t) * span, hipMemcpyDeviceToHost, ctx->st));
+    HIPCK(hipStreamSynchronize(ctx->st));
+    for (int64_t s = 0; s < n; s++) {
+        int64_t r = orig[s], o = offsets[r];
+        for (int64_t k = soff[s]; k < soff[s] + scnt[s]; k++) idx[o++] = orig[sidx[k]];
+        std::sort(idx + offsets[r], idx + offsets[r + 1]);
+    }
+    return SPH_OK;
+}
+
 // =========================================================== hydro
 extern "C" int sph_density(sph_ctx *ctx, int64_t n, const double *pos, const double *m, const double *h,
                            int precision, double *rho_out)

@@ -79,6 +79,16 @@ class DeviceSPH:
             out["rho"] = rho
         return out
 
+    def neighbours(self):
+        """(offsets[N+1], idx) of the neighbour sets the last force evaluation used (caller order)."""
+        n = self.n
+        off = np.empty(n + 1, np.int64)
+        c = self.ctx
+        c.check(c.lib.sph_state_neighbours(c.h, ptr(off), None, 0), "sph_state_neighbours")
+        idx = np.empty(int(off[-1]), np.int32)
+        c.check(c.lib.sph_state_neighbours(c.h, ptr(off), ptr(idx), len(idx)), "sph_state_neighbours")
+        return off, idx
+
     def stats(self) -> dict:
         s = Stats()
         c = self.ctx

@@ -93,3 +93,24 @@ def test_step_conserves_momentum_and_reports_phases(gpu):
     st = sim.stats()
     assert st["steps"] == 3 and all(st["phase_calls"][k] == 3 for k in ["density", "accsph", "dudt"])
     assert st["phase_ms"]["ngb"] > 0
+
+
+@pytest.mark.parametrize("steps", [0, 3])
+def test_step_neighbour_sets_bit_exact(gpu, steps):
+    """The step builds its neighbour rows from the smoothing-length candidates
+    (plus the pairs only the other particle's candidates hold); the sets must
+    equal S_i of the oracle tree at the state's positions and h exactly."""
+    n = 40000
+    d = ic.evrard(n, seed=24)
+    vel = np.random.default_rng(5).normal(size=(n, 3)) * 0.1
+    sim = DeviceSPH(d["pos"], vel, d["m"], d["u"], d["h0"], StepConfig())
+    sim.prime()
+    for _ in range(steps):
+        sim.step()
+    g = sim.get()
+    off, idx = sim.neighbours()
+    off0, idx0 = O.Tree(g["pos"], d["m"], h=g["h"]).neighbours()
+    st = sim.stats()
+    print("steps", steps, "entries", len(idx), "pairs from the other side", st["ngb_extra"])
+    assert np.array_equal(off, off0) and np.array_equal(idx, idx0)
+    assert st["ngb_total"] == len(idx0)
