@@ -137,6 +137,7 @@ def mlp_section(n_inf):
 
     tab = C.CoolingTable()
     net = C.CoolingSurrogate(tab, seed=0)
+    mlp_section.net = net  # reused by the cooling step
     n_samples, epochs = 2_000_000, 200
     t0 = time.perf_counter()
     losses = net.train(n_samples=n_samples, epochs=epochs)
@@ -170,6 +171,78 @@ def mlp_section(n_inf):
     }
 
 
+def cooling_section(args, cfg, run):
+    """Evrard collapse plus cooling at N = cooling_n (4,194,304 by default): per-particle
+    Z log-uniform in [1e-3, 1] Z_sun and J21 in [1e-2, 1e2], code units M = 1e10 M_sun,
+    R = 10 kpc, G = 1; after each step's u update the explicit cooling update with
+    log Lambda from the gated surrogate (table fallback at 0.05 dex) and log Gamma and
+    T(u) from tables.  The all-FP64 form takes the table for log Lambda (there is no
+    FP64 network).  CPU: the same all-pairs restatement on a quarter of the 1M row
+    sample (rows cost 4x at 4x N) plus the oracle's exact cooling update."""
+    from sph_3_amd import cooling as C
+    from sph_3_amd import ic
+
+    n = args.cooling_n
+    d = ic.evrard(n, seed=44)
+    Z, J = ic.cooling_fields(n, seed=45)
+    net = getattr(mlp_section, "net", None)
+    mode = "mlp" if net is not None else "table"
+    units = C.CoolingParams.from_mass_length(1e10 * C.M_SUN_G, 10 * C.KPC_CM, mode=mode, tol=0.05)
+
+    def cooling(mode_):
+        def f(sim):
+            u = C.CoolingParams(**{**units.__dict__, "mode": mode_})
+            sim.set_cooling(Z, J, u, surrogate=net if mode_ == "mlp" else None)
+        return f
+
+    sim, dt = run("mixed", args.steps, args.warmup, d=d, cooling=cooling(mode))
+    st = sim.stats()
+    g = sim.get()
+    ms = 1e3 * dt / args.steps
+    per_step = {k: st["phase_ms"][k] / args.steps for k in st["phase_ms"] if st["phase_calls"][k]}
+    res = {
+        "workload": f"evrard_collapse + cooling, n = {n}", "n": n, "cooling_mode": mode, "tol_dex": units.tol,
+        "units": {"mass_g": 1e10 * C.M_SUN_G, "length_cm": 10 * C.KPC_CM, "density_cgs": units.unit_density_cgs,
+                  "u_cgs": units.unit_u_cgs, "time_s": units.unit_time_s},
+        "ms_per_step": ms, "value": n * args.steps / dt, "unit": "particle-steps/s",
+        "phase_ms_per_step": per_step,
+        "cooling_fallback_fraction_last_step": st["cool_fallback"] / n,
+        "cooling_floored_fraction_last_step": st["cool_floor"] / n,
+    }
+    del sim
+    if not args.no_fp64_form:
+        steps64 = max(2, args.steps // 2)
+        sim64, dt64 = run("fp64", steps64, 1, d=d, cooling=cooling("table"))
+        res["fp64_form"] = {"ms_per_step": 1e3 * dt64 / steps64, "mixed_speedup": (1e3 * dt64 / steps64) / ms,
+                            "cooling_mode": "table"}
+        del sim64
+    if not args.no_cpu_baseline:
+        import oracle as O
+
+        rows = max(64, args.cpu_rows // 4)
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        log(f"[bench] cooling workload cpu baseline: {rows} rows x {n} sources on {threads} threads")
+        step_s, sample_s, parts = cpu_baseline(g, n, d["m"], rows)
+        sel = np.random.default_rng(9).choice(n, rows, replace=False)
+        t0 = time.perf_counter()
+        box = C.TABLE_RANGES
+        O.cool_update(g["rho"][sel], g["u"][sel], Z[sel], J[sel], cfg.dt,
+                      (units.unit_density_cgs, units.unit_u_cgs, units.unit_time_s, units.XH, units.u_floor_cgs),
+                      [b[0] for b in box], [b[1] for b in box])
+        t_cool = time.perf_counter() - t0
+        parts["cooling_update"] = t_cool
+        step_s += t_cool * n / rows
+        res["cpu_baseline"] = {
+            "value": n / step_s, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{rows} random target rows x all {n} sources of the FP64 all-pairs restatement of the "
+                      f"step plus the exact FP64 cooling update of {rows} particles; "
+                      f"{sample_s + t_cool:.1f} s measured, extrapolated x{n / rows:.0f}",
+            "step_seconds_extrapolated": step_s, "function_seconds_sample": parts,
+        }
+        res["speedup_vs_cpu"] = res["value"] / res["cpu_baseline"]["value"]
+    return res
+
+
 def copy_probe(nbytes):
     """A plain device copy of known size (the calibration run of the HBM traffic counters)."""
     import torch
@@ -193,6 +266,8 @@ def main():
     ap.add_argument("--no-fp64-form", action="store_true")
     ap.add_argument("--no-mlp", action="store_true")
     ap.add_argument("--mlp-n", type=int, default=4194304)
+    ap.add_argument("--no-cooling-step", action="store_true")
+    ap.add_argument("--cooling-n", type=int, default=4194304)
     ap.add_argument("--copy-probe", type=int, default=0, help="only run a device copy of this many bytes")
     args = ap.parse_args()
     if args.copy_probe:
@@ -211,9 +286,11 @@ def main():
     cfg = StepConfig(dt=1e-3, gamma=5 / 3, alpha=1.0, G=1.0, theta=0.5, softening="plummer", eps_plummer=1e-3,
                      n_ngb=64.0, h_tol=1e-10, precision="mixed", balsara=True)
 
-    def run(precision, steps, warmup):
+    def run(precision, steps, warmup, d=d, cooling=None):
         cfg.precision = precision
         sim = DeviceSPH(d["pos"], d["vel"], d["m"], d["u"], d["h0"], cfg)
+        if cooling is not None:
+            cooling(sim)
         log(f"[bench] {precision}: prime")
         sim.prime()
         for w in range(warmup):
@@ -241,6 +318,7 @@ def main():
 
     sim, dt = run("mixed", args.steps, args.warmup)
     st = sim.stats()
+    g_mixed = sim.get()  # the context is shared: read the state before the next run replaces it
     ms_step = 1e3 * dt / args.steps
     value = n * args.steps / dt
     phases = {k: v / max(1, st["phase_calls"][k]) for k, v in st["phase_ms"].items() if st["phase_calls"][k]}
@@ -274,7 +352,7 @@ def main():
     # HBM fractions of the CSR passes (reported beside the dominant kernel)
     hbm = {k: {"ms": phases[k], "GB/s": bytes_pp[k] * n / (phases[k] * 1e-3) / 1e9,
                "frac": bytes_pp[k] * n / (phases[k] * 1e-3) / 1e9 / HBM_PEAK_GBS}
-           for k in bytes_pp if k in phases}
+           for k in bytes_pp if k in phases and not (k == "moments" and cfg.softening == "plummer")}
     grav_tflops = None
     if "gravity" in phases:
         grav_tflops = FLOP_PER_INTERACTION * (st["grav_pp"] + st["grav_pn"]) / (phases["gravity"] * 1e-3) / 1e12
@@ -300,7 +378,7 @@ def main():
         "pass_bandwidth": hbm,
         "roofline": roof,
     }
-    tr = traffic_table().get(roof["kernel"])
+    tr = traffic_table().get(roof["kernel"]) if n == 1048576 else None  # measured at this N only
     if tr:
         roof["traffic"] = tr["fetch_bytes"] + tr["write_bytes"]
         roof["traffic_source"] = "profiles/hbm_traffic_evrard_1M.md"
@@ -313,7 +391,7 @@ def main():
                             "mixed_speedup": (1e3 * dt64 / max(2, args.steps // 2)) / ms_step}
         del sim64
     if not args.no_cpu_baseline:
-        g = sim.get()
+        g = g_mixed
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         log(f"[bench] cpu baseline: {args.cpu_rows} rows x {n} sources on {threads} threads")
         step_s, sample_s, parts = cpu_baseline(g, n, d["m"], args.cpu_rows)
@@ -326,6 +404,8 @@ def main():
             "function_seconds_sample": parts,
         }
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
+    if not args.no_cooling_step:
+        out["evrard_cooling"] = cooling_section(args, cfg, run)
     print(json.dumps(out), flush=True)
 
 

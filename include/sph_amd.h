@@ -190,6 +190,40 @@ int sph_mlp_cooling(sph_ctx *ctx, int64_t n, const double *lognH, const double *
  * network ms, out[3] fallback fraction at tol. */
 int sph_mlp_timing(sph_ctx *ctx, int64_t n, int reps, uint64_t seed, double tol, double *out);
 
+/* ---- per-step cooling update (CR/v4t.h:1768-1884 doCloudyHCooling, called after the u
+ *      update at CR/v3t.cu:1527-1536) ---------------------------------------------
+ * In code units: rho_cgs = rho * unit_density_cgs, n_H = XH rho_cgs / m_H, u_cgs = u *
+ * unit_u_cgs.  (n_H, T, Z, J21) are clamped into the cooling table's box before the rates
+ * are evaluated (doCloudyHCooling clamps to its table bounds); T(u) is the ionisation-
+ * equilibrium temperature of photolibs3.py:379-420.  The update is explicit:
+ * u_cgs' = max(u_cgs + (Gamma - Lambda) n_H^2 / rho_cgs * dt * unit_time_s, u_floor_cgs),
+ * Gamma, Lambda per n_H^2 at the clamped point and n_H unclamped.  Modes:
+ * SPH_COOL_EXACT: FP64 T(u) and rates; SPH_COOL_TABLE: T from a temperature table over
+ * (log n_H, log u, log J21) and log Lambda, log Gamma from the 4D table;
+ * SPH_COOL_MLP: as TABLE with log Lambda from the gated surrogate (sph_mlp_cooling).
+ * The table (and for SPH_COOL_MLP the calibrated surrogate) must be loaded in the context. */
+enum sph_cool_mode { SPH_COOL_EXACT = 0, SPH_COOL_TABLE = 1, SPH_COOL_MLP = 2 };
+typedef struct sph_cool_params {
+    double unit_density_cgs; /* UnitDensity_in_cgs */
+    double unit_u_cgs;       /* Unit_u_in_cgs */
+    double unit_time_s;      /* unitTime_in_s */
+    double XH;               /* hydrogen mass fraction (0.7 in doCloudyHCooling) */
+    double u_floor_cgs;      /* 1.8e11 erg/g in doCloudyHCooling */
+    double tol;              /* SPH_COOL_MLP: gate tolerance [dex] */
+    int mode;                /* sph_cool_mode */
+    int ref;                 /* SPH_COOL_MLP: sph_mlp_ref of the gate */
+} sph_cool_params;
+/* u_out[n] after one step dt (code units) for particles (rho, u, Z, J21) in caller order;
+ * fallback_out[n] (may be NULL, SPH_COOL_MLP) flags table results; *n_floor, *n_fallback
+ * (may be NULL) count floored particles and table fallbacks.  Replaces doCloudyHCooling. */
+int sph_cooling_update(sph_ctx *ctx, int64_t n, const double *rho, const double *u, const double *Z,
+                       const double *J21, double dt, const sph_cool_params *cp, double *u_out,
+                       int32_t *fallback_out, int64_t *n_floor, int64_t *n_fallback);
+/* T(u_cgs, n_H, J21) by the temperature table of SPH_COOL_TABLE / SPH_COOL_MLP (built for XH
+ * on first use); the table's counterpart of convert_u_to_temp_h (photolibs3.py:379). */
+int sph_cooling_temperature(sph_ctx *ctx, int64_t n, const double *u, const double *nH, const double *J21,
+                            double XH, double *T_out);
+
 /* ---- device-resident time loop (sph_serial_v2.py:148-218 on the GPU) ---- */
 typedef struct sph_step_params {
     double dt;          /* fixed step */
@@ -217,6 +251,8 @@ typedef struct sph_stats {
     int64_t h_tests;         /* last h solve: FP64 distance tests of its walks */
     int64_t h_terms;         /* last h solve: kernel terms summed by its Newton iterations */
     int64_t ngb_extra;       /* last step: neighbour pairs found only from the other particle's candidates */
+    int64_t cool_floor;      /* last step: particles the cooling update set to u_floor */
+    int64_t cool_fallback;   /* last step: table fallbacks of the surrogate (SPH_COOL_MLP) */
 } sph_stats;
 
 /* phase indices of sph_stats.phase_ms (SPH_PH_HMOMENTS: the node boxes built for the h walk;
@@ -224,7 +260,7 @@ typedef struct sph_stats {
 enum sph_phase {
     SPH_PH_KICKDRIFT = 0, SPH_PH_ORDER, SPH_PH_TOPOLOGY, SPH_PH_HSOLVE, SPH_PH_MOMENTS, SPH_PH_NGB,
     SPH_PH_DENSITY, SPH_PH_GRAVITY, SPH_PH_DIVCURL, SPH_PH_ACCSPH, SPH_PH_KICK, SPH_PH_DUDT,
-    SPH_PH_UUPDATE, SPH_PH_HMOMENTS, SPH_PH_COUNT
+    SPH_PH_UUPDATE, SPH_PH_HMOMENTS, SPH_PH_COOLING, SPH_PH_COUNT
 };
 
 /* Load the resident state (caller order); u and h are per particle. */
@@ -232,6 +268,10 @@ int sph_state_set(sph_ctx *ctx, int64_t n, const double *pos, const double *vel,
                   const double *u, const double *h);
 /* Read it back in caller order; any pointer may be NULL. */
 int sph_state_get(sph_ctx *ctx, double *pos, double *vel, double *u, double *h, double *rho, double *acc);
+/* Per-particle metallicity Z [Z_sun] and UV amplitude J21 (caller order) of the resident
+ * state and the cooling of every later sph_step, applied after its u update
+ * (CR/v3t.cu:1527-1536); cp = NULL switches cooling off.  Call after sph_state_set. */
+int sph_state_set_cooling(sph_ctx *ctx, const double *Z, const double *J21, const sph_cool_params *cp);
 /* Forces at the current positions and the first u update (sph_serial_v2.py:94-143). */
 int sph_prime(sph_ctx *ctx, const sph_step_params *p);
 /* One KDK step (sph_serial_v2.py:148-203), entirely on the device. */

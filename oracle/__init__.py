@@ -373,3 +373,16 @@ def cool_rate_from_u(u, nH, XH=0.76, J21=J21_REFERENCE, Z=0.0):
 
 def do_cooling(rho, u_old, dt, XH=0.76, J21=J21_REFERENCE, Z=0.0, u_floor=1e8):
     return lib().or_do_cooling(float(rho), float(u_old), float(dt), float(XH), float(J21), float(Z), float(u_floor))
+
+
+def cool_update(rho, u, Z, J21, dt, units, lo, hi):
+    """Explicit per-step cooling update (or_cool_update_n): units = (rho_cgs, u_cgs,
+    t_s, XH, u_floor_cgs), lo/hi = log10 box of (nH, T, Z, J21).  Returns (u', floored)."""
+    rho = _f64(rho); n = len(rho)
+    u = np.broadcast_to(_f64(u), (n,)).copy()
+    Z = np.broadcast_to(_f64(Z), (n,)).copy()
+    J21 = np.broadcast_to(_f64(J21), (n,)).copy()
+    out = np.zeros(n); fl = np.zeros(n, np.int32)
+    lib().or_cool_update_n(_i64(n), _p(rho), _p(u), _p(Z), _p(J21), ctypes.c_double(dt), _p(_f64(units)),
+                           _p(_f64(lo)), _p(_f64(hi)), _p(out), _p(fl))
+    return out, fl.astype(bool)

@@ -223,3 +223,36 @@ double or_do_cooling(double rho, double u_old, double dt, double XH, double J21,
     }
     return u;
 }
+
+static double clampd(double v, double lo, double hi) { return fmin(fmax(v, lo), hi); }
+
+/* Explicit per-step cooling update (CR/v4t.h:1768-1884 doCloudyHCooling, called after
+ * the u update at CR/v3t.cu:1527-1536) with the exact rates of this file in place of the
+ * CLOUDY table: rho_cgs = rho * units[0], nH = XH rho_cgs / m_H (units[3] = XH),
+ * u_cgs = u * units[1]; nH, T(u), Z, J21 are clamped into the box 10^lo[a] .. 10^hi[a]
+ * (a = nH, T, Z, J21) as doCloudyHCooling clamps to its table bounds; the net rate per
+ * volume (Gamma - Lambda) nH^2 uses the unclamped nH; u_cgs' = max(u_cgs + rate / rho_cgs
+ * * dt * units[2], units[4]) (1.8e11 erg/g in doCloudyHCooling), returned in code units.
+ * floored[i] (may be NULL) = 1 where the floor was applied. */
+void or_cool_update_n(int64_t n, const double *rho, const double *u, const double *Z, const double *J21, double dt,
+                      const double *units, const double *lo, const double *hi, double *out, int32_t *floored)
+{
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < n; i++) {
+        const double XH = units[3];
+        const double rho_cgs = rho[i] * units[0];
+        const double nH = XH * rho_cgs / CL_MH;
+        const double u_cgs = u[i] * units[1];
+        const double nHc = clampd(nH, pow(10.0, lo[0]), pow(10.0, hi[0]));
+        const double zc = clampd(Z[i], pow(10.0, lo[2]), pow(10.0, hi[2]));
+        const double jc = clampd(J21[i], pow(10.0, lo[3]), pow(10.0, hi[3]));
+        const double Tc = clampd(or_cool_u_to_T(u_cgs, nHc, XH, jc), pow(10.0, lo[1]), pow(10.0, hi[1]));
+        double hl[2];
+        or_cool_heat_lambda(Tc, nHc, jc, zc, hl);
+        double un = u_cgs + (hl[0] - hl[1]) * (nH * nH / rho_cgs) * (dt * units[2]);
+        const int f = !(un >= units[4]);
+        if (f) un = units[4];
+        out[i] = un / units[1];
+        if (floored) floored[i] = f;
+    }
+}

@@ -21,7 +21,7 @@ SOFT = {"spline": 0, "plummer": 1}
 H_NEWTON, H_REF_COUNT, H_KTH = 0, 1, 2
 
 PHASES = ["kick_drift", "order", "topology", "hsolve", "moments", "ngb", "density", "gravity", "divcurl",
-          "accsph", "kick", "dudt", "u_update", "moments_h"]
+          "accsph", "kick", "dudt", "u_update", "moments_h", "cooling"]
 
 _d = ctypes.POINTER(ctypes.c_double)
 _f = ctypes.POINTER(ctypes.c_float)
@@ -43,7 +43,17 @@ class Stats(ctypes.Structure):
                 ("grav_pp", ctypes.c_int64), ("grav_pn", ctypes.c_int64), ("h_unconverged", ctypes.c_int64),
                 ("h_rounds", ctypes.c_int64), ("phase_ms", ctypes.c_double * 16),
                 ("phase_calls", ctypes.c_int64 * 16), ("h_tests", ctypes.c_int64), ("h_terms", ctypes.c_int64),
-                ("ngb_extra", ctypes.c_int64)]
+                ("ngb_extra", ctypes.c_int64), ("cool_floor", ctypes.c_int64), ("cool_fallback", ctypes.c_int64)]
+
+
+class CoolParams(ctypes.Structure):
+    _fields_ = [("unit_density_cgs", ctypes.c_double), ("unit_u_cgs", ctypes.c_double),
+                ("unit_time_s", ctypes.c_double), ("XH", ctypes.c_double), ("u_floor_cgs", ctypes.c_double),
+                ("tol", ctypes.c_double), ("mode", ctypes.c_int), ("ref", ctypes.c_int)]
+
+
+COOL_MODE = {"exact": 0, "table": 1, "mlp": 2}
+MLP_REF = {"exact": 0, "table": 1}
 
 
 _SIGS = {
@@ -85,6 +95,10 @@ _SIGS = {
     "sph_prime": ([_vp, ctypes.POINTER(StepParams)], ctypes.c_int),
     "sph_step": ([_vp, ctypes.POINTER(StepParams)], ctypes.c_int),
     "sph_state_neighbours": ([_vp, _i64p, _i32p, _i64], ctypes.c_int),
+    "sph_cooling_update": ([_vp, _i64, _d, _d, _d, _d, ctypes.c_double, ctypes.POINTER(CoolParams), _d, _i32p,
+                            _i64p, _i64p], ctypes.c_int),
+    "sph_cooling_temperature": ([_vp, _i64, _d, _d, _d, ctypes.c_double, _d], ctypes.c_int),
+    "sph_state_set_cooling": ([_vp, _d, _d, ctypes.POINTER(CoolParams)], ctypes.c_int),
     "sph_get_stats": ([_vp, ctypes.POINTER(Stats)], ctypes.c_int),
     "sph_reset_stats": ([_vp], ctypes.c_int),
 }

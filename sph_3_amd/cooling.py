@@ -11,13 +11,14 @@ the Rosen & Bregman 1995 metal-line term; 0 = the reference's primordial gas).
 from __future__ import annotations
 
 import ctypes
+from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import MLP_NPARAM, Context, context, f64, ptr
+from ._lib import COOL_MODE, MLP_NPARAM, MLP_REF, Context, CoolParams, context, f64, ptr
 
 __all__ = ["coolingHeatingRates", "convert_u_to_temp_h", "coolingRateFromU_h", "DoCooling_h", "CoolingTable",
-           "CoolingSurrogate", "J21_REFERENCE", "TABLE_RANGES"]
+           "CoolingSurrogate", "CoolingParams", "J21_REFERENCE", "TABLE_RANGES"]
 
 J21_REFERENCE = 1e-7
 # (lo, hi, points) per table axis: log10 nH, log10 T, log10 Z, log10 J21
@@ -89,6 +90,14 @@ def DoCooling_h(rho, u_old, dt, XH, *, J21=J21_REFERENCE, Z=0.0, u_floor=1e8, de
     return float(out[0]) if scalar else out
 
 
+def install_table(ctx: Context, ranges=TABLE_RANGES):
+    """Build the cooling table in a given context (the one of a resident state)."""
+    lo = np.array([r[0] for r in ranges], np.float64)
+    hi = np.array([r[1] for r in ranges], np.float64)
+    npts = (ctypes.c_int * 4)(*[int(r[2]) for r in ranges])
+    ctx.check(ctx.lib.sph_cooling_table_build(ctx.h, ptr(lo), ptr(hi), npts), "cooling table build")
+
+
 class CoolingTable:
     """4D table of log10 Lambda and log10 Gamma (per nH^2) over (log10 nH,
     log10 T, log10 Z, log10 J21), built on the device from the exact function;
@@ -96,13 +105,9 @@ class CoolingTable:
 
     def __init__(self, ranges=TABLE_RANGES, device=0):
         self.ranges = tuple(tuple(r) for r in ranges)
-        lo = np.array([r[0] for r in ranges], np.float64)
-        hi = np.array([r[1] for r in ranges], np.float64)
-        npts = (ctypes.c_int * 4)(*[int(r[2]) for r in ranges])
         self.device = device
         self.ctx = Context(device)  # one context (stream + buffers) per table
-        c = self.ctx
-        c.check(c.lib.sph_cooling_table_build(c.h, ptr(lo), ptr(hi), npts), "cooling table build")
+        install_table(self.ctx, self.ranges)
         self.nbytes = 2 * 4 * int(np.prod([r[2] for r in ranges]))
 
     def lookup(self, lognH, logT, logZ, logJ21):
@@ -113,6 +118,70 @@ class CoolingTable:
         c.check(c.lib.sph_cooling_table_lookup(c.h, n, ptr(a), ptr(b), ptr(z), ptr(u), ptr(L), ptr(G)),
                 "cooling table lookup")
         return L, G
+
+    def temperature(self, u, nHcgs, J21, XH=0.7):
+        """T(u) [K] by the temperature table of the "table" and "mlp" update modes
+        (log10 T over log10 nH, log10 u, log10 J21; the table's counterpart of
+        convert_u_to_temp_h)."""
+        _, (u, nH, J) = _vec(u, nHcgs, J21)
+        T = np.empty(len(u))
+        c = self.ctx
+        c.check(c.lib.sph_cooling_temperature(c.h, len(u), ptr(u), ptr(nH), ptr(J), float(XH), ptr(T)),
+                "cooling temperature")
+        return T
+
+    def cooling_update(self, rho, u, Z, J21, dt, params: CoolingParams):
+        """One step of the explicit cooling update (doCloudyHCooling, CR/v4t.h:1768-1884)
+        in code units; mode "mlp" uses the surrogate trained and calibrated on this
+        table.  Returns (u_new, info) with info = {"floored", "fallback" (counts),
+        "fallback_mask"}."""
+        _, (rho, u, Z, J) = _vec(rho, u, Z, J21)
+        n = len(rho)
+        out = np.empty(n)
+        fb = np.zeros(n, np.int32)
+        nfl = ctypes.c_int64(); nfb = ctypes.c_int64()
+        cp = params.c()
+        c = self.ctx
+        c.check(c.lib.sph_cooling_update(c.h, n, ptr(rho), ptr(u), ptr(Z), ptr(J), float(dt), ctypes.byref(cp),
+                                         ptr(out), ptr(fb), ctypes.byref(nfl), ctypes.byref(nfb)), "cooling update")
+        return out, dict(floored=int(nfl.value), fallback=int(nfb.value), fallback_mask=fb.astype(bool))
+
+
+G_CGS = 6.674e-8
+M_SUN_G = 1.989e33
+KPC_CM = 3.0857e21
+
+
+@dataclass
+class CoolingParams:
+    """Units and switches of the per-step cooling update (the explicit update of
+    CR/v4t.h:1768-1884 doCloudyHCooling; defaults XH = 0.7 and u floor 1.8e11 erg/g
+    are that kernel's).  mode: "exact" (FP64 T(u) and rates), "table" (temperature
+    table + 4D rate table) or "mlp" (as "table" with log Lambda from the gated
+    surrogate); tol / ref: the surrogate's gate (CoolingSurrogate.rates)."""
+    unit_density_cgs: float
+    unit_u_cgs: float
+    unit_time_s: float
+    XH: float = 0.7
+    u_floor_cgs: float = 1.8e11
+    mode: str = "mlp"
+    tol: float = 0.05
+    ref: str = "exact"
+
+    @classmethod
+    def from_mass_length(cls, mass_g: float, length_cm: float, **kw) -> "CoolingParams":
+        """Code units with G = 1 for a unit mass and length in cgs: density M/L^3,
+        specific energy G M / L, time sqrt(L^3 / (G M))."""
+        v2 = G_CGS * mass_g / length_cm
+        return cls(mass_g / length_cm ** 3, v2, length_cm / np.sqrt(v2), **kw)
+
+    def c(self) -> CoolParams:
+        if self.mode not in COOL_MODE:
+            raise ValueError(f"mode must be one of {sorted(COOL_MODE)}")
+        if self.ref not in MLP_REF:
+            raise ValueError(f"ref must be one of {sorted(MLP_REF)}")
+        return CoolParams(self.unit_density_cgs, self.unit_u_cgs, self.unit_time_s, self.XH, self.u_floor_cgs,
+                          self.tol, COOL_MODE[self.mode], MLP_REF[self.ref])
 
 
 def _f32(a, shape=None):
@@ -213,7 +282,21 @@ class CoolingSurrogate:
         c.check(c.lib.sph_mlp_calibrate(c.h, cl, int(per), int(seed), ptr(et), ptr(ex), ptr(te)),
                 "surrogate calibrate")
         self.cells = tuple(int(v) for v in cells)
+        self.calibration = (self.cells, int(per), int(seed))
         return et.reshape(cells), ex.reshape(cells), te.reshape(cells)
+
+    def install(self, ctx: Context):
+        """Load this table, network and error map into another context (the one of a
+        resident state, for its per-step cooling): the table is rebuilt and the map
+        re-measured with the same cells, samples and seed, so the gate is identical."""
+        if self.cells is None:
+            raise RuntimeError("calibrate the surrogate first")
+        params, norm = self.get_params()
+        install_table(ctx, self.table.ranges)
+        ctx.check(ctx.lib.sph_mlp_set_params(ctx.h, ptr(params), ptr(norm)), "surrogate set_params")
+        cells, per, seed = self.calibration
+        cl = (ctypes.c_int * 4)(*cells)
+        ctx.check(ctx.lib.sph_mlp_calibrate(ctx.h, cl, per, seed, None, None, None), "surrogate calibrate")
 
     # ---- gated inference
     def rates(self, lognH, logT, logZ, logJ21, tol=0.02, ref="exact"):

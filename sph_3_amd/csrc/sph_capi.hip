@@ -29,7 +29,8 @@
 #define SPR_H 0            // spread counters of the h solve (5)
 #define SPR_G 5            // spread counters of the gravity walk (3)
 #define SPR_N 8            // spread counters of the padded neighbour rows (2)
-#define SPR_TOTAL 10
+#define SPR_C 10           // spread counter of the cooling update (1)
+#define SPR_TOTAL 11
 #define SPH_NGB_STRIDE 128 // entries per padded neighbour row
 
 namespace {
@@ -78,7 +79,7 @@ struct sph_ctx {
     // tree
     Buf nodes, right, parent, flags, front, groups;
     // neighbour lists
-    Buf cnt, off, scantmp, idx;
+    Buf cnt, rcnt0, off, scantmp, idx;  // rcnt0: row lengths before the appended pairs
     // hydro scalars (8 bytes per particle, used as float or double)
     Buf rho, P, c, prho2, divV, curlV, bal, du;
     // h solve scratch
@@ -88,13 +89,20 @@ struct sph_ctx {
     Buf stage, kpart, kout;
     // spread event counters (sph_common.h): SPR_H.. h solve (5), SPR_G.. gravity (3);
     // their totals land in hcount: [0..4] h solve, [5..7] gravity, [8] walk overflow, [9] group count,
-    // [10..11] padded neighbour rows
+    // [10..11] padded neighbour rows, [12] cooling floor, [13] surrogate fallbacks
     Buf spread;
     // cooling: field integrals, table, elementwise scratch
     Buf rf1, ctabL, ctabG, cin[6], cout[2];
     bool rf1_ok = false;
     CoolAxes cax{};
     bool ctab_ok = false;
+    // cooling update: temperature table (built for ttab_XH), per-particle Z and J21 of the
+    // resident state (sorted), its parameters, and the surrogate path's columns
+    Buf ttab, zmet, j21, ucols, uyn, uL, uG, uflag, ucnt;
+    CoolTAxes tax{};
+    double ttab_XH = -1.0;
+    bool cool_on = false, cool_ran = false;
+    sph_cool_params cool_p{};
     // cooling surrogate: parameters, Adam state, BF16 weight copies, samples, error map
     Buf mP, mG, mM, mV, mW[4], mx, mxn, mt, my, mloss, memap, memap2, memap3, mgate, mcnt;
     MLPNorm mnorm{};
@@ -164,7 +172,7 @@ int capacity(sph_ctx *ctx, int64_t n)
                   &ctx->tmpd, &ctx->rho, &ctx->P, &ctx->c, &ctx->prho2, &ctx->divV, &ctx->curlV,
                   &ctx->bal, &ctx->du, &ctx->hR, &ctx->hout, &ctx->hin, &ctx->tx, &ctx->ty, &ctx->tz};
     for (Buf *b : dbl) CK(ensure(ctx, *b, nd));
-    Buf *i32[] = {&ctx->orig, &ctx->tmpi, &ctx->vals0, &ctx->vals1, &ctx->cnt, &ctx->hsteps, &ctx->hstate};
+    Buf *i32[] = {&ctx->orig, &ctx->tmpi, &ctx->vals0, &ctx->vals1, &ctx->cnt, &ctx->rcnt0, &ctx->hsteps, &ctx->hstate};
     for (Buf *b : i32) CK(ensure(ctx, *b, ni));
     CK(ensure(ctx, ctx->keys0, nd));
     CK(ensure(ctx, ctx->keys1, nd));
@@ -444,11 +452,12 @@ extern "C" int sph_ctx_destroy(sph_ctx *ctx)
                   &ctx->u, &ctx->duprev, &ctx->ax, &ctx->ay, &ctx->az, &ctx->gx, &ctx->gy, &ctx->gz,
                   &ctx->tmpd, &ctx->orig, &ctx->tmpi, &ctx->keys0, &ctx->keys1, &ctx->vals0, &ctx->vals1,
                   &ctx->hist, &ctx->bpart, &ctx->box, &ctx->nodes, &ctx->right, &ctx->parent, &ctx->flags, &ctx->front, &ctx->groups,
-                  &ctx->cnt, &ctx->off, &ctx->scantmp, &ctx->idx, &ctx->rho, &ctx->P, &ctx->c, &ctx->prho2,
+                  &ctx->cnt, &ctx->rcnt0, &ctx->off, &ctx->scantmp, &ctx->idx, &ctx->rho, &ctx->P, &ctx->c, &ctx->prho2,
                   &ctx->divV, &ctx->curlV, &ctx->bal, &ctx->du, &ctx->cand, &ctx->hR, &ctx->hsteps,
                   &ctx->hstate, &ctx->hcount, &ctx->hout, &ctx->hin, &ctx->tx, &ctx->ty, &ctx->tz,
                   &ctx->stage, &ctx->kpart, &ctx->kout, &ctx->spread, &ctx->ccidx, &ctx->cccnt,
-                  &ctx->cext, &ctx->cnext};
+                  &ctx->cext, &ctx->cnext, &ctx->ttab, &ctx->zmet, &ctx->j21, &ctx->ucols, &ctx->uyn,
+                  &ctx->uL, &ctx->uG, &ctx->uflag, &ctx->ucnt};
     for (Buf *b : all)
         if (b->p) (void)hipFree(b->p);
     Buf *cool[] = {&ctx->rf1,   &ctx->ctabL,  &ctx->ctabG,  &ctx->cin[0], &ctx->cin[1], &ctx->cin[2], &ctx->cin[3],
@@ -833,6 +842,7 @@ extern "C" int sph_state_set(sph_ctx *ctx, int64_t n, const double *pos, const d
     CK(check_common(ctx, n, SPH_PREC_FP64));
     if (!pos || !vel || !m || !u || !h) return fail(ctx, SPH_ERR_ARG, "null argument");
     ctx->resident = false;
+    ctx->cool_on = false;  // Z and J21 belong to the previous state
     invalidate_all(ctx);
     CK(set_pos(ctx, n, pos));
     CK(put_vec3(ctx, vel, ctx->vx, ctx->vy, ctx->vz));
@@ -885,10 +895,22 @@ int reorder_state(sph_ctx *ctx)
         HIPCK(sph::gather<double, double>(st, n, perm, (const double *)b->p, 1, 0, D(tmpd)));
         std::swap(*b, ctx->tmpd);
     }
+    if (ctx->cool_on) {
+        for (Buf *b : {&ctx->zmet, &ctx->j21}) {
+            HIPCK(sph::gather<double, double>(st, n, perm, (const double *)b->p, 1, 0, D(tmpd)));
+            std::swap(*b, ctx->tmpd);
+        }
+    }
     HIPCK(sph::compose(st, n, I32(orig), perm, I32(tmpi)));
     std::swap(ctx->orig, ctx->tmpi);
     return SPH_OK;
 }
+
+// Cooling update of n particles (sph_cooling_update's semantics); counts[0] = floored
+// particles, counts[1] = surrogate fallbacks (device words).
+template <typename RHO>
+int cool_run(sph_ctx *ctx, int64_t n, const RHO *rho, const double *u, const double *Z, const double *J21,
+             double dt, const sph_cool_params &cp, double *u_out, unsigned long long *counts);
 
 template <typename R>
 int forces(sph_ctx *ctx, const sph_step_params *p, int first)
@@ -939,7 +961,7 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
         CK(ensure(ctx, ctx->cnext, (size_t)SPH_SPREAD_WORDS * 8));
         int64_t nlong = 0, total = 0, nextra = 0;
         HIPCK(sph::neighbour_rows_from_candidates(
-            st, n, SPH_HCAP, D(cand), I32(ccidx), I32(cccnt), D(h), D(hR), SPH_NGB_STRIDE, I32(cnt), I32(idx),
+            st, n, SPH_HCAP, D(cand), I32(ccidx), I32(cccnt), D(h), D(hR), SPH_NGB_STRIDE, I32(cnt), I32(rcnt0), I32(idx),
             P_<int2>(ctx->cext), ext_cap, P_<unsigned long long>(ctx->cnext),
             P_<unsigned long long>(ctx->spread) + SPR_N * SPH_SPREAD_WORDS, P_<unsigned long long>(ctx->hcount) + 10,
             &nlong, &total, &nextra));
@@ -998,6 +1020,14 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     ph_begin(ctx, SPH_PH_UUPDATE);
     HIPCK(sph::u_update<R>(st, n, D(u), P_<R>(ctx->du), D(duprev), p->dt, first));
     ph_end(ctx, SPH_PH_UUPDATE);
+    // cooling after the u update of each step (CR/v3t.cu:1527-1536; none after the first update)
+    ctx->cool_ran = ctx->cool_on && !first;
+    if (ctx->cool_ran) {
+        ph_begin(ctx, SPH_PH_COOLING);
+        CK(cool_run<R>(ctx, n, P_<R>(ctx->rho), D(u), D(zmet), D(j21), p->dt, ctx->cool_p, D(u),
+                       P_<unsigned long long>(ctx->hcount) + 12));
+        ph_end(ctx, SPH_PH_COOLING);
+    }
     return SPH_OK;
 }
 
@@ -1021,6 +1051,13 @@ int finish_step(sph_ctx *ctx, const sph_step_params *p)
     if (hc[2]) return fail(ctx, SPH_ERR_CAPACITY, "gravity walk stack overflow");
     ctx->stats.grav_pp = (int64_t)hc[0];
     ctx->stats.grav_pn = (int64_t)hc[1];
+    if (ctx->cool_ran) {
+        unsigned long long cc[2];
+        HIPCK(hipMemcpy(cc, P_<unsigned long long>(ctx->hcount) + 12, sizeof(cc), hipMemcpyDeviceToHost));
+        ctx->stats.cool_floor = (int64_t)cc[0];
+        ctx->stats.cool_fallback = (int64_t)cc[1];
+        ctx->cool_ran = false;
+    }
     if (ctx->timing) {
         // only the phases this step recorded (an unrecorded event pair would fail and
         // leave a sticky error behind)
@@ -1193,6 +1230,7 @@ extern "C" int sph_cooling_table_build(sph_ctx *ctx, const double *lo, const dou
     HIPCK(hipStreamSynchronize(ctx->st));
     ctx->cax = ax;
     ctx->ctab_ok = true;
+    ctx->ttab_XH = -1.0;  // the temperature table spans this table's box
     return SPH_OK;
 }
 
@@ -1629,5 +1667,179 @@ extern "C" int sph_mlp_timing(sph_ctx *ctx, int64_t n, int reps, uint64_t seed, 
     out[1] = ms[1] / reps;
     out[2] = ms[2] / reps;
     out[3] = (double)nf / (double)n;
+    return SPH_OK;
+}
+
+// =========================================================== cooling update
+namespace {
+
+int cool_check(sph_ctx *ctx, const sph_cool_params *cp)
+{
+    if (!cp) return fail(ctx, SPH_ERR_ARG, "null cooling parameters");
+    if (cp->mode != SPH_COOL_EXACT && cp->mode != SPH_COOL_TABLE && cp->mode != SPH_COOL_MLP)
+        return fail(ctx, SPH_ERR_ARG, "mode must be SPH_COOL_EXACT, SPH_COOL_TABLE or SPH_COOL_MLP");
+    if (!(cp->unit_density_cgs > 0.0) || !(cp->unit_u_cgs > 0.0) || !(cp->unit_time_s > 0.0) ||
+        !(cp->XH > 0.0 && cp->XH <= 1.0) || !(cp->u_floor_cgs >= 0.0))
+        return fail(ctx, SPH_ERR_ARG, "cooling units must be > 0, 0 < XH <= 1 and u_floor_cgs >= 0");
+    if (!ctx->ctab_ok) return fail(ctx, SPH_ERR_STATE, "no cooling table (call sph_cooling_table_build)");
+    if (cp->mode == SPH_COOL_MLP) {
+        CK(mlp_need(ctx, true, true));
+        if (cp->ref != SPH_MLP_REF_EXACT && cp->ref != SPH_MLP_REF_TABLE)
+            return fail(ctx, SPH_ERR_ARG, "ref must be SPH_MLP_REF_EXACT or SPH_MLP_REF_TABLE");
+        if (!(cp->tol > 0.0)) return fail(ctx, SPH_ERR_ARG, "tol must be > 0");
+    }
+    return SPH_OK;
+}
+
+double tab_hi(const CoolAxes &ax, int a) { return ax.lo[a] + (ax.n[a] - 1) * ax.d[a]; }
+
+// Temperature table over the 4D table's n_H and J21 ranges (0.1 and 0.125 dex) and the
+// log u range that maps its T range for mean molecular weights 1.3 (neutral) to 0.55
+// (fully ionised) at gamma = 5/3, 0.02 dex.
+int ensure_ttable(sph_ctx *ctx, double XH)
+{
+    if (ctx->ttab_XH == XH) return SPH_OK;
+    const CoolAxes &ax = ctx->cax;
+    const double kb = 1.3807e-16, mh = 1.6726e-24;
+    double lo[3] = {ax.lo[0], std::log10(kb * std::pow(10.0, ax.lo[1]) / (2.0 / 3.0 * 1.3 * mh)) - 0.1, ax.lo[3]};
+    double hi[3] = {tab_hi(ax, 0), std::log10(kb * std::pow(10.0, tab_hi(ax, 1)) / (2.0 / 3.0 * 0.55 * mh)) + 0.1,
+                    tab_hi(ax, 3)};
+    const double step[3] = {0.1, 0.02, 0.125};
+    CoolTAxes t{};
+    int64_t total = 1;
+    for (int a = 0; a < 3; a++) {
+        t.lo[a] = lo[a];
+        t.n[a] = std::max(2, (int)std::ceil((hi[a] - lo[a]) / step[a] - 1e-9) + 1);
+        t.d[a] = (hi[a] - lo[a]) / (t.n[a] - 1);
+        total *= t.n[a];
+    }
+    if (total > ((int64_t)1 << 26)) return fail(ctx, SPH_ERR_ARG, "temperature table too large");
+    CK(ensure(ctx, ctx->ttab, (size_t)total * 4));
+    HIPCK(sph::cool_ttable_build(ctx->st, t, XH, D(rf1), P_<float>(ctx->ttab)));
+    HIPCK(hipStreamSynchronize(ctx->st));
+    ctx->tax = t;
+    ctx->ttab_XH = XH;
+    return SPH_OK;
+}
+
+CoolUpdate make_update(const sph_ctx *ctx, const sph_cool_params &cp, double dt)
+{
+    CoolUpdate cu{};
+    cu.rho_cgs = cp.unit_density_cgs;
+    cu.u_cgs = cp.unit_u_cgs;
+    cu.dt_s = dt * cp.unit_time_s;
+    cu.XH = cp.XH;
+    cu.u_floor = cp.u_floor_cgs;
+    for (int a = 0; a < 4; a++) {
+        cu.lo[a] = ctx->cax.lo[a];
+        cu.hi[a] = tab_hi(ctx->cax, a);
+        cu.lin_lo[a] = std::pow(10.0, cu.lo[a]);
+        cu.lin_hi[a] = std::pow(10.0, cu.hi[a]);
+    }
+    return cu;
+}
+
+template <typename RHO>
+int cool_run(sph_ctx *ctx, int64_t n, const RHO *rho, const double *u, const double *Z, const double *J21,
+             double dt, const sph_cool_params &cp, double *u_out, unsigned long long *counts)
+{
+    hipStream_t st = ctx->st;
+    const CoolUpdate cu = make_update(ctx, cp, dt);
+    unsigned long long *spr = P_<unsigned long long>(ctx->spread) + SPR_C * SPH_SPREAD_WORDS;
+    HIPCK(sph::fill<unsigned long long>(st, SPH_SPREAD_WORDS, spr, 0ull));
+    HIPCK(sph::fill<unsigned long long>(st, 1, counts + 1, 0ull));
+    if (cp.mode != SPH_COOL_EXACT) CK(ensure_ttable(ctx, cp.XH));
+    const float *tabT = P_<float>(ctx->ttab);
+    if (cp.mode != SPH_COOL_MLP) {
+        const int kind = cp.mode == SPH_COOL_EXACT ? SPH_COOLK_EXACT : SPH_COOLK_TABLE;
+        HIPCK(sph::cool_update<RHO>(st, n, kind, rho, u, Z, J21, cu, ctx->cax, P_<float>(ctx->ctabL),
+                                    P_<float>(ctx->ctabG), ctx->tax, tabT, D(rf1), u_out, nullptr, nullptr, spr));
+    } else {
+        CK(ensure(ctx, ctx->ucols, (size_t)n * 16));
+        CK(ensure(ctx, ctx->uyn, (size_t)n * 4));
+        CK(ensure(ctx, ctx->uL, (size_t)n * 4));
+        CK(ensure(ctx, ctx->uG, (size_t)n * 4));
+        CK(ensure(ctx, ctx->uflag, (size_t)n * 4));
+        float *cols = P_<float>(ctx->ucols);
+        HIPCK(sph::cool_update<RHO>(st, n, SPH_COOLK_PREP, rho, u, Z, J21, cu, ctx->cax, P_<float>(ctx->ctabL),
+                                    P_<float>(ctx->ctabG), ctx->tax, tabT, D(rf1), nullptr, cols,
+                                    P_<float>(ctx->uG), nullptr));
+        const float *emap = cp.ref == SPH_MLP_REF_EXACT ? P_<float>(ctx->mgate) : P_<float>(ctx->memap);
+        HIPCK(sph::mlp_infer(st, n, cols, ctx->mnorm, mlp_dev(ctx), emap, ctx->mcells, (float)cp.tol, ctx->cax,
+                             P_<float>(ctx->ctabL), P_<float>(ctx->uyn), P_<float>(ctx->uL), P_<int>(ctx->uflag),
+                             counts + 1));
+        HIPCK(sph::cool_apply<RHO>(st, n, rho, u, P_<float>(ctx->uL), P_<float>(ctx->uG), cu, u_out, spr));
+    }
+    HIPCK(sph::spread_total(st, 1, spr, counts));
+    return SPH_OK;
+}
+
+}  // namespace
+
+extern "C" int sph_cooling_update(sph_ctx *ctx, int64_t n, const double *rho, const double *u, const double *Z,
+                                  const double *J21, double dt, const sph_cool_params *cp, double *u_out,
+                                  int32_t *fallback_out, int64_t *n_floor, int64_t *n_fallback)
+{
+    CK(cool_prep(ctx, n));
+    if (!rho || !u || !Z || !J21 || !u_out) return fail(ctx, SPH_ERR_ARG, "null argument");
+    if (!(dt >= 0.0)) return fail(ctx, SPH_ERR_ARG, "dt must be >= 0");
+    CK(cool_check(ctx, cp));
+    CK(ensure(ctx, ctx->spread, (size_t)SPR_TOTAL * SPH_SPREAD_WORDS * 8));
+    CK(ensure(ctx, ctx->ucnt, 2 * 8));
+    const double *src[4] = {rho, u, Z, J21};
+    for (int k = 0; k < 4; k++)
+        HIPCK(hipMemcpyAsync(ctx->cin[k].p, src[k], sizeof(double) * n, hipMemcpyHostToDevice, ctx->st));
+    CK(cool_run<double>(ctx, n, D(cin[0]), D(cin[1]), D(cin[2]), D(cin[3]), dt, *cp, D(cout[0]),
+                        P_<unsigned long long>(ctx->ucnt)));
+    HIPCK(hipMemcpyAsync(u_out, ctx->cout[0].p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->st));
+    if (fallback_out && cp->mode == SPH_COOL_MLP)
+        HIPCK(hipMemcpyAsync(fallback_out, ctx->uflag.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->st));
+    unsigned long long cc[2] = {0, 0};
+    HIPCK(hipMemcpyAsync(cc, ctx->ucnt.p, sizeof(cc), hipMemcpyDeviceToHost, ctx->st));
+    HIPCK(hipStreamSynchronize(ctx->st));
+    if (fallback_out && cp->mode != SPH_COOL_MLP) std::memset(fallback_out, 0, sizeof(int32_t) * n);
+    if (n_floor) *n_floor = (int64_t)cc[0];
+    if (n_fallback) *n_fallback = (int64_t)cc[1];
+    return SPH_OK;
+}
+
+extern "C" int sph_cooling_temperature(sph_ctx *ctx, int64_t n, const double *u, const double *nH, const double *J21,
+                                       double XH, double *T_out)
+{
+    CK(cool_prep(ctx, n));
+    if (!u || !nH || !J21 || !T_out) return fail(ctx, SPH_ERR_ARG, "null argument");
+    if (!(XH > 0.0 && XH <= 1.0)) return fail(ctx, SPH_ERR_ARG, "need 0 < XH <= 1");
+    if (!ctx->ctab_ok) return fail(ctx, SPH_ERR_STATE, "no cooling table (call sph_cooling_table_build)");
+    CK(ensure_ttable(ctx, XH));
+    const double *src[3] = {u, nH, J21};
+    for (int k = 0; k < 3; k++)
+        HIPCK(hipMemcpyAsync(ctx->cin[k].p, src[k], sizeof(double) * n, hipMemcpyHostToDevice, ctx->st));
+    HIPCK(sph::cool_ttable_lookup(ctx->st, n, ctx->tax, P_<float>(ctx->ttab), D(cin[0]), D(cin[1]), D(cin[2]), 0.0,
+                                  D(cout[0])));
+    HIPCK(hipMemcpyAsync(T_out, ctx->cout[0].p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->st));
+    HIPCK(hipStreamSynchronize(ctx->st));
+    return SPH_OK;
+}
+
+extern "C" int sph_state_set_cooling(sph_ctx *ctx, const double *Z, const double *J21, const sph_cool_params *cp)
+{
+    if (!ctx) return SPH_ERR_ARG;
+    if (!ctx->resident) return fail(ctx, SPH_ERR_STATE, "no resident state (call sph_state_set first)");
+    HIPCK(hipSetDevice(ctx->device));
+    if (!cp) {
+        ctx->cool_on = false;
+        return SPH_OK;
+    }
+    if (!Z || !J21) return fail(ctx, SPH_ERR_ARG, "null argument");
+    CK(cool_prep(ctx, 1));
+    CK(cool_check(ctx, cp));
+    CK(ensure(ctx, ctx->zmet, (size_t)ctx->n * 8));
+    CK(ensure(ctx, ctx->j21, (size_t)ctx->n * 8));
+    CK(put_scalar(ctx, Z, ctx->zmet, nullptr, nullptr, nullptr));
+    CK(put_scalar(ctx, J21, ctx->j21, nullptr, nullptr, nullptr));
+    if (cp->mode != SPH_COOL_EXACT) CK(ensure_ttable(ctx, cp->XH));
+    HIPCK(hipStreamSynchronize(ctx->st));
+    ctx->cool_p = *cp;
+    ctx->cool_on = true;
     return SPH_OK;
 }

@@ -15,6 +15,12 @@
 // doCloudyHCooling clamps to its table bounds), O(1) index arithmetic.
 #include "sph_cool.h"
 
+// One rounding per operation in this file's FP64 physics and update (as the
+// oracle is compiled), so an update computes the same bits in every kernel that
+// performs it; the FP32 table interpolation (sph_cool.h, above this line) keeps
+// the default contraction it has in every other file that uses it.
+#pragma clang fp contract(off)
+
 #define CL_C 29979245800.0
 #define CL_HEV 4.135667696e-15
 #define CL_HPL 6.626196e-27
@@ -288,6 +294,118 @@ __global__ void __launch_bounds__(256) k_cool_points(int64_t n, const float *__r
     logL[p] = (float)log10(fmax(l, 1e-40));
 }
 
+// ---------------------------------------------------------------- temperature table
+__global__ void __launch_bounds__(256) k_cool_ttable(CoolTAxes ax, double XH, const double *__restrict__ rf1,
+                                                     float *__restrict__ logT)
+{
+    const int64_t total = (int64_t)ax.n[0] * ax.n[1] * ax.n[2];
+    int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= total) return;
+    int64_t r = q;
+    const int iJ = (int)(r % ax.n[2]); r /= ax.n[2];
+    const int iu = (int)(r % ax.n[1]); r /= ax.n[1];
+    const int iN = (int)r;
+    const double lnH = ax.lo[0] + iN * ax.d[0], lu = ax.lo[1] + iu * ax.d[1], lJ = ax.lo[2] + iJ * ax.d[2];
+    logT[q] = (float)log10(u_to_T(pow(10.0, lu), pow(10.0, lnH), XH, pow(10.0, lJ), rf1));
+}
+
+__global__ void __launch_bounds__(256) k_cool_tlookup(int64_t n, CoolTAxes ax, const float *__restrict__ logT,
+                                                      const double *__restrict__ u, const double *__restrict__ nH,
+                                                      const double *__restrict__ J21, double J21c,
+                                                      double *__restrict__ T)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float v[3] = {(float)log10(nH[i]), (float)log10(u[i]), (float)log10(J21 ? J21[i] : J21c)};
+    T[i] = pow(10.0, (double)cool_ttable_interp(ax, logT, v));
+}
+
+// ---------------------------------------------------------------- explicit update
+__device__ __forceinline__ double clampd(double v, double lo, double hi) { return fmin(fmax(v, lo), hi); }
+
+// u_cgs + rate n_H^2 / rho_cgs dt_s, floored; returns the code-unit value and
+// whether the floor was applied.
+__device__ __forceinline__ double cool_finish(const CoolUpdate &cu, double rho_cgs, double nH, double u_cgs,
+                                              double net, bool &floored)
+{
+    double un = u_cgs + net * (nH * nH / rho_cgs) * cu.dt_s;
+    floored = !(un >= cu.u_floor);
+    if (floored) un = cu.u_floor;
+    return un / cu.u_cgs;
+}
+
+__device__ __forceinline__ void count_floor(unsigned long long *nfloor, bool floored)
+{
+    const uint64_t b = __ballot(floored);
+    if ((threadIdx.x & 63) == 0 && b) spread_add(nfloor, 0, (unsigned long long)__popcll(b));
+}
+
+template <int KIND, typename RHO>
+__global__ void __launch_bounds__(256) k_cool_update(int64_t n, const RHO *__restrict__ rho, const double *u,
+                                                     const double *__restrict__ Z, const double *__restrict__ J21,
+                                                     CoolUpdate cu, CoolAxes ax, const float *__restrict__ tabL,
+                                                     const float *__restrict__ tabG, CoolTAxes tax,
+                                                     const float *__restrict__ tabT, const double *__restrict__ rf1,
+                                                     double *u_out, float *__restrict__ cols,
+                                                     float *__restrict__ logG, unsigned long long *nfloor)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    const int64_t k = valid ? i : 0;
+    const double rho_cgs = (double)rho[k] * cu.rho_cgs;
+    const double nH = cu.XH * rho_cgs / CL_MH;
+    const double u_cgs = u[k] * cu.u_cgs;
+    bool floored = false;
+    if (KIND == SPH_COOLK_EXACT) {
+        const double nHc = clampd(nH, cu.lin_lo[0], cu.lin_hi[0]);
+        const double zc = clampd(Z[k], cu.lin_lo[2], cu.lin_hi[2]);
+        const double jc = clampd(J21[k], cu.lin_lo[3], cu.lin_hi[3]);
+        const double Tc = clampd(u_to_T(u_cgs, nHc, cu.XH, jc, rf1), cu.lin_lo[1], cu.lin_hi[1]);
+        double heat, lam;
+        cool_heat_lambda(Tc, nHc, jc, zc, rf1, heat, lam);
+        const double un = cool_finish(cu, rho_cgs, nH, u_cgs, heat - lam, floored);
+        if (valid) u_out[i] = un;
+    } else {
+        const float lnH = (float)clampd(log10(nH), cu.lo[0], cu.hi[0]);
+        const float lZ = (float)clampd(log10(Z[k]), cu.lo[2], cu.hi[2]);
+        const float lJ = (float)clampd(log10(J21[k]), cu.lo[3], cu.hi[3]);
+        const float tv[3] = {lnH, (float)log10(u_cgs), lJ};
+        const float lT = fminf(fmaxf(cool_ttable_interp(tax, tabT, tv), (float)cu.lo[1]), (float)cu.hi[1]);
+        const float v[4] = {lnH, lT, lZ, lJ};
+        float L, G;
+        if (KIND == SPH_COOLK_TABLE) {
+            cool_table_interp(ax, tabL, tabG, v, L, G);
+            const double un = cool_finish(cu, rho_cgs, nH, u_cgs, exp10((double)G) - exp10((double)L), floored);
+            if (valid) u_out[i] = un;
+        } else {
+            cool_table_interp(ax, tabG, nullptr, v, G, L);  // Gamma only
+            if (valid) {
+#pragma unroll
+                for (int a = 0; a < 4; a++) cols[a * n + i] = v[a];
+                logG[i] = G;
+            }
+        }
+    }
+    if (KIND != SPH_COOLK_PREP) count_floor(nfloor, valid && floored);
+}
+
+template <typename RHO>
+__global__ void __launch_bounds__(256) k_cool_apply(int64_t n, const RHO *__restrict__ rho, const double *u,
+                                                    const float *__restrict__ logL, const float *__restrict__ logG,
+                                                    CoolUpdate cu, double *u_out, unsigned long long *nfloor)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    const int64_t k = valid ? i : 0;
+    const double rho_cgs = (double)rho[k] * cu.rho_cgs;
+    const double nH = cu.XH * rho_cgs / CL_MH;
+    bool floored = false;
+    const double un =
+        cool_finish(cu, rho_cgs, nH, u[k] * cu.u_cgs, exp10((double)logG[k]) - exp10((double)logL[k]), floored);
+    if (valid) u_out[i] = un;
+    count_floor(nfloor, valid && floored);
+}
+
 namespace sph {
 
 hipError_t cool_radfield(hipStream_t st, double *rf1)
@@ -338,5 +456,59 @@ hipError_t cool_log_lambda_points(hipStream_t st, int64_t n, const float *x4, co
     k_cool_points<<<sph_blocks(n, 256), 256, 0, st>>>(n, x4, rf1, logL);
     return hipGetLastError();
 }
+
+hipError_t cool_ttable_build(hipStream_t st, const CoolTAxes &tax, double XH, const double *rf1, float *logT)
+{
+    const int64_t total = (int64_t)tax.n[0] * tax.n[1] * tax.n[2];
+    k_cool_ttable<<<sph_blocks(total, 256), 256, 0, st>>>(tax, XH, rf1, logT);
+    return hipGetLastError();
+}
+
+hipError_t cool_ttable_lookup(hipStream_t st, int64_t n, const CoolTAxes &tax, const float *logT, const double *u,
+                              const double *nH, const double *J21, double J21c, double *T)
+{
+    k_cool_tlookup<<<sph_blocks(n, 256), 256, 0, st>>>(n, tax, logT, u, nH, J21, J21c, T);
+    return hipGetLastError();
+}
+
+template <typename RHO>
+hipError_t cool_update(hipStream_t st, int64_t n, int kind, const RHO *rho, const double *u, const double *Z,
+                       const double *J21, const CoolUpdate &cu, const CoolAxes &ax, const float *tabL,
+                       const float *tabG, const CoolTAxes &tax, const float *tabT, const double *rf1, double *u_out,
+                       float *cols, float *logG, unsigned long long *nfloor)
+{
+    const unsigned nb = sph_blocks(n, 256);
+    if (kind == SPH_COOLK_EXACT)
+        k_cool_update<SPH_COOLK_EXACT, RHO><<<nb, 256, 0, st>>>(n, rho, u, Z, J21, cu, ax, tabL, tabG, tax, tabT, rf1,
+                                                                 u_out, cols, logG, nfloor);
+    else if (kind == SPH_COOLK_TABLE)
+        k_cool_update<SPH_COOLK_TABLE, RHO><<<nb, 256, 0, st>>>(n, rho, u, Z, J21, cu, ax, tabL, tabG, tax, tabT, rf1,
+                                                                 u_out, cols, logG, nfloor);
+    else if (kind == SPH_COOLK_PREP)
+        k_cool_update<SPH_COOLK_PREP, RHO><<<nb, 256, 0, st>>>(n, rho, u, Z, J21, cu, ax, tabL, tabG, tax, tabT, rf1,
+                                                                u_out, cols, logG, nfloor);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+template <typename RHO>
+hipError_t cool_apply(hipStream_t st, int64_t n, const RHO *rho, const double *u, const float *logL,
+                      const float *logG, const CoolUpdate &cu, double *u_out, unsigned long long *nfloor)
+{
+    k_cool_apply<RHO><<<sph_blocks(n, 256), 256, 0, st>>>(n, rho, u, logL, logG, cu, u_out, nfloor);
+    return hipGetLastError();
+}
+
+#define INST(RHO)                                                                                                    \
+    template hipError_t cool_update<RHO>(hipStream_t, int64_t, int, const RHO *, const double *, const double *,      \
+                                         const double *, const CoolUpdate &, const CoolAxes &, const float *,        \
+                                         const float *, const CoolTAxes &, const float *, const double *, double *,  \
+                                         float *, float *, unsigned long long *);                                    \
+    template hipError_t cool_apply<RHO>(hipStream_t, int64_t, const RHO *, const double *, const float *,            \
+                                        const float *, const CoolUpdate &, double *, unsigned long long *);
+INST(float)
+INST(double)
+#undef INST
 
 }  // namespace sph

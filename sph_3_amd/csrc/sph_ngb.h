@@ -69,7 +69,7 @@ hipError_t neighbour_rows(hipStream_t st, int64_t ngroups, const int32_t *groups
 // is too long or an extra was lost -- rebuild with a walk.  Synchronises the stream.
 hipError_t neighbour_rows_from_candidates(hipStream_t st, int64_t n, int cap, const double *cand, const int32_t *cidx,
                                          const int32_t *ccnt, const double *h, const double *R, int stride,
-                                         int32_t *cnt, int32_t *idx, int2 *ext, int64_t ext_cap,
+                                         int32_t *cnt, int32_t *cnt0, int32_t *idx, int2 *ext, int64_t ext_cap,
                                          unsigned long long *next, unsigned long long *spread,
                                          unsigned long long *totals, int64_t *n_long, int64_t *total,
                                          int64_t *n_extra);

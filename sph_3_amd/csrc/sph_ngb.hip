@@ -487,14 +487,16 @@ __global__ void __launch_bounds__(64) k_ngb(const int32_t *__restrict__ groups, 
 //   h_j >  h_i:  likewise i in C_j.
 // k_cand_rows keeps { j in C_i : r^2 < (h_i+h_j)^2 } in candidate order and
 // records every pair with i in C_j, h_j > h_i, j not in C_i (r^2 >= R_i^2) as an
-// extra (target i, member j) from j's row; k_ext_append appends the extras.
+// extra (target i, member j) from j's row; k_ext_append appends the extras and
+// k_row_tail_sort puts each row's appended part in ascending order, so a row's
+// order (and every sum over it) does not depend on the order of the appends.
 // Spread counters: 0 += rows longer than stride, 1 += entries.
 template <int G>
 __global__ void __launch_bounds__(256) k_cand_rows(int64_t n, int cap, const double *__restrict__ cand,
                                                    const int32_t *__restrict__ cidx, const int32_t *__restrict__ ccnt,
                                                    const double *__restrict__ h, const double *__restrict__ R,
-                                                   int stride, int32_t *__restrict__ cnt, int32_t *__restrict__ idx,
-                                                   int2 *__restrict__ ext, int64_t ext_cap,
+                                                   int stride, int32_t *__restrict__ cnt, int32_t *__restrict__ cnt0,
+                                                   int32_t *__restrict__ idx, int2 *__restrict__ ext, int64_t ext_cap,
                                                    unsigned long long *__restrict__ next,
                                                    unsigned long long *__restrict__ spread)
 {
@@ -535,7 +537,10 @@ __global__ void __launch_bounds__(256) k_cand_rows(int64_t n, int cap, const dou
             if (e < (unsigned long long)rcap) ext[reg * rcap + (int64_t)e] = make_int2(j, (int)i);
         }
     }
-    if (valid && l == 0) cnt[i] = w;
+    if (valid && l == 0) {
+        cnt[i] = w;
+        cnt0[i] = w;
+    }
     unsigned long long nlong = (valid && l == 0 && w > stride) ? 1ull : 0ull;
     unsigned long long nent = (valid && l == 0) ? (unsigned long long)w : 0ull;
     for (int off = 32; off >= 1; off >>= 1) {
@@ -570,6 +575,28 @@ __global__ void __launch_bounds__(256) k_ext_append(int64_t ext_cap, const int2 
     }
 }
 
+// Ascending order of idx[i*stride + cnt0[i] .. i*stride + min(cnt[i], stride)):
+// insertion sort per row (the appended parts hold about one entry on average).
+__global__ void __launch_bounds__(256) k_row_tail_sort(int64_t n, const int32_t *__restrict__ cnt0,
+                                                       const int32_t *__restrict__ cnt, int32_t *__restrict__ idx,
+                                                       int stride)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b = cnt0[i], e = min(cnt[i], stride);
+    if (e - b < 2) return;
+    int32_t *row = idx + i * (int64_t)stride;
+    for (int k = b + 1; k < e; k++) {
+        const int32_t v = row[k];
+        int q = k - 1;
+        while (q >= b && row[q] > v) {
+            row[q + 1] = row[q];
+            q--;
+        }
+        row[q + 1] = v;
+    }
+}
+
 // *out = max_i h_new[i] / h_old[i] (1 + 8 eps): positive doubles order like
 // their bit patterns, so one unsigned 64-bit atomicMax per workgroup.
 __global__ void __launch_bounds__(256) k_hratio(int64_t n, const double *__restrict__ hn,
@@ -592,7 +619,7 @@ namespace sph {
 
 hipError_t neighbour_rows_from_candidates(hipStream_t st, int64_t n, int cap, const double *cand, const int32_t *cidx,
                                          const int32_t *ccnt, const double *h, const double *R, int stride,
-                                         int32_t *cnt, int32_t *idx, int2 *ext, int64_t ext_cap,
+                                         int32_t *cnt, int32_t *cnt0, int32_t *idx, int2 *ext, int64_t ext_cap,
                                          unsigned long long *next, unsigned long long *spread,
                                          unsigned long long *totals, int64_t *n_long, int64_t *total,
                                          int64_t *n_extra)
@@ -600,9 +627,10 @@ hipError_t neighbour_rows_from_candidates(hipStream_t st, int64_t n, int cap, co
     if (stride < 1 || cap < 1 || ext_cap < SPH_SPREAD) return hipErrorInvalidValue;
     fill<unsigned long long>(st, SPH_SPREAD_WORDS, next, 0ull);
     fill<unsigned long long>(st, 2 * SPH_SPREAD_WORDS, spread, 0ull);
-    k_cand_rows<16><<<sph_blocks(n * 16, 256), 256, 0, st>>>(n, cap, cand, cidx, ccnt, h, R, stride, cnt, idx, ext,
-                                                            ext_cap, next, spread);
+    k_cand_rows<16><<<sph_blocks(n * 16, 256), 256, 0, st>>>(n, cap, cand, cidx, ccnt, h, R, stride, cnt, cnt0, idx,
+                                                            ext, ext_cap, next, spread);
     k_ext_append<<<sph_blocks(ext_cap, 256), 256, 0, st>>>(ext_cap, ext, next, cnt, idx, stride, spread);
+    k_row_tail_sort<<<sph_blocks(n, 256), 256, 0, st>>>(n, cnt0, cnt, idx, stride);
     spread_total(st, 2, spread, totals);
     unsigned long long hc[2] = {0, 0}, nx[SPH_SPREAD];
     hipError_t e = hipMemcpyAsync(hc, totals, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);

@@ -37,3 +37,13 @@ def uniform_cube(n_side: int, seed: int = 0, jitter: float = 0.1, L: float = 1.0
     pos = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
     pos += rng.uniform(-jitter, jitter, pos.shape) * a
     return pos
+
+
+def cooling_fields(n: int, seed: int = 43):
+    """Per-particle metallicity Z [Z_sun], log-uniform in [1e-3, 1], and UV field
+    amplitude J21, log-uniform in [1e-2, 1e2], from a seeded PCG64 stream (the
+    Evrard-plus-cooling workload)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    Z = 10.0 ** rng.uniform(-3.0, 0.0, n)
+    J21 = 10.0 ** rng.uniform(-2.0, 2.0, n)
+    return Z, J21

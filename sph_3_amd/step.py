@@ -5,7 +5,8 @@ GPU between steps.
 Per step (KDK leapfrog):  v += a dt/2; r += v dt; Morton re-sort; tree;
 FP64 Newton-Raphson h solve; CSR neighbour lists; density + EOS; tree gravity;
 div/curl v (Balsara); pressure + AV acceleration (a = a_grav + a_sph);
-v += a dt/2; du/dt with the kicked v; u = u_prev + dt/2 (du + du_prev).
+v += a dt/2; du/dt with the kicked v; u = u_prev + dt/2 (du + du_prev);
+optionally the explicit cooling update of CR/v3t.cu:1527-1536 (``set_cooling``).
 """
 from __future__ import annotations
 
@@ -53,6 +54,29 @@ class DeviceSPH:
                 "sph_state_set")
         self.primed = False
 
+    def set_cooling(self, Z, J21, params, surrogate=None, table_ranges=None):
+        """Per-particle metallicity Z [Z_sun] and UV amplitude J21 (caller order) and the
+        cooling update (``cooling.CoolingParams``) of every later step.  The cooling
+        table is built in this state's context; mode "mlp" also loads ``surrogate``
+        (a calibrated ``cooling.CoolingSurrogate``).  ``params=None`` switches it off."""
+        from . import cooling as C
+
+        c = self.ctx
+        if params is None:
+            c.check(c.lib.sph_state_set_cooling(c.h, None, None, None), "sph_state_set_cooling")
+            return
+        if params.mode == "mlp":
+            if surrogate is None:
+                raise ValueError('mode "mlp" needs a calibrated CoolingSurrogate')
+            surrogate.install(c)
+        else:
+            C.install_table(c, table_ranges or (surrogate.table.ranges if surrogate else C.TABLE_RANGES))
+        n = self.n
+        Z = np.broadcast_to(f64(Z), (n,)).copy()
+        J = np.broadcast_to(f64(J21), (n,)).copy()
+        cp = params.c()
+        c.check(c.lib.sph_state_set_cooling(c.h, ptr(Z), ptr(J), ctypes.byref(cp)), "sph_state_set_cooling")
+
     def prime(self):
         """Forces at the initial positions and the first u update."""
         p = self.cfg.params()
@@ -95,6 +119,7 @@ class DeviceSPH:
         c.check(c.lib.sph_get_stats(c.h, ctypes.byref(s)), "sph_get_stats")
         return dict(n=s.n, steps=s.steps, ngb_total=s.ngb_total, grav_pp=s.grav_pp, grav_pn=s.grav_pn,
                     h_unconverged=s.h_unconverged, h_rounds=s.h_rounds, h_tests=s.h_tests, h_terms=s.h_terms, ngb_extra=s.ngb_extra,
+                    cool_floor=s.cool_floor, cool_fallback=s.cool_fallback,
                     phase_ms={k: s.phase_ms[i] for i, k in enumerate(PHASES)},
                     phase_calls={k: s.phase_calls[i] for i, k in enumerate(PHASES)})
 

@@ -66,3 +66,43 @@ def test_metal_term_rb95():
     h0, l0 = O.cool_heat_lambda(2e5, 1.0, 1.0, 0.0)
     h1, l1 = O.cool_heat_lambda(2e5, 1.0, 1.0, 0.3)
     assert h1 == h0 and abs((l1 - l0) - 0.3 * O.cool_rb95(2e5)) < 1e-12 * l1
+
+
+UNITS = (6.77e-25, 4.3e13, 4.7e15, 0.7, 1.8e11)  # rho_cgs, u_cgs, t_s, XH, u_floor
+BOX_LO, BOX_HI = (-4.0, 2.0, -3.0, -2.0), (4.0, 8.0, 0.0, 2.0)
+
+
+def test_cool_update_rate_is_photolibs3_rate():
+    """Inside the box with T(u) in [1e4, 1e8] K, (u' - u) rho / (nH^2 dt) is
+    coolingRateFromU_h (photolibs3.py:470-486, pinned by the reference's vectors)."""
+    XH = UNITS[3]
+    for rho_code, u_code, J, Z in [(1.0, 0.5, 1.0, 0.1), (30.0, 2.0, 0.05, 1.0), (0.2, 8.0, 50.0, 0.01)]:
+        dt = 1e-9
+        un, fl = O.cool_update([rho_code], [u_code], [Z], [J], dt, UNITS, BOX_LO, BOX_HI)
+        rc = rho_code * UNITS[0]; nH = XH * rc / 1.6726e-24; uc = u_code * UNITS[1]
+        T = O.cool_u_to_T(uc, nH, XH, J)
+        assert 1e4 < T < 1e8 and not fl[0]
+        rate = (un[0] - u_code) * UNITS[1] * rc / (nH * nH * dt * UNITS[2])
+        ref = O.cool_rate_from_u(uc, nH, XH, J, Z)
+        assert abs(rate / ref - 1) < 1e-6, (rate, ref)
+
+
+def test_cool_update_clamps_floor_and_dt():
+    XH = UNITS[3]
+    # dt = 0: u unchanged above the floor, the floor below it
+    u = np.array([1.0, 1e-9])
+    un, fl = O.cool_update([1.0, 1.0], u, [0.1, 0.1], [1.0, 1.0], 0.0, UNITS, BOX_LO, BOX_HI)
+    assert un[0] == 1.0 and not fl[0]
+    assert un[1] == UNITS[4] / UNITS[1] and fl[1]
+    # n_H above the box: the rate per nH^2 is the box edge's for any density there
+    nH_top = 10 ** BOX_HI[0]
+    rho = np.array([2.0, 50.0]) * nH_top * 1.6726e-24 / (XH * UNITS[0])
+    dt = 1e-14
+    un, fl = O.cool_update(rho, [3.0, 3.0], [0.1, 0.1], [1.0, 1.0], dt, UNITS, BOX_LO, BOX_HI)
+    rc = rho * UNITS[0]; nH = XH * rc / 1.6726e-24
+    per = (un - 3.0) * rc / (nH * nH)
+    assert not fl.any() and abs(per[0] / per[1] - 1) < 1e-9
+    # linear in dt while no particle floors
+    d1, _ = O.cool_update([1.0], [0.5], [0.1], [1.0], 1e-9, UNITS, BOX_LO, BOX_HI)
+    d2, _ = O.cool_update([1.0], [0.5], [0.1], [1.0], 2e-9, UNITS, BOX_LO, BOX_HI)
+    assert abs((d2[0] - 0.5) / (d1[0] - 0.5) - 2.0) < 1e-6
