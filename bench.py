@@ -130,7 +130,7 @@ def traffic_table():
         return {}
 
 
-def mlp_section(n_inf):
+def mlp_section(n_inf, epochs=200):
     """Cooling surrogate at the 4M-particle cooling configuration: on-device training
     (2M exact samples), calibration, gated inference throughput and accuracy."""
     from sph_3_amd import cooling as C
@@ -138,7 +138,7 @@ def mlp_section(n_inf):
     tab = C.CoolingTable()
     net = C.CoolingSurrogate(tab, seed=0)
     mlp_section.net = net  # reused by the cooling step
-    n_samples, epochs = 2_000_000, 200
+    n_samples = 2_000_000
     t0 = time.perf_counter()
     losses = net.train(n_samples=n_samples, epochs=epochs)
     t_train = time.perf_counter() - t0
@@ -266,6 +266,7 @@ def main():
     ap.add_argument("--no-fp64-form", action="store_true")
     ap.add_argument("--no-mlp", action="store_true")
     ap.add_argument("--mlp-n", type=int, default=4194304)
+    ap.add_argument("--mlp-epochs", type=int, default=200, help="training passes of the surrogate")
     ap.add_argument("--no-cooling-step", action="store_true")
     ap.add_argument("--cooling-n", type=int, default=4194304)
     ap.add_argument("--copy-probe", type=int, default=0, help="only run a device copy of this many bytes")
@@ -384,7 +385,7 @@ def main():
         roof["traffic_source"] = "profiles/hbm_traffic_evrard_1M.md"
     if not args.no_mlp:
         log("[bench] cooling surrogate: train, calibrate, time")
-        out["mlp_cooling"] = mlp_section(args.mlp_n)
+        out["mlp_cooling"] = mlp_section(args.mlp_n, args.mlp_epochs)
     if not args.no_fp64_form:
         sim64, dt64 = run("fp64", max(2, args.steps // 2), 1)
         out["fp64_form"] = {"ms_per_step": 1e3 * dt64 / max(2, args.steps // 2),

@@ -123,6 +123,12 @@ struct sph_ctx {
     hipEvent_t ev[2 * SPH_PH_COUNT];
     bool timing = false;
     uint32_t ph_rec = 0;  // phases recorded since the last finish_step
+    // deferred read-backs of the step: pinned host words ([0..4] h-solve totals of the first
+    // round, [8..9] neighbour-row totals, [10..10+SPH_SPREAD) extras region counters) and the
+    // events recorded after their copies
+    unsigned long long *hpin = nullptr;
+    hipEvent_t ev_h = nullptr, ev_n = nullptr;
+    int64_t hs_unc = 0, hs_work[2] = {0, 0};
     int rho_prec = SPH_PREC_MIXED;  // arithmetic type of the resident rho buffer
 };
 
@@ -439,6 +445,12 @@ extern "C" int sph_ctx_create(int device, sph_ctx **out)
             delete ctx;
             return SPH_ERR_HIP;
         }
+    if (hipHostMalloc((void **)&ctx->hpin, 128 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_h, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_n, hipEventDisableTiming) != hipSuccess) {
+        delete ctx;
+        return SPH_ERR_HIP;
+    }
     *out = ctx;
     return SPH_OK;
 }
@@ -468,6 +480,9 @@ extern "C" int sph_ctx_destroy(sph_ctx *ctx)
     for (Buf *b : cool)
         if (b->p) (void)hipFree(b->p);
     for (int k = 0; k < 2 * SPH_PH_COUNT; k++) (void)hipEventDestroy(ctx->ev[k]);
+    if (ctx->ev_h) (void)hipEventDestroy(ctx->ev_h);
+    if (ctx->ev_n) (void)hipEventDestroy(ctx->ev_n);
+    if (ctx->hpin) (void)hipHostFree(ctx->hpin);
     (void)hipStreamDestroy(ctx->st);
     delete ctx;
     return SPH_OK;
@@ -482,16 +497,17 @@ extern "C" const char *sph_last_error(const sph_ctx *ctx)
 // =========================================================== h solves
 namespace {
 
-int run_hsolve(sph_ctx *ctx, int64_t m, const double *tx, const double *ty, const double *tz, int mode,
-               double param, int n_low, int n_up, int max_steps, double tol, int64_t *n_unconv, int64_t *rounds_out,
-               int64_t *work = nullptr, bool keep_indices = false)
+// Parameters of an h solve over m targets (scratch in the context); host counters go to
+// unc (unconverged), rounds_out and work (distance tests, kernel terms).
+int hsolve_params(sph_ctx *ctx, int64_t m, const double *tx, const double *ty, const double *tz, int mode,
+                  double param, int n_low, int n_up, int max_steps, double tol, int64_t *unc, int64_t *rounds_out,
+                  int64_t *work, bool keep_indices, sph::HSolveParams &p)
 {
     CK(ensure(ctx, ctx->cand, (size_t)m * SPH_HCAP * 8));
     if (keep_indices) {
         CK(ensure(ctx, ctx->ccidx, (size_t)m * SPH_HCAP * 4));
         CK(ensure(ctx, ctx->cccnt, (size_t)m * 4));
     }
-    sph::HSolveParams p;
     p.m = m;
     p.tx = tx; p.ty = ty; p.tz = tz;
     p.x = D(x); p.y = D(y); p.z = D(z);
@@ -512,14 +528,26 @@ int run_hsolve(sph_ctx *ctx, int64_t m, const double *tx, const double *ty, cons
     p.state = I32(hstate);
     p.counters = P_<unsigned long long>(ctx->spread);
     p.totals = P_<unsigned long long>(ctx->hcount);
-    int64_t unc = 0;
-    p.unconverged_total = &unc;
+    p.unconverged_total = unc;
     p.groups = (tx == D(x)) ? I32(groups) : nullptr;  // targets are the sorted particles: tree-node groups
     p.ngroups = ctx->ngroups;
     p.rounds = rounds_out;
     p.work = work;
     p.cidx = keep_indices ? I32(ccidx) : nullptr;
     p.ccnt = keep_indices ? I32(cccnt) : nullptr;
+    p.host_totals = ctx->hpin;
+    p.done = ctx->ev_h;
+    return SPH_OK;
+}
+
+int run_hsolve(sph_ctx *ctx, int64_t m, const double *tx, const double *ty, const double *tz, int mode,
+               double param, int n_low, int n_up, int max_steps, double tol, int64_t *n_unconv, int64_t *rounds_out,
+               int64_t *work = nullptr, bool keep_indices = false)
+{
+    sph::HSolveParams p;
+    int64_t unc = 0;
+    CK(hsolve_params(ctx, m, tx, ty, tz, mode, param, n_low, n_up, max_steps, tol, &unc, rounds_out, work,
+                     keep_indices, p));
     int64_t pending = 0;
     HIPCK(sph::hsolve(ctx->st, p, SPH_HROUNDS, &pending));
     if (pending) return fail(ctx, SPH_ERR_NOCONV, "smoothing-length solve: " + std::to_string(pending) +
@@ -931,67 +959,41 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     ph_end(ctx, SPH_PH_HMOMENTS);
     ph_begin(ctx, SPH_PH_HSOLVE);
     HIPCK(hipMemcpyAsync(ctx->hin.p, ctx->h.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-    int64_t unc = 0;
-    ctx->stats.h_tests = ctx->stats.h_terms = 0;
-    int64_t work[2] = {0, 0};
-    CK(run_hsolve(ctx, n, D(x), D(y), D(z), SPH_H_NEWTON, p->n_ngb, 0, 0, 0, p->h_tol, &unc, &ctx->stats.h_rounds,
-                  work, true));
-    ctx->stats.h_tests = work[0];
-    ctx->stats.h_terms = work[1];
-    std::swap(ctx->h, ctx->hout);
-    ctx->stats.h_unconverged += unc;
+    ctx->hs_unc = 0;
+    ctx->hs_work[0] = ctx->hs_work[1] = 0;
+    sph::HSolveParams hp;
+    CK(hsolve_params(ctx, n, D(x), D(y), D(z), SPH_H_NEWTON, p->n_ngb, 0, 0, 0, p->h_tol, &ctx->hs_unc,
+                     &ctx->stats.h_rounds, ctx->hs_work, true, hp));
+    HIPCK(sph::hsolve_begin(st, hp));
     ph_end(ctx, SPH_PH_HSOLVE);
+    std::swap(ctx->h, ctx->hout);  // the solve writes hp.h_out: the new h
+    // The solve's first round is enqueued; its read-back (pending targets) is waited for
+    // only after the gravity walk is enqueued behind it when gravity does not depend on h
+    // (Plummer softening or no gravity), so the host wait does not leave the GPU idle.
+    auto finish_h = [&]() -> int {
+        int64_t pending = 0;
+        bool more = false;
+        HIPCK(sph::hsolve_end(st, hp, SPH_HROUNDS, &pending, &more));
+        if (pending)
+            return fail(ctx, SPH_ERR_NOCONV, "smoothing-length solve: " + std::to_string(pending) +
+                                                 " targets undecided after the maximum number of radius rounds");
+        ctx->stats.h_tests = ctx->hs_work[0];
+        ctx->stats.h_terms = ctx->hs_work[1];
+        ctx->stats.h_unconverged += ctx->hs_unc;
+        return SPH_OK;
+    };
     // --- node h_max (and eps_max = h_max for the spline softening) of the new h.  Boxes, masses
     // and centres of mass do not depend on h: with Plummer softening (or no gravity) the
     // neighbour walk prunes with the h_max of the boxes' pass scaled by max_i h_i/h_old_i
     // instead of a second bottom-up pass; the spline softening needs the exact eps_max.
     const bool spline = p->soft_kind == SPH_SOFT_SPLINE && p->G != 0.0;
     const double *eps = spline ? D(h) : nullptr;
+    if (spline) CK(finish_h());
     ph_begin(ctx, SPH_PH_MOMENTS);
     if (spline)
         HIPCK(sph::tree_moments(st, n, D(x), D(y), D(z), D(m), D(h), eps, P_<WNode>(ctx->nodes), I32(right),
                                 I32(parent), P_<uint32_t>(ctx->flags), I32(front)));
     ph_end(ctx, SPH_PH_MOMENTS);
-    // --- neighbour rows from the h solve's candidates; a tree walk if a row or the extras overflow
-    ph_begin(ctx, SPH_PH_NGB);
-    {
-        CK(ensure(ctx, ctx->idx, (size_t)n * SPH_NGB_STRIDE * 4));
-        const int64_t ext_cap = std::max<int64_t>(4 * n, SPH_SPREAD * 64);
-        CK(ensure(ctx, ctx->cext, (size_t)ext_cap * 8));
-        CK(ensure(ctx, ctx->cnext, (size_t)SPH_SPREAD_WORDS * 8));
-        int64_t nlong = 0, total = 0, nextra = 0;
-        HIPCK(sph::neighbour_rows_from_candidates(
-            st, n, SPH_HCAP, D(cand), I32(ccidx), I32(cccnt), D(h), D(hR), SPH_NGB_STRIDE, I32(cnt), I32(rcnt0), I32(idx),
-            P_<int2>(ctx->cext), ext_cap, P_<unsigned long long>(ctx->cnext),
-            P_<unsigned long long>(ctx->spread) + SPR_N * SPH_SPREAD_WORDS, P_<unsigned long long>(ctx->hcount) + 10,
-            &nlong, &total, &nextra));
-        ctx->stats.ngb_extra = nextra;
-        if (nlong == 0) {
-            ctx->ngb_stride = SPH_NGB_STRIDE;
-            ctx->csr_total = total;
-            ctx->stats.ngb_total = total;
-        } else {
-            const double *hscale = nullptr;
-            if (!spline) {  // node h_max are those of h_old: scale them
-                HIPCK(sph::h_ratio_max(st, n, D(h), D(hin), D(kout) + 7));  // slot 7 (kinetics uses 0..4)
-                hscale = D(kout) + 7;
-            }
-            CK(build_ngb(ctx, false, hscale));
-        }
-    }
-    ph_end(ctx, SPH_PH_NGB);
-    const NgbRows rows = ngb_rows(ctx);
-    // density + EOS, fused with div/curl v and the Balsara factor when the switch is on
-    R *bal = p->balsara ? P_<R>(ctx->bal) : nullptr;
-    ph_begin(ctx, SPH_PH_DENSITY);
-    if (p->balsara)
-        HIPCK(sph::density_divcurl<R>(st, n, D(x), D(y), D(z), D(vx), D(vy), D(vz), D(m), D(h), D(u), p->gamma, rows,
-                                      P_<R>(ctx->rho), P_<R>(ctx->P), P_<R>(ctx->c), P_<R>(ctx->prho2),
-                                      P_<R>(ctx->divV), P_<R>(ctx->curlV), bal));
-    else
-        HIPCK(sph::density<R>(st, n, D(x), D(y), D(z), D(m), D(h), D(u), p->gamma, rows, P_<R>(ctx->rho),
-                              P_<R>(ctx->P), P_<R>(ctx->c), P_<R>(ctx->prho2)));
-    ph_end(ctx, SPH_PH_DENSITY);
     ph_begin(ctx, SPH_PH_GRAVITY);
     unsigned long long *cnt = P_<unsigned long long>(ctx->spread) + SPR_G * SPH_SPREAD_WORDS;
     HIPCK(sph::fill<unsigned long long>(st, 3 * SPH_SPREAD_WORDS, cnt, 0ull));
@@ -1001,12 +1003,64 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     }
     HIPCK(sph::spread_total(st, 3, cnt, P_<unsigned long long>(ctx->hcount) + 5));
     ph_end(ctx, SPH_PH_GRAVITY);
-    ph_begin(ctx, SPH_PH_ACCSPH);
+    if (!spline) CK(finish_h());
+    // --- neighbour rows from the h solve's candidates.  Their overflow check (a row longer
+    // than the stride or a full extras region, then a tree walk) is read back after the
+    // density and acceleration passes are enqueued behind them; on overflow those two passes
+    // are run again on the rebuilt sets.
+    CK(ensure(ctx, ctx->idx, (size_t)n * SPH_NGB_STRIDE * 4));
+    const int64_t ext_cap = std::max<int64_t>(4 * n, SPH_SPREAD * 64);
+    CK(ensure(ctx, ctx->cext, (size_t)ext_cap * 8));
+    CK(ensure(ctx, ctx->cnext, (size_t)SPH_SPREAD_WORDS * 8));
+    ph_begin(ctx, SPH_PH_NGB);
+    HIPCK(sph::neighbour_rows_from_candidates_begin(
+        st, n, SPH_HCAP, D(cand), I32(ccidx), I32(cccnt), D(h), D(hR), SPH_NGB_STRIDE, I32(cnt), I32(rcnt0), I32(idx),
+        P_<int2>(ctx->cext), ext_cap, P_<unsigned long long>(ctx->cnext),
+        P_<unsigned long long>(ctx->spread) + SPR_N * SPH_SPREAD_WORDS, P_<unsigned long long>(ctx->hcount) + 10,
+        ctx->hpin + 8, ctx->ev_n));
+    ph_end(ctx, SPH_PH_NGB);
+    ctx->ngb_stride = SPH_NGB_STRIDE;
+    R *bal = p->balsara ? P_<R>(ctx->bal) : nullptr;
     const bool g = p->G != 0.0;
-    HIPCK(sph::accsph<R>(st, n, D(x), D(y), D(z), D(vx), D(vy), D(vz), D(m), D(h), P_<R>(ctx->rho),
-                         P_<R>(ctx->prho2), P_<R>(ctx->c), bal, p->alpha, rows, g ? D(gx) : nullptr,
-                         g ? D(gy) : nullptr, g ? D(gz) : nullptr, D(ax), D(ay), D(az)));
-    ph_end(ctx, SPH_PH_ACCSPH);
+    // density + EOS, fused with div/curl v and the Balsara factor when the switch is on; then
+    // the acceleration (a = a_grav + a_sph)
+    auto hydro_forces = [&]() -> int {
+        const NgbRows rows = ngb_rows(ctx);
+        ph_begin(ctx, SPH_PH_DENSITY);
+        if (p->balsara)
+            HIPCK(sph::density_divcurl<R>(st, n, D(x), D(y), D(z), D(vx), D(vy), D(vz), D(m), D(h), D(u), p->gamma,
+                                          rows, P_<R>(ctx->rho), P_<R>(ctx->P), P_<R>(ctx->c), P_<R>(ctx->prho2),
+                                          P_<R>(ctx->divV), P_<R>(ctx->curlV), bal));
+        else
+            HIPCK(sph::density<R>(st, n, D(x), D(y), D(z), D(m), D(h), D(u), p->gamma, rows, P_<R>(ctx->rho),
+                                  P_<R>(ctx->P), P_<R>(ctx->c), P_<R>(ctx->prho2)));
+        ph_end(ctx, SPH_PH_DENSITY);
+        ph_begin(ctx, SPH_PH_ACCSPH);
+        HIPCK(sph::accsph<R>(st, n, D(x), D(y), D(z), D(vx), D(vy), D(vz), D(m), D(h), P_<R>(ctx->rho),
+                             P_<R>(ctx->prho2), P_<R>(ctx->c), bal, p->alpha, rows, g ? D(gx) : nullptr,
+                             g ? D(gy) : nullptr, g ? D(gz) : nullptr, D(ax), D(ay), D(az)));
+        ph_end(ctx, SPH_PH_ACCSPH);
+        return SPH_OK;
+    };
+    CK(hydro_forces());
+    {
+        int64_t nlong = 0, total = 0, nextra = 0;
+        HIPCK(sph::neighbour_rows_from_candidates_end(ctx->ev_n, ctx->hpin + 8, ext_cap, &nlong, &total, &nextra));
+        ctx->stats.ngb_extra = nextra;
+        if (nlong == 0) {
+            ctx->csr_total = total;
+            ctx->stats.ngb_total = total;
+        } else {
+            const double *hscale = nullptr;
+            if (!spline) {  // node h_max are those of h_old: scale them
+                HIPCK(sph::h_ratio_max(st, n, D(h), D(hin), D(kout) + 7));  // slot 7 (kinetics uses 0..4)
+                hscale = D(kout) + 7;
+            }
+            CK(build_ngb(ctx, false, hscale));
+            CK(hydro_forces());
+        }
+    }
+    const NgbRows rows = ngb_rows(ctx);
     if (!first) {
         ph_begin(ctx, SPH_PH_KICK);
         HIPCK(sph::kick_drift(st, n, D(vx), D(vy), D(vz), D(ax), D(ay), D(az), 0.5 * p->dt, 0.0, nullptr, nullptr,

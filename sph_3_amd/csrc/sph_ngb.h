@@ -33,12 +33,25 @@ struct HSolveParams {
     const int32_t *groups;       // tree-node groups when the targets are the sorted particles, else null
     int64_t ngroups;
     int64_t *rounds;             // host: rounds used (may be null)
+    // hsolve_begin / hsolve_end: pinned host words (5) that receive the first round's totals
+    // and an event recorded after their copy
+    unsigned long long *host_totals = nullptr;
+    hipEvent_t done = nullptr;
 };
 
 // Runs rounds until every target is decided; *n_unconverged = targets still
 // pending after max_rounds (0 on success).  Synchronises the stream.  Returns
 // hipErrorIllegalState if a walk stack overflowed.
 hipError_t hsolve(hipStream_t st, const HSolveParams &p, int max_rounds, int64_t *n_unconverged);
+// The same in two halves (Newton, count-free modes 0 and 2): hsolve_begin enqueues the first
+// round and the asynchronous copy of its totals to p.host_totals (pinned) followed by
+// p.done, without waiting; the caller may enqueue independent work; hsolve_end waits for
+// p.done (not the stream) and runs further rounds synchronously if targets are pending
+// (*more_rounds = true: the results of anything enqueued from the first round's output
+// must be rebuilt).
+hipError_t hsolve_begin(hipStream_t st, const HSolveParams &p);
+hipError_t hsolve_end(hipStream_t st, const HSolveParams &p, int max_rounds, int64_t *n_unconverged,
+                      bool *more_rounds);
 
 // CSR of S_i = { j : r_ij^2 < (h_i+h_j)^2 } over sorted particles, one
 // wavefront per tree-node group.  hscale (device, may be null): the node h_max
@@ -65,14 +78,19 @@ hipError_t neighbour_rows(hipStream_t st, int64_t ngroups, const int32_t *groups
 // and indices (targets = the sorted particles, same positions and the solved h) instead of
 // a tree walk; exact by the containment argument at k_cand_rows.  ext: ext_cap int2 scratch
 // for the pairs found from the other side (SPH_SPREAD regions), next: SPH_SPREAD_WORDS
-// scratch words (one counter per region).  *n_long > 0: some row
-// is too long or an extra was lost -- rebuild with a walk.  Synchronises the stream.
-hipError_t neighbour_rows_from_candidates(hipStream_t st, int64_t n, int cap, const double *cand, const int32_t *cidx,
-                                         const int32_t *ccnt, const double *h, const double *R, int stride,
-                                         int32_t *cnt, int32_t *cnt0, int32_t *idx, int2 *ext, int64_t ext_cap,
-                                         unsigned long long *next, unsigned long long *spread,
-                                         unsigned long long *totals, int64_t *n_long, int64_t *total,
-                                         int64_t *n_extra);
+// scratch words (one counter per region), cnt0: n int32 scratch (row lengths before the
+// appended pairs).  *n_long > 0: some row is too long or an extra was lost -- rebuild with
+// a walk.  In two halves: _begin enqueues the kernels and the asynchronous copy of the
+// totals and region counters into host_words (pinned, 2 + SPH_SPREAD words) followed by
+// done; _end waits for done (not the stream) and evaluates them.
+hipError_t neighbour_rows_from_candidates_begin(hipStream_t st, int64_t n, int cap, const double *cand,
+                                               const int32_t *cidx, const int32_t *ccnt, const double *h,
+                                               const double *R, int stride, int32_t *cnt, int32_t *cnt0,
+                                               int32_t *idx, int2 *ext, int64_t ext_cap, unsigned long long *next,
+                                               unsigned long long *spread, unsigned long long *totals,
+                                               unsigned long long *host_words, hipEvent_t done);
+hipError_t neighbour_rows_from_candidates_end(hipEvent_t done, const unsigned long long *host_words, int64_t ext_cap,
+                                             int64_t *n_long, int64_t *total, int64_t *n_extra);
 // *out (device) = max_i h_new[i] / h_old[i], rounded up by 8 ulp: the factor that
 // keeps node h_max of h_old a valid bound for the new h.
 hipError_t h_ratio_max(hipStream_t st, int64_t n, const double *h_new, const double *h_old, double *out);

@@ -617,12 +617,12 @@ __global__ void __launch_bounds__(256) k_hratio(int64_t n, const double *__restr
 
 namespace sph {
 
-hipError_t neighbour_rows_from_candidates(hipStream_t st, int64_t n, int cap, const double *cand, const int32_t *cidx,
-                                         const int32_t *ccnt, const double *h, const double *R, int stride,
-                                         int32_t *cnt, int32_t *cnt0, int32_t *idx, int2 *ext, int64_t ext_cap,
-                                         unsigned long long *next, unsigned long long *spread,
-                                         unsigned long long *totals, int64_t *n_long, int64_t *total,
-                                         int64_t *n_extra)
+hipError_t neighbour_rows_from_candidates_begin(hipStream_t st, int64_t n, int cap, const double *cand,
+                                               const int32_t *cidx, const int32_t *ccnt, const double *h,
+                                               const double *R, int stride, int32_t *cnt, int32_t *cnt0,
+                                               int32_t *idx, int2 *ext, int64_t ext_cap, unsigned long long *next,
+                                               unsigned long long *spread, unsigned long long *totals,
+                                               unsigned long long *host_words, hipEvent_t done)
 {
     if (stride < 1 || cap < 1 || ext_cap < SPH_SPREAD) return hipErrorInvalidValue;
     fill<unsigned long long>(st, SPH_SPREAD_WORDS, next, 0ull);
@@ -632,23 +632,30 @@ hipError_t neighbour_rows_from_candidates(hipStream_t st, int64_t n, int cap, co
     k_ext_append<<<sph_blocks(ext_cap, 256), 256, 0, st>>>(ext_cap, ext, next, cnt, idx, stride, spread);
     k_row_tail_sort<<<sph_blocks(n, 256), 256, 0, st>>>(n, cnt0, cnt, idx, stride);
     spread_total(st, 2, spread, totals);
-    unsigned long long hc[2] = {0, 0}, nx[SPH_SPREAD];
-    hipError_t e = hipMemcpyAsync(hc, totals, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
+    hipError_t e = hipMemcpyAsync(host_words, totals, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
     if (e != hipSuccess) return e;
-    e = hipMemcpy2DAsync(nx, sizeof(unsigned long long), next, SPH_SPREAD_PAD * sizeof(unsigned long long),
+    e = hipMemcpy2DAsync(host_words + 2, sizeof(unsigned long long), next, SPH_SPREAD_PAD * sizeof(unsigned long long),
                          sizeof(unsigned long long), SPH_SPREAD, hipMemcpyDeviceToHost, st);
     if (e != hipSuccess) return e;
-    e = hipStreamSynchronize(st);
+    e = hipEventRecord(done, st);
+    if (e != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+hipError_t neighbour_rows_from_candidates_end(hipEvent_t done, const unsigned long long *host_words, int64_t ext_cap,
+                                             int64_t *n_long, int64_t *total, int64_t *n_extra)
+{
+    hipError_t e = hipEventSynchronize(done);
     if (e != hipSuccess) return e;
     int64_t nextra = 0, lost = 0;
     for (int r = 0; r < SPH_SPREAD; r++) {
-        nextra += (int64_t)nx[r];
-        if (nx[r] > (unsigned long long)(ext_cap / SPH_SPREAD)) lost = 1;  // a lost extra voids the rows
+        nextra += (int64_t)host_words[2 + r];
+        if (host_words[2 + r] > (unsigned long long)(ext_cap / SPH_SPREAD)) lost = 1;  // a lost extra voids the rows
     }
-    *n_long = (int64_t)hc[0] + lost;
-    *total = (int64_t)hc[1];
+    *n_long = (int64_t)host_words[0] + lost;
+    *total = (int64_t)host_words[1];
     *n_extra = nextra;
-    return hipGetLastError();
+    return hipSuccess;
 }
 
 hipError_t h_ratio_max(hipStream_t st, int64_t n, const double *h_new, const double *h_old, double *out)
@@ -659,7 +666,9 @@ hipError_t h_ratio_max(hipStream_t st, int64_t n, const double *h_new, const dou
     return hipGetLastError();
 }
 
-hipError_t hsolve(hipStream_t st, const HSolveParams &p, int max_rounds, int64_t *n_unconverged)
+namespace {
+
+HSolveArgs hsolve_args(const HSolveParams &p)
 {
     HSolveArgs a;
     a.m = p.m; a.tx = p.tx; a.ty = p.ty; a.tz = p.tz;
@@ -671,8 +680,80 @@ hipError_t hsolve(hipStream_t st, const HSolveParams &p, int max_rounds, int64_t
     a.pending = p.counters;
     a.groups = p.groups;
     a.gnodes = p.nodes;
+    return a;
+}
+
+// One Newton / k-th round: counters cleared, the solve, the fold into p.totals.
+void hsolve_round(hipStream_t st, const HSolveParams &p, const HSolveArgs &a)
+{
+    const int64_t nblk = p.groups ? p.ngroups : (p.m + 63) / 64;
+    fill<unsigned long long>(st, 5 * SPH_SPREAD_WORDS, p.counters, 0ull);
+    k_hsolve<<<(unsigned)nblk, 64, 0, st>>>(a);
+    spread_total(st, 5, p.counters, p.totals);
+}
+
+// Book-keeping of one round's totals hc; true when every target is decided.
+hipError_t hsolve_account(const HSolveParams &p, const unsigned long long *hc, int round, bool *done)
+{
+    if (p.work) {
+        p.work[0] += (int64_t)hc[3];
+        p.work[1] += (int64_t)hc[4];
+    }
+    if (hc[2]) return hipErrorIllegalState;  // walk stack overflow (cannot happen for valid trees)
+    if (p.unconverged_total) *p.unconverged_total += (int64_t)hc[1];
+    if (p.rounds) *p.rounds = round + 1;
+    *done = hc[0] == 0;
+    return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t hsolve_begin(hipStream_t st, const HSolveParams &p)
+{
+    if (p.mode == 1 || p.m <= 0 || !p.host_totals || !p.done) return hipErrorInvalidValue;
+    if (p.bucket > 64 || p.cap > SPH_HCAP_SLOTS || p.cap < 1) return hipErrorInvalidValue;
+    const HSolveArgs a = hsolve_args(p);
+    k_hsolve_init<<<sph_blocks(p.m, 256), 256, 0, st>>>(p.m, p.h_in, p.r_fac, p.R, p.h_out, p.steps, p.state);
+    hsolve_round(st, p, a);
+    hipError_t e = hipMemcpyAsync(p.host_totals, p.totals, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    e = hipEventRecord(p.done, st);
+    if (e != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+hipError_t hsolve_end(hipStream_t st, const HSolveParams &p, int max_rounds, int64_t *n_unconverged,
+                      bool *more_rounds)
+{
+    *more_rounds = false;
+    hipError_t e = hipEventSynchronize(p.done);
+    if (e != hipSuccess) return e;
+    bool done = false;
+    e = hsolve_account(p, p.host_totals, 0, &done);
+    if (e != hipSuccess) return e;
+    if (done) { *n_unconverged = 0; return hipSuccess; }
+    *more_rounds = true;
+    const HSolveArgs a = hsolve_args(p);
+    unsigned long long hc[5] = {p.host_totals[0], 0, 0, 0, 0};
+    for (int r = 1; r < max_rounds; r++) {
+        hsolve_round(st, p, a);
+        e = hipMemcpyAsync(hc, p.totals, sizeof(hc), hipMemcpyDeviceToHost, st);
+        if (e != hipSuccess) return e;
+        e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return e;
+        e = hsolve_account(p, hc, r, &done);
+        if (e != hipSuccess) return e;
+        if (done) { *n_unconverged = 0; return hipGetLastError(); }
+    }
+    *n_unconverged = (int64_t)hc[0];
+    return hipGetLastError();
+}
+
+hipError_t hsolve(hipStream_t st, const HSolveParams &p, int max_rounds, int64_t *n_unconverged)
+{
     if (p.m <= 0) { *n_unconverged = 0; return hipSuccess; }
     if (p.bucket > 64 || p.cap > SPH_HCAP_SLOTS || p.cap < 1) return hipErrorInvalidValue;
+    const HSolveArgs a = hsolve_args(p);
     k_hsolve_init<<<sph_blocks(p.m, 256), 256, 0, st>>>(p.m, p.h_in, p.r_fac, p.R, p.h_out, p.steps, p.state);
     unsigned long long hc[5] = {0, 0, 0, 0, 0};
     if (p.mode == 1) {
@@ -687,23 +768,16 @@ hipError_t hsolve(hipStream_t st, const HSolveParams &p, int max_rounds, int64_t
         *n_unconverged = 0;
         return hipGetLastError();
     }
-    const int64_t nblk = p.groups ? p.ngroups : (p.m + 63) / 64;
     for (int r = 0; r < max_rounds; r++) {
-        fill<unsigned long long>(st, 5 * SPH_SPREAD_WORDS, p.counters, 0ull);
-        k_hsolve<<<(unsigned)nblk, 64, 0, st>>>(a);
-        spread_total(st, 5, p.counters, p.totals);
+        hsolve_round(st, p, a);
         hipError_t e = hipMemcpyAsync(hc, p.totals, sizeof(hc), hipMemcpyDeviceToHost, st);
         if (e != hipSuccess) return e;
         e = hipStreamSynchronize(st);
         if (e != hipSuccess) return e;
-        if (p.work) {
-            p.work[0] += (int64_t)hc[3];
-            p.work[1] += (int64_t)hc[4];
-        }
-        if (hc[2]) return hipErrorIllegalState;  // walk stack overflow (cannot happen for valid trees)
-        if (p.unconverged_total) *p.unconverged_total += (int64_t)hc[1];
-        if (p.rounds) *p.rounds = r + 1;
-        if (hc[0] == 0) { *n_unconverged = 0; return hipGetLastError(); }
+        bool done = false;
+        e = hsolve_account(p, hc, r, &done);
+        if (e != hipSuccess) return e;
+        if (done) { *n_unconverged = 0; return hipGetLastError(); }
     }
     *n_unconverged = (int64_t)hc[0];
     return hipGetLastError();
