@@ -243,6 +243,59 @@ def cooling_section(args, cfg, run):
     return res
 
 
+def agn_section(args, cfg, run):
+    """AGN-outflow-like workload: a lognormal clumpy cloud (sigma_ln rho = 1.5, Gaussian
+    field on 256^3 cells) of agn_cloud particles at T = 1e4 K plus agn_wind hot (1e7 K)
+    wind particles leaving r = 0.05 R at v = 0.1; gravity, SPH and the gated-surrogate
+    cooling (table fallback at 0.05 dex); dt = 1e-5 (the wind shell's Courant limit)."""
+    from sph_3_amd import cooling as C
+    from sph_3_amd import ic
+
+    net = getattr(mlp_section, "net", None)
+    mode = "mlp" if net is not None else "table"
+    units = C.CoolingParams.from_mass_length(1e10 * C.M_SUN_G, 10 * C.KPC_CM, mode=mode, tol=0.05)
+    t0 = time.perf_counter()
+    d = ic.agn_cloud(args.agn_cloud, args.agn_wind, n_grid=256, u_cloud=ic.u_of_T(1e4, units.unit_u_cgs),
+                     u_wind=ic.u_of_T(1e7, units.unit_u_cgs))
+    n = len(d["m"])
+    Z, J = ic.cooling_fields(n, seed=49)
+    log(f"[bench] agn workload: {n} particles generated in {time.perf_counter() - t0:.1f} s")
+    dt_saved = cfg.dt
+    cfg.dt = 1e-5
+
+    def cooling(mode_):
+        def f(sim):
+            u = C.CoolingParams(**{**units.__dict__, "mode": mode_})
+            sim.set_cooling(Z, J, u, surrogate=net if mode_ == "mlp" else None)
+        return f
+
+    try:
+        sim, dt = run("mixed", args.steps, args.warmup, d=d, cooling=cooling(mode))
+        st = sim.stats()
+        ms = 1e3 * dt / args.steps
+        res = {
+            "workload": f"agn-like: {args.agn_cloud} cloud + {args.agn_wind} wind particles", "n": n,
+            "cooling_mode": mode, "dt": cfg.dt, "ms_per_step": ms, "value": n * args.steps / dt,
+            "unit": "particle-steps/s",
+            "phase_ms_per_step": {k: st["phase_ms"][k] / args.steps for k in st["phase_ms"] if st["phase_calls"][k]},
+            "neighbours_per_particle": st["ngb_total"] / n, "neighbour_layout": "padded rows or CSR fallback",
+            "gravity_interactions_per_particle": (st["grav_pp"] + st["grav_pn"]) / n,
+            "h_solve_rounds_last_step": st["h_rounds"],
+            "h_solve_walk_solved_targets_last_step": st["h_big"],
+            "cooling_fallback_fraction_last_step": st["cool_fallback"] / n,
+        }
+        del sim
+        if not args.no_fp64_form:
+            steps64 = max(2, args.steps // 2)
+            sim64, dt64 = run("fp64", steps64, 1, d=d, cooling=cooling("table"))
+            res["fp64_form"] = {"ms_per_step": 1e3 * dt64 / steps64, "mixed_speedup": (1e3 * dt64 / steps64) / ms,
+                                "cooling_mode": "table"}
+            del sim64
+    finally:
+        cfg.dt = dt_saved
+    return res
+
+
 def copy_probe(nbytes):
     """A plain device copy of known size (the calibration run of the HBM traffic counters)."""
     import torch
@@ -269,6 +322,9 @@ def main():
     ap.add_argument("--mlp-epochs", type=int, default=200, help="training passes of the surrogate")
     ap.add_argument("--no-cooling-step", action="store_true")
     ap.add_argument("--cooling-n", type=int, default=4194304)
+    ap.add_argument("--no-agn", action="store_true")
+    ap.add_argument("--agn-cloud", type=int, default=7340032)
+    ap.add_argument("--agn-wind", type=int, default=1048576)
     ap.add_argument("--copy-probe", type=int, default=0, help="only run a device copy of this many bytes")
     args = ap.parse_args()
     if args.copy_probe:
@@ -374,6 +430,7 @@ def main():
         "neighbours_per_particle": kbar,
         "neighbour_pairs_from_other_side": st["ngb_extra"],
         "h_solve_rounds_last_step": st["h_rounds"],
+        "h_solve_walk_solved_targets_last_step": st["h_big"],
         "gravity_interactions_per_particle": (st["grav_pp"] + st["grav_pn"]) / n,
         "gravity_tflops": grav_tflops,
         "pass_bandwidth": hbm,
@@ -407,6 +464,8 @@ def main():
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
     if not args.no_cooling_step:
         out["evrard_cooling"] = cooling_section(args, cfg, run)
+    if not args.no_agn:
+        out["agn_outflow"] = agn_section(args, cfg, run)
     print(json.dumps(out), flush=True)
 
 

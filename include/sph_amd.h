@@ -253,6 +253,8 @@ typedef struct sph_stats {
     int64_t ngb_extra;       /* last step: neighbour pairs found only from the other particle's candidates */
     int64_t cool_floor;      /* last step: particles the cooling update set to u_floor */
     int64_t cool_fallback;   /* last step: table fallbacks of the surrogate (SPH_COOL_MLP) */
+    int64_t h_big;           /* last h solve: targets whose solution needs more candidates than a
+                                row holds (solved by a candidate-free tree-walk Newton) */
 } sph_stats;
 
 /* phase indices of sph_stats.phase_ms (SPH_PH_HMOMENTS: the node boxes built for the h walk;

@@ -43,7 +43,8 @@ class Stats(ctypes.Structure):
                 ("grav_pp", ctypes.c_int64), ("grav_pn", ctypes.c_int64), ("h_unconverged", ctypes.c_int64),
                 ("h_rounds", ctypes.c_int64), ("phase_ms", ctypes.c_double * 16),
                 ("phase_calls", ctypes.c_int64 * 16), ("h_tests", ctypes.c_int64), ("h_terms", ctypes.c_int64),
-                ("ngb_extra", ctypes.c_int64), ("cool_floor", ctypes.c_int64), ("cool_fallback", ctypes.c_int64)]
+                ("ngb_extra", ctypes.c_int64), ("cool_floor", ctypes.c_int64), ("cool_fallback", ctypes.c_int64),
+                ("h_big", ctypes.c_int64)]
 
 
 class CoolParams(ctypes.Structure):

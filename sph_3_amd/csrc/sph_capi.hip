@@ -85,11 +85,13 @@ struct sph_ctx {
     // h solve scratch
     Buf cand, hR, hsteps, hstate, hcount, hout, hin, tx, ty, tz;
     Buf ccidx, cccnt, cext, cnext;  // candidate indices, counts, extra pairs and their region counters
+    Buf hbig;                       // targets of the candidate-free h solver
     // staging and reductions
     Buf stage, kpart, kout;
     // spread event counters (sph_common.h): SPR_H.. h solve (5), SPR_G.. gravity (3);
     // their totals land in hcount: [0..4] h solve, [5..7] gravity, [8] walk overflow, [9] group count,
-    // [10..11] padded neighbour rows, [12] cooling floor, [13] surrogate fallbacks
+    // [10..11] padded neighbour rows, [12] cooling floor, [13] surrogate fallbacks, [14] targets of
+    // the candidate-free h solver
     Buf spread;
     // cooling: field integrals, table, elementwise scratch
     Buf rf1, ctabL, ctabG, cin[6], cout[2];
@@ -469,7 +471,7 @@ extern "C" int sph_ctx_destroy(sph_ctx *ctx)
                   &ctx->hstate, &ctx->hcount, &ctx->hout, &ctx->hin, &ctx->tx, &ctx->ty, &ctx->tz,
                   &ctx->stage, &ctx->kpart, &ctx->kout, &ctx->spread, &ctx->ccidx, &ctx->cccnt,
                   &ctx->cext, &ctx->cnext, &ctx->ttab, &ctx->zmet, &ctx->j21, &ctx->ucols, &ctx->uyn,
-                  &ctx->uL, &ctx->uG, &ctx->uflag, &ctx->ucnt};
+                  &ctx->uL, &ctx->uG, &ctx->uflag, &ctx->ucnt, &ctx->hbig};
     for (Buf *b : all)
         if (b->p) (void)hipFree(b->p);
     Buf *cool[] = {&ctx->rf1,   &ctx->ctabL,  &ctx->ctabG,  &ctx->cin[0], &ctx->cin[1], &ctx->cin[2], &ctx->cin[3],
@@ -537,6 +539,12 @@ int hsolve_params(sph_ctx *ctx, int64_t m, const double *tx, const double *ty, c
     p.ccnt = keep_indices ? I32(cccnt) : nullptr;
     p.host_totals = ctx->hpin;
     p.done = ctx->ev_h;
+    if (mode == SPH_H_NEWTON) {
+        CK(ensure(ctx, ctx->hbig, (size_t)m * 4));
+        p.big_list = I32(hbig);
+        p.big_count = P_<unsigned long long>(ctx->hcount) + 14;
+        p.n_big = &ctx->stats.h_big;
+    }
     return SPH_OK;
 }
 

@@ -37,6 +37,12 @@ struct HSolveParams {
     // and an event recorded after their copy
     unsigned long long *host_totals = nullptr;
     hipEvent_t done = nullptr;
+    // mode 0 (may be null): m int32 and one device word for the targets whose solution needs
+    // more candidates than a row holds; they are solved by a candidate-free walk solver after
+    // the rounds and their candidate count is set to cap + 1 (incomplete row)
+    int32_t *big_list = nullptr;
+    unsigned long long *big_count = nullptr;
+    int64_t *n_big = nullptr;  // host (may be null): targets given to that solver
 };
 
 // Runs rounds until every target is decided; *n_unconverged = targets still
@@ -44,7 +50,7 @@ struct HSolveParams {
 // hipErrorIllegalState if a walk stack overflowed.
 hipError_t hsolve(hipStream_t st, const HSolveParams &p, int max_rounds, int64_t *n_unconverged);
 // The same in two halves (Newton, count-free modes 0 and 2): hsolve_begin enqueues the first
-// round and the asynchronous copy of its totals to p.host_totals (pinned) followed by
+// round and the asynchronous copy of its totals to p.host_totals (pinned, 6 words) followed by
 // p.done, without waiting; the caller may enqueue independent work; hsolve_end waits for
 // p.done (not the stream) and runs further rounds synchronously if targets are pending
 // (*more_rounds = true: the results of anything enqueued from the first round's output

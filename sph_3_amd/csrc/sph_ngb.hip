@@ -96,6 +96,8 @@ struct HSolveArgs {
                                   // 3 distance tests, 4 Newton kernel terms
     const int32_t *groups;     // tree-node groups of the targets (null: blocks of 64 targets)
     const WNode *gnodes;       // node records the group ids refer to
+    int32_t *big_list;         // mode 0 (may be null): targets whose solution needs more
+    unsigned long long *big_count;  //   candidates than a row holds (state 3), for k_hsolve_big
 };
 
 // Group range: tree node groups[blk] or the block [64 blk, 64 blk + 64) of targets.
@@ -344,11 +346,103 @@ __global__ void __launch_bounds__(64) k_hsolve(HSolveArgs a)
         spread_add(a.pending, 4, (unsigned long long)nterm);
     }
     if (!valid) return;
+    if (a.mode == 0 && a.big_list) {
+        // a target that must grow R after its candidates overflowed a row needs more
+        // candidates than a row holds at its solution: k_hsolve_big solves it
+        const int fl = a.steps[t];
+        if (over) {
+            a.steps[t] = fl | 1;
+        } else if (st_res == 0 && (fl & 1)) {
+            st_res = 3;
+            a.big_list[atomicAdd(a.big_count, 1ull)] = (int32_t)t;
+        }
+    }
     a.h[t] = h_res;
     a.R[t] = R_res;
     a.state[t] = st_res;
     if (st_res == 0) spread_add(a.pending, 0, 1ull);
     if (st_res == 2) spread_add(a.pending, 1, 1ull);
+}
+
+// Newton solve of targets whose solution needs more candidates than a row holds (a
+// target beside a much denser structure): one wavefront per target; each evaluation
+// of f(h) and f'(h) walks the tree over all sources within 2h (terms beyond 2h are
+// zero), the 64 lanes splitting the staged sources, FP64 sums reduced by butterflies
+// (uniform in the wave).  f is non-decreasing in h with f(0+) < 0, so Newton is
+// safeguarded by the bracket [lo, hi] (hi doubles until f(hi) >= 0); the stopping
+// rule is the row solver's.  The target's candidate count is set to cap + 1: its row
+// of candidates is incomplete (neighbour rows built from candidates must not use it).
+__global__ void __launch_bounds__(64) k_hsolve_big(HSolveArgs a, const int32_t *__restrict__ list)
+{
+    __shared__ WalkLDS L;
+    __shared__ double sx[WSRC], sy[WSRC], sz[WSRC];
+    const int64_t t = list[blockIdx.x];
+    const int lane = threadIdx.x;
+    const double xi = a.tx[t], yi = a.ty[t], zi = a.tz[t];
+    const double gb[6] = {xi, yi, zi, xi, yi, zi};
+    long long ntest = 0, nterm = 0;
+    auto feval = [&](double hh, double &f, double &df) {
+        const double lim = 2.0 * hh, L2 = lim * lim, ih = 1.0 / hh;
+        double sl = 0.0, sq = 0.0;
+        (void)group_walk(
+            a.nodes, a.bucket, L, [&](const WNode *nd) { return box_box_dist2(gb, nd) >= L2; },
+            [&](int q, int j) {
+                sx[q] = a.x[j];
+                sy[q] = a.y[j];
+                sz[q] = a.z[j];
+            },
+            [&](int cnt) {
+                if (lane == 0) ntest += cnt;
+                for (int k = lane; k < cnt; k += 64) {
+                    const double dx = sx[k] - xi, dy = sy[k] - yi, dz = sz[k] - zi;
+                    const double r2 = dx * dx + dy * dy + dz * dz;
+                    if (r2 < L2) {
+                        nterm++;
+                        const double q = __builtin_sqrt(r2) * ih;
+                        const double aa = fmax(2.0 - q, 0.0), bb = fmax(1.0 - q, 0.0);
+                        const double a2 = aa * aa, b2 = bb * bb;
+                        sl = fma(0.25 * a2, aa, fma(-b2, bb, sl));
+                        sq = fma(fma(3.0, b2, -0.75 * a2), q, sq);
+                    }
+                }
+            });
+        for (int off = 32; off >= 1; off >>= 1) {
+            sl += __shfl_xor(sl, off);
+            sq += __shfl_xor(sq, off);
+        }
+        f = (32.0 / 3.0) * sl - a.n_ngb;
+        df = -(32.0 / 3.0) * sq * ih;
+    };
+    const double qtol = __builtin_sqrt(0.1 * a.tol);
+    double lo = 0.0, hi = INFINITY, hh = a.h[t], f, df;
+    if (!(hh > 0.0)) hh = 0.5 * a.R[t];
+    int st = 2;
+    for (int it = 0; it < 200; it++) {
+        feval(hh, f, df);
+        if (f < 0.0) lo = hh;
+        else hi = hh;
+        double hn = (df > 0.0) ? hh - f / df : -1.0;
+        bool newton = true;
+        if (!(hn > lo && hn < hi)) {
+            hn = (hi < INFINITY) ? 0.5 * (lo + hi) : 2.0 * hh;
+            newton = false;
+        }
+        const double dh = fabs(hn - hh);
+        hh = hn;
+        if (dh <= a.tol * hh || (newton && dh <= qtol * hh)) {
+            st = 1;
+            break;
+        }
+    }
+    for (int off = 32; off >= 1; off >>= 1) nterm += __shfl_xor(nterm, off);
+    if (lane == 0) {
+        spread_add(a.pending, 3, (unsigned long long)ntest);
+        spread_add(a.pending, 4, (unsigned long long)nterm);
+        a.h[t] = hh;
+        a.R[t] = 2.0 * hh;
+        a.state[t] = st;
+        if (a.ccnt) a.ccnt[t] = a.cap + 1;
+    }
 }
 
 // mode 1: the reference's count stepping, exactly (libsx.py:804-845): each
@@ -507,7 +601,9 @@ __global__ void __launch_bounds__(256) k_cand_rows(int64_t n, int cap, const dou
     const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
     const uint64_t gmask = (G == 64 ? ~0ull : ((1ull << G) - 1)) << (lane & ~(G - 1));
     const double hi = valid ? h[i] : 0.0;
-    const int c = valid ? ccnt[i] : 0;
+    const int craw = valid ? ccnt[i] : 0;
+    const bool incomplete = craw > cap;  // solved without a complete candidate row (k_hsolve_big)
+    const int c = incomplete ? 0 : craw;
     const double *rr = cand + (valid ? i : 0) * (int64_t)cap;
     const int32_t *rj = cidx + (valid ? i : 0) * (int64_t)cap;
     int32_t *row = idx + (valid ? i : 0) * (int64_t)stride;
@@ -541,7 +637,7 @@ __global__ void __launch_bounds__(256) k_cand_rows(int64_t n, int cap, const dou
         cnt[i] = w;
         cnt0[i] = w;
     }
-    unsigned long long nlong = (valid && l == 0 && w > stride) ? 1ull : 0ull;
+    unsigned long long nlong = (valid && l == 0 && (w > stride || incomplete)) ? 1ull : 0ull;
     unsigned long long nent = (valid && l == 0) ? (unsigned long long)w : 0ull;
     for (int off = 32; off >= 1; off >>= 1) {
         nlong += __shfl_xor(nlong, off);
@@ -680,7 +776,18 @@ HSolveArgs hsolve_args(const HSolveParams &p)
     a.pending = p.counters;
     a.groups = p.groups;
     a.gnodes = p.nodes;
+    a.big_list = p.mode == 0 ? p.big_list : nullptr;
+    a.big_count = p.big_count;
     return a;
+}
+
+// The targets k_hsolve left to the candidate-free solver (count read back by the caller).
+hipError_t hsolve_big(hipStream_t st, const HSolveParams &p, const HSolveArgs &a, unsigned long long nbig)
+{
+    if (p.n_big) *p.n_big = (int64_t)nbig;
+    if (nbig == 0 || !a.big_list) return hipSuccess;
+    k_hsolve_big<<<(unsigned)nbig, 64, 0, st>>>(a, a.big_list);
+    return hipGetLastError();
 }
 
 // One Newton / k-th round: counters cleared, the solve, the fold into p.totals.
@@ -714,9 +821,14 @@ hipError_t hsolve_begin(hipStream_t st, const HSolveParams &p)
     if (p.bucket > 64 || p.cap > SPH_HCAP_SLOTS || p.cap < 1) return hipErrorInvalidValue;
     const HSolveArgs a = hsolve_args(p);
     k_hsolve_init<<<sph_blocks(p.m, 256), 256, 0, st>>>(p.m, p.h_in, p.r_fac, p.R, p.h_out, p.steps, p.state);
+    if (a.big_list) fill<unsigned long long>(st, 1, p.big_count, 0ull);
     hsolve_round(st, p, a);
     hipError_t e = hipMemcpyAsync(p.host_totals, p.totals, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
     if (e != hipSuccess) return e;
+    if (a.big_list) {
+        e = hipMemcpyAsync(p.host_totals + 5, p.big_count, sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
+        if (e != hipSuccess) return e;
+    }
     e = hipEventRecord(p.done, st);
     if (e != hipSuccess) return e;
     return hipGetLastError();
@@ -731,19 +843,31 @@ hipError_t hsolve_end(hipStream_t st, const HSolveParams &p, int max_rounds, int
     bool done = false;
     e = hsolve_account(p, p.host_totals, 0, &done);
     if (e != hipSuccess) return e;
-    if (done) { *n_unconverged = 0; return hipSuccess; }
-    *more_rounds = true;
     const HSolveArgs a = hsolve_args(p);
-    unsigned long long hc[5] = {p.host_totals[0], 0, 0, 0, 0};
+    if (done) {
+        *n_unconverged = 0;
+        return hsolve_big(st, p, a, a.big_list ? p.host_totals[5] : 0);
+    }
+    *more_rounds = true;
+    unsigned long long hc[6] = {p.host_totals[0], 0, 0, 0, 0, 0};
     for (int r = 1; r < max_rounds; r++) {
         hsolve_round(st, p, a);
-        e = hipMemcpyAsync(hc, p.totals, sizeof(hc), hipMemcpyDeviceToHost, st);
+        e = hipMemcpyAsync(hc, p.totals, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
         if (e != hipSuccess) return e;
+        if (a.big_list) {
+            e = hipMemcpyAsync(hc + 5, p.big_count, sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
+            if (e != hipSuccess) return e;
+        }
         e = hipStreamSynchronize(st);
         if (e != hipSuccess) return e;
         e = hsolve_account(p, hc, r, &done);
         if (e != hipSuccess) return e;
-        if (done) { *n_unconverged = 0; return hipGetLastError(); }
+        if (done) {
+            *n_unconverged = 0;
+            e = hsolve_big(st, p, a, hc[5]);
+            if (e != hipSuccess) return e;
+            return hipGetLastError();
+        }
     }
     *n_unconverged = (int64_t)hc[0];
     return hipGetLastError();
@@ -755,7 +879,8 @@ hipError_t hsolve(hipStream_t st, const HSolveParams &p, int max_rounds, int64_t
     if (p.bucket > 64 || p.cap > SPH_HCAP_SLOTS || p.cap < 1) return hipErrorInvalidValue;
     const HSolveArgs a = hsolve_args(p);
     k_hsolve_init<<<sph_blocks(p.m, 256), 256, 0, st>>>(p.m, p.h_in, p.r_fac, p.R, p.h_out, p.steps, p.state);
-    unsigned long long hc[5] = {0, 0, 0, 0, 0};
+    if (a.big_list) fill<unsigned long long>(st, 1, p.big_count, 0ull);
+    unsigned long long hc[6] = {0, 0, 0, 0, 0, 0};
     if (p.mode == 1) {
         fill<unsigned long long>(st, 5 * SPH_SPREAD_WORDS, p.counters, 0ull);
         k_hstep<<<sph_blocks(p.m, 256), 256, 0, st>>>(a);
@@ -770,14 +895,23 @@ hipError_t hsolve(hipStream_t st, const HSolveParams &p, int max_rounds, int64_t
     }
     for (int r = 0; r < max_rounds; r++) {
         hsolve_round(st, p, a);
-        hipError_t e = hipMemcpyAsync(hc, p.totals, sizeof(hc), hipMemcpyDeviceToHost, st);
+        hipError_t e = hipMemcpyAsync(hc, p.totals, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
         if (e != hipSuccess) return e;
+        if (a.big_list) {
+            e = hipMemcpyAsync(hc + 5, p.big_count, sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
+            if (e != hipSuccess) return e;
+        }
         e = hipStreamSynchronize(st);
         if (e != hipSuccess) return e;
         bool done = false;
         e = hsolve_account(p, hc, r, &done);
         if (e != hipSuccess) return e;
-        if (done) { *n_unconverged = 0; return hipGetLastError(); }
+        if (done) {
+            *n_unconverged = 0;
+            e = hsolve_big(st, p, a, hc[5]);
+            if (e != hipSuccess) return e;
+            return hipGetLastError();
+        }
     }
     *n_unconverged = (int64_t)hc[0];
     return hipGetLastError();

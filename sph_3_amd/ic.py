@@ -47,3 +47,74 @@ def cooling_fields(n: int, seed: int = 43):
     Z = 10.0 ** rng.uniform(-3.0, 0.0, n)
     J21 = 10.0 ** rng.uniform(-2.0, 2.0, n)
     return Z, J21
+
+
+def gaussian_field(n_grid: int, slope: float = -11.0 / 3.0, seed: int = 0):
+    """Seeded Gaussian random field on an n_grid^3 periodic grid with an isotropic
+    power-law spectrum P(k) ~ k^slope (the k = 0 mode removed), normalised to zero
+    mean and unit variance; float32."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    white = rng.standard_normal((n_grid,) * 3).astype(np.float32)
+    f = np.fft.rfftn(white)
+    k1 = np.fft.fftfreq(n_grid) * n_grid
+    kz = np.fft.rfftfreq(n_grid) * n_grid
+    k2 = (k1[:, None, None] ** 2 + k1[None, :, None] ** 2 + kz[None, None, :] ** 2).astype(np.float32)
+    k2[0, 0, 0] = 1.0
+    f *= k2 ** np.float32(slope / 4.0)  # amplitude ~ sqrt(P) = k^(slope/2)
+    f[0, 0, 0] = 0.0
+    g = np.fft.irfftn(f, s=(n_grid,) * 3).astype(np.float32)
+    g -= g.mean()
+    g /= g.std()
+    return g
+
+
+def agn_cloud(n_cloud: int, n_wind: int, n_grid: int = 256, sigma_lnrho: float = 1.5, R: float = 1.0,
+              M: float = 1.0, r_wind: float = 0.05, v_wind: float = 0.1, u_cloud: float = 0.0,
+              u_wind: float = 0.0, slope: float = -11.0 / 3.0, seed: int = 46, n_ngb: float = 64.0):
+    """AGN-outflow-like initial conditions: a spherical cloud of radius R and mass M
+    whose density is lognormal, rho ~ exp(sigma g - sigma^2/2) with g the Gaussian
+    field of ``gaussian_field`` on n_grid^3 cells over [-R, R]^3 (particles are
+    drawn cell by cell with probability ~ rho and placed uniformly in the cell), at
+    rest with specific energy u_cloud; plus n_wind equal-mass particles in the
+    shell 0.9 r_wind < r < r_wind (uniform in volume, isotropic) moving radially
+    outward at v_wind with u_wind.  Returns dict(pos, vel, m, u, h0, wind) with h0
+    from the cell density at the target neighbour number (a starting guess)."""
+    rng = np.random.Generator(np.random.PCG64(seed + 1))
+    g = gaussian_field(n_grid, slope, seed)
+    a = 2.0 * R / n_grid
+    c = (np.arange(n_grid, dtype=np.float32) + 0.5) * a - R
+    r2 = c[:, None, None] ** 2 + c[None, :, None] ** 2 + c[None, None, :] ** 2
+    w = np.exp(np.float32(sigma_lnrho) * g - np.float32(0.5 * sigma_lnrho ** 2))
+    w[r2 >= R * R] = 0.0
+    w = w.ravel().astype(np.float64)
+    cdf = np.cumsum(w)
+    cdf /= cdf[-1]
+    cell = np.searchsorted(cdf, rng.random(n_cloud), side="right")
+    cell = np.minimum(cell, len(w) - 1)
+    ix, iy, iz = np.unravel_index(cell, (n_grid,) * 3)
+    pos_c = np.stack([c[ix], c[iy], c[iz]], axis=1).astype(np.float64) + rng.uniform(-0.5, 0.5, (n_cloud, 3)) * a
+    m_p = M / n_cloud
+    rho_cell = M * w[cell] / (w.sum() * a ** 3)  # mass density of the particle's cell
+    # wind shell
+    rr = r_wind * np.cbrt(rng.uniform(0.9 ** 3, 1.0, n_wind))
+    ct = rng.uniform(-1.0, 1.0, n_wind)
+    ph = rng.uniform(0.0, 2.0 * np.pi, n_wind)
+    st = np.sqrt(1.0 - ct * ct)
+    nhat = np.stack([st * np.cos(ph), st * np.sin(ph), ct], axis=1)
+    pos_w = rr[:, None] * nhat
+    rho_w = n_wind * m_p / (4.0 / 3.0 * np.pi * r_wind ** 3 * (1.0 - 0.9 ** 3))
+    pos = np.concatenate([pos_c, pos_w])
+    vel = np.concatenate([np.zeros((n_cloud, 3)), v_wind * nhat])
+    u = np.concatenate([np.full(n_cloud, u_cloud), np.full(n_wind, u_wind)])
+    rho0 = np.concatenate([rho_cell, np.full(n_wind, rho_w)])
+    n = n_cloud + n_wind
+    m = np.full(n, m_p)
+    h0 = 0.5 * np.cbrt(3.0 * n_ngb * m_p / (4.0 * np.pi * rho0))
+    wind = np.zeros(n, bool)
+    wind[n_cloud:] = True
+    return dict(pos=pos, vel=vel, m=m, u=u, h0=h0, wind=wind)
+
+
+def u_of_T(T: float, unit_u_cgs: float, mu: float = 0.6, gamma: float = 5.0 / 3.0) -> float:
+    """Specific energy in code units of gas at temperature T [K] with mean molecular weight mu."""
+    return 1.3807e-16 * T / ((gamma - 1.0) * mu * 1.6726e-24) / unit_u_cgs

@@ -119,7 +119,7 @@ class DeviceSPH:
         c.check(c.lib.sph_get_stats(c.h, ctypes.byref(s)), "sph_get_stats")
         return dict(n=s.n, steps=s.steps, ngb_total=s.ngb_total, grav_pp=s.grav_pp, grav_pn=s.grav_pn,
                     h_unconverged=s.h_unconverged, h_rounds=s.h_rounds, h_tests=s.h_tests, h_terms=s.h_terms, ngb_extra=s.ngb_extra,
-                    cool_floor=s.cool_floor, cool_fallback=s.cool_fallback,
+                    cool_floor=s.cool_floor, cool_fallback=s.cool_fallback, h_big=s.h_big,
                     phase_ms={k: s.phase_ms[i] for i, k in enumerate(PHASES)},
                     phase_calls={k: s.phase_calls[i] for i, k in enumerate(PHASES)})
 
