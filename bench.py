@@ -296,6 +296,41 @@ def agn_section(args, cfg, run):
     return res
 
 
+def sedov_section(args, cfg, run):
+    """Periodic 3D Sedov blast: sedov_side^3 particles on a jittered lattice (+-0.1
+    spacing) in the unit periodic box, rho = 1, u = 1e-5, E = 1 deposited
+    kernel-weighted on the 64 particles nearest the centre, no gravity; minimum-image
+    neighbour search and hydro, dt = 5e-6 (the blast's Courant limit)."""
+    from sph_3_amd import ic
+
+    t0 = time.perf_counter()
+    d = ic.sedov(args.sedov_side)
+    n = len(d["m"])
+    log(f"[bench] sedov: {n} particles generated in {time.perf_counter() - t0:.1f} s")
+    saved = (cfg.dt, cfg.G, cfg.period)
+    cfg.dt, cfg.G, cfg.period = 5e-6, 0.0, 1.0
+    try:
+        sim, dt = run("mixed", args.steps, args.warmup, d=d)
+        st = sim.stats()
+        ms = 1e3 * dt / args.steps
+        res = {
+            "workload": f"sedov blast, periodic unit box, {args.sedov_side}^3 = {n} particles", "n": n,
+            "dt": cfg.dt, "ms_per_step": ms, "value": n * args.steps / dt, "unit": "particle-steps/s",
+            "phase_ms_per_step": {k: st["phase_ms"][k] / args.steps for k in st["phase_ms"] if st["phase_calls"][k]},
+            "neighbours_per_particle": st["ngb_total"] / n, "h_solve_rounds_last_step": st["h_rounds"],
+            "h_solve_walk_solved_targets_last_step": st["h_big"],
+        }
+        del sim
+        if not args.no_fp64_form:
+            steps64 = max(2, args.steps // 2)
+            sim64, dt64 = run("fp64", steps64, 1, d=d)
+            res["fp64_form"] = {"ms_per_step": 1e3 * dt64 / steps64, "mixed_speedup": (1e3 * dt64 / steps64) / ms}
+            del sim64
+    finally:
+        cfg.dt, cfg.G, cfg.period = saved
+    return res
+
+
 def copy_probe(nbytes):
     """A plain device copy of known size (the calibration run of the HBM traffic counters)."""
     import torch
@@ -323,6 +358,8 @@ def main():
     ap.add_argument("--no-cooling-step", action="store_true")
     ap.add_argument("--cooling-n", type=int, default=4194304)
     ap.add_argument("--no-agn", action="store_true")
+    ap.add_argument("--no-sedov", action="store_true")
+    ap.add_argument("--sedov-side", type=int, default=128)
     ap.add_argument("--agn-cloud", type=int, default=7340032)
     ap.add_argument("--agn-wind", type=int, default=1048576)
     ap.add_argument("--copy-probe", type=int, default=0, help="only run a device copy of this many bytes")
@@ -466,6 +503,8 @@ def main():
         out["evrard_cooling"] = cooling_section(args, cfg, run)
     if not args.no_agn:
         out["agn_outflow"] = agn_section(args, cfg, run)
+    if not args.no_sedov:
+        out["sedov_blast"] = sedov_section(args, cfg, run)
     print(json.dumps(out), flush=True)
 
 

@@ -234,6 +234,8 @@ typedef struct sph_step_params {
     double eps_plummer; /* Plummer softening (soft_kind = SPH_SOFT_PLUMMER) */
     double n_ngb;       /* Newton h solve target */
     double h_tol;       /* Newton relative tolerance */
+    double period;      /* > 0: side of a periodic box [0, period)^3 with minimum-image distances
+                           (positions wrapped by the drift; needs G = 0); 0: open boundaries */
     int soft_kind;      /* SPH_SOFT_SPLINE uses eps_i = h_i (EV/lib.h:584) */
     int precision;      /* SPH_PREC_MIXED / SPH_PREC_FP64 */
     int balsara;        /* 1: shear.py forms, 0: libsx.py forms */

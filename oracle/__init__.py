@@ -58,6 +58,7 @@ def lib():
         L.or_cool_u_to_T.argtypes = [ctypes.c_double] * 4
         L.or_cool_rate_from_u.argtypes = [ctypes.c_double] * 5
         L.or_do_cooling.argtypes = [ctypes.c_double] * 7
+        L.or_set_period.argtypes = [ctypes.c_double]
         _LIB = L
     return _LIB
 
@@ -386,3 +387,20 @@ def cool_update(rho, u, Z, J21, dt, units, lo, hi):
     lib().or_cool_update_n(_i64(n), _p(rho), _p(u), _p(Z), _p(J21), ctypes.c_double(dt), _p(_f64(units)),
                            _p(_f64(lo)), _p(_f64(hi)), _p(out), _p(fl))
     return out, fl.astype(bool)
+
+
+class periodic:
+    """Context manager: minimum-image distances in a periodic box of side L for
+    ``neighbours_brute``, ``h_newton`` and the ``*_csr`` functions (open
+    boundaries again on exit)."""
+
+    def __init__(self, L):
+        self.L = float(L)
+
+    def __enter__(self):
+        lib().or_set_period(self.L)
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_set_period(0.0)
+        return False

@@ -54,6 +54,23 @@ static inline double kernel_nW(double rr, double hij)
  * -------------------------------------------------------------------------*/
 
 /* getDensity / getDensity_mpi, libsx.py:10-44, 50-86 */
+/* Minimum-image convention of a periodic box of side g_period (0: open
+ * boundaries, the default).  Only the neighbour-set, smoothing-length and CSR
+ * functions below use it (the periodic Sedov box); the all-pairs restatements of
+ * libsx.py and the gravity functions are open-boundary as in the reference. */
+static double g_period = 0.0;
+
+void or_set_period(double L) { g_period = L > 0.0 ? L : 0.0; }
+
+static inline double wrapd(double d)
+{
+    if (g_period > 0.0) {
+        if (d > 0.5 * g_period) d = d - g_period;
+        else if (d < -0.5 * g_period) d = d + g_period;
+    }
+    return d;
+}
+
 void or_density_allpairs(int64_t N, int64_t nbeg, int64_t nend, const double *pos,
                          const double *m, const double *h, double *rho)
 {
@@ -355,8 +372,8 @@ void or_neighbours_brute(int64_t N, const double *pos, const double *h, int64_t 
         for (int64_t i = 0; i < N; i++) {
             int64_t c = 0;
             for (int64_t j = 0; j < N; j++) {
-                double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
-                       dz = pos[3 * i + 2] - pos[3 * j + 2];
+                double dx = wrapd(pos[3 * i] - pos[3 * j]), dy = wrapd(pos[3 * i + 1] - pos[3 * j + 1]),
+                       dz = wrapd(pos[3 * i + 2] - pos[3 * j + 2]);
                 double r2 = dx * dx + dy * dy + dz * dz, t = h[i] + h[j];
                 c += r2 < t * t;
             }
@@ -369,8 +386,8 @@ void or_neighbours_brute(int64_t N, const double *pos, const double *h, int64_t 
     for (int64_t i = 0; i < N; i++) {
         int64_t k = offsets[i];
         for (int64_t j = 0; j < N; j++) {
-            double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
-                   dz = pos[3 * i + 2] - pos[3 * j + 2];
+            double dx = wrapd(pos[3 * i] - pos[3 * j]), dy = wrapd(pos[3 * i + 1] - pos[3 * j + 1]),
+                   dz = wrapd(pos[3 * i + 2] - pos[3 * j + 2]);
             double r2 = dx * dx + dy * dy + dz * dz, t = h[i] + h[j];
             if (r2 < t * t) idx[k++] = (int32_t)j;
         }
@@ -819,8 +836,8 @@ int64_t or_h_newton(int64_t N, const double *pos, const double *h_in, double n_n
 #pragma omp for schedule(dynamic, 8)
         for (int64_t i = 0; i < N; i++) {
             for (int64_t j = 0; j < N; j++) {
-                double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
-                       dz = pos[3 * i + 2] - pos[3 * j + 2];
+                double dx = wrapd(pos[3 * i] - pos[3 * j]), dy = wrapd(pos[3 * i + 1] - pos[3 * j + 1]),
+                       dz = wrapd(pos[3 * i + 2] - pos[3 * j + 2]);
                 r2[j] = dx * dx + dy * dy + dz * dz;
             }
             double h = h_in[i], lo = 0.0, hi = INFINITY, f, df;
@@ -853,8 +870,8 @@ void or_density_csr(int64_t N, const double *pos, const double *m, const double 
         double s = 0.0;
         for (int64_t k = off[i]; k < off[i + 1]; k++) {
             int64_t j = idx[k];
-            double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
-                   dz = pos[3 * i + 2] - pos[3 * j + 2];
+            double dx = wrapd(pos[3 * i] - pos[3 * j]), dy = wrapd(pos[3 * i + 1] - pos[3 * j + 1]),
+                   dz = wrapd(pos[3 * i + 2] - pos[3 * j + 2]);
             double rr = sqrt(dx * dx + dy * dy + dz * dz);
             s += m[j] * kernel_W(rr, 0.5 * (h[i] + h[j]));
         }
@@ -871,8 +888,8 @@ void or_divcurl_csr(int64_t N, const double *pos, const double *v, const double 
         double s = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
         for (int64_t k = off[i]; k < off[i + 1]; k++) {
             int64_t j = idx[k];
-            double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
-                   dz = pos[3 * i + 2] - pos[3 * j + 2];
+            double dx = wrapd(pos[3 * i] - pos[3 * j]), dy = wrapd(pos[3 * i + 1] - pos[3 * j + 1]),
+                   dz = wrapd(pos[3 * i + 2] - pos[3 * j + 2]);
             double rr = sqrt(dx * dx + dy * dy + dz * dz);
             double nW = kernel_nW(rr, 0.5 * (h[i] + h[j]));
             double gx = nW * dx, gy = nW * dy, gz = nW * dz;
@@ -900,8 +917,8 @@ void or_accsph_csr(int64_t N, const double *pos, const double *v, const double *
         double fi = balsara ? balsara_f(divV[i], curlV[i], c[i], h[i]) : 1.0;
         for (int64_t k = off[i]; k < off[i + 1]; k++) {
             int64_t j = idx[k];
-            double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
-                   dz = pos[3 * i + 2] - pos[3 * j + 2];
+            double dx = wrapd(pos[3 * i] - pos[3 * j]), dy = wrapd(pos[3 * i + 1] - pos[3 * j + 1]),
+                   dz = wrapd(pos[3 * i + 2] - pos[3 * j + 2]);
             double rr = sqrt(dx * dx + dy * dy + dz * dz);
             double nW = kernel_nW(rr, 0.5 * (h[i] + h[j]));
             double gx = nW * dx, gy = nW * dy, gz = nW * dz;
@@ -932,8 +949,8 @@ void or_dudt_csr(int64_t N, const double *pos, const double *v, const double *rh
         double fi = balsara ? balsara_f(divV[i], curlV[i], c[i], h[i]) : 1.0;
         for (int64_t k = off[i]; k < off[i + 1]; k++) {
             int64_t j = idx[k];
-            double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
-                   dz = pos[3 * i + 2] - pos[3 * j + 2];
+            double dx = wrapd(pos[3 * i] - pos[3 * j]), dy = wrapd(pos[3 * i + 1] - pos[3 * j + 1]),
+                   dz = wrapd(pos[3 * i + 2] - pos[3 * j + 2]);
             double rr = sqrt(dx * dx + dy * dy + dz * dz);
             double nW = kernel_nW(rr, 0.5 * (h[i] + h[j]));
             double gx = nW * dx, gy = nW * dy, gz = nW * dz;

@@ -34,7 +34,8 @@ _vp = ctypes.c_void_p
 class StepParams(ctypes.Structure):
     _fields_ = [("dt", ctypes.c_double), ("gamma", ctypes.c_double), ("alpha", ctypes.c_double),
                 ("G", ctypes.c_double), ("theta", ctypes.c_double), ("eps_plummer", ctypes.c_double),
-                ("n_ngb", ctypes.c_double), ("h_tol", ctypes.c_double), ("soft_kind", ctypes.c_int),
+                ("n_ngb", ctypes.c_double), ("h_tol", ctypes.c_double), ("period", ctypes.c_double),
+                ("soft_kind", ctypes.c_int),
                 ("precision", ctypes.c_int), ("balsara", ctypes.c_int), ("timing", ctypes.c_int)]
 
 

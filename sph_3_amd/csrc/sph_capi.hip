@@ -118,6 +118,7 @@ struct sph_ctx {
     bool resident = false, primed = false;
     int64_t csr_total = 0;
     int ngb_stride = 0;  // layout of the neighbour sets: 0 CSR, else padded rows of this width
+    double ngb_per = 0.0;  // side of the periodic box the sets were built in (0: open boundaries)
     int64_t ngroups = 0;  // tree-node target groups of the current tree
 
     // statistics
@@ -338,9 +339,10 @@ int ensure_moments(sph_ctx *ctx)
 // Neighbour sets of the current positions and h: padded rows of
 // SPH_NGB_STRIDE entries built in one walk, or (csr, or a row longer than the
 // stride) the exact two-pass CSR.
-int build_ngb(sph_ctx *ctx, bool csr, const double *hscale = nullptr)
+int build_ngb(sph_ctx *ctx, bool csr, const double *hscale = nullptr, double per = 0.0)
 {
     const int64_t n = ctx->n;
+    ctx->ngb_per = per;
     unsigned long long *ovf = P_<unsigned long long>(ctx->hcount) + 8;
     if (!csr) {
         CK(ensure(ctx, ctx->idx, (size_t)n * SPH_NGB_STRIDE * 4));
@@ -348,7 +350,7 @@ int build_ngb(sph_ctx *ctx, bool csr, const double *hscale = nullptr)
         HIPCK(sph::neighbour_rows(ctx->st, ctx->ngroups, I32(groups), D(x), D(y), D(z), D(h), P_<WNode>(ctx->nodes),
                                   SPH_BUCKET, SPH_NGB_STRIDE, I32(cnt), I32(idx), ovf,
                                   P_<unsigned long long>(ctx->spread) + SPR_N * SPH_SPREAD_WORDS,
-                                  P_<unsigned long long>(ctx->hcount) + 10, &nlong, &total, hscale));
+                                  P_<unsigned long long>(ctx->hcount) + 10, &nlong, &total, hscale, per));
         if (nlong == 0) {
             ctx->ngb_stride = SPH_NGB_STRIDE;
             ctx->csr_total = total;
@@ -360,11 +362,11 @@ int build_ngb(sph_ctx *ctx, bool csr, const double *hscale = nullptr)
     int64_t cap = ctx->idx.p ? (int64_t)(ctx->idx.bytes / 4) : 0;
     HIPCK(sph::neighbours(ctx->st, n, ctx->ngroups, I32(groups), D(x), D(y), D(z), D(h), P_<WNode>(ctx->nodes),
                           SPH_BUCKET, I32(cnt), P_<int64_t>(ctx->off), P_<int64_t>(ctx->scantmp), ovf, I32(idx), cap,
-                          &total, hscale));
+                          &total, hscale, per));
     if (total > cap) {
         CK(ensure(ctx, ctx->idx, (size_t)total * 4));
         HIPCK(sph::neighbours_fill(ctx->st, ctx->ngroups, I32(groups), D(x), D(y), D(z), D(h), P_<WNode>(ctx->nodes),
-                                   SPH_BUCKET, P_<int64_t>(ctx->off), I32(idx), ovf, hscale));
+                                   SPH_BUCKET, P_<int64_t>(ctx->off), I32(idx), ovf, hscale, per));
     }
     ctx->ngb_stride = 0;
     ctx->csr_total = total;
@@ -374,7 +376,7 @@ int build_ngb(sph_ctx *ctx, bool csr, const double *hscale = nullptr)
 
 NgbRows ngb_rows(sph_ctx *ctx)
 {
-    return NgbRows{P_<int64_t>(ctx->off), I32(cnt), I32(idx), ctx->ngb_stride};
+    return NgbRows{P_<int64_t>(ctx->off), I32(cnt), I32(idx), ctx->ngb_stride, ctx->ngb_per};
 }
 
 // csr: the caller needs true CSR offsets (sph_neighbours).
@@ -972,6 +974,7 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     sph::HSolveParams hp;
     CK(hsolve_params(ctx, n, D(x), D(y), D(z), SPH_H_NEWTON, p->n_ngb, 0, 0, 0, p->h_tol, &ctx->hs_unc,
                      &ctx->stats.h_rounds, ctx->hs_work, true, hp));
+    hp.per = p->period;
     HIPCK(sph::hsolve_begin(st, hp));
     ph_end(ctx, SPH_PH_HSOLVE);
     std::swap(ctx->h, ctx->hout);  // the solve writes hp.h_out: the new h
@@ -1021,13 +1024,25 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     CK(ensure(ctx, ctx->cext, (size_t)ext_cap * 8));
     CK(ensure(ctx, ctx->cnext, (size_t)SPH_SPREAD_WORDS * 8));
     ph_begin(ctx, SPH_PH_NGB);
+    sph::BigRows big;  // targets of the candidate-free h solver: rows by a walk each
+    big.list = I32(hbig);
+    big.count = ctx->stats.h_big;
+    big.x = D(x); big.y = D(y); big.z = D(z);
+    big.nodes = P_<WNode>(ctx->nodes);
+    big.bucket = SPH_BUCKET;
+    big.per = p->period;
+    if (big.count > 0 && !spline) {  // node h_max are those of h_old: scale them
+        HIPCK(sph::h_ratio_max(st, n, D(h), D(hin), D(kout) + 7));  // slot 7 (kinetics uses 0..4)
+        big.hscale = D(kout) + 7;
+    }
     HIPCK(sph::neighbour_rows_from_candidates_begin(
         st, n, SPH_HCAP, D(cand), I32(ccidx), I32(cccnt), D(h), D(hR), SPH_NGB_STRIDE, I32(cnt), I32(rcnt0), I32(idx),
         P_<int2>(ctx->cext), ext_cap, P_<unsigned long long>(ctx->cnext),
         P_<unsigned long long>(ctx->spread) + SPR_N * SPH_SPREAD_WORDS, P_<unsigned long long>(ctx->hcount) + 10,
-        ctx->hpin + 8, ctx->ev_n));
+        ctx->hpin + 8, ctx->ev_n, big));
     ph_end(ctx, SPH_PH_NGB);
     ctx->ngb_stride = SPH_NGB_STRIDE;
+    ctx->ngb_per = p->period;
     R *bal = p->balsara ? P_<R>(ctx->bal) : nullptr;
     const bool g = p->G != 0.0;
     // density + EOS, fused with div/curl v and the Balsara factor when the switch is on; then
@@ -1064,7 +1079,7 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
                 HIPCK(sph::h_ratio_max(st, n, D(h), D(hin), D(kout) + 7));  // slot 7 (kinetics uses 0..4)
                 hscale = D(kout) + 7;
             }
-            CK(build_ngb(ctx, false, hscale));
+            CK(build_ngb(ctx, false, hscale, p->period));
             CK(hydro_forces());
         }
     }
@@ -1101,6 +1116,9 @@ int check_params(sph_ctx *ctx, const sph_step_params *p)
     if (p->soft_kind != SPH_SOFT_SPLINE && p->soft_kind != SPH_SOFT_PLUMMER) return fail(ctx, SPH_ERR_ARG, "bad soft_kind");
     if (!(p->dt > 0.0) || !(p->n_ngb > 0.0) || !(p->h_tol > 0.0) || !(p->theta >= 0.0))
         return fail(ctx, SPH_ERR_ARG, "dt, n_ngb, h_tol must be > 0 and theta >= 0");
+    if (!(p->period >= 0.0)) return fail(ctx, SPH_ERR_ARG, "period must be >= 0");
+    if (p->period > 0.0 && p->G != 0.0)
+        return fail(ctx, SPH_ERR_ARG, "a periodic box needs G = 0 (periodic gravity is not provided)");
     HIPCK(hipSetDevice(ctx->device));
     return SPH_OK;
 }
@@ -1158,7 +1176,8 @@ extern "C" int sph_step(sph_ctx *ctx, const sph_step_params *p)
     ctx->ph_rec = 0;
     int64_t n = ctx->n;
     ph_begin(ctx, SPH_PH_KICKDRIFT);
-    HIPCK(sph::kick_drift(ctx->st, n, D(vx), D(vy), D(vz), D(ax), D(ay), D(az), 0.5 * p->dt, p->dt, D(x), D(y), D(z)));
+    HIPCK(sph::kick_drift(ctx->st, n, D(vx), D(vy), D(vz), D(ax), D(ay), D(az), 0.5 * p->dt, p->dt, D(x), D(y), D(z),
+                          p->period));
     ph_end(ctx, SPH_PH_KICKDRIFT);
     int r = p->precision == SPH_PREC_MIXED ? forces<float>(ctx, p, 0) : forces<double>(ctx, p, 0);
     if (r != SPH_OK) return r;

@@ -129,6 +129,20 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb)
     return x * q + (x < r ? x : r) + k;
 }
 
+// Minimum-image difference of two coordinates in a periodic box of side L
+// (L = 0: open boundaries): the wrap is decided on the FP64 difference, one
+// rounding per operation, identically in every kernel and in the oracle.
+// PER = false compiles to nothing (kernels are instantiated for both).
+template <bool PER>
+__device__ __forceinline__ double wrap_d(double d, double L)
+{
+    if constexpr (PER) {
+        if (d > 0.5 * L) d = d - L;
+        else if (d < -0.5 * L) d = d + L;
+    }
+    return d;
+}
+
 // Host-side launch helper.
 static inline unsigned sph_blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 

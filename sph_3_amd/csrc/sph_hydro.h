@@ -12,6 +12,7 @@ struct NgbRows {
     const int32_t *cnt;
     const int32_t *idx;
     int stride;
+    double L;  // side of the periodic box the sets were built in (0: open boundaries)
 };
 
 __device__ __forceinline__ void row_range(const NgbRows &r, int64_t i, int64_t &b, int64_t &e)

@@ -16,7 +16,7 @@
 
 #define HG 16  // lanes per particle
 
-template <typename R, int G>
+template <typename R, int G, bool PER>
 __global__ void __launch_bounds__(256) k_density(int64_t n, const double *__restrict__ x,
                                                  const double *__restrict__ y, const double *__restrict__ z,
                                                  const double *__restrict__ m, const double *__restrict__ h,
@@ -36,7 +36,8 @@ __global__ void __launch_bounds__(256) k_density(int64_t n, const double *__rest
         row_range(rows, i, k0, e);
         for (int64_t k = k0 + l; k < e; k += G) {
             int j = rows.idx[k];
-            R dx = (R)(xi - x[j]), dy = (R)(yi - y[j]), dz = (R)(zi - z[j]);
+            R dx = (R)wrap_d<PER>(xi - x[j], rows.L), dy = (R)wrap_d<PER>(yi - y[j], rows.L),
+              dz = (R)wrap_d<PER>(zi - z[j], rows.L);
             R rr = sph_sqrt<R>(dx * dx + dy * dy + dz * dz);
             R hij = (R)(0.5 * (hi + h[j]));
             acc.add((R)m[j] * spline_W<R>(rr, hij));
@@ -55,7 +56,7 @@ __global__ void __launch_bounds__(256) k_density(int64_t n, const double *__rest
     }
 }
 
-template <typename R, int G>
+template <typename R, int G, bool PER>
 __global__ void __launch_bounds__(256) k_divcurl(int64_t n, const double *__restrict__ x,
                                                  const double *__restrict__ y, const double *__restrict__ z,
                                                  const double *__restrict__ vx, const double *__restrict__ vy,
@@ -76,7 +77,8 @@ __global__ void __launch_bounds__(256) k_divcurl(int64_t n, const double *__rest
         row_range(rows, i, k0, e);
         for (int64_t k = k0 + l; k < e; k += G) {
             int j = rows.idx[k];
-            R dx = (R)(xi - x[j]), dy = (R)(yi - y[j]), dz = (R)(zi - z[j]);
+            R dx = (R)wrap_d<PER>(xi - x[j], rows.L), dy = (R)wrap_d<PER>(yi - y[j], rows.L),
+              dz = (R)wrap_d<PER>(zi - z[j], rows.L);
             R rr = sph_sqrt<R>(dx * dx + dy * dy + dz * dz);
             R nW = spline_nW<R>(rr, (R)(0.5 * (hi + h[j])));
             R gx = nW * dx, gy = nW * dy, gz = nW * dz;
@@ -108,7 +110,7 @@ __global__ void __launch_bounds__(256) k_divcurl(int64_t n, const double *__rest
 // curl need only the particle's own density, so both sums share the
 // neighbour gathers and are divided by rho_i at the end.  Per-quantity
 // arithmetic and summation order are those of k_density and k_divcurl.
-template <typename R, int G>
+template <typename R, int G, bool PER>
 __global__ void __launch_bounds__(256) k_density_divcurl(
     int64_t n, const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
     const double *__restrict__ vx, const double *__restrict__ vy, const double *__restrict__ vz,
@@ -128,7 +130,8 @@ __global__ void __launch_bounds__(256) k_density_divcurl(
         row_range(rows, i, k0, e);
         for (int64_t k = k0 + l; k < e; k += G) {
             int j = rows.idx[k];
-            R dx = (R)(xi - x[j]), dy = (R)(yi - y[j]), dz = (R)(zi - z[j]);
+            R dx = (R)wrap_d<PER>(xi - x[j], rows.L), dy = (R)wrap_d<PER>(yi - y[j], rows.L),
+              dz = (R)wrap_d<PER>(zi - z[j], rows.L);
             R rr = sph_sqrt<R>(dx * dx + dy * dy + dz * dz);
             R hij = (R)(0.5 * (hi + h[j]));
             R mj = (R)m[j];
@@ -175,7 +178,7 @@ __device__ __forceinline__ R av_pi(R vr, R rr, R ci, R cj, R rhoi, R rhoj, R alp
     return (R)-0.5 * alpha * vsig * w / ((R)0.5 * (rhoi + rhoj));
 }
 
-template <typename R, int G>
+template <typename R, int G, bool PER>
 __global__ void __launch_bounds__(256) k_accsph(int64_t n, const double *__restrict__ x,
                                                 const double *__restrict__ y, const double *__restrict__ z,
                                                 const double *__restrict__ vx, const double *__restrict__ vy,
@@ -202,7 +205,8 @@ __global__ void __launch_bounds__(256) k_accsph(int64_t n, const double *__restr
         row_range(rows, i, k0, e);
         for (int64_t k = k0 + l; k < e; k += G) {
             int j = rows.idx[k];
-            R dx = (R)(xi - x[j]), dy = (R)(yi - y[j]), dz = (R)(zi - z[j]);
+            R dx = (R)wrap_d<PER>(xi - x[j], rows.L), dy = (R)wrap_d<PER>(yi - y[j], rows.L),
+              dz = (R)wrap_d<PER>(zi - z[j], rows.L);
             R rr = sph_sqrt<R>(dx * dx + dy * dy + dz * dz);
             R nW = spline_nW<R>(rr, (R)(0.5 * (hi + h[j])));
             R ux = (R)(vxi - vx[j]), uy = (R)(vyi - vy[j]), uz = (R)(vzi - vz[j]);
@@ -226,7 +230,7 @@ __global__ void __launch_bounds__(256) k_accsph(int64_t n, const double *__restr
     }
 }
 
-template <typename R, int G>
+template <typename R, int G, bool PER>
 __global__ void __launch_bounds__(256) k_dudt(int64_t n, const double *__restrict__ x,
                                               const double *__restrict__ y, const double *__restrict__ z,
                                               const double *__restrict__ vx, const double *__restrict__ vy,
@@ -251,7 +255,8 @@ __global__ void __launch_bounds__(256) k_dudt(int64_t n, const double *__restric
         row_range(rows, i, k0, e);
         for (int64_t k = k0 + l; k < e; k += G) {
             int j = rows.idx[k];
-            R dx = (R)(xi - x[j]), dy = (R)(yi - y[j]), dz = (R)(zi - z[j]);
+            R dx = (R)wrap_d<PER>(xi - x[j], rows.L), dy = (R)wrap_d<PER>(yi - y[j], rows.L),
+              dz = (R)wrap_d<PER>(zi - z[j], rows.L);
             R rr = sph_sqrt<R>(dx * dx + dy * dy + dz * dz);
             R nW = spline_nW<R>(rr, (R)(0.5 * (hi + h[j])));
             R ux = (R)(vxi - vx[j]), uy = (R)(vyi - vy[j]), uz = (R)(vzi - vz[j]);
@@ -272,7 +277,10 @@ hipError_t density(hipStream_t st, int64_t n, const double *x, const double *y, 
                    const double *h, const double *u, double gamma, const NgbRows &rows, R *rho, R *P,
                    R *c, R *prho2)
 {
-    k_density<R, HG><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, m, h, u, gamma, rows, rho, P, c, prho2);
+    if (rows.L > 0.0)
+        k_density<R, HG, true><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, m, h, u, gamma, rows, rho, P, c, prho2);
+    else
+        k_density<R, HG, false><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, m, h, u, gamma, rows, rho, P, c, prho2);
     return hipGetLastError();
 }
 
@@ -281,7 +289,11 @@ hipError_t divcurl(hipStream_t st, int64_t n, const double *x, const double *y, 
                    const double *vy, const double *vz, const double *m, const double *h, const R *rho, const R *c,
                    const NgbRows &rows, R *divV, R *curlV, R *bal)
 {
-    k_divcurl<R, HG><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, rho, c, rows, divV,
+    if (rows.L > 0.0)
+        k_divcurl<R, HG, true><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, rho, c, rows, divV,
+                                                              curlV, bal);
+    else
+        k_divcurl<R, HG, false><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, rho, c, rows, divV,
                                                               curlV, bal);
     return hipGetLastError();
 }
@@ -292,7 +304,11 @@ hipError_t density_divcurl(hipStream_t st, int64_t n, const double *x, const dou
                            const double *u, double gamma, const NgbRows &rows, R *rho, R *P, R *c,
                            R *prho2, R *divV, R *curlV, R *bal)
 {
-    k_density_divcurl<R, HG><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, u, gamma, rows,
+    if (rows.L > 0.0)
+        k_density_divcurl<R, HG, true><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, u, gamma, rows,
+                                                                      rho, P, c, prho2, divV, curlV, bal);
+    else
+        k_density_divcurl<R, HG, false><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, u, gamma, rows,
                                                                       rho, P, c, prho2, divV, curlV, bal);
     return hipGetLastError();
 }
@@ -303,7 +319,11 @@ hipError_t accsph(hipStream_t st, int64_t n, const double *x, const double *y, c
                   const R *prho2, const R *c, const R *bal, double alpha, const NgbRows &rows,
                   const double *gx, const double *gy, const double *gz, double *ax, double *ay, double *az)
 {
-    k_accsph<R, HG><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, rho, prho2, c, bal, alpha,
+    if (rows.L > 0.0)
+        k_accsph<R, HG, true><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, rho, prho2, c, bal, alpha,
+                                                             rows, gx, gy, gz, ax, ay, az);
+    else
+        k_accsph<R, HG, false><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, rho, prho2, c, bal, alpha,
                                                              rows, gx, gy, gz, ax, ay, az);
     return hipGetLastError();
 }
@@ -313,7 +333,11 @@ hipError_t dudt(hipStream_t st, int64_t n, const double *x, const double *y, con
                 const double *vy, const double *vz, const double *m, const double *h, const R *rho, const R *prho2,
                 const R *c, const R *bal, double alpha, const NgbRows &rows, R *out)
 {
-    k_dudt<R, HG><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, rho, prho2, c, bal, alpha,
+    if (rows.L > 0.0)
+        k_dudt<R, HG, true><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, rho, prho2, c, bal, alpha,
+                                                           rows, out);
+    else
+        k_dudt<R, HG, false><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, rho, prho2, c, bal, alpha,
                                                            rows, out);
     return hipGetLastError();
 }

@@ -6,7 +6,8 @@ namespace sph {
 
 // v += a*dtk; if x != null: x += v*dtd (with the kicked v).
 hipError_t kick_drift(hipStream_t st, int64_t n, double *vx, double *vy, double *vz, const double *ax,
-                      const double *ay, const double *az, double dtk, double dtd, double *x, double *y, double *z);
+                      const double *ay, const double *az, double dtk, double dtd, double *x, double *y, double *z,
+                      double L = 0.0);
 // first: u += du*dt; else u += dt/2 (du + du_prev).  Then du_prev = du.
 template <typename R>
 hipError_t u_update(hipStream_t st, int64_t n, double *u, const R *du, double *du_prev, double dt, int first);

@@ -13,16 +13,24 @@ __global__ void __launch_bounds__(256) k_kick_drift(int64_t n, double *__restric
                                                     double *__restrict__ vz, const double *__restrict__ ax,
                                                     const double *__restrict__ ay, const double *__restrict__ az,
                                                     double dtk, double dtd, double *__restrict__ x,
-                                                    double *__restrict__ y, double *__restrict__ z)
+                                                    double *__restrict__ y, double *__restrict__ z, double L)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     double a = vx[i] + ax[i] * dtk, b = vy[i] + ay[i] * dtk, c = vz[i] + az[i] * dtk;
     vx[i] = a; vy[i] = b; vz[i] = c;
     if (x) {
-        x[i] = x[i] + a * dtd;
-        y[i] = y[i] + b * dtd;
-        z[i] = z[i] + c * dtd;
+        double p[3] = {x[i] + a * dtd, y[i] + b * dtd, z[i] + c * dtd};
+        if (L > 0.0) {  // back into [0, L) (one image; a step moves a particle less than L)
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                if (p[d] >= L) p[d] = p[d] - L;
+                else if (p[d] < 0.0) p[d] = p[d] + L;
+            }
+        }
+        x[i] = p[0];
+        y[i] = p[1];
+        z[i] = p[2];
     }
 }
 
@@ -98,9 +106,10 @@ __global__ void __launch_bounds__(256) k_kin_final(int nb, const double *__restr
 namespace sph {
 
 hipError_t kick_drift(hipStream_t st, int64_t n, double *vx, double *vy, double *vz, const double *ax,
-                      const double *ay, const double *az, double dtk, double dtd, double *x, double *y, double *z)
+                      const double *ay, const double *az, double dtk, double dtd, double *x, double *y, double *z,
+                      double L)
 {
-    k_kick_drift<<<sph_blocks(n, 256), 256, 0, st>>>(n, vx, vy, vz, ax, ay, az, dtk, dtd, x, y, z);
+    k_kick_drift<<<sph_blocks(n, 256), 256, 0, st>>>(n, vx, vy, vz, ax, ay, az, dtk, dtd, x, y, z, L);
     return hipGetLastError();
 }
 

@@ -43,6 +43,7 @@ struct HSolveParams {
     int32_t *big_list = nullptr;
     unsigned long long *big_count = nullptr;
     int64_t *n_big = nullptr;  // host (may be null): targets given to that solver
+    double per = 0.0;          // side of a periodic box: minimum-image distances (modes 0 and 2; 0: open)
 };
 
 // Runs rounds until every target is decided; *n_unconverged = targets still
@@ -60,16 +61,17 @@ hipError_t hsolve_end(hipStream_t st, const HSolveParams &p, int max_rounds, int
                       bool *more_rounds);
 
 // CSR of S_i = { j : r_ij^2 < (h_i+h_j)^2 } over sorted particles, one
-// wavefront per tree-node group.  hscale (device, may be null): the node h_max
+// wavefront per tree-node group (per > 0: minimum-image distances in a periodic box).  hscale (device, may be null): the node h_max
 // were built from an earlier h and are scaled by *hscale in the prune.  Counts, scans (off[0..n]), reads back
 // *total; fills idx if idx_cap >= *total.  overflow: 1 scratch counter.
 hipError_t neighbours(hipStream_t st, int64_t n, int64_t ngroups, const int32_t *groups, const double *x,
                       const double *y, const double *z, const double *h, const WNode *nodes, int bucket, int32_t *cnt,
                       int64_t *off, int64_t *scan_tmp, unsigned long long *overflow, int32_t *idx, int64_t idx_cap,
-                      int64_t *total, const double *hscale = nullptr);
+                      int64_t *total, const double *hscale = nullptr, double per = 0.0);
 hipError_t neighbours_fill(hipStream_t st, int64_t ngroups, const int32_t *groups, const double *x, const double *y,
                            const double *z, const double *h, const WNode *nodes, int bucket, const int64_t *off,
-                           int32_t *idx, unsigned long long *overflow, const double *hscale = nullptr);
+                           int32_t *idx, unsigned long long *overflow, const double *hscale = nullptr,
+                           double per = 0.0);
 // The same sets in one walk as padded rows: idx[i*stride + c], c < cnt[i] (sorted
 // particles).  *n_long = rows with more than stride entries (those rows are
 // truncated: rebuild as CSR), *total = entries.  spread: 2 * SPH_SPREAD_WORDS
@@ -79,22 +81,34 @@ hipError_t neighbour_rows(hipStream_t st, int64_t ngroups, const int32_t *groups
                           const double *z, const double *h, const WNode *nodes, int bucket, int stride, int32_t *cnt,
                           int32_t *idx, unsigned long long *overflow, unsigned long long *spread,
                           unsigned long long *totals, int64_t *n_long, int64_t *total,
-                          const double *hscale = nullptr);
+                          const double *hscale = nullptr, double per = 0.0);
 // Padded neighbour rows (as neighbour_rows) built from the h solve's final candidate rows
 // and indices (targets = the sorted particles, same positions and the solved h) instead of
 // a tree walk; exact by the containment argument at k_cand_rows.  ext: ext_cap int2 scratch
 // for the pairs found from the other side (SPH_SPREAD regions), next: SPH_SPREAD_WORDS
 // scratch words (one counter per region), cnt0: n int32 scratch (row lengths before the
-// appended pairs).  *n_long > 0: some row is too long or an extra was lost -- rebuild with
-// a walk.  In two halves: _begin enqueues the kernels and the asynchronous copy of the
+// appended pairs).  Targets the candidate-free h solver handled (incomplete candidate rows)
+// get their rows from one walk each (big).  *n_long > 0: some row is too long or an extra
+// was lost -- rebuild with a walk.  In two halves: _begin enqueues the kernels and the asynchronous copy of the
 // totals and region counters into host_words (pinned, 2 + SPH_SPREAD words) followed by
 // done; _end waits for done (not the stream) and evaluates them.
+// big: the targets of the candidate-free h solver (their rows are built by a walk each).
+struct BigRows {
+    const int32_t *list = nullptr;
+    int64_t count = 0;
+    const double *x = nullptr, *y = nullptr, *z = nullptr;
+    const WNode *nodes = nullptr;
+    int bucket = 0;
+    const double *hscale = nullptr;  // device: node h_max belong to an earlier h (may be null)
+    double per = 0.0;                // side of a periodic box (0: open boundaries)
+};
 hipError_t neighbour_rows_from_candidates_begin(hipStream_t st, int64_t n, int cap, const double *cand,
                                                const int32_t *cidx, const int32_t *ccnt, const double *h,
                                                const double *R, int stride, int32_t *cnt, int32_t *cnt0,
                                                int32_t *idx, int2 *ext, int64_t ext_cap, unsigned long long *next,
                                                unsigned long long *spread, unsigned long long *totals,
-                                               unsigned long long *host_words, hipEvent_t done);
+                                               unsigned long long *host_words, hipEvent_t done,
+                                               const BigRows &big);
 hipError_t neighbour_rows_from_candidates_end(hipEvent_t done, const unsigned long long *host_words, int64_t ext_cap,
                                              int64_t *n_long, int64_t *total, int64_t *n_extra);
 // *out (device) = max_i h_new[i] / h_old[i], rounded up by 8 ulp: the factor that

@@ -98,6 +98,7 @@ struct HSolveArgs {
     const WNode *gnodes;       // node records the group ids refer to
     int32_t *big_list;         // mode 0 (may be null): targets whose solution needs more
     unsigned long long *big_count;  //   candidates than a row holds (state 3), for k_hsolve_big
+    double per;                // side of a periodic box (0: open boundaries; modes 0 and 2)
 };
 
 // Group range: tree node groups[blk] or the block [64 blk, 64 blk + 64) of targets.
@@ -114,6 +115,7 @@ __device__ __forceinline__ void group_range(const int32_t *groups, const WNode *
     }
 }
 
+template <bool PER>
 __global__ void __launch_bounds__(64) k_hsolve(HSolveArgs a)
 {
     __shared__ WalkLDS L;
@@ -144,7 +146,7 @@ __global__ void __launch_bounds__(64) k_hsolve(HSolveArgs a)
     int c = 0;
     long long ntest = 0, nterm = 0;  // algorithmic work (distance tests, kernel terms)
     const bool ovf = group_walk(
-        a.nodes, a.bucket, L, [&](const WNode *nd) { return box_box_dist2(gb, nd) >= Rg2; },
+        a.nodes, a.bucket, L, [&](const WNode *nd) { return box_box_dist2_p<PER>(gb, nd, a.per) >= Rg2; },
         [&](int q, int j) {
             sx[q] = a.x[j];
             sy[q] = a.y[j];
@@ -155,7 +157,8 @@ __global__ void __launch_bounds__(64) k_hsolve(HSolveArgs a)
             if (valid) ntest += cnt;
             if (valid)
                 for (int k = 0; k < cnt; k++) {
-                    double dx = sx[k] - xi, dy = sy[k] - yi, dz = sz[k] - zi;
+                    double dx = wrap_d<PER>(sx[k] - xi, a.per), dy = wrap_d<PER>(sy[k] - yi, a.per),
+                           dz = wrap_d<PER>(sz[k] - zi, a.per);
                     double r2 = dx * dx + dy * dy + dz * dz;
                     if (r2 < R2) {
                         const int b = c & 7;
@@ -372,6 +375,7 @@ __global__ void __launch_bounds__(64) k_hsolve(HSolveArgs a)
 // safeguarded by the bracket [lo, hi] (hi doubles until f(hi) >= 0); the stopping
 // rule is the row solver's.  The target's candidate count is set to cap + 1: its row
 // of candidates is incomplete (neighbour rows built from candidates must not use it).
+template <bool PER>
 __global__ void __launch_bounds__(64) k_hsolve_big(HSolveArgs a, const int32_t *__restrict__ list)
 {
     __shared__ WalkLDS L;
@@ -385,7 +389,7 @@ __global__ void __launch_bounds__(64) k_hsolve_big(HSolveArgs a, const int32_t *
         const double lim = 2.0 * hh, L2 = lim * lim, ih = 1.0 / hh;
         double sl = 0.0, sq = 0.0;
         (void)group_walk(
-            a.nodes, a.bucket, L, [&](const WNode *nd) { return box_box_dist2(gb, nd) >= L2; },
+            a.nodes, a.bucket, L, [&](const WNode *nd) { return box_box_dist2_p<PER>(gb, nd, a.per) >= L2; },
             [&](int q, int j) {
                 sx[q] = a.x[j];
                 sy[q] = a.y[j];
@@ -394,7 +398,8 @@ __global__ void __launch_bounds__(64) k_hsolve_big(HSolveArgs a, const int32_t *
             [&](int cnt) {
                 if (lane == 0) ntest += cnt;
                 for (int k = lane; k < cnt; k += 64) {
-                    const double dx = sx[k] - xi, dy = sy[k] - yi, dz = sz[k] - zi;
+                    const double dx = wrap_d<PER>(sx[k] - xi, a.per), dy = wrap_d<PER>(sy[k] - yi, a.per),
+                                 dz = wrap_d<PER>(sz[k] - zi, a.per);
                     const double r2 = dx * dx + dy * dy + dz * dz;
                     if (r2 < L2) {
                         nterm++;
@@ -506,7 +511,7 @@ __global__ void __launch_bounds__(256) k_hsolve_init(int64_t m, const double *__
 // MODE 0: counts only; 1: fill CSR rows at off[i]; 2: fill padded rows
 // idx[i*stride + c] (c < stride) and counts in one walk, with spread counters
 // 0 += rows longer than stride, 1 += entries.
-template <int MODE>
+template <int MODE, bool PER>
 __global__ void __launch_bounds__(64) k_ngb(const int32_t *__restrict__ groups, const double *__restrict__ x,
                                             const double *__restrict__ y, const double *__restrict__ z,
                                             const double *__restrict__ h, const WNode *__restrict__ nodes, int bucket,
@@ -514,7 +519,7 @@ __global__ void __launch_bounds__(64) k_ngb(const int32_t *__restrict__ groups, 
                                             int32_t *__restrict__ idx, int stride,
                                             unsigned long long *__restrict__ overflow,
                                             unsigned long long *__restrict__ spread,
-                                            const double *__restrict__ hscale)
+                                            const double *__restrict__ hscale, double per)
 {
     __shared__ WalkLDS L;
     __shared__ double sx[WSRC], sy[WSRC], sz[WSRC], sh[WSRC];
@@ -537,7 +542,7 @@ __global__ void __launch_bounds__(64) k_ngb(const int32_t *__restrict__ groups, 
         nodes, bucket, L,
         [&](const WNode *nd) {
             double t = hg + nd->hmax * hs;
-            return box_box_dist2(gb, nd) >= t * t;
+            return box_box_dist2_p<PER>(gb, nd, per) >= t * t;
         },
         [&](int q, int j) {
             sx[q] = x[j];
@@ -549,7 +554,7 @@ __global__ void __launch_bounds__(64) k_ngb(const int32_t *__restrict__ groups, 
         [&](int n) {
             if (valid)
                 for (int k = 0; k < n; k++) {
-                    double dx = xi - sx[k], dy = yi - sy[k], dz = zi - sz[k];
+                    double dx = wrap_d<PER>(xi - sx[k], per), dy = wrap_d<PER>(yi - sy[k], per), dz = wrap_d<PER>(zi - sz[k], per);
                     double r2 = dx * dx + dy * dy + dz * dz, t = hi + sh[k];
                     if (r2 < t * t) {
                         if (MODE == 1 || (MODE == 2 && c < stride)) idx[o + c] = sj[k];
@@ -616,7 +621,7 @@ __global__ void __launch_bounds__(256) k_cand_rows(int64_t n, int cap, const dou
             j = rj[k];
             const double r2 = rr[k], hj = h[j], t = hi + hj, t2 = t * t;
             in = r2 < t2;
-            if (in && hi > hj) {
+            if (in && hi > hj && ccnt[j] <= cap) {  // (a row of k_hsolve_big's is walked: k_big_rows)
                 const double Rj = R[j];
                 extra = r2 >= Rj * Rj;
             }
@@ -633,12 +638,14 @@ __global__ void __launch_bounds__(256) k_cand_rows(int64_t n, int cap, const dou
             if (e < (unsigned long long)rcap) ext[reg * rcap + (int64_t)e] = make_int2(j, (int)i);
         }
     }
-    if (valid && l == 0) {
+    // the row and counts of an incomplete candidate row are written by k_big_rows
+    const bool own = valid && l == 0 && !incomplete;
+    if (own) {
         cnt[i] = w;
         cnt0[i] = w;
     }
-    unsigned long long nlong = (valid && l == 0 && (w > stride || incomplete)) ? 1ull : 0ull;
-    unsigned long long nent = (valid && l == 0) ? (unsigned long long)w : 0ull;
+    unsigned long long nlong = (own && w > stride) ? 1ull : 0ull;
+    unsigned long long nent = own ? (unsigned long long)w : 0ull;
     for (int off = 32; off >= 1; off >>= 1) {
         nlong += __shfl_xor(nlong, off);
         nent += __shfl_xor(nent, off);
@@ -646,6 +653,86 @@ __global__ void __launch_bounds__(256) k_cand_rows(int64_t n, int cap, const dou
     if (lane == 0) {
         if (nlong) spread_add(spread, 0, nlong);
         spread_add(spread, 1, nent);
+    }
+}
+
+// Rows of the targets solved by k_hsolve_big (their candidate rows are incomplete):
+// one wavefront per target j walks the tree for S_j (prune as k_ngb, node h_max
+// scaled by *hscale when they belong to an earlier h) and writes row j in walk order
+// (lanes compacted by ballots).  For a member i whose candidate row is complete, with
+// h_i < h_j and r^2 >= R_i^2 (j is not among i's candidates, so k_cand_rows cannot
+// add it), it records the extra (i, j).  r^2 uses x_j - x_i, the same bits as i's
+// candidate test.  Spread counters: 0 += rows longer than stride, 1 += entries.
+template <bool PER>
+__global__ void __launch_bounds__(64) k_big_rows(const int32_t *__restrict__ list, const double *__restrict__ x,
+                                                 const double *__restrict__ y, const double *__restrict__ z,
+                                                 const double *__restrict__ h, const double *__restrict__ R,
+                                                 const int32_t *__restrict__ ccnt, int cap,
+                                                 const WNode *__restrict__ nodes, int bucket,
+                                                 const double *__restrict__ hscale, int stride,
+                                                 int32_t *__restrict__ cnt, int32_t *__restrict__ cnt0,
+                                                 int32_t *__restrict__ idx, int2 *__restrict__ ext, int64_t ext_cap,
+                                                 unsigned long long *__restrict__ next,
+                                                 unsigned long long *__restrict__ spread, double per)
+{
+    __shared__ WalkLDS L;
+    __shared__ double sx[WSRC], sy[WSRC], sz[WSRC], sh[WSRC];
+    __shared__ int sj[WSRC];
+    const int64_t jt = list[blockIdx.x];
+    const int lane = threadIdx.x;
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const double xj = x[jt], yj = y[jt], zj = z[jt], hj = h[jt];
+    const double gb[6] = {xj, yj, zj, xj, yj, zj};
+    const double hs = hscale ? *hscale : 1.0;
+    int32_t *row = idx + jt * (int64_t)stride;
+    const int64_t rcap = ext_cap / SPH_SPREAD, reg = blockIdx.x & (SPH_SPREAD - 1);
+    int w = 0;
+    (void)group_walk(
+        nodes, bucket, L,
+        [&](const WNode *nd) {
+            const double t = hj + nd->hmax * hs;
+            return box_box_dist2_p<PER>(gb, nd, per) >= t * t;
+        },
+        [&](int q, int i) {
+            sx[q] = x[i];
+            sy[q] = y[i];
+            sz[q] = z[i];
+            sh[q] = h[i];
+            sj[q] = i;
+        },
+        [&](int ns) {
+            for (int k0 = 0; k0 < ns; k0 += 64) {
+                const int k = k0 + lane;
+                bool in = false, extra = false;
+                int i = 0;
+                if (k < ns) {
+                    const double dx = wrap_d<PER>(xj - sx[k], per), dy = wrap_d<PER>(yj - sy[k], per),
+                                 dz = wrap_d<PER>(zj - sz[k], per);
+                    const double r2 = dx * dx + dy * dy + dz * dz, t = hj + sh[k];
+                    in = r2 < t * t;
+                    i = sj[k];
+                    if (in && sh[k] < hj && ccnt[i] <= cap) {
+                        const double Ri = R[i];
+                        extra = r2 >= Ri * Ri;
+                    }
+                }
+                const uint64_t mb = __ballot(in);
+                if (in) {
+                    const int pos = w + (int)__popcll(mb & below);
+                    if (pos < stride) row[pos] = i;
+                }
+                w += (int)__popcll(mb);
+                if (extra) {
+                    const unsigned long long e = atomicAdd(next + reg * SPH_SPREAD_PAD, 1ull);
+                    if (e < (unsigned long long)rcap) ext[reg * rcap + (int64_t)e] = make_int2(i, (int)jt);
+                }
+            }
+        });
+    if (lane == 0) {
+        cnt[jt] = w;
+        cnt0[jt] = w;
+        if (w > stride) spread_add(spread, 0, 1ull);
+        spread_add(spread, 1, (unsigned long long)w);
     }
 }
 
@@ -718,13 +805,25 @@ hipError_t neighbour_rows_from_candidates_begin(hipStream_t st, int64_t n, int c
                                                const double *R, int stride, int32_t *cnt, int32_t *cnt0,
                                                int32_t *idx, int2 *ext, int64_t ext_cap, unsigned long long *next,
                                                unsigned long long *spread, unsigned long long *totals,
-                                               unsigned long long *host_words, hipEvent_t done)
+                                               unsigned long long *host_words, hipEvent_t done,
+                                               const BigRows &big)
 {
     if (stride < 1 || cap < 1 || ext_cap < SPH_SPREAD) return hipErrorInvalidValue;
     fill<unsigned long long>(st, SPH_SPREAD_WORDS, next, 0ull);
     fill<unsigned long long>(st, 2 * SPH_SPREAD_WORDS, spread, 0ull);
     k_cand_rows<16><<<sph_blocks(n * 16, 256), 256, 0, st>>>(n, cap, cand, cidx, ccnt, h, R, stride, cnt, cnt0, idx,
                                                             ext, ext_cap, next, spread);
+    if (big.count > 0) {
+        const unsigned nb = (unsigned)big.count;
+        if (big.per > 0.0)
+            k_big_rows<true><<<nb, 64, 0, st>>>(big.list, big.x, big.y, big.z, h, R, ccnt, cap, big.nodes, big.bucket,
+                                                big.hscale, stride, cnt, cnt0, idx, ext, ext_cap, next, spread,
+                                                big.per);
+        else
+            k_big_rows<false><<<nb, 64, 0, st>>>(big.list, big.x, big.y, big.z, h, R, ccnt, cap, big.nodes, big.bucket,
+                                                 big.hscale, stride, cnt, cnt0, idx, ext, ext_cap, next, spread,
+                                                 big.per);
+    }
     k_ext_append<<<sph_blocks(ext_cap, 256), 256, 0, st>>>(ext_cap, ext, next, cnt, idx, stride, spread);
     k_row_tail_sort<<<sph_blocks(n, 256), 256, 0, st>>>(n, cnt0, cnt, idx, stride);
     spread_total(st, 2, spread, totals);
@@ -778,6 +877,7 @@ HSolveArgs hsolve_args(const HSolveParams &p)
     a.gnodes = p.nodes;
     a.big_list = p.mode == 0 ? p.big_list : nullptr;
     a.big_count = p.big_count;
+    a.per = p.per;
     return a;
 }
 
@@ -786,7 +886,10 @@ hipError_t hsolve_big(hipStream_t st, const HSolveParams &p, const HSolveArgs &a
 {
     if (p.n_big) *p.n_big = (int64_t)nbig;
     if (nbig == 0 || !a.big_list) return hipSuccess;
-    k_hsolve_big<<<(unsigned)nbig, 64, 0, st>>>(a, a.big_list);
+    if (a.per > 0.0)
+        k_hsolve_big<true><<<(unsigned)nbig, 64, 0, st>>>(a, a.big_list);
+    else
+        k_hsolve_big<false><<<(unsigned)nbig, 64, 0, st>>>(a, a.big_list);
     return hipGetLastError();
 }
 
@@ -795,7 +898,10 @@ void hsolve_round(hipStream_t st, const HSolveParams &p, const HSolveArgs &a)
 {
     const int64_t nblk = p.groups ? p.ngroups : (p.m + 63) / 64;
     fill<unsigned long long>(st, 5 * SPH_SPREAD_WORDS, p.counters, 0ull);
-    k_hsolve<<<(unsigned)nblk, 64, 0, st>>>(a);
+    if (a.per > 0.0)
+        k_hsolve<true><<<(unsigned)nblk, 64, 0, st>>>(a);
+    else
+        k_hsolve<false><<<(unsigned)nblk, 64, 0, st>>>(a);
     spread_total(st, 5, p.counters, p.totals);
 }
 
@@ -882,6 +988,7 @@ hipError_t hsolve(hipStream_t st, const HSolveParams &p, int max_rounds, int64_t
     if (a.big_list) fill<unsigned long long>(st, 1, p.big_count, 0ull);
     unsigned long long hc[6] = {0, 0, 0, 0, 0, 0};
     if (p.mode == 1) {
+        if (p.per > 0.0) return hipErrorInvalidValue;  // the count walk has open boundaries only
         fill<unsigned long long>(st, 5 * SPH_SPREAD_WORDS, p.counters, 0ull);
         k_hstep<<<sph_blocks(p.m, 256), 256, 0, st>>>(a);
         spread_total(st, 5, p.counters, p.totals);
@@ -920,12 +1027,16 @@ hipError_t hsolve(hipStream_t st, const HSolveParams &p, int max_rounds, int64_t
 hipError_t neighbours(hipStream_t st, int64_t n, int64_t ngroups, const int32_t *groups, const double *x,
                       const double *y, const double *z, const double *h, const WNode *nodes, int bucket, int32_t *cnt,
                       int64_t *off, int64_t *scan_tmp, unsigned long long *overflow, int32_t *idx, int64_t idx_cap,
-                      int64_t *total, const double *hscale)
+                      int64_t *total, const double *hscale, double per)
 {
     if (bucket > 64) return hipErrorInvalidValue;
     fill<unsigned long long>(st, 1, overflow, 0ull);
-    k_ngb<0><<<(unsigned)ngroups, 64, 0, st>>>(groups, x, y, z, h, nodes, bucket, cnt, nullptr, nullptr, 0, overflow,
-                                               nullptr, hscale);
+    if (per > 0.0)
+        k_ngb<0, true><<<(unsigned)ngroups, 64, 0, st>>>(groups, x, y, z, h, nodes, bucket, cnt, nullptr, nullptr, 0, overflow,
+                                               nullptr, hscale, per);
+    else
+        k_ngb<0, false><<<(unsigned)ngroups, 64, 0, st>>>(groups, x, y, z, h, nodes, bucket, cnt, nullptr, nullptr, 0, overflow,
+                                               nullptr, hscale, per);
     hipError_t e = scan_counts(st, n, cnt, off, scan_tmp);
     if (e != hipSuccess) return e;
     unsigned long long hov = 0;
@@ -937,30 +1048,43 @@ hipError_t neighbours(hipStream_t st, int64_t n, int64_t ngroups, const int32_t 
     if (e != hipSuccess) return e;
     if (hov) return hipErrorIllegalState;
     if (*total > idx_cap || idx == nullptr) return hipSuccess;  // caller grows idx and calls neighbours_fill
-    k_ngb<1><<<(unsigned)ngroups, 64, 0, st>>>(groups, x, y, z, h, nodes, bucket, nullptr, off, idx, 0, overflow,
-                                               nullptr, hscale);
+    if (per > 0.0)
+        k_ngb<1, true><<<(unsigned)ngroups, 64, 0, st>>>(groups, x, y, z, h, nodes, bucket, nullptr, off, idx, 0, overflow,
+                                               nullptr, hscale, per);
+    else
+        k_ngb<1, false><<<(unsigned)ngroups, 64, 0, st>>>(groups, x, y, z, h, nodes, bucket, nullptr, off, idx, 0, overflow,
+                                               nullptr, hscale, per);
     return hipGetLastError();
 }
 
 hipError_t neighbours_fill(hipStream_t st, int64_t ngroups, const int32_t *groups, const double *x, const double *y,
                            const double *z, const double *h, const WNode *nodes, int bucket, const int64_t *off,
-                           int32_t *idx, unsigned long long *overflow, const double *hscale)
+                           int32_t *idx, unsigned long long *overflow, const double *hscale, double per)
 {
-    k_ngb<1><<<(unsigned)ngroups, 64, 0, st>>>(groups, x, y, z, h, nodes, bucket, nullptr, off, idx, 0, overflow,
-                                               nullptr, hscale);
+    if (per > 0.0)
+        k_ngb<1, true><<<(unsigned)ngroups, 64, 0, st>>>(groups, x, y, z, h, nodes, bucket, nullptr, off, idx, 0, overflow,
+                                               nullptr, hscale, per);
+    else
+        k_ngb<1, false><<<(unsigned)ngroups, 64, 0, st>>>(groups, x, y, z, h, nodes, bucket, nullptr, off, idx, 0, overflow,
+                                               nullptr, hscale, per);
     return hipGetLastError();
 }
 
 hipError_t neighbour_rows(hipStream_t st, int64_t ngroups, const int32_t *groups, const double *x, const double *y,
                           const double *z, const double *h, const WNode *nodes, int bucket, int stride, int32_t *cnt,
                           int32_t *idx, unsigned long long *overflow, unsigned long long *spread,
-                          unsigned long long *totals, int64_t *n_long, int64_t *total, const double *hscale)
+                          unsigned long long *totals, int64_t *n_long, int64_t *total, const double *hscale,
+                          double per)
 {
     if (bucket > 64 || stride < 1) return hipErrorInvalidValue;
     fill<unsigned long long>(st, 1, overflow, 0ull);
     fill<unsigned long long>(st, 2 * SPH_SPREAD_WORDS, spread, 0ull);
-    k_ngb<2><<<(unsigned)ngroups, 64, 0, st>>>(groups, x, y, z, h, nodes, bucket, cnt, nullptr, idx, stride, overflow,
-                                               spread, hscale);
+    if (per > 0.0)
+        k_ngb<2, true><<<(unsigned)ngroups, 64, 0, st>>>(groups, x, y, z, h, nodes, bucket, cnt, nullptr, idx, stride, overflow,
+                                               spread, hscale, per);
+    else
+        k_ngb<2, false><<<(unsigned)ngroups, 64, 0, st>>>(groups, x, y, z, h, nodes, bucket, cnt, nullptr, idx, stride, overflow,
+                                               spread, hscale, per);
     spread_total(st, 2, spread, totals);
     unsigned long long hc[3] = {0, 0, 0};
     hipError_t e = hipMemcpyAsync(hc, totals, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);

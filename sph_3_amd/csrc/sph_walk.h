@@ -7,10 +7,13 @@
 // is breadth-first and lane-parallel: each step pops up to 64 node ids from an
 // LDS stack and every lane tests one node with the caller's prune predicate
 // (a conservative group-level test).  Children of surviving internal nodes are
-// pushed; surviving buckets (count <= bucket) have their particles staged in
-// LDS by the caller's `stage(slot, j)` and, once 64 or more are staged, the
-// caller's `eval(cnt)` runs on all lanes over the staged slots.  No lane
-// diverges from the wave's control flow.
+// pushed; the particles of surviving buckets (count <= bucket) are staged in
+// LDS by the caller's `stage(slot, j)`, all buckets of a step at once (each lane
+// finds its particle's bucket from the step's bucket prefix sums, so the loads
+// of a step are issued together instead of one bucket after another), in
+// bucket order; whenever the WSRC slots are full the caller's `eval(cnt)` runs
+// on all lanes over the staged slots.  No lane diverges from the wave's
+// control flow.
 #pragma once
 #include "sph_common.h"
 
@@ -46,9 +49,38 @@ __device__ __forceinline__ double box_box_dist2(const double *gb, const WNode *_
     return ex * ex + ey * ey + ez * ez;
 }
 
+// The same for a periodic box of side L (PER = false: box_box_dist2): per axis the
+// smallest gap over the node's images at -L, 0, +L, less 4 ulp of L (the wraps of the
+// pair differences round), so no pair of points of the boxes has a computed
+// minimum-image distance below it.
+template <bool PER>
+__device__ __forceinline__ double box_box_dist2_p(const double *gb, const WNode *__restrict__ nd, double L)
+{
+#pragma clang fp contract(off)
+    if constexpr (!PER) return box_box_dist2(gb, nd);
+    const double lo[3] = {nd->lx, nd->ly, nd->lz}, hi[3] = {nd->hx, nd->hy, nd->hz};
+    const double slack = 8.9e-16 * L;
+    double s = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double e = INFINITY;
+#pragma unroll
+        for (int k = -1; k <= 1; k++) {
+            const double a = lo[d] + k * L, b = hi[d] + k * L;
+            double g = 0.0;
+            if (a - gb[3 + d] > g) g = a - gb[3 + d];
+            if (gb[d] - b > g) g = gb[d] - b;
+            e = fmin(e, g);
+        }
+        e = fmax(e - slack, 0.0);
+        s = s + e * e;
+    }
+    return s;
+}
+
 struct WalkLDS {
     int stk[WSTK];
-    int bfirst[64], bcount[64];
+    int bfirst[64], bstart[65];
 };
 
 // Returns true if the stack ever overflowed (impossible for trees shallower
@@ -91,19 +123,37 @@ __device__ __forceinline__ bool group_walk(const WNode *__restrict__ nodes, int 
         if (sp + 2 * (int)__popcll(mo) > WSTK) overflow = true;
         sp += 2 * (int)__popcll(mo);
         if (sp > WSTK) sp = WSTK;
+        // bucket prefix sums in ballot (= lane) order by a wave scan of the counts
+        int incl = bkt ? ncount : 0;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
+        }
+        const int total = __shfl(incl, 63);
         if (bkt) {
             const int pb = (int)__popcll(mb & lt);
             L.bfirst[pb] = nfirst;
-            L.bcount[pb] = ncount;
+            L.bstart[pb] = incl - ncount;
         }
         const int nb = (int)__popcll(mb);
+        if (lane == 0) L.bstart[nb] = total;
         __syncthreads();
-        for (int b = 0; b < nb; b++) {
-            const int f = L.bfirst[b], c = L.bcount[b];
-            if (lane < c) stage(nsrc + lane, f + lane);
-            nsrc += c;
+        for (int s0 = 0; s0 < total;) {
+            const int take = (WSRC - nsrc) < (total - s0) ? (WSRC - nsrc) : (total - s0);
+            for (int k = lane; k < take; k += 64) {
+                const int sidx = s0 + k;
+                int lo = 0, hi = nb - 1;  // last bucket b with bstart[b] <= sidx
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (L.bstart[mid] <= sidx) lo = mid;
+                    else hi = mid - 1;
+                }
+                stage(nsrc + k, L.bfirst[lo] + (sidx - L.bstart[lo]));
+            }
+            nsrc += take;
+            s0 += take;
             __syncthreads();
-            if (nsrc >= 64) {
+            if (nsrc == WSRC) {
                 eval(nsrc);
                 nsrc = 0;
                 __syncthreads();

@@ -118,3 +118,27 @@ def agn_cloud(n_cloud: int, n_wind: int, n_grid: int = 256, sigma_lnrho: float =
 def u_of_T(T: float, unit_u_cgs: float, mu: float = 0.6, gamma: float = 5.0 / 3.0) -> float:
     """Specific energy in code units of gas at temperature T [K] with mean molecular weight mu."""
     return 1.3807e-16 * T / ((gamma - 1.0) * mu * 1.6726e-24) / unit_u_cgs
+
+
+def sedov(n_side: int, jitter: float = 0.1, E: float = 1.0, n_deposit: int = 64, u0: float = 1e-5,
+          rho0: float = 1.0, seed: int = 50, n_ngb: float = 64.0):
+    """3D Sedov blast in the unit periodic box [0, 1)^3: a cubic lattice of n_side^3
+    particles with a seeded uniform jitter of +-jitter lattice spacings per axis
+    (wrapped into the box), density rho0 (m = rho0 / N), specific energy u0, and
+    the energy E deposited kernel-weighted on the n_deposit particles nearest the
+    centre: u_i += E w_i / (m sum_j w_j) with w = W(r, h_d) of the cubic spline and
+    2 h_d the distance of the (n_deposit+1)-th nearest particle.  Returns dict(pos,
+    vel, m, u, h0) with h0 the lattice value at the target neighbour number."""
+    pos = uniform_cube(n_side, seed=seed, jitter=jitter) % 1.0
+    n = len(pos)
+    m = np.full(n, rho0 / n)
+    u = np.full(n, u0)
+    d = np.linalg.norm(pos - 0.5, axis=1)
+    order = np.argsort(d, kind="stable")
+    near = order[:n_deposit]
+    hd = 0.5 * d[order[n_deposit]]
+    q = d[near] / hd
+    w = np.where(q < 1.0, 1.0 - 1.5 * q * q + 0.75 * q ** 3, np.where(q < 2.0, 0.25 * (2.0 - q) ** 3, 0.0))
+    u[near] += E * w / (m[near] * w.sum())
+    h0 = np.full(n, 0.5 * np.cbrt(3.0 * n_ngb * (rho0 / n) / (4.0 * np.pi * rho0)))
+    return dict(pos=pos, vel=np.zeros((n, 3)), m=m, u=u, h0=h0)

@@ -31,10 +31,12 @@ class StepConfig:
     h_tol: float = 1e-10
     precision: str = "mixed"
     balsara: bool = True
+    period: float = 0.0  # > 0: periodic box [0, period)^3 (minimum image; needs G = 0)
 
     def params(self, timing=False) -> StepParams:
         return StepParams(self.dt, self.gamma, self.alpha, self.G, self.theta, self.eps_plummer, self.n_ngb,
-                          self.h_tol, SOFT[self.softening], PREC[self.precision], int(self.balsara), int(timing))
+                          self.h_tol, self.period, SOFT[self.softening], PREC[self.precision], int(self.balsara),
+                          int(timing))
 
 
 class DeviceSPH:

@@ -306,3 +306,31 @@ def test_csr_sums_equal_allpairs():
     u1 = O.dudt_allpairs(pos, v, rho, P, c, h, m, 1.0, dv, cv)
     u2 = O.dudt_csr(pos, v, rho, P, c, h, m, 1.0, off, idx, dv, cv)
     assert np.abs(u1 - u2).max() < 1e-12 * np.abs(u1).max()
+
+
+def test_periodic_minimum_image_oracle():
+    """Periodic box: on a perfect cubic lattice every particle has the same number of
+    neighbours and the same density (translation invariance holds only with the
+    wrap); a particle at the face sees its neighbours across the boundary; a set
+    shifted by whole lattice spacings (exact in FP64) is the same set."""
+    n = 8
+    g = (np.arange(n) + 0.5) / n
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    pos = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    N = len(pos)
+    h = np.full(N, 1.3 / n)
+    m = np.full(N, 1.0 / N)
+    with O.periodic(1.0):
+        off, idx = O.neighbours_brute(pos, h)
+        cnt = np.diff(off)
+        assert cnt.min() == cnt.max() and cnt[0] > 1
+        rho = O.density_csr(pos, m, h, off, idx)
+        assert np.max(np.abs(rho / rho[0] - 1)) < 1e-12
+        h2, bad = O.h_newton(pos, h, n_ngb=40.0, tol=1e-13)
+        assert bad == 0 and np.max(np.abs(h2 / h2[0] - 1)) < 1e-9
+        shift = np.roll(np.arange(n), 1)  # one lattice spacing along x, wrapped
+        pos_s = pos.copy(); pos_s[:, 0] = g[shift][np.searchsorted(g, pos[:, 0])]
+        off_s, idx_s = O.neighbours_brute(pos_s, h)
+        assert np.array_equal(np.diff(off_s), cnt)
+    off_open, _ = O.neighbours_brute(pos, h)
+    assert np.diff(off_open).min() < cnt[0]  # open boundaries: faces have fewer neighbours
