@@ -114,6 +114,64 @@ def cpu_baseline(sim_state, n, m, rows, seed=0):
     return step_s, sample_s, t
 
 
+def cpu_tree_oracle(sim_state, m, cfg):
+    """The oracle's own FP64 tree path (not the reference's algorithm): tree neighbour
+    sets, CSR density, div/curl, acceleration, du/dt and group-walk tree gravity at the
+    device state's positions and h (the h solve is not timed: the oracle's Newton is
+    all-pairs).  Returns (seconds, per-function seconds)."""
+    import oracle as O
+
+    pos = sim_state["pos"]; vel = sim_state["vel"]; h = sim_state["h"]; u = sim_state["u"]
+    t = {}
+    t0 = time.perf_counter()
+    off, idx = O.Tree(pos, m, h=h).neighbours()
+    t["tree_neighbours"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rho = O.density_csr(pos, m, h, off, idx)
+    t["density"] = time.perf_counter() - t0
+    P = (cfg.gamma - 1) * rho * u
+    c = np.sqrt(cfg.gamma * (cfg.gamma - 1) * u)
+    t0 = time.perf_counter()
+    dv, cv = O.divcurl_csr(pos, vel, rho, m, h, off, idx)
+    t["divcurl"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.accsph_csr(pos, vel, rho, P, c, h, m, cfg.alpha, off, idx, dv, cv)
+    t["accsph"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.dudt_csr(pos, vel, rho, P, c, h, m, cfg.alpha, off, idx, dv, cv)
+    t["dudt"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.Tree(pos, m).gravity(theta=cfg.theta, G=cfg.G, soft_kind=O.SOFT_PLUMMER, eps_plummer=cfg.eps_plummer)
+    t["tree_gravity"] = time.perf_counter() - t0
+    return sum(t.values()), t
+
+
+def c1_section(cfg, run):
+    """The CPU-runnable Evrard case, N = 10,000: one full step of the FP64 all-pairs
+    restatement of the reference step (every row, no sampling), the oracle's tree path,
+    and the device step at the same N (launch-bound at this size)."""
+    from sph_3_amd import ic
+
+    n = 10000
+    d = ic.evrard(n, seed=42)
+    sim, dt = run("mixed", 5, 2, d=d)
+    g = sim.get()
+    del sim
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    step_s, sample_s, parts = cpu_baseline(g, n, d["m"], n)
+    tree_s, tree_parts = cpu_tree_oracle(g, d["m"], cfg)
+    return {
+        "workload": "evrard_collapse, n = 10000", "n": n, "gpu_ms_per_step": 1e3 * dt / 5,
+        "gpu_value": n * 5 / dt,
+        "cpu_reference_step": {"seconds": step_s, "value": n / step_s, "unit": "particle-steps/s", "cores": threads,
+                               "kind": "port", "sample": "all 10000 rows of the all-pairs restatement (a full step)",
+                               "function_seconds": parts},
+        "cpu_tree_oracle_step": {"seconds": tree_s, "value": n / tree_s, "unit": "particle-steps/s", "cores": threads,
+                                 "kind": "oracle tree path (not the reference's algorithm), without the h solve",
+                                 "function_seconds": tree_parts},
+    }
+
+
 MLP_FLOP_PER_SAMPLE = 2 * (4 * 128 + 2 * 128 * 128 + 128)  # forward multiply-adds x 2
 BF16_MFMA_PEAK_TFLOPS = 2500.0
 
@@ -499,6 +557,16 @@ def main():
             "function_seconds_sample": parts,
         }
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
+        log(f"[bench] cpu tree oracle at n = {n} on {threads} threads")
+        tree_s, tree_parts = cpu_tree_oracle(g, d["m"], cfg)
+        out["cpu_tree_oracle"] = {
+            "value": n / tree_s, "unit": "particle-steps/s", "cores": threads,
+            "kind": "oracle tree path (not the reference's algorithm): tree neighbours, CSR hydro, group-walk "
+                    "gravity at the device's h, without the h solve; every row",
+            "seconds": tree_s, "function_seconds": tree_parts,
+        }
+        log("[bench] the cpu-runnable case, n = 10000")
+        out["evrard_10k"] = c1_section(cfg, run)
     if not args.no_cooling_step:
         out["evrard_cooling"] = cooling_section(args, cfg, run)
     if not args.no_agn:
