@@ -508,10 +508,8 @@ int hsolve_params(sph_ctx *ctx, int64_t m, const double *tx, const double *ty, c
                   int64_t *work, bool keep_indices, sph::HSolveParams &p)
 {
     CK(ensure(ctx, ctx->cand, (size_t)m * SPH_HCAP * 8));
-    if (keep_indices) {
-        CK(ensure(ctx, ctx->ccidx, (size_t)m * SPH_HCAP * 4));
-        CK(ensure(ctx, ctx->cccnt, (size_t)m * 4));
-    }
+    if (keep_indices) CK(ensure(ctx, ctx->ccidx, (size_t)m * SPH_HCAP * 4));
+    CK(ensure(ctx, ctx->cccnt, (size_t)m * 4));
     p.m = m;
     p.tx = tx; p.ty = ty; p.tz = tz;
     p.x = D(x); p.y = D(y); p.z = D(z);
@@ -538,7 +536,7 @@ int hsolve_params(sph_ctx *ctx, int64_t m, const double *tx, const double *ty, c
     p.rounds = rounds_out;
     p.work = work;
     p.cidx = keep_indices ? I32(ccidx) : nullptr;
-    p.ccnt = keep_indices ? I32(cccnt) : nullptr;
+    p.ccnt = I32(cccnt);
     p.host_totals = ctx->hpin;
     p.done = ctx->ev_h;
     if (mode == SPH_H_NEWTON) {

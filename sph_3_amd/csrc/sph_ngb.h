@@ -23,8 +23,9 @@ struct HSolveParams {
     const double *h_in;          // start values (device)
     double *h_out;               // result (device)
     double *cand, *R;            // scratch: cap*m, m
-    int32_t *cidx = nullptr;     // optional: cap*m candidate indices and m counts of the final rows
-    int32_t *ccnt = nullptr;     //   (targets must be the sorted sources; see neighbour_rows_from_candidates)
+    int32_t *cidx = nullptr;     // optional: cap*m candidate indices of the final rows (targets must
+                                 //   be the sorted sources; see neighbour_rows_from_candidates)
+    int32_t *ccnt = nullptr;     // m candidate counts (required by modes 0 and 2)
     int32_t *steps, *state;      // scratch: m, m
     unsigned long long *counters;  // scratch: spread array of 5 counters (5 * SPH_SPREAD_WORDS words)
     unsigned long long *totals;    // scratch: 5 words

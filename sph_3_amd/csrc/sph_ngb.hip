@@ -116,20 +116,20 @@ __device__ __forceinline__ void group_range(const int32_t *groups, const WNode *
 }
 
 template <bool PER>
-__global__ void __launch_bounds__(64) k_hsolve(HSolveArgs a)
+__global__ void __launch_bounds__(64) k_hwalk(HSolveArgs a)
 {
     __shared__ WalkLDS L;
     __shared__ double sx[WSRC], sy[WSRC], sz[WSRC];
-    __shared__ double buf[64][9];  // per-lane staging of candidate r^2 (padded row)
-    __shared__ int bufj[64][9];    // and of their indices
     __shared__ int sj[WSRC];
+    __shared__ double ttx[64], tty[64], ttz[64], tR2[64];  // the group's targets (R^2 < 0: none)
     int64_t first;
     int ng;
     group_range(a.groups, a.gnodes, a.m, first, ng);
     const int lane = threadIdx.x;
     const int64_t t = first + lane;
     const bool valid = lane < ng && a.state[t] == 0;
-    if (__ballot(valid) == 0) return;  // whole group decided (uniform)
+    const uint64_t vmask = __ballot(valid);
+    if (vmask == 0) return;  // whole group decided (uniform)
     double xi = 0, yi = 0, zi = 0, R = 0;
     if (valid) {
         xi = a.tx[t]; yi = a.ty[t]; zi = a.tz[t];
@@ -140,11 +140,15 @@ __global__ void __launch_bounds__(64) k_hsolve(HSolveArgs a)
     double Rg = R;
     for (int off = 32; off >= 1; off >>= 1) Rg = fmax(Rg, __shfl_xor(Rg, off));
     const double R2 = R * R, Rg2 = Rg * Rg;
+    ttx[lane] = xi; tty[lane] = yi; ttz[lane] = zi;
+    tR2[lane] = valid ? R2 : -1.0;
     const int cap = a.cap;
-    double *row = a.cand + (valid ? t : 0) * (int64_t)cap;
-    int32_t *rowj = a.cidx ? a.cidx + (valid ? t : 0) * (int64_t)cap : nullptr;
-    int c = 0;
-    long long ntest = 0, nterm = 0;  // algorithmic work (distance tests, kernel terms)
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    int c = 0;  // lane t: candidates of target t so far
+    long long ntest = 0;  // algorithmic work: distance tests
+    // Candidates: lanes take the staged sources, the group's targets in turn (their
+    // coordinates and radius are wave-uniform LDS reads); the hits of a target are
+    // compacted by a ballot into its row, so a row keeps the staging order.
     const bool ovf = group_walk(
         a.nodes, a.bucket, L, [&](const WNode *nd) { return box_box_dist2_p<PER>(gb, nd, a.per) >= Rg2; },
         [&](int q, int j) {
@@ -155,216 +159,193 @@ __global__ void __launch_bounds__(64) k_hsolve(HSolveArgs a)
         },
         [&](int cnt) {
             if (valid) ntest += cnt;
-            if (valid)
-                for (int k = 0; k < cnt; k++) {
-                    double dx = wrap_d<PER>(sx[k] - xi, a.per), dy = wrap_d<PER>(sy[k] - yi, a.per),
-                           dz = wrap_d<PER>(sz[k] - zi, a.per);
-                    double r2 = dx * dx + dy * dy + dz * dz;
-                    if (r2 < R2) {
-                        const int b = c & 7;
-                        buf[lane][b] = r2;
-                        bufj[lane][b] = sj[k];
-                        if (b == 7 && c < cap) {
-#pragma unroll
-                            for (int u = 0; u < 8; u++) row[c - 7 + u] = buf[lane][u];
-                            if (rowj)
-#pragma unroll
-                                for (int u = 0; u < 8; u++) rowj[c - 7 + u] = bufj[lane][u];
+            for (int k0 = 0; k0 < cnt; k0 += 64) {
+                const int k = k0 + lane;
+                const bool src = k < cnt;
+                const double px = src ? sx[k] : 0.0, py = src ? sy[k] : 0.0, pz = src ? sz[k] : 0.0;
+                const int pj = src ? sj[k] : 0;
+                for (uint64_t todo = vmask; todo; todo &= todo - 1) {
+                    const int tt = __builtin_ctzll(todo);
+                    const double dx = wrap_d<PER>(px - ttx[tt], a.per), dy = wrap_d<PER>(py - tty[tt], a.per),
+                                 dz = wrap_d<PER>(pz - ttz[tt], a.per);
+                    const double r2 = dx * dx + dy * dy + dz * dz;
+                    const bool hit = src && r2 < tR2[tt];
+                    const uint64_t hm = __ballot(hit);
+                    if (hm) {
+                        const int ct = __builtin_amdgcn_readlane(c, tt);
+                        if (hit) {
+                            const int pos = ct + (int)__popcll(hm & below);
+                            if (pos < cap) {
+                                const int64_t o = (first + tt) * (int64_t)cap + pos;
+                                a.cand[o] = r2;
+                                if (a.cidx) a.cidx[o] = pj;
+                            }
                         }
-                        c++;
+                        if (lane == tt) c += (int)__popcll(hm);
                     }
                 }
+            }
         });
     if (ovf && lane == 0) spread_add(a.pending, 2, 1ull);
-    const bool over = valid && c > cap;
-    if (valid && !over) {
-        for (int k = c & ~7; k < c; k++) row[k] = buf[lane][k & 7];
-        if (rowj) {
-            for (int k = c & ~7; k < c; k++) rowj[k] = bufj[lane][k & 7];
-            a.ccnt[t] = c;
-        }
+    if (valid) a.ccnt[t] = c;  // > cap: the row overflowed (k_hnewton shrinks R)
+    for (int off = 32; off >= 1; off >>= 1) ntest += __shfl_xor(ntest, off);
+    if (lane == 0) spread_add(a.pending, 3, (unsigned long long)ntest);
+}
+
+// Newton (mode 0) or k-th distance (mode 2) solve of the pending targets from their
+// candidate rows, 8 targets per wavefront and 8 lanes per target: lane l of a
+// sub-group holds candidates l, l+8, ... (up to 32, in registers) and the sub-group
+// reduces with butterflies, which give its 8 lanes bitwise the same sums, so each
+// sub-group's iteration is uniform.  A target whose row overflowed gets R * 0.75, one
+// whose solution lies beyond R/2 gets R * 1.5, and stays pending for another round.
+__global__ void __launch_bounds__(64) k_hnewton(HSolveArgs a)
+{
+    constexpr int HL = 8, HS = SPH_HCAP_SLOTS / HL;
+    const int lane = threadIdx.x, lsub = lane % HL;
+    const int64_t j = (int64_t)blockIdx.x * (64 / HL) + lane / HL;
+    const bool valid = j < a.m && a.state[j] == 0;
+    if (__ballot(valid) == 0) return;
+    const int cap = a.cap;
+    const int cjs = valid ? a.ccnt[j] : 0;
+    const bool over = valid && cjs > cap;
+    const bool solve = valid && !over;
+    const int cj = solve ? cjs : 0;
+    const double Rj = valid ? a.R[j] : 0.0, hj0 = valid ? a.h[j] : 0.0;
+    const double *rowj = a.cand + (solve ? j : 0) * (int64_t)cap;
+    const int nsl = (cj + HL - 1) / HL;  // uniform within the sub-group
+    long long nterm = 0;
+    double r[HS];
+#pragma unroll
+    for (int u = 0; u < HS; u++) {
+        const int k = lsub + HL * u;
+        r[u] = (k < cj) ? rowj[k] : INFINITY;  // squared distances
     }
-    __syncthreads();  // rows written by each lane are read by the whole wave below
-    // Solve HB targets at a time, HL lanes per target: lane l of the sub-group
-    // holds candidates l, l+HL, ... (up to HS, in registers) and the sub-group
-    // reduces with butterflies, which give its HL lanes bitwise the same sums,
-    // so each sub-group's iteration is uniform.
-    constexpr int HL = 8, HB = 64 / HL, HS = SPH_HCAP_SLOTS / HL;
-    double h_res = valid ? a.h[t] : 0.0, R_res = R;
-    int st_res = 0;
-    if (over) R_res = 0.75 * R;
-    uint64_t todo = __ballot(valid && !over);
-    const int sub = lane / HL, lsub = lane % HL;
-    while (todo) {
-        // targets of this batch: the next (up to) HB set bits
-        int js[HB];
+    double hnew = hj0, Rnew = Rj;
+    int st = 0;
+    if (solve && a.mode == 0) {
+        // distances; empty slots get 4 R, beyond the support of every h <= R/2
 #pragma unroll
-        for (int q = 0; q < HB; q++) {
-            js[q] = todo ? __builtin_ctzll(todo) : -1;
-            if (todo) todo &= todo - 1;
-        }
-        int j = -1;
+        for (int u = 0; u < HS; u++)
+            if (u < nsl) r[u] = (r[u] < INFINITY) ? __builtin_sqrt(r[u]) : 4.0 * Rj;
+        // f(h) = (32/3) sum_j w(r_j/h) - n_ngb and f'(h), with the branch-free
+        // cubic spline w(q) = (2-q)_+^3/4 - (1-q)_+^3 and q w'(q) = q (3 (1-q)_+^2 - 3/4 (2-q)_+^2)
+        auto feval = [&](double hh, double &f, double &df) {
+            if (lsub == 0) nterm += cj;
+            double sl = 0.0, sq = 0.0;
+            const double ih = 1.0 / hh;
 #pragma unroll
-        for (int q = 0; q < HB; q++)
-            if (q == sub) j = js[q];
-        const int jj = j < 0 ? 0 : j;
-        // every lane executes the cross-lane reads (a lane that is switched off
-        // at a permute would not supply its value)
-        const int cjs = __shfl(c, jj);
-        const double Rj = __shfl(R, jj), hj0 = __shfl(h_res, jj);
-        const int cj = j < 0 ? 0 : cjs;
-        const double *rowj = a.cand + (first + jj) * (int64_t)cap;
-        const int nsl = (cj + HL - 1) / HL;  // uniform within the sub-group
-        double r[HS];
-#pragma unroll
-        for (int u = 0; u < HS; u++) {
-            const int k = lsub + HL * u;
-            r[u] = (k < cj) ? rowj[k] : INFINITY;  // squared distances
-        }
-        double hnew = hj0, Rnew = Rj;
-        int st = 0;
-        if (j >= 0 && a.mode == 0) {
-            // distances; empty slots get 4 R, beyond the support of every h <= R/2
-#pragma unroll
-            for (int u = 0; u < HS; u++)
-                if (u < nsl) r[u] = (r[u] < INFINITY) ? __builtin_sqrt(r[u]) : 4.0 * Rj;
-            // f(h) = (32/3) sum_j w(r_j/h) - n_ngb and f'(h), with the branch-free
-            // cubic spline w(q) = (2-q)_+^3/4 - (1-q)_+^3 and q w'(q) = q (3 (1-q)_+^2 - 3/4 (2-q)_+^2)
-            auto feval = [&](double hh, double &f, double &df) {
-                if (lsub == 0) nterm += cj;
-                double sl = 0.0, sq = 0.0;
-                const double ih = 1.0 / hh;
-#pragma unroll
-                for (int u = 0; u < HS; u++) {
-                    if (u >= nsl) break;
-                    const double q = r[u] * ih;
-                    const double aa = fmax(2.0 - q, 0.0), bb = fmax(1.0 - q, 0.0);
-                    const double a2 = aa * aa, b2 = bb * bb;
-                    sl = fma(0.25 * a2, aa, fma(-b2, bb, sl));
-                    sq = fma(fma(3.0, b2, -0.75 * a2), q, sq);
-                }
-                for (int off = HL / 2; off >= 1; off >>= 1) {
-                    sl += __shfl_xor(sl, off, HL);
-                    sq += __shfl_xor(sq, off, HL);
-                }
-                f = (32.0 / 3.0) * sl - a.n_ngb;
-                df = -(32.0 / 3.0) * sq * ih;
-            };
-            // Safeguarded Newton from the previous h on the bracket (0, R/2]: the
-            // candidates (r < R) are complete for every h <= R/2.  f(R/2) is only
-            // evaluated when no iterate has shown f >= 0 and Newton leaves the
-            // bracket upwards; f(R/2) < 0 means the root lies beyond: grow R.
-            // Stop after a Newton step dh <= sqrt(tol/10) h: in the quadratic
-            // regime the error after it is K dh^2 <= tol h for curvature
-            // K = f''/(2 f') <= 10/h (K = 1/h for a uniform density), or after
-            // any step dh <= tol h.
-            const double hmaxv = 0.5 * Rj, qtol = __builtin_sqrt(0.1 * a.tol);
-            double lo = 0.0, hi = hmaxv, hh = hj0, f, df;
-            if (!(hh > 0.0 && hh < hi)) hh = 0.5 * hi;
-            bool hik = false, grow = false;
-            st = 2;
-            for (int it = 0; it < 100; it++) {
-                feval(hh, f, df);
-                if (f < 0.0) lo = hh;
-                else {
-                    hi = hh;
-                    hik = true;
-                }
-                double hn = (df > 0.0) ? hh - f / df : -1.0;
-                bool newton = true;
-                if (!(hn > lo && hn < hi)) {
-                    if (!hik) {
-                        double fm, dfm;
-                        feval(hmaxv, fm, dfm);
-                        if (fm < 0.0) {
-                            grow = true;
-                            break;
-                        }
-                        hik = true;
-                        hn = (dfm > 0.0) ? hmaxv - fm / dfm : -1.0;
-                        if (!(hn > lo && hn < hi)) hn = 0.5 * (lo + hi);
-                    } else {
-                        hn = 0.5 * (lo + hi);
+            for (int u = 0; u < HS; u++) {
+                if (u >= nsl) break;
+                const double q = r[u] * ih;
+                const double aa = fmax(2.0 - q, 0.0), bb = fmax(1.0 - q, 0.0);
+                const double a2 = aa * aa, b2 = bb * bb;
+                sl = fma(0.25 * a2, aa, fma(-b2, bb, sl));
+                sq = fma(fma(3.0, b2, -0.75 * a2), q, sq);
+            }
+            for (int off = HL / 2; off >= 1; off >>= 1) {
+                sl += __shfl_xor(sl, off, HL);
+                sq += __shfl_xor(sq, off, HL);
+            }
+            f = (32.0 / 3.0) * sl - a.n_ngb;
+            df = -(32.0 / 3.0) * sq * ih;
+        };
+        // Safeguarded Newton from the previous h on the bracket (0, R/2]: the
+        // candidates (r < R) are complete for every h <= R/2.  f(R/2) is only
+        // evaluated when no iterate has shown f >= 0 and Newton leaves the
+        // bracket upwards; f(R/2) < 0 means the root lies beyond: grow R.
+        // Stop after a Newton step dh <= sqrt(tol/10) h: in the quadratic
+        // regime the error after it is K dh^2 <= tol h for curvature
+        // K = f''/(2 f') <= 10/h (K = 1/h for a uniform density), or after
+        // any step dh <= tol h.
+        const double hmaxv = 0.5 * Rj, qtol = __builtin_sqrt(0.1 * a.tol);
+        double lo = 0.0, hi = hmaxv, hh = hj0, f, df;
+        if (!(hh > 0.0 && hh < hi)) hh = 0.5 * hi;
+        bool hik = false, grow = false;
+        st = 2;
+        for (int it = 0; it < 100; it++) {
+            feval(hh, f, df);
+            if (f < 0.0) lo = hh;
+            else {
+                hi = hh;
+                hik = true;
+            }
+            double hn = (df > 0.0) ? hh - f / df : -1.0;
+            bool newton = true;
+            if (!(hn > lo && hn < hi)) {
+                if (!hik) {
+                    double fm, dfm;
+                    feval(hmaxv, fm, dfm);
+                    if (fm < 0.0) {
+                        grow = true;
+                        break;
                     }
-                    newton = false;
+                    hik = true;
+                    hn = (dfm > 0.0) ? hmaxv - fm / dfm : -1.0;
+                    if (!(hn > lo && hn < hi)) hn = 0.5 * (lo + hi);
+                } else {
+                    hn = 0.5 * (lo + hi);
                 }
-                const double dh = fabs(hn - hh);
-                hh = hn;
-                if (dh <= a.tol * hh || (newton && dh <= qtol * hh)) {
-                    st = 1;
-                    break;
-                }
+                newton = false;
             }
-            if (grow) {
-                Rnew = 1.5 * Rj;
-                st = 0;
-            } else {
-                hnew = hh;
-            }
-        } else if (j >= 0) {
-            // (kth+1)-th smallest squared distance: ascending distinct values with multiplicity
-            const int need = a.kth + 1;
-            if (cj < need) {
-                Rnew = 1.5 * Rj;
-            } else {
-                double cur = -1.0, v = 0.0;
-                int taken = 0;
-                for (;;) {
-                    double mn = INFINITY;
-#pragma unroll
-                    for (int u = 0; u < HS; u++)
-                        if (r[u] > cur && r[u] < mn) mn = r[u];
-                    for (int off = HL / 2; off >= 1; off >>= 1) mn = fmin(mn, __shfl_xor(mn, off, HL));
-                    int mult = 0;
-#pragma unroll
-                    for (int u = 0; u < HS; u++) mult += (r[u] == mn);
-                    for (int off = HL / 2; off >= 1; off >>= 1) mult += __shfl_xor(mult, off, HL);
-                    if (taken + mult >= need) { v = mn; break; }
-                    taken += mult;
-                    cur = mn;
-                }
-                hnew = __builtin_sqrt(v) * 0.5;
+            const double dh = fabs(hn - hh);
+            hh = hn;
+            if (dh <= a.tol * hh || (newton && dh <= qtol * hh)) {
                 st = 1;
+                break;
             }
         }
-        // hand each sub-group's result to its target lane
+        if (grow) {
+            Rnew = 1.5 * Rj;
+            st = 0;
+        } else {
+            hnew = hh;
+        }
+    } else if (solve) {
+        // (kth+1)-th smallest squared distance: ascending distinct values with multiplicity
+        const int need = a.kth + 1;
+        if (cj < need) {
+            Rnew = 1.5 * Rj;
+        } else {
+            double cur = -1.0, v = 0.0;
+            int taken = 0;
+            for (;;) {
+                double mn = INFINITY;
 #pragma unroll
-        for (int q = 0; q < HB; q++) {
-            const double hq = __shfl(hnew, HL * q), Rq = __shfl(Rnew, HL * q);
-            const int sq = __shfl(st, HL * q);
-            if (js[q] >= 0 && lane == js[q]) {
-                h_res = hq;
-                R_res = Rq;
-                st_res = sq;
+                for (int u = 0; u < HS; u++)
+                    if (r[u] > cur && r[u] < mn) mn = r[u];
+                for (int off = HL / 2; off >= 1; off >>= 1) mn = fmin(mn, __shfl_xor(mn, off, HL));
+                int mult = 0;
+#pragma unroll
+                for (int u = 0; u < HS; u++) mult += (r[u] == mn);
+                for (int off = HL / 2; off >= 1; off >>= 1) mult += __shfl_xor(mult, off, HL);
+                if (taken + mult >= need) { v = mn; break; }
+                taken += mult;
+                cur = mn;
             }
+            hnew = __builtin_sqrt(v) * 0.5;
+            st = 1;
         }
     }
-    // work totals: one spread atomic per wave
-    for (int off = 32; off >= 1; off >>= 1) {
-        ntest += __shfl_xor(ntest, off);
-        nterm += __shfl_xor(nterm, off);
-    }
-    if (lane == 0) {
-        spread_add(a.pending, 3, (unsigned long long)ntest);
-        spread_add(a.pending, 4, (unsigned long long)nterm);
-    }
-    if (!valid) return;
+    if (over) Rnew = 0.75 * Rj;
+    for (int off = 32; off >= 1; off >>= 1) nterm += __shfl_xor(nterm, off);
+    if (lane == 0) spread_add(a.pending, 4, (unsigned long long)nterm);
+    if (!valid || lsub != 0) return;
     if (a.mode == 0 && a.big_list) {
         // a target that must grow R after its candidates overflowed a row needs more
         // candidates than a row holds at its solution: k_hsolve_big solves it
-        const int fl = a.steps[t];
+        const int fl = a.steps[j];
         if (over) {
-            a.steps[t] = fl | 1;
-        } else if (st_res == 0 && (fl & 1)) {
-            st_res = 3;
-            a.big_list[atomicAdd(a.big_count, 1ull)] = (int32_t)t;
+            a.steps[j] = fl | 1;
+        } else if (st == 0 && (fl & 1)) {
+            st = 3;
+            a.big_list[atomicAdd(a.big_count, 1ull)] = (int32_t)j;
         }
     }
-    a.h[t] = h_res;
-    a.R[t] = R_res;
-    a.state[t] = st_res;
-    if (st_res == 0) spread_add(a.pending, 0, 1ull);
-    if (st_res == 2) spread_add(a.pending, 1, 1ull);
+    a.h[j] = hnew;
+    a.R[j] = Rnew;
+    a.state[j] = st;
+    if (st == 0) spread_add(a.pending, 0, 1ull);
+    if (st == 2) spread_add(a.pending, 1, 1ull);
 }
 
 // Newton solve of targets whose solution needs more candidates than a row holds (a
@@ -899,9 +880,10 @@ void hsolve_round(hipStream_t st, const HSolveParams &p, const HSolveArgs &a)
     const int64_t nblk = p.groups ? p.ngroups : (p.m + 63) / 64;
     fill<unsigned long long>(st, 5 * SPH_SPREAD_WORDS, p.counters, 0ull);
     if (a.per > 0.0)
-        k_hsolve<true><<<(unsigned)nblk, 64, 0, st>>>(a);
+        k_hwalk<true><<<(unsigned)nblk, 64, 0, st>>>(a);
     else
-        k_hsolve<false><<<(unsigned)nblk, 64, 0, st>>>(a);
+        k_hwalk<false><<<(unsigned)nblk, 64, 0, st>>>(a);
+    k_hnewton<<<(unsigned)((p.m + 7) / 8), 64, 0, st>>>(a);
     spread_total(st, 5, p.counters, p.totals);
 }
 
