@@ -133,7 +133,7 @@ __global__ void __launch_bounds__(GG) k_gravity_group(const int32_t *__restrict_
 {
     __shared__ int stk[GSTK];
     __shared__ Src4<R> A[GSRC], B[GSRC];
-    __shared__ int bfirst[GG], bcount[GG], bnear[GG];
+    __shared__ int bfirst[GG], bstart[GG + 1], bnear[GG];
     const int lane = threadIdx.x;
     const int gnode = groups[xcd_block(blockIdx.x, gridDim.x)];
     const int g0 = nodes[gnode].first, ng = nodes[gnode].count;
@@ -210,23 +210,48 @@ __global__ void __launch_bounds__(GG) k_gravity_group(const int32_t *__restrict_
         nn += (int)__popcll(man);
         nfar += (int)__popcll(maf);
         n_nodes += (long long)__popcll(ma);
+        // opened buckets: prefix sums of their counts (lane order = ballot order), then their
+        // particles are staged 64 at a time, all loads of a chunk issued together
+        int incl = bkt ? ncount : 0;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
+        }
+        const int total = __shfl(incl, 63);
         if (bkt) {
             const int pb = (int)__popcll(mb & lt);
             bfirst[pb] = nfirst;
-            bcount[pb] = ncount;
+            bstart[pb] = incl - ncount;
             bnear[pb] = near ? 1 : 0;
         }
         const int nb = (int)__popcll(mb);
+        if (lane == 0) bstart[nb] = total;
         __syncthreads();
         if (nn + nfar >= GG) {
             eval_sources<R, SOFT>(nn, nfar, A, B, tx, ty, tz, tlx, tly, tlz, te, e2, k);
             nn = nfar = 0;
             __syncthreads();
         }
-        for (int bb = 0; bb < nb; bb++) {
-            const int f = bfirst[bb], c = bcount[bb], bn = bnear[bb];
-            if (lane < c) {
-                const int j = f + lane, q = bn ? nn + lane : GSRC - 1 - nfar - lane;
+        n_parts += total;
+        for (int s0 = 0; s0 < total; s0 += GG) {
+            // nn + nfar < GG here, so a chunk of <= GG fits the GSRC = 2 GG slots
+            const int sidx = s0 + lane;
+            const bool have = sidx < total;
+            int bsel = 0;
+            if (have) {
+                int lo = 0, hi = nb - 1;  // last bucket b with bstart[b] <= sidx
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (bstart[mid] <= sidx) lo = mid;
+                    else hi = mid - 1;
+                }
+                bsel = lo;
+            }
+            const bool pn = have && bnear[bsel], pf = have && !bnear[bsel];
+            const uint64_t mn = wave_ballot(pn), mf = wave_ballot(pf);
+            if (have) {
+                const int j = bfirst[bsel] + (sidx - bstart[bsel]);
+                const int q = pn ? nn + (int)__popcll(mn & lt) : GSRC - 1 - nfar - (int)__popcll(mf & lt);
                 Src4<R> a, b;
                 split<R>(x[j] - gcx, a.x, b.x);
                 split<R>(y[j] - gcy, a.y, b.y);
@@ -236,9 +261,8 @@ __global__ void __launch_bounds__(GG) k_gravity_group(const int32_t *__restrict_
                 A[q] = a;
                 B[q] = b;
             }
-            if (bn) nn += c;
-            else nfar += c;
-            n_parts += c;
+            nn += (int)__popcll(mn);
+            nfar += (int)__popcll(mf);
             __syncthreads();
             if (nn + nfar >= GG) {
                 eval_sources<R, SOFT>(nn, nfar, A, B, tx, ty, tz, tlx, tly, tlz, te, e2, k);
