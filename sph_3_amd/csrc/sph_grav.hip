@@ -26,7 +26,7 @@
 #include "sph_walk.h"
 
 #define GG 64        // maximum targets per group (one wavefront)
-#define GSTK 1024    // LDS stack capacity (node ids)
+#define GSTK 512     // LDS stack capacity (node ids)
 #define GSRC 128     // staged sources (< 64 before a flush + 64 appended)
 
 // Group opening test (FP64, no contraction): monopole for the whole group iff

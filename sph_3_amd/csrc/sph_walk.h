@@ -17,7 +17,7 @@
 #pragma once
 #include "sph_common.h"
 
-#define WSTK 1024  // LDS stack capacity (node ids)
+#define WSTK 512   // LDS stack capacity (node ids)
 #define WSRC 128   // staged sources before a flush (< 64 + one bucket of <= 64)
 
 // Box of the group's targets by wave reduction (lanes >= count pass +-inf).
