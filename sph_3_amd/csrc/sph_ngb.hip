@@ -193,14 +193,16 @@ __global__ void __launch_bounds__(64) k_hwalk(HSolveArgs a)
 }
 
 // Newton (mode 0) or k-th distance (mode 2) solve of the pending targets from their
-// candidate rows, 8 targets per wavefront and 8 lanes per target: lane l of a
-// sub-group holds candidates l, l+8, ... (up to 32, in registers) and the sub-group
+// candidate rows, 4 targets per wavefront and 16 lanes per target: lane l of a
+// sub-group holds candidates l, l+16, ... (up to 16, in registers) and the sub-group
 // reduces with butterflies, which give its 8 lanes bitwise the same sums, so each
 // sub-group's iteration is uniform.  A target whose row overflowed gets R * 0.75, one
 // whose solution lies beyond R/2 gets R * 1.5, and stays pending for another round.
+#define HN_LANES 16  // lanes per target of k_hnewton
+
 __global__ void __launch_bounds__(64) k_hnewton(HSolveArgs a)
 {
-    constexpr int HL = 8, HS = SPH_HCAP_SLOTS / HL;
+    constexpr int HL = HN_LANES, HS = SPH_HCAP_SLOTS / HL;
     const int lane = threadIdx.x, lsub = lane % HL;
     const int64_t j = (int64_t)blockIdx.x * (64 / HL) + lane / HL;
     const bool valid = j < a.m && a.state[j] == 0;
@@ -883,7 +885,7 @@ void hsolve_round(hipStream_t st, const HSolveParams &p, const HSolveArgs &a)
         k_hwalk<true><<<(unsigned)nblk, 64, 0, st>>>(a);
     else
         k_hwalk<false><<<(unsigned)nblk, 64, 0, st>>>(a);
-    k_hnewton<<<(unsigned)((p.m + 7) / 8), 64, 0, st>>>(a);
+    k_hnewton<<<(unsigned)((p.m + 64 / HN_LANES - 1) / (64 / HN_LANES)), 64, 0, st>>>(a);
     spread_total(st, 5, p.counters, p.totals);
 }
 
