@@ -158,13 +158,37 @@ __global__ void __launch_bounds__(64) k_hwalk(HSolveArgs a)
             sj[q] = j;
         },
         [&](int cnt) {
-            if (valid) ntest += cnt;
             for (int k0 = 0; k0 < cnt; k0 += 64) {
                 const int k = k0 + lane;
                 const bool src = k < cnt;
                 const double px = src ? sx[k] : 0.0, py = src ? sy[k] : 0.0, pz = src ? sz[k] : 0.0;
                 const int pj = src ? sj[k] : 0;
-                for (uint64_t todo = vmask; todo; todo &= todo - 1) {
+                const int nsrc = (cnt - k0) < 64 ? (cnt - k0) : 64;
+                uint64_t need = vmask;
+                if constexpr (!PER) {
+                    // skip the targets whose distance to the chunk's bounding box is >= R:
+                    // by monotone rounding no source of the chunk can then pass r^2 < R^2
+                    double bl[3] = {src ? px : INFINITY, src ? py : INFINITY, src ? pz : INFINITY};
+                    double bh[3] = {src ? px : -INFINITY, src ? py : -INFINITY, src ? pz : -INFINITY};
+                    for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+                        for (int d = 0; d < 3; d++) {
+                            bl[d] = fmin(bl[d], __shfl_xor(bl[d], off));
+                            bh[d] = fmax(bh[d], __shfl_xor(bh[d], off));
+                        }
+                    const double tp[3] = {xi, yi, zi};
+                    double e2 = 0.0;
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        double g = 0.0;
+                        if (bl[d] - tp[d] > g) g = bl[d] - tp[d];
+                        if (tp[d] - bh[d] > g) g = tp[d] - bh[d];
+                        e2 = e2 + g * g;
+                    }
+                    need = __ballot(valid && e2 < R2);
+                }
+                if (lane < 64 && ((need >> lane) & 1)) ntest += nsrc;  // distance tests performed
+                for (uint64_t todo = need; todo; todo &= todo - 1) {
                     const int tt = __builtin_ctzll(todo);
                     const double dx = wrap_d<PER>(px - ttx[tt], a.per), dy = wrap_d<PER>(py - tty[tt], a.per),
                                  dz = wrap_d<PER>(pz - ttz[tt], a.per);
