@@ -80,6 +80,7 @@ struct sph_ctx {
     Buf nodes, right, parent, flags, front, groups;
     // neighbour lists
     Buf cnt, rcnt0, off, scantmp, idx;  // rcnt0: row lengths before the appended pairs
+    Buf p4, v4;                          // packed (x, y, z, h) and (vx, vy, vz, m) of the hydro passes
     // hydro scalars (8 bytes per particle, used as float or double)
     Buf rho, P, c, prho2, divV, curlV, bal, du;
     // h solve scratch
@@ -374,9 +375,18 @@ int build_ngb(sph_ctx *ctx, bool csr, const double *hscale = nullptr, double per
     return SPH_OK;
 }
 
-NgbRows ngb_rows(sph_ctx *ctx)
+// The neighbour sets plus the hydro passes' packed inputs, (x, y, z, h) and (vx, vy, vz, m)
+// of the sorted particles, packed from the current arrays (call after any of them changes).
+int ngb_rows(sph_ctx *ctx, NgbRows &r)
 {
-    return NgbRows{P_<int64_t>(ctx->off), I32(cnt), I32(idx), ctx->ngb_stride, ctx->ngb_per};
+    const int64_t n = ctx->n;
+    CK(ensure(ctx, ctx->p4, (size_t)n * 32));
+    CK(ensure(ctx, ctx->v4, (size_t)n * 32));
+    HIPCK(sph::pack4(ctx->st, n, D(x), D(y), D(z), D(h), P_<double4>(ctx->p4)));
+    HIPCK(sph::pack4(ctx->st, n, D(vx), D(vy), D(vz), D(m), P_<double4>(ctx->v4)));
+    r = NgbRows{P_<int64_t>(ctx->off), I32(cnt), I32(idx), ctx->ngb_stride, ctx->ngb_per,
+                P_<double4>(ctx->p4), P_<double4>(ctx->v4)};
+    return SPH_OK;
 }
 
 // csr: the caller needs true CSR offsets (sph_neighbours).
@@ -473,7 +483,7 @@ extern "C" int sph_ctx_destroy(sph_ctx *ctx)
                   &ctx->hstate, &ctx->hcount, &ctx->hout, &ctx->hin, &ctx->tx, &ctx->ty, &ctx->tz,
                   &ctx->stage, &ctx->kpart, &ctx->kout, &ctx->spread, &ctx->ccidx, &ctx->cccnt,
                   &ctx->cext, &ctx->cnext, &ctx->ttab, &ctx->zmet, &ctx->j21, &ctx->ucols, &ctx->uyn,
-                  &ctx->uL, &ctx->uG, &ctx->uflag, &ctx->ucnt, &ctx->hbig};
+                  &ctx->uL, &ctx->uG, &ctx->uflag, &ctx->ucnt, &ctx->hbig, &ctx->p4, &ctx->v4};
     for (Buf *b : all)
         if (b->p) (void)hipFree(b->p);
     Buf *cool[] = {&ctx->rf1,   &ctx->ctabL,  &ctx->ctabG,  &ctx->cin[0], &ctx->cin[1], &ctx->cin[2], &ctx->cin[3],
@@ -705,14 +715,13 @@ extern "C" int sph_density(sph_ctx *ctx, int64_t n, const double *pos, const dou
     CK(check_common(ctx, n, precision));
     if (!pos || !m || !h || !rho_out) return fail(ctx, SPH_ERR_ARG, "null argument");
     CK(geom(ctx, n, pos, h, m));
-    const NgbRows rows = ngb_rows(ctx);
+    NgbRows rows;
+    CK(ngb_rows(ctx, rows));
     if (precision == SPH_PREC_MIXED) {
-        HIPCK(sph::density<float>(ctx->st, n, D(x), D(y), D(z), D(m), D(h), nullptr, 0.0, rows,
-                                  P_<float>(ctx->rho), nullptr, nullptr, nullptr));
+        HIPCK(sph::density<float>(ctx->st, n, nullptr, 0.0, rows, P_<float>(ctx->rho), nullptr, nullptr, nullptr));
         return get_scalar<float>(ctx, ctx->rho, rho_out);
     }
-    HIPCK(sph::density<double>(ctx->st, n, D(x), D(y), D(z), D(m), D(h), nullptr, 0.0, rows, D(rho),
-                               nullptr, nullptr, nullptr));
+    HIPCK(sph::density<double>(ctx->st, n, nullptr, 0.0, rows, D(rho), nullptr, nullptr, nullptr));
     return get_scalar<double>(ctx, ctx->rho, rho_out);
 }
 
@@ -723,8 +732,9 @@ int divcurl_impl(sph_ctx *ctx, const double *vel, const double *rho, double *dv,
     int64_t n = ctx->n;
     CK(put_vec3(ctx, vel, ctx->vx, ctx->vy, ctx->vz));
     CK(put_real<R>(ctx, rho, ctx->rho));
-    HIPCK(sph::divcurl<R>(ctx->st, n, D(x), D(y), D(z), D(vx), D(vy), D(vz), D(m), D(h), P_<R>(ctx->rho), nullptr,
-                          ngb_rows(ctx), P_<R>(ctx->divV), P_<R>(ctx->curlV), nullptr));
+    NgbRows rows;
+    CK(ngb_rows(ctx, rows));
+    HIPCK(sph::divcurl<R>(ctx->st, n, P_<R>(ctx->rho), nullptr, rows, P_<R>(ctx->divV), P_<R>(ctx->curlV), nullptr));
     CK(get_scalar<R>(ctx, ctx->divV, dv));
     return get_scalar<R>(ctx, ctx->curlV, cv);
 }
@@ -754,9 +764,11 @@ int acc_impl(sph_ctx *ctx, const double *vel, const double *rho, const double *P
 {
     CK(hydro_inputs<R>(ctx, vel, rho, P, c, divV, curlV));
     bool bal = divV && curlV;
-    HIPCK(sph::accsph<R>(ctx->st, ctx->n, D(x), D(y), D(z), D(vx), D(vy), D(vz), D(m), D(h), P_<R>(ctx->rho),
-                         P_<R>(ctx->prho2), P_<R>(ctx->c), bal ? P_<R>(ctx->bal) : nullptr, alpha,
-                         ngb_rows(ctx), nullptr, nullptr, nullptr, D(ax), D(ay), D(az)));
+    NgbRows rows;
+    CK(ngb_rows(ctx, rows));
+    HIPCK(sph::accsph<R>(ctx->st, ctx->n, P_<R>(ctx->rho), P_<R>(ctx->prho2), P_<R>(ctx->c),
+                         bal ? P_<R>(ctx->bal) : nullptr, alpha, rows, nullptr, nullptr, nullptr, D(ax), D(ay),
+                         D(az)));
     return get_vec3(ctx, ctx->ax, ctx->ay, ctx->az, acc);
 }
 
@@ -766,9 +778,10 @@ int dudt_impl(sph_ctx *ctx, const double *vel, const double *rho, const double *
 {
     CK(hydro_inputs<R>(ctx, vel, rho, P, c, divV, curlV));
     bool bal = divV && curlV;
-    HIPCK(sph::dudt<R>(ctx->st, ctx->n, D(x), D(y), D(z), D(vx), D(vy), D(vz), D(m), D(h), P_<R>(ctx->rho),
-                       P_<R>(ctx->prho2), P_<R>(ctx->c), bal ? P_<R>(ctx->bal) : nullptr, alpha,
-                       ngb_rows(ctx), P_<R>(ctx->du)));
+    NgbRows rows;
+    CK(ngb_rows(ctx, rows));
+    HIPCK(sph::dudt<R>(ctx->st, ctx->n, P_<R>(ctx->rho), P_<R>(ctx->prho2), P_<R>(ctx->c),
+                       bal ? P_<R>(ctx->bal) : nullptr, alpha, rows, P_<R>(ctx->du)));
     return get_scalar<R>(ctx, ctx->du, out);
 }
 }  // namespace
@@ -1046,20 +1059,19 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     // density + EOS, fused with div/curl v and the Balsara factor when the switch is on; then
     // the acceleration (a = a_grav + a_sph)
     auto hydro_forces = [&]() -> int {
-        const NgbRows rows = ngb_rows(ctx);
+        NgbRows rows;
+        CK(ngb_rows(ctx, rows));
         ph_begin(ctx, SPH_PH_DENSITY);
         if (p->balsara)
-            HIPCK(sph::density_divcurl<R>(st, n, D(x), D(y), D(z), D(vx), D(vy), D(vz), D(m), D(h), D(u), p->gamma,
-                                          rows, P_<R>(ctx->rho), P_<R>(ctx->P), P_<R>(ctx->c), P_<R>(ctx->prho2),
-                                          P_<R>(ctx->divV), P_<R>(ctx->curlV), bal));
+            HIPCK(sph::density_divcurl<R>(st, n, D(u), p->gamma, rows, P_<R>(ctx->rho), P_<R>(ctx->P), P_<R>(ctx->c),
+                                          P_<R>(ctx->prho2), P_<R>(ctx->divV), P_<R>(ctx->curlV), bal));
         else
-            HIPCK(sph::density<R>(st, n, D(x), D(y), D(z), D(m), D(h), D(u), p->gamma, rows, P_<R>(ctx->rho),
-                                  P_<R>(ctx->P), P_<R>(ctx->c), P_<R>(ctx->prho2)));
+            HIPCK(sph::density<R>(st, n, D(u), p->gamma, rows, P_<R>(ctx->rho), P_<R>(ctx->P), P_<R>(ctx->c),
+                                  P_<R>(ctx->prho2)));
         ph_end(ctx, SPH_PH_DENSITY);
         ph_begin(ctx, SPH_PH_ACCSPH);
-        HIPCK(sph::accsph<R>(st, n, D(x), D(y), D(z), D(vx), D(vy), D(vz), D(m), D(h), P_<R>(ctx->rho),
-                             P_<R>(ctx->prho2), P_<R>(ctx->c), bal, p->alpha, rows, g ? D(gx) : nullptr,
-                             g ? D(gy) : nullptr, g ? D(gz) : nullptr, D(ax), D(ay), D(az)));
+        HIPCK(sph::accsph<R>(st, n, P_<R>(ctx->rho), P_<R>(ctx->prho2), P_<R>(ctx->c), bal, p->alpha, rows,
+                             g ? D(gx) : nullptr, g ? D(gy) : nullptr, g ? D(gz) : nullptr, D(ax), D(ay), D(az)));
         ph_end(ctx, SPH_PH_ACCSPH);
         return SPH_OK;
     };
@@ -1081,7 +1093,6 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
             CK(hydro_forces());
         }
     }
-    const NgbRows rows = ngb_rows(ctx);
     if (!first) {
         ph_begin(ctx, SPH_PH_KICK);
         HIPCK(sph::kick_drift(st, n, D(vx), D(vy), D(vz), D(ax), D(ay), D(az), 0.5 * p->dt, 0.0, nullptr, nullptr,
@@ -1089,7 +1100,9 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
         ph_end(ctx, SPH_PH_KICK);
     }
     ph_begin(ctx, SPH_PH_DUDT);
-    HIPCK(sph::dudt<R>(st, n, D(x), D(y), D(z), D(vx), D(vy), D(vz), D(m), D(h), P_<R>(ctx->rho), P_<R>(ctx->prho2),
+    NgbRows rows;
+    CK(ngb_rows(ctx, rows));  // packs the kicked velocities
+    HIPCK(sph::dudt<R>(st, n, P_<R>(ctx->rho), P_<R>(ctx->prho2),
                        P_<R>(ctx->c), bal, p->alpha, rows, P_<R>(ctx->du)));
     ph_end(ctx, SPH_PH_DUDT);
     ph_begin(ctx, SPH_PH_UUPDATE);

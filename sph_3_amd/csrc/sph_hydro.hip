@@ -17,10 +17,7 @@
 #define HG 16  // lanes per particle
 
 template <typename R, int G, bool PER>
-__global__ void __launch_bounds__(256) k_density(int64_t n, const double *__restrict__ x,
-                                                 const double *__restrict__ y, const double *__restrict__ z,
-                                                 const double *__restrict__ m, const double *__restrict__ h,
-                                                 const double *__restrict__ u, double gamma,
+__global__ void __launch_bounds__(256) k_density(int64_t n, const double *__restrict__ u, double gamma,
                                                  NgbRows rows,
                                                  R *__restrict__ rho, R *__restrict__ P, R *__restrict__ c,
                                                  R *__restrict__ prho2)
@@ -31,16 +28,18 @@ __global__ void __launch_bounds__(256) k_density(int64_t n, const double *__rest
     bool valid = i < n;
     Kahan<R> acc;
     if (valid) {
-        const double xi = x[i], yi = y[i], zi = z[i], hi = h[i];
+        const double4 pi4 = rows.p4[i];
+        const double xi = pi4.x, yi = pi4.y, zi = pi4.z, hi = pi4.w;
         int64_t k0, e;
         row_range(rows, i, k0, e);
         for (int64_t k = k0 + l; k < e; k += G) {
-            int j = rows.idx[k];
-            R dx = (R)wrap_d<PER>(xi - x[j], rows.L), dy = (R)wrap_d<PER>(yi - y[j], rows.L),
-              dz = (R)wrap_d<PER>(zi - z[j], rows.L);
+            const int j = rows.idx[k];
+            const double4 pj = rows.p4[j], vj = rows.v4[j];
+            R dx = (R)wrap_d<PER>(xi - pj.x, rows.L), dy = (R)wrap_d<PER>(yi - pj.y, rows.L),
+              dz = (R)wrap_d<PER>(zi - pj.z, rows.L);
             R rr = sph_sqrt<R>(dx * dx + dy * dy + dz * dz);
-            R hij = (R)(0.5 * (hi + h[j]));
-            acc.add((R)m[j] * spline_W<R>(rr, hij));
+            R hij = (R)(0.5 * (hi + pj.w));
+            acc.add((R)vj.w * spline_W<R>(rr, hij));
         }
     }
     R s = group_sum<R, G>(acc.s, -acc.c);
@@ -57,11 +56,7 @@ __global__ void __launch_bounds__(256) k_density(int64_t n, const double *__rest
 }
 
 template <typename R, int G, bool PER>
-__global__ void __launch_bounds__(256) k_divcurl(int64_t n, const double *__restrict__ x,
-                                                 const double *__restrict__ y, const double *__restrict__ z,
-                                                 const double *__restrict__ vx, const double *__restrict__ vy,
-                                                 const double *__restrict__ vz, const double *__restrict__ m,
-                                                 const double *__restrict__ h, const R *__restrict__ rho,
+__global__ void __launch_bounds__(256) k_divcurl(int64_t n, const R *__restrict__ rho,
                                                  const R *__restrict__ c, NgbRows rows, R *__restrict__ divV,
                                                  R *__restrict__ curlV, R *__restrict__ bal)
 {
@@ -71,19 +66,22 @@ __global__ void __launch_bounds__(256) k_divcurl(int64_t n, const double *__rest
     bool valid = i < n;
     Kahan<R> sd, scx, scy, scz;
     if (valid) {
-        const double xi = x[i], yi = y[i], zi = z[i], hi = h[i];
-        const double vxi = vx[i], vyi = vy[i], vzi = vz[i];
+        const double4 pi4 = rows.p4[i];
+        const double xi = pi4.x, yi = pi4.y, zi = pi4.z, hi = pi4.w;
+        const double4 vi4 = rows.v4[i];
+        const double vxi = vi4.x, vyi = vi4.y, vzi = vi4.z;
         int64_t k0, e;
         row_range(rows, i, k0, e);
         for (int64_t k = k0 + l; k < e; k += G) {
-            int j = rows.idx[k];
-            R dx = (R)wrap_d<PER>(xi - x[j], rows.L), dy = (R)wrap_d<PER>(yi - y[j], rows.L),
-              dz = (R)wrap_d<PER>(zi - z[j], rows.L);
+            const int j = rows.idx[k];
+            const double4 pj = rows.p4[j], vj = rows.v4[j];
+            R dx = (R)wrap_d<PER>(xi - pj.x, rows.L), dy = (R)wrap_d<PER>(yi - pj.y, rows.L),
+              dz = (R)wrap_d<PER>(zi - pj.z, rows.L);
             R rr = sph_sqrt<R>(dx * dx + dy * dy + dz * dz);
-            R nW = spline_nW<R>(rr, (R)(0.5 * (hi + h[j])));
+            R nW = spline_nW<R>(rr, (R)(0.5 * (hi + pj.w)));
             R gx = nW * dx, gy = nW * dy, gz = nW * dz;
-            R ux = (R)(vxi - vx[j]), uy = (R)(vyi - vy[j]), uz = (R)(vzi - vz[j]);
-            R mj = (R)m[j];
+            R ux = (R)(vxi - vj.x), uy = (R)(vyi - vj.y), uz = (R)(vzi - vj.z);
+            R mj = (R)vj.w;
             sd.add(-mj * (ux * gx + uy * gy + uz * gz));
             scx.add(mj * (uy * gz - uz * gy));
             scy.add(mj * (uz * gx - ux * gz));
@@ -101,7 +99,7 @@ __global__ void __launch_bounds__(256) k_divcurl(int64_t n, const double *__rest
         R cv = sph_sqrt<R>(cx * cx + cy * cy + cz * cz) * ir;
         divV[i] = dv;
         curlV[i] = cv;
-        if (bal) bal[i] = dv / (dv + cv + (R)0.0001 * c[i] / (R)h[i]);
+        if (bal) bal[i] = dv / (dv + cv + (R)0.0001 * c[i] / (R)rows.p4[i].w);
     }
 }
 
@@ -112,10 +110,7 @@ __global__ void __launch_bounds__(256) k_divcurl(int64_t n, const double *__rest
 // arithmetic and summation order are those of k_density and k_divcurl.
 template <typename R, int G, bool PER>
 __global__ void __launch_bounds__(256) k_density_divcurl(
-    int64_t n, const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
-    const double *__restrict__ vx, const double *__restrict__ vy, const double *__restrict__ vz,
-    const double *__restrict__ m, const double *__restrict__ h, const double *__restrict__ u, double gamma,
-    NgbRows rows, R *__restrict__ rho, R *__restrict__ P,
+    int64_t n, const double *__restrict__ u, double gamma, NgbRows rows, R *__restrict__ rho, R *__restrict__ P,
     R *__restrict__ c, R *__restrict__ prho2, R *__restrict__ divV, R *__restrict__ curlV, R *__restrict__ bal)
 {
     int64_t gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -124,21 +119,24 @@ __global__ void __launch_bounds__(256) k_density_divcurl(
     bool valid = i < n;
     Kahan<R> acc, sd, scx, scy, scz;
     if (valid) {
-        const double xi = x[i], yi = y[i], zi = z[i], hi = h[i];
-        const double vxi = vx[i], vyi = vy[i], vzi = vz[i];
+        const double4 pi4 = rows.p4[i];
+        const double xi = pi4.x, yi = pi4.y, zi = pi4.z, hi = pi4.w;
+        const double4 vi4 = rows.v4[i];
+        const double vxi = vi4.x, vyi = vi4.y, vzi = vi4.z;
         int64_t k0, e;
         row_range(rows, i, k0, e);
         for (int64_t k = k0 + l; k < e; k += G) {
-            int j = rows.idx[k];
-            R dx = (R)wrap_d<PER>(xi - x[j], rows.L), dy = (R)wrap_d<PER>(yi - y[j], rows.L),
-              dz = (R)wrap_d<PER>(zi - z[j], rows.L);
+            const int j = rows.idx[k];
+            const double4 pj = rows.p4[j], vj = rows.v4[j];
+            R dx = (R)wrap_d<PER>(xi - pj.x, rows.L), dy = (R)wrap_d<PER>(yi - pj.y, rows.L),
+              dz = (R)wrap_d<PER>(zi - pj.z, rows.L);
             R rr = sph_sqrt<R>(dx * dx + dy * dy + dz * dz);
-            R hij = (R)(0.5 * (hi + h[j]));
-            R mj = (R)m[j];
+            R hij = (R)(0.5 * (hi + pj.w));
+            R mj = (R)vj.w;
             acc.add(mj * spline_W<R>(rr, hij));
             R nW = spline_nW<R>(rr, hij);
             R gx = nW * dx, gy = nW * dy, gz = nW * dz;
-            R ux = (R)(vxi - vx[j]), uy = (R)(vyi - vy[j]), uz = (R)(vzi - vz[j]);
+            R ux = (R)(vxi - vj.x), uy = (R)(vyi - vj.y), uz = (R)(vzi - vj.z);
             sd.add(-mj * (ux * gx + uy * gy + uz * gz));
             scx.add(mj * (uy * gz - uz * gy));
             scy.add(mj * (uz * gx - ux * gz));
@@ -164,7 +162,7 @@ __global__ void __launch_bounds__(256) k_density_divcurl(
         R cv = sph_sqrt<R>(cx * cx + cy * cy + cz * cz) * ir;
         divV[i] = dv;
         curlV[i] = cv;
-        if (bal) bal[i] = dv / (dv + cv + (R)0.0001 * ci / (R)h[i]);
+        if (bal) bal[i] = dv / (dv + cv + (R)0.0001 * ci / (R)rows.p4[i].w);
     }
 }
 
@@ -179,11 +177,7 @@ __device__ __forceinline__ R av_pi(R vr, R rr, R ci, R cj, R rhoi, R rhoj, R alp
 }
 
 template <typename R, int G, bool PER>
-__global__ void __launch_bounds__(256) k_accsph(int64_t n, const double *__restrict__ x,
-                                                const double *__restrict__ y, const double *__restrict__ z,
-                                                const double *__restrict__ vx, const double *__restrict__ vy,
-                                                const double *__restrict__ vz, const double *__restrict__ m,
-                                                const double *__restrict__ h, const R *__restrict__ rho,
+__global__ void __launch_bounds__(256) k_accsph(int64_t n, const R *__restrict__ rho,
                                                 const R *__restrict__ prho2, const R *__restrict__ c,
                                                 const R *__restrict__ bal, double alpha,
                                                 NgbRows rows,
@@ -197,23 +191,26 @@ __global__ void __launch_bounds__(256) k_accsph(int64_t n, const double *__restr
     bool valid = i < n;
     Kahan<R> sx, sy, sz;
     if (valid) {
-        const double xi = x[i], yi = y[i], zi = z[i], hi = h[i];
-        const double vxi = vx[i], vyi = vy[i], vzi = vz[i];
+        const double4 pi4 = rows.p4[i];
+        const double xi = pi4.x, yi = pi4.y, zi = pi4.z, hi = pi4.w;
+        const double4 vi4 = rows.v4[i];
+        const double vxi = vi4.x, vyi = vi4.y, vzi = vi4.z;
         const R rhoi = rho[i], pi_ = prho2[i], ci = c[i], fi = bal ? bal[i] : (R)1;
         const R al = (R)alpha;
         int64_t k0, e;
         row_range(rows, i, k0, e);
         for (int64_t k = k0 + l; k < e; k += G) {
-            int j = rows.idx[k];
-            R dx = (R)wrap_d<PER>(xi - x[j], rows.L), dy = (R)wrap_d<PER>(yi - y[j], rows.L),
-              dz = (R)wrap_d<PER>(zi - z[j], rows.L);
+            const int j = rows.idx[k];
+            const double4 pj = rows.p4[j], vj = rows.v4[j];
+            R dx = (R)wrap_d<PER>(xi - pj.x, rows.L), dy = (R)wrap_d<PER>(yi - pj.y, rows.L),
+              dz = (R)wrap_d<PER>(zi - pj.z, rows.L);
             R rr = sph_sqrt<R>(dx * dx + dy * dy + dz * dz);
-            R nW = spline_nW<R>(rr, (R)(0.5 * (hi + h[j])));
-            R ux = (R)(vxi - vx[j]), uy = (R)(vyi - vy[j]), uz = (R)(vzi - vz[j]);
+            R nW = spline_nW<R>(rr, (R)(0.5 * (hi + pj.w)));
+            R ux = (R)(vxi - vj.x), uy = (R)(vyi - vj.y), uz = (R)(vzi - vj.z);
             R vr = ux * dx + uy * dy + uz * dz;
             R PI = av_pi<R>(vr, rr, ci, c[j], rhoi, rho[j], al);
             if (bal) PI = (R)0.5 * (fi + bal[j]) * PI;
-            R t = (R)m[j] * (pi_ + prho2[j] + PI) * nW;
+            R t = (R)vj.w * (pi_ + prho2[j] + PI) * nW;
             sx.add(-t * dx);
             sy.add(-t * dy);
             sz.add(-t * dz);
@@ -231,11 +228,7 @@ __global__ void __launch_bounds__(256) k_accsph(int64_t n, const double *__restr
 }
 
 template <typename R, int G, bool PER>
-__global__ void __launch_bounds__(256) k_dudt(int64_t n, const double *__restrict__ x,
-                                              const double *__restrict__ y, const double *__restrict__ z,
-                                              const double *__restrict__ vx, const double *__restrict__ vy,
-                                              const double *__restrict__ vz, const double *__restrict__ m,
-                                              const double *__restrict__ h, const R *__restrict__ rho,
+__global__ void __launch_bounds__(256) k_dudt(int64_t n, const R *__restrict__ rho,
                                               const R *__restrict__ prho2, const R *__restrict__ c,
                                               const R *__restrict__ bal, double alpha,
                                               NgbRows rows,
@@ -247,23 +240,26 @@ __global__ void __launch_bounds__(256) k_dudt(int64_t n, const double *__restric
     bool valid = i < n;
     Kahan<R> s;
     if (valid) {
-        const double xi = x[i], yi = y[i], zi = z[i], hi = h[i];
-        const double vxi = vx[i], vyi = vy[i], vzi = vz[i];
+        const double4 pi4 = rows.p4[i];
+        const double xi = pi4.x, yi = pi4.y, zi = pi4.z, hi = pi4.w;
+        const double4 vi4 = rows.v4[i];
+        const double vxi = vi4.x, vyi = vi4.y, vzi = vi4.z;
         const R rhoi = rho[i], pi_ = prho2[i], ci = c[i], fi = bal ? bal[i] : (R)1;
         const R al = (R)alpha;
         int64_t k0, e;
         row_range(rows, i, k0, e);
         for (int64_t k = k0 + l; k < e; k += G) {
-            int j = rows.idx[k];
-            R dx = (R)wrap_d<PER>(xi - x[j], rows.L), dy = (R)wrap_d<PER>(yi - y[j], rows.L),
-              dz = (R)wrap_d<PER>(zi - z[j], rows.L);
+            const int j = rows.idx[k];
+            const double4 pj = rows.p4[j], vj = rows.v4[j];
+            R dx = (R)wrap_d<PER>(xi - pj.x, rows.L), dy = (R)wrap_d<PER>(yi - pj.y, rows.L),
+              dz = (R)wrap_d<PER>(zi - pj.z, rows.L);
             R rr = sph_sqrt<R>(dx * dx + dy * dy + dz * dz);
-            R nW = spline_nW<R>(rr, (R)(0.5 * (hi + h[j])));
-            R ux = (R)(vxi - vx[j]), uy = (R)(vyi - vy[j]), uz = (R)(vzi - vz[j]);
+            R nW = spline_nW<R>(rr, (R)(0.5 * (hi + pj.w)));
+            R ux = (R)(vxi - vj.x), uy = (R)(vyi - vj.y), uz = (R)(vzi - vj.z);
             R vr = ux * dx + uy * dy + uz * dz;
             R PI = av_pi<R>(vr, rr, ci, c[j], rhoi, rho[j], al);
             if (bal) PI = (R)0.5 * (fi + bal[j]) * PI;
-            s.add((R)m[j] * (pi_ + (R)0.5 * PI) * nW * vr);
+            s.add((R)vj.w * (pi_ + (R)0.5 * PI) * nW * vr);
         }
     }
     R r = group_sum<R, G>(s.s, -s.c);
@@ -272,95 +268,84 @@ __global__ void __launch_bounds__(256) k_dudt(int64_t n, const double *__restric
 
 namespace sph {
 
+__global__ void __launch_bounds__(256) k_pack4(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
+                                               const double *__restrict__ c, const double *__restrict__ d,
+                                               double4 *__restrict__ out)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = make_double4(a[i], b[i], c[i], d[i]);
+}
+
+hipError_t pack4(hipStream_t st, int64_t n, const double *a, const double *b, const double *c, const double *d,
+                 double4 *out)
+{
+    k_pack4<<<sph_blocks(n, 256), 256, 0, st>>>(n, a, b, c, d, out);
+    return hipGetLastError();
+}
+
+#define LAUNCH(K, ...)                                                                                             \
+    do {                                                                                                           \
+        if (rows.L > 0.0)                                                                                          \
+            K<R, HG, true><<<sph_blocks(n * HG, 256), 256, 0, st>>>(__VA_ARGS__);                                   \
+        else                                                                                                       \
+            K<R, HG, false><<<sph_blocks(n * HG, 256), 256, 0, st>>>(__VA_ARGS__);                                  \
+    } while (0)
+
 template <typename R>
-hipError_t density(hipStream_t st, int64_t n, const double *x, const double *y, const double *z, const double *m,
-                   const double *h, const double *u, double gamma, const NgbRows &rows, R *rho, R *P,
+hipError_t density(hipStream_t st, int64_t n, const double *u, double gamma, const NgbRows &rows, R *rho, R *P,
                    R *c, R *prho2)
 {
-    if (rows.L > 0.0)
-        k_density<R, HG, true><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, m, h, u, gamma, rows, rho, P, c, prho2);
-    else
-        k_density<R, HG, false><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, m, h, u, gamma, rows, rho, P, c, prho2);
+    LAUNCH(k_density, n, u, gamma, rows, rho, P, c, prho2);
     return hipGetLastError();
 }
 
 template <typename R>
-hipError_t divcurl(hipStream_t st, int64_t n, const double *x, const double *y, const double *z, const double *vx,
-                   const double *vy, const double *vz, const double *m, const double *h, const R *rho, const R *c,
-                   const NgbRows &rows, R *divV, R *curlV, R *bal)
+hipError_t divcurl(hipStream_t st, int64_t n, const R *rho, const R *c, const NgbRows &rows, R *divV, R *curlV,
+                   R *bal)
 {
-    if (rows.L > 0.0)
-        k_divcurl<R, HG, true><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, rho, c, rows, divV,
-                                                              curlV, bal);
-    else
-        k_divcurl<R, HG, false><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, rho, c, rows, divV,
-                                                              curlV, bal);
+    LAUNCH(k_divcurl, n, rho, c, rows, divV, curlV, bal);
     return hipGetLastError();
 }
 
 template <typename R>
-hipError_t density_divcurl(hipStream_t st, int64_t n, const double *x, const double *y, const double *z,
-                           const double *vx, const double *vy, const double *vz, const double *m, const double *h,
-                           const double *u, double gamma, const NgbRows &rows, R *rho, R *P, R *c,
-                           R *prho2, R *divV, R *curlV, R *bal)
+hipError_t density_divcurl(hipStream_t st, int64_t n, const double *u, double gamma, const NgbRows &rows, R *rho,
+                           R *P, R *c, R *prho2, R *divV, R *curlV, R *bal)
 {
-    if (rows.L > 0.0)
-        k_density_divcurl<R, HG, true><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, u, gamma, rows,
-                                                                      rho, P, c, prho2, divV, curlV, bal);
-    else
-        k_density_divcurl<R, HG, false><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, u, gamma, rows,
-                                                                      rho, P, c, prho2, divV, curlV, bal);
+    LAUNCH(k_density_divcurl, n, u, gamma, rows, rho, P, c, prho2, divV, curlV, bal);
     return hipGetLastError();
 }
 
 template <typename R>
-hipError_t accsph(hipStream_t st, int64_t n, const double *x, const double *y, const double *z, const double *vx,
-                  const double *vy, const double *vz, const double *m, const double *h, const R *rho,
-                  const R *prho2, const R *c, const R *bal, double alpha, const NgbRows &rows,
-                  const double *gx, const double *gy, const double *gz, double *ax, double *ay, double *az)
+hipError_t accsph(hipStream_t st, int64_t n, const R *rho, const R *prho2, const R *c, const R *bal, double alpha,
+                  const NgbRows &rows, const double *gx, const double *gy, const double *gz, double *ax, double *ay,
+                  double *az)
 {
-    if (rows.L > 0.0)
-        k_accsph<R, HG, true><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, rho, prho2, c, bal, alpha,
-                                                             rows, gx, gy, gz, ax, ay, az);
-    else
-        k_accsph<R, HG, false><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, rho, prho2, c, bal, alpha,
-                                                             rows, gx, gy, gz, ax, ay, az);
+    LAUNCH(k_accsph, n, rho, prho2, c, bal, alpha, rows, gx, gy, gz, ax, ay, az);
     return hipGetLastError();
 }
 
 template <typename R>
-hipError_t dudt(hipStream_t st, int64_t n, const double *x, const double *y, const double *z, const double *vx,
-                const double *vy, const double *vz, const double *m, const double *h, const R *rho, const R *prho2,
-                const R *c, const R *bal, double alpha, const NgbRows &rows, R *out)
+hipError_t dudt(hipStream_t st, int64_t n, const R *rho, const R *prho2, const R *c, const R *bal, double alpha,
+                const NgbRows &rows, R *out)
 {
-    if (rows.L > 0.0)
-        k_dudt<R, HG, true><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, rho, prho2, c, bal, alpha,
-                                                           rows, out);
-    else
-        k_dudt<R, HG, false><<<sph_blocks(n * HG, 256), 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, h, rho, prho2, c, bal, alpha,
-                                                           rows, out);
+    LAUNCH(k_dudt, n, rho, prho2, c, bal, alpha, rows, out);
     return hipGetLastError();
 }
+#undef LAUNCH
 
 #define INST(R)                                                                                                    \
-    template hipError_t density<R>(hipStream_t, int64_t, const double *, const double *, const double *,           \
-                                   const double *, const double *, const double *, double, const NgbRows &, R *,   \
-                                   R *, R *, R *);                                                                 \
-    template hipError_t divcurl<R>(hipStream_t, int64_t, const double *, const double *, const double *,           \
-                                   const double *, const double *, const double *, const double *, const double *, \
-                                   const R *, const R *, const NgbRows &, R *, R *, R *);                          \
-    template hipError_t accsph<R>(hipStream_t, int64_t, const double *, const double *, const double *,            \
-                                  const double *, const double *, const double *, const double *, const double *,  \
-                                  const R *, const R *, const R *, const R *, double, const NgbRows &,             \
-                                  const double *, const double *, const double *, double *, double *, double *);   \
-    template hipError_t dudt<R>(hipStream_t, int64_t, const double *, const double *, const double *,              \
-                                const double *, const double *, const double *, const double *, const double *,    \
-                                const R *, const R *, const R *, const R *, double, const NgbRows &, R *);         \
-    template hipError_t density_divcurl<R>(hipStream_t, int64_t, const double *, const double *, const double *,   \
-                                           const double *, const double *, const double *, const double *,         \
-                                           const double *, const double *, double, const NgbRows &, R *, R *, R *, \
-                                           R *, R *, R *, R *);
+    template hipError_t density<R>(hipStream_t, int64_t, const double *, double, const NgbRows &, R *, R *, R *,  \
+                                   R *);                                                                           \
+    template hipError_t divcurl<R>(hipStream_t, int64_t, const R *, const R *, const NgbRows &, R *, R *, R *);   \
+    template hipError_t density_divcurl<R>(hipStream_t, int64_t, const double *, double, const NgbRows &, R *,    \
+                                           R *, R *, R *, R *, R *, R *);                                          \
+    template hipError_t accsph<R>(hipStream_t, int64_t, const R *, const R *, const R *, const R *, double,       \
+                                  const NgbRows &, const double *, const double *, const double *, double *,      \
+                                  double *, double *);                                                            \
+    template hipError_t dudt<R>(hipStream_t, int64_t, const R *, const R *, const R *, const R *, double,         \
+                                const NgbRows &, R *);
 INST(float)
 INST(double)
+#undef INST
 
 }  // namespace sph
