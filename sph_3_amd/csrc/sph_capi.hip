@@ -80,7 +80,7 @@ struct sph_ctx {
     Buf nodes, right, parent, flags, front, groups;
     // neighbour lists
     Buf cnt, rcnt0, off, scantmp, idx;  // rcnt0: row lengths before the appended pairs
-    Buf p4, v4;                          // packed (x, y, z, h) and (vx, vy, vz, m) of the hydro passes
+    Buf p4, v4, s4;                      // packed (x, y, z, h), (vx, vy, vz, m), (rho, P/rho^2, c, f) records
     // hydro scalars (8 bytes per particle, used as float or double)
     Buf rho, P, c, prho2, divV, curlV, bal, du;
     // h solve scratch
@@ -483,7 +483,7 @@ extern "C" int sph_ctx_destroy(sph_ctx *ctx)
                   &ctx->hstate, &ctx->hcount, &ctx->hout, &ctx->hin, &ctx->tx, &ctx->ty, &ctx->tz,
                   &ctx->stage, &ctx->kpart, &ctx->kout, &ctx->spread, &ctx->ccidx, &ctx->cccnt,
                   &ctx->cext, &ctx->cnext, &ctx->ttab, &ctx->zmet, &ctx->j21, &ctx->ucols, &ctx->uyn,
-                  &ctx->uL, &ctx->uG, &ctx->uflag, &ctx->ucnt, &ctx->hbig, &ctx->p4, &ctx->v4};
+                  &ctx->uL, &ctx->uG, &ctx->uflag, &ctx->ucnt, &ctx->hbig, &ctx->p4, &ctx->v4, &ctx->s4};
     for (Buf *b : all)
         if (b->p) (void)hipFree(b->p);
     Buf *cool[] = {&ctx->rf1,   &ctx->ctabL,  &ctx->ctabG,  &ctx->cin[0], &ctx->cin[1], &ctx->cin[2], &ctx->cin[3],
@@ -766,9 +766,11 @@ int acc_impl(sph_ctx *ctx, const double *vel, const double *rho, const double *P
     bool bal = divV && curlV;
     NgbRows rows;
     CK(ngb_rows(ctx, rows));
-    HIPCK(sph::accsph<R>(ctx->st, ctx->n, P_<R>(ctx->rho), P_<R>(ctx->prho2), P_<R>(ctx->c),
-                         bal ? P_<R>(ctx->bal) : nullptr, alpha, rows, nullptr, nullptr, nullptr, D(ax), D(ay),
-                         D(az)));
+    CK(ensure(ctx, ctx->s4, (size_t)ctx->n * 4 * sizeof(R)));
+    HIPCK(sph::pack_s4<R>(ctx->st, ctx->n, P_<R>(ctx->rho), P_<R>(ctx->prho2), P_<R>(ctx->c),
+                          bal ? P_<R>(ctx->bal) : nullptr, P_<R4<R>>(ctx->s4)));
+    HIPCK(sph::accsph<R>(ctx->st, ctx->n, P_<R4<R>>(ctx->s4), bal, alpha, rows, nullptr, nullptr, nullptr, D(ax),
+                         D(ay), D(az)));
     return get_vec3(ctx, ctx->ax, ctx->ay, ctx->az, acc);
 }
 
@@ -780,8 +782,10 @@ int dudt_impl(sph_ctx *ctx, const double *vel, const double *rho, const double *
     bool bal = divV && curlV;
     NgbRows rows;
     CK(ngb_rows(ctx, rows));
-    HIPCK(sph::dudt<R>(ctx->st, ctx->n, P_<R>(ctx->rho), P_<R>(ctx->prho2), P_<R>(ctx->c),
-                       bal ? P_<R>(ctx->bal) : nullptr, alpha, rows, P_<R>(ctx->du)));
+    CK(ensure(ctx, ctx->s4, (size_t)ctx->n * 4 * sizeof(R)));
+    HIPCK(sph::pack_s4<R>(ctx->st, ctx->n, P_<R>(ctx->rho), P_<R>(ctx->prho2), P_<R>(ctx->c),
+                          bal ? P_<R>(ctx->bal) : nullptr, P_<R4<R>>(ctx->s4)));
+    HIPCK(sph::dudt<R>(ctx->st, ctx->n, P_<R4<R>>(ctx->s4), bal, alpha, rows, P_<R>(ctx->du)));
     return get_scalar<R>(ctx, ctx->du, out);
 }
 }  // namespace
@@ -1056,6 +1060,7 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     ctx->ngb_per = p->period;
     R *bal = p->balsara ? P_<R>(ctx->bal) : nullptr;
     const bool g = p->G != 0.0;
+    CK(ensure(ctx, ctx->s4, (size_t)n * 4 * sizeof(R)));
     // density + EOS, fused with div/curl v and the Balsara factor when the switch is on; then
     // the acceleration (a = a_grav + a_sph)
     auto hydro_forces = [&]() -> int {
@@ -1064,14 +1069,15 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
         ph_begin(ctx, SPH_PH_DENSITY);
         if (p->balsara)
             HIPCK(sph::density_divcurl<R>(st, n, D(u), p->gamma, rows, P_<R>(ctx->rho), P_<R>(ctx->P), P_<R>(ctx->c),
-                                          P_<R>(ctx->prho2), P_<R>(ctx->divV), P_<R>(ctx->curlV), bal));
+                                          P_<R>(ctx->prho2), P_<R>(ctx->divV), P_<R>(ctx->curlV), bal,
+                                          P_<R4<R>>(ctx->s4)));
         else
             HIPCK(sph::density<R>(st, n, D(u), p->gamma, rows, P_<R>(ctx->rho), P_<R>(ctx->P), P_<R>(ctx->c),
-                                  P_<R>(ctx->prho2)));
+                                  P_<R>(ctx->prho2), P_<R4<R>>(ctx->s4)));
         ph_end(ctx, SPH_PH_DENSITY);
         ph_begin(ctx, SPH_PH_ACCSPH);
-        HIPCK(sph::accsph<R>(st, n, P_<R>(ctx->rho), P_<R>(ctx->prho2), P_<R>(ctx->c), bal, p->alpha, rows,
-                             g ? D(gx) : nullptr, g ? D(gy) : nullptr, g ? D(gz) : nullptr, D(ax), D(ay), D(az)));
+        HIPCK(sph::accsph<R>(st, n, P_<R4<R>>(ctx->s4), bal != nullptr, p->alpha, rows, g ? D(gx) : nullptr,
+                             g ? D(gy) : nullptr, g ? D(gz) : nullptr, D(ax), D(ay), D(az)));
         ph_end(ctx, SPH_PH_ACCSPH);
         return SPH_OK;
     };
@@ -1102,8 +1108,7 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     ph_begin(ctx, SPH_PH_DUDT);
     NgbRows rows;
     CK(ngb_rows(ctx, rows));  // packs the kicked velocities
-    HIPCK(sph::dudt<R>(st, n, P_<R>(ctx->rho), P_<R>(ctx->prho2),
-                       P_<R>(ctx->c), bal, p->alpha, rows, P_<R>(ctx->du)));
+    HIPCK(sph::dudt<R>(st, n, P_<R4<R>>(ctx->s4), bal != nullptr, p->alpha, rows, P_<R>(ctx->du)));
     ph_end(ctx, SPH_PH_DUDT);
     ph_begin(ctx, SPH_PH_UUPDATE);
     HIPCK(sph::u_update<R>(st, n, D(u), P_<R>(ctx->du), D(duprev), p->dt, first));

@@ -30,31 +30,41 @@ __device__ __forceinline__ void row_range(const NgbRows &r, int64_t i, int64_t &
     }
 }
 
+// Four-wide record of the pass type: float4 (mixed form) or double4 (FP64 form).
+template <typename R> struct R4T;
+template <> struct R4T<float> { using type = float4; };
+template <> struct R4T<double> { using type = double4; };
+template <typename R> using R4 = typename R4T<R>::type;
+
 namespace sph {
 
 // out[i] = (a[i], b[i], c[i], d[i]).
 hipError_t pack4(hipStream_t st, int64_t n, const double *a, const double *b, const double *c, const double *d,
                  double4 *out);
+// s4[i] = (rho, P/rho^2, c, Balsara f or 1 when bal is null): the packed record the
+// acceleration and du/dt passes gather.
+template <typename R>
+hipError_t pack_s4(hipStream_t st, int64_t n, const R *rho, const R *prho2, const R *c, const R *bal, R4<R> *s4);
 // rho = sum m_j W(r_ij, hbar_ij); fused EOS: P = (gamma-1) rho u,
-// c = sqrt(gamma (gamma-1) u), prho2 = P / rho^2 (skipped when u is null).
+// c = sqrt(gamma (gamma-1) u), prho2 = P / rho^2 (skipped when u is null); s4 (may be
+// null, needs u): the packed record with f = 1.
 template <typename R>
 hipError_t density(hipStream_t st, int64_t n, const double *u, double gamma, const NgbRows &rows, R *rho, R *P,
-                   R *c, R *prho2);
+                   R *c, R *prho2, R4<R> *s4 = nullptr);
 // |div v|, |curl v| and (bal non-null) Balsara f = |div v| / (|div v| + |curl v| + 1e-4 c/h).
 template <typename R>
 hipError_t divcurl(hipStream_t st, int64_t n, const R *rho, const R *c, const NgbRows &rows, R *divV, R *curlV,
                    R *bal);
-// density() and divcurl() in one pass (u required; bal may be null).
+// density() and divcurl() in one pass (u required; bal may be null; s4 may be null).
 template <typename R>
 hipError_t density_divcurl(hipStream_t st, int64_t n, const double *u, double gamma, const NgbRows &rows, R *rho,
-                           R *P, R *c, R *prho2, R *divV, R *curlV, R *bal);
-// a = a_grav (optional, may be null) + pressure/AV acceleration; bal null = no Balsara.
+                           R *P, R *c, R *prho2, R *divV, R *curlV, R *bal, R4<R> *s4 = nullptr);
+// a = a_grav (optional, may be null) + pressure/AV acceleration from the packed records
+// s4 (pack_s4); balsara = false: no Balsara switch (f unused).
 template <typename R>
-hipError_t accsph(hipStream_t st, int64_t n, const R *rho, const R *prho2, const R *c, const R *bal, double alpha,
-                  const NgbRows &rows, const double *gx, const double *gy, const double *gz, double *ax, double *ay,
-                  double *az);
+hipError_t accsph(hipStream_t st, int64_t n, const R4<R> *s4, bool balsara, double alpha, const NgbRows &rows,
+                  const double *gx, const double *gy, const double *gz, double *ax, double *ay, double *az);
 template <typename R>
-hipError_t dudt(hipStream_t st, int64_t n, const R *rho, const R *prho2, const R *c, const R *bal, double alpha,
-                const NgbRows &rows, R *out);
+hipError_t dudt(hipStream_t st, int64_t n, const R4<R> *s4, bool balsara, double alpha, const NgbRows &rows, R *out);
 
 }  // namespace sph

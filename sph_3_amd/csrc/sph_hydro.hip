@@ -20,7 +20,7 @@ template <typename R, int G, bool PER>
 __global__ void __launch_bounds__(256) k_density(int64_t n, const double *__restrict__ u, double gamma,
                                                  NgbRows rows,
                                                  R *__restrict__ rho, R *__restrict__ P, R *__restrict__ c,
-                                                 R *__restrict__ prho2)
+                                                 R *__restrict__ prho2, R4<R> *__restrict__ s4)
 {
     int64_t gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     int64_t i = gid / G;
@@ -48,9 +48,11 @@ __global__ void __launch_bounds__(256) k_density(int64_t n, const double *__rest
         if (u) {
             R ui = (R)u[i];
             R Pi = (R)(gamma - 1.0) * s * ui;
+            const R ci = sph_sqrt<R>((R)(gamma * (gamma - 1.0)) * ui), pr = Pi / (s * s);
             P[i] = Pi;
-            c[i] = sph_sqrt<R>((R)(gamma * (gamma - 1.0)) * ui);
-            prho2[i] = Pi / (s * s);
+            c[i] = ci;
+            prho2[i] = pr;
+            if (s4) s4[i] = R4<R>{s, pr, ci, (R)1};
         }
     }
 }
@@ -111,7 +113,8 @@ __global__ void __launch_bounds__(256) k_divcurl(int64_t n, const R *__restrict_
 template <typename R, int G, bool PER>
 __global__ void __launch_bounds__(256) k_density_divcurl(
     int64_t n, const double *__restrict__ u, double gamma, NgbRows rows, R *__restrict__ rho, R *__restrict__ P,
-    R *__restrict__ c, R *__restrict__ prho2, R *__restrict__ divV, R *__restrict__ curlV, R *__restrict__ bal)
+    R *__restrict__ c, R *__restrict__ prho2, R *__restrict__ divV, R *__restrict__ curlV, R *__restrict__ bal,
+    R4<R> *__restrict__ s4)
 {
     int64_t gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     int64_t i = gid / G;
@@ -153,16 +156,22 @@ __global__ void __launch_bounds__(256) k_density_divcurl(
         R ui = (R)u[i];
         R Pi = (R)(gamma - 1.0) * s * ui;
         R ci = sph_sqrt<R>((R)(gamma * (gamma - 1.0)) * ui);
+        const R pr = Pi / (s * s);
         P[i] = Pi;
         c[i] = ci;
-        prho2[i] = Pi / (s * s);
+        prho2[i] = pr;
         R ir = (R)1 / s;
         R dv = d * ir;
         dv = dv < (R)0 ? -dv : dv;
         R cv = sph_sqrt<R>(cx * cx + cy * cy + cz * cz) * ir;
         divV[i] = dv;
         curlV[i] = cv;
-        if (bal) bal[i] = dv / (dv + cv + (R)0.0001 * ci / (R)rows.p4[i].w);
+        R f = (R)1;
+        if (bal) {
+            f = dv / (dv + cv + (R)0.0001 * ci / (R)rows.p4[i].w);
+            bal[i] = f;
+        }
+        if (s4) s4[i] = R4<R>{s, pr, ci, f};
     }
 }
 
@@ -177,9 +186,8 @@ __device__ __forceinline__ R av_pi(R vr, R rr, R ci, R cj, R rhoi, R rhoj, R alp
 }
 
 template <typename R, int G, bool PER>
-__global__ void __launch_bounds__(256) k_accsph(int64_t n, const R *__restrict__ rho,
-                                                const R *__restrict__ prho2, const R *__restrict__ c,
-                                                const R *__restrict__ bal, double alpha,
+__global__ void __launch_bounds__(256) k_accsph(int64_t n, const R4<R> *__restrict__ s4, bool balsara,
+                                                double alpha,
                                                 NgbRows rows,
                                                 const double *__restrict__ gx_in, const double *__restrict__ gy_in,
                                                 const double *__restrict__ gz_in, double *__restrict__ ax,
@@ -195,7 +203,8 @@ __global__ void __launch_bounds__(256) k_accsph(int64_t n, const R *__restrict__
         const double xi = pi4.x, yi = pi4.y, zi = pi4.z, hi = pi4.w;
         const double4 vi4 = rows.v4[i];
         const double vxi = vi4.x, vyi = vi4.y, vzi = vi4.z;
-        const R rhoi = rho[i], pi_ = prho2[i], ci = c[i], fi = bal ? bal[i] : (R)1;
+        const R4<R> si = s4[i];
+        const R rhoi = si.x, pi_ = si.y, ci = si.z, fi = si.w;
         const R al = (R)alpha;
         int64_t k0, e;
         row_range(rows, i, k0, e);
@@ -208,9 +217,10 @@ __global__ void __launch_bounds__(256) k_accsph(int64_t n, const R *__restrict__
             R nW = spline_nW<R>(rr, (R)(0.5 * (hi + pj.w)));
             R ux = (R)(vxi - vj.x), uy = (R)(vyi - vj.y), uz = (R)(vzi - vj.z);
             R vr = ux * dx + uy * dy + uz * dz;
-            R PI = av_pi<R>(vr, rr, ci, c[j], rhoi, rho[j], al);
-            if (bal) PI = (R)0.5 * (fi + bal[j]) * PI;
-            R t = (R)vj.w * (pi_ + prho2[j] + PI) * nW;
+            const R4<R> sj = s4[j];
+            R PI = av_pi<R>(vr, rr, ci, sj.z, rhoi, sj.x, al);
+            if (balsara) PI = (R)0.5 * (fi + sj.w) * PI;
+            R t = (R)vj.w * (pi_ + sj.y + PI) * nW;
             sx.add(-t * dx);
             sy.add(-t * dy);
             sz.add(-t * dz);
@@ -228,9 +238,8 @@ __global__ void __launch_bounds__(256) k_accsph(int64_t n, const R *__restrict__
 }
 
 template <typename R, int G, bool PER>
-__global__ void __launch_bounds__(256) k_dudt(int64_t n, const R *__restrict__ rho,
-                                              const R *__restrict__ prho2, const R *__restrict__ c,
-                                              const R *__restrict__ bal, double alpha,
+__global__ void __launch_bounds__(256) k_dudt(int64_t n, const R4<R> *__restrict__ s4, bool balsara,
+                                              double alpha,
                                               NgbRows rows,
                                               R *__restrict__ dudt)
 {
@@ -244,7 +253,8 @@ __global__ void __launch_bounds__(256) k_dudt(int64_t n, const R *__restrict__ r
         const double xi = pi4.x, yi = pi4.y, zi = pi4.z, hi = pi4.w;
         const double4 vi4 = rows.v4[i];
         const double vxi = vi4.x, vyi = vi4.y, vzi = vi4.z;
-        const R rhoi = rho[i], pi_ = prho2[i], ci = c[i], fi = bal ? bal[i] : (R)1;
+        const R4<R> si = s4[i];
+        const R rhoi = si.x, pi_ = si.y, ci = si.z, fi = si.w;
         const R al = (R)alpha;
         int64_t k0, e;
         row_range(rows, i, k0, e);
@@ -257,8 +267,9 @@ __global__ void __launch_bounds__(256) k_dudt(int64_t n, const R *__restrict__ r
             R nW = spline_nW<R>(rr, (R)(0.5 * (hi + pj.w)));
             R ux = (R)(vxi - vj.x), uy = (R)(vyi - vj.y), uz = (R)(vzi - vj.z);
             R vr = ux * dx + uy * dy + uz * dz;
-            R PI = av_pi<R>(vr, rr, ci, c[j], rhoi, rho[j], al);
-            if (bal) PI = (R)0.5 * (fi + bal[j]) * PI;
+            const R4<R> sj = s4[j];
+            R PI = av_pi<R>(vr, rr, ci, sj.z, rhoi, sj.x, al);
+            if (balsara) PI = (R)0.5 * (fi + sj.w) * PI;
             s.add((R)vj.w * (pi_ + (R)0.5 * PI) * nW * vr);
         }
     }
@@ -292,10 +303,26 @@ hipError_t pack4(hipStream_t st, int64_t n, const double *a, const double *b, co
     } while (0)
 
 template <typename R>
-hipError_t density(hipStream_t st, int64_t n, const double *u, double gamma, const NgbRows &rows, R *rho, R *P,
-                   R *c, R *prho2)
+__global__ void __launch_bounds__(256) k_pack_s4(int64_t n, const R *__restrict__ rho, const R *__restrict__ prho2,
+                                                 const R *__restrict__ c, const R *__restrict__ bal,
+                                                 R4<R> *__restrict__ s4)
 {
-    LAUNCH(k_density, n, u, gamma, rows, rho, P, c, prho2);
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) s4[i] = R4<R>{rho[i], prho2[i], c[i], bal ? bal[i] : (R)1};
+}
+
+template <typename R>
+hipError_t pack_s4(hipStream_t st, int64_t n, const R *rho, const R *prho2, const R *c, const R *bal, R4<R> *s4)
+{
+    k_pack_s4<R><<<sph_blocks(n, 256), 256, 0, st>>>(n, rho, prho2, c, bal, s4);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t density(hipStream_t st, int64_t n, const double *u, double gamma, const NgbRows &rows, R *rho, R *P,
+                   R *c, R *prho2, R4<R> *s4)
+{
+    LAUNCH(k_density, n, u, gamma, rows, rho, P, c, prho2, u ? s4 : nullptr);
     return hipGetLastError();
 }
 
@@ -309,41 +336,38 @@ hipError_t divcurl(hipStream_t st, int64_t n, const R *rho, const R *c, const Ng
 
 template <typename R>
 hipError_t density_divcurl(hipStream_t st, int64_t n, const double *u, double gamma, const NgbRows &rows, R *rho,
-                           R *P, R *c, R *prho2, R *divV, R *curlV, R *bal)
+                           R *P, R *c, R *prho2, R *divV, R *curlV, R *bal, R4<R> *s4)
 {
-    LAUNCH(k_density_divcurl, n, u, gamma, rows, rho, P, c, prho2, divV, curlV, bal);
+    LAUNCH(k_density_divcurl, n, u, gamma, rows, rho, P, c, prho2, divV, curlV, bal, s4);
     return hipGetLastError();
 }
 
 template <typename R>
-hipError_t accsph(hipStream_t st, int64_t n, const R *rho, const R *prho2, const R *c, const R *bal, double alpha,
-                  const NgbRows &rows, const double *gx, const double *gy, const double *gz, double *ax, double *ay,
-                  double *az)
+hipError_t accsph(hipStream_t st, int64_t n, const R4<R> *s4, bool balsara, double alpha, const NgbRows &rows,
+                  const double *gx, const double *gy, const double *gz, double *ax, double *ay, double *az)
 {
-    LAUNCH(k_accsph, n, rho, prho2, c, bal, alpha, rows, gx, gy, gz, ax, ay, az);
+    LAUNCH(k_accsph, n, s4, balsara, alpha, rows, gx, gy, gz, ax, ay, az);
     return hipGetLastError();
 }
 
 template <typename R>
-hipError_t dudt(hipStream_t st, int64_t n, const R *rho, const R *prho2, const R *c, const R *bal, double alpha,
-                const NgbRows &rows, R *out)
+hipError_t dudt(hipStream_t st, int64_t n, const R4<R> *s4, bool balsara, double alpha, const NgbRows &rows, R *out)
 {
-    LAUNCH(k_dudt, n, rho, prho2, c, bal, alpha, rows, out);
+    LAUNCH(k_dudt, n, s4, balsara, alpha, rows, out);
     return hipGetLastError();
 }
 #undef LAUNCH
 
 #define INST(R)                                                                                                    \
+    template hipError_t pack_s4<R>(hipStream_t, int64_t, const R *, const R *, const R *, const R *, R4<R> *);    \
     template hipError_t density<R>(hipStream_t, int64_t, const double *, double, const NgbRows &, R *, R *, R *,  \
-                                   R *);                                                                           \
+                                   R *, R4<R> *);                                                                  \
     template hipError_t divcurl<R>(hipStream_t, int64_t, const R *, const R *, const NgbRows &, R *, R *, R *);   \
     template hipError_t density_divcurl<R>(hipStream_t, int64_t, const double *, double, const NgbRows &, R *,    \
-                                           R *, R *, R *, R *, R *, R *);                                          \
-    template hipError_t accsph<R>(hipStream_t, int64_t, const R *, const R *, const R *, const R *, double,       \
-                                  const NgbRows &, const double *, const double *, const double *, double *,      \
-                                  double *, double *);                                                            \
-    template hipError_t dudt<R>(hipStream_t, int64_t, const R *, const R *, const R *, const R *, double,         \
-                                const NgbRows &, R *);
+                                           R *, R *, R *, R *, R *, R *, R4<R> *);                                 \
+    template hipError_t accsph<R>(hipStream_t, int64_t, const R4<R> *, bool, double, const NgbRows &,             \
+                                  const double *, const double *, const double *, double *, double *, double *);  \
+    template hipError_t dudt<R>(hipStream_t, int64_t, const R4<R> *, bool, double, const NgbRows &, R *);
 INST(float)
 INST(double)
 #undef INST
