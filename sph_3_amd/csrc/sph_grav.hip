@@ -74,13 +74,15 @@ __device__ __forceinline__ void interact(const Src4<R> &a, const Src4<R> &b, R t
     }
     R r2 = dx * dx + dy * dy + dz * dz;
     R w;
-    if (SOFT == 1) {
+    if (SOFT >= 1) {
         R inv = sph_rsqrt<R>(r2 + e2);
         w = a.w * (inv * inv * inv);
     } else {
         w = a.w * grav_fk_spline<R>(sph_sqrt<R>(r2), (R)0.5 * (epsi + b.w));
     }
-    w = r2 > (R)0 ? w : (R)0;  // the target itself (and exact coincidences) add nothing
+    // the target itself (and exact coincidences) add nothing: with a positive Plummer
+    // softening (SOFT == 2) w stays finite and w * 0 = 0 already
+    if (SOFT != 2) w = r2 > (R)0 ? w : (R)0;
     sx = fma(w, dx, sx);
     sy = fma(w, dy, sy);
     sz = fma(w, dz, sz);
@@ -338,7 +340,10 @@ hipError_t gravity(hipStream_t st, int64_t ngroups, const int32_t *groups, const
     if (bucket > GG) return hipErrorInvalidValue;  // a bucket is expanded by one lane per particle
     if (ngroups <= 0) return hipSuccess;
     double theta2 = theta * theta;
-    if (soft_kind == 0)
+    if (soft_kind == 1 && eps_plummer > 0.0)
+        k_gravity_group<R, 2><<<(unsigned)ngroups, GG, 0, st>>>(groups, x, y, z, m, eps, nodes, bucket, theta2, G,
+                                                               eps_plummer, ax, ay, az, counters);
+    else if (soft_kind == 0)
         k_gravity_group<R, 0><<<(unsigned)ngroups, GG, 0, st>>>(groups, x, y, z, m, eps, nodes, bucket, theta2, G,
                                                                eps_plummer, ax, ay, az, counters);
     else
