@@ -143,7 +143,6 @@ __global__ void __launch_bounds__(64) k_hwalk(HSolveArgs a)
     ttx[lane] = xi; tty[lane] = yi; ttz[lane] = zi;
     tR2[lane] = valid ? R2 : -1.0;
     const int cap = a.cap;
-    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
     int c = 0;  // lane t: candidates of target t so far
     long long ntest = 0;  // algorithmic work: distance tests
     // Candidates: lanes take the staged sources, the group's targets in turn (their
@@ -198,14 +197,16 @@ __global__ void __launch_bounds__(64) k_hwalk(HSolveArgs a)
                     if (hm) {
                         const int ct = __builtin_amdgcn_readlane(c, tt);
                         if (hit) {
-                            const int pos = ct + (int)__popcll(hm & below);
+                            const int pos =
+                                ct + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(hm >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)hm, 0u));
                             if (pos < cap) {
                                 const int64_t o = (first + tt) * (int64_t)cap + pos;
                                 a.cand[o] = r2;
                                 if (a.cidx) a.cidx[o] = pj;
                             }
                         }
-                        if (lane == tt) c += (int)__popcll(hm);
+                        c = (lane == tt) ? ct + (int)__popcll(hm) : c;
                     }
                 }
             }
@@ -219,10 +220,33 @@ __global__ void __launch_bounds__(64) k_hwalk(HSolveArgs a)
 // Newton (mode 0) or k-th distance (mode 2) solve of the pending targets from their
 // candidate rows, 4 targets per wavefront and 16 lanes per target: lane l of a
 // sub-group holds candidates l, l+16, ... (up to 16, in registers) and the sub-group
-// reduces with butterflies, which give its 8 lanes bitwise the same sums, so each
+// reduces with DPP lane exchanges, which give its 16 lanes bitwise the same sums, so each
 // sub-group's iteration is uniform.  A target whose row overflowed gets R * 0.75, one
 // whose solution lies beyond R/2 gets R * 1.5, and stays pending for another round.
 #define HN_LANES 16  // lanes per target of k_hnewton
+
+// One DPP lane move of a double (two 32-bit moves); every source lane of the patterns used
+// here lies in the reading lane's own row of 16.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over each row of 16 lanes with the same bits in every lane of the row: each step adds
+// a partner lane's value and partners exchange (IEEE addition commutes): xor 1 and xor 2
+// (quad permutes), then the mirror within 8 lanes and the mirror within 16 lanes.
+__device__ __forceinline__ double row_sum16(double v)
+{
+    v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141>(v);  // row_half_mirror
+    v += dpp_f64<0x140>(v);  // row_mirror
+    return v;
+}
 
 __global__ void __launch_bounds__(64) k_hnewton(HSolveArgs a)
 {
@@ -239,10 +263,18 @@ __global__ void __launch_bounds__(64) k_hnewton(HSolveArgs a)
     const double Rj = valid ? a.R[j] : 0.0, hj0 = valid ? a.h[j] : 0.0;
     const double *rowj = a.cand + (solve ? j : 0) * (int64_t)cap;
     const int nsl = (cj + HL - 1) / HL;  // uniform within the sub-group
+    // the most slots a sub-group of the wave uses: the slot loops run to this wave-uniform
+    // bound (scalar branches); a slot past a target's own candidates holds 4 R, whose
+    // kernel terms are exactly zero for every h <= R/2
+    // (slots u >= nslw are neither loaded nor read)
+    int nslw = nsl;
+    for (int off = 32; off >= HL; off >>= 1) nslw = max(nslw, __shfl_xor(nslw, off));
+    nslw = __builtin_amdgcn_readfirstlane(nslw);
     long long nterm = 0;
     double r[HS];
 #pragma unroll
     for (int u = 0; u < HS; u++) {
+        if (u >= nslw) break;
         const int k = lsub + HL * u;
         r[u] = (k < cj) ? rowj[k] : INFINITY;  // squared distances
     }
@@ -251,8 +283,10 @@ __global__ void __launch_bounds__(64) k_hnewton(HSolveArgs a)
     if (solve && a.mode == 0) {
         // distances; empty slots get 4 R, beyond the support of every h <= R/2
 #pragma unroll
-        for (int u = 0; u < HS; u++)
-            if (u < nsl) r[u] = (r[u] < INFINITY) ? __builtin_sqrt(r[u]) : 4.0 * Rj;
+        for (int u = 0; u < HS; u++) {
+            if (u >= nslw) break;
+            r[u] = (r[u] < INFINITY) ? __builtin_sqrt(r[u]) : 4.0 * Rj;
+        }
         // f(h) = (32/3) sum_j w(r_j/h) - n_ngb and f'(h), with the branch-free
         // cubic spline w(q) = (2-q)_+^3/4 - (1-q)_+^3 and q w'(q) = q (3 (1-q)_+^2 - 3/4 (2-q)_+^2)
         auto feval = [&](double hh, double &f, double &df) {
@@ -261,17 +295,16 @@ __global__ void __launch_bounds__(64) k_hnewton(HSolveArgs a)
             const double ih = 1.0 / hh;
 #pragma unroll
             for (int u = 0; u < HS; u++) {
-                if (u >= nsl) break;
+                if (u >= nslw) break;
                 const double q = r[u] * ih;
                 const double aa = fmax(2.0 - q, 0.0), bb = fmax(1.0 - q, 0.0);
                 const double a2 = aa * aa, b2 = bb * bb;
                 sl = fma(0.25 * a2, aa, fma(-b2, bb, sl));
                 sq = fma(fma(3.0, b2, -0.75 * a2), q, sq);
             }
-            for (int off = HL / 2; off >= 1; off >>= 1) {
-                sl += __shfl_xor(sl, off, HL);
-                sq += __shfl_xor(sq, off, HL);
-            }
+            static_assert(HL == 16, "row_sum16 reduces rows of 16 lanes");
+            sl = row_sum16(sl);
+            sq = row_sum16(sq);
             f = (32.0 / 3.0) * sl - a.n_ngb;
             df = -(32.0 / 3.0) * sq * ih;
         };
@@ -337,12 +370,17 @@ __global__ void __launch_bounds__(64) k_hnewton(HSolveArgs a)
             for (;;) {
                 double mn = INFINITY;
 #pragma unroll
-                for (int u = 0; u < HS; u++)
+                for (int u = 0; u < HS; u++) {
+                    if (u >= nslw) break;
                     if (r[u] > cur && r[u] < mn) mn = r[u];
+                }
                 for (int off = HL / 2; off >= 1; off >>= 1) mn = fmin(mn, __shfl_xor(mn, off, HL));
                 int mult = 0;
 #pragma unroll
-                for (int u = 0; u < HS; u++) mult += (r[u] == mn);
+                for (int u = 0; u < HS; u++) {
+                    if (u >= nslw) break;
+                    mult += (r[u] == mn);
+                }
                 for (int off = HL / 2; off >= 1; off >>= 1) mult += __shfl_xor(mult, off, HL);
                 if (taken + mult >= need) { v = mn; break; }
                 taken += mult;
