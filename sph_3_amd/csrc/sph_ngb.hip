@@ -197,13 +197,14 @@ __global__ void __launch_bounds__(64) k_hwalk(HSolveArgs a)
                     if (hm) {
                         const int ct = __builtin_amdgcn_readlane(c, tt);
                         if (hit) {
-                            const int pos =
-                                ct + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(hm >> 32),
-                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)hm, 0u));
-                            if (pos < cap) {
-                                const int64_t o = (first + tt) * (int64_t)cap + pos;
-                                a.cand[o] = r2;
-                                if (a.cidx) a.cidx[o] = pj;
+                            const uint32_t pos =
+                                (uint32_t)ct + __builtin_amdgcn_mbcnt_hi((uint32_t)(hm >> 32),
+                                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)hm, 0u));
+                            if (pos < (uint32_t)cap) {
+                                // uniform row base + a 32-bit byte offset (scalar-base addressing)
+                                const int64_t row = (first + tt) * (int64_t)cap;
+                                *(double *)((char *)(a.cand + row) + pos * 8u) = r2;
+                                if (a.cidx) *(int32_t *)((char *)(a.cidx + row) + pos * 4u) = pj;
                             }
                         }
                         c = (lane == tt) ? ct + (int)__popcll(hm) : c;
