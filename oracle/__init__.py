@@ -229,9 +229,10 @@ class Tree:
             lib().or_tree_parents(_i64(N), _p(self.left), _p(self.right), _p(self.parent))
         else:
             self.parent[0] = -1
-        lib().or_tree_moments(_i64(N), _p(self.spos), _p(self.sm), _p(self.sh), _p(self.seps), _p(self.left),
-                              _p(self.right), _p(self.nbox), _p(self.com), _p(self.mass), _p(self.hmax),
-                              _p(self.epsmax), _p(self.size))
+        lib().or_tree_moments(_i64(N), _p(self.spos), _p(self.sm), _p(self.sh), _p(self.seps), _p(self.first),
+                              _p(self.count), _p(self.left), _p(self.right), ctypes.c_int(self.bucket),
+                              _p(self.nbox), _p(self.com), _p(self.mass), _p(self.hmax), _p(self.epsmax),
+                              _p(self.size))
 
     def gravity(self, theta=0.5, G=1.0, soft_kind=SOFT_SPLINE, eps_plummer=0.0, group=64):
         """Group-walk tree gravity (the device's interaction lists: the particles

@@ -544,36 +544,74 @@ static void combine(int64_t p, int64_t a, int64_t b, double *box, double *com, d
     size[p] = s;
 }
 
-/* Node moments from sorted particle data (spos = sorted positions).  Post-order
- * via an explicit stack; same combination formula as the device build. */
+/* Record of a node of at most `bucket` particles summed directly from its particles in
+ * index order: box, mass, centre of mass sum m x / sum m (box centre for zero mass),
+ * size = largest box extent, hmax, epsmax (the device's bucket records,
+ * sph_3_amd/csrc/sph_tree.hip bucket_record). */
+static void direct_record(int64_t id, int64_t f, int64_t c, const double *spos, const double *sm,
+                          const double *sh, const double *seps, double *box, double *com, double *mass,
+                          double *hmax, double *epsmax, double *size)
+{
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    double mt = 0.0, sx[3] = {0.0, 0.0, 0.0}, hm = 0.0, em = 0.0;
+    for (int64_t j = f; j < f + c; j++) {
+        double mj = sm ? sm[j] : 0.0;
+        for (int d = 0; d < 3; d++) {
+            double p = spos[3 * j + d];
+            lo[d] = p < lo[d] ? p : lo[d];
+            hi[d] = p > hi[d] ? p : hi[d];
+        }
+        mt = mt + mj;
+        for (int d = 0; d < 3; d++) sx[d] = sx[d] + mj * spos[3 * j + d];
+        if (sh && sh[j] > hm) hm = sh[j];
+        if (seps && seps[j] > em) em = seps[j];
+    }
+    for (int d = 0; d < 3; d++) {
+        box[6 * id + d] = lo[d];
+        box[6 * id + 3 + d] = hi[d];
+        com[3 * id + d] = mt > 0.0 ? sx[d] / mt : 0.5 * (lo[d] + hi[d]);
+    }
+    mass[id] = mt;
+    double s = hi[0] - lo[0];
+    if (hi[1] - lo[1] > s) s = hi[1] - lo[1];
+    if (hi[2] - lo[2] > s) s = hi[2] - lo[2];
+    size[id] = s;
+    hmax[id] = hm;
+    epsmax[id] = em;
+}
+
+/* Node moments from sorted particle data (spos = sorted positions): every node
+ * of at most `bucket` particles (leaves included) is summed directly from its
+ * particles, larger nodes combine their children in post-order (explicit
+ * stack) -- the device's records for the buckets and the nodes above them. */
 void or_tree_moments(int64_t N, const double *spos, const double *sm, const double *sh,
-                     const double *seps, const int32_t *left, const int32_t *right,
-                     double *box, double *com, double *mass, double *hmax, double *epsmax,
-                     double *size)
+                     const double *seps, const int32_t *first, const int32_t *count, const int32_t *left,
+                     const int32_t *right, int bucket, double *box, double *com, double *mass,
+                     double *hmax, double *epsmax, double *size)
 {
     int64_t nl = N - 1;
-    for (int64_t s = 0; s < N; s++) {
-        int64_t id = nl + s;
-        for (int d = 0; d < 3; d++) {
-            box[6 * id + d] = box[6 * id + 3 + d] = spos[3 * s + d];
-            com[3 * id + d] = spos[3 * s + d];
-        }
-        mass[id] = sm[s];
-        hmax[id] = sh ? sh[s] : 0.0;
-        epsmax[id] = seps ? seps[s] : 0.0;
-        size[id] = 0.0;
+    if (bucket < 1) bucket = 1;
+    if (N < 2 || count[0] <= bucket) {
+        for (int64_t k = 0; k < 2 * N - 1; k++)
+            if (count[k] <= bucket)
+                direct_record(k, first[k], count[k], spos, sm, sh, seps, box, com, mass, hmax, epsmax, size);
+        return;
     }
-    if (N < 2) return;
     int64_t *stack = (int64_t *)malloc(sizeof(int64_t) * 2 * N);
-    char *done = (char *)calloc(N, 1);
+    char *done = (char *)calloc(2 * N, 1);
+    for (int64_t k = 0; k < 2 * N - 1; k++)
+        if (count[k] <= bucket) {
+            direct_record(k, first[k], count[k], spos, sm, sh, seps, box, com, mass, hmax, epsmax, size);
+            done[k] = 1;
+        }
     int64_t sp = 0;
     stack[sp++] = 0;
     while (sp) {
         int64_t k = stack[sp - 1];
         int64_t a = left[k], b = right[k];
         int pend = 0;
-        if (b < nl && !done[b]) { stack[sp++] = b; pend = 1; }
-        if (a < nl && !done[a]) { stack[sp++] = a; pend = 1; }
+        if (!done[b]) { stack[sp++] = b; pend = 1; }
+        if (!done[a]) { stack[sp++] = a; pend = 1; }
         if (pend) continue;
         sp--;
         combine(k, a, b, box, com, mass, hmax, epsmax, size);

@@ -330,7 +330,7 @@ int get_vec3(sph_ctx *ctx, Buf &a, Buf &b, Buf &c, double *out)
 int ensure_moments(sph_ctx *ctx)
 {
     if (ctx->moments_ok) return SPH_OK;
-    HIPCK(sph::tree_moments(ctx->st, ctx->n, D(x), D(y), D(z), ctx->m_ok ? D(m) : nullptr,
+    HIPCK(sph::tree_moments(ctx->st, ctx->n, SPH_BUCKET, D(x), D(y), D(z), ctx->m_ok ? D(m) : nullptr,
                             ctx->h_ok ? D(h) : nullptr, ctx->eps_ok ? D(eps) : nullptr, P_<WNode>(ctx->nodes),
                             I32(right), I32(parent), P_<uint32_t>(ctx->flags), I32(front)));
     ctx->moments_ok = true;
@@ -567,6 +567,7 @@ int run_hsolve(sph_ctx *ctx, int64_t m, const double *tx, const double *ty, cons
     CK(hsolve_params(ctx, m, tx, ty, tz, mode, param, n_low, n_up, max_steps, tol, &unc, rounds_out, work,
                      keep_indices, p));
     int64_t pending = 0;
+    if (mode == SPH_H_REF_COUNT) HIPCK(sph::tree_escape(ctx->st, ctx->n, P_<WNode>(ctx->nodes)));  // stackless walk
     HIPCK(sph::hsolve(ctx->st, p, SPH_HROUNDS, &pending));
     if (pending) return fail(ctx, SPH_ERR_NOCONV, "smoothing-length solve: " + std::to_string(pending) +
                                                       " targets undecided after the maximum number of radius rounds");
@@ -979,7 +980,7 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     ph_end(ctx, SPH_PH_TOPOLOGY);
     // boxes are needed by the h walk; hmax is refreshed after the solve
     ph_begin(ctx, SPH_PH_HMOMENTS);
-    HIPCK(sph::tree_moments(st, n, D(x), D(y), D(z), D(m), D(h), nullptr, P_<WNode>(ctx->nodes), I32(right),
+    HIPCK(sph::tree_moments(st, n, SPH_BUCKET, D(x), D(y), D(z), D(m), D(h), nullptr, P_<WNode>(ctx->nodes), I32(right),
                             I32(parent), P_<uint32_t>(ctx->flags), I32(front)));
     ph_end(ctx, SPH_PH_HMOMENTS);
     ph_begin(ctx, SPH_PH_HSOLVE);
@@ -1017,7 +1018,7 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     if (spline) CK(finish_h());
     ph_begin(ctx, SPH_PH_MOMENTS);
     if (spline)
-        HIPCK(sph::tree_moments(st, n, D(x), D(y), D(z), D(m), D(h), eps, P_<WNode>(ctx->nodes), I32(right),
+        HIPCK(sph::tree_moments(st, n, SPH_BUCKET, D(x), D(y), D(z), D(m), D(h), eps, P_<WNode>(ctx->nodes), I32(right),
                                 I32(parent), P_<uint32_t>(ctx->flags), I32(front)));
     ph_end(ctx, SPH_PH_MOMENTS);
     ph_begin(ctx, SPH_PH_GRAVITY);

@@ -119,23 +119,34 @@ __device__ __forceinline__ void combine(WNode *__restrict__ nodes, int64_t p, in
 // Bottom-up moments without device-scope fences.  Device-wide release
 // fences write back the L2 of every XCD and cost microseconds each, and
 // same-address device atomics serialise, so the climb runs in levels whose
-// workgroups each own a contiguous particle range:
-//  (1) k_moments_local: one thread per leaf writes the leaf record and climbs
-//      while the parent's particle range lies inside this workgroup's
-//      MOM_BLOCK particles; the second child to arrive combines
-//      (workgroup-scope atomics on the arrival counters).  A finished node
-//      whose parent leaves the range joins the workgroup's frontier, kept in
-//      LDS and written to the block's own region of MOM_FMAX slots.
+// workgroups each own a contiguous range of node ids and particles:
+//  (1) k_moments_local: one thread per internal node of the workgroup's
+//      MOM_BLOCK ids.  Only the buckets (nodes of at most `bucket` particles
+//      whose parent holds more, or the root) and the nodes above them get
+//      records -- every walk stops at a bucket and stages its particles, so
+//      nothing reads a node below one.  A bucket's record is summed directly
+//      from its particles (in index order) by the thread of its id, or, for a
+//      single-particle bucket (a leaf child of a larger node), by the thread of
+//      its parent; that thread then climbs while the parent's particle range
+//      lies inside this workgroup's MOM_BLOCK particles (internal node k has k
+//      as an end of its range, so both children of such a parent belong to
+//      this workgroup); the second child to arrive combines (workgroup-scope
+//      atomics on the arrival counters).  A finished node whose parent leaves
+//      the range joins the workgroup's frontier, kept in LDS and written to
+//      the block's own region of MOM_FMAX slots.
 //  (2) k_moments_up: one workgroup per MOM_FAN level-1 blocks climbs from
 //      their frontier nodes the same way inside MOM_BLOCK*MOM_FAN particles,
 //      and a last single workgroup climbs from the remaining frontier to the
 //      root.
-// A block's frontier nodes are the maximal subtrees inside its range: at
-// most two per tree level, and the radix tree has at most 97 levels (64 key
-// plus 32 index bits of common prefix), so 194 < MOM_FMAX slots suffice.
+// A block's frontier nodes are the maximal subtrees inside its range (at most
+// two per tree level; the radix tree has at most 97 levels: 64 key plus 32
+// index bits of common prefix), the buckets that cross the range's ends (at
+// most two), and leaf buckets outside the range, each the child of a node of
+// this block that crosses one of the range's ends (nested, so at most 97 per
+// end): 2*97 + 2 + 2*97 = 390 < MOM_FMAX slots.
 #define MOM_BLOCK 1024
 #define MOM_FAN 64
-#define MOM_FMAX 256
+#define MOM_FMAX 512
 
 __device__ __forceinline__ void front_push(int32_t *flist, unsigned int *fn, int32_t node)
 {
@@ -150,7 +161,64 @@ __device__ __forceinline__ void front_flush(int32_t *flist, unsigned int fn, int
     if (threadIdx.x == 0) cnt[blk] = c;
 }
 
-__global__ void __launch_bounds__(MOM_BLOCK) k_moments_local(int64_t n, const double *__restrict__ x,
+// Record of a bucket: box, mass, centre of mass sum_j m_j x_j / sum_j m_j (sums in index
+// order; the box centre when the mass is zero), size = largest box extent, hmax, epsmax.
+__device__ __forceinline__ void bucket_record(WNode *__restrict__ nodes, int64_t id, int first, int cnt,
+                                              const double *__restrict__ x, const double *__restrict__ y,
+                                              const double *__restrict__ z, const double *__restrict__ m,
+                                              const double *__restrict__ h, const double *__restrict__ eps)
+{
+    double lx = INFINITY, ly = INFINITY, lz = INFINITY, hx = -INFINITY, hy = -INFINITY, hz = -INFINITY;
+    double mt = 0.0, sx = 0.0, sy = 0.0, sz = 0.0, hm = 0.0, em = 0.0;
+    for (int j = first; j < first + cnt; j++) {
+        const double px = x[j], py = y[j], pz = z[j], mj = m ? m[j] : 0.0;
+        lx = px < lx ? px : lx; ly = py < ly ? py : ly; lz = pz < lz ? pz : lz;
+        hx = px > hx ? px : hx; hy = py > hy ? py : hy; hz = pz > hz ? pz : hz;
+        mt = mt + mj;
+        sx = sx + mj * px; sy = sy + mj * py; sz = sz + mj * pz;
+        if (h && h[j] > hm) hm = h[j];
+        if (eps && eps[j] > em) em = eps[j];
+    }
+    WNode &P = nodes[id];
+    P.lx = lx; P.ly = ly; P.lz = lz;
+    P.hx = hx; P.hy = hy; P.hz = hz;
+    if (mt > 0.0) {
+        P.cx = sx / mt; P.cy = sy / mt; P.cz = sz / mt;
+    } else {
+        P.cx = 0.5 * (lx + hx); P.cy = 0.5 * (ly + hy); P.cz = 0.5 * (lz + hz);
+    }
+    P.mass = mt;
+    double sz3 = hx - lx;
+    if (hy - ly > sz3) sz3 = hy - ly;
+    if (hz - lz > sz3) sz3 = hz - lz;
+    P.size = sz3;
+    P.hmax = hm;
+    P.epsmax = em;
+}
+
+// Climb from the finished node id: combine each parent whose range lies inside this
+// workgroup's particle block when the second child arrives; hand the node to the frontier
+// when the parent leaves the block.
+__device__ __forceinline__ void climb_local(WNode *__restrict__ nodes, const int32_t *__restrict__ right,
+                                            const int32_t *__restrict__ parent, uint32_t *__restrict__ flags,
+                                            int32_t *flist, unsigned int *fn, int64_t id)
+{
+    int64_t p = parent[id];
+    while (p >= 0) {
+        const int64_t f = nodes[p].first, last = f + nodes[p].count - 1;
+        if (f / MOM_BLOCK != last / MOM_BLOCK) {  // parent leaves the block: hand over
+            front_push(flist, fn, (int32_t)id);
+            return;
+        }
+        const uint32_t old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (old == 0) return;  // first arrival: the sibling finishes this node
+        combine(nodes, p, nodes[p].left, right[p]);
+        id = p;
+        p = parent[p];
+    }
+}
+
+__global__ void __launch_bounds__(MOM_BLOCK) k_moments_local(int64_t n, int bucket, const double *__restrict__ x,
                                                              const double *__restrict__ y,
                                                              const double *__restrict__ z,
                                                              const double *__restrict__ m,
@@ -166,34 +234,26 @@ __global__ void __launch_bounds__(MOM_BLOCK) k_moments_local(int64_t n, const do
     __shared__ unsigned int fn;
     if (threadIdx.x == 0) fn = 0;
     __syncthreads();
-    const int64_t s = blockIdx.x * (int64_t)MOM_BLOCK + threadIdx.x;
-    if (s < n) {
-        const int64_t nl = n - 1;
-        int64_t id = nl + s;
-        {
-            WNode &L = nodes[id];
-            const double px = x[s], py = y[s], pz = z[s];
-            L.lx = L.hx = L.cx = px;
-            L.ly = L.hy = L.cy = py;
-            L.lz = L.hz = L.cz = pz;
-            L.mass = m ? m[s] : 0.0;
-            L.size = 0.0;
-            L.hmax = h ? h[s] : 0.0;
-            L.epsmax = eps ? eps[s] : 0.0;
-        }
-        int64_t p = n < 2 ? -1 : parent[id];
-        while (p >= 0) {
-            const int64_t f = nodes[p].first, last = f + nodes[p].count - 1;
-            if (f / MOM_BLOCK != last / MOM_BLOCK) {  // parent leaves the block: hand over
-                front_push(flist, &fn, (int32_t)id);
-                break;
+    const int64_t nl = n - 1;
+    const int64_t i = blockIdx.x * (int64_t)MOM_BLOCK + threadIdx.x;
+    if (n == 1) {
+        if (i == 0) bucket_record(nodes, 0, 0, 1, x, y, z, m, h, eps);  // the root is the single leaf
+    } else if (i < nl) {
+        const int ci = nodes[i].count;
+        if (ci <= bucket) {
+            const int64_t pi = parent[i];
+            if (pi < 0 || nodes[pi].count > bucket) {
+                bucket_record(nodes, i, nodes[i].first, ci, x, y, z, m, h, eps);
+                climb_local(nodes, right, parent, flags, flist, &fn, i);
             }
-            const uint32_t old =
-                __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (old == 0) break;  // first arrival: the sibling finishes this node
-            combine(nodes, p, nodes[p].left, right[p]);
-            id = p;
-            p = parent[p];
+        } else {
+            // a leaf child of a node above the buckets is a bucket of one particle
+            const int64_t c[2] = {nodes[i].left, right[i]};
+            for (int k = 0; k < 2; k++)
+                if (c[k] >= nl) {
+                    bucket_record(nodes, c[k], (int)(c[k] - nl), 1, x, y, z, m, h, eps);
+                    climb_local(nodes, right, parent, flags, flist, &fn, c[k]);
+                }
         }
     }
     __syncthreads();
@@ -263,6 +323,11 @@ hipError_t tree_topology(hipStream_t st, int64_t n, const uint64_t *skeys, WNode
                          int32_t *parent)
 {
     k_topology<<<sph_blocks(n, 256), 256, 0, st>>>(n, skeys, nodes, right, parent);
+    return hipGetLastError();
+}
+
+hipError_t tree_escape(hipStream_t st, int64_t n, WNode *nodes)
+{
     k_escape<<<sph_blocks(2 * n - 1, 256), 256, 0, st>>>(n, nodes);
     return hipGetLastError();
 }
@@ -273,17 +338,18 @@ int64_t moments_scratch_entries(int64_t n)
     return (b1 + b2) * (MOM_FMAX + 1);
 }
 
-hipError_t tree_moments(hipStream_t st, int64_t n, const double *x, const double *y, const double *z,
+hipError_t tree_moments(hipStream_t st, int64_t n, int bucket, const double *x, const double *y, const double *z,
                         const double *m, const double *h, const double *eps, WNode *nodes,
                         const int32_t *right, const int32_t *parent, uint32_t *flags, int32_t *frontier)
 {
+    if (bucket < 1) return hipErrorInvalidValue;
     // flags[0..n-2]: arrival counters of the internal nodes
     const int64_t b1 = (n + MOM_BLOCK - 1) / MOM_BLOCK, b2 = (b1 + MOM_FAN - 1) / MOM_FAN;
     if (b2 > 1024) return hipErrorInvalidValue;  // the last level's prefix table holds 1024 regions
     int32_t *reg1 = frontier, *reg2 = reg1 + b1 * MOM_FMAX, *cnt1 = reg2 + b2 * MOM_FMAX, *cnt2 = cnt1 + b1;
     if (n > 1) fill<uint32_t>(st, n - 1, flags, 0u);
-    k_moments_local<<<(unsigned)b1, MOM_BLOCK, 0, st>>>(n, x, y, z, m, h, eps, nodes, right, parent, flags, reg1,
-                                                        cnt1);
+    k_moments_local<<<(unsigned)b1, MOM_BLOCK, 0, st>>>(n, bucket, x, y, z, m, h, eps, nodes, right, parent, flags,
+                                                        reg1, cnt1);
     if (b1 > 1) {
         k_moments_up<<<(unsigned)b2, 1024, 0, st>>>(b1, MOM_FAN, (int64_t)MOM_BLOCK * MOM_FAN, nodes, right, parent,
                                                     flags, reg1, cnt1, reg2, cnt2);
