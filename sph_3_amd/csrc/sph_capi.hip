@@ -73,6 +73,7 @@ struct sph_ctx {
 
     // sorted particle state
     Buf x, y, z, vx, vy, vz, m, h, eps, u, duprev, ax, ay, az, gx, gy, gz, tmpd;
+    Buf shadow[SPH_GATHER_MAX];  // targets of the reorder gathers (swapped with the state)
     Buf orig, tmpi;
     // ordering
     Buf keys0, keys1, vals0, vals1, hist, bpart, box;
@@ -486,6 +487,8 @@ extern "C" int sph_ctx_destroy(sph_ctx *ctx)
                   &ctx->uL, &ctx->uG, &ctx->uflag, &ctx->ucnt, &ctx->hbig, &ctx->p4, &ctx->v4, &ctx->s4};
     for (Buf *b : all)
         if (b->p) (void)hipFree(b->p);
+    for (Buf &b : ctx->shadow)
+        if (b.p) (void)hipFree(b.p);
     Buf *cool[] = {&ctx->rf1,   &ctx->ctabL,  &ctx->ctabG,  &ctx->cin[0], &ctx->cin[1], &ctx->cin[2], &ctx->cin[3],
                    &ctx->cin[4], &ctx->cin[5], &ctx->cout[0], &ctx->cout[1], &ctx->mP,     &ctx->mG,     &ctx->mM,
                    &ctx->mV,     &ctx->mW[0],  &ctx->mW[1],  &ctx->mW[2],  &ctx->mW[3],  &ctx->mx,     &ctx->mxn,
@@ -943,19 +946,24 @@ int reorder_state(sph_ctx *ctx)
     hipStream_t st = ctx->st;
     CK(sort_current(ctx, n));
     const int32_t *perm = I32(vals0);
-    Buf *arr[] = {&ctx->x, &ctx->y, &ctx->z, &ctx->vx, &ctx->vy, &ctx->vz, &ctx->m, &ctx->h,
-                  &ctx->u, &ctx->duprev, &ctx->ax, &ctx->ay, &ctx->az};
-    for (Buf *b : arr) {
-        HIPCK(sph::gather<double, double>(st, n, perm, (const double *)b->p, 1, 0, D(tmpd)));
-        std::swap(*b, ctx->tmpd);
-    }
+    // every per-particle state array and the caller-index map in one gather pass into the
+    // shadow buffers, which then take the arrays' places
+    Buf *arr[SPH_GATHER_MAX] = {&ctx->x, &ctx->y, &ctx->z, &ctx->vx, &ctx->vy, &ctx->vz, &ctx->m, &ctx->h,
+                                &ctx->u, &ctx->duprev, &ctx->ax, &ctx->ay, &ctx->az};
+    int k = 13;
     if (ctx->cool_on) {
-        for (Buf *b : {&ctx->zmet, &ctx->j21}) {
-            HIPCK(sph::gather<double, double>(st, n, perm, (const double *)b->p, 1, 0, D(tmpd)));
-            std::swap(*b, ctx->tmpd);
-        }
+        arr[k++] = &ctx->zmet;
+        arr[k++] = &ctx->j21;
     }
-    HIPCK(sph::compose(st, n, I32(orig), perm, I32(tmpi)));
+    const double *src[SPH_GATHER_MAX];
+    double *dst[SPH_GATHER_MAX];
+    for (int j = 0; j < k; j++) {
+        CK(ensure(ctx, ctx->shadow[j], sizeof(double) * n));
+        src[j] = (const double *)arr[j]->p;
+        dst[j] = (double *)ctx->shadow[j].p;
+    }
+    HIPCK(sph::gather_set(st, n, perm, k, src, dst, I32(orig), I32(tmpi)));
+    for (int j = 0; j < k; j++) std::swap(*arr[j], ctx->shadow[j]);
     std::swap(ctx->orig, ctx->tmpi);
     return SPH_OK;
 }

@@ -21,6 +21,11 @@ hipError_t scan_counts(hipStream_t st, int64_t n, const int32_t *cnt, int64_t *o
 // dst[i] = src[perm[i]*stride + comp]
 template <typename T, typename U>
 hipError_t gather(hipStream_t st, int64_t n, const int32_t *perm, const T *src, int stride, int comp, U *dst);
+// dst[j][i] = src[j][perm[i]] for j < k <= SPH_GATHER_MAX FP64 arrays and, when idst is not
+// null, idst[i] = isrc[perm[i]], in one pass.
+#define SPH_GATHER_MAX 16
+hipError_t gather_set(hipStream_t st, int64_t n, const int32_t *perm, int k, const double *const *src,
+                      double *const *dst, const int32_t *isrc, int32_t *idst);
 // dst[perm[i]*stride + comp] = src[i]
 template <typename T>
 hipError_t scatter(hipStream_t st, int64_t n, const int32_t *perm, const T *src, int stride, int comp, double *dst);

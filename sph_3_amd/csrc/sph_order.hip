@@ -200,13 +200,15 @@ __global__ void __launch_bounds__(256) k_rs_scan_digit(int nblk, uint32_t *__res
 __global__ void __launch_bounds__(256) k_rs_scan_base(int nblk, uint32_t *__restrict__ hist,
                                                       const uint32_t *__restrict__ dtot)
 {
-    __shared__ uint32_t base;
-    if (threadIdx.x == 0) {
-        uint32_t s = 0;
-        for (unsigned d = 0; d < blockIdx.x; d++) s += dtot[d];
-        base = s;
-    }
+    // base = sum of the totals of the digits below this one (one load per thread, wave
+    // sums, then the four wave sums)
+    __shared__ uint32_t wsum[256 / SPH_WAVE];
+    uint32_t v = threadIdx.x < blockIdx.x ? dtot[threadIdx.x] : 0u;
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    if ((threadIdx.x & (SPH_WAVE - 1)) == 0) wsum[threadIdx.x / SPH_WAVE] = v;
     __syncthreads();
+    uint32_t base = 0;
+    for (int w = 0; w < 256 / SPH_WAVE; w++) base += wsum[w];
     uint32_t *a = hist + (int64_t)blockIdx.x * nblk;
     for (int i = threadIdx.x; i < nblk; i += 256) a[i] += base;
 }
@@ -334,6 +336,31 @@ __global__ void __launch_bounds__(256) k_gather(int64_t n, const int32_t *__rest
     if (i < n) dst[i] = (U)src[(int64_t)perm[i] * stride + comp];
 }
 
+// Up to SPH_GATHER_MAX FP64 arrays and one optional int32 array in one pass:
+// dst_k[i] = src_k[perm[i]], idst[i] = isrc[perm[i]].
+struct GatherSet {
+    const double *src[SPH_GATHER_MAX];
+    double *dst[SPH_GATHER_MAX];
+    const int32_t *isrc;
+    int32_t *idst;
+    int k;
+};
+
+__global__ void __launch_bounds__(256) k_gather_set(int64_t n, const int32_t *__restrict__ perm, GatherSet g)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t s = perm[i];
+    double v[SPH_GATHER_MAX];
+#pragma unroll
+    for (int k = 0; k < SPH_GATHER_MAX; k++)
+        if (k < g.k) v[k] = g.src[k][s];  // all loads issued before the stores
+#pragma unroll
+    for (int k = 0; k < SPH_GATHER_MAX; k++)
+        if (k < g.k) g.dst[k][i] = v[k];
+    if (g.idst) g.idst[i] = g.isrc[s];
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_scatter(int64_t n, const int32_t *__restrict__ perm,
                                                  const T *__restrict__ src, int stride, int comp,
@@ -423,6 +450,23 @@ template <typename T>
 hipError_t scatter(hipStream_t st, int64_t n, const int32_t *perm, const T *src, int stride, int comp, double *dst)
 {
     k_scatter<T><<<sph_blocks(n, 256), 256, 0, st>>>(n, perm, src, stride, comp, dst);
+    return hipGetLastError();
+}
+
+hipError_t gather_set(hipStream_t st, int64_t n, const int32_t *perm, int k, const double *const *src,
+                      double *const *dst, const int32_t *isrc, int32_t *idst)
+{
+    if (k < 0 || k > SPH_GATHER_MAX) return hipErrorInvalidValue;
+    if (n <= 0) return hipSuccess;
+    GatherSet g{};
+    for (int j = 0; j < k; j++) {
+        g.src[j] = src[j];
+        g.dst[j] = dst[j];
+    }
+    g.k = k;
+    g.isrc = isrc;
+    g.idst = idst;
+    k_gather_set<<<sph_blocks(n, 256), 256, 0, st>>>(n, perm, g);
     return hipGetLastError();
 }
 
