@@ -74,6 +74,7 @@ struct sph_ctx {
     // sorted particle state
     Buf x, y, z, vx, vy, vz, m, h, eps, u, duprev, ax, ay, az, gx, gy, gz, tmpd;
     Buf shadow[SPH_GATHER_MAX];  // targets of the reorder gathers (swapped with the state)
+    double m_max = INFINITY;     // largest particle mass of the current masses
     Buf orig, tmpi;
     // ordering
     Buf keys0, keys1, vals0, vals1, hist, bpart, box;
@@ -286,6 +287,11 @@ int put_scalar(sph_ctx *ctx, const double *src, Buf &dst, std::vector<double> *c
         *ok = true;
     }
     if (changed) *changed = true;
+    if (&dst == &ctx->m) {
+        double mx = 0.0;
+        for (int64_t i = 0; i < n; i++) mx = std::max(mx, std::fabs(src[i]));
+        ctx->m_max = mx;
+    }
     return SPH_OK;
 }
 
@@ -854,10 +860,12 @@ extern "C" int sph_gravity(sph_ctx *ctx, int64_t n, const double *pos, const dou
     const int64_t ng = ctx->ngroups;
     if (precision == SPH_PREC_MIXED)
         HIPCK(sph::gravity<float>(ctx->st, ng, I32(groups), D(x), D(y), D(z), D(m), e, P_<WNode>(ctx->nodes),
-                                  SPH_BUCKET, theta, G, soft_kind, eps_plummer, D(gx), D(gy), D(gz), cnt));
+                                  SPH_BUCKET, theta, G, soft_kind, eps_plummer, ctx->m_max, D(gx), D(gy), D(gz),
+                                  cnt));
     else
         HIPCK(sph::gravity<double>(ctx->st, ng, I32(groups), D(x), D(y), D(z), D(m), e, P_<WNode>(ctx->nodes),
-                                   SPH_BUCKET, theta, G, soft_kind, eps_plummer, D(gx), D(gy), D(gz), cnt));
+                                   SPH_BUCKET, theta, G, soft_kind, eps_plummer, ctx->m_max, D(gx), D(gy), D(gz),
+                                   cnt));
     unsigned long long hc[3];
     HIPCK(sph::spread_total(ctx->st, 3, cnt, P_<unsigned long long>(ctx->hcount) + 5));
     HIPCK(hipMemcpyAsync(hc, P_<unsigned long long>(ctx->hcount) + 5, sizeof(hc), hipMemcpyDeviceToHost, ctx->st));
@@ -1034,7 +1042,7 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     HIPCK(sph::fill<unsigned long long>(st, 3 * SPH_SPREAD_WORDS, cnt, 0ull));
     if (p->G != 0.0) {
         HIPCK(sph::gravity<R>(st, ctx->ngroups, I32(groups), D(x), D(y), D(z), D(m), eps, P_<WNode>(ctx->nodes), SPH_BUCKET,
-                              p->theta, p->G, p->soft_kind, p->eps_plummer, D(gx), D(gy), D(gz), cnt));
+                              p->theta, p->G, p->soft_kind, p->eps_plummer, ctx->m_max, D(gx), D(gy), D(gz), cnt));
     }
     HIPCK(sph::spread_total(st, 3, cnt, P_<unsigned long long>(ctx->hcount) + 5));
     ph_end(ctx, SPH_PH_GRAVITY);

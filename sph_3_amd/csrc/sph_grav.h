@@ -17,11 +17,12 @@ hipError_t gravity_groups(hipStream_t st, int64_t n, const WNode *nodes, const i
 // gives the exact direct sum.  counters: spread array (sph_common.h) of three
 // counters, 0 += particle-particle, 1 += particle-node interactions, 2 += stack
 // overflows
-// (must stay 0).
+// (must stay 0).  m_max: an upper bound of the particle masses (INFINITY if unknown), which
+// decides whether a Plummer softening > 0 keeps each particle's weight on itself finite.
 template <typename R>
 hipError_t gravity(hipStream_t st, int64_t ngroups, const int32_t *groups, const double *x, const double *y,
                    const double *z, const double *m, const double *eps, const WNode *nodes, int bucket, double theta,
-                   double G, int soft_kind, double eps_plummer, double *ax, double *ay, double *az,
+                   double G, int soft_kind, double eps_plummer, double m_max, double *ax, double *ay, double *az,
                    unsigned long long *counters);
 
 }  // namespace sph

@@ -72,12 +72,13 @@ __device__ __forceinline__ void interact(const Src4<R> &a, const Src4<R> &b, R t
         dy += b.y - tly;
         dz += b.z - tlz;
     }
-    R r2 = dx * dx + dy * dy + dz * dz;
-    R w;
+    R r2, w;
     if (SOFT >= 1) {
-        R inv = sph_rsqrt<R>(r2 + e2);
+        r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, e2)));  // r^2 + eps^2 (e2 = 0 for SOFT == 1)
+        R inv = sph_rsqrt<R>(r2);
         w = a.w * (inv * inv * inv);
     } else {
+        r2 = dx * dx + dy * dy + dz * dz;
         w = a.w * grav_fk_spline<R>(sph_sqrt<R>(r2), (R)0.5 * (epsi + b.w));
     }
     // the target itself (and exact coincidences) add nothing: with a positive Plummer
@@ -334,13 +335,18 @@ hipError_t gravity_groups(hipStream_t st, int64_t n, const WNode *nodes, const i
 template <typename R>
 hipError_t gravity(hipStream_t st, int64_t ngroups, const int32_t *groups, const double *x, const double *y,
                    const double *z, const double *m, const double *eps, const WNode *nodes, int bucket, double theta,
-                   double G, int soft_kind, double eps_plummer, double *ax, double *ay, double *az,
+                   double G, int soft_kind, double eps_plummer, double m_max, double *ax, double *ay, double *az,
                    unsigned long long *counters)
 {
     if (bucket > GG) return hipErrorInvalidValue;  // a bucket is expanded by one lane per particle
     if (ngroups <= 0) return hipSuccess;
     double theta2 = theta * theta;
-    if (soft_kind == 1 && eps_plummer > 0.0)
+    // Plummer softening without the self-pair select: a particle's term on itself is
+    // w * 0 with w = m / eps^3, exactly zero while w is finite in the pass type R
+    const double wlim = sizeof(R) == 4 ? 1e30 : 1e300;
+    const double e3 = eps_plummer * eps_plummer * eps_plummer;
+    const bool self_zero = eps_plummer > 0.0 && (R)(eps_plummer * eps_plummer) > (R)0 && m_max < wlim * e3;
+    if (soft_kind == 1 && self_zero)
         k_gravity_group<R, 2><<<(unsigned)ngroups, GG, 0, st>>>(groups, x, y, z, m, eps, nodes, bucket, theta2, G,
                                                                eps_plummer, ax, ay, az, counters);
     else if (soft_kind == 0)
@@ -354,9 +360,9 @@ hipError_t gravity(hipStream_t st, int64_t ngroups, const int32_t *groups, const
 
 template hipError_t gravity<float>(hipStream_t, int64_t, const int32_t *, const double *, const double *,
                                    const double *, const double *, const double *, const WNode *, int, double, double,
-                                   int, double, double *, double *, double *, unsigned long long *);
+                                   int, double, double, double *, double *, double *, unsigned long long *);
 template hipError_t gravity<double>(hipStream_t, int64_t, const int32_t *, const double *, const double *,
                                     const double *, const double *, const double *, const WNode *, int, double,
-                                    double, int, double, double *, double *, double *, unsigned long long *);
+                                    double, int, double, double, double *, double *, double *, unsigned long long *);
 
 }  // namespace sph
