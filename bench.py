@@ -66,7 +66,7 @@ def per_particle_bytes(kbar):
 KERNEL_OF_PHASE = {"order": "k_rs_scatter+k_gather", "moments": "k_moments_local+k_moments_up",
                    "hsolve": "k_hsolve", "ngb": "k_ngb<2>", "density": "k_density_divcurl<float,16>",
                    "divcurl": "k_divcurl<float,16>", "accsph": "k_accsph<float,16>", "dudt": "k_dudt<float,16>",
-                   "gravity": "k_gravity_group<float,1>", "moments_h": "k_moments_local+k_moments_up"}
+                   "gravity": "k_gravity_group<float,2>", "moments_h": "k_moments_local+k_moments_up"}
 
 
 def cpu_baseline(sim_state, n, m, rows, seed=0):

@@ -79,7 +79,7 @@ struct sph_ctx {
     // ordering
     Buf keys0, keys1, vals0, vals1, hist, bpart, box;
     // tree
-    Buf nodes, right, parent, flags, front, groups;
+    Buf nodes, right, parent, ncount, flags, front, groups;
     // neighbour lists
     Buf cnt, rcnt0, off, scantmp, idx;  // rcnt0: row lengths before the appended pairs
     Buf p4, v4, s4;                      // packed (x, y, z, h), (vx, vy, vz, m), (rho, P/rho^2, c, f) records
@@ -194,6 +194,7 @@ int capacity(sph_ctx *ctx, int64_t n)
     CK(ensure(ctx, ctx->nodes, (size_t)(2 * n) * sizeof(WNode)));
     CK(ensure(ctx, ctx->right, ni));
     CK(ensure(ctx, ctx->parent, 2 * ni));
+    CK(ensure(ctx, ctx->ncount, 2 * ni));
     CK(ensure(ctx, ctx->flags, ni));
     CK(ensure(ctx, ctx->front, std::max(2 * ni, (size_t)sph::moments_scratch_entries(n) * 4)));
     CK(ensure(ctx, ctx->groups, 2 * ni));
@@ -237,8 +238,8 @@ int sort_current(sph_ctx *ctx, int64_t n)
 int topology(sph_ctx *ctx)
 {
     HIPCK(sph::tree_topology(ctx->st, ctx->n, P_<uint64_t>(ctx->keys0), P_<WNode>(ctx->nodes), I32(right),
-                             I32(parent)));
-    HIPCK(sph::gravity_groups(ctx->st, ctx->n, P_<WNode>(ctx->nodes), I32(parent), I32(groups),
+                             I32(parent), I32(ncount)));
+    HIPCK(sph::gravity_groups(ctx->st, ctx->n, I32(ncount), I32(parent), I32(groups),
                               (unsigned int *)(P_<unsigned long long>(ctx->hcount) + 9), &ctx->ngroups));
     return SPH_OK;
 }
@@ -339,7 +340,7 @@ int ensure_moments(sph_ctx *ctx)
     if (ctx->moments_ok) return SPH_OK;
     HIPCK(sph::tree_moments(ctx->st, ctx->n, SPH_BUCKET, D(x), D(y), D(z), ctx->m_ok ? D(m) : nullptr,
                             ctx->h_ok ? D(h) : nullptr, ctx->eps_ok ? D(eps) : nullptr, P_<WNode>(ctx->nodes),
-                            I32(right), I32(parent), P_<uint32_t>(ctx->flags), I32(front)));
+                            I32(right), I32(parent), I32(ncount), P_<uint32_t>(ctx->flags), I32(front)));
     ctx->moments_ok = true;
     return SPH_OK;
 }
@@ -484,7 +485,7 @@ extern "C" int sph_ctx_destroy(sph_ctx *ctx)
     Buf *all[] = {&ctx->x, &ctx->y, &ctx->z, &ctx->vx, &ctx->vy, &ctx->vz, &ctx->m, &ctx->h, &ctx->eps,
                   &ctx->u, &ctx->duprev, &ctx->ax, &ctx->ay, &ctx->az, &ctx->gx, &ctx->gy, &ctx->gz,
                   &ctx->tmpd, &ctx->orig, &ctx->tmpi, &ctx->keys0, &ctx->keys1, &ctx->vals0, &ctx->vals1,
-                  &ctx->hist, &ctx->bpart, &ctx->box, &ctx->nodes, &ctx->right, &ctx->parent, &ctx->flags, &ctx->front, &ctx->groups,
+                  &ctx->hist, &ctx->bpart, &ctx->box, &ctx->nodes, &ctx->right, &ctx->parent, &ctx->ncount, &ctx->flags, &ctx->front, &ctx->groups,
                   &ctx->cnt, &ctx->rcnt0, &ctx->off, &ctx->scantmp, &ctx->idx, &ctx->rho, &ctx->P, &ctx->c, &ctx->prho2,
                   &ctx->divV, &ctx->curlV, &ctx->bal, &ctx->du, &ctx->cand, &ctx->hR, &ctx->hsteps,
                   &ctx->hstate, &ctx->hcount, &ctx->hout, &ctx->hin, &ctx->tx, &ctx->ty, &ctx->tz,
@@ -997,7 +998,7 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     // boxes are needed by the h walk; hmax is refreshed after the solve
     ph_begin(ctx, SPH_PH_HMOMENTS);
     HIPCK(sph::tree_moments(st, n, SPH_BUCKET, D(x), D(y), D(z), D(m), D(h), nullptr, P_<WNode>(ctx->nodes), I32(right),
-                            I32(parent), P_<uint32_t>(ctx->flags), I32(front)));
+                            I32(parent), I32(ncount), P_<uint32_t>(ctx->flags), I32(front)));
     ph_end(ctx, SPH_PH_HMOMENTS);
     ph_begin(ctx, SPH_PH_HSOLVE);
     HIPCK(hipMemcpyAsync(ctx->hin.p, ctx->h.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
@@ -1035,7 +1036,7 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     ph_begin(ctx, SPH_PH_MOMENTS);
     if (spline)
         HIPCK(sph::tree_moments(st, n, SPH_BUCKET, D(x), D(y), D(z), D(m), D(h), eps, P_<WNode>(ctx->nodes), I32(right),
-                                I32(parent), P_<uint32_t>(ctx->flags), I32(front)));
+                                I32(parent), I32(ncount), P_<uint32_t>(ctx->flags), I32(front)));
     ph_end(ctx, SPH_PH_MOMENTS);
     ph_begin(ctx, SPH_PH_GRAVITY);
     unsigned long long *cnt = P_<unsigned long long>(ctx->spread) + SPR_G * SPH_SPREAD_WORDS;

@@ -8,7 +8,7 @@ namespace sph {
 // Target groups: the maximal tree nodes with at most 64 particles.  Writes
 // their node ids to groups[] (capacity 2N-1) and the count to *ngroups
 // (synchronises the stream).
-hipError_t gravity_groups(hipStream_t st, int64_t n, const WNode *nodes, const int32_t *parent, int32_t *groups,
+hipError_t gravity_groups(hipStream_t st, int64_t n, const int32_t *ncount, const int32_t *parent, int32_t *groups,
                           unsigned int *ngroups_dev, int64_t *ngroups);
 
 // a_i = G sum m_j f(r_ij) (r_j - r_i) over the group's interaction list (self

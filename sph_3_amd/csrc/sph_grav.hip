@@ -289,7 +289,7 @@ __global__ void __launch_bounds__(GG) k_gravity_group(const int32_t *__restrict_
 // Groups: node k with count <= gmax whose parent holds more (or the root).
 // Compaction with one global atomic per 1024-thread block (wave offsets by
 // ballot, block offsets in LDS): same-address atomics serialise.
-__global__ void __launch_bounds__(1024) k_find_groups(int64_t nn, const WNode *__restrict__ nodes,
+__global__ void __launch_bounds__(1024) k_find_groups(int64_t nn, const int32_t *__restrict__ ncount,
                                                       const int32_t *__restrict__ parent, int gmax,
                                                       int32_t *__restrict__ groups, unsigned int *__restrict__ ngroups)
 {
@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(1024) k_find_groups(int64_t nn, const WNode *_
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     bool g = false;
-    if (k < nn && nodes[k].count <= gmax) g = (k == 0) || nodes[parent[k]].count > gmax;
+    if (k < nn && ncount[k] <= gmax) g = (k == 0) || ncount[parent[k]] > gmax;
     const uint64_t m = __ballot(g);
     if (lane == 0) wcount[wave] = (unsigned int)__popcll(m);
     __syncthreads();
@@ -319,11 +319,11 @@ __global__ void __launch_bounds__(1024) k_find_groups(int64_t nn, const WNode *_
 
 namespace sph {
 
-hipError_t gravity_groups(hipStream_t st, int64_t n, const WNode *nodes, const int32_t *parent, int32_t *groups,
+hipError_t gravity_groups(hipStream_t st, int64_t n, const int32_t *ncount, const int32_t *parent, int32_t *groups,
                           unsigned int *ngroups_dev, int64_t *ngroups)
 {
     fill<unsigned int>(st, 1, ngroups_dev, 0u);
-    k_find_groups<<<sph_blocks(2 * n - 1, 1024), 1024, 0, st>>>(2 * n - 1, nodes, parent, GG, groups, ngroups_dev);
+    k_find_groups<<<sph_blocks(2 * n - 1, 1024), 1024, 0, st>>>(2 * n - 1, ncount, parent, GG, groups, ngroups_dev);
     unsigned int h = 0;
     hipError_t e = hipMemcpyAsync(&h, ngroups_dev, sizeof(h), hipMemcpyDeviceToHost, st);
     if (e != hipSuccess) return e;

@@ -28,7 +28,8 @@ __device__ __forceinline__ int delta(const uint64_t *__restrict__ k, int64_t n, 
 __global__ void __launch_bounds__(256) k_topology(int64_t n, const uint64_t *__restrict__ k,
                                                   WNode *__restrict__ nodes,
                                                   int32_t *__restrict__ right,
-                                                  int32_t *__restrict__ parent)
+                                                  int32_t *__restrict__ parent,
+                                                  int32_t *__restrict__ ncount)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     int64_t nl = n - 1;
@@ -38,6 +39,7 @@ __global__ void __launch_bounds__(256) k_topology(int64_t n, const uint64_t *__r
         L->count = 1;
         L->left = -1;
         L->right = -1;
+        ncount[nl + i] = 1;
     }
     if (i >= n - 1) return;
     int d = (delta(k, n, i, i + 1) - delta(k, n, i, i - 1)) > 0 ? 1 : -1;
@@ -61,6 +63,7 @@ __global__ void __launch_bounds__(256) k_topology(int64_t n, const uint64_t *__r
     WNode *P = &nodes[i];
     P->first = (int)lo;
     P->count = (int)(hi - lo + 1);
+    ncount[i] = (int)(hi - lo + 1);
     P->left = lc;
     P->right = rc;
     right[i] = rc;
@@ -227,6 +230,7 @@ __global__ void __launch_bounds__(MOM_BLOCK) k_moments_local(int64_t n, int buck
                                                              WNode *__restrict__ nodes,
                                                              const int32_t *__restrict__ right,
                                                              const int32_t *__restrict__ parent,
+                                                             const int32_t *__restrict__ ncount,
                                                              uint32_t *__restrict__ flags,
                                                              int32_t *__restrict__ reg, int32_t *__restrict__ cnt)
 {
@@ -239,10 +243,10 @@ __global__ void __launch_bounds__(MOM_BLOCK) k_moments_local(int64_t n, int buck
     if (n == 1) {
         if (i == 0) bucket_record(nodes, 0, 0, 1, x, y, z, m, h, eps);  // the root is the single leaf
     } else if (i < nl) {
-        const int ci = nodes[i].count;
+        const int ci = ncount[i];
         if (ci <= bucket) {
             const int64_t pi = parent[i];
-            if (pi < 0 || nodes[pi].count > bucket) {
+            if (pi < 0 || ncount[pi] > bucket) {
                 bucket_record(nodes, i, nodes[i].first, ci, x, y, z, m, h, eps);
                 climb_local(nodes, right, parent, flags, flist, &fn, i);
             }
@@ -320,9 +324,9 @@ __global__ void __launch_bounds__(1024) k_moments_up(int64_t nblk_in, int fan, i
 namespace sph {
 
 hipError_t tree_topology(hipStream_t st, int64_t n, const uint64_t *skeys, WNode *nodes, int32_t *right,
-                         int32_t *parent)
+                         int32_t *parent, int32_t *ncount)
 {
-    k_topology<<<sph_blocks(n, 256), 256, 0, st>>>(n, skeys, nodes, right, parent);
+    k_topology<<<sph_blocks(n, 256), 256, 0, st>>>(n, skeys, nodes, right, parent, ncount);
     return hipGetLastError();
 }
 
@@ -340,7 +344,8 @@ int64_t moments_scratch_entries(int64_t n)
 
 hipError_t tree_moments(hipStream_t st, int64_t n, int bucket, const double *x, const double *y, const double *z,
                         const double *m, const double *h, const double *eps, WNode *nodes,
-                        const int32_t *right, const int32_t *parent, uint32_t *flags, int32_t *frontier)
+                        const int32_t *right, const int32_t *parent, const int32_t *ncount, uint32_t *flags,
+                        int32_t *frontier)
 {
     if (bucket < 1) return hipErrorInvalidValue;
     // flags[0..n-2]: arrival counters of the internal nodes
@@ -348,8 +353,8 @@ hipError_t tree_moments(hipStream_t st, int64_t n, int bucket, const double *x, 
     if (b2 > 1024) return hipErrorInvalidValue;  // the last level's prefix table holds 1024 regions
     int32_t *reg1 = frontier, *reg2 = reg1 + b1 * MOM_FMAX, *cnt1 = reg2 + b2 * MOM_FMAX, *cnt2 = cnt1 + b1;
     if (n > 1) fill<uint32_t>(st, n - 1, flags, 0u);
-    k_moments_local<<<(unsigned)b1, MOM_BLOCK, 0, st>>>(n, bucket, x, y, z, m, h, eps, nodes, right, parent, flags,
-                                                        reg1, cnt1);
+    k_moments_local<<<(unsigned)b1, MOM_BLOCK, 0, st>>>(n, bucket, x, y, z, m, h, eps, nodes, right, parent, ncount,
+                                                        flags, reg1, cnt1);
     if (b1 > 1) {
         k_moments_up<<<(unsigned)b2, 1024, 0, st>>>(b1, MOM_FAN, (int64_t)MOM_BLOCK * MOM_FAN, nodes, right, parent,
                                                     flags, reg1, cnt1, reg2, cnt2);
