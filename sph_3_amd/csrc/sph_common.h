@@ -96,6 +96,32 @@ __device__ __forceinline__ R spline_nW(R rr, R hij)
     return nW;
 }
 
+// FP32 forms of the two kernels for the mixed-precision passes: one hardware reciprocal of
+// h_ij (v_rcp_f32, 1 ulp) instead of the correctly rounded divisions, and both pieces
+// evaluated and selected instead of branching (the pieces vanish past q = 2 by t = 0).
+// Within a few ulp of the generic forms, which the FP64 passes keep.
+template <>
+__device__ __forceinline__ float spline_W<float>(float rr, float hij)
+{
+    const float sig = (float)(1.0 / SPH_PI);
+    const float ih = __builtin_amdgcn_rcpf(hij), q = rr * ih;
+    const float w1 = 1.0f - 1.5f * q * q + 0.75f * q * q * q;
+    const float t = fmaxf(2.0f - q, 0.0f);
+    const float w2 = 0.25f * t * t * t;
+    return sig * (ih * ih * ih) * (q <= 1.0f ? w1 : w2);
+}
+
+template <>
+__device__ __forceinline__ float spline_nW<float>(float rr, float hij)
+{
+    const float sig = (float)(1.0 / SPH_PI);
+    const float ih = __builtin_amdgcn_rcpf(hij), q = rr * ih, ih2 = ih * ih;
+    const float n1 = -3.0f + 2.25f * q;
+    const float t = fmaxf(2.0f - q, 0.0f);
+    const float n2 = -0.75f * t * t * __builtin_amdgcn_rcpf(q + 1e-20f);
+    return sig * (ih2 * ih2 * ih) * (q <= 1.0f ? n1 : n2);
+}
+
 // Spline-softened gravity factor, sph_code/libsx.py:640-652.
 template <typename R>
 __device__ __forceinline__ R grav_fk_spline(R rr, R e)
@@ -112,7 +138,8 @@ __device__ __forceinline__ R grav_fk_spline(R rr, R e)
 }
 
 template <typename R> __device__ __forceinline__ R sph_sqrt(R x);
-template <> __device__ __forceinline__ float sph_sqrt<float>(float x) { return __builtin_sqrtf(x); }
+// FP32: the hardware square root (v_sqrt_f32, 1 ulp) without the correctly rounded expansion
+template <> __device__ __forceinline__ float sph_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 template <> __device__ __forceinline__ double sph_sqrt<double>(double x) { return __builtin_sqrt(x); }
 
 template <typename R> __device__ __forceinline__ R sph_rsqrt(R x);

@@ -185,6 +185,17 @@ __device__ __forceinline__ R av_pi(R vr, R rr, R ci, R cj, R rhoi, R rhoj, R alp
     return (R)-0.5 * alpha * vsig * w / ((R)0.5 * (rhoi + rhoj));
 }
 
+// FP32 form: hardware reciprocals (1 ulp) and a select instead of the branch.
+template <>
+__device__ __forceinline__ float av_pi<float>(float vr, float rr, float ci, float cj, float rhoi, float rhoj,
+                                              float alpha)
+{
+    const float w = vr * __builtin_amdgcn_rcpf(rr + 1e-5f);
+    const float vsig = ci + cj - 3.0f * w;
+    const float pi = -0.5f * alpha * vsig * w * __builtin_amdgcn_rcpf(0.5f * (rhoi + rhoj));
+    return vr > 0.0f ? 0.0f : pi;
+}
+
 template <typename R, int G, bool PER>
 __global__ void __launch_bounds__(256) k_accsph(int64_t n, const R4<R> *__restrict__ s4, bool balsara,
                                                 double alpha,
