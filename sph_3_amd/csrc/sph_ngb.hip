@@ -659,13 +659,11 @@ __global__ void __launch_bounds__(256) k_cand_rows(int64_t n, int cap, const dou
     const int32_t *rj = cidx + (valid ? i : 0) * (int64_t)cap;
     int32_t *row = idx + (valid ? i : 0) * (int64_t)stride;
     int w = 0;
-    for (int k0 = 0; k0 < c; k0 += G) {
-        const int k = k0 + l;
+    // one candidate of the row: member test, ballot-compacted append, extra pair
+    auto visit = [&](bool have, int j, double r2, double hj) {
         bool in = false, extra = false;
-        int j = 0;
-        if (k < c) {
-            j = rj[k];
-            const double r2 = rr[k], hj = h[j], t = hi + hj, t2 = t * t;
+        if (have) {
+            const double t = hi + hj, t2 = t * t;
             in = r2 < t2;
             if (in && hi > hj && ccnt[j] <= cap) {  // (a row of k_hsolve_big's is walked: k_big_rows)
                 const double Rj = R[j];
@@ -683,6 +681,17 @@ __global__ void __launch_bounds__(256) k_cand_rows(int64_t n, int cap, const dou
             const unsigned long long e = atomicAdd(next + reg * SPH_SPREAD_PAD, 1ull);
             if (e < (unsigned long long)rcap) ext[reg * rcap + (int64_t)e] = make_int2(j, (int)i);
         }
+    };
+    // candidates k0 + l and k0 + G + l per step: both indices, distances and h_j are loaded
+    // before either is visited (two dependent gathers in flight per lane), visits in row order
+    for (int k0 = 0; k0 < c; k0 += 2 * G) {
+        const int ka = k0 + l, kb = ka + G;
+        const bool va = ka < c, vb = kb < c;
+        const int ja = va ? rj[ka] : 0, jb = vb ? rj[kb] : 0;
+        const double ra = va ? rr[ka] : 0.0, rb = vb ? rr[kb] : 0.0;
+        const double ha = va ? h[ja] : 0.0, hb = vb ? h[jb] : 0.0;
+        visit(va, ja, ra, ha);
+        visit(vb, jb, rb, hb);
     }
     // the row and counts of an incomplete candidate row are written by k_big_rows
     const bool own = valid && l == 0 && !incomplete;
