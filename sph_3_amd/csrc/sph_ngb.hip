@@ -636,6 +636,7 @@ __global__ void __launch_bounds__(64) k_ngb(const int32_t *__restrict__ groups, 
 // k_row_tail_sort puts each row's appended part in ascending order, so a row's
 // order (and every sum over it) does not depend on the order of the appends.
 // Spread counters: 0 += rows longer than stride, 1 += entries.
+#define CR_UNROLL 3  // candidates per lane step of k_cand_rows
 template <int G>
 __global__ void __launch_bounds__(256) k_cand_rows(int64_t n, int cap, const double *__restrict__ cand,
                                                    const int32_t *__restrict__ cidx, const int32_t *__restrict__ ccnt,
@@ -682,16 +683,22 @@ __global__ void __launch_bounds__(256) k_cand_rows(int64_t n, int cap, const dou
             if (e < (unsigned long long)rcap) ext[reg * rcap + (int64_t)e] = make_int2(j, (int)i);
         }
     };
-    // candidates k0 + l and k0 + G + l per step: both indices, distances and h_j are loaded
-    // before either is visited (two dependent gathers in flight per lane), visits in row order
-    for (int k0 = 0; k0 < c; k0 += 2 * G) {
-        const int ka = k0 + l, kb = ka + G;
-        const bool va = ka < c, vb = kb < c;
-        const int ja = va ? rj[ka] : 0, jb = vb ? rj[kb] : 0;
-        const double ra = va ? rr[ka] : 0.0, rb = vb ? rr[kb] : 0.0;
-        const double ha = va ? h[ja] : 0.0, hb = vb ? h[jb] : 0.0;
-        visit(va, ja, ra, ha);
-        visit(vb, jb, rb, hb);
+    // candidates k0 + l + u G (u < CR_UNROLL) per step: all their indices, distances and h_j
+    // are loaded before any is visited (CR_UNROLL dependent gathers in flight per lane),
+    // visits in row order
+    for (int k0 = 0; k0 < c; k0 += CR_UNROLL * G) {
+        int jv[CR_UNROLL];
+        double rv[CR_UNROLL], hv[CR_UNROLL];
+#pragma unroll
+        for (int u = 0; u < CR_UNROLL; u++) {
+            const int k = k0 + l + u * G;
+            jv[u] = k < c ? rj[k] : 0;
+            rv[u] = k < c ? rr[k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < CR_UNROLL; u++) hv[u] = (k0 + l + u * G < c) ? h[jv[u]] : 0.0;
+#pragma unroll
+        for (int u = 0; u < CR_UNROLL; u++) visit(k0 + l + u * G < c, jv[u], rv[u], hv[u]);
     }
     // the row and counts of an incomplete candidate row are written by k_big_rows
     const bool own = valid && l == 0 && !incomplete;
