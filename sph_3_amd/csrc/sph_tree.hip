@@ -201,9 +201,10 @@ __device__ __forceinline__ void bucket_record(WNode *__restrict__ nodes, int64_t
 
 // Climb from the finished node id: combine each parent whose range lies inside this
 // workgroup's particle block when the second child arrives; hand the node to the frontier
-// when the parent leaves the block.
+// when the parent leaves the block.  Such a parent's id lies in the block too (an internal
+// node's id is an end of its range), so its arrival counter is lflag[p - base] in LDS.
 __device__ __forceinline__ void climb_local(WNode *__restrict__ nodes, const int32_t *__restrict__ right,
-                                            const int32_t *__restrict__ parent, uint32_t *__restrict__ flags,
+                                            const int32_t *__restrict__ parent, uint32_t *lflag, int64_t base,
                                             int32_t *flist, unsigned int *fn, int64_t id)
 {
     int64_t p = parent[id];
@@ -213,7 +214,8 @@ __device__ __forceinline__ void climb_local(WNode *__restrict__ nodes, const int
             front_push(flist, fn, (int32_t)id);
             return;
         }
-        const uint32_t old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const uint32_t old =
+            __hip_atomic_fetch_add(&lflag[p - base], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
         if (old == 0) return;  // first arrival: the sibling finishes this node
         combine(nodes, p, nodes[p].left, right[p]);
         id = p;
@@ -235,11 +237,14 @@ __global__ void __launch_bounds__(MOM_BLOCK) k_moments_local(int64_t n, int buck
                                                              int32_t *__restrict__ reg, int32_t *__restrict__ cnt)
 {
     __shared__ int32_t flist[MOM_FMAX];
+    __shared__ uint32_t lflag[MOM_BLOCK];  // arrival counters of this block's internal nodes
     __shared__ unsigned int fn;
     if (threadIdx.x == 0) fn = 0;
+    lflag[threadIdx.x] = 0u;
     __syncthreads();
     const int64_t nl = n - 1;
-    const int64_t i = blockIdx.x * (int64_t)MOM_BLOCK + threadIdx.x;
+    const int64_t base = blockIdx.x * (int64_t)MOM_BLOCK;
+    const int64_t i = base + threadIdx.x;
     if (n == 1) {
         if (i == 0) bucket_record(nodes, 0, 0, 1, x, y, z, m, h, eps);  // the root is the single leaf
     } else if (i < nl) {
@@ -248,7 +253,7 @@ __global__ void __launch_bounds__(MOM_BLOCK) k_moments_local(int64_t n, int buck
             const int64_t pi = parent[i];
             if (pi < 0 || ncount[pi] > bucket) {
                 bucket_record(nodes, i, nodes[i].first, ci, x, y, z, m, h, eps);
-                climb_local(nodes, right, parent, flags, flist, &fn, i);
+                climb_local(nodes, right, parent, lflag, base, flist, &fn, i);
             }
         } else {
             // a leaf child of a node above the buckets is a bucket of one particle
@@ -256,7 +261,7 @@ __global__ void __launch_bounds__(MOM_BLOCK) k_moments_local(int64_t n, int buck
             for (int k = 0; k < 2; k++)
                 if (c[k] >= nl) {
                     bucket_record(nodes, c[k], (int)(c[k] - nl), 1, x, y, z, m, h, eps);
-                    climb_local(nodes, right, parent, flags, flist, &fn, c[k]);
+                    climb_local(nodes, right, parent, lflag, base, flist, &fn, c[k]);
                 }
         }
     }
