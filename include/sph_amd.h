@@ -224,6 +224,25 @@ int sph_cooling_update(sph_ctx *ctx, int64_t n, const double *rho, const double 
 int sph_cooling_temperature(sph_ctx *ctx, int64_t n, const double *u, const double *nH, const double *J21,
                             double XH, double *T_out);
 
+/* ---- column density from the density gradient --------------------------
+ * Replaces grad_rho_norm_ngb (Particles_in_BH_Tree_III/CLOUDY_reloaded_Here/run_simulation/
+ * hfvCLibs_v7_hfv_v8_gradRho_v4t.h:716-786) and the host loop after it
+ * (…/hfvGpuSphBHtree_gradRho_v3t.cu:1214-1224):
+ *   g_i = |sum_{j in S_i} m_j grad W(r_ij, (h_i+h_j)/2)|,
+ *   N_H,i = 0.5 (h_i / cbrt(n_ngb) + rho_i / g_i) unit_length_cm * rho_i unit_density_cgs XH / m_H
+ * with m_H = 1.673534e-24 g; log_NH = log10 N_H [cm^-2]. */
+typedef struct sph_column_params {
+    double unit_density_cgs; /* UnitDensity_in_cgs */
+    double unit_length_cm;   /* unitLength_in_cm */
+    double XH;               /* hydrogen mass fraction (0.7 there) */
+    double n_ngb;            /* Nngb_f of h_inter = h / cbrt(Nngb_f) (64 there) */
+} sph_column_params;
+/* g (may be NULL) and log_NH for particles (pos, m, h, rho) in caller order; the pair sums in
+ * the precision's type, the N_H arithmetic in FP64. */
+int sph_column_density(sph_ctx *ctx, int64_t n, const double *pos, const double *m, const double *h,
+                       const double *rho, const sph_column_params *cp, int precision, double *grad_out,
+                       double *log_nh_out);
+
 /* ---- device-resident time loop (sph_serial_v2.py:148-218 on the GPU) ---- */
 typedef struct sph_step_params {
     double dt;          /* fixed step */
@@ -264,7 +283,7 @@ typedef struct sph_stats {
 enum sph_phase {
     SPH_PH_KICKDRIFT = 0, SPH_PH_ORDER, SPH_PH_TOPOLOGY, SPH_PH_HSOLVE, SPH_PH_MOMENTS, SPH_PH_NGB,
     SPH_PH_DENSITY, SPH_PH_GRAVITY, SPH_PH_DIVCURL, SPH_PH_ACCSPH, SPH_PH_KICK, SPH_PH_DUDT,
-    SPH_PH_UUPDATE, SPH_PH_HMOMENTS, SPH_PH_COOLING, SPH_PH_COUNT
+    SPH_PH_UUPDATE, SPH_PH_HMOMENTS, SPH_PH_COOLING, SPH_PH_COLUMN, SPH_PH_COUNT
 };
 
 /* Load the resident state (caller order); u and h are per particle. */
@@ -276,6 +295,12 @@ int sph_state_get(sph_ctx *ctx, double *pos, double *vel, double *u, double *h, 
  * state and the cooling of every later sph_step, applied after its u update
  * (CR/v3t.cu:1527-1536); cp = NULL switches cooling off.  Call after sph_state_set. */
 int sph_state_set_cooling(sph_ctx *ctx, const double *Z, const double *J21, const sph_cool_params *cp);
+/* Column density of every later force evaluation (after its density pass, as the reference
+ * computes it each step, hfvGpuSphBHtree_gradRho_v3t.cu:1204-1226); cp = NULL switches it
+ * off.  Call after sph_state_set (which switches it off).  sph_state_get_column reads the last
+ * one back in caller order (either pointer may be NULL). */
+int sph_state_set_column(sph_ctx *ctx, const sph_column_params *cp);
+int sph_state_get_column(sph_ctx *ctx, double *grad_out, double *log_nh_out);
 /* Forces at the current positions and the first u update (sph_serial_v2.py:94-143). */
 int sph_prime(sph_ctx *ctx, const sph_step_params *p);
 /* One KDK step (sph_serial_v2.py:148-203), entirely on the device. */

@@ -300,6 +300,21 @@ def density_csr(pos, m, h, off, idx):
     return out
 
 
+def grad_rho_csr(pos, m, h, off, idx):
+    """|grad rho| of grad_rho_norm_ngb (FP64) over CSR sets."""
+    pos = _f64(pos, (-1, 3)); out = np.zeros(len(pos))
+    lib().or_grad_rho_csr(_i64(len(pos)), _p(pos), _p(_f64(m)), _p(_f64(h)), _p(off), _p(idx), _p(out))
+    return out
+
+
+def column_density(rho, g, h, unit_density_cgs, unit_length_cm, XH=0.7, n_ngb=64.0):
+    """log10 N_H [cm^-2] from rho, |grad rho| and h (the reference's host loop, FP64)."""
+    rho = _f64(rho); out = np.zeros(len(rho))
+    lib().or_column_density(_i64(len(rho)), _p(rho), _p(_f64(g)), _p(_f64(h)), ctypes.c_double(unit_density_cgs),
+                            ctypes.c_double(unit_length_cm), ctypes.c_double(XH), ctypes.c_double(n_ngb), _p(out))
+    return out
+
+
 def divcurl_csr(pos, v, rho, m, h, off, idx):
     pos = _f64(pos, (-1, 3)); N = len(pos)
     dv = np.zeros(N); cv = np.zeros(N)

@@ -917,6 +917,50 @@ void or_density_csr(int64_t N, const double *pos, const double *m, const double 
     }
 }
 
+/* |grad rho|_i = |sum_j m_j nW (x_j - x_i)| over the CSR sets, FP64: grad_rho_norm_ngb of
+ * Particles_in_BH_Tree_III/CLOUDY_reloaded_Here/run_simulation/hfvCLibs_v7_hfv_v8_gradRho_v4t.h:716-786
+ * (dx = x_j - x_i; hij = (h_i + h_j)/2; terms for q <= 2 only; nW = sig/hij^5 (-3 + 9/4 q) for
+ * q <= 1, -3 sig/(4 hij^5) (2 - q)^2 / (q + 1e-10) for 1 < q <= 2). */
+void or_grad_rho_csr(int64_t N, const double *pos, const double *m, const double *h,
+                     const int64_t *off, const int32_t *idx, double *g)
+{
+    const double sig = 1.0 / M_PI;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < N; i++) {
+        double gx = 0.0, gy = 0.0, gz = 0.0;
+        for (int64_t k = off[i]; k < off[i + 1]; k++) {
+            int64_t j = idx[k];
+            double dx = wrapd(pos[3 * j] - pos[3 * i]), dy = wrapd(pos[3 * j + 1] - pos[3 * i + 1]),
+                   dz = wrapd(pos[3 * j + 2] - pos[3 * i + 2]);
+            double rr = sqrt(dx * dx + dy * dy + dz * dz);
+            double hij = 0.5 * (h[i] + h[j]), q = rr / hij;
+            if (q > 2.0) continue;
+            double hij5 = hij * hij * hij * hij * hij, nW;
+            if (q <= 1.0) nW = sig / hij5 * (-3.0 + (9.0 / 4.0) * q);
+            else nW = -3.0 * sig / (4.0 * hij5) * (2.0 - q) * (2.0 - q) / (q + 1e-10);
+            gx += m[j] * nW * dx;
+            gy += m[j] * nW * dy;
+            gz += m[j] * nW * dz;
+        }
+        g[i] = sqrt(gx * gx + gy * gy + gz * gz);
+    }
+}
+
+/* log10 N_H per particle from rho, |grad rho| and h: the host loop of
+ * Particles_in_BH_Tree_III/CLOUDY_reloaded_Here/run_simulation/hfvGpuSphBHtree_gradRho_v3t.cu:1214-1224
+ * (nH = rho UnitDensity XH / mH, h_inter = h / cbrt(Nngb), N_H = 0.5 (h_inter + rho / g) L nH),
+ * in FP64 with mH = 1.673534e-24 (line 42). */
+void or_column_density(int64_t N, const double *rho, const double *g, const double *h, double unit_density_cgs,
+                       double unit_length_cm, double XH, double n_ngb, double *log_nh)
+{
+    const double mH = 1.673534e-24;
+    for (int64_t i = 0; i < N; i++) {
+        double nH = rho[i] * unit_density_cgs * XH / mH;
+        double h_inter = h[i] / cbrt(n_ngb);
+        log_nh[i] = log10(0.5 * (h_inter + rho[i] / g[i]) * unit_length_cm * nH);
+    }
+}
+
 void or_divcurl_csr(int64_t N, const double *pos, const double *v, const double *rho,
                     const double *m, const double *h, const int64_t *off, const int32_t *idx,
                     double *divV, double *curlV)

@@ -21,7 +21,7 @@ SOFT = {"spline": 0, "plummer": 1}
 H_NEWTON, H_REF_COUNT, H_KTH = 0, 1, 2
 
 PHASES = ["kick_drift", "order", "topology", "hsolve", "moments", "ngb", "density", "gravity", "divcurl",
-          "accsph", "kick", "dudt", "u_update", "moments_h", "cooling"]
+          "accsph", "kick", "dudt", "u_update", "moments_h", "cooling", "column"]
 
 _d = ctypes.POINTER(ctypes.c_double)
 _f = ctypes.POINTER(ctypes.c_float)
@@ -52,6 +52,11 @@ class CoolParams(ctypes.Structure):
     _fields_ = [("unit_density_cgs", ctypes.c_double), ("unit_u_cgs", ctypes.c_double),
                 ("unit_time_s", ctypes.c_double), ("XH", ctypes.c_double), ("u_floor_cgs", ctypes.c_double),
                 ("tol", ctypes.c_double), ("mode", ctypes.c_int), ("ref", ctypes.c_int)]
+
+
+class ColumnParams(ctypes.Structure):
+    _fields_ = [("unit_density_cgs", ctypes.c_double), ("unit_length_cm", ctypes.c_double),
+                ("XH", ctypes.c_double), ("n_ngb", ctypes.c_double)]
 
 
 COOL_MODE = {"exact": 0, "table": 1, "mlp": 2}
@@ -101,6 +106,10 @@ _SIGS = {
                             _i64p, _i64p], ctypes.c_int),
     "sph_cooling_temperature": ([_vp, _i64, _d, _d, _d, ctypes.c_double, _d], ctypes.c_int),
     "sph_state_set_cooling": ([_vp, _d, _d, ctypes.POINTER(CoolParams)], ctypes.c_int),
+    "sph_column_density": ([_vp, _i64, _d, _d, _d, _d, ctypes.POINTER(ColumnParams), ctypes.c_int, _d, _d],
+                           ctypes.c_int),
+    "sph_state_set_column": ([_vp, ctypes.POINTER(ColumnParams)], ctypes.c_int),
+    "sph_state_get_column": ([_vp, _d, _d], ctypes.c_int),
     "sph_get_stats": ([_vp, ctypes.POINTER(Stats)], ctypes.c_int),
     "sph_reset_stats": ([_vp], ctypes.c_int),
 }

@@ -107,6 +107,9 @@ struct sph_ctx {
     CoolTAxes tax{};
     double ttab_XH = -1.0;
     bool cool_on = false, cool_ran = false;
+    bool col_on = false, col_ok = false;  // column density in the step; computed for the current state
+    sph::ColumnParams col_p{};
+    Buf colg, coln;  // |grad rho|, log10 N_H (sorted order, FP64)
     sph_cool_params cool_p{};
     // cooling surrogate: parameters, Adam state, BF16 weight copies, samples, error map
     Buf mP, mG, mM, mV, mW[4], mx, mxn, mt, my, mloss, memap, memap2, memap3, mgate, mcnt;
@@ -491,7 +494,8 @@ extern "C" int sph_ctx_destroy(sph_ctx *ctx)
                   &ctx->hstate, &ctx->hcount, &ctx->hout, &ctx->hin, &ctx->tx, &ctx->ty, &ctx->tz,
                   &ctx->stage, &ctx->kpart, &ctx->kout, &ctx->spread, &ctx->ccidx, &ctx->cccnt,
                   &ctx->cext, &ctx->cnext, &ctx->ttab, &ctx->zmet, &ctx->j21, &ctx->ucols, &ctx->uyn,
-                  &ctx->uL, &ctx->uG, &ctx->uflag, &ctx->ucnt, &ctx->hbig, &ctx->p4, &ctx->v4, &ctx->s4};
+                  &ctx->uL, &ctx->uG, &ctx->uflag, &ctx->ucnt, &ctx->hbig, &ctx->p4, &ctx->v4, &ctx->s4,
+                  &ctx->colg, &ctx->coln};
     for (Buf *b : all)
         if (b->p) (void)hipFree(b->p);
     for (Buf &b : ctx->shadow)
@@ -909,6 +913,7 @@ extern "C" int sph_state_set(sph_ctx *ctx, int64_t n, const double *pos, const d
     if (!pos || !vel || !m || !u || !h) return fail(ctx, SPH_ERR_ARG, "null argument");
     ctx->resident = false;
     ctx->cool_on = false;  // Z and J21 belong to the previous state
+    ctx->col_on = ctx->col_ok = false;
     invalidate_all(ctx);
     CK(set_pos(ctx, n, pos));
     CK(put_vec3(ctx, vel, ctx->vx, ctx->vy, ctx->vz));
@@ -1093,6 +1098,12 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
             HIPCK(sph::density<R>(st, n, D(u), p->gamma, rows, P_<R>(ctx->rho), P_<R>(ctx->P), P_<R>(ctx->c),
                                   P_<R>(ctx->prho2), P_<R4<R>>(ctx->s4)));
         ph_end(ctx, SPH_PH_DENSITY);
+        if (ctx->col_on) {  // column density from this density pass (gradient over the same rows)
+            ph_begin(ctx, SPH_PH_COLUMN);
+            HIPCK(sph::column_density<R>(st, n, P_<R>(ctx->rho), rows, ctx->col_p, D(colg), D(coln)));
+            ph_end(ctx, SPH_PH_COLUMN);
+            ctx->col_ok = true;
+        }
         ph_begin(ctx, SPH_PH_ACCSPH);
         HIPCK(sph::accsph<R>(st, n, P_<R4<R>>(ctx->s4), bal != nullptr, p->alpha, rows, g ? D(gx) : nullptr,
                              g ? D(gy) : nullptr, g ? D(gz) : nullptr, D(ax), D(ay), D(az)));
@@ -1933,6 +1944,72 @@ extern "C" int sph_cooling_temperature(sph_ctx *ctx, int64_t n, const double *u,
                                   D(cout[0])));
     HIPCK(hipMemcpyAsync(T_out, ctx->cout[0].p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->st));
     HIPCK(hipStreamSynchronize(ctx->st));
+    return SPH_OK;
+}
+
+namespace {
+int column_check(sph_ctx *ctx, const sph_column_params *cp)
+{
+    if (!(cp->unit_density_cgs > 0.0) || !(cp->unit_length_cm > 0.0) || !(cp->XH > 0.0) || !(cp->n_ngb > 0.0))
+        return fail(ctx, SPH_ERR_ARG, "column density: unit_density_cgs, unit_length_cm, XH, n_ngb must be > 0");
+    return SPH_OK;
+}
+sph::ColumnParams column_params(const sph_column_params *cp)
+{
+    return sph::ColumnParams{cp->unit_density_cgs, cp->unit_length_cm, cp->XH, cp->n_ngb};
+}
+}  // namespace
+
+extern "C" int sph_column_density(sph_ctx *ctx, int64_t n, const double *pos, const double *m, const double *h,
+                                  const double *rho, const sph_column_params *cp, int precision, double *grad_out,
+                                  double *log_nh_out)
+{
+    CK(check_common(ctx, n, precision));
+    if (!pos || !m || !h || !rho || !cp || !log_nh_out) return fail(ctx, SPH_ERR_ARG, "null argument");
+    CK(column_check(ctx, cp));
+    CK(geom(ctx, n, pos, h, m));
+    CK(ensure(ctx, ctx->colg, (size_t)n * 8));
+    CK(ensure(ctx, ctx->coln, (size_t)n * 8));
+    NgbRows rows;
+    CK(ngb_rows(ctx, rows));
+    const sph::ColumnParams p = column_params(cp);
+    if (precision == SPH_PREC_MIXED) {
+        CK(put_real<float>(ctx, rho, ctx->rho));
+        HIPCK(sph::column_density<float>(ctx->st, n, P_<float>(ctx->rho), rows, p, D(colg), D(coln)));
+    } else {
+        CK(put_real<double>(ctx, rho, ctx->rho));
+        HIPCK(sph::column_density<double>(ctx->st, n, D(rho), rows, p, D(colg), D(coln)));
+    }
+    if (grad_out) CK(get_scalar<double>(ctx, ctx->colg, grad_out));
+    return get_scalar<double>(ctx, ctx->coln, log_nh_out);
+}
+
+extern "C" int sph_state_set_column(sph_ctx *ctx, const sph_column_params *cp)
+{
+    if (!ctx) return SPH_ERR_ARG;
+    if (!ctx->resident) return fail(ctx, SPH_ERR_STATE, "no resident state (call sph_state_set first)");
+    HIPCK(hipSetDevice(ctx->device));
+    if (!cp) {
+        ctx->col_on = false;
+        return SPH_OK;
+    }
+    CK(column_check(ctx, cp));
+    CK(ensure(ctx, ctx->colg, (size_t)ctx->n * 8));
+    CK(ensure(ctx, ctx->coln, (size_t)ctx->n * 8));
+    ctx->col_p = column_params(cp);
+    ctx->col_on = true;
+    return SPH_OK;
+}
+
+extern "C" int sph_state_get_column(sph_ctx *ctx, double *grad_out, double *log_nh_out)
+{
+    if (!ctx) return SPH_ERR_ARG;
+    if (!ctx->resident) return fail(ctx, SPH_ERR_STATE, "no resident state (call sph_state_set first)");
+    if (!ctx->col_ok)
+        return fail(ctx, SPH_ERR_STATE, "no column density yet (sph_state_set_column, then sph_prime or sph_step)");
+    HIPCK(hipSetDevice(ctx->device));
+    if (grad_out) CK(get_scalar<double>(ctx, ctx->colg, grad_out));
+    if (log_nh_out) CK(get_scalar<double>(ctx, ctx->coln, log_nh_out));
     return SPH_OK;
 }
 

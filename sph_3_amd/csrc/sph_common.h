@@ -17,6 +17,9 @@
 
 #define SPH_WAVE 64
 #define SPH_PI 3.14159265358979323846
+// hydrogen atom mass [g] of the column-density formula (the value of
+// Particles_in_BH_Tree_III/CLOUDY_reloaded_Here/run_simulation/hfvGpuSphBHtree_gradRho_v3t.cu:42)
+#define SPH_M_H 1.673534e-24
 
 // Packed per-node record read by the walks with wave-uniform (scalar) loads.
 struct __align__(16) WNode {

@@ -67,4 +67,16 @@ hipError_t accsph(hipStream_t st, int64_t n, const R4<R> *s4, bool balsara, doub
 template <typename R>
 hipError_t dudt(hipStream_t st, int64_t n, const R4<R> *s4, bool balsara, double alpha, const NgbRows &rows, R *out);
 
+// Column density from the density gradient (the reference's grad_rho_norm_ngb and the
+// host loop after it): g_i = |sum_j m_j nW(r_ij, hbar_ij) (x_j - x_i)| over the rows
+// (pair arithmetic in R, the kernel gradient's q + 1e-10 as in that kernel), then in FP64
+// N_H = 0.5 (h_i / cbrt(n_ngb) + rho_i / g_i) unit_length_cm * rho_i unit_density_cgs XH / m_H,
+// log_nh[i] = log10 N_H (cm^-2) and grad[i] = g_i (code units; may be null).
+struct ColumnParams {
+    double unit_density_cgs, unit_length_cm, XH, n_ngb;
+};
+template <typename R>
+hipError_t column_density(hipStream_t st, int64_t n, const R *rho, const NgbRows &rows, const ColumnParams &cp,
+                          double *grad, double *log_nh);
+
 }  // namespace sph

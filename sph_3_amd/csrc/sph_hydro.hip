@@ -288,6 +288,74 @@ __global__ void __launch_bounds__(256) k_dudt(int64_t n, const R4<R> *__restrict
     if (valid && l == 0) dudt[i] = r;
 }
 
+// Kernel-gradient factor of the column-density pass (grad_rho_norm_ngb,
+// Particles_in_BH_Tree_III/CLOUDY_reloaded_Here/run_simulation/hfvCLibs_v7_hfv_v8_gradRho_v4t.h:748-770):
+// the pieces of spline_nW with q + 1e-10 in the outer piece's denominator.
+template <typename R>
+__device__ __forceinline__ R grad_nW(R rr, R hij)
+{
+    const R sig = (R)(1.0 / SPH_PI);
+    R q = rr / hij, h5 = hij * hij * hij * hij * hij, nW = 0;
+    if (q <= (R)1) nW = sig / h5 * ((R)-3 + (R)2.25 * q);
+    else if (q <= (R)2) nW = (R)-3 * sig / ((R)4 * h5) * ((R)2 - q) * ((R)2 - q) / (q + (R)1e-10);
+    return nW;
+}
+template <>
+__device__ __forceinline__ float grad_nW<float>(float rr, float hij)
+{
+    const float sig = (float)(1.0 / SPH_PI);
+    const float ih = __builtin_amdgcn_rcpf(hij), q = rr * ih, ih2 = ih * ih;
+    const float n1 = -3.0f + 2.25f * q;
+    const float t = fmaxf(2.0f - q, 0.0f);
+    const float n2 = -0.75f * t * t * __builtin_amdgcn_rcpf(q + 1e-10f);
+    return sig * (ih2 * ih2 * ih) * (q <= 1.0f ? n1 : n2);
+}
+
+// g_i = |sum_j m_j grad W_ij| over row i (16 lanes per particle, Kahan sums, TwoSum lane
+// reduction as the other passes; the same file's :716-786), then N_H in FP64 by the
+// particle's first lane (the host loop of
+// Particles_in_BH_Tree_III/CLOUDY_reloaded_Here/run_simulation/hfvGpuSphBHtree_gradRho_v3t.cu:1214-1224).
+template <typename R, int G, bool PER>
+__global__ void __launch_bounds__(256) k_column(int64_t n, const R *__restrict__ rho, NgbRows rows,
+                                                sph::ColumnParams cp, double *__restrict__ grad,
+                                                double *__restrict__ log_nh)
+{
+    int64_t gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    int64_t i = gid / G;
+    int l = (int)(gid % G);
+    bool valid = i < n;
+    Kahan<R> sx, sy, sz;
+    double hi = 0.0;
+    if (valid) {
+        const double4 pi4 = rows.p4[i];
+        const double xi = pi4.x, yi = pi4.y, zi = pi4.z;
+        hi = pi4.w;
+        int64_t k0, e;
+        row_range(rows, i, k0, e);
+        for (int64_t k = k0 + l; k < e; k += G) {
+            const int j = rows.idx[k];
+            const double4 pj = rows.p4[j];
+            R dx = (R)wrap_d<PER>(pj.x - xi, rows.L), dy = (R)wrap_d<PER>(pj.y - yi, rows.L),
+              dz = (R)wrap_d<PER>(pj.z - zi, rows.L);
+            R rr = sph_sqrt<R>(dx * dx + dy * dy + dz * dz);
+            R mnW = (R)rows.v4[j].w * grad_nW<R>(rr, (R)(0.5 * (hi + pj.w)));
+            sx.add(mnW * dx);
+            sy.add(mnW * dy);
+            sz.add(mnW * dz);
+        }
+    }
+    R gx = group_sum<R, G>(sx.s, -sx.c);
+    R gy = group_sum<R, G>(sy.s, -sy.c);
+    R gz = group_sum<R, G>(sz.s, -sz.c);
+    if (valid && l == 0) {
+        const double g = sqrt((double)gx * gx + (double)gy * gy + (double)gz * gz), ri = (double)rho[i];
+        const double nh = ri * cp.unit_density_cgs * cp.XH / SPH_M_H;
+        const double col = 0.5 * (hi / cbrt(cp.n_ngb) + ri / g) * cp.unit_length_cm * nh;
+        if (grad) grad[i] = g;
+        log_nh[i] = log10(col);
+    }
+}
+
 namespace sph {
 
 __global__ void __launch_bounds__(256) k_pack4(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
@@ -367,6 +435,14 @@ hipError_t dudt(hipStream_t st, int64_t n, const R4<R> *s4, bool balsara, double
     LAUNCH(k_dudt, n, s4, balsara, alpha, rows, out);
     return hipGetLastError();
 }
+template <typename R>
+hipError_t column_density(hipStream_t st, int64_t n, const R *rho, const NgbRows &rows, const ColumnParams &cp,
+                          double *grad, double *log_nh)
+{
+    if (n <= 0) return hipSuccess;
+    LAUNCH(k_column, n, rho, rows, cp, grad, log_nh);
+    return hipGetLastError();
+}
 #undef LAUNCH
 
 #define INST(R)                                                                                                    \
@@ -378,7 +454,9 @@ hipError_t dudt(hipStream_t st, int64_t n, const R4<R> *s4, bool balsara, double
                                            R *, R *, R *, R *, R *, R *, R4<R> *);                                 \
     template hipError_t accsph<R>(hipStream_t, int64_t, const R4<R> *, bool, double, const NgbRows &,             \
                                   const double *, const double *, const double *, double *, double *, double *);  \
-    template hipError_t dudt<R>(hipStream_t, int64_t, const R4<R> *, bool, double, const NgbRows &, R *);
+    template hipError_t dudt<R>(hipStream_t, int64_t, const R4<R> *, bool, double, const NgbRows &, R *);         \
+    template hipError_t column_density<R>(hipStream_t, int64_t, const R *, const NgbRows &, const ColumnParams &,  \
+                                          double *, double *);
 INST(float)
 INST(double)
 #undef INST

@@ -79,6 +79,24 @@ class DeviceSPH:
         cp = params.c()
         c.check(c.lib.sph_state_set_cooling(c.h, ptr(Z), ptr(J), ctypes.byref(cp)), "sph_state_set_cooling")
 
+    def set_column(self, params):
+        """Column density (``column.ColumnParams``) computed by every later force
+        evaluation after its density pass; ``params=None`` switches it off."""
+        c = self.ctx
+        if params is None:
+            c.check(c.lib.sph_state_set_column(c.h, None), "sph_state_set_column")
+            return
+        cp = params.c()
+        c.check(c.lib.sph_state_set_column(c.h, ctypes.byref(cp)), "sph_state_set_column")
+
+    def column(self):
+        """(log10 N_H [cm^-2], |grad rho|) of the last force evaluation, caller order."""
+        n = self.n
+        lognh = np.empty(n); g = np.empty(n)
+        c = self.ctx
+        c.check(c.lib.sph_state_get_column(c.h, ptr(g), ptr(lognh)), "sph_state_get_column")
+        return lognh, g
+
     def prime(self):
         """Forces at the initial positions and the first u update."""
         p = self.cfg.params()

@@ -50,3 +50,4 @@ def test_step_params_layout_matches_header():
     assert ctypes.sizeof(L.Stats) == 7 * 8 + 16 * 8 + 16 * 8 + 6 * 8
     # 6 doubles + 2 ints
     assert ctypes.sizeof(L.CoolParams) == 6 * 8 + 2 * 4
+    assert ctypes.sizeof(L.ColumnParams) == 4 * 8
