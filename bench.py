@@ -389,6 +389,31 @@ def sedov_section(args, cfg, run):
     return res
 
 
+def column_section(args, cfg, run, ms_step, kbar):
+    """Column density N_H from |grad rho| (the CLOUDY driver's grad_rho_norm_ngb and N_H
+    loop) computed in every force evaluation of the 1M Evrard step after its density pass,
+    units 1e10 M_sun and 10 kpc: the pass's HIP-event time, its algorithmic bandwidth and
+    the step time with it against the headline step."""
+    from sph_3_amd import column as C
+
+    units = C.ColumnParams.from_mass_length(1.989e43, 3.0857e22)
+    sim, dt = run("mixed", args.steps, args.warmup, cooling=lambda s: s.set_column(units))
+    st = sim.stats()
+    n = st["n"]
+    ms_col = st["phase_ms"]["column"] / max(1, st["phase_calls"]["column"])
+    lognh, _ = sim.column()
+    del sim
+    bpp = 4.0 * kbar + 8.0 + 24 + 8 + 8 + 4 + 16  # row, x, h, m, rho in; |grad rho|, log N_H out
+    return {
+        "workload": f"evrard_collapse + column density, n = {n}", "n": n,
+        "ms_column_pass": ms_col,
+        "pass_bandwidth": {"GB/s": bpp * n / (ms_col * 1e-3) / 1e9,
+                           "frac": bpp * n / (ms_col * 1e-3) / 1e9 / HBM_PEAK_GBS, "bytes_per_particle": bpp},
+        "ms_per_step_with_column": 1e3 * dt / args.steps, "ms_per_step_without": ms_step,
+        "log_NH_range_cm2": [float(np.min(lognh)), float(np.median(lognh)), float(np.max(lognh))],
+    }
+
+
 def copy_probe(nbytes):
     """A plain device copy of known size (the calibration run of the HBM traffic counters)."""
     import torch
@@ -417,6 +442,7 @@ def main():
     ap.add_argument("--cooling-n", type=int, default=4194304)
     ap.add_argument("--no-agn", action="store_true")
     ap.add_argument("--no-sedov", action="store_true")
+    ap.add_argument("--no-column", action="store_true", help="skip the column-density measurement")
     ap.add_argument("--sedov-side", type=int, default=128)
     ap.add_argument("--agn-cloud", type=int, default=7340032)
     ap.add_argument("--agn-wind", type=int, default=1048576)
@@ -567,6 +593,9 @@ def main():
         }
         log("[bench] the cpu-runnable case, n = 10000")
         out["evrard_10k"] = c1_section(cfg, run)
+    if not args.no_column:
+        log("[bench] column density in the step")
+        out["column_density"] = column_section(args, cfg, run, ms_step, kbar)
     if not args.no_cooling_step:
         out["evrard_cooling"] = cooling_section(args, cfg, run)
     if not args.no_agn:
