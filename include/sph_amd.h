@@ -243,6 +243,39 @@ int sph_column_density(sph_ctx *ctx, int64_t n, const double *pos, const double 
                        const double *rho, const sph_column_params *cp, int precision, double *grad_out,
                        double *log_nh_out);
 
+/* ---- AGN source terms of the outflow driver ------------------------------
+ * Isothermal halo (galaxy_isothermal_potential, Particles_in_BH_Tree_III/Evrard_Collapse_2024/
+ * hfvCLibs_v7_hfv_v8.1.h:1411-1428): every force evaluation adds a_i += -G M(r) x_i / r^3 with
+ * M(r) = 2 sigma^2 r / G, i.e. -2 sigma^2 x_i / r^2, to the total acceleration.
+ * Outflow injection (outflow_injector2, same file :1286-1406; after each step,
+ * Particles_in_BH_Tree_III/Main_Code/hfvGpuSphBHtree_newCool_v8.cu:1636): the mass M_dot dt plus
+ * the leftover buys floor(mass / (2 m_inject)) pairs, the rest carries over; pair j sits at +-r with
+ * |r| = r_inject U0 and direction (theta = 2 pi U1, phi = acos(2 U2 - 1)), moves at +-v_in along r,
+ * has u = u_inject and the h of the last injected particle (before the first injection: of caller
+ * index h_ref_index).  U_k come from a counter-based generator of (seed, step, j, k) instead of
+ * cuRAND (see inject_uniform in sph_agn.h).  Injected particles get the next caller indices. */
+typedef struct sph_agn_params {
+    double sigma;        /* halo velocity dispersion (code units); 0: no halo */
+    double M_dot;        /* injected mass per unit time; 0: no injection */
+    double v_in;         /* injection speed */
+    double m_inject;     /* mass of an injected particle (m_sph_high_res) */
+    double u_inject;     /* u of an injected particle (u_for_10K_Temp) */
+    double r_inject;     /* injection radius (0.001, one pc, in the driver's units) */
+    double Z_inject;     /* metallicity [Z_sun] and J21 of injected particles under cooling */
+    double J21_inject;
+    uint64_t seed;
+    int64_t h_ref_index; /* caller index whose h the first pairs take (ndx_BH - 1 there) */
+} sph_agn_params;
+/* Room for n_max particles in the resident state, placed by the next sph_state_set (call it
+ * first when particles will be injected; an injection beyond the room fails with
+ * SPH_ERR_CAPACITY). */
+int sph_state_reserve(sph_ctx *ctx, int64_t n_max);
+/* Source terms of every later force evaluation (halo) and step (injection); p = NULL: off.
+ * Call after sph_state_set (which switches them off). */
+int sph_state_set_agn(sph_ctx *ctx, const sph_agn_params *p);
+/* Current particle count of the resident state and mass waiting for the next pair. */
+int sph_state_count(sph_ctx *ctx, int64_t *n, double *leftover_mass);
+
 /* ---- device-resident time loop (sph_serial_v2.py:148-218 on the GPU) ---- */
 typedef struct sph_step_params {
     double dt;          /* fixed step */

@@ -315,6 +315,28 @@ def column_density(rho, g, h, unit_density_cgs, unit_length_cm, XH=0.7, n_ngb=64
     return out
 
 
+def inject_uniform(seed, step, pair, k):
+    """The injection's counter-based uniform draw in [0, 1)."""
+    f = lib().or_inject_uniform
+    f.restype = ctypes.c_double
+    return f(ctypes.c_uint64(seed), ctypes.c_uint64(step), ctypes.c_uint64(pair), ctypes.c_int(k))
+
+
+def inject_pairs(pairs, seed, step, r_inject, v_in):
+    """(pos, vel) of outflow_injector2's pairs with the counter-based draws."""
+    pos = np.zeros((2 * pairs, 3)); vel = np.zeros((2 * pairs, 3))
+    lib().or_inject_pairs(_i64(pairs), ctypes.c_uint64(seed), ctypes.c_uint64(step), ctypes.c_double(r_inject),
+                          ctypes.c_double(v_in), _p(pos), _p(vel))
+    return pos, vel
+
+
+def halo_accel(pos, sigma, G=1.0):
+    """galaxy_isothermal_potential's acceleration (FP64)."""
+    pos = _f64(pos, (-1, 3)); acc = np.zeros_like(pos)
+    lib().or_halo_accel(_i64(len(pos)), _p(pos), ctypes.c_double(sigma), ctypes.c_double(G), _p(acc))
+    return acc
+
+
 def divcurl_csr(pos, v, rho, m, h, off, idx):
     pos = _f64(pos, (-1, 3)); N = len(pos)
     dv = np.zeros(N); cv = np.zeros(N)

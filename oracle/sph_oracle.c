@@ -1047,3 +1047,56 @@ void or_dudt_csr(int64_t N, const double *pos, const double *v, const double *rh
         dudt[i] = s;
     }
 }
+
+/* ---------------------------------------------------------------- AGN source terms */
+/* Counter-based uniform draw in [0, 1) of (seed, step, pair, component): SplitMix64
+ * finalisers (Steele, Lea & Flood 2014), top 53 bits scaled by 2^-53 -- the generator the
+ * device's injection uses in place of cuRAND. */
+static uint64_t or_mix64(uint64_t v)
+{
+    v += 0x9E3779B97F4A7C15ull;
+    v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ull;
+    v = (v ^ (v >> 27)) * 0x94D049BB133111EBull;
+    return v ^ (v >> 31);
+}
+
+double or_inject_uniform(uint64_t seed, uint64_t step, uint64_t pair, int k)
+{
+    uint64_t v = or_mix64(seed ^ or_mix64(step ^ or_mix64(pair * 4 + (uint64_t)k)));
+    return (double)(v >> 11) * (1.0 / 9007199254740992.0);
+}
+
+/* Positions and velocities of `pairs` outflow pairs (2 * pairs rows of 3): outflow_injector2,
+ * Particles_in_BH_Tree_III/Evrard_Collapse_2024/hfvCLibs_v7_hfv_v8.1.h:1336-1395 -- rt = U0 r_inject,
+ * theta = 2 pi U1, phi = acos(2 U2 - 1), the pair at +-(rt sin phi cos theta, rt sin phi sin theta,
+ * rt cos phi) moving at +-v_in along that direction. */
+void or_inject_pairs(int64_t pairs, uint64_t seed, uint64_t step, double r_inject, double v_in, double *pos,
+                     double *vel)
+{
+    for (int64_t j = 0; j < pairs; j++) {
+        double rt = or_inject_uniform(seed, step, (uint64_t)j, 0) * r_inject;
+        double theta = or_inject_uniform(seed, step, (uint64_t)j, 1) * (2.0 * M_PI);
+        double phi = acos(2.0 * or_inject_uniform(seed, step, (uint64_t)j, 2) - 1.0);
+        double e[3] = {sin(phi) * cos(theta), sin(phi) * sin(theta), cos(phi)};
+        for (int k = 0; k < 2; k++) {
+            double sg = k ? -1.0 : 1.0;
+            for (int d = 0; d < 3; d++) {
+                pos[3 * (2 * j + k) + d] = sg * rt * e[d];
+                vel[3 * (2 * j + k) + d] = sg * v_in * e[d];
+            }
+        }
+    }
+}
+
+/* Isothermal halo acceleration: galaxy_isothermal_potential, the same file :1411-1428 --
+ * M(r) = 2 sigma^2 r / G, a = -G M x / r^3 (added per particle; r = 0 adds nothing). */
+void or_halo_accel(int64_t N, const double *pos, double sigma, double G, double *acc)
+{
+    for (int64_t i = 0; i < N; i++) {
+        const double *x = pos + 3 * i;
+        double rr = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+        if (!(rr > 0.0)) continue;
+        double enclosed = 2.0 * sigma * sigma * rr / G;
+        for (int d = 0; d < 3; d++) acc[3 * i + d] += -G * enclosed * x[d] / rr / rr / rr;
+    }
+}

@@ -59,6 +59,13 @@ class ColumnParams(ctypes.Structure):
                 ("XH", ctypes.c_double), ("n_ngb", ctypes.c_double)]
 
 
+class AgnParams(ctypes.Structure):
+    _fields_ = [("sigma", ctypes.c_double), ("M_dot", ctypes.c_double), ("v_in", ctypes.c_double),
+                ("m_inject", ctypes.c_double), ("u_inject", ctypes.c_double), ("r_inject", ctypes.c_double),
+                ("Z_inject", ctypes.c_double), ("J21_inject", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("h_ref_index", ctypes.c_int64)]
+
+
 COOL_MODE = {"exact": 0, "table": 1, "mlp": 2}
 MLP_REF = {"exact": 0, "table": 1}
 
@@ -109,6 +116,9 @@ _SIGS = {
     "sph_column_density": ([_vp, _i64, _d, _d, _d, _d, ctypes.POINTER(ColumnParams), ctypes.c_int, _d, _d],
                            ctypes.c_int),
     "sph_state_set_column": ([_vp, ctypes.POINTER(ColumnParams)], ctypes.c_int),
+    "sph_state_reserve": ([_vp, _i64], ctypes.c_int),
+    "sph_state_set_agn": ([_vp, ctypes.POINTER(AgnParams)], ctypes.c_int),
+    "sph_state_count": ([_vp, _i64p, _d], ctypes.c_int),
     "sph_state_get_column": ([_vp, _d, _d], ctypes.c_int),
     "sph_get_stats": ([_vp, ctypes.POINTER(Stats)], ctypes.c_int),
     "sph_reset_stats": ([_vp], ctypes.c_int),

@@ -12,6 +12,7 @@
 
 #include "../../include/sph_amd.h"
 #include "sph_common.h"
+#include "sph_agn.h"
 #include "sph_cool.h"
 #include "sph_grav.h"
 #include "sph_hydro.h"
@@ -75,6 +76,13 @@ struct sph_ctx {
     Buf x, y, z, vx, vy, vz, m, h, eps, u, duprev, ax, ay, az, gx, gy, gz, tmpd;
     Buf shadow[SPH_GATHER_MAX];  // targets of the reorder gathers (swapped with the state)
     double m_max = INFINITY;     // largest particle mass of the current masses
+    int64_t n_reserve = 0;       // per-particle buffers hold at least this many (injection room)
+    int64_t n_placed = 0;        // room of the resident state's buffers (the reserve when it was set)
+    bool agn_on = false;
+    sph_agn_params agn{};
+    double agn_leftover = 0.0;   // injection mass carried to the next step
+    int64_t agn_injected = 0;    // particles injected since the state was set
+    uint64_t agn_step = 0;       // steps since the source terms were set (the generator's step)
     Buf orig, tmpi;
     // ordering
     Buf keys0, keys1, vals0, vals1, hist, bpart, box;
@@ -181,6 +189,7 @@ template <typename T> T *P_(Buf &b) { return (T *)b.p; }
 
 int capacity(sph_ctx *ctx, int64_t n)
 {
+    n = std::max(n, ctx->n_reserve);  // room for injected particles: no state buffer ever moves
     size_t nd = (size_t)n * 8, ni = (size_t)n * 4;
     Buf *dbl[] = {&ctx->x, &ctx->y, &ctx->z, &ctx->vx, &ctx->vy, &ctx->vz, &ctx->m, &ctx->h, &ctx->eps,
                   &ctx->u, &ctx->duprev, &ctx->ax, &ctx->ay, &ctx->az, &ctx->gx, &ctx->gy, &ctx->gz,
@@ -914,8 +923,12 @@ extern "C" int sph_state_set(sph_ctx *ctx, int64_t n, const double *pos, const d
     ctx->resident = false;
     ctx->cool_on = false;  // Z and J21 belong to the previous state
     ctx->col_on = ctx->col_ok = false;
+    ctx->agn_on = false;
+    ctx->agn_leftover = 0.0;
+    ctx->agn_injected = 0;
     invalidate_all(ctx);
     CK(set_pos(ctx, n, pos));
+    ctx->n_placed = std::max(n, ctx->n_reserve);  // set_pos placed every per-particle buffer for this many
     CK(put_vec3(ctx, vel, ctx->vx, ctx->vy, ctx->vz));
     CK(put_scalar(ctx, m, ctx->m, nullptr, nullptr, nullptr));
     CK(put_scalar(ctx, u, ctx->u, nullptr, nullptr, nullptr));
@@ -972,7 +985,7 @@ int reorder_state(sph_ctx *ctx)
     const double *src[SPH_GATHER_MAX];
     double *dst[SPH_GATHER_MAX];
     for (int j = 0; j < k; j++) {
-        CK(ensure(ctx, ctx->shadow[j], sizeof(double) * n));
+        CK(ensure(ctx, ctx->shadow[j], sizeof(double) * std::max(n, ctx->n_reserve)));
         src[j] = (const double *)arr[j]->p;
         dst[j] = (double *)ctx->shadow[j].p;
     }
@@ -1128,6 +1141,8 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
             CK(hydro_forces());
         }
     }
+    if (ctx->agn_on && ctx->agn.sigma != 0.0)  // isothermal halo, added to the total acceleration
+        HIPCK(sph::halo_accel(st, n, D(x), D(y), D(z), ctx->agn.sigma, D(ax), D(ay), D(az)));
     if (!first) {
         ph_begin(ctx, SPH_PH_KICK);
         HIPCK(sph::kick_drift(st, n, D(vx), D(vy), D(vz), D(ax), D(ay), D(az), 0.5 * p->dt, 0.0, nullptr, nullptr,
@@ -1165,6 +1180,39 @@ int check_params(sph_ctx *ctx, const sph_step_params *p)
     if (p->period > 0.0 && p->G != 0.0)
         return fail(ctx, SPH_ERR_ARG, "a periodic box needs G = 0 (periodic gravity is not provided)");
     HIPCK(hipSetDevice(ctx->device));
+    return SPH_OK;
+}
+
+// Outflow pairs after a step (the driver's outflow_injector2 after t += dt): the host spends
+// M_dot dt plus the leftover on whole pairs, the device appends them at the end of the
+// sorted arrays (the next step's ordering sorts them in).
+int inject_outflow(sph_ctx *ctx, double dt)
+{
+    const sph_agn_params &a = ctx->agn;
+    ctx->agn_step += 1;
+    if (!(a.M_dot > 0.0)) return SPH_OK;
+    double mass = a.M_dot * dt + ctx->agn_leftover;
+    const int64_t pairs = (int64_t)std::floor(mass / (2.0 * a.m_inject));
+    ctx->agn_leftover = mass - (double)pairs * 2.0 * a.m_inject;
+    if (pairs <= 0) return SPH_OK;
+    const int64_t n = ctx->n;
+    if (n + 2 * pairs > ctx->n_placed || n + 2 * pairs > (int64_t)INT32_MAX)
+        return fail(ctx, SPH_ERR_CAPACITY, "outflow injection needs " + std::to_string(n + 2 * pairs) +
+                                               " particles; reserve more with sph_state_reserve");
+    hipStream_t st = ctx->st;
+    // h of the last injected particle (caller index n - 1), before the first injection of h_ref_index
+    const int64_t src = ctx->agn_injected > 0 ? n - 1 : a.h_ref_index;
+    double *hsrc = D(kout) + 6;  // slot 6 (kinetics uses 0..4, the h ratio 7)
+    HIPCK(sph::h_of_caller(st, n, I32(orig), D(h), src, hsrc));
+    sph::InjectParams ip{a.r_inject, a.v_in, a.m_inject, a.u_inject, a.Z_inject, a.J21_inject, a.seed, ctx->agn_step};
+    sph::InjectTarget t{D(x), D(y), D(z), D(vx), D(vy), D(vz), D(m), D(h), D(u), D(duprev), D(ax), D(ay), D(az),
+                        I32(orig), ctx->cool_on ? D(zmet) : nullptr, ctx->cool_on ? D(j21) : nullptr};
+    HIPCK(sph::inject_pairs(st, n, pairs, ip, hsrc, t));
+    HIPCK(hipStreamSynchronize(st));
+    ctx->n = n + 2 * pairs;
+    ctx->stats.n = ctx->n;
+    ctx->agn_injected += 2 * pairs;
+    ctx->m_max = std::max(ctx->m_max, std::fabs(a.m_inject));
     return SPH_OK;
 }
 
@@ -1229,6 +1277,7 @@ extern "C" int sph_step(sph_ctx *ctx, const sph_step_params *p)
     CK(finish_step(ctx, p));
     ctx->stats.steps += 1;
     ctx->timing = false;
+    if (ctx->agn_on) CK(inject_outflow(ctx, p->dt));
     return SPH_OK;
 }
 
@@ -1984,6 +2033,44 @@ extern "C" int sph_column_density(sph_ctx *ctx, int64_t n, const double *pos, co
     return get_scalar<double>(ctx, ctx->coln, log_nh_out);
 }
 
+extern "C" int sph_state_reserve(sph_ctx *ctx, int64_t n_max)
+{
+    if (!ctx) return SPH_ERR_ARG;
+    if (n_max < 0 || n_max > (int64_t)INT32_MAX) return fail(ctx, SPH_ERR_ARG, "n_max out of range");
+    ctx->n_reserve = std::max(ctx->n_reserve, n_max);  // placed by the next sph_state_set
+    return SPH_OK;
+}
+
+extern "C" int sph_state_set_agn(sph_ctx *ctx, const sph_agn_params *p)
+{
+    if (!ctx) return SPH_ERR_ARG;
+    if (!ctx->resident) return fail(ctx, SPH_ERR_STATE, "no resident state (call sph_state_set first)");
+    if (!p) {
+        ctx->agn_on = false;
+        return SPH_OK;
+    }
+    if (!(p->sigma >= 0.0) || !(p->M_dot >= 0.0) || !std::isfinite(p->v_in) || !std::isfinite(p->u_inject) ||
+        !(p->r_inject >= 0.0))
+        return fail(ctx, SPH_ERR_ARG, "AGN source terms: sigma, M_dot, r_inject must be >= 0, v_in and u finite");
+    if (p->M_dot > 0.0 && !(p->m_inject > 0.0))
+        return fail(ctx, SPH_ERR_ARG, "AGN injection needs m_inject > 0");
+    if (p->M_dot > 0.0 && (p->h_ref_index < 0 || p->h_ref_index >= ctx->n))
+        return fail(ctx, SPH_ERR_ARG, "h_ref_index is not a particle of the resident state");
+    ctx->agn = *p;
+    ctx->agn_on = true;
+    ctx->agn_step = 0;
+    return SPH_OK;
+}
+
+extern "C" int sph_state_count(sph_ctx *ctx, int64_t *n, double *leftover_mass)
+{
+    if (!ctx) return SPH_ERR_ARG;
+    if (!ctx->resident) return fail(ctx, SPH_ERR_STATE, "no resident state (call sph_state_set first)");
+    if (n) *n = ctx->n;
+    if (leftover_mass) *leftover_mass = ctx->agn_leftover;
+    return SPH_OK;
+}
+
 extern "C" int sph_state_set_column(sph_ctx *ctx, const sph_column_params *cp)
 {
     if (!ctx) return SPH_ERR_ARG;
@@ -2025,8 +2112,8 @@ extern "C" int sph_state_set_cooling(sph_ctx *ctx, const double *Z, const double
     if (!Z || !J21) return fail(ctx, SPH_ERR_ARG, "null argument");
     CK(cool_prep(ctx, 1));
     CK(cool_check(ctx, cp));
-    CK(ensure(ctx, ctx->zmet, (size_t)ctx->n * 8));
-    CK(ensure(ctx, ctx->j21, (size_t)ctx->n * 8));
+    CK(ensure(ctx, ctx->zmet, (size_t)std::max(ctx->n, ctx->n_reserve) * 8));
+    CK(ensure(ctx, ctx->j21, (size_t)std::max(ctx->n, ctx->n_reserve) * 8));
     CK(put_scalar(ctx, Z, ctx->zmet, nullptr, nullptr, nullptr));
     CK(put_scalar(ctx, J21, ctx->j21, nullptr, nullptr, nullptr));
     if (cp->mode != SPH_COOL_EXACT) CK(ensure_ttable(ctx, cp->XH));

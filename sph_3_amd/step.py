@@ -19,6 +19,30 @@ from ._lib import PHASES, PREC, SOFT, Stats, StepParams, context, f64, ptr
 
 
 @dataclass
+class AgnParams:
+    """AGN source terms of the outflow driver (code units): isothermal halo of velocity
+    dispersion ``sigma`` and outflow pairs bought with ``M_dot`` per unit time, of mass
+    ``m_inject``, speed ``v_in``, internal energy ``u_inject``, within ``r_inject`` of the
+    origin; ``h_ref_index`` (caller index) lends its h to the first pairs."""
+    sigma: float = 0.0
+    M_dot: float = 0.0
+    v_in: float = 0.0
+    m_inject: float = 0.0
+    u_inject: float = 0.0
+    r_inject: float = 1e-3
+    Z_inject: float = 0.0
+    J21_inject: float = 1e-7  # photolibs3's fixed UV amplitude
+    seed: int = 0
+    h_ref_index: int = 0
+
+    def c(self):
+        from ._lib import AgnParams as _AP
+
+        return _AP(self.sigma, self.M_dot, self.v_in, self.m_inject, self.u_inject, self.r_inject, self.Z_inject,
+                   self.J21_inject, int(self.seed) & 0xFFFFFFFFFFFFFFFF, int(self.h_ref_index))
+
+
+@dataclass
 class StepConfig:
     dt: float = 1e-3
     gamma: float = 5.0 / 3.0
@@ -42,7 +66,9 @@ class StepConfig:
 class DeviceSPH:
     """Resident particle state on one GPU plus the step driver."""
 
-    def __init__(self, pos, vel, m, u, h, config: StepConfig | None = None, device: int = 0):
+    def __init__(self, pos, vel, m, u, h, config: StepConfig | None = None, device: int = 0,
+                 n_max: int | None = None):
+        """``n_max``: room for particles injected later (``set_agn``)."""
         self.cfg = config or StepConfig()
         self.ctx = context(device)
         pos = f64(pos, (-1, 3))
@@ -52,9 +78,27 @@ class DeviceSPH:
         m = np.broadcast_to(f64(m), (n,)).copy()
         h = np.broadcast_to(f64(h), (n,)).copy()
         c = self.ctx
+        if n_max is not None:
+            c.check(c.lib.sph_state_reserve(c.h, int(n_max)), "sph_state_reserve")
         c.check(c.lib.sph_state_set(c.h, n, ptr(pos), ptr(f64(vel, (-1, 3))), ptr(m), ptr(u), ptr(h)),
                 "sph_state_set")
         self.primed = False
+
+    def set_agn(self, params):
+        """AGN source terms (``AgnParams``): the isothermal halo in every force evaluation
+        and the outflow pairs injected after every step; ``None`` switches them off."""
+        c = self.ctx
+        if params is None:
+            c.check(c.lib.sph_state_set_agn(c.h, None), "sph_state_set_agn")
+            return
+        ap = params.c()
+        c.check(c.lib.sph_state_set_agn(c.h, ctypes.byref(ap)), "sph_state_set_agn")
+
+    def leftover_mass(self) -> float:
+        lo = np.zeros(1)
+        c = self.ctx
+        c.check(c.lib.sph_state_count(c.h, None, ptr(lo)), "sph_state_count")
+        return float(lo[0])
 
     def set_cooling(self, Z, J21, params, surrogate=None, table_ranges=None):
         """Per-particle metallicity Z [Z_sun] and UV amplitude J21 (caller order) and the
@@ -110,6 +154,9 @@ class DeviceSPH:
         p = self.cfg.params(timing)
         c = self.ctx
         c.check(c.lib.sph_step(c.h, ctypes.byref(p)), "sph_step")
+        cnt = np.zeros(1, np.int64)
+        c.check(c.lib.sph_state_count(c.h, ptr(cnt), None), "sph_state_count")
+        self.n = int(cnt[0])  # grows when outflow pairs are injected
 
     def get(self):
         n = self.n

@@ -51,3 +51,4 @@ def test_step_params_layout_matches_header():
     # 6 doubles + 2 ints
     assert ctypes.sizeof(L.CoolParams) == 6 * 8 + 2 * 4
     assert ctypes.sizeof(L.ColumnParams) == 4 * 8
+    assert ctypes.sizeof(L.AgnParams) == 10 * 8
