@@ -305,9 +305,13 @@ def agn_section(args, cfg, run):
     """AGN-outflow-like workload: a lognormal clumpy cloud (sigma_ln rho = 1.5, Gaussian
     field on 256^3 cells) of agn_cloud particles at T = 1e4 K plus agn_wind hot (1e7 K)
     wind particles leaving r = 0.05 R at v = 0.1; gravity, SPH and the gated-surrogate
-    cooling (table fallback at 0.05 dex); dt = 1e-5 (the wind shell's Courant limit)."""
+    cooling (table fallback at 0.05 dex); dt = 1e-5 (the wind shell's Courant limit); the
+    outflow driver's source terms: an isothermal halo (sigma = 2, about 130 km/s in these
+    units) and 50.5 outflow pairs per step on average injected within 1 pc (1e-4) of the
+    origin at v = 30 (about 2,000 km/s), T = 1e4 K."""
     from sph_3_amd import cooling as C
     from sph_3_amd import ic
+    from sph_3_amd.step import AgnParams
 
     net = getattr(mlp_section, "net", None)
     mode = "mlp" if net is not None else "table"
@@ -321,16 +325,24 @@ def agn_section(args, cfg, run):
     dt_saved = cfg.dt
     cfg.dt = 1e-5
 
+    m_inj = float(d["m"][0])
+    agn = AgnParams(sigma=2.0, M_dot=50.5 * 2 * m_inj / 1e-5, v_in=30.0, m_inject=m_inj,
+                    u_inject=ic.u_of_T(1e4, units.unit_u_cgs), r_inject=1e-4, Z_inject=1.0, J21_inject=1.0,
+                    seed=2025, h_ref_index=n - 1)
+    n_max = n + 2 * 51 * (args.steps + args.warmup + 4)
+
     def cooling(mode_):
         def f(sim):
             u = C.CoolingParams(**{**units.__dict__, "mode": mode_})
             sim.set_cooling(Z, J, u, surrogate=net if mode_ == "mlp" else None)
+            sim.set_agn(agn)
         return f
 
     try:
-        sim, dt = run("mixed", args.steps, args.warmup, d=d, cooling=cooling(mode))
+        sim, dt = run("mixed", args.steps, args.warmup, d=d, cooling=cooling(mode), n_max=n_max)
         st = sim.stats()
         ms = 1e3 * dt / args.steps
+        n_end = sim.n
         res = {
             "workload": f"agn-like: {args.agn_cloud} cloud + {args.agn_wind} wind particles", "n": n,
             "cooling_mode": mode, "dt": cfg.dt, "ms_per_step": ms, "value": n * args.steps / dt,
@@ -341,11 +353,13 @@ def agn_section(args, cfg, run):
             "h_solve_rounds_last_step": st["h_rounds"],
             "h_solve_walk_solved_targets_last_step": st["h_big"],
             "cooling_fallback_fraction_last_step": st["cool_fallback"] / n,
+            "source_terms": {"halo_sigma": agn.sigma, "pairs_per_step": "50 or 51 (50.5 on average)",
+                             "particles_injected": n_end - n},
         }
         del sim
         if not args.no_fp64_form:
             steps64 = max(2, args.steps // 2)
-            sim64, dt64 = run("fp64", steps64, 1, d=d, cooling=cooling("table"))
+            sim64, dt64 = run("fp64", steps64, 1, d=d, cooling=cooling("table"), n_max=n_max)
             res["fp64_form"] = {"ms_per_step": 1e3 * dt64 / steps64, "mixed_speedup": (1e3 * dt64 / steps64) / ms,
                                 "cooling_mode": "table"}
             del sim64
@@ -464,9 +478,9 @@ def main():
     cfg = StepConfig(dt=1e-3, gamma=5 / 3, alpha=1.0, G=1.0, theta=0.5, softening="plummer", eps_plummer=1e-3,
                      n_ngb=64.0, h_tol=1e-10, precision="mixed", balsara=True)
 
-    def run(precision, steps, warmup, d=d, cooling=None):
+    def run(precision, steps, warmup, d=d, cooling=None, n_max=None):
         cfg.precision = precision
-        sim = DeviceSPH(d["pos"], d["vel"], d["m"], d["u"], d["h0"], cfg)
+        sim = DeviceSPH(d["pos"], d["vel"], d["m"], d["u"], d["h0"], cfg, n_max=n_max)
         if cooling is not None:
             cooling(sim)
         log(f"[bench] {precision}: prime")
