@@ -273,8 +273,9 @@ int sph_state_reserve(sph_ctx *ctx, int64_t n_max);
 /* Source terms of every later force evaluation (halo) and step (injection); p = NULL: off.
  * Call after sph_state_set (which switches them off). */
 int sph_state_set_agn(sph_ctx *ctx, const sph_agn_params *p);
-/* Current particle count of the resident state and mass waiting for the next pair. */
-int sph_state_count(sph_ctx *ctx, int64_t *n, double *leftover_mass);
+/* Current particle count of the resident state, mass waiting for the next pair, steps since
+ * sph_state_set_agn and particles injected since sph_state_set (any pointer may be NULL). */
+int sph_state_count(sph_ctx *ctx, int64_t *n, double *leftover_mass, uint64_t *agn_step, int64_t *agn_injected);
 
 /* ---- device-resident time loop (sph_serial_v2.py:148-218 on the GPU) ---- */
 typedef struct sph_step_params {
@@ -334,6 +335,14 @@ int sph_state_set_cooling(sph_ctx *ctx, const double *Z, const double *J21, cons
  * one back in caller order (either pointer may be NULL). */
 int sph_state_set_column(sph_ctx *ctx, const sph_column_params *cp);
 int sph_state_get_column(sph_ctx *ctx, double *grad_out, double *log_nh_out);
+/* Restart: sph_state_get_duprev reads the du/dt the next u update averages with (caller order);
+ * sph_state_set_restart, after sph_state_set (and sph_state_set_agn / _cooling when used), loads
+ * the last force evaluation's acceleration and that du/dt (caller order), the injection's
+ * leftover mass, step counter and injected count, and marks the state primed, so sph_step
+ * continues the saved run.  The reference has no resume; its snapshots lack these fields. */
+int sph_state_get_duprev(sph_ctx *ctx, double *duprev_out);
+int sph_state_set_restart(sph_ctx *ctx, const double *acc, const double *duprev, double agn_leftover,
+                          uint64_t agn_step, int64_t agn_injected);
 /* Forces at the current positions and the first u update (sph_serial_v2.py:94-143). */
 int sph_prime(sph_ctx *ctx, const sph_step_params *p);
 /* One KDK step (sph_serial_v2.py:148-203), entirely on the device. */

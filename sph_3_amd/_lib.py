@@ -118,7 +118,9 @@ _SIGS = {
     "sph_state_set_column": ([_vp, ctypes.POINTER(ColumnParams)], ctypes.c_int),
     "sph_state_reserve": ([_vp, _i64], ctypes.c_int),
     "sph_state_set_agn": ([_vp, ctypes.POINTER(AgnParams)], ctypes.c_int),
-    "sph_state_count": ([_vp, _i64p, _d], ctypes.c_int),
+    "sph_state_count": ([_vp, _i64p, _d, ctypes.POINTER(ctypes.c_uint64), _i64p], ctypes.c_int),
+    "sph_state_get_duprev": ([_vp, _d], ctypes.c_int),
+    "sph_state_set_restart": ([_vp, _d, _d, ctypes.c_double, ctypes.c_uint64, _i64], ctypes.c_int),
     "sph_state_get_column": ([_vp, _d, _d], ctypes.c_int),
     "sph_get_stats": ([_vp, ctypes.POINTER(Stats)], ctypes.c_int),
     "sph_reset_stats": ([_vp], ctypes.c_int),

@@ -2033,6 +2033,33 @@ extern "C" int sph_column_density(sph_ctx *ctx, int64_t n, const double *pos, co
     return get_scalar<double>(ctx, ctx->coln, log_nh_out);
 }
 
+extern "C" int sph_state_get_duprev(sph_ctx *ctx, double *duprev_out)
+{
+    if (!ctx || !duprev_out) return SPH_ERR_ARG;
+    if (!ctx->resident) return fail(ctx, SPH_ERR_STATE, "no resident state (call sph_state_set first)");
+    HIPCK(hipSetDevice(ctx->device));
+    return get_scalar<double>(ctx, ctx->duprev, duprev_out);
+}
+
+extern "C" int sph_state_set_restart(sph_ctx *ctx, const double *acc, const double *duprev, double agn_leftover,
+                                     uint64_t agn_step, int64_t agn_injected)
+{
+    if (!ctx) return SPH_ERR_ARG;
+    if (!acc || !duprev) return fail(ctx, SPH_ERR_ARG, "null argument");
+    if (!ctx->resident) return fail(ctx, SPH_ERR_STATE, "no resident state (call sph_state_set first)");
+    if (!(agn_leftover >= 0.0) || agn_injected < 0 || agn_injected > ctx->n)
+        return fail(ctx, SPH_ERR_ARG, "bad injection bookkeeping");
+    HIPCK(hipSetDevice(ctx->device));
+    CK(put_vec3(ctx, acc, ctx->ax, ctx->ay, ctx->az));
+    CK(put_scalar(ctx, duprev, ctx->duprev, nullptr, nullptr, nullptr));
+    HIPCK(hipStreamSynchronize(ctx->st));
+    ctx->agn_leftover = agn_leftover;
+    ctx->agn_step = agn_step;
+    ctx->agn_injected = agn_injected;
+    ctx->primed = true;
+    return SPH_OK;
+}
+
 extern "C" int sph_state_reserve(sph_ctx *ctx, int64_t n_max)
 {
     if (!ctx) return SPH_ERR_ARG;
@@ -2062,12 +2089,15 @@ extern "C" int sph_state_set_agn(sph_ctx *ctx, const sph_agn_params *p)
     return SPH_OK;
 }
 
-extern "C" int sph_state_count(sph_ctx *ctx, int64_t *n, double *leftover_mass)
+extern "C" int sph_state_count(sph_ctx *ctx, int64_t *n, double *leftover_mass, uint64_t *agn_step,
+                               int64_t *agn_injected)
 {
     if (!ctx) return SPH_ERR_ARG;
     if (!ctx->resident) return fail(ctx, SPH_ERR_STATE, "no resident state (call sph_state_set first)");
     if (n) *n = ctx->n;
     if (leftover_mass) *leftover_mass = ctx->agn_leftover;
+    if (agn_step) *agn_step = ctx->agn_step;
+    if (agn_injected) *agn_injected = ctx->agn_injected;
     return SPH_OK;
 }
 

@@ -83,6 +83,8 @@ class DeviceSPH:
         c.check(c.lib.sph_state_set(c.h, n, ptr(pos), ptr(f64(vel, (-1, 3))), ptr(m), ptr(u), ptr(h)),
                 "sph_state_set")
         self.primed = False
+        self._m0 = m
+        self._agn = None
 
     def set_agn(self, params):
         """AGN source terms (``AgnParams``): the isothermal halo in every force evaluation
@@ -93,12 +95,28 @@ class DeviceSPH:
             return
         ap = params.c()
         c.check(c.lib.sph_state_set_agn(c.h, ctypes.byref(ap)), "sph_state_set_agn")
+        self._agn = params
 
     def leftover_mass(self) -> float:
         lo = np.zeros(1)
         c = self.ctx
-        c.check(c.lib.sph_state_count(c.h, None, ptr(lo)), "sph_state_count")
+        c.check(c.lib.sph_state_count(c.h, None, ptr(lo), None, None), "sph_state_count")
         return float(lo[0])
+
+    def counts(self):
+        """(n, leftover mass, steps since set_agn, particles injected since the state was set)."""
+        n = np.zeros(1, np.int64); lo = np.zeros(1); st = np.zeros(1, np.uint64); inj = np.zeros(1, np.int64)
+        c = self.ctx
+        c.check(c.lib.sph_state_count(c.h, ptr(n), ptr(lo), st.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                      ptr(inj)), "sph_state_count")
+        return int(n[0]), float(lo[0]), int(st[0]), int(inj[0])
+
+    def masses(self):
+        """Particle masses in caller order (the state's plus the injected particles')."""
+        extra = self.n - len(self._m0)
+        if extra <= 0:
+            return self._m0.copy()
+        return np.concatenate([self._m0, np.full(extra, self._agn.m_inject if self._agn else 0.0)])
 
     def set_cooling(self, Z, J21, params, surrogate=None, table_ranges=None):
         """Per-particle metallicity Z [Z_sun] and UV amplitude J21 (caller order) and the
@@ -155,7 +173,7 @@ class DeviceSPH:
         c = self.ctx
         c.check(c.lib.sph_step(c.h, ctypes.byref(p)), "sph_step")
         cnt = np.zeros(1, np.int64)
-        c.check(c.lib.sph_state_count(c.h, ptr(cnt), None), "sph_state_count")
+        c.check(c.lib.sph_state_count(c.h, ptr(cnt), None, None, None), "sph_state_count")
         self.n = int(cnt[0])  # grows when outflow pairs are injected
 
     def get(self):
