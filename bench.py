@@ -428,6 +428,28 @@ def column_section(args, cfg, run, ms_step, kbar):
     }
 
 
+def io_section(g, m):
+    """Host file I/O of the 1M state in the reference's snapshot layout (saveArraysToBinary)
+    and IC layout (readVectorsFromFile), through the page cache of the box's /tmp."""
+    import tempfile
+
+    from sph_3_amd import snapshot as S
+
+    d = S.state_arrays(g["pos"], g["vel"], m, g["u"], g["h"], g["rho"])
+    n = len(m)
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, write, read, nbytes in [
+                ("snapshot", lambda f: S.write_snapshot(f, d), S.read_snapshot, 4 + 44 * n),
+                ("ic_header", lambda f: S.write_ic(f, d, layout="header"), lambda f: S.read_ic(f, "header"),
+                 4 + 44 * n)]:
+            fn = os.path.join(tmp, name + ".bin")
+            t0 = time.perf_counter(); write(fn); tw = time.perf_counter() - t0
+            t0 = time.perf_counter(); read(fn); tr = time.perf_counter() - t0
+            out[name] = {"bytes": nbytes, "write_GB/s": nbytes / tw / 1e9, "read_GB/s": nbytes / tr / 1e9}
+    return out
+
+
 def copy_probe(nbytes):
     """A plain device copy of known size (the calibration run of the HBM traffic counters)."""
     import torch
@@ -610,6 +632,7 @@ def main():
         }
         log("[bench] the cpu-runnable case, n = 10000")
         out["evrard_10k"] = c1_section(cfg, run)
+    out["file_io"] = io_section(g_mixed, d["m"])
     if not args.no_column:
         log("[bench] column density in the step")
         out["column_density"] = column_section(args, cfg, run, ms_step, kbar)
