@@ -405,17 +405,24 @@ def sedov_section(args, cfg, run):
 
 def column_section(args, cfg, run, ms_step, kbar):
     """Column density N_H from |grad rho| (the CLOUDY driver's grad_rho_norm_ngb and N_H
-    loop) computed in every force evaluation of the 1M Evrard step after its density pass,
-    units 1e10 M_sun and 10 kpc: the pass's HIP-event time, its algorithmic bandwidth and
-    the step time with it against the headline step."""
+    loop) and the per-particle time-step criteria (dt_array_indiv_dt, C_CFL = 0.3) computed
+    in every force evaluation of the 1M Evrard step, units 1e10 M_sun and 10 kpc: each pass's
+    HIP-event time, the column pass's algorithmic bandwidth and the step time with both
+    against the headline step."""
     from sph_3_amd import column as C
 
     units = C.ColumnParams.from_mass_length(1.989e43, 3.0857e22)
-    sim, dt = run("mixed", args.steps, args.warmup, cooling=lambda s: s.set_column(units))
+    cfg.C_CFL = 0.3
+    try:
+        sim, dt = run("mixed", args.steps, args.warmup, cooling=lambda s: s.set_column(units))
+    finally:
+        cfg.C_CFL = 0.0
     st = sim.stats()
     n = st["n"]
     ms_col = st["phase_ms"]["column"] / max(1, st["phase_calls"]["column"])
+    ms_ts = st["phase_ms"]["timestep"] / max(1, st["phase_calls"]["timestep"])
     lognh, _ = sim.column()
+    dt_min = st["dt_min"]
     del sim
     bpp = 4.0 * kbar + 8.0 + 24 + 8 + 8 + 4 + 16  # row, x, h, m, rho in; |grad rho|, log N_H out
     return {
@@ -423,7 +430,8 @@ def column_section(args, cfg, run, ms_step, kbar):
         "ms_column_pass": ms_col,
         "pass_bandwidth": {"GB/s": bpp * n / (ms_col * 1e-3) / 1e9,
                            "frac": bpp * n / (ms_col * 1e-3) / 1e9 / HBM_PEAK_GBS, "bytes_per_particle": bpp},
-        "ms_per_step_with_column": 1e3 * dt / args.steps, "ms_per_step_without": ms_step,
+        "ms_timestep_pass": ms_ts, "dt_min_C_CFL_0.3": dt_min,
+        "ms_per_step_with_both": 1e3 * dt / args.steps, "ms_per_step_without": ms_step,
         "log_NH_range_cm2": [float(np.min(lognh)), float(np.median(lognh)), float(np.max(lognh))],
     }
 
@@ -634,7 +642,7 @@ def main():
         out["evrard_10k"] = c1_section(cfg, run)
     out["file_io"] = io_section(g_mixed, d["m"])
     if not args.no_column:
-        log("[bench] column density in the step")
+        log("[bench] column density and time-step criteria in the step")
         out["column_density"] = column_section(args, cfg, run, ms_step, kbar)
     if not args.no_cooling_step:
         out["evrard_cooling"] = cooling_section(args, cfg, run)

@@ -289,6 +289,10 @@ typedef struct sph_step_params {
     double h_tol;       /* Newton relative tolerance */
     double period;      /* > 0: side of a periodic box [0, period)^3 with minimum-image distances
                            (positions wrapped by the drift; needs G = 0); 0: open boundaries */
+    double C_CFL;       /* > 0: after every force evaluation the per-particle time-step criteria of
+                           dt_array_indiv_dt (EV/lib.h:1188-1268) with this Courant factor, the
+                           signal velocity's maximum over S_i; the smallest is sph_stats.dt_min and
+                           sph_state_get_timestep reads them back.  0: off */
     int soft_kind;      /* SPH_SOFT_SPLINE uses eps_i = h_i (EV/lib.h:584) */
     int precision;      /* SPH_PREC_MIXED / SPH_PREC_FP64 */
     int balsara;        /* 1: shear.py forms, 0: libsx.py forms */
@@ -301,8 +305,8 @@ typedef struct sph_stats {
     int64_t grav_pp, grav_pn; /* interactions of the last gravity pass */
     int64_t h_unconverged;   /* cumulative */
     int64_t h_rounds;        /* rounds of the last h solve */
-    double phase_ms[16];     /* accumulated HIP-event milliseconds per phase */
-    int64_t phase_calls[16];
+    double phase_ms[20];     /* accumulated HIP-event milliseconds per phase (sph_phase) */
+    int64_t phase_calls[20];
     int64_t h_tests;         /* last h solve: FP64 distance tests of its walks */
     int64_t h_terms;         /* last h solve: kernel terms summed by its Newton iterations */
     int64_t ngb_extra;       /* last step: neighbour pairs found only from the other particle's candidates */
@@ -310,6 +314,7 @@ typedef struct sph_stats {
     int64_t cool_fallback;   /* last step: table fallbacks of the surrogate (SPH_COOL_MLP) */
     int64_t h_big;           /* last h solve: targets whose solution needs more candidates than a
                                 row holds (solved by a candidate-free tree-walk Newton) */
+    double dt_min;           /* last force evaluation with C_CFL > 0: min_i dt_i (else 0) */
 } sph_stats;
 
 /* phase indices of sph_stats.phase_ms (SPH_PH_HMOMENTS: the node boxes built for the h walk;
@@ -317,7 +322,7 @@ typedef struct sph_stats {
 enum sph_phase {
     SPH_PH_KICKDRIFT = 0, SPH_PH_ORDER, SPH_PH_TOPOLOGY, SPH_PH_HSOLVE, SPH_PH_MOMENTS, SPH_PH_NGB,
     SPH_PH_DENSITY, SPH_PH_GRAVITY, SPH_PH_DIVCURL, SPH_PH_ACCSPH, SPH_PH_KICK, SPH_PH_DUDT,
-    SPH_PH_UUPDATE, SPH_PH_HMOMENTS, SPH_PH_COOLING, SPH_PH_COLUMN, SPH_PH_COUNT
+    SPH_PH_UUPDATE, SPH_PH_HMOMENTS, SPH_PH_COOLING, SPH_PH_COLUMN, SPH_PH_TIMESTEP, SPH_PH_COUNT
 };
 
 /* Load the resident state (caller order); u and h are per particle. */
@@ -335,6 +340,9 @@ int sph_state_set_cooling(sph_ctx *ctx, const double *Z, const double *J21, cons
  * one back in caller order (either pointer may be NULL). */
 int sph_state_set_column(sph_ctx *ctx, const sph_column_params *cp);
 int sph_state_get_column(sph_ctx *ctx, double *grad_out, double *log_nh_out);
+/* Per-particle dt_i and dh/dt_i = h div v / 3 of the last force evaluation with C_CFL > 0
+ * (caller order; particles injected after it read NaN); either pointer may be NULL. */
+int sph_state_get_timestep(sph_ctx *ctx, double *dt_out, double *dhdt_out);
 /* Restart: sph_state_get_duprev reads the du/dt the next u update averages with (caller order);
  * sph_state_set_restart, after sph_state_set (and sph_state_set_agn / _cooling when used), loads
  * the last force evaluation's acceleration and that du/dt (caller order), the injection's

@@ -337,6 +337,19 @@ def halo_accel(pos, sigma, G=1.0):
     return acc
 
 
+def timestep(pos, vel, h, c, divV, acc, acc_g, eps, C_CFL, alpha, off=None, idx=None):
+    """(dt_i, dh/dt_i) of dt_array_indiv_dt (FP64); vsig over the CSR sets, or over all
+    particles (the reference's loop) when off is None."""
+    pos = _f64(pos, (-1, 3)); N = len(pos)
+    dt = np.zeros(N); dh = np.zeros(N)
+    eps = np.broadcast_to(_f64(eps), (N,)).copy()
+    lib().or_timestep(_i64(N), _p(pos), _p(_f64(vel, (-1, 3))), _p(_f64(h)), _p(_f64(c)), _p(_f64(divV)),
+                      _p(_f64(acc, (-1, 3))), _p(_f64(acc_g, (-1, 3))), _p(eps), ctypes.c_double(C_CFL),
+                      ctypes.c_double(alpha), _p(off) if off is not None else None,
+                      _p(idx) if idx is not None else None, _p(dt), _p(dh))
+    return dt, dh
+
+
 def divcurl_csr(pos, v, rho, m, h, off, idx):
     pos = _f64(pos, (-1, 3)); N = len(pos)
     dv = np.zeros(N); cv = np.zeros(N)

@@ -1100,3 +1100,41 @@ void or_halo_accel(int64_t N, const double *pos, double sigma, double G, double 
         for (int d = 0; d < 3; d++) acc[3 * i + d] += -G * enclosed * x[d] / rr / rr / rr;
     }
 }
+
+/* ---------------------------------------------------------------- time-step criteria */
+/* dt_array_indiv_dt (Particles_in_BH_Tree_III/Evrard_Collapse_2024/hfvCLibs_v7_hfv_v8.1.h:1188-1268)
+ * in FP64 with the signal velocity's maximum over the CSR sets (off/idx) or, when off is NULL,
+ * over all particles as the reference loops: vsig_i = max_j (c_i + c_j - 3 w_ij),
+ * w_ij = v_ij . r_ij / (r_ij + 1e-5); dt_hyd = C h / vsig, dt_cour = C h / (h |divV| +
+ * max(c, |v|)(1 + 0.6 alpha)), dt_G1 = 0.4 sqrt(eps / |a_tot|), dt_G2 = 0.4 eps / |v|,
+ * dt_f = sqrt(h / |a_g|), dt_kin = sqrt(h / |a_tot|); dt = the smallest; dh/dt = h divV / 3. */
+void or_timestep(int64_t N, const double *pos, const double *vel, const double *h, const double *c,
+                 const double *divV, const double *acc, const double *acc_g, const double *eps, double C,
+                 double alpha, const int64_t *off, const int32_t *idx, double *dt, double *dhdt)
+{
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < N; i++) {
+        int64_t kb = off ? off[i] : 0, ke = off ? off[i + 1] : N;
+        double vmax = 0.0;
+        for (int64_t k = kb; k < ke; k++) {
+            int64_t j = off ? idx[k] : k;
+            double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1], dz = pos[3 * i + 2] - pos[3 * j + 2];
+            double rr = sqrt(dx * dx + dy * dy + dz * dz);
+            double vr = (vel[3 * i] - vel[3 * j]) * dx + (vel[3 * i + 1] - vel[3 * j + 1]) * dy +
+                        (vel[3 * i + 2] - vel[3 * j + 2]) * dz;
+            double vsig = c[i] + c[j] - 3.0 * (vr / (rr + 1e-5));
+            if (vsig > vmax) vmax = vsig;
+        }
+        const double *v = vel + 3 * i, *a = acc + 3 * i, *ag = acc_g + 3 * i;
+        double av = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        double at = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+        double agn = sqrt(ag[0] * ag[0] + ag[1] * ag[1] + ag[2] * ag[2]);
+        double t[6] = {C * h[i] / vmax, C * h[i] / (h[i] * fabs(divV[i]) + fmax(c[i], av) * (1.0 + 0.6 * alpha)),
+                       sqrt(h[i] / agn), sqrt(h[i] / at), 0.4 * sqrt(eps[i] / at), 0.4 * eps[i] / av};
+        double m = t[0];
+        for (int k = 1; k < 6; k++)
+            if (t[k] < m) m = t[k];
+        dt[i] = m;
+        dhdt[i] = h[i] * divV[i] / 3.0;
+    }
+}

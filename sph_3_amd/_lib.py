@@ -21,7 +21,7 @@ SOFT = {"spline": 0, "plummer": 1}
 H_NEWTON, H_REF_COUNT, H_KTH = 0, 1, 2
 
 PHASES = ["kick_drift", "order", "topology", "hsolve", "moments", "ngb", "density", "gravity", "divcurl",
-          "accsph", "kick", "dudt", "u_update", "moments_h", "cooling", "column"]
+          "accsph", "kick", "dudt", "u_update", "moments_h", "cooling", "column", "timestep"]
 
 _d = ctypes.POINTER(ctypes.c_double)
 _f = ctypes.POINTER(ctypes.c_float)
@@ -35,17 +35,17 @@ class StepParams(ctypes.Structure):
     _fields_ = [("dt", ctypes.c_double), ("gamma", ctypes.c_double), ("alpha", ctypes.c_double),
                 ("G", ctypes.c_double), ("theta", ctypes.c_double), ("eps_plummer", ctypes.c_double),
                 ("n_ngb", ctypes.c_double), ("h_tol", ctypes.c_double), ("period", ctypes.c_double),
-                ("soft_kind", ctypes.c_int),
+                ("C_CFL", ctypes.c_double), ("soft_kind", ctypes.c_int),
                 ("precision", ctypes.c_int), ("balsara", ctypes.c_int), ("timing", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int64), ("steps", ctypes.c_int64), ("ngb_total", ctypes.c_int64),
                 ("grav_pp", ctypes.c_int64), ("grav_pn", ctypes.c_int64), ("h_unconverged", ctypes.c_int64),
-                ("h_rounds", ctypes.c_int64), ("phase_ms", ctypes.c_double * 16),
-                ("phase_calls", ctypes.c_int64 * 16), ("h_tests", ctypes.c_int64), ("h_terms", ctypes.c_int64),
+                ("h_rounds", ctypes.c_int64), ("phase_ms", ctypes.c_double * 20),
+                ("phase_calls", ctypes.c_int64 * 20), ("h_tests", ctypes.c_int64), ("h_terms", ctypes.c_int64),
                 ("ngb_extra", ctypes.c_int64), ("cool_floor", ctypes.c_int64), ("cool_fallback", ctypes.c_int64),
-                ("h_big", ctypes.c_int64)]
+                ("h_big", ctypes.c_int64), ("dt_min", ctypes.c_double)]
 
 
 class CoolParams(ctypes.Structure):
@@ -120,6 +120,7 @@ _SIGS = {
     "sph_state_set_agn": ([_vp, ctypes.POINTER(AgnParams)], ctypes.c_int),
     "sph_state_count": ([_vp, _i64p, _d, ctypes.POINTER(ctypes.c_uint64), _i64p], ctypes.c_int),
     "sph_state_get_duprev": ([_vp, _d], ctypes.c_int),
+    "sph_state_get_timestep": ([_vp, _d, _d], ctypes.c_int),
     "sph_state_set_restart": ([_vp, _d, _d, ctypes.c_double, ctypes.c_uint64, _i64], ctypes.c_int),
     "sph_state_get_column": ([_vp, _d, _d], ctypes.c_int),
     "sph_get_stats": ([_vp, ctypes.POINTER(Stats)], ctypes.c_int),

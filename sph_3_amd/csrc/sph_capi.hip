@@ -118,6 +118,9 @@ struct sph_ctx {
     bool col_on = false, col_ok = false;  // column density in the step; computed for the current state
     sph::ColumnParams col_p{};
     Buf colg, coln;  // |grad rho|, log10 N_H (sorted order, FP64)
+    bool ts_ok = false;     // time-step criteria of the last force evaluation
+    int64_t ts_n = 0;       // particles they cover (later injections are not covered)
+    Buf tsdt, tsdh;         // dt_i, dh/dt_i (sorted order, FP64)
     sph_cool_params cool_p{};
     // cooling surrogate: parameters, Adam state, BF16 weight copies, samples, error map
     Buf mP, mG, mM, mV, mW[4], mx, mxn, mt, my, mloss, memap, memap2, memap3, mgate, mcnt;
@@ -504,7 +507,7 @@ extern "C" int sph_ctx_destroy(sph_ctx *ctx)
                   &ctx->stage, &ctx->kpart, &ctx->kout, &ctx->spread, &ctx->ccidx, &ctx->cccnt,
                   &ctx->cext, &ctx->cnext, &ctx->ttab, &ctx->zmet, &ctx->j21, &ctx->ucols, &ctx->uyn,
                   &ctx->uL, &ctx->uG, &ctx->uflag, &ctx->ucnt, &ctx->hbig, &ctx->p4, &ctx->v4, &ctx->s4,
-                  &ctx->colg, &ctx->coln};
+                  &ctx->colg, &ctx->coln, &ctx->tsdt, &ctx->tsdh};
     for (Buf *b : all)
         if (b->p) (void)hipFree(b->p);
     for (Buf &b : ctx->shadow)
@@ -532,6 +535,8 @@ extern "C" const char *sph_last_error(const sph_ctx *ctx)
 }
 
 // =========================================================== h solves
+static_assert(SPH_PH_COUNT <= 20, "sph_stats holds 20 phases");
+
 namespace {
 
 // Parameters of an h solve over m targets (scratch in the context); host counters go to
@@ -923,6 +928,7 @@ extern "C" int sph_state_set(sph_ctx *ctx, int64_t n, const double *pos, const d
     ctx->resident = false;
     ctx->cool_on = false;  // Z and J21 belong to the previous state
     ctx->col_on = ctx->col_ok = false;
+    ctx->ts_ok = false;
     ctx->agn_on = false;
     ctx->agn_leftover = 0.0;
     ctx->agn_injected = 0;
@@ -1165,6 +1171,30 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
                        P_<unsigned long long>(ctx->hcount) + 12));
         ph_end(ctx, SPH_PH_COOLING);
     }
+    // per-particle time-step criteria of this evaluation (velocities after the kick)
+    ctx->ts_ok = false;
+    if (p->C_CFL > 0.0) {
+        if (!bal)  // the libsx forms skip div v: the Courant criterion needs it
+            HIPCK(sph::divcurl<R>(st, n, P_<R>(ctx->rho), P_<R>(ctx->c), rows, P_<R>(ctx->divV), P_<R>(ctx->curlV),
+                                  nullptr));
+        const size_t cap = (size_t)std::max(n, ctx->n_reserve) * 8;
+        CK(ensure(ctx, ctx->tsdt, cap));
+        CK(ensure(ctx, ctx->tsdh, cap));
+        unsigned long long *dmin = P_<unsigned long long>(ctx->hcount) + 15;
+        HIPCK(sph::fill<unsigned long long>(st, 1, dmin, ~0ull));
+        // softening of the gravity criteria: the Plummer length, else each particle's h (the
+        // reference's eps = h of its initial conditions)
+        const bool eps_h = p->soft_kind == SPH_SOFT_SPLINE || !(p->eps_plummer > 0.0);
+        const sph::StepCriteria sc{p->C_CFL, p->alpha, p->eps_plummer};
+        const bool g = p->G != 0.0;
+        ph_begin(ctx, SPH_PH_TIMESTEP);
+        HIPCK(sph::timestep<R>(st, n, P_<R4<R>>(ctx->s4), P_<R>(ctx->divV), rows, D(ax), D(ay), D(az),
+                               g ? D(gx) : nullptr, g ? D(gy) : nullptr, g ? D(gz) : nullptr, eps_h ? D(h) : nullptr,
+                               sc, D(tsdt), D(tsdh), dmin));
+        ph_end(ctx, SPH_PH_TIMESTEP);
+        ctx->ts_ok = true;
+        ctx->ts_n = n;
+    }
     return SPH_OK;
 }
 
@@ -1177,6 +1207,7 @@ int check_params(sph_ctx *ctx, const sph_step_params *p)
     if (!(p->dt > 0.0) || !(p->n_ngb > 0.0) || !(p->h_tol > 0.0) || !(p->theta >= 0.0))
         return fail(ctx, SPH_ERR_ARG, "dt, n_ngb, h_tol must be > 0 and theta >= 0");
     if (!(p->period >= 0.0)) return fail(ctx, SPH_ERR_ARG, "period must be >= 0");
+    if (!(p->C_CFL >= 0.0)) return fail(ctx, SPH_ERR_ARG, "C_CFL must be >= 0");
     if (p->period > 0.0 && p->G != 0.0)
         return fail(ctx, SPH_ERR_ARG, "a periodic box needs G = 0 (periodic gravity is not provided)");
     HIPCK(hipSetDevice(ctx->device));
@@ -1245,6 +1276,14 @@ int finish_step(sph_ctx *ctx, const sph_step_params *p)
     }
     ctx->ph_rec = 0;
     ctx->rho_prec = p->precision;
+    ctx->stats.dt_min = 0.0;
+    if (ctx->ts_ok) {
+        unsigned long long bits = 0;
+        HIPCK(hipMemcpy(&bits, P_<unsigned long long>(ctx->hcount) + 15, sizeof(bits), hipMemcpyDeviceToHost));
+        double v;
+        std::memcpy(&v, &bits, sizeof(v));
+        ctx->stats.dt_min = bits == ~0ull ? INFINITY : v;
+    }
     return SPH_OK;
 }
 
@@ -2031,6 +2070,21 @@ extern "C" int sph_column_density(sph_ctx *ctx, int64_t n, const double *pos, co
     }
     if (grad_out) CK(get_scalar<double>(ctx, ctx->colg, grad_out));
     return get_scalar<double>(ctx, ctx->coln, log_nh_out);
+}
+
+extern "C" int sph_state_get_timestep(sph_ctx *ctx, double *dt_out, double *dhdt_out)
+{
+    if (!ctx) return SPH_ERR_ARG;
+    if (!ctx->resident) return fail(ctx, SPH_ERR_STATE, "no resident state (call sph_state_set first)");
+    if (!ctx->ts_ok) return fail(ctx, SPH_ERR_STATE, "no time-step criteria (set C_CFL > 0 in sph_step_params)");
+    HIPCK(hipSetDevice(ctx->device));
+    if (ctx->n > ctx->ts_n) {  // particles injected after the evaluation
+        HIPCK(sph::fill<double>(ctx->st, ctx->n - ctx->ts_n, D(tsdt) + ctx->ts_n, NAN));
+        HIPCK(sph::fill<double>(ctx->st, ctx->n - ctx->ts_n, D(tsdh) + ctx->ts_n, NAN));
+    }
+    if (dt_out) CK(get_scalar<double>(ctx, ctx->tsdt, dt_out));
+    if (dhdt_out) CK(get_scalar<double>(ctx, ctx->tsdh, dhdt_out));
+    return SPH_OK;
 }
 
 extern "C" int sph_state_get_duprev(sph_ctx *ctx, double *duprev_out)

@@ -72,6 +72,25 @@ hipError_t dudt(hipStream_t st, int64_t n, const R4<R> *s4, bool balsara, double
 // (pair arithmetic in R, the kernel gradient's q + 1e-10 as in that kernel), then in FP64
 // N_H = 0.5 (h_i / cbrt(n_ngb) + rho_i / g_i) unit_length_cm * rho_i unit_density_cgs XH / m_H,
 // log_nh[i] = log10 N_H (cm^-2) and grad[i] = g_i (code units; may be null).
+// Per-particle time-step criteria over the rows (dt_array_indiv_dt,
+// Particles_in_BH_Tree_III/Evrard_Collapse_2024/hfvCLibs_v7_hfv_v8.1.h:1188-1268, with the signal
+// velocity's maximum over S_i instead of over all particles):
+//   vsig_i = max_j (c_i + c_j - 3 w_ij), w_ij = v_ij . r_ij / (r_ij + 1e-5)   (pairs in R)
+//   dt_hyd = C h / vsig, dt_cour = C h / (h |div v| + max(c, |v|)(1 + 0.6 alpha)),
+//   dt_G1 = 0.4 sqrt(eps / |a_tot|), dt_G2 = 0.4 eps / |v|, dt_f = sqrt(h / |a_g|),
+//   dt_kin = sqrt(h / |a_tot|); dt_i = the smallest (FP64), dh/dt = h div v / 3.
+// s4.z = c; divv in R; a_tot = (ax, ay, az); a_g = (gx, gy, gz) or null (then a_tot);
+// eps per particle or, when null, eps_const.  dt (required), dhdt (may be null); *dt_min_bits
+// (device, initialise to ~0) receives the minimum's bit pattern (positive doubles order as integers).
+struct StepCriteria {
+    double C_CFL, alpha, eps_const;
+};
+template <typename R>
+hipError_t timestep(hipStream_t st, int64_t n, const R4<R> *s4, const R *divv, const NgbRows &rows,
+                    const double *ax, const double *ay, const double *az, const double *gx, const double *gy,
+                    const double *gz, const double *eps, const StepCriteria &sc, double *dt, double *dhdt,
+                    unsigned long long *dt_min_bits);
+
 struct ColumnParams {
     double unit_density_cgs, unit_length_cm, XH, n_ngb;
 };

@@ -13,6 +13,8 @@
 // Pair arithmetic is in R; coordinate and velocity differences are taken in
 // FP64 and rounded once to R.
 #include "sph_hydro.h"
+#include <algorithm>
+#include <type_traits>
 
 #define HG 16  // lanes per particle
 
@@ -356,6 +358,81 @@ __global__ void __launch_bounds__(256) k_column(int64_t n, const R *__restrict__
     }
 }
 
+// Signal-velocity maximum over row i (16 lanes, max is exact in any order), then the six
+// criteria of the reference's dt_array_indiv_dt in FP64 by the particle's first lane.
+template <typename R, int G, bool PER>
+__global__ void __launch_bounds__(256) k_timestep(int64_t n, const R4<R> *__restrict__ s4, const R *__restrict__ divv,
+                                                  NgbRows rows, const double *__restrict__ ax,
+                                                  const double *__restrict__ ay, const double *__restrict__ az,
+                                                  const double *__restrict__ gx, const double *__restrict__ gy,
+                                                  const double *__restrict__ gz, const double *__restrict__ eps,
+                                                  sph::StepCriteria sc, double *__restrict__ dt,
+                                                  double *__restrict__ dhdt)
+{
+    int64_t gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    int64_t i = gid / G;
+    int l = (int)(gid % G);
+    bool valid = i < n;
+    R vmax = (R)0;  // the row holds i itself (w = 0): vsig >= 2 c_i >= 0
+    if (valid) {
+        const double4 pi4 = rows.p4[i], vi4 = rows.v4[i];
+        const R ci = s4[i].z;
+        int64_t k0, e;
+        row_range(rows, i, k0, e);
+        for (int64_t k = k0 + l; k < e; k += G) {
+            const int j = rows.idx[k];
+            const double4 pj = rows.p4[j], vj = rows.v4[j];
+            R dx = (R)wrap_d<PER>(pi4.x - pj.x, rows.L), dy = (R)wrap_d<PER>(pi4.y - pj.y, rows.L),
+              dz = (R)wrap_d<PER>(pi4.z - pj.z, rows.L);
+            R rr = sph_sqrt<R>(dx * dx + dy * dy + dz * dz);
+            R vr = (R)(vi4.x - vj.x) * dx + (R)(vi4.y - vj.y) * dy + (R)(vi4.z - vj.z) * dz;
+            R w;
+            if constexpr (std::is_same<R, float>::value)
+                w = vr * __builtin_amdgcn_rcpf(rr + 1e-5f);  // 1 ulp reciprocal
+            else
+                w = vr / (rr + 1e-5);
+            R vsig = ci + s4[j].z - (R)3 * w;
+            vmax = vsig > vmax ? vsig : vmax;
+        }
+    }
+    for (int off = G / 2; off >= 1; off >>= 1) {
+        const R o = __shfl_xor(vmax, off, G);
+        vmax = o > vmax ? o : vmax;
+    }
+    if (valid && l == 0) {
+        const double h = rows.p4[i].w, c = (double)s4[i].z, dv = (double)divv[i];
+        const double4 v = rows.v4[i];
+        const double av = sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+        const double at = sqrt(ax[i] * ax[i] + ay[i] * ay[i] + az[i] * az[i]);
+        const double ag = gx ? sqrt(gx[i] * gx[i] + gy[i] * gy[i] + gz[i] * gz[i]) : at;
+        const double ep = eps ? eps[i] : sc.eps_const;
+        const double dt_hyd = sc.C_CFL * h / (double)vmax;
+        const double dt_cour = sc.C_CFL * h / (h * fabs(dv) + fmax(c, av) * (1.0 + 0.6 * sc.alpha));
+        const double dt_g1 = 0.4 * sqrt(ep / at), dt_g2 = 0.4 * ep / av;
+        const double dt_f = sqrt(h / ag), dt_kin = sqrt(h / at);
+        dt[i] = fmin(fmin(fmin(dt_hyd, dt_cour), fmin(dt_f, dt_kin)), fmin(dt_g1, dt_g2));
+        if (dhdt) dhdt[i] = h * dv / 3.0;
+    }
+}
+
+// *out = bits of min_i v[i] (v >= 0): grid-stride, wave and workgroup minima, one atomic per
+// workgroup (positive doubles order as their bit patterns).
+__global__ void __launch_bounds__(256) k_min_bits(int64_t n, const double *__restrict__ v,
+                                                  unsigned long long *__restrict__ out)
+{
+    __shared__ double wmin[4];
+    double m = INFINITY;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        m = fmin(m, v[i]);
+    for (int off = 32; off >= 1; off >>= 1) m = fmin(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmin(fmin(wmin[0], wmin[1]), fmin(wmin[2], wmin[3]));
+        if (m < INFINITY) atomicMin(out, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
 namespace sph {
 
 __global__ void __launch_bounds__(256) k_pack4(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
@@ -436,6 +513,20 @@ hipError_t dudt(hipStream_t st, int64_t n, const R4<R> *s4, bool balsara, double
     return hipGetLastError();
 }
 template <typename R>
+hipError_t timestep(hipStream_t st, int64_t n, const R4<R> *s4, const R *divv, const NgbRows &rows,
+                    const double *ax, const double *ay, const double *az, const double *gx, const double *gy,
+                    const double *gz, const double *eps, const StepCriteria &sc, double *dt, double *dhdt,
+                    unsigned long long *dt_min_bits)
+{
+    if (n <= 0) return hipSuccess;
+    if (!dt) return hipErrorInvalidValue;
+    LAUNCH(k_timestep, n, s4, divv, rows, ax, ay, az, gx, gy, gz, eps, sc, dt, dhdt);
+    const unsigned nb = (unsigned)std::min<int64_t>(1024, (n + 255) / 256);
+    k_min_bits<<<nb, 256, 0, st>>>(n, dt, dt_min_bits);
+    return hipGetLastError();
+}
+
+template <typename R>
 hipError_t column_density(hipStream_t st, int64_t n, const R *rho, const NgbRows &rows, const ColumnParams &cp,
                           double *grad, double *log_nh)
 {
@@ -456,7 +547,11 @@ hipError_t column_density(hipStream_t st, int64_t n, const R *rho, const NgbRows
                                   const double *, const double *, const double *, double *, double *, double *);  \
     template hipError_t dudt<R>(hipStream_t, int64_t, const R4<R> *, bool, double, const NgbRows &, R *);         \
     template hipError_t column_density<R>(hipStream_t, int64_t, const R *, const NgbRows &, const ColumnParams &,  \
-                                          double *, double *);
+                                          double *, double *);                                                     \
+    template hipError_t timestep<R>(hipStream_t, int64_t, const R4<R> *, const R *, const NgbRows &,              \
+                                    const double *, const double *, const double *, const double *,              \
+                                    const double *, const double *, const double *, const StepCriteria &,        \
+                                    double *, double *, unsigned long long *);
 INST(float)
 INST(double)
 #undef INST

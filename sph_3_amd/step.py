@@ -56,10 +56,11 @@ class StepConfig:
     precision: str = "mixed"
     balsara: bool = True
     period: float = 0.0  # > 0: periodic box [0, period)^3 (minimum image; needs G = 0)
+    C_CFL: float = 0.0   # > 0: per-particle time-step criteria after every force evaluation
 
     def params(self, timing=False) -> StepParams:
         return StepParams(self.dt, self.gamma, self.alpha, self.G, self.theta, self.eps_plummer, self.n_ngb,
-                          self.h_tol, self.period, SOFT[self.softening], PREC[self.precision], int(self.balsara),
+                          self.h_tol, self.period, self.C_CFL, SOFT[self.softening], PREC[self.precision], int(self.balsara),
                           int(timing))
 
 
@@ -96,6 +97,13 @@ class DeviceSPH:
         ap = params.c()
         c.check(c.lib.sph_state_set_agn(c.h, ctypes.byref(ap)), "sph_state_set_agn")
         self._agn = params
+
+    def timestep(self):
+        """(dt_i, dh/dt_i) of the last force evaluation (StepConfig.C_CFL > 0), caller order."""
+        dt = np.empty(self.n); dh = np.empty(self.n)
+        c = self.ctx
+        c.check(c.lib.sph_state_get_timestep(c.h, ptr(dt), ptr(dh)), "sph_state_get_timestep")
+        return dt, dh
 
     def leftover_mass(self) -> float:
         lo = np.zeros(1)
@@ -204,7 +212,7 @@ class DeviceSPH:
         c.check(c.lib.sph_get_stats(c.h, ctypes.byref(s)), "sph_get_stats")
         return dict(n=s.n, steps=s.steps, ngb_total=s.ngb_total, grav_pp=s.grav_pp, grav_pn=s.grav_pn,
                     h_unconverged=s.h_unconverged, h_rounds=s.h_rounds, h_tests=s.h_tests, h_terms=s.h_terms, ngb_extra=s.ngb_extra,
-                    cool_floor=s.cool_floor, cool_fallback=s.cool_fallback, h_big=s.h_big,
+                    cool_floor=s.cool_floor, cool_fallback=s.cool_fallback, h_big=s.h_big, dt_min=s.dt_min,
                     phase_ms={k: s.phase_ms[i] for i, k in enumerate(PHASES)},
                     phase_calls={k: s.phase_calls[i] for i, k in enumerate(PHASES)})
 

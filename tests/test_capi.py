@@ -46,8 +46,8 @@ def test_ctx_create_rejects_bad_args_without_gpu():
 
 def test_step_params_layout_matches_header():
     # 9 doubles + 4 ints
-    assert ctypes.sizeof(L.StepParams) == 9 * 8 + 4 * 4
-    assert ctypes.sizeof(L.Stats) == 7 * 8 + 16 * 8 + 16 * 8 + 6 * 8
+    assert ctypes.sizeof(L.StepParams) == 10 * 8 + 4 * 4
+    assert ctypes.sizeof(L.Stats) == 7 * 8 + 20 * 8 + 20 * 8 + 6 * 8 + 8
     # 6 doubles + 2 ints
     assert ctypes.sizeof(L.CoolParams) == 6 * 8 + 2 * 4
     assert ctypes.sizeof(L.ColumnParams) == 4 * 8
