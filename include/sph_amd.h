@@ -290,7 +290,7 @@ typedef struct sph_step_params {
     double period;      /* > 0: side of a periodic box [0, period)^3 with minimum-image distances
                            (positions wrapped by the drift; needs G = 0); 0: open boundaries */
     double C_CFL;       /* > 0: after every force evaluation the per-particle time-step criteria of
-                           dt_array_indiv_dt (EV/lib.h:1188-1268) with this Courant factor, the
+                           dt_array_indiv_dt (EV/lib.h:1191-1270) with this Courant factor, the
                            signal velocity's maximum over S_i; the smallest is sph_stats.dt_min and
                            sph_state_get_timestep reads them back.  0: off */
     int soft_kind;      /* SPH_SOFT_SPLINE uses eps_i = h_i (EV/lib.h:584) */

@@ -1102,7 +1102,7 @@ void or_halo_accel(int64_t N, const double *pos, double sigma, double G, double 
 }
 
 /* ---------------------------------------------------------------- time-step criteria */
-/* dt_array_indiv_dt (Particles_in_BH_Tree_III/Evrard_Collapse_2024/hfvCLibs_v7_hfv_v8.1.h:1188-1268)
+/* dt_array_indiv_dt (Particles_in_BH_Tree_III/Evrard_Collapse_2024/hfvCLibs_v7_hfv_v8.1.h:1191-1270)
  * in FP64 with the signal velocity's maximum over the CSR sets (off/idx) or, when off is NULL,
  * over all particles as the reference loops: vsig_i = max_j (c_i + c_j - 3 w_ij),
  * w_ij = v_ij . r_ij / (r_ij + 1e-5); dt_hyd = C h / vsig, dt_cour = C h / (h |divV| +

@@ -73,7 +73,7 @@ hipError_t dudt(hipStream_t st, int64_t n, const R4<R> *s4, bool balsara, double
 // N_H = 0.5 (h_i / cbrt(n_ngb) + rho_i / g_i) unit_length_cm * rho_i unit_density_cgs XH / m_H,
 // log_nh[i] = log10 N_H (cm^-2) and grad[i] = g_i (code units; may be null).
 // Per-particle time-step criteria over the rows (dt_array_indiv_dt,
-// Particles_in_BH_Tree_III/Evrard_Collapse_2024/hfvCLibs_v7_hfv_v8.1.h:1188-1268, with the signal
+// Particles_in_BH_Tree_III/Evrard_Collapse_2024/hfvCLibs_v7_hfv_v8.1.h:1191-1270, with the signal
 // velocity's maximum over S_i instead of over all particles):
 //   vsig_i = max_j (c_i + c_j - 3 w_ij), w_ij = v_ij . r_ij / (r_ij + 1e-5)   (pairs in R)
 //   dt_hyd = C h / vsig, dt_cour = C h / (h |div v| + max(c, |v|)(1 + 0.6 alpha)),
