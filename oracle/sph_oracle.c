@@ -589,7 +589,6 @@ void or_tree_moments(int64_t N, const double *spos, const double *sm, const doub
                      const int32_t *right, int bucket, double *box, double *com, double *mass,
                      double *hmax, double *epsmax, double *size)
 {
-    int64_t nl = N - 1;
     if (bucket < 1) bucket = 1;
     if (N < 2 || count[0] <= bucket) {
         for (int64_t k = 0; k < 2 * N - 1; k++)
@@ -1107,7 +1106,8 @@ void or_halo_accel(int64_t N, const double *pos, double sigma, double G, double 
  * over all particles as the reference loops: vsig_i = max_j (c_i + c_j - 3 w_ij),
  * w_ij = v_ij . r_ij / (r_ij + 1e-5); dt_hyd = C h / vsig, dt_cour = C h / (h |divV| +
  * max(c, |v|)(1 + 0.6 alpha)), dt_G1 = 0.4 sqrt(eps / |a_tot|), dt_G2 = 0.4 eps / |v|,
- * dt_f = sqrt(h / |a_g|), dt_kin = sqrt(h / |a_tot|); dt = the smallest; dh/dt = h divV / 3. */
+ * dt_f = sqrt(h / |a_g|), dt_kin = sqrt(h / |a_tot|); dt = the smallest; dh/dt = h divV / 3.
+ * Pair offsets take the minimum image when a period is set (or_set_period). */
 void or_timestep(int64_t N, const double *pos, const double *vel, const double *h, const double *c,
                  const double *divV, const double *acc, const double *acc_g, const double *eps, double C,
                  double alpha, const int64_t *off, const int32_t *idx, double *dt, double *dhdt)
@@ -1118,7 +1118,8 @@ void or_timestep(int64_t N, const double *pos, const double *vel, const double *
         double vmax = 0.0;
         for (int64_t k = kb; k < ke; k++) {
             int64_t j = off ? idx[k] : k;
-            double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1], dz = pos[3 * i + 2] - pos[3 * j + 2];
+            double dx = wrapd(pos[3 * i] - pos[3 * j]), dy = wrapd(pos[3 * i + 1] - pos[3 * j + 1]),
+                   dz = wrapd(pos[3 * i + 2] - pos[3 * j + 2]);
             double rr = sqrt(dx * dx + dy * dy + dz * dz);
             double vr = (vel[3 * i] - vel[3 * j]) * dx + (vel[3 * i + 1] - vel[3 * j + 1]) * dy +
                         (vel[3 * i + 2] - vel[3 * j + 2]) * dz;

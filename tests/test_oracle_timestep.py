@@ -48,3 +48,19 @@ def test_csr_maximum_equals_all_particles_when_all_are_neighbours():
     a = O.timestep(pos, vel, h, c, divV, acc, acc, 0.01, 0.3, 1.0, off, idx)
     b = O.timestep(pos, vel, h, c, divV, acc, acc, 0.01, 0.3, 1.0)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_periodic_pair_through_the_boundary_equals_the_unwrapped_pair():
+    """A pair approaching through x = 0 of the unit box has the signal velocity of the
+    same pair placed side by side without the box (minimum image)."""
+    h = np.array([0.05, 0.05]); c = np.array([1.0, 1.0]); dv = np.zeros(2)
+    vel = np.array([[-0.5, 0, 0], [0.5, 0, 0]])
+    a = np.ones((2, 3))
+    wrapped = np.array([[0.01, 0.5, 0.5], [0.99, 0.5, 0.5]])
+    plain = np.array([[0.01, 0.5, 0.5], [-0.01, 0.5, 0.5]])
+    ref, _ = O.timestep(plain, vel, h, c, dv, a, a, 1.0, 0.3, 1.0)  # eps = 1: dt_hyd is the minimum
+    open_box, _ = O.timestep(wrapped, vel, h, c, dv, a, a, 1.0, 0.3, 1.0)
+    with O.periodic(1.0):
+        per, _ = O.timestep(wrapped, vel, h, c, dv, a, a, 1.0, 0.3, 1.0)
+    np.testing.assert_allclose(per, ref, rtol=1e-13)
+    assert np.all(open_box > per)  # without the wrap the pair looks 0.98 apart and receding
