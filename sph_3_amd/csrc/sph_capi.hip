@@ -400,12 +400,13 @@ int build_ngb(sph_ctx *ctx, bool csr, const double *hscale = nullptr, double per
 
 // The neighbour sets plus the hydro passes' packed inputs, (x, y, z, h) and (vx, vy, vz, m)
 // of the sorted particles, packed from the current arrays (call after any of them changes).
-int ngb_rows(sph_ctx *ctx, NgbRows &r)
+// vel_only: only the velocities changed since the last call (positions, h, m and n as then).
+int ngb_rows(sph_ctx *ctx, NgbRows &r, bool vel_only = false)
 {
     const int64_t n = ctx->n;
     CK(ensure(ctx, ctx->p4, (size_t)n * 32));
     CK(ensure(ctx, ctx->v4, (size_t)n * 32));
-    HIPCK(sph::pack4(ctx->st, n, D(x), D(y), D(z), D(h), P_<double4>(ctx->p4)));
+    if (!vel_only) HIPCK(sph::pack4(ctx->st, n, D(x), D(y), D(z), D(h), P_<double4>(ctx->p4)));
     HIPCK(sph::pack4(ctx->st, n, D(vx), D(vy), D(vz), D(m), P_<double4>(ctx->v4)));
     r = NgbRows{P_<int64_t>(ctx->off), I32(cnt), I32(idx), ctx->ngb_stride, ctx->ngb_per,
                 P_<double4>(ctx->p4), P_<double4>(ctx->v4)};
@@ -1157,7 +1158,7 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     }
     ph_begin(ctx, SPH_PH_DUDT);
     NgbRows rows;
-    CK(ngb_rows(ctx, rows));  // packs the kicked velocities
+    CK(ngb_rows(ctx, rows, true));  // packs the kicked velocities (positions and h as in the force passes)
     HIPCK(sph::dudt<R>(st, n, P_<R4<R>>(ctx->s4), bal != nullptr, p->alpha, rows, P_<R>(ctx->du)));
     ph_end(ctx, SPH_PH_DUDT);
     ph_begin(ctx, SPH_PH_UUPDATE);
