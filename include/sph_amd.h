@@ -111,6 +111,25 @@ int sph_gravity(sph_ctx *ctx, int64_t n, const double *pos, const double *m, con
 /* out5 = [KE, px, py, pz, M] (FP64).  Replaces libsx.py:571 getKE. */
 int sph_kinetics(sph_ctx *ctx, int64_t n, const double *vel, const double *m, double *out5);
 
+/* Gravitational potential energy PE = -1/2 sum_i m_i phi_i with phi_i = G sum_j m_j psi(r_ij) over
+ * the tree walk's interaction list (theta = 0: every pair, exactly the reference's double sum).
+ * Replaces libsx.py:585 getPE(pos, m, G, epsilon) (soft_kind SPH_SOFT_SPLINE: its spline potential
+ * with eps_ij = (eps_i+eps_j)/2; coincident pairs add nothing, as there); SPH_SOFT_PLUMMER:
+ * psi = 1/sqrt(r^2 + eps_plummer^2).  The pair sums run in the precision's type, the total in FP64.
+ * phi_out (n values, caller order) may be NULL. */
+int sph_potential_energy(sph_ctx *ctx, int64_t n, const double *pos, const double *m, const double *eps, double G,
+                         double theta, int soft_kind, double eps_plummer, int precision, double *pe_out,
+                         double *phi_out);
+
+/* Host-side check of a particle set, as every entry point that takes particles runs it before
+ * uploading anything: finite positions with a non-zero bounding box when n > 1 (coincident
+ * particles give no Morton order), and where given finite velocities, masses >= 0, u >= 0,
+ * h > 0.  Any array may be NULL.  Returns SPH_OK or SPH_ERR_ARG with the reason in msg
+ * (msg_cap bytes, may be 0).  The reference has no such check: its drivers print and exit
+ * (EV/ngb.h:254-258, EV/drv.cu:1161-1168) or go on with NaNs.  Needs no GPU. */
+int sph_validate_particles(int64_t n, const double *pos, const double *vel, const double *m, const double *u,
+                           const double *h, char *msg, int64_t msg_cap);
+
 /* ---- cooling (Particles_in_BH_Tree_III/hcool_with_unit_tests_Here/photolibs3.py and
  *      CR/v4t.h:1768-1884 doCloudyHCooling) ----------------------------------
  * Arrays are elementwise in caller order (cgs units).  J21: amplitude of the
@@ -297,6 +316,8 @@ typedef struct sph_step_params {
     int precision;      /* SPH_PREC_MIXED / SPH_PREC_FP64 */
     int balsara;        /* 1: shear.py forms, 0: libsx.py forms */
     int timing;         /* 1: record per-phase HIP-event times */
+    int energy;         /* 1: the gravity pass also sums the potential, and each force evaluation ends
+                           with FP64 totals of the state (sph_stats.energy, momentum, ang_momentum) */
 } sph_step_params;
 
 typedef struct sph_stats {
@@ -315,6 +336,14 @@ typedef struct sph_stats {
     int64_t h_big;           /* last h solve: targets whose solution needs more candidates than a
                                 row holds (solved by a candidate-free tree-walk Newton) */
     double dt_min;           /* last force evaluation with C_CFL > 0: min_i dt_i (else 0) */
+    /* last force evaluation with sph_step_params.energy (FP64 sums at the end of the evaluation:
+     * positions of the forces, velocities after the second kick, u after its update and cooling;
+     * the priming evaluation: the initial velocities).  energy = {KE = sum m v^2/2, U = sum m u,
+     * PE = -1/2 sum m phi of the self-gravity tree walk (0 without gravity; the AGN halo's external
+     * potential is not included)}; momentum = sum m v; ang_momentum = sum m x cross v. */
+    double energy[3];
+    double momentum[3];
+    double ang_momentum[3];
 } sph_stats;
 
 /* phase indices of sph_stats.phase_ms (SPH_PH_HMOMENTS: the node boxes built for the h walk;

@@ -152,9 +152,12 @@ def ke(v, m):
     return lib().or_ke(_i64(len(v)), _p(v), _p(_f64(m)))
 
 
-def pe(pos, m, G, eps):
-    pos = _f64(pos, (-1, 3))
-    return lib().or_pe(_i64(len(pos)), _p(pos), _p(_f64(m)), ctypes.c_double(G), _p(_f64(eps)))
+def pe(pos, m, G, eps, soft_kind=SOFT_SPLINE, eps_plummer=0.0):
+    """Potential energy: getPE's spline form (eps_ij = (eps_i + eps_j)/2) or Plummer."""
+    pos = _f64(pos, (-1, 3)); N = len(pos)
+    eps = np.zeros(N) if eps is None else np.broadcast_to(_f64(eps), (N,)).copy()
+    return lib().or_pe(_i64(N), _p(pos), _p(_f64(m)), ctypes.c_double(G), _p(eps), ctypes.c_int(soft_kind),
+                       ctypes.c_double(eps_plummer))
 
 
 def h_initial(pos, sub=None, k=50):
@@ -291,6 +294,17 @@ def h_newton(pos, h0, n_ngb=64.0, tol=1e-12, max_iter=200):
     out = np.zeros(len(pos))
     bad = lib().or_h_newton(_i64(len(pos)), _p(pos), _p(_f64(h0)), ctypes.c_double(n_ngb), ctypes.c_double(tol),
                             _i64(max_iter), _p(out))
+    return out, int(bad)
+
+
+def h_newton_rows(pos, rows, h0, n_ngb=64.0, tol=1e-12, max_iter=200):
+    """h_newton for the particles ``rows`` only (O(len(rows) N)); h0 per row."""
+    pos = _f64(pos, (-1, 3))
+    rows = np.ascontiguousarray(rows, np.int64)
+    out = np.zeros(len(rows))
+    lib().or_h_newton_rows.restype = ctypes.c_int64
+    bad = lib().or_h_newton_rows(_i64(len(pos)), _p(pos), _i64(len(rows)), _p(rows), _p(_f64(h0)),
+                                 ctypes.c_double(n_ngb), ctypes.c_double(tol), _i64(max_iter), _p(out))
     return out, int(bad)
 
 

@@ -262,15 +262,22 @@ double or_ke(int64_t N, const double *v, const double *m)
     return ke;
 }
 
-/* getPE, libsx.py:583-620 (spline potential, eps_ij = (eps_i+eps_j)/2) */
-double or_pe(int64_t N, const double *pos, const double *m, double G, const double *eps)
+/* getPE, libsx.py:583-620 (spline potential, eps_ij = (eps_i+eps_j)/2) for
+ * soft_kind 0; soft_kind 1: Plummer, -G sum_{i<j} m_i m_j / sqrt(r^2 + eps^2)
+ * with the scalar eps_plummer (the potential of grav_fk_plummer's force). */
+double or_pe(int64_t N, const double *pos, const double *m, double G, const double *eps, int soft_kind,
+             double eps_plummer)
 {
-    double pe = 0.0;
+    double pe = 0.0, e2 = eps_plummer * eps_plummer;
 #pragma omp parallel for reduction(+ : pe) schedule(dynamic, 16)
     for (int64_t i = 0; i < N; i++) {
         for (int64_t j = i + 1; j < N; j++) {
             double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
                    dz = pos[3 * i + 2] - pos[3 * j + 2];
+            if (soft_kind == 1) {
+                pe -= G * m[i] * m[j] / sqrt(dx * dx + dy * dy + dz * dz + e2);
+                continue;
+            }
             double rr = sqrt(dx * dx + dy * dy + dz * dz), fk = 0.0, inv_r = 0.0;
             if (rr != 0.0) inv_r = 1.0 / rr;
             double e = 0.5 * (eps[i] + eps[j]), q = rr / e;
@@ -860,9 +867,33 @@ static void nr_f(const double *r2, int64_t n, double h, double n_ngb, double *f,
     *df = (32.0 / 3.0) * ds;
 }
 
-/* Newton-Raphson smoothing length (the build's FP64 h solve): root of
- * f(h) = 0 above, bracketed, |dh| <= tol*h.  Brute-force distances; this is
- * the checker of the device solve.  Returns number of rows that failed. */
+/* Newton-Raphson smoothing length (the build's FP64 h solve) of one target at
+ * position t: root of f(h) = 0 above over all N sources, bracketed,
+ * |dh| <= tol*h.  r2 is scratch of N doubles.  Returns 1 on convergence. */
+static int h_newton_row(int64_t N, const double *pos, const double *t, double h0, double n_ngb, double tol,
+                        int64_t max_iter, double *r2, double *h_out)
+{
+    for (int64_t j = 0; j < N; j++) {
+        double dx = wrapd(t[0] - pos[3 * j]), dy = wrapd(t[1] - pos[3 * j + 1]), dz = wrapd(t[2] - pos[3 * j + 2]);
+        r2[j] = dx * dx + dy * dy + dz * dz;
+    }
+    double h = h0, lo = 0.0, hi = INFINITY, f, df;
+    int ok = 0;
+    for (int64_t it = 0; it < max_iter; it++) {
+        nr_f(r2, N, h, n_ngb, &f, &df);
+        if (f < 0.0) lo = h; else hi = h;
+        double hn = (df > 0.0) ? h - f / df : NAN;
+        if (!(hn > lo && hn < hi)) hn = isinf(hi) ? 2.0 * h : 0.5 * (lo + hi);
+        double dh = fabs(hn - h);
+        h = hn;
+        if (dh <= tol * h) { ok = 1; break; }
+    }
+    *h_out = h;
+    return ok;
+}
+
+/* Newton-Raphson smoothing length of every particle (brute-force distances);
+ * this is the checker of the device solve.  Returns number of rows that failed. */
 int64_t or_h_newton(int64_t N, const double *pos, const double *h_in, double n_ngb, double tol,
                     int64_t max_iter, double *h_out)
 {
@@ -871,26 +902,25 @@ int64_t or_h_newton(int64_t N, const double *pos, const double *h_in, double n_n
     {
         double *r2 = (double *)malloc(sizeof(double) * N);
 #pragma omp for schedule(dynamic, 8)
-        for (int64_t i = 0; i < N; i++) {
-            for (int64_t j = 0; j < N; j++) {
-                double dx = wrapd(pos[3 * i] - pos[3 * j]), dy = wrapd(pos[3 * i + 1] - pos[3 * j + 1]),
-                       dz = wrapd(pos[3 * i + 2] - pos[3 * j + 2]);
-                r2[j] = dx * dx + dy * dy + dz * dz;
-            }
-            double h = h_in[i], lo = 0.0, hi = INFINITY, f, df;
-            int ok = 0;
-            for (int64_t it = 0; it < max_iter; it++) {
-                nr_f(r2, N, h, n_ngb, &f, &df);
-                if (f < 0.0) lo = h; else hi = h;
-                double hn = (df > 0.0) ? h - f / df : NAN;
-                if (!(hn > lo && hn < hi)) hn = isinf(hi) ? 2.0 * h : 0.5 * (lo + hi);
-                double dh = fabs(hn - h);
-                h = hn;
-                if (dh <= tol * h) { ok = 1; break; }
-            }
-            if (!ok) bad++;
-            h_out[i] = h;
-        }
+        for (int64_t i = 0; i < N; i++)
+            bad += !h_newton_row(N, pos, pos + 3 * i, h_in[i], n_ngb, tol, max_iter, r2, h_out + i);
+        free(r2);
+    }
+    return bad;
+}
+
+/* The same solve for the M particles rows[0..M) only (sampled checks at large N);
+ * h_in and h_out have M entries. */
+int64_t or_h_newton_rows(int64_t N, const double *pos, int64_t M, const int64_t *rows, const double *h_in,
+                         double n_ngb, double tol, int64_t max_iter, double *h_out)
+{
+    int64_t bad = 0;
+#pragma omp parallel reduction(+ : bad)
+    {
+        double *r2 = (double *)malloc(sizeof(double) * N);
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t k = 0; k < M; k++)
+            bad += !h_newton_row(N, pos, pos + 3 * rows[k], h_in[k], n_ngb, tol, max_iter, r2, h_out + k);
         free(r2);
     }
     return bad;

@@ -18,6 +18,17 @@ SPH_OK, SPH_ERR_ARG, SPH_ERR_HIP, SPH_ERR_NOCONV, SPH_ERR_CAPACITY, SPH_ERR_STAT
 MLP_NPARAM = 33793
 PREC = {"mixed": 0, "fp64": 1}
 SOFT = {"spline": 0, "plummer": 1}
+# opening rule: "geometric" accepts a node when size < theta d; "reference" is the
+# reference walk's rule size^2 / theta <= d^2 (EV/bh.h:505-532), i.e. size < sqrt(theta) d
+MAC = ("geometric", "reference")
+
+
+def theta_of(theta, mac):
+    """Opening angle the device's geometric test gets for the rule ``mac``."""
+    if mac not in MAC:
+        raise ValueError(f"mac must be one of {MAC}")
+    theta = float(theta)
+    return theta ** 0.5 if mac == "reference" else theta
 H_NEWTON, H_REF_COUNT, H_KTH = 0, 1, 2
 
 PHASES = ["kick_drift", "order", "topology", "hsolve", "moments", "ngb", "density", "gravity", "divcurl",
@@ -36,7 +47,8 @@ class StepParams(ctypes.Structure):
                 ("G", ctypes.c_double), ("theta", ctypes.c_double), ("eps_plummer", ctypes.c_double),
                 ("n_ngb", ctypes.c_double), ("h_tol", ctypes.c_double), ("period", ctypes.c_double),
                 ("C_CFL", ctypes.c_double), ("soft_kind", ctypes.c_int),
-                ("precision", ctypes.c_int), ("balsara", ctypes.c_int), ("timing", ctypes.c_int)]
+                ("precision", ctypes.c_int), ("balsara", ctypes.c_int), ("timing", ctypes.c_int),
+                ("energy", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -45,7 +57,8 @@ class Stats(ctypes.Structure):
                 ("h_rounds", ctypes.c_int64), ("phase_ms", ctypes.c_double * 20),
                 ("phase_calls", ctypes.c_int64 * 20), ("h_tests", ctypes.c_int64), ("h_terms", ctypes.c_int64),
                 ("ngb_extra", ctypes.c_int64), ("cool_floor", ctypes.c_int64), ("cool_fallback", ctypes.c_int64),
-                ("h_big", ctypes.c_int64), ("dt_min", ctypes.c_double)]
+                ("h_big", ctypes.c_int64), ("dt_min", ctypes.c_double), ("energy", ctypes.c_double * 3),
+                ("momentum", ctypes.c_double * 3), ("ang_momentum", ctypes.c_double * 3)]
 
 
 class CoolParams(ctypes.Structure):
@@ -86,6 +99,9 @@ _SIGS = {
     "sph_gravity": ([_vp, _i64, _d, _d, _d, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                      ctypes.c_int, _d, _i64p], ctypes.c_int),
     "sph_kinetics": ([_vp, _i64, _d, _d, _d], ctypes.c_int),
+    "sph_potential_energy": ([_vp, _i64, _d, _d, _d, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                              ctypes.c_int, _d, _d], ctypes.c_int),
+    "sph_validate_particles": ([_i64, _d, _d, _d, _d, _d, ctypes.c_char_p, _i64], ctypes.c_int),
     "sph_cooling_rates": ([_vp, _i64, _d, _d, _d, ctypes.c_double, _d, ctypes.c_double, _d, _d], ctypes.c_int),
     "sph_cooling_from_u": ([_vp, _i64, ctypes.c_int, _d, _d, ctypes.c_double, _d, ctypes.c_double, _d,
                             ctypes.c_double, _d], ctypes.c_int),

@@ -53,7 +53,7 @@ def _prec(p):
         raise ValueError(f"precision must be one of {sorted(PREC)}") from None
 
 
-def column_density(pos, m, h, rho, params: ColumnParams, *, precision="mixed", device=0, with_grad=False):
+def column_density(pos, m, h, rho, params: ColumnParams, *, precision="fp64", device=0, with_grad=False):
     """log10 N_H [cm^-2] per particle (and |grad rho| in code units when
     ``with_grad``), caller order."""
     pos = f64(pos, (-1, 3)); n = len(pos)
@@ -67,7 +67,7 @@ def column_density(pos, m, h, rho, params: ColumnParams, *, precision="mixed", d
     return (lognh, grad) if with_grad else lognh
 
 
-def grad_rho_norm_ngb(pos, m, h, *, precision="mixed", device=0):
+def grad_rho_norm_ngb(pos, m, h, *, precision="fp64", device=0):
     """|grad rho|_i = |sum_j m_j grad W_ij| (code units), caller order -- the
     reference kernel of the same name."""
     pos = f64(pos, (-1, 3))

@@ -118,6 +118,9 @@ struct sph_ctx {
     bool col_on = false, col_ok = false;  // column density in the step; computed for the current state
     sph::ColumnParams col_p{};
     Buf colg, coln;  // |grad rho|, log10 N_H (sorted order, FP64)
+    int64_t n_eval = 0;     // particles of the last force evaluation (later injections are not in it)
+    Buf phi, tpart, tout;   // potential of the energy option, FP64 totals (partials, result)
+    bool tot_ran = false;   // the last force evaluation computed the totals
     bool ts_ok = false;     // time-step criteria of the last force evaluation
     int64_t ts_n = 0;       // particles they cover (later injections are not covered)
     Buf tsdt, tsdh;         // dt_i, dh/dt_i (sorted order, FP64)
@@ -172,6 +175,57 @@ int fail(sph_ctx *c, int code, const std::string &msg)
         int r_ = (__VA_ARGS__);                                                                      \
         if (r_ != SPH_OK) return r_;                                                                 \
     } while (0)
+
+// Host checks of a particle set (sph_validate_particles); false with the reason in why.
+bool particles_ok(int64_t n, const double *pos, const double *vel, const double *m, const double *u,
+                  const double *h, std::string &why)
+{
+    auto bad = [&](const char *what, int64_t i, double v) {
+        char b[160];
+        std::snprintf(b, sizeof(b), "%s of particle %lld is %.17g", what, (long long)i, v);
+        why = b;
+        return false;
+    };
+    if (n < 1) {
+        why = "n must be >= 1";
+        return false;
+    }
+    if (pos) {
+        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int64_t i = 0; i < n; i++)
+            for (int d = 0; d < 3; d++) {
+                const double v = pos[3 * i + d];
+                if (!std::isfinite(v)) return bad("a position coordinate", i, v);
+                lo[d] = std::min(lo[d], v);
+                hi[d] = std::max(hi[d], v);
+            }
+        if (n > 1 && !(hi[0] > lo[0] || hi[1] > lo[1] || hi[2] > lo[2])) {
+            why = "all particles coincide: the bounding box has zero extent, so there is no spatial order";
+            return false;
+        }
+    }
+    if (vel)
+        for (int64_t i = 0; i < 3 * n; i++)
+            if (!std::isfinite(vel[i])) return bad("a velocity component", i / 3, vel[i]);
+    if (m)
+        for (int64_t i = 0; i < n; i++)
+            if (!(m[i] >= 0.0) || !std::isfinite(m[i])) return bad("the mass (must be finite and >= 0)", i, m[i]);
+    if (u)
+        for (int64_t i = 0; i < n; i++)
+            if (!(u[i] >= 0.0) || !std::isfinite(u[i])) return bad("u (must be finite and >= 0)", i, u[i]);
+    if (h)
+        for (int64_t i = 0; i < n; i++)
+            if (!(h[i] > 0.0) || !std::isfinite(h[i])) return bad("h (must be finite and > 0)", i, h[i]);
+    return true;
+}
+
+int check_particles(sph_ctx *ctx, int64_t n, const double *pos, const double *vel, const double *m, const double *u,
+                    const double *h)
+{
+    std::string why;
+    if (!particles_ok(n, pos, vel, m, u, h, why)) return fail(ctx, SPH_ERR_ARG, why);
+    return SPH_OK;
+}
 
 int ensure(sph_ctx *ctx, Buf &b, size_t bytes)
 {
@@ -270,6 +324,7 @@ int set_pos(sph_ctx *ctx, int64_t n, const double *pos)
     if (ctx->order_ok && n == ctx->n && (int64_t)ctx->cpos.size() == 3 * n &&
         std::memcmp(ctx->cpos.data(), pos, sizeof(double) * 3 * n) == 0)
         return SPH_OK;
+    CK(check_particles(ctx, n, pos, nullptr, nullptr, nullptr, nullptr));
     invalidate_all(ctx);
     CK(capacity(ctx, n));
     ctx->n = n;
@@ -426,6 +481,7 @@ int ensure_csr(sph_ctx *ctx, bool csr = false)
 // Common prologue of the hydro entry points.
 int geom(sph_ctx *ctx, int64_t n, const double *pos, const double *h, const double *m)
 {
+    CK(check_particles(ctx, n, nullptr, nullptr, m, nullptr, h));
     CK(set_pos(ctx, n, pos));
     bool ch = false, cm = false;
     CK(put_scalar(ctx, h, ctx->h, &ctx->ch, &ctx->h_ok, &ch));
@@ -508,7 +564,7 @@ extern "C" int sph_ctx_destroy(sph_ctx *ctx)
                   &ctx->stage, &ctx->kpart, &ctx->kout, &ctx->spread, &ctx->ccidx, &ctx->cccnt,
                   &ctx->cext, &ctx->cnext, &ctx->ttab, &ctx->zmet, &ctx->j21, &ctx->ucols, &ctx->uyn,
                   &ctx->uL, &ctx->uG, &ctx->uflag, &ctx->ucnt, &ctx->hbig, &ctx->p4, &ctx->v4, &ctx->s4,
-                  &ctx->colg, &ctx->coln, &ctx->tsdt, &ctx->tsdh};
+                  &ctx->colg, &ctx->coln, &ctx->tsdt, &ctx->tsdh, &ctx->phi, &ctx->tpart, &ctx->tout};
     for (Buf *b : all)
         if (b->p) (void)hipFree(b->p);
     for (Buf &b : ctx->shadow)
@@ -619,6 +675,14 @@ extern "C" int sph_smoothing_length(sph_ctx *ctx, int64_t n, const double *pos, 
     if (mode == SPH_H_KTH && (int)param + 1 > SPH_HCAP)
         return fail(ctx, SPH_ERR_ARG, "kth + 1 exceeds the candidate capacity (256)");
     if (targets && m < 1) return fail(ctx, SPH_ERR_ARG, "m must be >= 1");
+    if (targets) {
+        std::string why;
+        for (int64_t i = 0; i < 3 * m; i++)
+            if (!std::isfinite(targets[i])) return fail(ctx, SPH_ERR_ARG, "a target coordinate is not finite");
+        if (h_in && !particles_ok(m, nullptr, nullptr, nullptr, nullptr, h_in, why)) return fail(ctx, SPH_ERR_ARG, why);
+    } else if (h_in) {
+        CK(check_particles(ctx, n, nullptr, nullptr, nullptr, nullptr, h_in));
+    }
     CK(set_pos(ctx, n, pos));
     CK(ensure_moments(ctx));
     hipStream_t st = ctx->st;
@@ -698,24 +762,26 @@ extern "C" int sph_neighbours(sph_ctx *ctx, int64_t n, const double *pos, const 
 }
 
 // Neighbour sets of the resident state (the last sph_prime / sph_step), in the caller's
-// numbering: the same output as sph_neighbours at the state's positions and h.
+// numbering: the same output as sph_neighbours at the state's positions and h.  Particles
+// injected after that evaluation (sorted slots >= n_eval) have no set yet: empty rows.
 extern "C" int sph_state_neighbours(sph_ctx *ctx, int64_t *offsets, int32_t *idx, int64_t idx_cap)
 {
     if (!ctx || !offsets) return SPH_ERR_ARG;
-    if (!ctx->resident || !ctx->primed) return fail(ctx, SPH_ERR_STATE, "no forces evaluated yet (sph_prime)");
+    if (!ctx->resident || !ctx->primed || ctx->n_eval < 1)
+        return fail(ctx, SPH_ERR_STATE, "no forces evaluated yet (sph_prime)");
     HIPCK(hipSetDevice(ctx->device));
-    const int64_t n = ctx->n, K = ctx->ngb_stride;
-    std::vector<int32_t> orig(n), scnt(n);
-    std::vector<int64_t> soff(n + 1);
+    const int64_t n = ctx->n, ne = ctx->n_eval, K = ctx->ngb_stride;
+    std::vector<int32_t> orig(n), scnt(n, 0);
+    std::vector<int64_t> soff(n + 1, 0);
     HIPCK(hipMemcpyAsync(orig.data(), ctx->orig.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->st));
     if (K > 0) {
-        HIPCK(hipMemcpyAsync(scnt.data(), ctx->cnt.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->st));
+        HIPCK(hipMemcpyAsync(scnt.data(), ctx->cnt.p, sizeof(int32_t) * ne, hipMemcpyDeviceToHost, ctx->st));
         HIPCK(hipStreamSynchronize(ctx->st));
-        for (int64_t s = 0; s < n; s++) soff[s] = s * K;
+        for (int64_t s = 0; s < ne; s++) soff[s] = s * K;
     } else {
-        HIPCK(hipMemcpyAsync(soff.data(), ctx->off.p, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, ctx->st));
+        HIPCK(hipMemcpyAsync(soff.data(), ctx->off.p, sizeof(int64_t) * (ne + 1), hipMemcpyDeviceToHost, ctx->st));
         HIPCK(hipStreamSynchronize(ctx->st));
-        for (int64_t s = 0; s < n; s++) scnt[s] = (int32_t)(soff[s + 1] - soff[s]);
+        for (int64_t s = 0; s < ne; s++) scnt[s] = (int32_t)(soff[s + 1] - soff[s]);
     }
     std::vector<int64_t> cnt(n);
     for (int64_t s = 0; s < n; s++) cnt[orig[s]] = scnt[s];
@@ -726,11 +792,11 @@ extern "C" int sph_state_neighbours(sph_ctx *ctx, int64_t *offsets, int32_t *idx
         return idx ? fail(ctx, SPH_ERR_CAPACITY, "idx capacity " + std::to_string(idx_cap) + " < " +
                                                      std::to_string(total))
                    : SPH_OK;
-    const int64_t span = K > 0 ? n * K : total;
+    const int64_t span = K > 0 ? ne * K : total;
     std::vector<int32_t> sidx(span);
     HIPCK(hipMemcpyAsync(sidx.data(), ctx->idx.p, sizeof(int32_t) * span, hipMemcpyDeviceToHost, ctx->st));
     HIPCK(hipStreamSynchronize(ctx->st));
-    for (int64_t s = 0; s < n; s++) {
+    for (int64_t s = 0; s < ne; s++) {
         int64_t r = orig[s], o = offsets[r];
         for (int64_t k = soff[s]; k < soff[s] + scnt[s]; k++) idx[o++] = orig[sidx[k]];
         std::sort(idx + offsets[r], idx + offsets[r + 1]);
@@ -825,6 +891,7 @@ extern "C" int sph_div_curl(sph_ctx *ctx, int64_t n, const double *pos, const do
 {
     CK(check_common(ctx, n, precision));
     if (!pos || !vel || !rho || !m || !h || !divV_out || !curlV_out) return fail(ctx, SPH_ERR_ARG, "null argument");
+    CK(check_particles(ctx, n, nullptr, vel, nullptr, nullptr, nullptr));
     CK(geom(ctx, n, pos, h, m));
     return precision == SPH_PREC_MIXED ? divcurl_impl<float>(ctx, vel, rho, divV_out, curlV_out)
                                        : divcurl_impl<double>(ctx, vel, rho, divV_out, curlV_out);
@@ -837,6 +904,7 @@ extern "C" int sph_acc_hydro(sph_ctx *ctx, int64_t n, const double *pos, const d
     CK(check_common(ctx, n, precision));
     if (!pos || !vel || !rho || !P || !c || !h || !m || !acc_out) return fail(ctx, SPH_ERR_ARG, "null argument");
     if ((divV == nullptr) != (curlV == nullptr)) return fail(ctx, SPH_ERR_ARG, "divV and curlV go together");
+    CK(check_particles(ctx, n, nullptr, vel, nullptr, nullptr, nullptr));
     CK(geom(ctx, n, pos, h, m));
     return precision == SPH_PREC_MIXED ? acc_impl<float>(ctx, vel, rho, P, c, divV, curlV, alpha, acc_out)
                                        : acc_impl<double>(ctx, vel, rho, P, c, divV, curlV, alpha, acc_out);
@@ -849,20 +917,30 @@ extern "C" int sph_dudt(sph_ctx *ctx, int64_t n, const double *pos, const double
     CK(check_common(ctx, n, precision));
     if (!pos || !vel || !rho || !P || !c || !h || !m || !dudt_out) return fail(ctx, SPH_ERR_ARG, "null argument");
     if ((divV == nullptr) != (curlV == nullptr)) return fail(ctx, SPH_ERR_ARG, "divV and curlV go together");
+    CK(check_particles(ctx, n, nullptr, vel, nullptr, nullptr, nullptr));
     CK(geom(ctx, n, pos, h, m));
     return precision == SPH_PREC_MIXED ? dudt_impl<float>(ctx, vel, rho, P, c, divV, curlV, alpha, dudt_out)
                                        : dudt_impl<double>(ctx, vel, rho, P, c, divV, curlV, alpha, dudt_out);
 }
 
 // =========================================================== gravity
-extern "C" int sph_gravity(sph_ctx *ctx, int64_t n, const double *pos, const double *m, const double *eps, double G,
-                           double theta, int soft_kind, double eps_plummer, int precision, double *acc_out,
-                           int64_t *n_interactions)
+namespace {
+// The tree walk of sph_gravity / sph_potential_energy up to its results in the sorted
+// arrays (gx, gy, gz; phi when pot).
+int gravity_pass(sph_ctx *ctx, int64_t n, const double *pos, const double *m, const double *eps, double G,
+                 double theta, int soft_kind, double eps_plummer, int precision, bool pot, int64_t *n_interactions)
 {
     CK(check_common(ctx, n, precision));
-    if (!pos || !m || !acc_out) return fail(ctx, SPH_ERR_ARG, "null argument");
+    if (!pos || !m) return fail(ctx, SPH_ERR_ARG, "null argument");
     if (soft_kind != SPH_SOFT_SPLINE && soft_kind != SPH_SOFT_PLUMMER) return fail(ctx, SPH_ERR_ARG, "bad soft_kind");
     if (!(theta >= 0.0)) return fail(ctx, SPH_ERR_ARG, "theta must be >= 0");
+    if (!std::isfinite(G) || !(eps_plummer >= 0.0) || !std::isfinite(eps_plummer))
+        return fail(ctx, SPH_ERR_ARG, "G and eps_plummer must be finite, eps_plummer >= 0");
+    CK(check_particles(ctx, n, nullptr, nullptr, m, nullptr, nullptr));
+    if (eps)
+        for (int64_t i = 0; i < n; i++)
+            if (!(eps[i] >= 0.0) || !std::isfinite(eps[i]))
+                return fail(ctx, SPH_ERR_ARG, "softening lengths must be finite and >= 0");
     CK(set_pos(ctx, n, pos));
     bool cm = false, ce = false;
     CK(put_scalar(ctx, m, ctx->m, &ctx->cm, &ctx->m_ok, &cm));
@@ -874,18 +952,20 @@ extern "C" int sph_gravity(sph_ctx *ctx, int64_t n, const double *pos, const dou
     }
     if (cm || ce) ctx->moments_ok = false;
     CK(ensure_moments(ctx));
+    if (pot) CK(ensure(ctx, ctx->phi, (size_t)n * 8));
     unsigned long long *cnt = P_<unsigned long long>(ctx->spread) + SPR_G * SPH_SPREAD_WORDS;
     HIPCK(sph::fill<unsigned long long>(ctx->st, 3 * SPH_SPREAD_WORDS, cnt, 0ull));
     const double *e = ctx->eps_ok ? D(eps) : nullptr;
     const int64_t ng = ctx->ngroups;
+    double *phi = pot ? D(phi) : nullptr;
     if (precision == SPH_PREC_MIXED)
         HIPCK(sph::gravity<float>(ctx->st, ng, I32(groups), D(x), D(y), D(z), D(m), e, P_<WNode>(ctx->nodes),
                                   SPH_BUCKET, theta, G, soft_kind, eps_plummer, ctx->m_max, D(gx), D(gy), D(gz),
-                                  cnt));
+                                  cnt, phi));
     else
         HIPCK(sph::gravity<double>(ctx->st, ng, I32(groups), D(x), D(y), D(z), D(m), e, P_<WNode>(ctx->nodes),
                                    SPH_BUCKET, theta, G, soft_kind, eps_plummer, ctx->m_max, D(gx), D(gy), D(gz),
-                                   cnt));
+                                   cnt, phi));
     unsigned long long hc[3];
     HIPCK(sph::spread_total(ctx->st, 3, cnt, P_<unsigned long long>(ctx->hcount) + 5));
     HIPCK(hipMemcpyAsync(hc, P_<unsigned long long>(ctx->hcount) + 5, sizeof(hc), hipMemcpyDeviceToHost, ctx->st));
@@ -895,7 +975,48 @@ extern "C" int sph_gravity(sph_ctx *ctx, int64_t n, const double *pos, const dou
         n_interactions[0] = (int64_t)hc[0];
         n_interactions[1] = (int64_t)hc[1];
     }
+    return SPH_OK;
+}
+}  // namespace
+
+extern "C" int sph_gravity(sph_ctx *ctx, int64_t n, const double *pos, const double *m, const double *eps, double G,
+                           double theta, int soft_kind, double eps_plummer, int precision, double *acc_out,
+                           int64_t *n_interactions)
+{
+    if (ctx && !acc_out) return fail(ctx, SPH_ERR_ARG, "null argument");
+    CK(gravity_pass(ctx, n, pos, m, eps, G, theta, soft_kind, eps_plummer, precision, false, n_interactions));
     return get_vec3(ctx, ctx->gx, ctx->gy, ctx->gz, acc_out);
+}
+
+extern "C" int sph_potential_energy(sph_ctx *ctx, int64_t n, const double *pos, const double *m, const double *eps,
+                                    double G, double theta, int soft_kind, double eps_plummer, int precision,
+                                    double *pe_out, double *phi_out)
+{
+    if (ctx && !pe_out) return fail(ctx, SPH_ERR_ARG, "null argument");
+    CK(gravity_pass(ctx, n, pos, m, eps, G, theta, soft_kind, eps_plummer, precision, true, nullptr));
+    CK(ensure(ctx, ctx->tpart, 10 * 1024 * 8));
+    CK(ensure(ctx, ctx->tout, 16 * 8));
+    HIPCK(sph::totals(ctx->st, n, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, D(m), nullptr, D(phi), D(tpart),
+                      D(tout)));
+    double t[10];
+    HIPCK(hipMemcpyAsync(t, ctx->tout.p, sizeof(t), hipMemcpyDeviceToHost, ctx->st));
+    HIPCK(hipStreamSynchronize(ctx->st));
+    *pe_out = t[2];
+    if (phi_out) CK(get_scalar<double>(ctx, ctx->phi, phi_out));
+    return SPH_OK;
+}
+
+extern "C" int sph_validate_particles(int64_t n, const double *pos, const double *vel, const double *m,
+                                      const double *u, const double *h, char *msg, int64_t msg_cap)
+{
+    std::string why;
+    const bool ok = particles_ok(n, pos, vel, m, u, h, why);
+    if (msg && msg_cap > 0) {
+        const size_t k = std::min((size_t)(msg_cap - 1), why.size());
+        std::memcpy(msg, why.data(), k);
+        msg[k] = 0;
+    }
+    return ok ? SPH_OK : SPH_ERR_ARG;
 }
 
 extern "C" int sph_kinetics(sph_ctx *ctx, int64_t n, const double *vel, const double *m, double *out5)
@@ -926,10 +1047,13 @@ extern "C" int sph_state_set(sph_ctx *ctx, int64_t n, const double *pos, const d
 {
     CK(check_common(ctx, n, SPH_PREC_FP64));
     if (!pos || !vel || !m || !u || !h) return fail(ctx, SPH_ERR_ARG, "null argument");
+    CK(check_particles(ctx, n, pos, vel, m, u, h));
     ctx->resident = false;
     ctx->cool_on = false;  // Z and J21 belong to the previous state
     ctx->col_on = ctx->col_ok = false;
     ctx->ts_ok = false;
+    ctx->tot_ran = false;
+    ctx->n_eval = 0;
     ctx->agn_on = false;
     ctx->agn_leftover = 0.0;
     ctx->agn_injected = 0;
@@ -964,9 +1088,16 @@ extern "C" int sph_state_get(sph_ctx *ctx, double *pos, double *vel, double *u, 
     if (h) CK(get_scalar<double>(ctx, ctx->h, h));
     if (acc) CK(get_vec3(ctx, ctx->ax, ctx->ay, ctx->az, acc));
     if (rho) {
-        if (!ctx->primed) return fail(ctx, SPH_ERR_STATE, "rho is available after sph_prime/sph_step");
-        if (ctx->rho_prec == SPH_PREC_FP64) CK(get_scalar<double>(ctx, ctx->rho, rho));
-        else CK(get_scalar<float>(ctx, ctx->rho, rho));
+        if (!ctx->primed || ctx->n_eval < 1)
+            return fail(ctx, SPH_ERR_STATE, "rho is available after sph_prime/sph_step");
+        const int64_t ne = ctx->n_eval, extra = ctx->n - ne;  // injected after the evaluation: NaN
+        if (ctx->rho_prec == SPH_PREC_FP64) {
+            if (extra > 0) HIPCK(sph::fill<double>(ctx->st, extra, D(rho) + ne, NAN));
+            CK(get_scalar<double>(ctx, ctx->rho, rho));
+        } else {
+            if (extra > 0) HIPCK(sph::fill<float>(ctx->st, extra, P_<float>(ctx->rho) + ne, NAN));
+            CK(get_scalar<float>(ctx, ctx->rho, rho));
+        }
     }
     return SPH_OK;
 }
@@ -1066,9 +1197,12 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     ph_begin(ctx, SPH_PH_GRAVITY);
     unsigned long long *cnt = P_<unsigned long long>(ctx->spread) + SPR_G * SPH_SPREAD_WORDS;
     HIPCK(sph::fill<unsigned long long>(st, 3 * SPH_SPREAD_WORDS, cnt, 0ull));
+    const bool pot = p->energy && p->G != 0.0;  // the energy option: potential from the same walk
+    if (pot) CK(ensure(ctx, ctx->phi, (size_t)std::max(n, ctx->n_reserve) * 8));
     if (p->G != 0.0) {
         HIPCK(sph::gravity<R>(st, ctx->ngroups, I32(groups), D(x), D(y), D(z), D(m), eps, P_<WNode>(ctx->nodes), SPH_BUCKET,
-                              p->theta, p->G, p->soft_kind, p->eps_plummer, ctx->m_max, D(gx), D(gy), D(gz), cnt));
+                              p->theta, p->G, p->soft_kind, p->eps_plummer, ctx->m_max, D(gx), D(gy), D(gz), cnt,
+                              pot ? D(phi) : nullptr));
     }
     HIPCK(sph::spread_total(st, 3, cnt, P_<unsigned long long>(ctx->hcount) + 5));
     ph_end(ctx, SPH_PH_GRAVITY);
@@ -1118,7 +1252,18 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
             HIPCK(sph::density<R>(st, n, D(u), p->gamma, rows, P_<R>(ctx->rho), P_<R>(ctx->P), P_<R>(ctx->c),
                                   P_<R>(ctx->prho2), P_<R4<R>>(ctx->s4)));
         ph_end(ctx, SPH_PH_DENSITY);
+        if (!p->balsara && p->C_CFL > 0.0) {
+            // div v of the time-step criteria from the velocities of this force evaluation (before the
+            // second kick), as the Balsara form's density pass computes it and as the reference's
+            // div_curlVel_ngb runs in the force pass (MC/drv.cu:1271) before dt_array_indiv_dt (:1610)
+            ph_begin(ctx, SPH_PH_DIVCURL);
+            HIPCK(sph::divcurl<R>(st, n, P_<R>(ctx->rho), P_<R>(ctx->c), rows, P_<R>(ctx->divV), P_<R>(ctx->curlV),
+                                  nullptr));
+            ph_end(ctx, SPH_PH_DIVCURL);
+        }
         if (ctx->col_on) {  // column density from this density pass (gradient over the same rows)
+            CK(ensure(ctx, ctx->colg, (size_t)std::max(n, ctx->n_reserve) * 8));
+            CK(ensure(ctx, ctx->coln, (size_t)std::max(n, ctx->n_reserve) * 8));
             ph_begin(ctx, SPH_PH_COLUMN);
             HIPCK(sph::column_density<R>(st, n, P_<R>(ctx->rho), rows, ctx->col_p, D(colg), D(coln)));
             ph_end(ctx, SPH_PH_COLUMN);
@@ -1175,9 +1320,6 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     // per-particle time-step criteria of this evaluation (velocities after the kick)
     ctx->ts_ok = false;
     if (p->C_CFL > 0.0) {
-        if (!bal)  // the libsx forms skip div v: the Courant criterion needs it
-            HIPCK(sph::divcurl<R>(st, n, P_<R>(ctx->rho), P_<R>(ctx->c), rows, P_<R>(ctx->divV), P_<R>(ctx->curlV),
-                                  nullptr));
         const size_t cap = (size_t)std::max(n, ctx->n_reserve) * 8;
         CK(ensure(ctx, ctx->tsdt, cap));
         CK(ensure(ctx, ctx->tsdh, cap));
@@ -1196,6 +1338,15 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
         ctx->ts_ok = true;
         ctx->ts_n = n;
     }
+    // FP64 totals of the state at the end of the evaluation (energy option)
+    ctx->tot_ran = p->energy != 0;
+    if (ctx->tot_ran) {
+        CK(ensure(ctx, ctx->tpart, 10 * 1024 * 8));
+        CK(ensure(ctx, ctx->tout, 16 * 8));
+        HIPCK(sph::totals(st, n, D(x), D(y), D(z), D(vx), D(vy), D(vz), D(m), D(u), pot ? D(phi) : nullptr, D(tpart),
+                          D(tout)));
+    }
+    ctx->n_eval = n;
     return SPH_OK;
 }
 
@@ -1211,6 +1362,11 @@ int check_params(sph_ctx *ctx, const sph_step_params *p)
     if (!(p->C_CFL >= 0.0)) return fail(ctx, SPH_ERR_ARG, "C_CFL must be >= 0");
     if (p->period > 0.0 && p->G != 0.0)
         return fail(ctx, SPH_ERR_ARG, "a periodic box needs G = 0 (periodic gravity is not provided)");
+    // (4 pi/3)(2h)^3 sum_j W(r_ij, h) tends to (32/3) n from below as h grows: no h reaches more
+    if (!(p->n_ngb < (32.0 / 3.0) * (double)ctx->n))
+        return fail(ctx, SPH_ERR_ARG, "n_ngb = " + std::to_string(p->n_ngb) + " cannot be reached with " +
+                                          std::to_string(ctx->n) + " particles (needs n_ngb < 32 n / 3)");
+    if (p->energy != 0 && p->energy != 1) return fail(ctx, SPH_ERR_ARG, "energy must be 0 or 1");
     HIPCK(hipSetDevice(ctx->device));
     return SPH_OK;
 }
@@ -1277,6 +1433,15 @@ int finish_step(sph_ctx *ctx, const sph_step_params *p)
     }
     ctx->ph_rec = 0;
     ctx->rho_prec = p->precision;
+    if (ctx->tot_ran) {
+        double t[10];
+        HIPCK(hipMemcpy(t, ctx->tout.p, sizeof(t), hipMemcpyDeviceToHost));
+        for (int k = 0; k < 3; k++) {
+            ctx->stats.energy[k] = t[k];
+            ctx->stats.momentum[k] = t[3 + k];
+            ctx->stats.ang_momentum[k] = t[6 + k];
+        }
+    }
     ctx->stats.dt_min = 0.0;
     if (ctx->ts_ok) {
         unsigned long long bits = 0;
@@ -2166,8 +2331,8 @@ extern "C" int sph_state_set_column(sph_ctx *ctx, const sph_column_params *cp)
         return SPH_OK;
     }
     CK(column_check(ctx, cp));
-    CK(ensure(ctx, ctx->colg, (size_t)ctx->n * 8));
-    CK(ensure(ctx, ctx->coln, (size_t)ctx->n * 8));
+    CK(ensure(ctx, ctx->colg, (size_t)std::max(ctx->n, ctx->n_reserve) * 8));
+    CK(ensure(ctx, ctx->coln, (size_t)std::max(ctx->n, ctx->n_reserve) * 8));
     ctx->col_p = column_params(cp);
     ctx->col_on = true;
     return SPH_OK;
@@ -2180,6 +2345,10 @@ extern "C" int sph_state_get_column(sph_ctx *ctx, double *grad_out, double *log_
     if (!ctx->col_ok)
         return fail(ctx, SPH_ERR_STATE, "no column density yet (sph_state_set_column, then sph_prime or sph_step)");
     HIPCK(hipSetDevice(ctx->device));
+    if (ctx->n > ctx->n_eval) {  // particles injected after the evaluation
+        HIPCK(sph::fill<double>(ctx->st, ctx->n - ctx->n_eval, D(colg) + ctx->n_eval, NAN));
+        HIPCK(sph::fill<double>(ctx->st, ctx->n - ctx->n_eval, D(coln) + ctx->n_eval, NAN));
+    }
     if (grad_out) CK(get_scalar<double>(ctx, ctx->colg, grad_out));
     if (log_nh_out) CK(get_scalar<double>(ctx, ctx->coln, log_nh_out));
     return SPH_OK;

@@ -140,6 +140,26 @@ __device__ __forceinline__ R grav_fk_spline(R rr, R e)
     return inv_r3;
 }
 
+// Spline-softened potential per unit source mass, psi(r, e) with phi_i = G sum_j m_j psi(r_ij, e_ij)
+// and PE = -1/2 sum_i m_i phi_i: sph_code/libsx.py:597-613 (getPE's fk / m_j).  Coincident pairs
+// (q = 0) add nothing there, and here.
+template <typename R>
+__device__ __forceinline__ R grav_pot_spline(R rr, R e)
+{
+    R q = rr / e;
+    if (!(rr > (R)0)) return (R)0;
+    if (q <= (R)1) {
+        R q2 = q * q;
+        return ((R)-2 / e) * ((R)(1.0 / 3.0) * q2 - (R)0.15 * q2 * q2 + (R)0.05 * q2 * q2 * q) + (R)1.4 / e;
+    }
+    if (q <= (R)2) {
+        R q2 = q * q, q3 = q2 * q;
+        return (R)(-1.0 / 15.0) / rr -
+               ((R)1 / e) * ((R)(4.0 / 3.0) * q2 - q3 + (R)0.3 * q3 * q - (R)(1.0 / 30.0) * q3 * q2) + (R)1.6 / e;
+    }
+    return (R)1 / rr;
+}
+
 template <typename R> __device__ __forceinline__ R sph_sqrt(R x);
 // FP32: the hardware square root (v_sqrt_f32, 1 ulp) without the correctly rounded expansion
 template <> __device__ __forceinline__ float sph_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }

@@ -19,10 +19,12 @@ hipError_t gravity_groups(hipStream_t st, int64_t n, const int32_t *ncount, cons
 // overflows
 // (must stay 0).  m_max: an upper bound of the particle masses (INFINITY if unknown), which
 // decides whether a Plummer softening > 0 keeps each particle's weight on itself finite.
+// phi (may be null): also the potential phi_i = G sum_j m_j psi(r_ij) over the same list
+// (psi = 1/sqrt(r^2 + eps^2) or the spline potential of getPE; coincident pairs excluded).
 template <typename R>
 hipError_t gravity(hipStream_t st, int64_t ngroups, const int32_t *groups, const double *x, const double *y,
                    const double *z, const double *m, const double *eps, const WNode *nodes, int bucket, double theta,
                    double G, int soft_kind, double eps_plummer, double m_max, double *ax, double *ay, double *az,
-                   unsigned long long *counters);
+                   unsigned long long *counters, double *phi = nullptr);
 
 }  // namespace sph

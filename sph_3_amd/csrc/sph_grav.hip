@@ -61,9 +61,11 @@ __device__ __forceinline__ void split(double d, R &hi, R &lo)
     lo = (R)(d - (double)hi);
 }
 
-template <typename R, int SOFT, bool LO>
+// One source against the lane's target.  POT: also the potential sum sp += m_j psi(r, e_ij)
+// (Plummer: m_j / sqrt(r^2 + eps^2); spline: grav_pot_spline), coincident pairs excluded.
+template <typename R, int SOFT, bool LO, bool POT>
 __device__ __forceinline__ void interact(const Src4<R> &a, const Src4<R> &b, R tx, R ty, R tz, R tlx, R tly, R tlz,
-                                         R epsi, R e2, R &sx, R &sy, R &sz)
+                                         R epsi, R e2, R &sx, R &sy, R &sz, R &sp)
 {
     constexpr bool SPLIT = LO && sizeof(R) == 4;
     R dx = a.x - tx, dy = a.y - ty, dz = a.z - tz;
@@ -77,9 +79,15 @@ __device__ __forceinline__ void interact(const Src4<R> &a, const Src4<R> &b, R t
         r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, e2)));  // r^2 + eps^2 (e2 = 0 for SOFT == 1)
         R inv = sph_rsqrt<R>(r2);
         w = a.w * (inv * inv * inv);
+        if (POT) {
+            const R d2 = dx * dx + dy * dy + dz * dz;
+            sp = fma(a.w, d2 > (R)0 ? inv : (R)0, sp);
+        }
     } else {
         r2 = dx * dx + dy * dy + dz * dz;
-        w = a.w * grav_fk_spline<R>(sph_sqrt<R>(r2), (R)0.5 * (epsi + b.w));
+        const R rr = sph_sqrt<R>(r2), e = (R)0.5 * (epsi + b.w);
+        w = a.w * grav_fk_spline<R>(rr, e);
+        if (POT) sp = fma(a.w, grav_pot_spline<R>(rr, e), sp);
     }
     // the target itself (and exact coincidences) add nothing: with a positive Plummer
     // softening (SOFT == 2) w stays finite and w * 0 = 0 already
@@ -93,20 +101,21 @@ __device__ __forceinline__ void interact(const Src4<R> &a, const Src4<R> &b, R t
 // independent chains of plain sums (ILP); each chain's partial over this flush
 // (<= GSRC/2 terms) is then added to its Kahan accumulator, so the list total
 // is compensated at the granularity of one flush.
-template <typename R, int SOFT, bool LO>
+template <typename R, int SOFT, bool LO, bool POT>
 __device__ __forceinline__ void eval_run(int cnt, const Src4<R> *__restrict__ A, const Src4<R> *__restrict__ B,
                                          R tx, R ty, R tz, R tlx, R tly, R tlz, R epsi, R e2, Kahan<R> *k)
 {
-    R s[6] = {(R)0, (R)0, (R)0, (R)0, (R)0, (R)0};
+    constexpr int NK = POT ? 8 : 6;
+    R s[8] = {(R)0, (R)0, (R)0, (R)0, (R)0, (R)0, (R)0, (R)0};
     int q = 0;
     for (; q + 1 < cnt; q += 2) {
         const Src4<R> a0 = A[q], b0 = B[q], a1 = A[q + 1], b1 = B[q + 1];
-        interact<R, SOFT, LO>(a0, b0, tx, ty, tz, tlx, tly, tlz, epsi, e2, s[0], s[1], s[2]);
-        interact<R, SOFT, LO>(a1, b1, tx, ty, tz, tlx, tly, tlz, epsi, e2, s[3], s[4], s[5]);
+        interact<R, SOFT, LO, POT>(a0, b0, tx, ty, tz, tlx, tly, tlz, epsi, e2, s[0], s[1], s[2], s[6]);
+        interact<R, SOFT, LO, POT>(a1, b1, tx, ty, tz, tlx, tly, tlz, epsi, e2, s[3], s[4], s[5], s[7]);
     }
-    if (q < cnt) interact<R, SOFT, LO>(A[q], B[q], tx, ty, tz, tlx, tly, tlz, epsi, e2, s[0], s[1], s[2]);
+    if (q < cnt) interact<R, SOFT, LO, POT>(A[q], B[q], tx, ty, tz, tlx, tly, tlz, epsi, e2, s[0], s[1], s[2], s[6]);
 #pragma unroll
-    for (int c = 0; c < 6; c++) k[c].add(s[c]);
+    for (int c = 0; c < NK; c++) k[c].add(s[c]);
 }
 
 // The staging buffer holds near sources at [0, nn) and far ones at
@@ -115,23 +124,26 @@ __device__ __forceinline__ void eval_run(int cnt, const Src4<R> *__restrict__ A,
 // round the source by <= 2^-24 (|d| + D/2) and the target by <= 2^-24 D/2 per
 // axis, so a far offset (|d| >= D/4) keeps a relative error <= 5 * 2^-24
 // without the lo parts; near ones need them.
-template <typename R, int SOFT>
+template <typename R, int SOFT, bool POT>
 __device__ __forceinline__ void eval_sources(int nn, int nfar, const Src4<R> *__restrict__ A,
                                              const Src4<R> *__restrict__ B, R tx, R ty, R tz, R tlx, R tly, R tlz,
                                              R epsi, R e2, Kahan<R> *k)
 {
-    if (nn > 0) eval_run<R, SOFT, true>(nn, A, B, tx, ty, tz, tlx, tly, tlz, epsi, e2, k);
+    if (nn > 0) eval_run<R, SOFT, true, POT>(nn, A, B, tx, ty, tz, tlx, tly, tlz, epsi, e2, k);
     if (nfar > 0)
-        eval_run<R, SOFT, false>(nfar, A + (GSRC - nfar), B + (GSRC - nfar), tx, ty, tz, tlx, tly, tlz, epsi, e2, k);
+        eval_run<R, SOFT, false, POT>(nfar, A + (GSRC - nfar), B + (GSRC - nfar), tx, ty, tz, tlx, tly, tlz, epsi,
+                                      e2, k);
 }
 
-template <typename R, int SOFT>
+// POT: phi[i] = G sum_j m_j psi(r_ij) over the same interaction list as the force.
+template <typename R, int SOFT, bool POT>
 __global__ void __launch_bounds__(GG) k_gravity_group(const int32_t *__restrict__ groups, const double *__restrict__ x,
                                                       const double *__restrict__ y, const double *__restrict__ z,
                                                       const double *__restrict__ m, const double *__restrict__ eps,
                                                       const WNode *__restrict__ nodes, int bucket, double theta2,
                                                       double G, double eps_plummer, double *__restrict__ ax,
                                                       double *__restrict__ ay, double *__restrict__ az,
+                                                      double *__restrict__ phi,
                                                       unsigned long long *__restrict__ counters)
 {
     __shared__ int stk[GSTK];
@@ -159,7 +171,7 @@ __global__ void __launch_bounds__(GG) k_gravity_group(const int32_t *__restrict_
     split<R>(zi - gcz, tz, tlz);
     const R te = (R)epsi;
     const R e2 = (R)(eps_plummer * eps_plummer);
-    Kahan<R> k[6];
+    Kahan<R> k[POT ? 8 : 6];
     long long n_nodes = 0, n_parts = 0;
     int nn = 0, nfar = 0;  // staged near sources [0, nn), far ones [GSRC - nfar, GSRC)
     int sp = 1;
@@ -231,7 +243,7 @@ __global__ void __launch_bounds__(GG) k_gravity_group(const int32_t *__restrict_
         if (lane == 0) bstart[nb] = total;
         __syncthreads();
         if (nn + nfar >= GG) {
-            eval_sources<R, SOFT>(nn, nfar, A, B, tx, ty, tz, tlx, tly, tlz, te, e2, k);
+            eval_sources<R, SOFT, POT>(nn, nfar, A, B, tx, ty, tz, tlx, tly, tlz, te, e2, k);
             nn = nfar = 0;
             __syncthreads();
         }
@@ -268,17 +280,18 @@ __global__ void __launch_bounds__(GG) k_gravity_group(const int32_t *__restrict_
             nfar += (int)__popcll(mf);
             __syncthreads();
             if (nn + nfar >= GG) {
-                eval_sources<R, SOFT>(nn, nfar, A, B, tx, ty, tz, tlx, tly, tlz, te, e2, k);
+                eval_sources<R, SOFT, POT>(nn, nfar, A, B, tx, ty, tz, tlx, tly, tlz, te, e2, k);
                 nn = nfar = 0;
                 __syncthreads();
             }
         }
     }
-    if (nn + nfar > 0) eval_sources<R, SOFT>(nn, nfar, A, B, tx, ty, tz, tlx, tly, tlz, te, e2, k);
+    if (nn + nfar > 0) eval_sources<R, SOFT, POT>(nn, nfar, A, B, tx, ty, tz, tlx, tly, tlz, te, e2, k);
     if (valid) {
         ax[i] = G * ((double)k[0].value() + (double)k[3].value());
         ay[i] = G * ((double)k[1].value() + (double)k[4].value());
         az[i] = G * ((double)k[2].value() + (double)k[5].value());
+        if constexpr (POT) phi[i] = G * ((double)k[6].value() + (double)k[7].value());
     }
     if (lane == 0) {
         spread_add(counters, 0, (unsigned long long)(n_parts * ng - ng));
@@ -332,11 +345,28 @@ hipError_t gravity_groups(hipStream_t st, int64_t n, const int32_t *ncount, cons
     return e;
 }
 
+template <typename R, bool POT>
+static void launch_gravity(hipStream_t st, int soft, int64_t ngroups, const int32_t *groups, const double *x,
+                           const double *y, const double *z, const double *m, const double *eps, const WNode *nodes,
+                           int bucket, double theta2, double G, double eps_plummer, double *ax, double *ay, double *az,
+                           double *phi, unsigned long long *counters)
+{
+    if (soft == 2)
+        k_gravity_group<R, 2, POT><<<(unsigned)ngroups, GG, 0, st>>>(groups, x, y, z, m, eps, nodes, bucket, theta2, G,
+                                                                    eps_plummer, ax, ay, az, phi, counters);
+    else if (soft == 0)
+        k_gravity_group<R, 0, POT><<<(unsigned)ngroups, GG, 0, st>>>(groups, x, y, z, m, eps, nodes, bucket, theta2, G,
+                                                                    eps_plummer, ax, ay, az, phi, counters);
+    else
+        k_gravity_group<R, 1, POT><<<(unsigned)ngroups, GG, 0, st>>>(groups, x, y, z, m, eps, nodes, bucket, theta2, G,
+                                                                    eps_plummer, ax, ay, az, phi, counters);
+}
+
 template <typename R>
 hipError_t gravity(hipStream_t st, int64_t ngroups, const int32_t *groups, const double *x, const double *y,
                    const double *z, const double *m, const double *eps, const WNode *nodes, int bucket, double theta,
                    double G, int soft_kind, double eps_plummer, double m_max, double *ax, double *ay, double *az,
-                   unsigned long long *counters)
+                   unsigned long long *counters, double *phi)
 {
     if (bucket > GG) return hipErrorInvalidValue;  // a bucket is expanded by one lane per particle
     if (ngroups <= 0) return hipSuccess;
@@ -346,23 +376,22 @@ hipError_t gravity(hipStream_t st, int64_t ngroups, const int32_t *groups, const
     const double wlim = sizeof(R) == 4 ? 1e30 : 1e300;
     const double e3 = eps_plummer * eps_plummer * eps_plummer;
     const bool self_zero = eps_plummer > 0.0 && (R)(eps_plummer * eps_plummer) > (R)0 && m_max < wlim * e3;
-    if (soft_kind == 1 && self_zero)
-        k_gravity_group<R, 2><<<(unsigned)ngroups, GG, 0, st>>>(groups, x, y, z, m, eps, nodes, bucket, theta2, G,
-                                                               eps_plummer, ax, ay, az, counters);
-    else if (soft_kind == 0)
-        k_gravity_group<R, 0><<<(unsigned)ngroups, GG, 0, st>>>(groups, x, y, z, m, eps, nodes, bucket, theta2, G,
-                                                               eps_plummer, ax, ay, az, counters);
+    const int soft = (soft_kind == 1 && self_zero) ? 2 : soft_kind == 0 ? 0 : 1;
+    if (phi)
+        launch_gravity<R, true>(st, soft, ngroups, groups, x, y, z, m, eps, nodes, bucket, theta2, G, eps_plummer, ax,
+                                ay, az, phi, counters);
     else
-        k_gravity_group<R, 1><<<(unsigned)ngroups, GG, 0, st>>>(groups, x, y, z, m, eps, nodes, bucket, theta2, G,
-                                                               eps_plummer, ax, ay, az, counters);
+        launch_gravity<R, false>(st, soft, ngroups, groups, x, y, z, m, eps, nodes, bucket, theta2, G, eps_plummer,
+                                 ax, ay, az, phi, counters);
     return hipGetLastError();
 }
 
 template hipError_t gravity<float>(hipStream_t, int64_t, const int32_t *, const double *, const double *,
                                    const double *, const double *, const double *, const WNode *, int, double, double,
-                                   int, double, double, double *, double *, double *, unsigned long long *);
+                                   int, double, double, double *, double *, double *, unsigned long long *, double *);
 template hipError_t gravity<double>(hipStream_t, int64_t, const int32_t *, const double *, const double *,
                                     const double *, const double *, const double *, const WNode *, int, double,
-                                    double, int, double, double, double *, double *, double *, unsigned long long *);
+                                    double, int, double, double, double *, double *, double *, unsigned long long *,
+                                    double *);
 
 }  // namespace sph

@@ -15,6 +15,12 @@ hipError_t u_update(hipStream_t st, int64_t n, double *u, const R *du, double *d
 template <typename R>
 hipError_t prep(hipStream_t st, int64_t n, const R *rho, const R *P, const R *c, const R *divV, const R *curlV,
                 const double *h, R *prho2, R *bal);
+// out10 = [KE, U, PE, px, py, pz, Lx, Ly, Lz, M] in FP64 with KE = sum m v^2/2, U = sum m u,
+// PE = -1/2 sum m phi, p = sum m v, L = sum m x cross v; null arrays count as zeros;
+// part: 10*1024 doubles.
+hipError_t totals(hipStream_t st, int64_t n, const double *x, const double *y, const double *z, const double *vx,
+                  const double *vy, const double *vz, const double *m, const double *u, const double *phi,
+                  double *part, double *out10);
 // out5 = [KE, px, py, pz, M]; part: 5*1024 doubles.
 hipError_t kinetics(hipStream_t st, int64_t n, const double *vx, const double *vy, const double *vz,
                     const double *m, double *part, double *out5);

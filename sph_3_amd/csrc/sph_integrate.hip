@@ -103,7 +103,79 @@ __global__ void __launch_bounds__(256) k_kin_final(int nb, const double *__restr
     if (threadIdx.x < 5) out[threadIdx.x] = s[threadIdx.x][0];
 }
 
+// Conserved totals of the state, FP64, one pass: [KE, U, PE, px, py, pz, Lx, Ly, Lz, M] per
+// block (KE = sum m v^2 / 2, U = sum m u, PE = -1/2 sum m phi, p = sum m v, L = sum m x cross v);
+// a null array contributes zeros.  Folded by one block in block order (deterministic).
+#define SPH_NTOT 10
+__global__ void __launch_bounds__(256) k_tot_partial(int64_t n, const double *__restrict__ x,
+                                                     const double *__restrict__ y, const double *__restrict__ z,
+                                                     const double *__restrict__ vx, const double *__restrict__ vy,
+                                                     const double *__restrict__ vz, const double *__restrict__ m,
+                                                     const double *__restrict__ u, const double *__restrict__ phi,
+                                                     double *__restrict__ part)
+{
+    __shared__ double s[SPH_NTOT][256];
+    double t[SPH_NTOT];
+#pragma unroll
+    for (int d = 0; d < SPH_NTOT; d++) t[d] = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double mm = m[i];
+        const double a = vx ? vx[i] : 0.0, b = vy ? vy[i] : 0.0, c = vz ? vz[i] : 0.0;
+        const double px = x ? x[i] : 0.0, py = y ? y[i] : 0.0, pz = z ? z[i] : 0.0;
+        t[0] += 0.5 * mm * (a * a + b * b + c * c);
+        if (u) t[1] += mm * u[i];
+        if (phi) t[2] -= 0.5 * mm * phi[i];
+        t[3] += mm * a;
+        t[4] += mm * b;
+        t[5] += mm * c;
+        t[6] += mm * (py * c - pz * b);
+        t[7] += mm * (pz * a - px * c);
+        t[8] += mm * (px * b - py * a);
+        t[9] += mm;
+    }
+#pragma unroll
+    for (int d = 0; d < SPH_NTOT; d++) s[d][threadIdx.x] = t[d];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < (unsigned)off)
+            for (int d = 0; d < SPH_NTOT; d++) s[d][threadIdx.x] += s[d][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x < SPH_NTOT) part[blockIdx.x * SPH_NTOT + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ void __launch_bounds__(256) k_tot_final(int nb, const double *__restrict__ part, double *__restrict__ out)
+{
+    __shared__ double s[SPH_NTOT][256];
+    double t[SPH_NTOT];
+#pragma unroll
+    for (int d = 0; d < SPH_NTOT; d++) t[d] = 0.0;
+    for (int p = threadIdx.x; p < nb; p += blockDim.x)
+        for (int d = 0; d < SPH_NTOT; d++) t[d] += part[p * SPH_NTOT + d];
+#pragma unroll
+    for (int d = 0; d < SPH_NTOT; d++) s[d][threadIdx.x] = t[d];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < (unsigned)off)
+            for (int d = 0; d < SPH_NTOT; d++) s[d][threadIdx.x] += s[d][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x < SPH_NTOT) out[threadIdx.x] = s[threadIdx.x][0];
+}
+
 namespace sph {
+
+hipError_t totals(hipStream_t st, int64_t n, const double *x, const double *y, const double *z, const double *vx,
+                  const double *vy, const double *vz, const double *m, const double *u, const double *phi,
+                  double *part, double *out10)
+{
+    int nb = (int)((n + 255) / 256);
+    if (nb > 1024) nb = 1024;
+    if (nb < 1) nb = 1;
+    k_tot_partial<<<nb, 256, 0, st>>>(n, x, y, z, vx, vy, vz, m, u, phi, part);
+    k_tot_final<<<1, 256, 0, st>>>(nb, part, out10);
+    return hipGetLastError();
+}
 
 hipError_t kick_drift(hipStream_t st, int64_t n, double *vx, double *vy, double *vz, const double *ax,
                       const double *ay, const double *az, double dtk, double dtd, double *x, double *y, double *z,

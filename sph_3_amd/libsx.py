@@ -5,10 +5,12 @@ in, fresh numpy float64 out, inputs never mutated).  Every function runs on
 the GPU through libsph_amd.so; extra behaviour is reachable only through
 keyword-only arguments whose defaults follow the reference:
 
-* ``precision``: ``"mixed"`` (FP32 pair arithmetic with Kahan sums, FP64
-  positions/velocities/centres of mass/h solve) or ``"fp64"``;
-* ``theta`` (gravity): Barnes-Hut opening angle, ``0`` = the reference's exact
-  direct sum;
+* ``precision``: ``"fp64"`` (the default, the reference's arithmetic) or
+  ``"mixed"`` (FP32 pair arithmetic with Kahan sums, FP64
+  positions/velocities/centres of mass/h solve);
+* ``theta`` (gravity, potential): Barnes-Hut opening angle; the default ``0`` is
+  the reference's exact direct sum; ``mac="reference"`` applies the reference
+  GPU walk's opening rule size^2/theta <= d^2 instead of size < theta d;
 * ``device``: GPU ordinal.
 
 Neighbour sums are taken over S_i = { j : |r_i - r_j| < h_i + h_j }, which
@@ -22,11 +24,13 @@ sum over a row partition is the full field, as the reference's does).
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 
-from ._lib import H_KTH, H_REF_COUNT, PREC, SOFT, context, f64, ptr
+from ._lib import H_KTH, H_REF_COUNT, PREC, SOFT, context, f64, ptr, theta_of
 
-__all__ = ["getDensity", "getDensity_mpi", "getPressure", "getAcc_sph", "getAcc_sph_mpi", "get_dU", "get_dU_mpi",
+__all__ = ["validate_particles", "getDensity", "getDensity_mpi", "getPressure", "getAcc_sph", "getAcc_sph_mpi", "get_dU", "get_dU_mpi",
            "getKE", "getPE", "getAcc_g_smth", "getAcc_g_smth_mpi", "getAcc_g_smth_mimj_mpi", "do_smoothingX",
            "smoothing_length_mpi", "h_smooth_fast", "h_smooth_fast_mpi", "neighbour_lists"]
 
@@ -38,8 +42,24 @@ def _prec(p):
         raise ValueError(f"precision must be one of {sorted(PREC)}") from None
 
 
+# ---------------------------------------------------------------- checks
+def validate_particles(pos, vel=None, m=None, u=None, h=None):
+    """The library's host-side check of a particle set (runs without a GPU): raises
+    ValueError with the reason when positions are not finite or all coincide, or a
+    given velocity is not finite, a mass or u negative, an h not positive."""
+    from ._lib import load
+
+    pos = f64(pos, (-1, 3)); n = len(pos)
+    arr = [None if a is None else (f64(a, (-1, 3)) if k == 0 else np.broadcast_to(f64(a), (n,)).copy())
+           for k, a in enumerate((vel, m, u, h))]
+    msg = ctypes.create_string_buffer(256)
+    rc = load().sph_validate_particles(n, ptr(pos), *(ptr(a) for a in arr), msg, 256)
+    if rc != 0:
+        raise ValueError(msg.value.decode())
+
+
 # ---------------------------------------------------------------- density
-def getDensity(pos, m, h, *, precision="mixed", device=0):
+def getDensity(pos, m, h, *, precision="fp64", device=0):
     """rho_i = sum_j m_j W(r_ij, (h_i+h_j)/2) -- sph_code/libsx.py:10-44."""
     pos = f64(pos, (-1, 3)); n = len(pos)
     out = np.empty(n)
@@ -84,7 +104,7 @@ def _dudt(pos, v, rho, P, c, h, m, alpha, divV, curlV, precision, device):
     return out
 
 
-def getAcc_sph(pos, v, rho, P, c, h, m, gama, eta, alpha, beta, *, precision="mixed", device=0):
+def getAcc_sph(pos, v, rho, P, c, h, m, gama, eta, alpha, beta, *, precision="fp64", device=0):
     """Pressure + Gadget-2 artificial-viscosity acceleration, no Balsara
     switch -- sph_code/libsx.py:100-177 (gama, eta, beta unused there too)."""
     return _acc(pos, v, rho, P, c, h, m, alpha, None, None, precision, device)
@@ -95,7 +115,7 @@ def getAcc_sph_mpi(nbeg, nend, pos, v, rho, P, c, h, m, gama, eta, alpha, beta, 
     return getAcc_sph(pos, v, rho, P, c, h, m, gama, eta, alpha, beta, **kw)[nbeg:nend].copy()
 
 
-def get_dU(pos, v, rho, P, c, h, m, gama, eta, alpha, beta, *, precision="mixed", device=0):
+def get_dU(pos, v, rho, P, c, h, m, gama, eta, alpha, beta, *, precision="fp64", device=0):
     """du/dt without Balsara switch -- sph_code/libsx.py:353-421."""
     return _dudt(pos, v, rho, P, c, h, m, alpha, None, None, precision, device)
 
@@ -115,25 +135,38 @@ def getKE(v, m, *, device=0):
     return float(out[0])
 
 
-def getPE(pos, m, G, epsilon):
-    """sph_code/libsx.py:583-620 (spline potential energy) is not part of the
-    per-step path; the device library does not provide it."""
-    raise NotImplementedError("getPE: the potential-energy sum is not implemented on the device")
+def getPE(pos, m, G, epsilon, *, theta=0.0, mac="geometric", softening="spline", eps_plummer=0.0,
+          precision="fp64", device=0, return_phi=False):
+    """PE = -G sum_{i<j} m_i m_j psi(r_ij, eps_ij), the spline potential with
+    eps_ij = (eps_i + eps_j)/2 -- sph_code/libsx.py:583-620 -- computed on the device as
+    -1/2 sum_i m_i phi_i over the tree walk (``theta=0``: every pair, the reference's
+    double sum; FP64 total).  ``softening="plummer"`` uses the scalar ``eps_plummer``.
+    ``return_phi``: also phi_i = G sum_j m_j psi(r_ij)."""
+    pos = f64(pos, (-1, 3)); n = len(pos)
+    eps = None if epsilon is None else np.broadcast_to(f64(epsilon), (n,)).copy()
+    pe = np.zeros(1)
+    phi = np.empty(n) if return_phi else None
+    c = context(device)
+    c.check(c.lib.sph_potential_energy(c.h, n, ptr(pos), ptr(f64(m)), ptr(eps), float(G), theta_of(theta, mac),
+                                       SOFT[softening], float(eps_plummer), _prec(precision), ptr(pe), ptr(phi)),
+            "getPE")
+    return (float(pe[0]), phi) if return_phi else float(pe[0])
 
 
 # ---------------------------------------------------------------- gravity
-def getAcc_g_smth(pos, mass, G, epsilon, *, theta=0.5, softening="spline", eps_plummer=0.0, precision="mixed",
-                  device=0, return_counts=False):
+def getAcc_g_smth(pos, mass, G, epsilon, *, theta=0.0, mac="geometric", softening="spline", eps_plummer=0.0,
+                  precision="fp64", device=0, return_counts=False):
     """Spline-softened self-gravity, eps_ij = (eps_i + eps_j)/2 --
-    sph_code/libsx.py:623-662.  Computed by a Barnes-Hut tree walk at opening
-    angle ``theta``; ``theta=0`` is the exact direct sum of the reference.
+    sph_code/libsx.py:623-662.  The default ``theta=0`` is the reference's exact
+    direct sum; ``theta > 0`` is a Barnes-Hut tree walk at that opening angle
+    (``mac="reference"``: the reference GPU walk's rule, EV/bh.h:505-532).
     ``softening="plummer"`` uses the scalar ``eps_plummer`` instead."""
     pos = f64(pos, (-1, 3)); n = len(pos)
     eps = None if epsilon is None else np.broadcast_to(f64(epsilon), (n,)).copy()
     out = np.empty((n, 3))
     cnt = np.zeros(2, np.int64)
     c = context(device)
-    c.check(c.lib.sph_gravity(c.h, n, ptr(pos), ptr(f64(mass)), ptr(eps), float(G), float(theta), SOFT[softening],
+    c.check(c.lib.sph_gravity(c.h, n, ptr(pos), ptr(f64(mass)), ptr(eps), float(G), theta_of(theta, mac), SOFT[softening],
                               float(eps_plummer), _prec(precision), ptr(out), ptr(cnt)), "getAcc_g_smth")
     if return_counts:
         return out, (int(cnt[0]), int(cnt[1]))

@@ -11,7 +11,7 @@ __all__ = ["div_curlVel", "div_curlVel_mpi", "getAcc_sph_shear", "getAcc_sph_she
            "get_dU_shear_mpi"]
 
 
-def div_curlVel(pos, v, rho, m, h, *, precision="mixed", device=0):
+def div_curlVel(pos, v, rho, m, h, *, precision="fp64", device=0):
     """(|div v|, |curl v|) -- sph_code/shear.py:6-69."""
     pos = f64(pos, (-1, 3)); n = len(pos)
     dv = np.empty(n); cv = np.empty(n)
@@ -27,7 +27,7 @@ def div_curlVel_mpi(nbeg, nend, pos, v, rho, m, h, **kw):
     return dv[nbeg:nend].copy(), cv[nbeg:nend].copy()
 
 
-def getAcc_sph_shear(pos, v, rho, P, c, h, m, divV, curlV, alpha, *, precision="mixed", device=0):
+def getAcc_sph_shear(pos, v, rho, P, c, h, m, divV, curlV, alpha, *, precision="fp64", device=0):
     """Acceleration with the Balsara switch -- sph_code/shear.py:149-235."""
     return _acc(pos, v, rho, P, c, h, m, alpha, divV, curlV, precision, device)
 
@@ -37,7 +37,7 @@ def getAcc_sph_shear_mpi(nbeg, nend, pos, v, rho, P, c, h, m, divV, curlV, alpha
     return getAcc_sph_shear(pos, v, rho, P, c, h, m, divV, curlV, alpha, **kw)[nbeg:nend].copy()
 
 
-def get_dU_shear(pos, v, rho, P, c, h, m, divV, curlV, alpha, *, precision="mixed", device=0):
+def get_dU_shear(pos, v, rho, P, c, h, m, divV, curlV, alpha, *, precision="fp64", device=0):
     """du/dt with the Balsara switch -- sph_code/shear.py:332-406."""
     return _dudt(pos, v, rho, P, c, h, m, alpha, divV, curlV, precision, device)
 

@@ -15,7 +15,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import PHASES, PREC, SOFT, Stats, StepParams, context, f64, ptr
+from ._lib import PHASES, PREC, SOFT, Stats, StepParams, context, f64, ptr, theta_of
 
 
 @dataclass
@@ -57,11 +57,13 @@ class StepConfig:
     balsara: bool = True
     period: float = 0.0  # > 0: periodic box [0, period)^3 (minimum image; needs G = 0)
     C_CFL: float = 0.0   # > 0: per-particle time-step criteria after every force evaluation
+    energy: bool = False  # FP64 totals (KE, U, PE, momentum, angular momentum) after every evaluation
+    mac: str = "geometric"  # "reference": the reference walk's opening rule size^2/theta <= d^2
 
     def params(self, timing=False) -> StepParams:
-        return StepParams(self.dt, self.gamma, self.alpha, self.G, self.theta, self.eps_plummer, self.n_ngb,
-                          self.h_tol, self.period, self.C_CFL, SOFT[self.softening], PREC[self.precision], int(self.balsara),
-                          int(timing))
+        return StepParams(self.dt, self.gamma, self.alpha, self.G, theta_of(self.theta, self.mac), self.eps_plummer,
+                          self.n_ngb, self.h_tol, self.period, self.C_CFL, SOFT[self.softening], PREC[self.precision],
+                          int(self.balsara), int(timing), int(self.energy))
 
 
 class DeviceSPH:
@@ -213,6 +215,9 @@ class DeviceSPH:
         return dict(n=s.n, steps=s.steps, ngb_total=s.ngb_total, grav_pp=s.grav_pp, grav_pn=s.grav_pn,
                     h_unconverged=s.h_unconverged, h_rounds=s.h_rounds, h_tests=s.h_tests, h_terms=s.h_terms, ngb_extra=s.ngb_extra,
                     cool_floor=s.cool_floor, cool_fallback=s.cool_fallback, h_big=s.h_big, dt_min=s.dt_min,
+                    energy=dict(kinetic=s.energy[0], thermal=s.energy[1], potential=s.energy[2],
+                                total=s.energy[0] + s.energy[1] + s.energy[2]),
+                    momentum=np.array(s.momentum[:]), ang_momentum=np.array(s.ang_momentum[:]),
                     phase_ms={k: s.phase_ms[i] for i, k in enumerate(PHASES)},
                     phase_calls={k: s.phase_calls[i] for i, k in enumerate(PHASES)})
 

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from sph_3_amd import column as C
 from sph_3_amd import ic
 from sph_3_amd.step import AgnParams, DeviceSPH, StepConfig
 
@@ -63,3 +64,40 @@ def test_isothermal_halo_adds_closed_form(gpu):
     ref = O.halo_accel(pos, 0.6, G=1.0)
     scale = np.abs(acc[0.6]).max()
     assert np.max(np.abs(diff - ref)) <= 1e-13 * scale
+
+
+def test_column_and_readbacks_with_injection(gpu):
+    """Injection that grows the state by far more than 12.5 % with the column density on:
+    the column buffers hold the grown state (every particle gets a finite log N_H at the next
+    evaluation), and read-backs right after an injecting step describe the last evaluation:
+    injected particles have empty neighbour rows and NaN density and column, the other rows
+    equal the oracle's sets of the evaluated particles."""
+    n0 = 20000
+    d = ic.evrard(n0, seed=43)
+    cfg = StepConfig(dt=1e-3)
+    m = d["m"][0]
+    ap = AgnParams(M_dot=600 * 2 * m / cfg.dt, v_in=0.05, m_inject=m, u_inject=0.05, r_inject=0.2, seed=3,
+                   h_ref_index=11)
+    sim = DeviceSPH(d["pos"], d["vel"], d["m"], d["u"], d["h0"], cfg, n_max=n0 + 6400)
+    sim.set_agn(ap)
+    sim.set_column(C.ColumnParams.from_mass_length(1.989e43, 3.0857e22))
+    sim.prime()
+    for step in range(4):
+        n_eval = sim.n
+        sim.step()
+        assert sim.n - n_eval >= 1000  # about 600 pairs per step
+        g = sim.get()
+        lognh, grad = sim.column()
+        assert np.all(np.isfinite(lognh[:n_eval])) and np.all(np.isnan(lognh[n_eval:]))
+        assert np.all(np.isfinite(g["rho"][:n_eval])) and np.all(np.isnan(g["rho"][n_eval:]))
+        off, idx = sim.neighbours()
+        assert len(off) == sim.n + 1 and np.all(np.diff(off)[n_eval:] == 0)
+        if step == 3:
+            masses = sim.masses()
+            off0, idx0 = O.Tree(g["pos"][:n_eval], masses[:n_eval], h=g["h"][:n_eval]).neighbours()
+            assert np.array_equal(off[:n_eval + 1], off0) and np.array_equal(idx, idx0)
+    assert sim.n - n0 > n0 // 8
+    n_eval = sim.n
+    sim.step()
+    lognh, _ = sim.column()
+    assert np.all(np.isfinite(lognh[:n_eval]))

@@ -99,3 +99,33 @@ def test_timestep_criteria_periodic(gpu):
     print("periodic dt rel max", rel.max())
     assert rel.max() < 1e-5
     assert np.max(np.abs(dh - ref_dh)) <= 1e-5 * np.max(np.abs(ref_dh))
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+@pytest.mark.parametrize("balsara", [True, False])
+def test_timestep_criteria_after_a_step(gpu, prec, balsara):
+    """After sph_step the criteria take v_sig from the velocities after the second kick and
+    |div v| from the force evaluation's velocities (before that kick), with and without the
+    Balsara switch, as the reference computes div v in the force pass (MC/drv.cu:1271) and the
+    criteria after the kick (:1610).  c is the evaluation's sound speed (u before its update)."""
+    n = 20000
+    d = ic.evrard(n, seed=63)
+    vel = np.random.default_rng(7).normal(size=(n, 3)) * 0.1
+    cfg = StepConfig(G=0.0, precision=prec, balsara=balsara, C_CFL=0.3, eps_plummer=1e-3)
+    sim = DeviceSPH(d["pos"], vel, d["m"], d["u"], d["h0"], cfg)
+    sim.prime()
+    u_eval = sim.get()["u"]  # u of the next evaluation (the step updates it after its forces)
+    sim.step()
+    g = sim.get()
+    dt, dh = sim.timestep()
+    v_half = g["vel"] - g["acc"] * (cfg.dt / 2)  # the evaluation's velocities (before the second kick)
+    off, idx = O.Tree(g["pos"], d["m"], h=g["h"]).neighbours()
+    c = np.sqrt(cfg.gamma * (cfg.gamma - 1) * u_eval)
+    dv, _ = O.divcurl_csr(g["pos"], v_half, g["rho"], d["m"], g["h"], off, idx)
+    ref, ref_dh = O.timestep(g["pos"], g["vel"], g["h"], c, dv, g["acc"], g["acc"], cfg.eps_plummer, cfg.C_CFL,
+                             cfg.alpha, off, idx)
+    rel = np.abs(dt / ref - 1)
+    tol = 1e-5 if prec == "mixed" else 1e-10
+    print(prec, balsara, "after a step: dt rel max", rel.max())
+    assert rel.max() < tol
+    assert np.max(np.abs(dh - ref_dh)) <= tol * np.max(np.abs(ref_dh))

@@ -158,6 +158,22 @@ def test_gravity_force_is_minus_potential_gradient():
         assert abs(a - dpe) < 1e-6 * abs(dpe)
 
 
+def test_plummer_force_is_minus_potential_gradient():
+    # Plummer force vs the Plummer potential -G m_i m_j / sqrt(r^2 + eps^2): F = -dPE/dr,
+    # and the potential energy of three particles is the sum over the three pairs
+    m = np.array([1.5, 2.0])
+    for r in [0.01, 0.1, 0.7]:
+        e = 1e-6
+        pe = lambda rr: O.pe(np.array([[0, 0, 0], [rr, 0, 0.0]]), m, 0.5, None, O.SOFT_PLUMMER, 0.1)
+        dpe = (pe(r + e) - pe(r - e)) / (2 * e)
+        a = O.grav_direct(np.array([[0, 0, 0], [r, 0, 0.0]]), m, None, G=0.5, soft_kind=O.SOFT_PLUMMER,
+                          eps_plummer=0.1)[0, 0]
+        assert abs(m[0] * a - dpe) < 1e-6 * abs(dpe)
+    p3 = np.array([[0, 0, 0], [0.3, 0, 0], [0, 0.4, 0.0]]); m3 = np.array([1.0, 2.0, 3.0])
+    exp = -(2.0 / np.sqrt(0.09 + 0.01) + 3.0 / np.sqrt(0.16 + 0.01) + 6.0 / np.sqrt(0.25 + 0.01))
+    assert abs(O.pe(p3, m3, 1.0, None, O.SOFT_PLUMMER, 0.1) - exp) < 1e-14 * abs(exp)
+
+
 def test_gravity_third_law_and_plummer():
     pos = sphere(300, 7)
     m = np.random.default_rng(1).uniform(0.5, 1.5, 300)
