@@ -111,6 +111,15 @@ int sph_gravity(sph_ctx *ctx, int64_t n, const double *pos, const double *m, con
 /* out5 = [KE, px, py, pz, M] (FP64).  Replaces libsx.py:571 getKE. */
 int sph_kinetics(sph_ctx *ctx, int64_t n, const double *vel, const double *m, double *out5);
 
+/* Allocation-free variant of sph_kinetics, same arguments and results.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It stages vel, m and the
+ * reduction's partials in a static device array of the library (about 32 MiB per GPU, held
+ * until the process ends) instead of context buffers and calls no hipMalloc/hipFree.
+ * n <= 1,048,576 (larger n: SPH_ERR_ARG).  Calls that overlap on one GPU share that array
+ * and race: serialise them. */
+int sph_kinetics_noalloc(sph_ctx *ctx, int64_t n, const double *vel, const double *m, double *out5);
+
 /* Gravitational potential energy PE = -1/2 sum_i m_i phi_i with phi_i = G sum_j m_j psi(r_ij) over
  * the tree walk's interaction list (theta = 0: every pair, exactly the reference's double sum).
  * Replaces libsx.py:585 getPE(pos, m, G, epsilon) (soft_kind SPH_SOFT_SPLINE: its spline potential

@@ -99,6 +99,7 @@ _SIGS = {
     "sph_gravity": ([_vp, _i64, _d, _d, _d, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                      ctypes.c_int, _d, _i64p], ctypes.c_int),
     "sph_kinetics": ([_vp, _i64, _d, _d, _d], ctypes.c_int),
+    "sph_kinetics_noalloc": ([_vp, _i64, _d, _d, _d], ctypes.c_int),
     "sph_potential_energy": ([_vp, _i64, _d, _d, _d, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                               ctypes.c_int, _d, _d], ctypes.c_int),
     "sph_validate_particles": ([_i64, _d, _d, _d, _d, _d, ctypes.c_char_p, _i64], ctypes.c_int),

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -64,6 +65,38 @@ __global__ void k_copy_d(int64_t n, const double *__restrict__ s, double *__rest
     if (i < n) d[i] = s[i];
 }
 
+}  // namespace
+
+// ---- scratch of the allocation-free entry point (sph_kinetics_noalloc): one static device
+// array per GPU, held from module load until the process ends.  Layout (doubles): the
+// caller's n x 3 velocities at [0, 3n), masses at [3n, 4n), the reduction's 1,024 block
+// partials of 5 values at [4 SPH_NOALLOC_MAX_N, + 5120), the 5 results after them.  In host
+// code `scratch` names a host-side stand-in: kernels and copies use the address
+// hipGetSymbolAddress returned for the device (g_scratch[device]).
+#define SPH_NOALLOC_MAX_N (1 << 20)
+#define SPH_NOALLOC_PART (4 * (size_t)SPH_NOALLOC_MAX_N)
+#define SPH_NOALLOC_WORDS (SPH_NOALLOC_PART + 5 * 1024 + 8)
+#define SPH_MAX_DEVICES 64
+static __device__ __align__(256) double scratch[SPH_NOALLOC_WORDS];
+
+namespace {
+std::mutex g_scratch_mu;
+double *g_scratch[SPH_MAX_DEVICES];           // device address of scratch per device (null: unresolved)
+hipError_t g_scratch_err[SPH_MAX_DEVICES];    // why it is unresolved
+
+// With `device` current: resolve that device's address of scratch once.
+double *resolve_scratch(int device)
+{
+    if (device < 0 || device >= SPH_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e == hipSuccess) g_scratch[device] = (double *)p;
+        else g_scratch_err[device] = e;
+    }
+    return g_scratch[device];
+}
 }  // namespace
 
 struct sph_ctx {
@@ -153,6 +186,7 @@ struct sph_ctx {
     hipEvent_t ev_h = nullptr, ev_n = nullptr;
     int64_t hs_unc = 0, hs_work[2] = {0, 0};
     int rho_prec = SPH_PREC_MIXED;  // arithmetic type of the resident rho buffer
+    double *scratch = nullptr;      // this device's address of the static scratch array
 };
 
 namespace {
@@ -539,6 +573,7 @@ extern "C" int sph_ctx_create(int device, sph_ctx **out)
             delete ctx;
             return SPH_ERR_HIP;
         }
+    ctx->scratch = resolve_scratch(device);  // the device is current here
     if (hipHostMalloc((void **)&ctx->hpin, 128 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_h, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_n, hipEventDisableTiming) != hipSuccess) {
@@ -1037,6 +1072,33 @@ extern "C" int sph_kinetics(sph_ctx *ctx, int64_t n, const double *vel, const do
     k_deinterleave<<<sph_blocks(n, 256), 256, 0, st>>>(n, D(stage), D(tx), D(ty), D(tz));
     HIPCK(sph::kinetics(st, n, D(tx), D(ty), D(tz), D(tmpd), D(kpart), D(kout)));
     HIPCK(hipMemcpyAsync(out5, ctx->kout.p, sizeof(double) * 5, hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+    return SPH_OK;
+}
+
+// Allocation-free twin of sph_kinetics: the caller's arrays and the reduction's partials are
+// staged in this device's static scratch array instead of context buffers, so the call makes
+// no device allocation (for callers that cannot allocate).  n <= SPH_NOALLOC_MAX_N.  Calls
+// that overlap on one GPU (two contexts, two threads) share that array and race.
+extern "C" int sph_kinetics_noalloc(sph_ctx *ctx, int64_t n, const double *vel, const double *m, double *out5)
+{
+    CK(check_common(ctx, n, SPH_PREC_FP64));
+    if (!vel || !m || !out5) return fail(ctx, SPH_ERR_ARG, "null argument");
+    if (n > SPH_NOALLOC_MAX_N)
+        return fail(ctx, SPH_ERR_ARG, "sph_kinetics_noalloc takes at most " + std::to_string(SPH_NOALLOC_MAX_N) +
+                                          " particles (its static scratch); n = " + std::to_string(n));
+    double *sp = ctx->scratch;
+    if (!sp) {
+        const hipError_t e = ctx->device < SPH_MAX_DEVICES ? g_scratch_err[ctx->device] : hipErrorInvalidDevice;
+        return fail(ctx, SPH_ERR_STATE, std::string("no device address for the static scratch: ") +
+                                            hipGetErrorString(e));
+    }
+    hipStream_t st = ctx->st;
+    double *dv = sp, *dm = sp + 3 * n, *part = sp + SPH_NOALLOC_PART, *res = part + 5 * 1024;
+    HIPCK(hipMemcpyAsync(dv, vel, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
+    HIPCK(hipMemcpyAsync(dm, m, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    HIPCK(sph::kinetics(st, n, dv, dv + 1, dv + 2, dm, part, res, 3));
+    HIPCK(hipMemcpyAsync(out5, res, sizeof(double) * 5, hipMemcpyDeviceToHost, st));
     HIPCK(hipStreamSynchronize(st));
     return SPH_OK;
 }

@@ -21,8 +21,9 @@ hipError_t prep(hipStream_t st, int64_t n, const R *rho, const R *P, const R *c,
 hipError_t totals(hipStream_t st, int64_t n, const double *x, const double *y, const double *z, const double *vx,
                   const double *vy, const double *vz, const double *m, const double *u, const double *phi,
                   double *part, double *out10);
-// out5 = [KE, px, py, pz, M]; part: 5*1024 doubles.
+// out5 = [KE, px, py, pz, M]; part: 5*1024 doubles.  Velocity component k of particle i
+// is vk[i * vstride] (1: three arrays; 3: vx = v, vy = v + 1, vz = v + 2 of one n x 3 array).
 hipError_t kinetics(hipStream_t st, int64_t n, const double *vx, const double *vy, const double *vz,
-                    const double *m, double *part, double *out5);
+                    const double *m, double *part, double *out5, int64_t vstride = 1);
 
 }  // namespace sph

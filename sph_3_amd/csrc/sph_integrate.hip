@@ -62,15 +62,17 @@ __global__ void __launch_bounds__(256) k_prep(int64_t n, const R *__restrict__ r
     }
 }
 
-// [KE, px, py, pz, M] per block, then folded by one block (deterministic).
+// [KE, px, py, pz, M] per block, then folded by one block (deterministic).  Velocity
+// component k of particle i is vk[i * vs] (vs = 1: three arrays; vs = 3: one n x 3 array).
 __global__ void __launch_bounds__(256) k_kin_partial(int64_t n, const double *__restrict__ vx,
                                                      const double *__restrict__ vy, const double *__restrict__ vz,
-                                                     const double *__restrict__ m, double *__restrict__ part)
+                                                     int64_t vs, const double *__restrict__ m,
+                                                     double *__restrict__ part)
 {
     __shared__ double s[5][256];
     double v[5] = {0, 0, 0, 0, 0};
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        double a = vx[i], b = vy[i], c = vz[i], mm = m[i];
+        double a = vx[i * vs], b = vy[i * vs], c = vz[i * vs], mm = m[i];
         v[0] += 0.5 * mm * (a * a + b * b + c * c);
         v[1] += mm * a;
         v[2] += mm * b;
@@ -201,12 +203,12 @@ hipError_t prep(hipStream_t st, int64_t n, const R *rho, const R *P, const R *c,
 }
 
 hipError_t kinetics(hipStream_t st, int64_t n, const double *vx, const double *vy, const double *vz,
-                    const double *m, double *part, double *out5)
+                    const double *m, double *part, double *out5, int64_t vstride)
 {
     int nb = (int)((n + 255) / 256);
     if (nb > 1024) nb = 1024;
     if (nb < 1) nb = 1;
-    k_kin_partial<<<nb, 256, 0, st>>>(n, vx, vy, vz, m, part);
+    k_kin_partial<<<nb, 256, 0, st>>>(n, vx, vy, vz, vstride, m, part);
     k_kin_final<<<1, 256, 0, st>>>(nb, part, out5);
     return hipGetLastError();
 }
