@@ -1,19 +1,27 @@
 """Throughput benchmark of the SPH step on one MI355X.
 
-Workload: Evrard collapse, N = 1,048,576 equal-mass gas particles (M = 1,
-R = 1, G = 1, rho ~ 1/r from r = R sqrt(U), seeded isotropic directions,
-u = 0.05, gamma = 5/3), cubic spline, N_ngb = 64 (FP64 Newton h solve),
-theta = 0.5, Plummer eps = 1e-3, alpha = 1 with the Balsara switch,
-dt = 1e-3.  One "step" is the full KDK step of sph_serial_v2.py on the
-device: kick/drift, Morton sort, tree, h solve, neighbour rows,
-density+EOS+div/curl, tree gravity, acceleration, kick, du/dt, u update.
+Headline workload (``value``): Evrard collapse plus cooling, N = 4,194,304
+equal-mass gas particles (M = 1, R = 1, G = 1 in code units of 1e10 M_sun and
+10 kpc, rho ~ 1/r from r = R sqrt(U), seeded isotropic directions, u = 0.05,
+gamma = 5/3), cubic spline, N_ngb = 64 (FP64 Newton h solve), theta = 0.5,
+Plummer eps = 1e-3, alpha = 1 with the Balsara switch, dt = 1e-3, per-particle
+Z and J21, and after each u update the explicit cooling update with log Lambda
+from the BF16 surrogate (table fallback).  One "step" is the full KDK step of
+sph_serial_v2.py on the device: kick/drift, Morton sort, tree, h solve,
+neighbour rows, density+EOS+div/curl, tree gravity, acceleration, kick,
+du/dt, u update, cooling.
 
 Prints ONE JSON line: particle-steps/s (whole job, inputs resident in HBM),
 the roofline of the dominant kernel from HIP events on the library's stream
-(HBM traffic per launch from the committed PMC summary), the all-FP64 form
-of the same kernels, the CPU baseline (the oracle's FP64 all-pairs
-restatement of the reference step, row-sampled) and the cooling surrogate at
-4,194,304 particles (training, gated inference, MFMA roofline, fallback rate).
+(HBM traffic per launch from the committed PMC summary at that N), the
+all-FP64 form of the same kernels with the acceleration error between the
+forms, the CPU baseline (the oracle's FP64 all-pairs restatement of the
+reference step on 4,096 sampled rows, with its sampling spread and the host's
+nproc/lscpu), and the other configurations as keys: the surrogate's training
+and inference (``mlp_cooling``), the 1,048,576-particle Evrard step with its
+errors against the FP64 oracle (``evrard_1m``), the 10,000-particle
+CPU-runnable case (``evrard_10k``), the AGN-like 8.4M workload and the
+periodic Sedov blast, each with its own roofline.
 """
 import argparse
 import json
@@ -66,14 +74,36 @@ def per_particle_bytes(kbar):
 KERNEL_OF_PHASE = {"order": "k_rs_scatter+k_gather", "moments": "k_moments_local+k_moments_up",
                    "hsolve": "k_hsolve", "ngb": "k_ngb<2>", "density": "k_density_divcurl<float,16>",
                    "divcurl": "k_divcurl<float,16>", "accsph": "k_accsph<float,16>", "dudt": "k_dudt<float,16>",
-                   "gravity": "k_gravity_group<float,2>", "moments_h": "k_moments_local+k_moments_up"}
+                   "gravity": "k_gravity_group<float,2,false>", "moments_h": "k_moments_local+k_moments_up"}
 
 
-def cpu_baseline(sim_state, n, m, rows, seed=0):
+def host_info():
+    """nproc and the lscpu lines that describe the host's cores."""
+    import subprocess
+
+    info = {"nproc": None, "affinity_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+    try:
+        info["nproc"] = int(subprocess.run(["nproc"], capture_output=True, text=True, timeout=10).stdout.strip())
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        keep = ("Model name", "CPU(s)", "Thread(s) per core", "Core(s) per socket", "Socket(s)", "CPU max MHz",
+                "NUMA node(s)", "L3 cache")
+        info["lscpu"] = {k.strip(): v.strip() for k, v in (ln.split(":", 1) for ln in out.splitlines() if ":" in ln)
+                         if k.strip() in keep}
+    except (OSError, subprocess.SubprocessError):
+        info["lscpu"] = None
+    return info
+
+
+def cpu_baseline(sim_state, n, m, rows, seed=0, batches=8):
     """Time the oracle's FP64 all-pairs restatement of the reference step
     (libsx.h_smooth_fast, getDensity, getAcc_g_smth, shear.div_curlVel,
     getAcc_sph_shear, get_dU_shear) on `rows` sampled target rows against all
-    n sources; extrapolate linearly in rows (rows are independent)."""
+    n sources; extrapolate linearly in rows (rows are independent).  Each function
+    runs over `batches` consecutive batches of the sample, whose spread gives the
+    relative standard error of its extrapolated time."""
     import oracle as O
 
     O.lib()
@@ -89,29 +119,35 @@ def cpu_baseline(sim_state, n, m, rows, seed=0):
     pos_p, vel_p, h_p, m_p = pos[perm], vel[perm], h[perm], m[perm]
     rho_p, P_p, c_p = rho[perm], P[perm], c[perm]
     dv = np.zeros(n); cv = np.zeros(n)
-    t = {}
-    t0 = time.perf_counter()
-    O.h_smooth_fast(pos_p, h_p * (50.0 / 64.0) ** (1 / 3), nbeg=0, nend=rows)
-    t["h_smooth_fast"] = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    O.density_allpairs(pos_p, m_p, h_p, 0, rows)
-    t["getDensity"] = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    O.grav_direct(pos_p, m_p, eps, 1.0, O.SOFT_SPLINE, 0.0, 0, rows)
-    # the reference's symmetric j > i loop does half the pairs per row on average
-    t["getAcc_g_smth"] = 0.5 * (time.perf_counter() - t0)
-    t0 = time.perf_counter()
-    O.divcurl_allpairs(pos_p, vel_p, rho_p, m_p, h_p, 0, rows)
-    t["div_curlVel"] = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    O.accsph_allpairs(pos_p, vel_p, rho_p, P_p, c_p, h_p, m_p, 1.0, dv, cv, 0, rows)
-    t["getAcc_sph_shear"] = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    O.dudt_allpairs(pos_p, vel_p, rho_p, P_p, c_p, h_p, m_p, 1.0, dv, cv, 0, rows)
-    t["get_dU_shear"] = time.perf_counter() - t0
+    h_start = h_p * (50.0 / 64.0) ** (1 / 3)
+    funcs = {
+        "h_smooth_fast": lambda a, b: O.h_smooth_fast(pos_p, h_start, nbeg=a, nend=b),
+        "getDensity": lambda a, b: O.density_allpairs(pos_p, m_p, h_p, a, b),
+        # the reference's symmetric j > i loop does half the pairs per row on average: timed x 1/2
+        "getAcc_g_smth": lambda a, b: O.grav_direct(pos_p, m_p, eps, 1.0, O.SOFT_SPLINE, 0.0, a, b),
+        "div_curlVel": lambda a, b: O.divcurl_allpairs(pos_p, vel_p, rho_p, m_p, h_p, a, b),
+        "getAcc_sph_shear": lambda a, b: O.accsph_allpairs(pos_p, vel_p, rho_p, P_p, c_p, h_p, m_p, 1.0, dv, cv,
+                                                           a, b),
+        "get_dU_shear": lambda a, b: O.dudt_allpairs(pos_p, vel_p, rho_p, P_p, c_p, h_p, m_p, 1.0, dv, cv, a, b),
+    }
+    edges = np.linspace(0, rows, batches + 1).astype(int)
+    t, rse = {}, {}
+    var = 0.0
+    for name, f in funcs.items():
+        bt = []
+        for a, b in zip(edges[:-1], edges[1:]):
+            t0 = time.perf_counter()
+            f(int(a), int(b))
+            bt.append((time.perf_counter() - t0) / (b - a))  # seconds per row
+        bt = np.array(bt) * (0.5 if name == "getAcc_g_smth" else 1.0)
+        t[name] = float(bt.mean() * rows)
+        se = float(bt.std(ddof=1) / np.sqrt(len(bt))) if len(bt) > 1 else 0.0
+        rse[name] = se / float(bt.mean())
+        var += (se * n) ** 2
     sample_s = sum(t.values())
     step_s = sample_s * n / rows
-    return step_s, sample_s, t
+    return step_s, sample_s, t, {"per_function_rel_std_error": rse, "step_rel_std_error": float(np.sqrt(var)) / step_s,
+                                 "batches": batches}
 
 
 def cpu_tree_oracle(sim_state, m, cfg):
@@ -135,15 +171,15 @@ def cpu_tree_oracle(sim_state, m, cfg):
     dv, cv = O.divcurl_csr(pos, vel, rho, m, h, off, idx)
     t["divcurl"] = time.perf_counter() - t0
     t0 = time.perf_counter()
-    O.accsph_csr(pos, vel, rho, P, c, h, m, cfg.alpha, off, idx, dv, cv)
+    a_h = O.accsph_csr(pos, vel, rho, P, c, h, m, cfg.alpha, off, idx, dv, cv)
     t["accsph"] = time.perf_counter() - t0
     t0 = time.perf_counter()
-    O.dudt_csr(pos, vel, rho, P, c, h, m, cfg.alpha, off, idx, dv, cv)
+    du = O.dudt_csr(pos, vel, rho, P, c, h, m, cfg.alpha, off, idx, dv, cv)
     t["dudt"] = time.perf_counter() - t0
     t0 = time.perf_counter()
-    O.Tree(pos, m).gravity(theta=cfg.theta, G=cfg.G, soft_kind=O.SOFT_PLUMMER, eps_plummer=cfg.eps_plummer)
+    a_g, cnt = O.Tree(pos, m).gravity(theta=cfg.theta, G=cfg.G, soft_kind=O.SOFT_PLUMMER, eps_plummer=cfg.eps_plummer)
     t["tree_gravity"] = time.perf_counter() - t0
-    return sum(t.values()), t
+    return sum(t.values()), t, dict(off=off, idx=idx, rho=rho, acc=a_h + a_g, du=du, nint=cnt)
 
 
 def c1_section(cfg, run):
@@ -158,8 +194,8 @@ def c1_section(cfg, run):
     g = sim.get()
     del sim
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    step_s, sample_s, parts = cpu_baseline(g, n, d["m"], n)
-    tree_s, tree_parts = cpu_tree_oracle(g, d["m"], cfg)
+    step_s, sample_s, parts, _ = cpu_baseline(g, n, d["m"], n, batches=1)
+    tree_s, tree_parts, _ = cpu_tree_oracle(g, d["m"], cfg)
     return {
         "workload": "evrard_collapse, n = 10000", "n": n, "gpu_ms_per_step": 1e3 * dt / 5,
         "gpu_value": n * 5 / dt,
@@ -176,16 +212,78 @@ MLP_FLOP_PER_SAMPLE = 2 * (4 * 128 + 2 * 128 * 128 + 128)  # forward multiply-ad
 BF16_MFMA_PEAK_TFLOPS = 2500.0
 
 
-def traffic_table():
+TRAFFIC_PROFILES = {1048576: "profiles/hbm_traffic_evrard_1M.md",
+                    4194304: "profiles/hbm_traffic_evrard_cooling_4M.md"}
+
+
+def traffic_table(n):
     """HBM bytes per launch measured by rocprofv3 FETCH_SIZE/WRITE_SIZE runs (scaled by
-    the copy-kernel calibration), from the JSON block of profiles/hbm_traffic_evrard_1M.md."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_evrard_1M.md")
+    the copy-kernel calibration) at this N, from the JSON block of the committed profile
+    summary ({} when there is none for this N)."""
+    rel = TRAFFIC_PROFILES.get(n)
+    if rel is None:
+        return {}, None
     try:
-        text = open(path).read()
+        text = open(os.path.join(ROOT, rel)).read()
         block = text.split("```json", 1)[1].split("```", 1)[0]
-        return json.loads(block)
+        return json.loads(block), rel
     except (OSError, IndexError, ValueError):
-        return {}
+        return {}, None
+
+
+def roofline(st, n, softening="plummer"):
+    """Roofline of the step's dominant phase (most HIP-event time per step, timed steps only)
+    and the algorithmic bandwidth of the HBM-bound passes.  Gravity: FP32 VALU, 20 flop per
+    counted interaction; h solve: FP64 VALU from its device counters; the rest: HBM bytes."""
+    phases = {k: v / max(1, st["phase_calls"][k]) for k, v in st["phase_ms"].items() if st["phase_calls"][k]}
+    per_step = {k: st["phase_ms"][k] / max(1, st["steps"]) for k in phases}
+    kbar = st["ngb_total"] / n
+    bytes_pp = per_particle_bytes(kbar)
+    bytes_pp["moments_h"] = bytes_pp["moments"]
+    dom = max(per_step, key=per_step.get)
+    h_flops = H_FLOP_PER_TEST * st["h_tests"] + H_FLOP_PER_TERM * st["h_terms"]
+    if dom == "hsolve":
+        ach = h_flops / (phases[dom] * 1e-3) / 1e12
+        roof = {"kernel": "k_hwalk+k_hnewton", "bound": "valu_fp64", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
+                "algorithmic": f"{H_FLOP_PER_TEST} flop x {st['h_tests']} walk distance tests + "
+                               f"{H_FLOP_PER_TERM} flop x {st['h_terms']} Newton kernel terms",
+                "ms_per_launch": phases[dom]}
+    elif dom == "gravity":
+        flops = FLOP_PER_INTERACTION * (st["grav_pp"] + st["grav_pn"])
+        ach = flops / (phases[dom] * 1e-3) / 1e12
+        roof = {"kernel": KERNEL_OF_PHASE[dom], "bound": "valu", "achieved": ach, "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": ach / FP32_PEAK_TFLOPS, "traffic": None,
+                "algorithmic": f"{FLOP_PER_INTERACTION} flop x {st['grav_pp'] + st['grav_pn']} interactions",
+                "ms_per_launch": phases[dom],
+                # the 157.3 TFLOP/s peak counts packed FP32 (two FMAs per lane and cycle), which the
+                # build disables; the non-packed VALU ceiling is half of it
+                "peak_nonpacked": FP32_PEAK_TFLOPS / 2, "frac_of_nonpacked": ach / (FP32_PEAK_TFLOPS / 2)}
+    else:
+        b = bytes_pp.get(dom, 0.0)
+        ach = b * n / (phases[dom] * 1e-3) / 1e9
+        roof = {"kernel": KERNEL_OF_PHASE.get(dom, dom), "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                "algorithmic": f"{b:.1f} B/particle x {n} particles", "ms_per_launch": phases[dom]}
+    table, src = traffic_table(n)
+    tr = table.get(roof["kernel"])
+    if tr:
+        roof["traffic"] = tr["fetch_bytes"] + tr["write_bytes"]
+        roof["traffic_source"] = src
+    hbm = {k: {"ms": phases[k], "GB/s": bytes_pp[k] * n / (phases[k] * 1e-3) / 1e9,
+               "frac": bytes_pp[k] * n / (phases[k] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+           for k in bytes_pp if k in phases and not (k == "moments" and softening == "plummer")}
+    extra = {
+        "phases_ms": phases, "phase_ms_per_step": per_step,
+        "h_solve_fp64_tflops": h_flops / (phases["hsolve"] * 1e-3) / 1e12 if "hsolve" in phases else None,
+        "neighbours_per_particle": kbar, "neighbour_pairs_from_other_side": st["ngb_extra"],
+        "h_solve_rounds_last_step": st["h_rounds"], "h_solve_walk_solved_targets_last_step": st["h_big"],
+        "gravity_interactions_per_particle": (st["grav_pp"] + st["grav_pn"]) / n,
+        "gravity_tflops": (FLOP_PER_INTERACTION * (st["grav_pp"] + st["grav_pn"]) / (phases["gravity"] * 1e-3) / 1e12
+                           if "gravity" in phases else None),
+        "pass_bandwidth": hbm,
+    }
+    return roof, extra
 
 
 def mlp_section(n_inf, epochs=200):
@@ -229,14 +327,13 @@ def mlp_section(n_inf, epochs=200):
     }
 
 
-def cooling_section(args, cfg, run):
-    """Evrard collapse plus cooling at N = cooling_n (4,194,304 by default): per-particle
-    Z log-uniform in [1e-3, 1] Z_sun and J21 in [1e-2, 1e2], code units M = 1e10 M_sun,
-    R = 10 kpc, G = 1; after each step's u update the explicit cooling update with
-    log Lambda from the gated surrogate (table fallback at 0.05 dex) and log Gamma and
-    T(u) from tables.  The all-FP64 form takes the table for log Lambda (there is no
-    FP64 network).  CPU: the same all-pairs restatement on a quarter of the 1M row
-    sample (rows cost 4x at 4x N) plus the oracle's exact cooling update."""
+def cooling_setup(args):
+    """The Evrard-plus-cooling workload at N = cooling_n (4,194,304 by default): seeded
+    Evrard sphere, per-particle Z log-uniform in [1e-3, 1] Z_sun and J21 in [1e-2, 1e2],
+    code units M = 1e10 M_sun, R = 10 kpc, G = 1.  Returns (d, Z, J21, units, cooling)
+    with cooling(mode) a set-up callback: after each step's u update the explicit cooling
+    update with log Lambda from the gated surrogate ("mlp", table fallback at 0.05 dex)
+    or the table ("table"), log Gamma and T(u) from tables."""
     from sph_3_amd import cooling as C
     from sph_3_amd import ic
 
@@ -244,8 +341,7 @@ def cooling_section(args, cfg, run):
     d = ic.evrard(n, seed=44)
     Z, J = ic.cooling_fields(n, seed=45)
     net = getattr(mlp_section, "net", None)
-    mode = "mlp" if net is not None else "table"
-    units = C.CoolingParams.from_mass_length(1e10 * C.M_SUN_G, 10 * C.KPC_CM, mode=mode, tol=0.05)
+    units = C.CoolingParams.from_mass_length(1e10 * C.M_SUN_G, 10 * C.KPC_CM, mode="mlp" if net else "table", tol=0.05)
 
     def cooling(mode_):
         def f(sim):
@@ -253,49 +349,153 @@ def cooling_section(args, cfg, run):
             sim.set_cooling(Z, J, u, surrogate=net if mode_ == "mlp" else None)
         return f
 
-    sim, dt = run("mixed", args.steps, args.warmup, d=d, cooling=cooling(mode))
+    return d, Z, J, units, cooling
+
+
+def prime_states(d, cfg, cooling=None, neighbours=False):
+    """The first force evaluation of both forms from the same initial state (the h solve is
+    FP64 in both, so the tree, h and neighbour sets agree and the forms differ by the FP32
+    pair and gravity arithmetic).  Returns {form: state} (+ the mixed form's sets)."""
+    from sph_3_amd.step import DeviceSPH
+
+    out = {}
+    saved = cfg.precision
+    try:
+        for prec in ("fp64", "mixed"):
+            cfg.precision = prec
+            sim = DeviceSPH(d["pos"], d["vel"], d["m"], d["u"], d["h0"], cfg)
+            if cooling is not None:
+                cooling(sim)
+            sim.prime()
+            out[prec] = sim.get()
+            if neighbours and prec == "mixed":
+                out["sets"] = sim.neighbours()
+            del sim
+    finally:
+        cfg.precision = saved
+    return out
+
+
+def accel_error(states):
+    a64, am = states["fp64"]["acc"], states["mixed"]["acc"]
+    na = np.linalg.norm(a64, axis=1)
+    de = np.linalg.norm(am - a64, axis=1)
+    rel = de / np.maximum(na, np.finfo(float).tiny)
+    return {"definition": "|a_mixed - a_fp64| / |a_fp64| per particle, first force evaluation",
+            "max": float(rel.max()), "p99.9": float(np.quantile(rel, 0.999)), "rms": float(np.sqrt(np.mean(rel ** 2))),
+            "max_over_rms_accel": float(de.max() / np.sqrt(np.mean(na ** 2))), "bound": 1e-3,
+            "within_bound": bool(rel.max() <= 1e-3)}
+
+
+def oracle_errors(states, ref, d, dt):
+    """Errors of the device's first force evaluation against the FP64 oracle evaluated at the
+    device's positions and h: rho relative, acceleration and du/dt per particle relative
+    (|x - y| / |y|) and normalised (|x - y| / rms |y|), max and rms."""
+    out = {}
+    for prec in ("mixed", "fp64"):
+        g = states[prec]
+        e_rho = np.abs(g["rho"] / ref["rho"] - 1)
+        da = np.linalg.norm(g["acc"] - ref["acc"], axis=1)
+        na = np.linalg.norm(ref["acc"], axis=1)
+        du = (g["u"] - d["u"]) / dt
+        ddu = np.abs(du - ref["du"])
+        rms_a, rms_du = np.sqrt(np.mean(na ** 2)), np.sqrt(np.mean(ref["du"] ** 2))
+        out[prec] = {
+            "rho_rel": {"max": float(e_rho.max()), "rms": float(np.sqrt(np.mean(e_rho ** 2)))},
+            "acc_rel": {"max": float((da / na).max()), "p99.9": float(np.quantile(da / na, 0.999)),
+                        "rms": float(np.sqrt(np.mean((da / na) ** 2)))},
+            "acc_normalised": {"max": float(da.max() / rms_a), "rms": float(np.sqrt(np.mean(da ** 2)) / rms_a)},
+            "dudt_normalised": {"max": float(ddu.max() / rms_du), "rms": float(np.sqrt(np.mean(ddu ** 2)) / rms_du)},
+        }
+    return out
+
+
+def cooling_cpu_baseline(args, cfg, g, d, Z, J, units):
+    """CPU baseline of the cooling workload: the all-pairs restatement on cpu_rows sampled
+    rows x all N sources plus the oracle's exact FP64 cooling update of as many particles."""
+    import oracle as O
+    from sph_3_amd import cooling as C
+
+    n = len(d["m"])
+    rows = args.cpu_rows
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    log(f"[bench] cooling workload cpu baseline: {rows} rows x {n} sources on {threads} threads")
+    step_s, sample_s, parts, spread = cpu_baseline(g, n, d["m"], rows)
+    sel = np.random.default_rng(9).choice(n, rows, replace=False)
+    t0 = time.perf_counter()
+    box = C.TABLE_RANGES
+    O.cool_update(g["rho"][sel], g["u"][sel], Z[sel], J[sel], cfg.dt,
+                  (units.unit_density_cgs, units.unit_u_cgs, units.unit_time_s, units.XH, units.u_floor_cgs),
+                  [b[0] for b in box], [b[1] for b in box])
+    t_cool = time.perf_counter() - t0
+    parts["cooling_update"] = t_cool
+    step_s += t_cool * n / rows
+    return {
+        "value": n / step_s, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+        "sample": f"{rows} random target rows x all {n} sources of the FP64 all-pairs restatement of "
+                  f"sph_serial_v2.py's step (h_smooth_fast, getDensity, getAcc_g_smth, div_curlVel, getAcc_sph_shear, "
+                  f"get_dU_shear) plus the exact FP64 cooling update of {rows} particles; "
+                  f"{sample_s + t_cool:.1f} s measured, extrapolated x{n / rows:.0f}",
+        "step_seconds_extrapolated": step_s, "function_seconds_sample": parts, "sampling": spread,
+        "host": host_info(),
+    }
+
+
+def c2_section(args, cfg, run):
+    """Evrard collapse N = 1,048,576 without cooling: throughput and roofline, the FP64 form,
+    and the first force evaluation against the FP64 oracle's tree path at the device's
+    positions and h (neighbour sets compared as integer sets; rho, acceleration, du/dt
+    errors), with that oracle path and the all-pairs restatement timed on the host."""
+    from sph_3_amd import ic
+
+    n = args.n
+    d = ic.evrard(n, seed=42)
+    sim, dt = run("mixed", args.steps, args.warmup, d=d)
     st = sim.stats()
     g = sim.get()
-    ms = 1e3 * dt / args.steps
-    per_step = {k: st["phase_ms"][k] / args.steps for k in st["phase_ms"] if st["phase_calls"][k]}
-    res = {
-        "workload": f"evrard_collapse + cooling, n = {n}", "n": n, "cooling_mode": mode, "tol_dex": units.tol,
-        "units": {"mass_g": 1e10 * C.M_SUN_G, "length_cm": 10 * C.KPC_CM, "density_cgs": units.unit_density_cgs,
-                  "u_cgs": units.unit_u_cgs, "time_s": units.unit_time_s},
-        "ms_per_step": ms, "value": n * args.steps / dt, "unit": "particle-steps/s",
-        "phase_ms_per_step": per_step,
-        "cooling_fallback_fraction_last_step": st["cool_fallback"] / n,
-        "cooling_floored_fraction_last_step": st["cool_floor"] / n,
-    }
     del sim
+    ms = 1e3 * dt / args.steps
+    roof, extra = roofline(st, n)
+    res = {"workload": f"evrard_collapse, n = {n}", "n": n, "ms_per_step": ms, "value": n * args.steps / dt,
+           "unit": "particle-steps/s", "roofline": roof, **extra}
     if not args.no_fp64_form:
         steps64 = max(2, args.steps // 2)
-        sim64, dt64 = run("fp64", steps64, 1, d=d, cooling=cooling("table"))
-        res["fp64_form"] = {"ms_per_step": 1e3 * dt64 / steps64, "mixed_speedup": (1e3 * dt64 / steps64) / ms,
-                            "cooling_mode": "table"}
+        sim64, dt64 = run("fp64", steps64, 1, d=d)
+        res["fp64_form"] = {"ms_per_step": 1e3 * dt64 / steps64, "mixed_speedup": (1e3 * dt64 / steps64) / ms}
         del sim64
+    log("[bench] 1M: first force evaluation of both forms")
+    # the parity evaluation starts from the same particles moving with seeded velocities
+    # (0.02 rms per axis), so that viscosity, the Balsara switch and du/dt are exercised
+    dv = dict(d, vel=np.random.default_rng(9).normal(size=(n, 3)) * 0.02)
+    states = prime_states(dv, cfg, neighbours=not args.no_cpu_baseline)
+    res["accel_error_mixed_vs_fp64"] = accel_error(states)
+    res["column_density"] = column_section(args, cfg, run, ms, st["ngb_total"] / n, d) if not args.no_column else None
+    res["file_io"] = io_section(g, d["m"])
     if not args.no_cpu_baseline:
-        import oracle as O
-
-        rows = max(64, args.cpu_rows // 4)
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        log(f"[bench] cooling workload cpu baseline: {rows} rows x {n} sources on {threads} threads")
-        step_s, sample_s, parts = cpu_baseline(g, n, d["m"], rows)
-        sel = np.random.default_rng(9).choice(n, rows, replace=False)
-        t0 = time.perf_counter()
-        box = C.TABLE_RANGES
-        O.cool_update(g["rho"][sel], g["u"][sel], Z[sel], J[sel], cfg.dt,
-                      (units.unit_density_cgs, units.unit_u_cgs, units.unit_time_s, units.XH, units.u_floor_cgs),
-                      [b[0] for b in box], [b[1] for b in box])
-        t_cool = time.perf_counter() - t0
-        parts["cooling_update"] = t_cool
-        step_s += t_cool * n / rows
+        log(f"[bench] 1M: oracle tree path on {threads} threads")
+        tree_s, tree_parts, ref = cpu_tree_oracle(states["mixed"], d["m"], cfg)
+        off, idx = states["sets"]
+        res["oracle_parity"] = {
+            "what": "first force evaluation from the same particles; oracle = FP64 tree neighbours, CSR hydro "
+                    "and group-walk gravity (same tree and opening test) at the device's positions and h",
+            "neighbour_sets_bit_exact": bool(np.array_equal(off, ref["off"]) and np.array_equal(idx, ref["idx"])),
+            "neighbour_entries": int(len(ref["idx"])),
+            **oracle_errors(states, ref, dv, cfg.dt),
+        }
+        res["cpu_tree_oracle"] = {
+            "value": n / tree_s, "unit": "particle-steps/s", "cores": threads,
+            "kind": "oracle tree path (not the reference's algorithm): tree neighbours, CSR hydro, group-walk "
+                    "gravity at the device's h, without the h solve; every row",
+            "seconds": tree_s, "function_seconds": tree_parts,
+        }
+        log(f"[bench] 1M: cpu baseline {args.cpu_rows} rows")
+        step_s, sample_s, parts, spread = cpu_baseline(g, n, d["m"], args.cpu_rows)
         res["cpu_baseline"] = {
             "value": n / step_s, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{rows} random target rows x all {n} sources of the FP64 all-pairs restatement of the "
-                      f"step plus the exact FP64 cooling update of {rows} particles; "
-                      f"{sample_s + t_cool:.1f} s measured, extrapolated x{n / rows:.0f}",
-            "step_seconds_extrapolated": step_s, "function_seconds_sample": parts,
+            "sample": f"{args.cpu_rows} random target rows x all {n} sources of the FP64 all-pairs restatement "
+                      f"of sph_serial_v2.py's step; {sample_s:.1f} s measured, extrapolated x{n / args.cpu_rows:.0f}",
+            "step_seconds_extrapolated": step_s, "function_seconds_sample": parts, "sampling": spread,
         }
         res["speedup_vs_cpu"] = res["value"] / res["cpu_baseline"]["value"]
     return res
@@ -343,6 +543,7 @@ def agn_section(args, cfg, run):
         st = sim.stats()
         ms = 1e3 * dt / args.steps
         n_end = sim.n
+        roof, _ = roofline(st, n)
         res = {
             "workload": f"agn-like: {args.agn_cloud} cloud + {args.agn_wind} wind particles", "n": n,
             "cooling_mode": mode, "dt": cfg.dt, "ms_per_step": ms, "value": n * args.steps / dt,
@@ -355,6 +556,7 @@ def agn_section(args, cfg, run):
             "cooling_fallback_fraction_last_step": st["cool_fallback"] / n,
             "source_terms": {"halo_sigma": agn.sigma, "pairs_per_step": "50 or 51 (50.5 on average)",
                              "particles_injected": n_end - n},
+            "roofline": roof,
         }
         del sim
         if not args.no_fp64_form:
@@ -385,7 +587,9 @@ def sedov_section(args, cfg, run):
         sim, dt = run("mixed", args.steps, args.warmup, d=d)
         st = sim.stats()
         ms = 1e3 * dt / args.steps
+        roof, _ = roofline(st, n)
         res = {
+            "roofline": roof,
             "workload": f"sedov blast, periodic unit box, {args.sedov_side}^3 = {n} particles", "n": n,
             "dt": cfg.dt, "ms_per_step": ms, "value": n * args.steps / dt, "unit": "particle-steps/s",
             "phase_ms_per_step": {k: st["phase_ms"][k] / args.steps for k in st["phase_ms"] if st["phase_calls"][k]},
@@ -403,7 +607,7 @@ def sedov_section(args, cfg, run):
     return res
 
 
-def column_section(args, cfg, run, ms_step, kbar):
+def column_section(args, cfg, run, ms_step, kbar, d):
     """Column density N_H from |grad rho| (the CLOUDY driver's grad_rho_norm_ngb and N_H
     loop) and the per-particle time-step criteria (dt_array_indiv_dt, C_CFL = 0.3) computed
     in every force evaluation of the 1M Evrard step, units 1e10 M_sun and 10 kpc: each pass's
@@ -414,7 +618,7 @@ def column_section(args, cfg, run, ms_step, kbar):
     units = C.ColumnParams.from_mass_length(1.989e43, 3.0857e22)
     cfg.C_CFL = 0.3
     try:
-        sim, dt = run("mixed", args.steps, args.warmup, cooling=lambda s: s.set_column(units))
+        sim, dt = run("mixed", args.steps, args.warmup, d=d, cooling=lambda s: s.set_column(units))
     finally:
         cfg.C_CFL = 0.0
     st = sim.stats()
@@ -470,43 +674,21 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": nbytes, "launches": 3}), flush=True)
 
 
-def accel_error(d, cfg):
-    """Acceleration of the mixed form against the FP64 form after the first force evaluation
-    from the same initial state (the h solve is FP64 in both, so the tree, h and neighbour
-    sets agree and the difference is the FP32 pair and gravity arithmetic)."""
-    from sph_3_amd.step import DeviceSPH
-
-    acc = {}
-    for prec in ("fp64", "mixed"):
-        cfg.precision = prec
-        sim = DeviceSPH(d["pos"], d["vel"], d["m"], d["u"], d["h0"], cfg)
-        sim.prime()
-        acc[prec] = sim.get()["acc"]
-        del sim
-    a64, am = acc["fp64"], acc["mixed"]
-    na = np.linalg.norm(a64, axis=1)
-    de = np.linalg.norm(am - a64, axis=1)
-    rel = de / np.maximum(na, np.finfo(float).tiny)
-    return {"definition": "|a_mixed - a_fp64| / |a_fp64| per particle, first force evaluation",
-            "max": float(rel.max()), "p99.9": float(np.quantile(rel, 0.999)), "rms": float(np.sqrt(np.mean(rel ** 2))),
-            "max_over_rms_accel": float(de.max() / np.sqrt(np.mean(na ** 2))), "bound": 1e-3,
-            "within_bound": bool(rel.max() <= 1e-3)}
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=1048576)
+    ap.add_argument("--n", type=int, default=1048576, help="particles of the Evrard step without cooling")
     ap.add_argument("--cpu-rows", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-form", action="store_true")
-    ap.add_argument("--no-mlp", action="store_true")
+    ap.add_argument("--no-mlp", action="store_true", help="cooling by the table instead of the surrogate")
     ap.add_argument("--mlp-n", type=int, default=4194304)
     ap.add_argument("--mlp-epochs", type=int, default=200, help="training passes of the surrogate")
-    ap.add_argument("--no-cooling-step", action="store_true")
-    ap.add_argument("--cooling-n", type=int, default=4194304)
+    ap.add_argument("--cooling-n", type=int, default=4194304, help="particles of the headline step")
+    ap.add_argument("--no-c2", action="store_true", help="skip the 1M Evrard block")
+    ap.add_argument("--no-c1", action="store_true", help="skip the 10,000-particle block")
     ap.add_argument("--no-agn", action="store_true")
     ap.add_argument("--no-sedov", action="store_true")
     ap.add_argument("--no-column", action="store_true", help="skip the column-density measurement")
@@ -523,24 +705,22 @@ def main():
               "set over several GPUs is designed but not built or measured", file=sys.stderr)
         sys.exit(2)
 
-    from sph_3_amd import ic
+    from sph_3_amd.cooling import KPC_CM, M_SUN_G
     from sph_3_amd.step import DeviceSPH, StepConfig
 
-    n = args.n
-    d = ic.evrard(n, seed=42)
     cfg = StepConfig(dt=1e-3, gamma=5 / 3, alpha=1.0, G=1.0, theta=0.5, softening="plummer", eps_plummer=1e-3,
                      n_ngb=64.0, h_tol=1e-10, precision="mixed", balsara=True)
 
-    def run(precision, steps, warmup, d=d, cooling=None, n_max=None):
+    def run(precision, steps, warmup, d, cooling=None, n_max=None):
         cfg.precision = precision
         sim = DeviceSPH(d["pos"], d["vel"], d["m"], d["u"], d["h0"], cfg, n_max=n_max)
         if cooling is not None:
             cooling(sim)
-        log(f"[bench] {precision}: prime")
+        log(f"[bench] {precision} n = {len(d['m'])}: prime")
         sim.prime()
         for w in range(warmup):
             sim.step()
-            log(f"[bench] {precision}: warmup step {w + 1}/{warmup} done")
+        log(f"[bench] {precision}: {warmup} warm-up steps done")
         sim.reset_stats()
         try:
             import torch
@@ -561,116 +741,60 @@ def main():
         log(f"[bench] {precision}: {steps} timed steps in {dt:.3f} s")
         return sim, dt
 
-    sim, dt = run("mixed", args.steps, args.warmup)
+    # ---- the cooling surrogate (the headline step's log Lambda), trained on the device
+    mlp = None
+    if not args.no_mlp:
+        log("[bench] cooling surrogate: train, calibrate, time")
+        mlp = mlp_section(args.mlp_n, args.mlp_epochs)
+    # ---- headline: Evrard collapse + cooling at N = 4,194,304
+    d, Z, J, units, cooling = cooling_setup(args)
+    n = len(d["m"])
+    mode = "mlp" if mlp is not None else "table"
+    sim, dt = run("mixed", args.steps, args.warmup, d=d, cooling=cooling(mode))
     st = sim.stats()
-    g_mixed = sim.get()  # the context is shared: read the state before the next run replaces it
+    g = sim.get()
+    del sim
     ms_step = 1e3 * dt / args.steps
     value = n * args.steps / dt
-    phases = {k: v / max(1, st["phase_calls"][k]) for k, v in st["phase_ms"].items() if st["phase_calls"][k]}
-    per_step = {k: st["phase_ms"][k] / args.steps for k in phases}
-    kbar = st["ngb_total"] / n
-    bytes_pp = per_particle_bytes(kbar)
-    bytes_pp["moments_h"] = bytes_pp["moments"]
-    # dominant kernel: the phase with the most time per step
-    dom = max(per_step, key=per_step.get)
-    h_flops = H_FLOP_PER_TEST * st["h_tests"] + H_FLOP_PER_TERM * st["h_terms"]
-    if dom == "hsolve":
-        ach = h_flops / (phases[dom] * 1e-3) / 1e12
-        roof = {"kernel": "k_hsolve", "bound": "valu_fp64", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
-                "algorithmic": f"{H_FLOP_PER_TEST} flop x {st['h_tests']} walk distance tests + "
-                               f"{H_FLOP_PER_TERM} flop x {st['h_terms']} Newton kernel terms",
-                "ms_per_launch": phases[dom]}
-    elif dom == "gravity":
-        flops = FLOP_PER_INTERACTION * (st["grav_pp"] + st["grav_pn"])
-        ach = flops / (phases[dom] * 1e-3) / 1e12
-        roof = {"kernel": KERNEL_OF_PHASE[dom], "bound": "valu", "achieved": ach, "peak": FP32_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": ach / FP32_PEAK_TFLOPS, "traffic": None,
-                "algorithmic": f"{FLOP_PER_INTERACTION} flop x {st['grav_pp'] + st['grav_pn']} interactions",
-                "ms_per_launch": phases[dom],
-                # the 157.3 TFLOP/s peak counts packed FP32 (two FMAs per lane and cycle), which the
-                # build disables; the non-packed VALU ceiling is half of it
-                "peak_nonpacked": FP32_PEAK_TFLOPS / 2, "frac_of_nonpacked": ach / (FP32_PEAK_TFLOPS / 2)}
-    else:
-        b = bytes_pp.get(dom, 0.0)
-        ach = b * n / (phases[dom] * 1e-3) / 1e9
-        roof = {"kernel": KERNEL_OF_PHASE.get(dom, dom), "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
-                "algorithmic": f"{b:.1f} B/particle x {n} particles", "ms_per_launch": phases[dom]}
-    # HBM fractions of the CSR passes (reported beside the dominant kernel)
-    hbm = {k: {"ms": phases[k], "GB/s": bytes_pp[k] * n / (phases[k] * 1e-3) / 1e9,
-               "frac": bytes_pp[k] * n / (phases[k] * 1e-3) / 1e9 / HBM_PEAK_GBS}
-           for k in bytes_pp if k in phases and not (k == "moments" and cfg.softening == "plummer")}
-    grav_tflops = None
-    if "gravity" in phases:
-        grav_tflops = FLOP_PER_INTERACTION * (st["grav_pp"] + st["grav_pn"]) / (phases["gravity"] * 1e-3) / 1e12
+    roof, extra = roofline(st, n)
     out = {
         "metric": "SPH particle-steps/sec on 1 GPU, mixed-precision vs FP64, at <=1e-3 accel error",
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32",
         "precision": "mixed: f32 pair/gravity arithmetic with Kahan f32 sums; f64 positions, velocities, "
-                     "centres of mass, h solve, reductions",
-        "data": "synthetic (seeded Evrard sphere)",
-        "config": {"workload": "evrard_collapse", "n": n, "M": 1.0, "R": 1.0, "G": 1.0, "u0": 0.05,
+                     "centres of mass, h solve, reductions; cooling rate by a bf16 (f32-accumulating) network "
+                     "with an f32 table fallback, f64 u update",
+        "data": "synthetic (seeded Evrard sphere, seeded per-particle Z and J21)",
+        "config": {"workload": "evrard_collapse + cooling", "n": n, "M": 1.0, "R": 1.0, "G": 1.0, "u0": 0.05,
                    "gamma": 5 / 3, "n_ngb": 64, "theta": 0.5, "softening": "plummer", "eps": 1e-3,
-                   "alpha": 1.0, "balsara": True, "dt": 1e-3},
-        "phases_ms": phases,
-        "phase_ms_per_step": per_step,
-        "h_solve_fp64_tflops": h_flops / (phases["hsolve"] * 1e-3) / 1e12 if "hsolve" in phases else None,
-        "neighbours_per_particle": kbar,
-        "neighbour_pairs_from_other_side": st["ngb_extra"],
-        "h_solve_rounds_last_step": st["h_rounds"],
-        "h_solve_walk_solved_targets_last_step": st["h_big"],
-        "gravity_interactions_per_particle": (st["grav_pp"] + st["grav_pn"]) / n,
-        "gravity_tflops": grav_tflops,
-        "pass_bandwidth": hbm,
+                   "alpha": 1.0, "balsara": True, "dt": 1e-3, "cooling": mode, "cooling_tol_dex": units.tol,
+                   "units": {"mass_g": 1e10 * M_SUN_G, "length_cm": 10 * KPC_CM}},
         "roofline": roof,
+        **extra,
+        "cooling_fallback_fraction_last_step": st["cool_fallback"] / n,
+        "cooling_floored_fraction_last_step": st["cool_floor"] / n,
     }
-    tr = traffic_table().get(roof["kernel"]) if n == 1048576 else None  # measured at this N only
-    if tr:
-        roof["traffic"] = tr["fetch_bytes"] + tr["write_bytes"]
-        roof["traffic_source"] = "profiles/hbm_traffic_evrard_1M.md"
-    if not args.no_mlp:
-        log("[bench] cooling surrogate: train, calibrate, time")
-        out["mlp_cooling"] = mlp_section(args.mlp_n, args.mlp_epochs)
     if not args.no_fp64_form:
-        sim64, dt64 = run("fp64", max(2, args.steps // 2), 1)
-        out["fp64_form"] = {"ms_per_step": 1e3 * dt64 / max(2, args.steps // 2),
-                            "mixed_speedup": (1e3 * dt64 / max(2, args.steps // 2)) / ms_step}
+        steps64 = max(2, args.steps // 2)
+        sim64, dt64 = run("fp64", steps64, 1, d=d, cooling=cooling("table"))
+        out["fp64_form"] = {"ms_per_step": 1e3 * dt64 / steps64, "mixed_speedup": (1e3 * dt64 / steps64) / ms_step,
+                            "cooling_mode": "table (no FP64 network)"}
         del sim64
-        log("[bench] acceleration error of the mixed form against the FP64 form")
-        out["fp64_form"]["accel_error"] = accel_error(d, cfg)
+        log("[bench] headline: acceleration of the mixed form against the FP64 form")
+        out["fp64_form"]["accel_error"] = accel_error(prime_states(d, cfg, cooling=cooling("table")))
     if not args.no_cpu_baseline:
-        g = g_mixed
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        log(f"[bench] cpu baseline: {args.cpu_rows} rows x {n} sources on {threads} threads")
-        step_s, sample_s, parts = cpu_baseline(g, n, d["m"], args.cpu_rows)
-        out["cpu_baseline"] = {
-            "value": n / step_s, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{args.cpu_rows} random target rows x all {n} sources of the FP64 all-pairs restatement "
-                      f"of sph_serial_v2.py's step (h_smooth_fast, getDensity, getAcc_g_smth, div_curlVel, "
-                      f"getAcc_sph_shear, get_dU_shear); {sample_s:.1f} s measured, extrapolated x{n / args.cpu_rows:.0f}",
-            "step_seconds_extrapolated": step_s,
-            "function_seconds_sample": parts,
-        }
+        out["cpu_baseline"] = cooling_cpu_baseline(args, cfg, g, d, Z, J, units)
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
-        log(f"[bench] cpu tree oracle at n = {n} on {threads} threads")
-        tree_s, tree_parts = cpu_tree_oracle(g, d["m"], cfg)
-        out["cpu_tree_oracle"] = {
-            "value": n / tree_s, "unit": "particle-steps/s", "cores": threads,
-            "kind": "oracle tree path (not the reference's algorithm): tree neighbours, CSR hydro, group-walk "
-                    "gravity at the device's h, without the h solve; every row",
-            "seconds": tree_s, "function_seconds": tree_parts,
-        }
+    del g, d
+    if mlp is not None:
+        out["mlp_cooling"] = mlp
+    # ---- the other configurations (each with its own roofline)
+    if not args.no_c2:
+        out["evrard_1m"] = c2_section(args, cfg, run)
+    if not args.no_c1 and not args.no_cpu_baseline:
         log("[bench] the cpu-runnable case, n = 10000")
         out["evrard_10k"] = c1_section(cfg, run)
-    out["file_io"] = io_section(g_mixed, d["m"])
-    if not args.no_column:
-        log("[bench] column density and time-step criteria in the step")
-        out["column_density"] = column_section(args, cfg, run, ms_step, kbar)
-    if not args.no_cooling_step:
-        out["evrard_cooling"] = cooling_section(args, cfg, run)
     if not args.no_agn:
         out["agn_outflow"] = agn_section(args, cfg, run)
     if not args.no_sedov:
