@@ -249,9 +249,31 @@ __device__ __forceinline__ double row_sum16(double v)
     return v;
 }
 
+// sqrt(x) of a squared distance x >= 0 for the Newton sums: the hardware reciprocal square
+// root refined by two Newton-Raphson corrections (error about 1 ulp; the solve needs relative
+// 1e-13), without the scaling and special-value handling of the correctly rounded library
+// square root, which x = 0 (the target itself: a select) and x in [1e-280, inf) do not need;
+// 0 < x < 1e-280 takes the library path.
+__device__ __forceinline__ double sqrt_dist2(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double s = x * y, hh = 0.5 * y;
+    const double e = fma(-hh, s, 0.5);
+    s = fma(s, e, s);
+    hh = fma(hh, e, hh);
+    double d = fma(-s, s, x);
+    s = fma(d, hh, s);
+    d = fma(-s, s, x);
+    s = fma(d, hh, s);
+    if (x > 0.0 && x < 1e-280) s = __builtin_sqrt(x);
+    return x > 0.0 ? s : 0.0;
+}
+
+#define HN_SLOTS 8  // candidates per lane held in registers (HN_LANES * HN_SLOTS per target)
+
 __global__ void __launch_bounds__(64) k_hnewton(HSolveArgs a)
 {
-    constexpr int HL = HN_LANES, HS = SPH_HCAP_SLOTS / HL;
+    constexpr int HL = HN_LANES, HS = HN_SLOTS, HR = HL * HS;
     const int lane = threadIdx.x, lsub = lane % HL;
     const int64_t j = (int64_t)blockIdx.x * (64 / HL) + lane / HL;
     const bool valid = j < a.m && a.state[j] == 0;
@@ -263,46 +285,48 @@ __global__ void __launch_bounds__(64) k_hnewton(HSolveArgs a)
     const int cj = solve ? cjs : 0;
     const double Rj = valid ? a.R[j] : 0.0, hj0 = valid ? a.h[j] : 0.0;
     const double *rowj = a.cand + (solve ? j : 0) * (int64_t)cap;
-    const int nsl = (cj + HL - 1) / HL;  // uniform within the sub-group
-    // the most slots a sub-group of the wave uses: the slot loops run to this wave-uniform
-    // bound (scalar branches); a slot past a target's own candidates holds 4 R, whose
-    // kernel terms are exactly zero for every h <= R/2
-    // (slots u >= nslw are neither loaded nor read)
-    int nslw = nsl;
-    for (int off = 32; off >= HL; off >>= 1) nslw = max(nslw, __shfl_xor(nslw, off));
-    nslw = __builtin_amdgcn_readfirstlane(nslw);
+    // the most candidates of a target in the wave (uniform): register slots in use, and
+    // whether some row continues past them (then read from the row in every evaluation)
+    int cw = cj;
+    for (int off = 32; off >= 1; off >>= 1) cw = max(cw, __shfl_xor(cw, off));
+    cw = __builtin_amdgcn_readfirstlane(cw);
+    const int nsl = min((cw + HL - 1) / HL, HS);
+    const bool tail = cw > HR;
     long long nterm = 0;
+    // slot u of lane lsub holds candidate lsub + HL u (static register indices); a slot past
+    // the target's candidates holds INFINITY, later 4 R, whose kernel terms are exactly zero
+    // for every h <= R/2
     double r[HS];
 #pragma unroll
     for (int u = 0; u < HS; u++) {
-        if (u >= nslw) break;
         const int k = lsub + HL * u;
-        r[u] = (k < cj) ? rowj[k] : INFINITY;  // squared distances
+        r[u] = INFINITY;
+        if (u < nsl && k < cj) r[u] = rowj[k];
     }
     double hnew = hj0, Rnew = Rj;
     int st = 0;
     if (solve && a.mode == 0) {
-        // distances; empty slots get 4 R, beyond the support of every h <= R/2
 #pragma unroll
-        for (int u = 0; u < HS; u++) {
-            if (u >= nslw) break;
-            r[u] = (r[u] < INFINITY) ? __builtin_sqrt(r[u]) : 4.0 * Rj;
-        }
+        for (int u = 0; u < HS; u++)
+            if (u < nsl) r[u] = (r[u] < INFINITY) ? sqrt_dist2(r[u]) : 4.0 * Rj;
         // f(h) = (32/3) sum_j w(r_j/h) - n_ngb and f'(h), with the branch-free
         // cubic spline w(q) = (2-q)_+^3/4 - (1-q)_+^3 and q w'(q) = q (3 (1-q)_+^2 - 3/4 (2-q)_+^2)
         auto feval = [&](double hh, double &f, double &df) {
             if (lsub == 0) nterm += cj;
             double sl = 0.0, sq = 0.0;
             const double ih = 1.0 / hh;
-#pragma unroll
-            for (int u = 0; u < HS; u++) {
-                if (u >= nslw) break;
-                const double q = r[u] * ih;
+            auto term = [&](double rr) {
+                const double q = rr * ih;
                 const double aa = fmax(2.0 - q, 0.0), bb = fmax(1.0 - q, 0.0);
                 const double a2 = aa * aa, b2 = bb * bb;
                 sl = fma(0.25 * a2, aa, fma(-b2, bb, sl));
                 sq = fma(fma(3.0, b2, -0.75 * a2), q, sq);
-            }
+            };
+#pragma unroll
+            for (int u = 0; u < HS; u++)
+                if (u < nsl) term(r[u]);
+            if (tail)
+                for (int k = HR + lsub; k < cj; k += HL) term(sqrt_dist2(rowj[k]));
             static_assert(HL == 16, "row_sum16 reduces rows of 16 lanes");
             sl = row_sum16(sl);
             sq = row_sum16(sq);
@@ -371,17 +395,20 @@ __global__ void __launch_bounds__(64) k_hnewton(HSolveArgs a)
             for (;;) {
                 double mn = INFINITY;
 #pragma unroll
-                for (int u = 0; u < HS; u++) {
-                    if (u >= nslw) break;
-                    if (r[u] > cur && r[u] < mn) mn = r[u];
-                }
+                for (int u = 0; u < HS; u++)
+                    if (u < nsl && r[u] > cur && r[u] < mn) mn = r[u];
+                if (tail)
+                    for (int k = HR + lsub; k < cj; k += HL) {
+                        const double x = rowj[k];
+                        if (x > cur && x < mn) mn = x;
+                    }
                 for (int off = HL / 2; off >= 1; off >>= 1) mn = fmin(mn, __shfl_xor(mn, off, HL));
                 int mult = 0;
 #pragma unroll
-                for (int u = 0; u < HS; u++) {
-                    if (u >= nslw) break;
-                    mult += (r[u] == mn);
-                }
+                for (int u = 0; u < HS; u++)
+                    if (u < nsl) mult += (r[u] == mn);
+                if (tail)
+                    for (int k = HR + lsub; k < cj; k += HL) mult += (rowj[k] == mn);
                 for (int off = HL / 2; off >= 1; off >>= 1) mult += __shfl_xor(mult, off, HL);
                 if (taken + mult >= need) { v = mn; break; }
                 taken += mult;
