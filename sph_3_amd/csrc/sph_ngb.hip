@@ -219,12 +219,13 @@ __global__ void __launch_bounds__(64) k_hwalk(HSolveArgs a)
 }
 
 // Newton (mode 0) or k-th distance (mode 2) solve of the pending targets from their
-// candidate rows, 4 targets per wavefront and 16 lanes per target: lane l of a
-// sub-group holds candidates l, l+16, ... (up to 16, in registers) and the sub-group
-// reduces with DPP lane exchanges, which give its 16 lanes bitwise the same sums, so each
-// sub-group's iteration is uniform.  A target whose row overflowed gets R * 0.75, one
-// whose solution lies beyond R/2 gets R * 1.5, and stays pending for another round.
-#define HN_LANES 16  // lanes per target of k_hnewton
+// candidate rows, 8 targets per wavefront and 8 lanes per target: lane l of a sub-group
+// holds candidates l, l+8, ... (up to 16 in registers; longer rows are read again from
+// memory in every evaluation) and the sub-group reduces with DPP lane exchanges, which give
+// its 8 lanes bitwise the same sums, so each sub-group's iteration is uniform.  A target
+// whose row overflowed gets R * 0.75, one whose solution lies beyond R/2 gets R * 1.5, and
+// stays pending for another round.
+#define HN_LANES 8  // lanes per target of k_hnewton
 
 // One DPP lane move of a double (two 32-bit moves); every source lane of the patterns used
 // here lies in the reading lane's own row of 16.
@@ -237,23 +238,26 @@ __device__ __forceinline__ double dpp_f64(double v)
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Sum over each row of 16 lanes with the same bits in every lane of the row: each step adds
-// a partner lane's value and partners exchange (IEEE addition commutes): xor 1 and xor 2
-// (quad permutes), then the mirror within 8 lanes and the mirror within 16 lanes.
-__device__ __forceinline__ double row_sum16(double v)
+// Sum over each aligned group of W = 4, 8 or 16 lanes with the same bits in every lane of
+// the group: each step adds a partner lane's value and partners exchange (IEEE addition
+// commutes): xor 1 and xor 2 (quad permutes), then (W >= 8) the mirror within 8 lanes and
+// (W = 16) the mirror within 16 lanes.
+template <int W>
+__device__ __forceinline__ double lane_group_sum(double v)
 {
+    static_assert(W == 4 || W == 8 || W == 16, "groups of 4, 8 or 16 lanes");
     v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
     v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
-    v += dpp_f64<0x141>(v);  // row_half_mirror
-    v += dpp_f64<0x140>(v);  // row_mirror
+    if constexpr (W >= 8) v += dpp_f64<0x141>(v);  // row_half_mirror
+    if constexpr (W == 16) v += dpp_f64<0x140>(v);  // row_mirror
     return v;
 }
 
-// sqrt(x) of a squared distance x >= 0 for the Newton sums: the hardware reciprocal square
+// sqrt(x) of a squared distance x > 0 for the Newton sums: the hardware reciprocal square
 // root refined by two Newton-Raphson corrections (error about 1 ulp; the solve needs relative
 // 1e-13), without the scaling and special-value handling of the correctly rounded library
-// square root, which x = 0 (the target itself: a select) and x in [1e-280, inf) do not need;
-// 0 < x < 1e-280 takes the library path.
+// square root.  Valid for x >= 0x1p-900 (callers take the library path for a wave holding a
+// smaller nonzero value); x = 0 (the target itself) gives 0.
 __device__ __forceinline__ double sqrt_dist2(double x)
 {
     const double y = __builtin_amdgcn_rsq(x);
@@ -265,11 +269,21 @@ __device__ __forceinline__ double sqrt_dist2(double x)
     s = fma(d, hh, s);
     d = fma(-s, s, x);
     s = fma(d, hh, s);
-    if (x > 0.0 && x < 1e-280) s = __builtin_sqrt(x);
     return x > 0.0 ? s : 0.0;
 }
 
-#define HN_SLOTS 8  // candidates per lane held in registers (HN_LANES * HN_SLOTS per target)
+// 1/x for x > 0 in the normal range: the hardware reciprocal refined by two Newton-Raphson
+// corrections (within 1 ulp), instead of the correctly rounded division sequence.
+__device__ __forceinline__ double rcp_refined(double x)
+{
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+
+#define HN_SLOTS 16  // candidates per lane held in registers (HN_LANES * HN_SLOTS per target)
 
 __global__ void __launch_bounds__(64) k_hnewton(HSolveArgs a)
 {
@@ -306,15 +320,26 @@ __global__ void __launch_bounds__(64) k_hnewton(HSolveArgs a)
     double hnew = hj0, Rnew = Rj;
     int st = 0;
     if (solve && a.mode == 0) {
+        // distances; empty slots get 4 R, beyond the support of every h <= R/2
+        bool tiny = false;
 #pragma unroll
         for (int u = 0; u < HS; u++)
-            if (u < nsl) r[u] = (r[u] < INFINITY) ? sqrt_dist2(r[u]) : 4.0 * Rj;
+            if (u < nsl) tiny = tiny || (r[u] > 0.0 && r[u] < 0x1p-900);
+        if (__ballot(tiny)) {
+#pragma unroll
+            for (int u = 0; u < HS; u++)
+                if (u < nsl) r[u] = (r[u] < INFINITY) ? __builtin_sqrt(r[u]) : 4.0 * Rj;
+        } else {
+#pragma unroll
+            for (int u = 0; u < HS; u++)
+                if (u < nsl) r[u] = (r[u] < INFINITY) ? sqrt_dist2(r[u]) : 4.0 * Rj;
+        }
         // f(h) = (32/3) sum_j w(r_j/h) - n_ngb and f'(h), with the branch-free
         // cubic spline w(q) = (2-q)_+^3/4 - (1-q)_+^3 and q w'(q) = q (3 (1-q)_+^2 - 3/4 (2-q)_+^2)
         auto feval = [&](double hh, double &f, double &df) {
             if (lsub == 0) nterm += cj;
             double sl = 0.0, sq = 0.0;
-            const double ih = 1.0 / hh;
+            const double ih = rcp_refined(hh);
             auto term = [&](double rr) {
                 const double q = rr * ih;
                 const double aa = fmax(2.0 - q, 0.0), bb = fmax(1.0 - q, 0.0);
@@ -326,10 +351,9 @@ __global__ void __launch_bounds__(64) k_hnewton(HSolveArgs a)
             for (int u = 0; u < HS; u++)
                 if (u < nsl) term(r[u]);
             if (tail)
-                for (int k = HR + lsub; k < cj; k += HL) term(sqrt_dist2(rowj[k]));
-            static_assert(HL == 16, "row_sum16 reduces rows of 16 lanes");
-            sl = row_sum16(sl);
-            sq = row_sum16(sq);
+                for (int k = HR + lsub; k < cj; k += HL) term(__builtin_sqrt(rowj[k]));
+            sl = lane_group_sum<HL>(sl);
+            sq = lane_group_sum<HL>(sq);
             f = (32.0 / 3.0) * sl - a.n_ngb;
             df = -(32.0 / 3.0) * sq * ih;
         };
@@ -353,7 +377,7 @@ __global__ void __launch_bounds__(64) k_hnewton(HSolveArgs a)
                 hi = hh;
                 hik = true;
             }
-            double hn = (df > 0.0) ? hh - f / df : -1.0;
+            double hn = (df > 0.0) ? hh - f * rcp_refined(df) : -1.0;
             bool newton = true;
             if (!(hn > lo && hn < hi)) {
                 if (!hik) {
@@ -364,7 +388,7 @@ __global__ void __launch_bounds__(64) k_hnewton(HSolveArgs a)
                         break;
                     }
                     hik = true;
-                    hn = (dfm > 0.0) ? hmaxv - fm / dfm : -1.0;
+                    hn = (dfm > 0.0) ? hmaxv - fm * rcp_refined(dfm) : -1.0;
                     if (!(hn > lo && hn < hi)) hn = 0.5 * (lo + hi);
                 } else {
                     hn = 0.5 * (lo + hi);
