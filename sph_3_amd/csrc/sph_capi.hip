@@ -129,6 +129,7 @@ struct sph_ctx {
     // h solve scratch
     Buf cand, hR, hsteps, hstate, hcount, hout, hin, tx, ty, tz;
     Buf ccidx, cccnt, cext, cnext;  // candidate indices, counts, extra pairs and their region counters
+    Buf chr;                        // (h_j, R_j) packed for the rows from candidates
     Buf hbig;                       // targets of the candidate-free h solver
     // staging and reductions
     Buf stage, kpart, kout;
@@ -597,7 +598,7 @@ extern "C" int sph_ctx_destroy(sph_ctx *ctx)
                   &ctx->divV, &ctx->curlV, &ctx->bal, &ctx->du, &ctx->cand, &ctx->hR, &ctx->hsteps,
                   &ctx->hstate, &ctx->hcount, &ctx->hout, &ctx->hin, &ctx->tx, &ctx->ty, &ctx->tz,
                   &ctx->stage, &ctx->kpart, &ctx->kout, &ctx->spread, &ctx->ccidx, &ctx->cccnt,
-                  &ctx->cext, &ctx->cnext, &ctx->ttab, &ctx->zmet, &ctx->j21, &ctx->ucols, &ctx->uyn,
+                  &ctx->cext, &ctx->cnext, &ctx->chr, &ctx->ttab, &ctx->zmet, &ctx->j21, &ctx->ucols, &ctx->uyn,
                   &ctx->uL, &ctx->uG, &ctx->uflag, &ctx->ucnt, &ctx->hbig, &ctx->p4, &ctx->v4, &ctx->s4,
                   &ctx->colg, &ctx->coln, &ctx->tsdt, &ctx->tsdh, &ctx->phi, &ctx->tpart, &ctx->tout};
     for (Buf *b : all)
@@ -1277,6 +1278,7 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     const int64_t ext_cap = std::max<int64_t>(4 * n, SPH_SPREAD * 64);
     CK(ensure(ctx, ctx->cext, (size_t)ext_cap * 8));
     CK(ensure(ctx, ctx->cnext, (size_t)SPH_SPREAD_WORDS * 8));
+    CK(ensure(ctx, ctx->chr, (size_t)n * 16));
     ph_begin(ctx, SPH_PH_NGB);
     sph::BigRows big;  // targets of the candidate-free h solver: rows by a walk each
     big.list = I32(hbig);
@@ -1293,7 +1295,7 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
         st, n, SPH_HCAP, D(cand), I32(ccidx), I32(cccnt), D(h), D(hR), SPH_NGB_STRIDE, I32(cnt), I32(rcnt0), I32(idx),
         P_<int2>(ctx->cext), ext_cap, P_<unsigned long long>(ctx->cnext),
         P_<unsigned long long>(ctx->spread) + SPR_N * SPH_SPREAD_WORDS, P_<unsigned long long>(ctx->hcount) + 10,
-        ctx->hpin + 8, ctx->ev_n, big));
+        ctx->hpin + 8, ctx->ev_n, big, P_<double2>(ctx->chr)));
     ph_end(ctx, SPH_PH_NGB);
     ctx->ngb_stride = SPH_NGB_STRIDE;
     ctx->ngb_per = p->period;

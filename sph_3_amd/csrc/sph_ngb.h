@@ -94,6 +94,7 @@ hipError_t neighbour_rows(hipStream_t st, int64_t ngroups, const int32_t *groups
 // totals and region counters into host_words (pinned, 2 + SPH_SPREAD words) followed by
 // done; _end waits for done (not the stream) and evaluates them.
 // big: the targets of the candidate-free h solver (their rows are built by a walk each).
+// hr: scratch of n double2 (h_j and R_j per source, packed for one gather per candidate).
 struct BigRows {
     const int32_t *list = nullptr;
     int64_t count = 0;
@@ -109,7 +110,7 @@ hipError_t neighbour_rows_from_candidates_begin(hipStream_t st, int64_t n, int c
                                                int32_t *idx, int2 *ext, int64_t ext_cap, unsigned long long *next,
                                                unsigned long long *spread, unsigned long long *totals,
                                                unsigned long long *host_words, hipEvent_t done,
-                                               const BigRows &big);
+                                               const BigRows &big, double2 *hr);
 hipError_t neighbour_rows_from_candidates_end(hipEvent_t done, const unsigned long long *host_words, int64_t ext_cap,
                                              int64_t *n_long, int64_t *total, int64_t *n_extra);
 // *out (device) = max_i h_new[i] / h_old[i], rounded up by 8 ulp: the factor that

@@ -143,6 +143,8 @@ __global__ void __launch_bounds__(64) k_hwalk(HSolveArgs a)
     ttx[lane] = xi; tty[lane] = yi; ttz[lane] = zi;
     tR2[lane] = valid ? R2 : -1.0;
     const int cap = a.cap;
+    double *const cbase = a.cand + first * (int64_t)cap;  // rows of the group's targets
+    int32_t *const ibase = a.cidx ? a.cidx + first * (int64_t)cap : nullptr;
     int c = 0;  // lane t: candidates of target t so far
     long long ntest = 0;  // algorithmic work: distance tests
     // Candidates: lanes take the staged sources, the group's targets in turn (their
@@ -187,12 +189,13 @@ __global__ void __launch_bounds__(64) k_hwalk(HSolveArgs a)
                     need = __ballot(valid && e2 < R2);
                 }
                 if (lane < 64 && ((need >> lane) & 1)) ntest += nsrc;  // distance tests performed
-                for (uint64_t todo = need; todo; todo &= todo - 1) {
-                    const int tt = __builtin_ctzll(todo);
-                    const double dx = wrap_d<PER>(px - ttx[tt], a.per), dy = wrap_d<PER>(py - tty[tt], a.per),
-                                 dz = wrap_d<PER>(pz - ttz[tt], a.per);
+                // targets in turn; the next target's coordinates are read from LDS while the
+                // current one is tested (two register sets used alternately)
+                auto test = [&](int tt, double qx, double qy, double qz, double qr) {
+                    const double dx = wrap_d<PER>(px - qx, a.per), dy = wrap_d<PER>(py - qy, a.per),
+                                 dz = wrap_d<PER>(pz - qz, a.per);
                     const double r2 = dx * dx + dy * dy + dz * dz;
-                    const bool hit = src && r2 < tR2[tt];
+                    const bool hit = src && r2 < qr;
                     const uint64_t hm = __ballot(hit);
                     if (hm) {
                         const int ct = __builtin_amdgcn_readlane(c, tt);
@@ -201,13 +204,30 @@ __global__ void __launch_bounds__(64) k_hwalk(HSolveArgs a)
                                 (uint32_t)ct + __builtin_amdgcn_mbcnt_hi((uint32_t)(hm >> 32),
                                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)hm, 0u));
                             if (pos < (uint32_t)cap) {
-                                // uniform row base + a 32-bit byte offset (scalar-base addressing)
-                                const int64_t row = (first + tt) * (int64_t)cap;
-                                *(double *)((char *)(a.cand + row) + pos * 8u) = r2;
-                                if (a.cidx) *(int32_t *)((char *)(a.cidx + row) + pos * 4u) = pj;
+                                // the group's uniform row base + a 32-bit offset (scalar-base addressing)
+                                const uint32_t e = (uint32_t)(tt * cap) + pos;
+                                *(double *)((char *)cbase + e * 8u) = r2;
+                                if (a.cidx) *(int32_t *)((char *)ibase + e * 4u) = pj;
                             }
                         }
                         c = (lane == tt) ? ct + (int)__popcll(hm) : c;
+                    }
+                };
+                uint64_t todo = need;
+                if (todo) {
+                    int ta = __builtin_ctzll(todo);
+                    double ax = ttx[ta], ay = tty[ta], az = ttz[ta], ar = tR2[ta];
+                    for (;;) {
+                        todo &= todo - 1;
+                        const int tb = todo ? __builtin_ctzll(todo) : ta;
+                        const double bx = ttx[tb], by = tty[tb], bz = ttz[tb], br = tR2[tb];
+                        test(ta, ax, ay, az, ar);
+                        if (!todo) break;
+                        todo &= todo - 1;
+                        ta = todo ? __builtin_ctzll(todo) : tb;
+                        ax = ttx[ta]; ay = tty[ta]; az = ttz[ta]; ar = tR2[ta];
+                        test(tb, bx, by, bz, br);
+                        if (!todo) break;
                     }
                 }
             }
@@ -687,11 +707,21 @@ __global__ void __launch_bounds__(64) k_ngb(const int32_t *__restrict__ groups, 
 // k_row_tail_sort puts each row's appended part in ascending order, so a row's
 // order (and every sum over it) does not depend on the order of the appends.
 // Spread counters: 0 += rows longer than stride, 1 += entries.
-#define CR_UNROLL 3  // candidates per lane step of k_cand_rows
+// hr[j] = (h_j, R_j), R_j replaced by -1 for a row of k_hsolve_big's (incomplete: such a j's
+// pairs are found by its own walk, k_big_rows), so one gather gives k_cand_rows both.
+__global__ void __launch_bounds__(256) k_pack_hr(int64_t n, int cap, const double *__restrict__ h,
+                                                 const double *__restrict__ R, const int32_t *__restrict__ ccnt,
+                                                 double2 *__restrict__ hr)
+{
+    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j < n) hr[j] = make_double2(h[j], ccnt[j] <= cap ? R[j] : -1.0);
+}
+
+#define CR_UNROLL 5  // candidates per lane step of k_cand_rows
 template <int G>
 __global__ void __launch_bounds__(256) k_cand_rows(int64_t n, int cap, const double *__restrict__ cand,
                                                    const int32_t *__restrict__ cidx, const int32_t *__restrict__ ccnt,
-                                                   const double *__restrict__ h, const double *__restrict__ R,
+                                                   const double *__restrict__ h, const double2 *__restrict__ hr,
                                                    int stride, int32_t *__restrict__ cnt, int32_t *__restrict__ cnt0,
                                                    int32_t *__restrict__ idx, int2 *__restrict__ ext, int64_t ext_cap,
                                                    unsigned long long *__restrict__ next,
@@ -712,15 +742,13 @@ __global__ void __launch_bounds__(256) k_cand_rows(int64_t n, int cap, const dou
     int32_t *row = idx + (valid ? i : 0) * (int64_t)stride;
     int w = 0;
     // one candidate of the row: member test, ballot-compacted append, extra pair
-    auto visit = [&](bool have, int j, double r2, double hj) {
+    auto visit = [&](bool have, int j, double r2, double hj, double Rj) {
         bool in = false, extra = false;
         if (have) {
             const double t = hi + hj, t2 = t * t;
             in = r2 < t2;
-            if (in && hi > hj && ccnt[j] <= cap) {  // (a row of k_hsolve_big's is walked: k_big_rows)
-                const double Rj = R[j];
-                extra = r2 >= Rj * Rj;
-            }
+            // (Rj < 0: a row of k_hsolve_big's, whose pairs k_big_rows finds by a walk)
+            extra = in && hi > hj && Rj > 0.0 && r2 >= Rj * Rj;
         }
         const uint64_t mb = __ballot(in) & gmask;
         if (in) {
@@ -734,12 +762,13 @@ __global__ void __launch_bounds__(256) k_cand_rows(int64_t n, int cap, const dou
             if (e < (unsigned long long)rcap) ext[reg * rcap + (int64_t)e] = make_int2(j, (int)i);
         }
     };
-    // candidates k0 + l + u G (u < CR_UNROLL) per step: all their indices, distances and h_j
-    // are loaded before any is visited (CR_UNROLL dependent gathers in flight per lane),
+    // candidates k0 + l + u G (u < CR_UNROLL) per step: all their indices, distances and
+    // (h_j, R_j) are loaded before any is visited (CR_UNROLL gathers in flight per lane),
     // visits in row order
     for (int k0 = 0; k0 < c; k0 += CR_UNROLL * G) {
         int jv[CR_UNROLL];
-        double rv[CR_UNROLL], hv[CR_UNROLL];
+        double rv[CR_UNROLL];
+        double2 hv[CR_UNROLL];
 #pragma unroll
         for (int u = 0; u < CR_UNROLL; u++) {
             const int k = k0 + l + u * G;
@@ -747,9 +776,10 @@ __global__ void __launch_bounds__(256) k_cand_rows(int64_t n, int cap, const dou
             rv[u] = k < c ? rr[k] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < CR_UNROLL; u++) hv[u] = (k0 + l + u * G < c) ? h[jv[u]] : 0.0;
+        for (int u = 0; u < CR_UNROLL; u++)
+            hv[u] = (k0 + l + u * G < c) ? hr[jv[u]] : make_double2(0.0, 0.0);
 #pragma unroll
-        for (int u = 0; u < CR_UNROLL; u++) visit(k0 + l + u * G < c, jv[u], rv[u], hv[u]);
+        for (int u = 0; u < CR_UNROLL; u++) visit(k0 + l + u * G < c, jv[u], rv[u], hv[u].x, hv[u].y);
     }
     // the row and counts of an incomplete candidate row are written by k_big_rows
     const bool own = valid && l == 0 && !incomplete;
@@ -919,12 +949,13 @@ hipError_t neighbour_rows_from_candidates_begin(hipStream_t st, int64_t n, int c
                                                int32_t *idx, int2 *ext, int64_t ext_cap, unsigned long long *next,
                                                unsigned long long *spread, unsigned long long *totals,
                                                unsigned long long *host_words, hipEvent_t done,
-                                               const BigRows &big)
+                                               const BigRows &big, double2 *hr)
 {
     if (stride < 1 || cap < 1 || ext_cap < SPH_SPREAD) return hipErrorInvalidValue;
     fill<unsigned long long>(st, SPH_SPREAD_WORDS, next, 0ull);
     fill<unsigned long long>(st, 2 * SPH_SPREAD_WORDS, spread, 0ull);
-    k_cand_rows<16><<<sph_blocks(n * 16, 256), 256, 0, st>>>(n, cap, cand, cidx, ccnt, h, R, stride, cnt, cnt0, idx,
+    k_pack_hr<<<sph_blocks(n, 256), 256, 0, st>>>(n, cap, h, R, ccnt, hr);
+    k_cand_rows<16><<<sph_blocks(n * 16, 256), 256, 0, st>>>(n, cap, cand, cidx, ccnt, h, hr, stride, cnt, cnt0, idx,
                                                             ext, ext_cap, next, spread);
     if (big.count > 0) {
         const unsigned nb = (unsigned)big.count;

@@ -18,7 +18,7 @@
 #include "sph_common.h"
 
 #define WSTK 512   // LDS stack capacity (node ids)
-#define WSRC 128   // staged sources before a flush (< 64 + one bucket of <= 64)
+#define WSRC 64    // staged sources before a flush (one chunk of the 64-lane evaluations)
 
 // Box of the group's targets by wave reduction (lanes >= count pass +-inf).
 __device__ __forceinline__ void group_box(bool valid, double x, double y, double z, double *gb)
