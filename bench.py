@@ -320,7 +320,7 @@ def mlp_section(n_inf, epochs=200):
         "fallback_fraction": tm["fallback_fraction"], "tol_dex": tol,
         "rms_error_dex": {"gated": rms(lg - exact), "table": rms(lt - exact)},
         "bytes": {"network": net.nbytes, "table_log_lambda": 4 * int(np.prod([r[2] for r in C.TABLE_RANGES]))},
-        "roofline": {"kernel": "k_mlp_forward<true>", "bound": "mfma", "achieved": fwd,
+        "roofline": {"kernel": "k_mlp_forward", "bound": "mfma", "achieved": fwd,
                      "peak": BF16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": fwd / BF16_MFMA_PEAK_TFLOPS,
                      "algorithmic": f"{MLP_FLOP_PER_SAMPLE} flop x {n_inf} samples",
                      "ms_per_launch": tm["ms_network"]},

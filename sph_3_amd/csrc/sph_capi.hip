@@ -160,7 +160,7 @@ struct sph_ctx {
     Buf tsdt, tsdh;         // dt_i, dh/dt_i (sorted order, FP64)
     sph_cool_params cool_p{};
     // cooling surrogate: parameters, Adam state, BF16 weight copies, samples, error map
-    Buf mP, mG, mM, mV, mW[4], mx, mxn, mt, my, mloss, memap, memap2, memap3, mgate, mcnt;
+    Buf mP, mG, mM, mV, mW[4], mx, mxn, mt, my, mloss, memap, memap2, memap3, mgate, mcnt, mfl;
     MLPNorm mnorm{};
     bool mlp_ok = false, mlp_cal = false;
     int mlp_step = 0;
@@ -609,7 +609,7 @@ extern "C" int sph_ctx_destroy(sph_ctx *ctx)
                    &ctx->cin[4], &ctx->cin[5], &ctx->cout[0], &ctx->cout[1], &ctx->mP,     &ctx->mG,     &ctx->mM,
                    &ctx->mV,     &ctx->mW[0],  &ctx->mW[1],  &ctx->mW[2],  &ctx->mW[3],  &ctx->mx,     &ctx->mxn,
                    &ctx->mt,     &ctx->my,     &ctx->mloss,  &ctx->memap,  &ctx->memap2, &ctx->memap3,
-                   &ctx->mgate,  &ctx->mcnt};
+                   &ctx->mgate,  &ctx->mcnt,   &ctx->mfl};
     for (Buf *b : cool)
         if (b->p) (void)hipFree(b->p);
     for (int k = 0; k < 2 * SPH_PH_COUNT; k++) (void)hipEventDestroy(ctx->ev[k]);
@@ -2052,8 +2052,10 @@ extern "C" int sph_mlp_cooling(sph_ctx *ctx, int64_t n, const double *lognH, con
     float *oL = (float *)ctx->cout[0].p;
     int *flag = (int *)ctx->cout[1].p;
     HIPCK(hipMemsetAsync(ctx->mcnt.p, 0, 8, ctx->st));
+    CK(ensure(ctx, ctx->mfl, (size_t)sph::mlp_infer_scratch_ints(n) * 4));
     HIPCK(sph::mlp_infer(ctx->st, n, cols, ctx->mnorm, mlp_dev(ctx), emap, ctx->mcells, (float)tol, ctx->cax,
-                         P_<float>(ctx->ctabL), P_<float>(ctx->my), oL, flag, P_<unsigned long long>(ctx->mcnt)));
+                         P_<float>(ctx->ctabL), P_<float>(ctx->my), oL, flag, P_<unsigned long long>(ctx->mcnt),
+                         I32(mfl)));
     k_f2d<<<sph_blocks(n, 256), 256, 0, ctx->st>>>(n, oL, D(cin[0]));
     HIPCK(hipMemcpyAsync(logL_out, ctx->cin[0].p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->st));
     if (fallback_out)
@@ -2080,6 +2082,7 @@ extern "C" int sph_mlp_timing(sph_ctx *ctx, int64_t n, int reps, uint64_t seed, 
     float *oL = (float *)ctx->cout[0].p;
     int *flag = (int *)ctx->cout[1].p;
     const MLPDev d = mlp_dev(ctx);
+    CK(ensure(ctx, ctx->mfl, (size_t)sph::mlp_infer_scratch_ints(n) * 4));
     hipEvent_t e0, e1;
     HIPCK(hipEventCreate(&e0));
     HIPCK(hipEventCreate(&e1));
@@ -2091,7 +2094,7 @@ extern "C" int sph_mlp_timing(sph_ctx *ctx, int64_t n, int reps, uint64_t seed, 
                 HIPCK(hipMemsetAsync(ctx->mcnt.p, 0, 8, ctx->st));
                 HIPCK(sph::mlp_infer(ctx->st, n, cols, ctx->mnorm, d, P_<float>(ctx->mgate), ctx->mcells, (float)tol,
                                      ctx->cax, P_<float>(ctx->ctabL), P_<float>(ctx->mt), oL, flag,
-                                     P_<unsigned long long>(ctx->mcnt)));
+                                     P_<unsigned long long>(ctx->mcnt), I32(mfl)));
             } else if (which == 1) {
                 HIPCK(sph::cool_table_lookup(ctx->st, n, ctx->cax, P_<float>(ctx->ctabL), nullptr, cols, cols + n,
                                              cols + 2 * n, cols + 3 * n, oL, nullptr));
@@ -2209,9 +2212,10 @@ int cool_run(sph_ctx *ctx, int64_t n, const RHO *rho, const double *u, const dou
                                     P_<float>(ctx->ctabG), ctx->tax, tabT, D(rf1), nullptr, cols,
                                     P_<float>(ctx->uG), nullptr));
         const float *emap = cp.ref == SPH_MLP_REF_EXACT ? P_<float>(ctx->mgate) : P_<float>(ctx->memap);
+        CK(ensure(ctx, ctx->mfl, (size_t)sph::mlp_infer_scratch_ints(n) * 4));
         HIPCK(sph::mlp_infer(st, n, cols, ctx->mnorm, mlp_dev(ctx), emap, ctx->mcells, (float)cp.tol, ctx->cax,
                              P_<float>(ctx->ctabL), P_<float>(ctx->uyn), P_<float>(ctx->uL), P_<int>(ctx->uflag),
-                             counts + 1));
+                             counts + 1, I32(mfl)));
         HIPCK(sph::cool_apply<RHO>(st, n, rho, u, P_<float>(ctx->uL), P_<float>(ctx->uG), cu, u_out, spr));
     }
     HIPCK(sph::spread_total(st, 1, spr, counts));

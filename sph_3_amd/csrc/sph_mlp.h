@@ -26,10 +26,13 @@ namespace sph {
 // Network output (normalised) for n normalised inputs x[n][4].
 hipError_t mlp_forward_raw(hipStream_t st, int64_t n, const float *x, const MLPDev &d, float *y);
 // Gated inference at log10 columns x[a*n + i] (a = nH, T, Z, J21): the network inside the box
-// where the cell map emap <= tol, the table elsewhere; yn[n] is scratch (network output).
+// where the cell map emap <= tol, the table elsewhere; yn[n] receives the network output,
+// flag[n] (may be null) 1 where the table was used, *nfall += the number of those.
+// scratch: mlp_infer_scratch_ints(n) int32 (the compacted list of table samples).
 hipError_t mlp_infer(hipStream_t st, int64_t n, const float *x, const MLPNorm &nm, const MLPDev &d,
                      const float *emap, const int *ec, float tol, const CoolAxes &ax, const float *tabL, float *yn,
-                     float *outL, int *flag, unsigned long long *nfall);
+                     float *outL, int *flag, unsigned long long *nfall, int32_t *scratch);
+inline int64_t mlp_infer_scratch_ints(int64_t n) { return n + 64; }
 // Gradient of the batch MSE (samples src = (mult*j + off) mod n, j = first..first+bsz-1) added to d.G;
 // loss (batch mean) added to *loss.
 hipError_t mlp_train_batch(hipStream_t st, int64_t n, int64_t first, int64_t bsz, int64_t mult, int64_t off,

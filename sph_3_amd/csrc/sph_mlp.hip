@@ -27,6 +27,9 @@
 // and B[4(l/32)+i][l%32], i = 0..3; accumulator r holds
 // D[8(r/4) + 4(l/32) + r%4][l%32], r = 0..15.
 #include "sph_mlp.h"
+#include "sph_order.h"
+
+#include <algorithm>
 
 #include <hip/amd_detail/amd_hip_unsafe_atomics.h>
 
@@ -255,24 +258,17 @@ __device__ __forceinline__ float mlp_forward64(const float xin[4], const float *
     return (half ? y[1] : y[0]) + P[O_B4];
 }
 
-// Network output for rows of normalised inputs (COLS = false: x[n][4]) or for
-// log10 columns x[a*n + i] normalised here with nm (COLS = true).
-template <bool COLS>
+// Network output for rows of normalised inputs x[n][4].
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
-k_mlp_forward(int64_t n, const float *__restrict__ x, MLPNorm nm, const float *__restrict__ P,
+k_mlp_forward(int64_t n, const float *__restrict__ x, const float *__restrict__ P,
               const unsigned short *__restrict__ W2b, const unsigned short *__restrict__ W3b, float *__restrict__ y)
 {
     __shared__ __align__(16) unsigned short act[ITILE];
     const int64_t s = blockIdx.x * (int64_t)IS + threadIdx.x;
     float xin[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     if (s < n) {
-        if (COLS) {
-#pragma unroll
-            for (int k = 0; k < 4; k++) xin[k] = 2.0f * (x[k * n + s] - nm.lo[k]) / (nm.hi[k] - nm.lo[k]) - 1.0f;
-        } else {
-            const float4 v = *reinterpret_cast<const float4 *>(x + 4 * s);
-            xin[0] = v.x; xin[1] = v.y; xin[2] = v.z; xin[3] = v.w;
-        }
+        const float4 v = *reinterpret_cast<const float4 *>(x + 4 * s);
+        xin[0] = v.x; xin[1] = v.y; xin[2] = v.z; xin[3] = v.w;
     }
     const float out = mlp_forward64(xin, P, W2b, W3b, act);
     if (s < n) y[s] = out;
@@ -292,34 +288,67 @@ __device__ __forceinline__ int mlp_cell(const float v[4], const MLPNorm &nm, con
     return id;
 }
 
-// Gate of the inference at log10 columns x[a*n + i]: the network's value
-// (from yn) where the input is in the box and its cell's map value <= tol,
-// the table's interpolation elsewhere.
-__global__ void __launch_bounds__(256) k_mlp_gate(int64_t n, const float *__restrict__ x, const float *__restrict__ yn,
-                                                  MLPNorm nm, const float *__restrict__ emap, int ec0, int ec1, int ec2,
-                                                  int ec3, float tol, CoolAxes ax, const float *__restrict__ tabL,
-                                                  float *__restrict__ outL, int *__restrict__ flag,
-                                                  unsigned long long *__restrict__ nfall)
+// Gated inference of samples blockIdx.x*64 + lane from log10 columns x[a*n + i]: the network's
+// value, and the gate (outside the training box, or a cell whose map value exceeds tol: the
+// table instead) evaluated in the same wavefront; the samples that take the table are appended
+// to list (ballot-compacted, one atomic per wavefront) for k_mlp_fallback, so the table's
+// gathers run on full wavefronts of fallback samples only.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+k_mlp_infer(int64_t n, const float *__restrict__ x, MLPNorm nm, const float *__restrict__ P,
+            const unsigned short *__restrict__ W2b, const unsigned short *__restrict__ W3b,
+            const float *__restrict__ emap, int ec0, int ec1, int ec2, int ec3, float tol, float *__restrict__ yn,
+            float *__restrict__ outL, int *__restrict__ flag, int32_t *__restrict__ list,
+            unsigned long long *__restrict__ nlist)
 {
-    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    __shared__ __align__(16) unsigned short act[ITILE];
+    const int64_t s = blockIdx.x * (int64_t)IS + threadIdx.x;
     const bool valid = s < n;
+    float v[4] = {0.0f, 0.0f, 0.0f, 0.0f}, xin[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (valid)
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            v[k] = x[k * n + s];
+            xin[k] = 2.0f * (v[k] - nm.lo[k]) / (nm.hi[k] - nm.lo[k]) - 1.0f;
+        }
+    const float out = mlp_forward64(xin, P, W2b, W3b, act);
     bool fb = false;
     if (valid) {
-        const float v[4] = {x[s], x[n + s], x[2 * n + s], x[3 * n + s]};
         const int nc[4] = {ec0, ec1, ec2, ec3};
         bool inside;
         const int cell = mlp_cell(v, nm, nc, inside);
         fb = !inside || !(emap[cell] <= tol);
-        float res = fmaf(yn[s], nm.sigma, nm.mu);
-        if (fb) {
-            float g;
-            cool_table_interp(ax, tabL, nullptr, v, res, g);
-        }
-        outL[s] = res;
+        yn[s] = out;
+        outL[s] = fmaf(out, nm.sigma, nm.mu);
         if (flag) flag[s] = fb ? 1 : 0;
     }
-    const uint64_t nb = __ballot(fb);
-    if ((threadIdx.x & 63) == 0 && nb) atomicAdd(nfall, (unsigned long long)__popcll(nb));
+    const uint64_t m = __ballot(fb);
+    if (m) {
+        unsigned long long base = 0;
+        if (threadIdx.x == 0) base = atomicAdd(nlist, (unsigned long long)__popcll(m));
+        base = __shfl(base, 0);
+        if (fb)
+            list[base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] =
+                (int32_t)s;
+    }
+}
+
+// Table interpolation of the listed samples (grid-stride over the device count); the
+// count is then added to *nfall.
+__global__ void __launch_bounds__(256) k_mlp_fallback(int64_t n, const float *__restrict__ x, CoolAxes ax,
+                                                      const float *__restrict__ tabL, const int32_t *__restrict__ list,
+                                                      const unsigned long long *__restrict__ nlist,
+                                                      float *__restrict__ outL, unsigned long long *__restrict__ nfall)
+{
+    const int64_t cnt = (int64_t)*nlist;
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    for (int64_t k = g; k < cnt; k += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = list[k];
+        const float v[4] = {x[s], x[n + s], x[2 * n + s], x[3 * n + s]};
+        float res = 0.0f, gg;
+        cool_table_interp(ax, tabL, nullptr, v, res, gg);
+        outL[s] = res;
+    }
+    if (g == 0 && cnt) atomicAdd(nfall, (unsigned long long)cnt);
 }
 
 // ---------------------------------------------------------------- training
@@ -563,17 +592,22 @@ namespace sph {
 
 hipError_t mlp_forward_raw(hipStream_t st, int64_t n, const float *x, const MLPDev &d, float *y)
 {
-    k_mlp_forward<false><<<(unsigned)((n + IS - 1) / IS), 64, 0, st>>>(n, x, MLPNorm{}, d.P, d.W2b, d.W3b, y);
+    k_mlp_forward<<<(unsigned)((n + IS - 1) / IS), 64, 0, st>>>(n, x, d.P, d.W2b, d.W3b, y);
     return hipGetLastError();
 }
 
 hipError_t mlp_infer(hipStream_t st, int64_t n, const float *x, const MLPNorm &nm, const MLPDev &d,
                      const float *emap, const int *ec, float tol, const CoolAxes &ax, const float *tabL, float *yn,
-                     float *outL, int *flag, unsigned long long *nfall)
+                     float *outL, int *flag, unsigned long long *nfall, int32_t *scratch)
 {
-    k_mlp_forward<true><<<(unsigned)((n + IS - 1) / IS), 64, 0, st>>>(n, x, nm, d.P, d.W2b, d.W3b, yn);
-    k_mlp_gate<<<sph_blocks(n, 256), 256, 0, st>>>(n, x, yn, nm, emap, ec[0], ec[1], ec[2], ec[3], tol, ax, tabL, outL,
-                                                   flag, nfall);
+    if (n < 1) return hipSuccess;
+    unsigned long long *nlist = reinterpret_cast<unsigned long long *>(scratch);  // 64-int header: the count
+    int32_t *list = scratch + 64;
+    fill<unsigned long long>(st, 1, nlist, 0ull);
+    k_mlp_infer<<<(unsigned)((n + IS - 1) / IS), 64, 0, st>>>(n, x, nm, d.P, d.W2b, d.W3b, emap, ec[0], ec[1], ec[2],
+                                                              ec[3], tol, yn, outL, flag, list, nlist);
+    const int64_t nb = std::min<int64_t>(sph_blocks(n, 256), 2048);
+    k_mlp_fallback<<<(unsigned)nb, 256, 0, st>>>(n, x, ax, tabL, list, nlist, outL, nfall);
     return hipGetLastError();
 }
 

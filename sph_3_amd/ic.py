@@ -62,7 +62,7 @@ def gaussian_field(n_grid: int, slope: float = -11.0 / 3.0, seed: int = 0):
     k2[0, 0, 0] = 1.0
     f *= k2 ** np.float32(slope / 4.0)  # amplitude ~ sqrt(P) = k^(slope/2)
     f[0, 0, 0] = 0.0
-    g = np.fft.irfftn(f, s=(n_grid,) * 3).astype(np.float32)
+    g = np.fft.irfftn(f, s=(n_grid,) * 3, axes=(0, 1, 2)).astype(np.float32)
     g -= g.mean()
     g /= g.std()
     return g
