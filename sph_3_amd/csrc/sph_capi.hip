@@ -128,6 +128,7 @@ struct sph_ctx {
     Buf rho, P, c, prho2, divV, curlV, bal, du;
     // h solve scratch
     Buf cand, hR, hsteps, hstate, hcount, hout, hin, tx, ty, tz;
+    Buf hplist;  // pending groups and targets of the h-solve rounds after the first (+ two counters)
     Buf ccidx, cccnt, cext, cnext;  // candidate indices, counts, extra pairs and their region counters
     Buf chr;                        // (h_j, R_j) packed for the rows from candidates
     Buf hbig;                       // targets of the candidate-free h solver
@@ -596,7 +597,7 @@ extern "C" int sph_ctx_destroy(sph_ctx *ctx)
                   &ctx->hist, &ctx->bpart, &ctx->box, &ctx->nodes, &ctx->right, &ctx->parent, &ctx->ncount, &ctx->flags, &ctx->front, &ctx->groups,
                   &ctx->cnt, &ctx->rcnt0, &ctx->off, &ctx->scantmp, &ctx->idx, &ctx->rho, &ctx->P, &ctx->c, &ctx->prho2,
                   &ctx->divV, &ctx->curlV, &ctx->bal, &ctx->du, &ctx->cand, &ctx->hR, &ctx->hsteps,
-                  &ctx->hstate, &ctx->hcount, &ctx->hout, &ctx->hin, &ctx->tx, &ctx->ty, &ctx->tz,
+                  &ctx->hstate, &ctx->hcount, &ctx->hplist, &ctx->hout, &ctx->hin, &ctx->tx, &ctx->ty, &ctx->tz,
                   &ctx->stage, &ctx->kpart, &ctx->kout, &ctx->spread, &ctx->ccidx, &ctx->cccnt,
                   &ctx->cext, &ctx->cnext, &ctx->chr, &ctx->ttab, &ctx->zmet, &ctx->j21, &ctx->ucols, &ctx->uyn,
                   &ctx->uL, &ctx->uG, &ctx->uflag, &ctx->ucnt, &ctx->hbig, &ctx->p4, &ctx->v4, &ctx->s4,
@@ -670,6 +671,12 @@ int hsolve_params(sph_ctx *ctx, int64_t m, const double *tx, const double *ty, c
     p.ccnt = I32(cccnt);
     p.host_totals = ctx->hpin;
     p.done = ctx->ev_h;
+    if (mode != SPH_H_REF_COUNT) {
+        const int64_t ng = p.groups ? ctx->ngroups : 0;
+        CK(ensure(ctx, ctx->hplist, (size_t)(sph::hsolve_plist_ints(m, ng) + 64) * 4));
+        p.plist_counts = P_<unsigned long long>(ctx->hplist);
+        p.plist = I32(hplist) + 64;
+    }
     if (mode == SPH_H_NEWTON) {
         CK(ensure(ctx, ctx->hbig, (size_t)m * 4));
         p.big_list = I32(hbig);

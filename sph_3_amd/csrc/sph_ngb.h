@@ -3,6 +3,8 @@
 #include "sph_common.h"
 #include "sph_order.h"
 
+#include <algorithm>
+
 // Largest candidate capacity per target of the h solves (register slots of the solve).
 #define SPH_HCAP_SLOTS 256
 
@@ -45,7 +47,12 @@ struct HSolveParams {
     unsigned long long *big_count = nullptr;
     int64_t *n_big = nullptr;  // host (may be null): targets given to that solver
     double per = 0.0;          // side of a periodic box: minimum-image distances (modes 0 and 2; 0: open)
+    // modes 0 and 2 (may be null): hsolve_plist_ints(m, ngroups) int32 and 2 device words for the
+    // lists of pending groups and targets that the rounds after the first launch
+    int32_t *plist = nullptr;
+    unsigned long long *plist_counts = nullptr;
 };
+inline int64_t hsolve_plist_ints(int64_t m, int64_t ngroups) { return m + std::max<int64_t>(ngroups, (m + 63) / 64); }
 
 // Runs rounds until every target is decided; *n_unconverged = targets still
 // pending after max_rounds (0 on success).  Synchronises the stream.  Returns

@@ -19,6 +19,8 @@
 #include "sph_ngb.h"
 #include "sph_walk.h"
 
+#include <algorithm>
+
 #pragma clang fp contract(off)
 
 __device__ __forceinline__ double box_dist2(double x, double y, double z, const WNode *__restrict__ nd)
@@ -99,18 +101,22 @@ struct HSolveArgs {
     int32_t *big_list;         // mode 0 (may be null): targets whose solution needs more
     unsigned long long *big_count;  //   candidates than a row holds (state 3), for k_hsolve_big
     double per;                // side of a periodic box (0: open boundaries; modes 0 and 2)
+    // rounds after the first: the groups (glist) and targets (tlist) that still hold pending
+    // targets, with their device counts (null in the first round: every group and target)
+    const int32_t *glist, *tlist;
+    const unsigned long long *gcount, *tcount;
 };
 
 // Group range: tree node groups[blk] or the block [64 blk, 64 blk + 64) of targets.
-__device__ __forceinline__ void group_range(const int32_t *groups, const WNode *gnodes, int64_t m, int64_t &first,
-                                            int &count)
+__device__ __forceinline__ void group_range(const int32_t *groups, const WNode *gnodes, int64_t m, int64_t gi,
+                                            int64_t &first, int &count)
 {
     if (groups) {
-        const WNode *g = gnodes + groups[xcd_block(blockIdx.x, gridDim.x)];
+        const WNode *g = gnodes + groups[gi];
         first = g->first;
         count = g->count;
     } else {
-        first = xcd_block(blockIdx.x, gridDim.x) * 64;
+        first = gi * 64;
         count = (int)((m - first) < 64 ? (m - first) : 64);
     }
 }
@@ -122,9 +128,14 @@ __global__ void __launch_bounds__(64) k_hwalk(HSolveArgs a)
     __shared__ double sx[WSRC], sy[WSRC], sz[WSRC];
     __shared__ int sj[WSRC];
     __shared__ double ttx[64], tty[64], ttz[64], tR2[64];  // the group's targets (R^2 < 0: none)
+    int64_t gi = xcd_block(blockIdx.x, gridDim.x);
+    if (a.glist) {
+        if (blockIdx.x >= *a.gcount) return;  // (uniform)
+        gi = a.glist[blockIdx.x];
+    }
     int64_t first;
     int ng;
-    group_range(a.groups, a.gnodes, a.m, first, ng);
+    group_range(a.groups, a.gnodes, a.m, gi, first, ng);
     const int lane = threadIdx.x;
     const int64_t t = first + lane;
     const bool valid = lane < ng && a.state[t] == 0;
@@ -309,8 +320,13 @@ __global__ void __launch_bounds__(64) k_hnewton(HSolveArgs a)
 {
     constexpr int HL = HN_LANES, HS = HN_SLOTS, HR = HL * HS;
     const int lane = threadIdx.x, lsub = lane % HL;
-    const int64_t j = (int64_t)blockIdx.x * (64 / HL) + lane / HL;
-    const bool valid = j < a.m && a.state[j] == 0;
+    int64_t j = (int64_t)blockIdx.x * (64 / HL) + lane / HL;
+    bool inr = j < a.m;
+    if (a.tlist) {
+        inr = (unsigned long long)j < *a.tcount;
+        j = inr ? a.tlist[j] : 0;
+    }
+    const bool valid = inr && a.state[j] == 0;
     if (__ballot(valid) == 0) return;
     const int cap = a.cap;
     const int cjs = valid ? a.ccnt[j] : 0;
@@ -1005,6 +1021,34 @@ hipError_t h_ratio_max(hipStream_t st, int64_t n, const double *h_new, const dou
     return hipGetLastError();
 }
 
+// The groups that still hold pending targets (state 0) and those targets, for the rounds after
+// the first: one wavefront per group, ballot-compacted appends (one atomic per list and group).
+// List order follows the atomics; each group's and target's result does not depend on it.
+__global__ void __launch_bounds__(64) k_pending_lists(const int32_t *__restrict__ groups, const WNode *__restrict__ gnodes,
+                                                      int64_t m, const int32_t *__restrict__ state,
+                                                      int32_t *__restrict__ glist, int32_t *__restrict__ tlist,
+                                                      unsigned long long *__restrict__ counts)
+{
+    int64_t first;
+    int ng;
+    group_range(groups, gnodes, m, blockIdx.x, first, ng);
+    const int lane = threadIdx.x;
+    const bool pend = lane < ng && state[first + lane] == 0;
+    const uint64_t pm = __ballot(pend);
+    if (pm == 0) return;
+    unsigned long long gb = 0, tb = 0;
+    if (lane == 0) {
+        gb = atomicAdd(counts, 1ull);
+        tb = atomicAdd(counts + 1, (unsigned long long)__popcll(pm));
+    }
+    gb = __shfl(gb, 0);
+    tb = __shfl(tb, 0);
+    if (lane == 0) glist[gb] = (int32_t)blockIdx.x;
+    if (pend)
+        tlist[tb + __builtin_amdgcn_mbcnt_hi((uint32_t)(pm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)pm, 0u))] =
+            (int32_t)(first + lane);
+}
+
 namespace {
 
 HSolveArgs hsolve_args(const HSolveParams &p)
@@ -1022,6 +1066,8 @@ HSolveArgs hsolve_args(const HSolveParams &p)
     a.big_list = p.mode == 0 ? p.big_list : nullptr;
     a.big_count = p.big_count;
     a.per = p.per;
+    a.glist = a.tlist = nullptr;
+    a.gcount = a.tcount = nullptr;
     return a;
 }
 
@@ -1038,15 +1084,31 @@ hipError_t hsolve_big(hipStream_t st, const HSolveParams &p, const HSolveArgs &a
 }
 
 // One Newton / k-th round: counters cleared, the solve, the fold into p.totals.
-void hsolve_round(hipStream_t st, const HSolveParams &p, const HSolveArgs &a)
+// One round.  pending > 0 (a round after the first, with pending targets counted by the
+// previous round): only the groups and targets still pending are launched, from lists built on
+// the device (p.plist); the grids are sized by the host's pending count, which bounds both lists.
+void hsolve_round(hipStream_t st, const HSolveParams &p, HSolveArgs a, int64_t pending = 0)
 {
-    const int64_t nblk = p.groups ? p.ngroups : (p.m + 63) / 64;
+    int64_t nblk = p.groups ? p.ngroups : (p.m + 63) / 64;
+    int64_t nt = p.m;
+    if (pending > 0 && p.plist) {
+        unsigned long long *cnt = p.plist_counts;
+        int32_t *gl = p.plist, *tl = p.plist + nblk;
+        fill<unsigned long long>(st, 2, cnt, 0ull);
+        k_pending_lists<<<(unsigned)nblk, 64, 0, st>>>(p.groups, p.nodes, p.m, p.state, gl, tl, cnt);
+        a.glist = gl;
+        a.tlist = tl;
+        a.gcount = cnt;
+        a.tcount = cnt + 1;
+        nblk = std::min<int64_t>(nblk, pending);
+        nt = pending;
+    }
     fill<unsigned long long>(st, 5 * SPH_SPREAD_WORDS, p.counters, 0ull);
     if (a.per > 0.0)
         k_hwalk<true><<<(unsigned)nblk, 64, 0, st>>>(a);
     else
         k_hwalk<false><<<(unsigned)nblk, 64, 0, st>>>(a);
-    k_hnewton<<<(unsigned)((p.m + 64 / HN_LANES - 1) / (64 / HN_LANES)), 64, 0, st>>>(a);
+    k_hnewton<<<(unsigned)((nt + 64 / HN_LANES - 1) / (64 / HN_LANES)), 64, 0, st>>>(a);
     spread_total(st, 5, p.counters, p.totals);
 }
 
@@ -1102,7 +1164,7 @@ hipError_t hsolve_end(hipStream_t st, const HSolveParams &p, int max_rounds, int
     *more_rounds = true;
     unsigned long long hc[6] = {p.host_totals[0], 0, 0, 0, 0, 0};
     for (int r = 1; r < max_rounds; r++) {
-        hsolve_round(st, p, a);
+        hsolve_round(st, p, a, (int64_t)hc[0]);
         e = hipMemcpyAsync(hc, p.totals, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
         if (e != hipSuccess) return e;
         if (a.big_list) {
@@ -1146,7 +1208,7 @@ hipError_t hsolve(hipStream_t st, const HSolveParams &p, int max_rounds, int64_t
         return hipGetLastError();
     }
     for (int r = 0; r < max_rounds; r++) {
-        hsolve_round(st, p, a);
+        hsolve_round(st, p, a, r > 0 ? (int64_t)hc[0] : 0);
         hipError_t e = hipMemcpyAsync(hc, p.totals, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
         if (e != hipSuccess) return e;
         if (a.big_list) {
