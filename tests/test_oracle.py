@@ -303,6 +303,24 @@ def test_h_newton_solves_its_equation():
     assert abs(hl[c] / (0.5 * (64 * 3 / (4 * np.pi)) ** (1 / 3)) - 1) < 0.05
 
 
+def test_h_newton_rows_matches_full_solve():
+    """The row-sampled Newton (used at sizes where the full solve is too slow) gives the full
+    solve's h on the same rows from other starting values, and its h solves the equation."""
+    pos = sphere(2000, 31)
+    h, bad = O.h_newton(pos, O.h_initial(pos), n_ngb=64.0, tol=1e-13)
+    assert bad == 0
+    rows = np.random.default_rng(4).choice(len(pos), 60, replace=False)
+    start = h[rows] * np.random.default_rng(5).uniform(0.7, 1.3, len(rows))
+    hr, bad = O.h_newton_rows(pos, rows, start, n_ngb=64.0, tol=1e-13)
+    assert bad == 0
+    assert np.abs(hr / h[rows] - 1).max() < 1e-12
+    d = np.sqrt(((pos[rows][:, None, :] - pos[None, :, :]) ** 2).sum(-1))
+    q = d / hr[:, None]
+    w = np.where(q < 1, 1 - 1.5 * q ** 2 + 0.75 * q ** 3, np.where(q < 2, 0.25 * (2 - q) ** 3, 0))
+    f = (32.0 / 3.0) * w.sum(1) - 64.0
+    assert np.abs(f).max() < 1e-10 * 64.0
+
+
 def test_ke():
     v = np.array([[1.0, 2.0, 2.0], [0.0, 0.0, 3.0]])
     assert O.ke(v, np.array([2.0, 4.0])) == 0.5 * 2 * 9 + 0.5 * 4 * 9
