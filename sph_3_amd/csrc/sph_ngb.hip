@@ -254,8 +254,8 @@ __global__ void __launch_bounds__(64) k_hwalk(HSolveArgs a)
 // holds candidates l, l+8, ... (up to 16 in registers; longer rows are read again from
 // memory in every evaluation) and the sub-group reduces with DPP lane exchanges, which give
 // its 8 lanes bitwise the same sums, so each sub-group's iteration is uniform.  A target
-// whose row overflowed gets R * 0.75, one whose solution lies beyond R/2 gets R * 1.5, and
-// stays pending for another round.
+// whose row overflowed gets R * 0.75, one whose solution lies beyond R/2 a larger R estimated from
+// its weighted count at R/2, and stays pending for another round.
 #define HN_LANES 8  // lanes per target of k_hnewton
 
 // One DPP lane move of a double (two 32-bit moves); every source lane of the patterns used
@@ -405,6 +405,7 @@ __global__ void __launch_bounds__(64) k_hnewton(HSolveArgs a)
         double lo = 0.0, hi = hmaxv, hh = hj0, f, df;
         if (!(hh > 0.0 && hh < hi)) hh = 0.5 * hi;
         bool hik = false, grow = false;
+        double fgrow = 0.0;  // f(R/2) when the root lies beyond R/2
         st = 2;
         for (int it = 0; it < 100; it++) {
             feval(hh, f, df);
@@ -421,6 +422,7 @@ __global__ void __launch_bounds__(64) k_hnewton(HSolveArgs a)
                     feval(hmaxv, fm, dfm);
                     if (fm < 0.0) {
                         grow = true;
+                        fgrow = fm;
                         break;
                     }
                     hik = true;
@@ -439,7 +441,12 @@ __global__ void __launch_bounds__(64) k_hnewton(HSolveArgs a)
             }
         }
         if (grow) {
-            Rnew = 1.5 * Rj;
+            // R for the next round from the weighted count n(R/2) = f(R/2) + N_ngb: at a locally
+            // uniform density n grows as h^3, so the root lies near (R/2) (N_ngb / n)^(1/3); take
+            // 1.1 times that, at least 1.25 R and at most 4 R (a target that still falls short
+            // grows again in the next round)
+            const double neff = fmax(fgrow + a.n_ngb, a.n_ngb / 64.0);
+            Rnew = Rj * fmin(4.0, fmax(1.25, 1.1 * cbrt(a.n_ngb / neff)));
             st = 0;
         } else {
             hnew = hh;
