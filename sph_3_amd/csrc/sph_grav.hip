@@ -227,12 +227,15 @@ __global__ void __launch_bounds__(GG) k_gravity_group(const int32_t *__restrict_
         n_nodes += (long long)__popcll(ma);
         // opened buckets: prefix sums of their counts (lane order = ballot order), then their
         // particles are staged 64 at a time, all loads of a chunk issued together
-        int incl = bkt ? ncount : 0;
-        for (int off = 1; off < 64; off <<= 1) {
-            const int v = __shfl_up(incl, off);
-            if (lane >= off) incl += v;
+        // (the prefix by bit planes of the counts, counts <= bucket: one ballot and one masked
+        // population count per bit, no lane exchanges)
+        int excl = 0, total = 0;
+        for (int b = 0; (1 << b) <= bucket; b++) {
+            const uint64_t mbit = wave_ballot(bkt && ((ncount >> b) & 1));
+            excl += (int)__popcll(mbit & lt) << b;
+            total += (int)__popcll(mbit) << b;
         }
-        const int total = __shfl(incl, 63);
+        const int incl = excl + (bkt ? ncount : 0);
         if (bkt) {
             const int pb = (int)__popcll(mb & lt);
             bfirst[pb] = nfirst;

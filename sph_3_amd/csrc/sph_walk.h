@@ -123,13 +123,15 @@ __device__ __forceinline__ bool group_walk(const WNode *__restrict__ nodes, int 
         if (sp + 2 * (int)__popcll(mo) > WSTK) overflow = true;
         sp += 2 * (int)__popcll(mo);
         if (sp > WSTK) sp = WSTK;
-        // bucket prefix sums in ballot (= lane) order by a wave scan of the counts
-        int incl = bkt ? ncount : 0;
-        for (int off = 1; off < 64; off <<= 1) {
-            const int v = __shfl_up(incl, off);
-            if (lane >= off) incl += v;
+        // bucket prefix sums in ballot (= lane) order, by bit planes of the counts (counts <=
+        // bucket): one ballot and one masked population count per bit, no lane exchanges
+        int excl = 0, total = 0;
+        for (int b = 0; (1 << b) <= bucket; b++) {
+            const uint64_t mbit = wave_ballot(bkt && ((ncount >> b) & 1));
+            excl += (int)__popcll(mbit & lt) << b;
+            total += (int)__popcll(mbit) << b;
         }
-        const int total = __shfl(incl, 63);
+        const int incl = excl + (bkt ? ncount : 0);
         if (bkt) {
             const int pb = (int)__popcll(mb & lt);
             L.bfirst[pb] = nfirst;
