@@ -7,8 +7,9 @@
 // (4 -> 128 -> 64 -> 1, Adam on the mean squared error), which the reference
 // trains and runs offline only.  Here training and inference are device kernels:
 //   * weights and activations BF16, accumulation FP32; the two 128 x 128
-//     layers run on v_mfma_f32_32x32x8_bf16, one wavefront per tile of 32
-//     samples, activations staged in LDS;
+//     layers run on v_mfma_f32_32x32x8_bf16, a workgroup of four wavefronts per
+//     tile of 32 samples (each wavefront owns 32 of the 128 features of a
+//     layer), activations staged in LDS;
 //   * backward on MFMA as well (dh = dz W, dW = dz^T h); weight gradients are
 //     summed over tiles with FP32 atomics; Adam keeps FP32 master weights and
 //     refreshes the BF16 copies (row-major and transposed);
@@ -93,56 +94,6 @@ __device__ __forceinline__ floatx16 tile_gemm(const unsigned short *A, const uns
         acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
     }
     return acc;
-}
-
-// Forward pass of the wave's 32 samples.  xin: this lane's sample (lane%32).
-// h1..h3 receive the activations and, if non-null, z1..z3 the pre-activations
-// (inference passes h3 == h1 to ping-pong two buffers).  Returns the
-// normalised output (identical on both lane halves).
-__device__ __forceinline__ float mlp_forward(const float xin[4], const float *__restrict__ P,
-                                             const unsigned short *__restrict__ W2b,
-                                             const unsigned short *__restrict__ W3b, unsigned short *h1,
-                                             unsigned short *h2, unsigned short *h3, unsigned short *z1,
-                                             unsigned short *z2, unsigned short *z3)
-{
-    const int lane = threadIdx.x & 63, s = lane & 31, half = lane >> 5;
-    for (int q = 0; q < 64; q++) {
-        const int f = half * 64 + q;
-        float zz = P[O_B1 + f];
-#pragma unroll
-        for (int k = 0; k < 4; k++) zz = fmaf(P[O_W1 + f * 4 + k], xin[k], zz);
-        h1[s * LDP + f] = f2bf(silu(zz));
-        if (z1) z1[s * LDP + f] = f2bf(zz);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int L = 0; L < 2; L++) {
-        const unsigned short *W = L == 0 ? W2b : W3b;
-        const int ob = L == 0 ? O_B2 : O_B3;
-        const unsigned short *hin = L == 0 ? h1 : h2;
-        unsigned short *hout = L == 0 ? h2 : h3;
-        unsigned short *zout = L == 0 ? z2 : z3;
-        for (int nt = 0; nt < 4; nt++) {
-            const floatx16 acc = tile_gemm(hin, W + (size_t)nt * 32 * HW);
-            const int f = nt * 32 + (lane & 31);
-            const float bias = P[ob + f];
-#pragma unroll
-            for (int r = 0; r < 16; r++) {
-                const int row = drow(r, lane);
-                const float zz = acc[r] + bias;
-                hout[row * LDP + f] = f2bf(silu(zz));
-                if (zout) zout[row * LDP + f] = f2bf(zz);
-            }
-        }
-        __syncthreads();
-    }
-    float y = 0.0f;
-    for (int q = 0; q < 64; q++) {
-        const int f = half * 64 + q;
-        y = fmaf(P[O_W4 + f], bf2f(h3[s * LDP + f]), y);
-    }
-    y += __shfl_xor(y, 32);
-    return y + P[O_B4];
 }
 
 // ---------------------------------------------------------------- inference
@@ -352,22 +303,27 @@ __global__ void __launch_bounds__(256) k_mlp_fallback(int64_t n, const float *__
 }
 
 // ---------------------------------------------------------------- training
-// Forward and backward of one tile of 32 batch samples; batch sample j uses
-// training sample (mult*j + off) mod n.  Gradients of the batch-mean squared
-// error are added to G, the batch-mean loss to *loss.
-__global__ void __launch_bounds__(64) k_mlp_train(int64_t n, int64_t first, int64_t bsz, int64_t mult, int64_t off,
-                                                  const float *__restrict__ x, const float *__restrict__ t,
-                                                  const float *__restrict__ P, const unsigned short *__restrict__ W2b,
-                                                  const unsigned short *__restrict__ W3b,
-                                                  const unsigned short *__restrict__ W2tb,
-                                                  const unsigned short *__restrict__ W3tb, float *__restrict__ G,
-                                                  float *__restrict__ loss)
+// Forward and backward of one tile of 32 batch samples by a workgroup of four wavefronts;
+// batch sample j uses training sample (mult*j + off) mod n.  Every phase splits the 128
+// features (or the 32 x 32 MFMA tiles) of a layer among the four wavefronts, the activations
+// and pre-activations stay in LDS, and the phases are separated by workgroup barriers.
+// Gradients of the batch-mean squared error are added to G, the batch-mean loss to *loss.
+#define TW 4              // wavefronts per training tile
+__global__ void __launch_bounds__(64 * TW) k_mlp_train(int64_t n, int64_t first, int64_t bsz, int64_t mult,
+                                                       int64_t off, const float *__restrict__ x,
+                                                       const float *__restrict__ t, const float *__restrict__ P,
+                                                       const unsigned short *__restrict__ W2b,
+                                                       const unsigned short *__restrict__ W3b,
+                                                       const unsigned short *__restrict__ W2tb,
+                                                       const unsigned short *__restrict__ W3tb, float *__restrict__ G,
+                                                       float *__restrict__ loss)
 {
     __shared__ __align__(16) unsigned short buf[6 * TILE];
-    __shared__ float xs[TS][4], dys[TS];
+    __shared__ float xs[TS][4], dys[TS], ypart[TW][TS];
     unsigned short *h1 = buf, *h2 = buf + TILE, *h3 = buf + 2 * TILE;
     unsigned short *z1 = buf + 3 * TILE, *z2 = buf + 4 * TILE, *z3 = buf + 5 * TILE;
-    const int lane = threadIdx.x, sl = lane & 31, half = lane >> 5;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, sl = lane & 31, half = lane >> 5;
+    const int fb = 32 * w;  // this wavefront's 32 features
     const int64_t j = blockIdx.x * (int64_t)TS + sl;
     const bool valid = j < bsz;
     const int64_t src = valid ? (int64_t)(((unsigned long long)mult * (unsigned long long)(first + j) +
@@ -376,33 +332,79 @@ __global__ void __launch_bounds__(64) k_mlp_train(int64_t n, int64_t first, int6
     float xin[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) xin[k] = valid ? x[src * 4 + k] : 0.0f;
-    if (lane < 32)
+    if (w == 0 && lane < 32)
 #pragma unroll
         for (int k = 0; k < 4; k++) xs[sl][k] = xin[k];
-    const float y = mlp_forward(xin, P, W2b, W3b, h1, h2, h3, z1, z2, z3);
+    // layer 1 (FP32 on the VALU): sample sl, features fb + 16 half + q
+    for (int q = 0; q < 16; q++) {
+        const int f = fb + 16 * half + q;
+        float zz = P[O_B1 + f];
+#pragma unroll
+        for (int k = 0; k < 4; k++) zz = fmaf(P[O_W1 + f * 4 + k], xin[k], zz);
+        h1[sl * LDP + f] = f2bf(silu(zz));
+        z1[sl * LDP + f] = f2bf(zz);
+    }
+    __syncthreads();
+    // layers 2 and 3: wavefront w computes the output tile of features fb..fb+31
+#pragma unroll
+    for (int L = 0; L < 2; L++) {
+        const unsigned short *W = L == 0 ? W2b : W3b;
+        const int ob = L == 0 ? O_B2 : O_B3;
+        const unsigned short *hin = L == 0 ? h1 : h2;
+        unsigned short *hout = L == 0 ? h2 : h3;
+        unsigned short *zout = L == 0 ? z2 : z3;
+        const floatx16 acc = tile_gemm(hin, W + (size_t)fb * HW);
+        const int f = fb + sl;
+        const float bias = P[ob + f];
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int row = drow(r, lane);
+            const float zz = acc[r] + bias;
+            hout[row * LDP + f] = f2bf(silu(zz));
+            zout[row * LDP + f] = f2bf(zz);
+        }
+        __syncthreads();
+    }
+    // output layer: partial dot products over this wavefront's features, then their sum
+    {
+        float y = 0.0f;
+        for (int q = 0; q < 16; q++) {
+            const int f = fb + 16 * half + q;
+            y = fmaf(P[O_W4 + f], bf2f(h3[sl * LDP + f]), y);
+        }
+        y += __shfl_xor(y, 32);
+        if (lane < 32) ypart[w][sl] = y;
+    }
+    __syncthreads();
     const float inv_b = 1.0f / (float)bsz;
-    const float err = valid ? (y - t[src]) : 0.0f;
-    if (lane < 32) dys[sl] = 2.0f * err * inv_b;
-    float l2 = lane < 32 ? err * err : 0.0f;
-    for (int o = 16; o >= 1; o >>= 1) l2 += __shfl_xor(l2, o);
-    if (lane == 0) unsafeAtomicAdd(loss, l2 * inv_b);
+    if (w == 0) {
+        float y = P[O_B4];
+#pragma unroll
+        for (int k = 0; k < TW; k++) y += ypart[k][sl];
+        const float err = valid ? (y - t[src]) : 0.0f;
+        if (lane < 32) dys[sl] = 2.0f * err * inv_b;
+        float l2 = lane < 32 ? err * err : 0.0f;
+        for (int o = 16; o >= 1; o >>= 1) l2 += __shfl_xor(l2, o);
+        if (lane == 0) unsafeAtomicAdd(loss, l2 * inv_b);
+    }
     __syncthreads();
 
     // output layer: gW4 = sum_s dy_s h3[s], gb4 = sum_s dy_s
-    for (int q = 0; q < 2; q++) {
-        const int f = lane + 64 * q;
+    if (lane < 32) {
+        const int f = fb + lane;
         float gw = 0.0f;
         for (int s = 0; s < TS; s++) gw = fmaf(dys[s], bf2f(h3[s * LDP + f]), gw);
         unsafeAtomicAdd(&G[O_W4 + f], gw);
     }
-    if (lane == 0) {
+    if (w == 0 && lane == 0) {
         float gb = 0.0f;
         for (int s = 0; s < TS; s++) gb += dys[s];
         unsafeAtomicAdd(&G[O_B4], gb);
     }
+    __syncthreads();  // h3 is read above; z3 becomes dz3 below and h3 dz2 later
     // dz3 = dy W4 silu'(z3), in place over z3
-    for (int q = 0; q < 64; q++) {
-        const int f = half * 64 + q;
+    for (int q = 0; q < 16; q++) {
+        const int f = fb + 16 * half + q;
         const float zz = bf2f(z3[sl * LDP + f]);
         z3[sl * LDP + f] = f2bf(dys[sl] * P[O_W4 + f] * dsilu(zz));
     }
@@ -416,8 +418,10 @@ __global__ void __launch_bounds__(64) k_mlp_train(int64_t n, int64_t first, int6
         const unsigned short *zp = L == 0 ? z2 : z1;
         const unsigned short *Wt = L == 0 ? W3tb : W2tb;
         const int ow = L == 0 ? O_W3 : O_W2, obb = L == 0 ? O_B3 : O_B2;
-        // dW[o][i] = sum_s dz[s][o] hp[s][i]: A(o, s) = dz[s][o], B(s, i) = hp[s][i]
-        for (int mt = 0; mt < 4; mt++)
+        // dW[o][i] = sum_s dz[s][o] hp[s][i]: A(o, s) = dz[s][o], B(s, i) = hp[s][i]; wavefront w
+        // takes the output rows o in fb..fb+31
+        {
+            const int mt = w;
             for (int nt = 0; nt < 4; nt++) {
                 floatx16 acc = {};
 #pragma unroll
@@ -435,17 +439,19 @@ __global__ void __launch_bounds__(64) k_mlp_train(int64_t n, int64_t first, int6
                 for (int r = 0; r < 16; r++)
                     unsafeAtomicAdd(&G[ow + (mt * 32 + drow(r, lane)) * HW + nt * 32 + sl], acc[r]);
             }
-        for (int q = 0; q < 2; q++) {
-            const int f = lane + 64 * q;
+        }
+        if (lane < 32) {
+            const int f = fb + lane;
             float gb = 0.0f;
             for (int s = 0; s < TS; s++) gb += bf2f(dz[s * LDP + f]);
             unsafeAtomicAdd(&G[obb + f], gb);
         }
-        // dh_prev = dz W (B(o, i) = W[o][i] = Wt[i][o]); dz_prev = dh_prev silu'(z_prev)
+        // dh_prev = dz W (B(o, i) = W[o][i] = Wt[i][o]); dz_prev = dh_prev silu'(z_prev); wavefront
+        // w takes the input features i in fb..fb+31
         unsigned short *nx = L == 0 ? h3 : z3;
-        for (int nt = 0; nt < 4; nt++) {
-            const floatx16 acc = tile_gemm(dz, Wt + (size_t)nt * 32 * HW);
-            const int i = nt * 32 + sl;
+        {
+            const floatx16 acc = tile_gemm(dz, Wt + (size_t)fb * HW);
+            const int i = fb + sl;
 #pragma unroll
             for (int r = 0; r < 16; r++) {
                 const int s = drow(r, lane);
@@ -456,8 +462,8 @@ __global__ void __launch_bounds__(64) k_mlp_train(int64_t n, int64_t first, int6
     }
     // input layer: gW1[o][k] = sum_s dz1[s][o] x[s][k], gb1[o] = sum_s dz1[s][o]
     const unsigned short *dz1 = z3;
-    for (int q = 0; q < 2; q++) {
-        const int o = lane + 64 * q;
+    if (lane < 32) {
+        const int o = fb + lane;
         float g[4] = {0.0f, 0.0f, 0.0f, 0.0f}, gb = 0.0f;
         for (int s = 0; s < TS; s++) {
             const float d = bf2f(dz1[s * LDP + o]);
@@ -614,7 +620,7 @@ hipError_t mlp_infer(hipStream_t st, int64_t n, const float *x, const MLPNorm &n
 hipError_t mlp_train_batch(hipStream_t st, int64_t n, int64_t first, int64_t bsz, int64_t mult, int64_t off,
                            const float *x, const float *t, const MLPDev &d, float *loss)
 {
-    k_mlp_train<<<(unsigned)((bsz + TS - 1) / TS), 64, 0, st>>>(n, first, bsz, mult, off, x, t, d.P, d.W2b, d.W3b,
+    k_mlp_train<<<(unsigned)((bsz + TS - 1) / TS), 64 * TW, 0, st>>>(n, first, bsz, mult, off, x, t, d.P, d.W2b, d.W3b,
                                                                d.W2tb, d.W3tb, d.G, loss);
     return hipGetLastError();
 }
