@@ -306,15 +306,31 @@ int build_ngb(sph_ctx *ctx, bool csr, const double *hscale, double per)
 // The neighbour sets plus the hydro passes' packed inputs, (x, y, z, h) and (vx, vy, vz, m)
 // of the sorted particles, packed from the current arrays (call after any of them changes).
 // vel_only: only the velocities changed since the last call (positions, h, m and n as then).
-int ngb_rows(sph_ctx *ctx, NgbRows &r, bool vel_only)
+// compact: also the mixed form's compact records, on the grid of the Morton bounding box of the
+// current order (the positions must be those the order was built from); full = false leaves
+// the FP64 records out (null in r) when no pass of the caller reads them.
+int ngb_rows(sph_ctx *ctx, NgbRows &r, bool vel_only, bool compact, bool full)
 {
     const int64_t n = ctx->n;
-    CK(ensure(ctx, ctx->p4, (size_t)n * 32));
-    CK(ensure(ctx, ctx->v4, (size_t)n * 32));
-    if (!vel_only) HIPCK(sph::pack4(ctx->st, n, D(x), D(y), D(z), D(h), P_<double4>(ctx->p4)));
-    HIPCK(sph::pack4(ctx->st, n, D(vx), D(vy), D(vz), D(m), P_<double4>(ctx->v4)));
+    if (full) {
+        CK(ensure(ctx, ctx->p4, (size_t)n * 32));
+        CK(ensure(ctx, ctx->v4, (size_t)n * 32));
+        if (!vel_only) HIPCK(sph::pack4(ctx->st, n, D(x), D(y), D(z), D(h), P_<double4>(ctx->p4)));
+        HIPCK(sph::pack4(ctx->st, n, D(vx), D(vy), D(vz), D(m), P_<double4>(ctx->v4)));
+    }
     r = NgbRows{P_<int64_t>(ctx->off), I32(cnt), I32(idx), ctx->ngb_stride, ctx->ngb_per,
-                P_<double4>(ctx->p4), P_<double4>(ctx->v4)};
+                full ? P_<double4>(ctx->p4) : nullptr, full ? P_<double4>(ctx->v4) : nullptr};
+    if (compact) {
+        CK(ensure(ctx, ctx->pf, (size_t)n * 16));
+        CK(ensure(ctx, ctx->vf, (size_t)n * 16));
+        CK(ensure(ctx, ctx->cellc, (size_t)n * 4));
+        HIPCK(sph::pack_compact(ctx->st, n, D(x), D(y), D(z), D(h), D(vx), D(vy), D(vz), D(m), D(box),
+                                P_<float4>(ctx->pf), P_<float4>(ctx->vf), P_<uint32_t>(ctx->cellc), vel_only));
+        r.pf = P_<float4>(ctx->pf);
+        r.vf = P_<float4>(ctx->vf);
+        r.cell = P_<uint32_t>(ctx->cellc);
+        r.grid = D(box);
+    }
     return SPH_OK;
 }
 
@@ -414,7 +430,7 @@ extern "C" int sph_ctx_destroy(sph_ctx *ctx)
                   &ctx->hstate, &ctx->hcount, &ctx->hplist, &ctx->hout, &ctx->hin, &ctx->tx, &ctx->ty, &ctx->tz,
                   &ctx->stage, &ctx->kpart, &ctx->kout, &ctx->spread, &ctx->ccidx, &ctx->cccnt,
                   &ctx->cext, &ctx->cnext, &ctx->chr, &ctx->ttab, &ctx->zmet, &ctx->j21, &ctx->ucols, &ctx->uyn,
-                  &ctx->uL, &ctx->uG, &ctx->uflag, &ctx->ucnt, &ctx->hbig, &ctx->p4, &ctx->v4, &ctx->s4,
+                  &ctx->uL, &ctx->uG, &ctx->uflag, &ctx->ucnt, &ctx->hbig, &ctx->p4, &ctx->v4, &ctx->s4, &ctx->pf, &ctx->vf, &ctx->cellc,
                   &ctx->colg, &ctx->coln, &ctx->tsdt, &ctx->tsdh, &ctx->phi, &ctx->tpart, &ctx->tout};
     for (Buf *b : all)
         if (b->p) (void)hipFree(b->p);

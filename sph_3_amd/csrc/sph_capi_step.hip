@@ -2,6 +2,7 @@
 // evaluation, the step, statistics, restart, outflow injection, the column-density and
 // time-step read-backs.
 #include "sph_ctx.h"
+#include <type_traits>
 
 using namespace sphc;
 
@@ -203,11 +204,16 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     R *bal = p->balsara ? P_<R>(ctx->bal) : nullptr;
     const bool g = p->G != 0.0;
     CK(ensure(ctx, ctx->s4, (size_t)n * 4 * sizeof(R)));
+    // The mixed form with open boundaries reads the compact FP32 records in the density,
+    // acceleration and du/dt passes; the FP64 records are packed only for the passes that
+    // read them (div v without the Balsara switch, column density, time-step criteria).
+    const bool cmp = std::is_same<R, float>::value && p->period == 0.0;
+    const bool full = !cmp || ctx->col_on || p->C_CFL > 0.0;
     // density + EOS, fused with div/curl v and the Balsara factor when the switch is on; then
     // the acceleration (a = a_grav + a_sph)
     auto hydro_forces = [&]() -> int {
         NgbRows rows;
-        CK(ngb_rows(ctx, rows));
+        CK(ngb_rows(ctx, rows, false, cmp, full));
         ph_begin(ctx, SPH_PH_DENSITY);
         if (p->balsara)
             HIPCK(sph::density_divcurl<R>(st, n, D(u), p->gamma, rows, P_<R>(ctx->rho), P_<R>(ctx->P), P_<R>(ctx->c),
@@ -268,7 +274,7 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     }
     ph_begin(ctx, SPH_PH_DUDT);
     NgbRows rows;
-    CK(ngb_rows(ctx, rows, true));  // packs the kicked velocities (positions and h as in the force passes)
+    CK(ngb_rows(ctx, rows, true, cmp, full));  // packs the kicked velocities (positions and h as in the force passes)
     HIPCK(sph::dudt<R>(st, n, P_<R4<R>>(ctx->s4), bal != nullptr, p->alpha, rows, P_<R>(ctx->du)));
     ph_end(ctx, SPH_PH_DUDT);
     ph_begin(ctx, SPH_PH_UUPDATE);

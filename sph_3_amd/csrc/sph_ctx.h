@@ -66,6 +66,7 @@ struct sph_ctx {
     // neighbour lists
     Buf cnt, rcnt0, off, scantmp, idx;  // rcnt0: row lengths before the appended pairs
     Buf p4, v4, s4;                      // packed (x, y, z, h), (vx, vy, vz, m), (rho, P/rho^2, c, f) records
+    Buf pf, vf, cellc;                   // compact FP32 records of the mixed form (NgbRows::pf, vf, cell)
     // hydro scalars (8 bytes per particle, used as float or double)
     Buf rho, P, c, prho2, divV, curlV, bal, du;
     // h solve scratch
@@ -183,7 +184,7 @@ int put_vec3(sph_ctx *ctx, const double *src, Buf &a, Buf &b, Buf &c);
 int get_vec3(sph_ctx *ctx, Buf &a, Buf &b, Buf &c, double *out);
 int ensure_moments(sph_ctx *ctx);
 int build_ngb(sph_ctx *ctx, bool csr, const double *hscale = nullptr, double per = 0.0);
-int ngb_rows(sph_ctx *ctx, NgbRows &r, bool vel_only = false);
+int ngb_rows(sph_ctx *ctx, NgbRows &r, bool vel_only = false, bool compact = false, bool full = true);
 int ensure_csr(sph_ctx *ctx, bool csr = false);
 int geom(sph_ctx *ctx, int64_t n, const double *pos, const double *h, const double *m);
 int check_common(sph_ctx *ctx, int64_t n, int precision);

@@ -23,12 +23,72 @@
 #define HG_DENSITY 8
 #define HG_ACC 8
 
+// A particle's pair inputs as a pass reads them (its own and each neighbour's): the FP64
+// records p4/v4 (CMP = false), or the mixed form's compact records pf/vf/cell (CMP = true).
+template <bool CMP> struct Nb;
+template <> struct Nb<false> {
+    double4 p, v;
+};
+template <> struct Nb<true> {
+    float4 p, v;
+    uint32_t c;
+};
+
+template <bool CMP>
+__device__ __forceinline__ Nb<CMP> nb_load(const NgbRows &r, int64_t j)
+{
+    if constexpr (CMP) return Nb<true>{r.pf[j], r.vf[j], r.cell[j]};
+    else return Nb<false>{r.p4[j], r.v4[j]};
+}
+
+// r_i - r_j in R and hbar_ij = (h_i + h_j) / 2.  FP64 records: the FP64 difference (minimum
+// image when PER) rounded once.  Compact records: (c_i - c_j) cs + (o_i - o_j) per axis in
+// FP32, one fma; the cell difference times the power-of-two spacing is exact, so the result
+// carries one rounding of the sum plus that of the offsets' difference (<= 2^-25 cs), and it
+// is exactly antisymmetric in i and j.
+template <typename R, bool PER, bool CMP>
+__device__ __forceinline__ void nb_geom(const NgbRows &r, float cs, const Nb<CMP> &a, const Nb<CMP> &b, R &dx,
+                                        R &dy, R &dz, R &hij)
+{
+    if constexpr (CMP) {
+        const int ax = (int)(a.c & 1023u), ay = (int)((a.c >> 10) & 1023u), az = (int)(a.c >> 20);
+        const int bx = (int)(b.c & 1023u), by = (int)((b.c >> 10) & 1023u), bz = (int)(b.c >> 20);
+        dx = (R)__builtin_fmaf((float)(ax - bx), cs, a.p.x - b.p.x);
+        dy = (R)__builtin_fmaf((float)(ay - by), cs, a.p.y - b.p.y);
+        dz = (R)__builtin_fmaf((float)(az - bz), cs, a.p.z - b.p.z);
+        hij = (R)(0.5f * (a.p.w + b.p.w));
+    } else {
+        dx = (R)wrap_d<PER>(a.p.x - b.p.x, r.L);
+        dy = (R)wrap_d<PER>(a.p.y - b.p.y, r.L);
+        dz = (R)wrap_d<PER>(a.p.z - b.p.z, r.L);
+        hij = (R)(0.5 * (a.p.w + b.p.w));
+    }
+}
+
+// v_i - v_j and m_j in R (FP64 records: the FP64 difference rounded once; compact: FP32).
+template <typename R, bool CMP>
+__device__ __forceinline__ void nb_vel(const Nb<CMP> &a, const Nb<CMP> &b, R &ux, R &uy, R &uz, R &mj)
+{
+    ux = (R)(a.v.x - b.v.x);  // FP64 or FP32 difference as the records hold them
+    uy = (R)(a.v.y - b.v.y);
+    uz = (R)(a.v.z - b.v.z);
+    mj = (R)b.v.w;
+}
+
+// Spacing of the compact records' grid (0 when they are not used).
+template <bool CMP>
+__device__ __forceinline__ float nb_spacing(const NgbRows &r)
+{
+    if constexpr (CMP) return (float)r.grid[7];
+    else return 0.0f;
+}
+
 // Lane l of a G-lane group visits the row entries k0 + l, k0 + l + G, ... < e in this
-// order and calls body(j, p4[j], v4[j], x[j]) for each (x may be a dummy when X_ is false).
-// The next entry's records, and the index after it, are requested before the current
-// entry's body runs, so a lane keeps one neighbour gather in flight behind its arithmetic
-// instead of waiting for the index and then the records on every entry.
-template <int G, bool X_, typename X, typename Body>
+// order and calls body(nb_j, x[j]) for each (x may be a dummy when X_ is false).  The next
+// entry's records, and the index after it, are requested before the current entry's body
+// runs, so a lane keeps one neighbour gather in flight behind its arithmetic instead of
+// waiting for the index and then the records on every entry.
+template <int G, bool CMP, bool X_, typename X, typename Body>
 __device__ __forceinline__ void pair_loop(const NgbRows &rows, int64_t k0, int64_t e, int l,
                                           const X *__restrict__ xr, Body body)
 {
@@ -36,33 +96,30 @@ __device__ __forceinline__ void pair_loop(const NgbRows &rows, int64_t k0, int64
     if (k >= e) return;
     int j = rows.idx[k];
     int jn = (k + G < e) ? rows.idx[k + G] : j;
-    double4 pj = rows.p4[j], vj = rows.v4[j];
+    Nb<CMP> bj = nb_load<CMP>(rows, j);
     X xj{};
     if constexpr (X_) xj = xr[j];
     for (;;) {
         const int64_t kn = k + G;
         const bool more = kn < e;
-        double4 pn = pj, vn = vj;
+        Nb<CMP> bn = bj;
         X xn = xj;
         int jnn = jn;
         if (more) {
-            pn = rows.p4[jn];
-            vn = rows.v4[jn];
+            bn = nb_load<CMP>(rows, jn);
             if constexpr (X_) xn = xr[jn];
             if (kn + G < e) jnn = rows.idx[kn + G];
         }
-        body(j, pj, vj, xj);
+        body(bj, xj);
         if (!more) break;
         k = kn;
-        j = jn;
         jn = jnn;
-        pj = pn;
-        vj = vn;
+        bj = bn;
         xj = xn;
     }
 }
 
-template <typename R, int G, bool PER>
+template <typename R, int G, bool PER, bool CMP>
 __global__ void __launch_bounds__(256) k_density(int64_t n, const double *__restrict__ u, double gamma,
                                                  NgbRows rows,
                                                  R *__restrict__ rho, R *__restrict__ P, R *__restrict__ c,
@@ -74,19 +131,17 @@ __global__ void __launch_bounds__(256) k_density(int64_t n, const double *__rest
     bool valid = i < n;
     Kahan<R> acc;
     if (valid) {
-        const double4 pi4 = rows.p4[i];
-        const double xi = pi4.x, yi = pi4.y, zi = pi4.z, hi = pi4.w;
+        const float cs = nb_spacing<CMP>(rows);
+        const Nb<CMP> bi = nb_load<CMP>(rows, i);
         int64_t k0, e;
         row_range(rows, i, k0, e);
-        for (int64_t k = k0 + l; k < e; k += G) {
-            const int j = rows.idx[k];
-            const double4 pj = rows.p4[j], vj = rows.v4[j];
-            R dx = (R)wrap_d<PER>(xi - pj.x, rows.L), dy = (R)wrap_d<PER>(yi - pj.y, rows.L),
-              dz = (R)wrap_d<PER>(zi - pj.z, rows.L);
+        pair_loop<G, CMP, false, int>(rows, k0, e, l, nullptr, [&](const Nb<CMP> &bj, int) {
+            R dx, dy, dz, hij, ux, uy, uz, mj;
+            nb_geom<R, PER, CMP>(rows, cs, bi, bj, dx, dy, dz, hij);
+            nb_vel<R, CMP>(bi, bj, ux, uy, uz, mj);
             R rr = sph_sqrt<R>(dx * dx + dy * dy + dz * dz);
-            R hij = (R)(0.5 * (hi + pj.w));
-            acc.add((R)vj.w * spline_W<R>(rr, hij));
-        }
+            acc.add(mj * spline_W<R>(rr, hij));
+        });
     }
     R s = group_sum<R, G>(acc.s, -acc.c);
     if (valid && l == 0) {
@@ -156,7 +211,7 @@ __global__ void __launch_bounds__(256) k_divcurl(int64_t n, const R *__restrict_
 // curl need only the particle's own density, so both sums share the
 // neighbour gathers and are divided by rho_i at the end.  Per-quantity
 // arithmetic and summation order are those of k_density and k_divcurl.
-template <typename R, int G, bool PER>
+template <typename R, int G, bool PER, bool CMP>
 __global__ void __launch_bounds__(256) k_density_divcurl(
     int64_t n, const double *__restrict__ u, double gamma, NgbRows rows, R *__restrict__ rho, R *__restrict__ P,
     R *__restrict__ c, R *__restrict__ prho2, R *__restrict__ divV, R *__restrict__ curlV, R *__restrict__ bal,
@@ -167,23 +222,21 @@ __global__ void __launch_bounds__(256) k_density_divcurl(
     int l = (int)(gid % G);
     bool valid = i < n;
     Kahan<R> acc, sd, scx, scy, scz;
+    R hown = (R)0;
     if (valid) {
-        const double4 pi4 = rows.p4[i];
-        const double xi = pi4.x, yi = pi4.y, zi = pi4.z, hi = pi4.w;
-        const double4 vi4 = rows.v4[i];
-        const double vxi = vi4.x, vyi = vi4.y, vzi = vi4.z;
+        const float cs = nb_spacing<CMP>(rows);
+        const Nb<CMP> bi = nb_load<CMP>(rows, i);
+        hown = (R)bi.p.w;
         int64_t k0, e;
         row_range(rows, i, k0, e);
-        pair_loop<G, false, int>(rows, k0, e, l, nullptr, [&](int, const double4 &pj, const double4 &vj, int) {
-            R dx = (R)wrap_d<PER>(xi - pj.x, rows.L), dy = (R)wrap_d<PER>(yi - pj.y, rows.L),
-              dz = (R)wrap_d<PER>(zi - pj.z, rows.L);
+        pair_loop<G, CMP, false, int>(rows, k0, e, l, nullptr, [&](const Nb<CMP> &bj, int) {
+            R dx, dy, dz, hij, ux, uy, uz, mj;
+            nb_geom<R, PER, CMP>(rows, cs, bi, bj, dx, dy, dz, hij);
+            nb_vel<R, CMP>(bi, bj, ux, uy, uz, mj);
             R rr = sph_sqrt<R>(dx * dx + dy * dy + dz * dz);
-            R hij = (R)(0.5 * (hi + pj.w));
-            R mj = (R)vj.w;
             acc.add(mj * spline_W<R>(rr, hij));
             R nW = spline_nW<R>(rr, hij);
             R gx = nW * dx, gy = nW * dy, gz = nW * dz;
-            R ux = (R)(vxi - vj.x), uy = (R)(vyi - vj.y), uz = (R)(vzi - vj.z);
             sd.add(-mj * (ux * gx + uy * gy + uz * gz));
             scx.add(mj * (uy * gz - uz * gy));
             scy.add(mj * (uz * gx - ux * gz));
@@ -212,7 +265,7 @@ __global__ void __launch_bounds__(256) k_density_divcurl(
         curlV[i] = cv;
         R f = (R)1;
         if (bal) {
-            f = dv / (dv + cv + (R)0.0001 * ci / (R)rows.p4[i].w);
+            f = dv / (dv + cv + (R)0.0001 * ci / hown);
             bal[i] = f;
         }
         if (s4) s4[i] = R4<R>{s, pr, ci, f};
@@ -240,7 +293,7 @@ __device__ __forceinline__ float av_pi<float>(float vr, float rr, float ci, floa
     return vr > 0.0f ? 0.0f : pi;
 }
 
-template <typename R, int G, bool PER>
+template <typename R, int G, bool PER, bool CMP>
 __global__ void __launch_bounds__(256) k_accsph(int64_t n, const R4<R> *__restrict__ s4, bool balsara,
                                                 double alpha,
                                                 NgbRows rows,
@@ -254,25 +307,23 @@ __global__ void __launch_bounds__(256) k_accsph(int64_t n, const R4<R> *__restri
     bool valid = i < n;
     Kahan<R> sx, sy, sz;
     if (valid) {
-        const double4 pi4 = rows.p4[i];
-        const double xi = pi4.x, yi = pi4.y, zi = pi4.z, hi = pi4.w;
-        const double4 vi4 = rows.v4[i];
-        const double vxi = vi4.x, vyi = vi4.y, vzi = vi4.z;
+        const float cs = nb_spacing<CMP>(rows);
+        const Nb<CMP> bi = nb_load<CMP>(rows, i);
         const R4<R> si = s4[i];
         const R rhoi = si.x, pi_ = si.y, ci = si.z, fi = si.w;
         const R al = (R)alpha;
         int64_t k0, e;
         row_range(rows, i, k0, e);
-        pair_loop<G, true>(rows, k0, e, l, s4, [&](int, const double4 &pj, const double4 &vj, const R4<R> &sj) {
-            R dx = (R)wrap_d<PER>(xi - pj.x, rows.L), dy = (R)wrap_d<PER>(yi - pj.y, rows.L),
-              dz = (R)wrap_d<PER>(zi - pj.z, rows.L);
+        pair_loop<G, CMP, true>(rows, k0, e, l, s4, [&](const Nb<CMP> &bj, const R4<R> &sj) {
+            R dx, dy, dz, hij, ux, uy, uz, mj;
+            nb_geom<R, PER, CMP>(rows, cs, bi, bj, dx, dy, dz, hij);
+            nb_vel<R, CMP>(bi, bj, ux, uy, uz, mj);
             R rr = sph_sqrt<R>(dx * dx + dy * dy + dz * dz);
-            R nW = spline_nW<R>(rr, (R)(0.5 * (hi + pj.w)));
-            R ux = (R)(vxi - vj.x), uy = (R)(vyi - vj.y), uz = (R)(vzi - vj.z);
+            R nW = spline_nW<R>(rr, hij);
             R vr = ux * dx + uy * dy + uz * dz;
             R PI = av_pi<R>(vr, rr, ci, sj.z, rhoi, sj.x, al);
             if (balsara) PI = (R)0.5 * (fi + sj.w) * PI;
-            R t = (R)vj.w * (pi_ + sj.y + PI) * nW;
+            R t = mj * (pi_ + sj.y + PI) * nW;
             sx.add(-t * dx);
             sy.add(-t * dy);
             sz.add(-t * dz);
@@ -289,7 +340,7 @@ __global__ void __launch_bounds__(256) k_accsph(int64_t n, const R4<R> *__restri
     }
 }
 
-template <typename R, int G, bool PER>
+template <typename R, int G, bool PER, bool CMP>
 __global__ void __launch_bounds__(256) k_dudt(int64_t n, const R4<R> *__restrict__ s4, bool balsara,
                                               double alpha,
                                               NgbRows rows,
@@ -301,25 +352,23 @@ __global__ void __launch_bounds__(256) k_dudt(int64_t n, const R4<R> *__restrict
     bool valid = i < n;
     Kahan<R> s;
     if (valid) {
-        const double4 pi4 = rows.p4[i];
-        const double xi = pi4.x, yi = pi4.y, zi = pi4.z, hi = pi4.w;
-        const double4 vi4 = rows.v4[i];
-        const double vxi = vi4.x, vyi = vi4.y, vzi = vi4.z;
+        const float cs = nb_spacing<CMP>(rows);
+        const Nb<CMP> bi = nb_load<CMP>(rows, i);
         const R4<R> si = s4[i];
         const R rhoi = si.x, pi_ = si.y, ci = si.z, fi = si.w;
         const R al = (R)alpha;
         int64_t k0, e;
         row_range(rows, i, k0, e);
-        pair_loop<G, true>(rows, k0, e, l, s4, [&](int, const double4 &pj, const double4 &vj, const R4<R> &sj) {
-            R dx = (R)wrap_d<PER>(xi - pj.x, rows.L), dy = (R)wrap_d<PER>(yi - pj.y, rows.L),
-              dz = (R)wrap_d<PER>(zi - pj.z, rows.L);
+        pair_loop<G, CMP, true>(rows, k0, e, l, s4, [&](const Nb<CMP> &bj, const R4<R> &sj) {
+            R dx, dy, dz, hij, ux, uy, uz, mj;
+            nb_geom<R, PER, CMP>(rows, cs, bi, bj, dx, dy, dz, hij);
+            nb_vel<R, CMP>(bi, bj, ux, uy, uz, mj);
             R rr = sph_sqrt<R>(dx * dx + dy * dy + dz * dz);
-            R nW = spline_nW<R>(rr, (R)(0.5 * (hi + pj.w)));
-            R ux = (R)(vxi - vj.x), uy = (R)(vyi - vj.y), uz = (R)(vzi - vj.z);
+            R nW = spline_nW<R>(rr, hij);
             R vr = ux * dx + uy * dy + uz * dz;
             R PI = av_pi<R>(vr, rr, ci, sj.z, rhoi, sj.x, al);
             if (balsara) PI = (R)0.5 * (fi + sj.w) * PI;
-            s.add((R)vj.w * (pi_ + (R)0.5 * PI) * nW * vr);
+            s.add(mj * (pi_ + (R)0.5 * PI) * nW * vr);
         });
     }
     R r = group_sum<R, G>(s.s, -s.c);
@@ -486,6 +535,52 @@ hipError_t pack4(hipStream_t st, int64_t n, const double *a, const double *b, co
     return hipGetLastError();
 }
 
+// Compact records (declared in sph_hydro.h).  The grid: lo = grid[0..2], the largest extent
+// E = max(grid[3+d] - grid[d]) = f 2^k with f in [0.5, 1) (frexp), cs = 2^(k-10): the
+// smallest power of two with 1024 cs > E, so every coordinate in the box has a cell in
+// [0, 1023] (the clamp only guards positions outside the box).  Every thread derives cs
+// itself; thread 0 stores it in grid[7] for the passes.
+__global__ void __launch_bounds__(256) k_pack_compact(int64_t n, const double *__restrict__ x,
+                                                      const double *__restrict__ y, const double *__restrict__ z,
+                                                      const double *__restrict__ h, const double *__restrict__ vx,
+                                                      const double *__restrict__ vy, const double *__restrict__ vz,
+                                                      const double *__restrict__ m, double *__restrict__ grid,
+                                                      float4 *__restrict__ pf, float4 *__restrict__ vf,
+                                                      uint32_t *__restrict__ cell, bool vel_only)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    vf[i] = make_float4((float)vx[i], (float)vy[i], (float)vz[i], (float)m[i]);
+    if (vel_only) return;
+    const double lo[3] = {grid[0], grid[1], grid[2]};
+    const double ext = fmax(fmax(grid[3] - grid[0], grid[4] - grid[1]), grid[5] - grid[2]);
+    int k = 0;
+    frexp(ext, &k);
+    const double cs = ldexp(1.0, k - 10), ics = ldexp(1.0, 10 - k);
+    if (i == 0) grid[7] = cs;
+    const double p[3] = {x[i], y[i], z[i]};
+    float o[3];
+    uint32_t c = 0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double r = p[d] - lo[d];
+        double q = floor(r * ics);
+        q = fmin(fmax(q, 0.0), 1023.0);
+        o[d] = (float)(r - q * cs);
+        c |= (uint32_t)q << (10 * d);
+    }
+    pf[i] = make_float4(o[0], o[1], o[2], (float)h[i]);
+    cell[i] = c;
+}
+
+hipError_t pack_compact(hipStream_t st, int64_t n, const double *x, const double *y, const double *z,
+                        const double *h, const double *vx, const double *vy, const double *vz, const double *m,
+                        double *grid, float4 *pf, float4 *vf, uint32_t *cell, bool vel_only)
+{
+    k_pack_compact<<<sph_blocks(n, 256), 256, 0, st>>>(n, x, y, z, h, vx, vy, vz, m, grid, pf, vf, cell, vel_only);
+    return hipGetLastError();
+}
+
 #define LAUNCH_G(K, G, ...)                                                                                        \
     do {                                                                                                           \
         if (rows.L > 0.0)                                                                                          \
@@ -494,6 +589,17 @@ hipError_t pack4(hipStream_t st, int64_t n, const double *a, const double *b, co
             K<R, G, false><<<sph_blocks(n * G, 256), 256, 0, st>>>(__VA_ARGS__);                                    \
     } while (0)
 #define LAUNCH(K, ...) LAUNCH_G(K, HG, __VA_ARGS__)
+// Passes that read the compact records when the rows carry them (mixed form, open boundaries).
+#define LAUNCH_C(K, G, ...)                                                                                        \
+    do {                                                                                                           \
+        constexpr bool F32 = std::is_same<R, float>::value;                                                        \
+        if (rows.L > 0.0)                                                                                          \
+            K<R, G, true, false><<<sph_blocks(n * G, 256), 256, 0, st>>>(__VA_ARGS__);                              \
+        else if (F32 && rows.pf)                                                                                   \
+            K<R, G, false, F32><<<sph_blocks(n * G, 256), 256, 0, st>>>(__VA_ARGS__);                               \
+        else                                                                                                       \
+            K<R, G, false, false><<<sph_blocks(n * G, 256), 256, 0, st>>>(__VA_ARGS__);                             \
+    } while (0)
 
 template <typename R>
 __global__ void __launch_bounds__(256) k_pack_s4(int64_t n, const R *__restrict__ rho, const R *__restrict__ prho2,
@@ -515,7 +621,7 @@ template <typename R>
 hipError_t density(hipStream_t st, int64_t n, const double *u, double gamma, const NgbRows &rows, R *rho, R *P,
                    R *c, R *prho2, R4<R> *s4)
 {
-    LAUNCH(k_density, n, u, gamma, rows, rho, P, c, prho2, u ? s4 : nullptr);
+    LAUNCH_C(k_density, HG, n, u, gamma, rows, rho, P, c, prho2, u ? s4 : nullptr);
     return hipGetLastError();
 }
 
@@ -531,7 +637,7 @@ template <typename R>
 hipError_t density_divcurl(hipStream_t st, int64_t n, const double *u, double gamma, const NgbRows &rows, R *rho,
                            R *P, R *c, R *prho2, R *divV, R *curlV, R *bal, R4<R> *s4)
 {
-    LAUNCH_G(k_density_divcurl, HG_DENSITY, n, u, gamma, rows, rho, P, c, prho2, divV, curlV, bal, s4);
+    LAUNCH_C(k_density_divcurl, HG_DENSITY, n, u, gamma, rows, rho, P, c, prho2, divV, curlV, bal, s4);
     return hipGetLastError();
 }
 
@@ -539,14 +645,14 @@ template <typename R>
 hipError_t accsph(hipStream_t st, int64_t n, const R4<R> *s4, bool balsara, double alpha, const NgbRows &rows,
                   const double *gx, const double *gy, const double *gz, double *ax, double *ay, double *az)
 {
-    LAUNCH_G(k_accsph, HG_ACC, n, s4, balsara, alpha, rows, gx, gy, gz, ax, ay, az);
+    LAUNCH_C(k_accsph, HG_ACC, n, s4, balsara, alpha, rows, gx, gy, gz, ax, ay, az);
     return hipGetLastError();
 }
 
 template <typename R>
 hipError_t dudt(hipStream_t st, int64_t n, const R4<R> *s4, bool balsara, double alpha, const NgbRows &rows, R *out)
 {
-    LAUNCH(k_dudt, n, s4, balsara, alpha, rows, out);
+    LAUNCH_C(k_dudt, HG, n, s4, balsara, alpha, rows, out);
     return hipGetLastError();
 }
 template <typename R>
@@ -572,6 +678,7 @@ hipError_t column_density(hipStream_t st, int64_t n, const R *rho, const NgbRows
     return hipGetLastError();
 }
 #undef LAUNCH
+#undef LAUNCH_C
 
 #define INST(R)                                                                                                    \
     template hipError_t pack_s4<R>(hipStream_t, int64_t, const R *, const R *, const R *, const R *, R4<R> *);    \
