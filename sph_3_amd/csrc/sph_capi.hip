@@ -306,8 +306,9 @@ int build_ngb(sph_ctx *ctx, bool csr, const double *hscale, double per)
 // The neighbour sets plus the hydro passes' packed inputs, (x, y, z, h) and (vx, vy, vz, m)
 // of the sorted particles, packed from the current arrays (call after any of them changes).
 // vel_only: only the velocities changed since the last call (positions, h, m and n as then).
-// compact: also the mixed form's compact records, on the grid of the Morton bounding box of the
-// current order (the positions must be those the order was built from); full = false leaves
+// compact: also the mixed form's compact records, on the grid of the periodic box or (open
+// boundaries) of the Morton bounding box of the current order (the positions must be those
+// the order was built from); full = false leaves
 // the FP64 records out (null in r) when no pass of the caller reads them.
 int ngb_rows(sph_ctx *ctx, NgbRows &r, bool vel_only, bool compact, bool full)
 {
@@ -324,7 +325,7 @@ int ngb_rows(sph_ctx *ctx, NgbRows &r, bool vel_only, bool compact, bool full)
         CK(ensure(ctx, ctx->pf, (size_t)n * 16));
         CK(ensure(ctx, ctx->vf, (size_t)n * 16));
         CK(ensure(ctx, ctx->cellc, (size_t)n * 4));
-        HIPCK(sph::pack_compact(ctx->st, n, D(x), D(y), D(z), D(h), D(vx), D(vy), D(vz), D(m), D(box),
+        HIPCK(sph::pack_compact(ctx->st, n, D(x), D(y), D(z), D(h), D(vx), D(vy), D(vz), D(m), ctx->ngb_per, D(box),
                                 P_<float4>(ctx->pf), P_<float4>(ctx->vf), P_<uint32_t>(ctx->cellc), vel_only));
         r.pf = P_<float4>(ctx->pf);
         r.vf = P_<float4>(ctx->vf);

@@ -204,10 +204,10 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     R *bal = p->balsara ? P_<R>(ctx->bal) : nullptr;
     const bool g = p->G != 0.0;
     CK(ensure(ctx, ctx->s4, (size_t)n * 4 * sizeof(R)));
-    // The mixed form with open boundaries reads the compact FP32 records in the density,
-    // acceleration and du/dt passes; the FP64 records are packed only for the passes that
-    // read them (div v without the Balsara switch, column density, time-step criteria).
-    const bool cmp = std::is_same<R, float>::value && p->period == 0.0;
+    // The mixed form reads the compact FP32 records in the density, acceleration and du/dt
+    // passes; the FP64 records are packed only for the passes that read them (div v without
+    // the Balsara switch, column density, time-step criteria).
+    const bool cmp = std::is_same<R, float>::value;
     const bool full = !cmp || ctx->col_on || p->C_CFL > 0.0;
     // density + EOS, fused with div/curl v and the Balsara factor when the switch is on; then
     // the acceleration (a = a_grav + a_sph)

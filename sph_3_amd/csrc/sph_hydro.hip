@@ -43,19 +43,26 @@ __device__ __forceinline__ Nb<CMP> nb_load(const NgbRows &r, int64_t j)
 
 // r_i - r_j in R and hbar_ij = (h_i + h_j) / 2.  FP64 records: the FP64 difference (minimum
 // image when PER) rounded once.  Compact records: (c_i - c_j) cs + (o_i - o_j) per axis in
-// FP32, one fma; the cell difference times the power-of-two spacing is exact, so the result
-// carries one rounding of the sum plus that of the offsets' difference (<= 2^-25 cs), and it
-// is exactly antisymmetric in i and j.
+// FP32, one fma, exactly antisymmetric in i and j.  Open boundaries: the cell difference
+// times the power-of-two spacing is exact, so the result carries one rounding of the sum plus
+// that of the offsets' difference (<= 2^-25 cs).  Periodic box: the cell difference is taken
+// to the minimum image (|c_i - c_j| <= 512 cells; neighbours lie far inside half a period)
+// and cs = L / 1024 rounded to FP32 adds one more relative rounding.
 template <typename R, bool PER, bool CMP>
 __device__ __forceinline__ void nb_geom(const NgbRows &r, float cs, const Nb<CMP> &a, const Nb<CMP> &b, R &dx,
                                         R &dy, R &dz, R &hij)
 {
     if constexpr (CMP) {
-        const int ax = (int)(a.c & 1023u), ay = (int)((a.c >> 10) & 1023u), az = (int)(a.c >> 20);
-        const int bx = (int)(b.c & 1023u), by = (int)((b.c >> 10) & 1023u), bz = (int)(b.c >> 20);
-        dx = (R)__builtin_fmaf((float)(ax - bx), cs, a.p.x - b.p.x);
-        dy = (R)__builtin_fmaf((float)(ay - by), cs, a.p.y - b.p.y);
-        dz = (R)__builtin_fmaf((float)(az - bz), cs, a.p.z - b.p.z);
+        int cx = (int)(a.c & 1023u) - (int)(b.c & 1023u), cy = (int)((a.c >> 10) & 1023u) - (int)((b.c >> 10) & 1023u),
+            cz = (int)(a.c >> 20) - (int)(b.c >> 20);
+        if constexpr (PER) {  // minimum image in whole cells (1024 cells per period)
+            cx += cx > 512 ? -1024 : (cx < -512 ? 1024 : 0);
+            cy += cy > 512 ? -1024 : (cy < -512 ? 1024 : 0);
+            cz += cz > 512 ? -1024 : (cz < -512 ? 1024 : 0);
+        }
+        dx = (R)__builtin_fmaf((float)cx, cs, a.p.x - b.p.x);
+        dy = (R)__builtin_fmaf((float)cy, cs, a.p.y - b.p.y);
+        dz = (R)__builtin_fmaf((float)cz, cs, a.p.z - b.p.z);
         hij = (R)(0.5f * (a.p.w + b.p.w));
     } else {
         dx = (R)wrap_d<PER>(a.p.x - b.p.x, r.L);
@@ -535,16 +542,18 @@ hipError_t pack4(hipStream_t st, int64_t n, const double *a, const double *b, co
     return hipGetLastError();
 }
 
-// Compact records (declared in sph_hydro.h).  The grid: lo = grid[0..2], the largest extent
-// E = max(grid[3+d] - grid[d]) = f 2^k with f in [0.5, 1) (frexp), cs = 2^(k-10): the
-// smallest power of two with 1024 cs > E, so every coordinate in the box has a cell in
-// [0, 1023] (the clamp only guards positions outside the box).  Every thread derives cs
-// itself; thread 0 stores it in grid[7] for the passes.
+// Compact records (declared in sph_hydro.h).  Open boundaries (per = 0): lo = grid[0..2],
+// the largest extent E = max(grid[3+d] - grid[d]) = f 2^k with f in [0.5, 1) (frexp),
+// cs = 2^(k-10): the smallest power of two with 1024 cs > E, so every coordinate in the box
+// has a cell in [0, 1023] (the clamp only guards positions outside the box).  Periodic box of
+// side per: lo = 0, cs = per / 1024, cells taken modulo 1024 (a coordinate equal to per, which
+// the drift's wrap can round to, lands in cell 0 with an offset of about zero).  Every
+// thread derives cs itself; thread 0 stores it in grid[7] for the passes.
 __global__ void __launch_bounds__(256) k_pack_compact(int64_t n, const double *__restrict__ x,
                                                       const double *__restrict__ y, const double *__restrict__ z,
                                                       const double *__restrict__ h, const double *__restrict__ vx,
                                                       const double *__restrict__ vy, const double *__restrict__ vz,
-                                                      const double *__restrict__ m, double *__restrict__ grid,
+                                                      const double *__restrict__ m, double per, double *__restrict__ grid,
                                                       float4 *__restrict__ pf, float4 *__restrict__ vf,
                                                       uint32_t *__restrict__ cell, bool vel_only)
 {
@@ -552,11 +561,18 @@ __global__ void __launch_bounds__(256) k_pack_compact(int64_t n, const double *_
     if (i >= n) return;
     vf[i] = make_float4((float)vx[i], (float)vy[i], (float)vz[i], (float)m[i]);
     if (vel_only) return;
-    const double lo[3] = {grid[0], grid[1], grid[2]};
-    const double ext = fmax(fmax(grid[3] - grid[0], grid[4] - grid[1]), grid[5] - grid[2]);
-    int k = 0;
-    frexp(ext, &k);
-    const double cs = ldexp(1.0, k - 10), ics = ldexp(1.0, 10 - k);
+    double lo[3] = {0.0, 0.0, 0.0}, cs;
+    if (per > 0.0) {
+        cs = per / 1024.0;
+    } else {
+        lo[0] = grid[0];
+        lo[1] = grid[1];
+        lo[2] = grid[2];
+        const double ext = fmax(fmax(grid[3] - grid[0], grid[4] - grid[1]), grid[5] - grid[2]);
+        int k = 0;
+        frexp(ext, &k);
+        cs = ldexp(1.0, k - 10);
+    }
     if (i == 0) grid[7] = cs;
     const double p[3] = {x[i], y[i], z[i]};
     float o[3];
@@ -564,10 +580,11 @@ __global__ void __launch_bounds__(256) k_pack_compact(int64_t n, const double *_
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         const double r = p[d] - lo[d];
-        double q = floor(r * ics);
-        q = fmin(fmax(q, 0.0), 1023.0);
+        double q = floor(r / cs);
+        q = per > 0.0 ? fmin(fmax(q, -1.0), 1024.0) : fmin(fmax(q, 0.0), 1023.0);
         o[d] = (float)(r - q * cs);
-        c |= (uint32_t)q << (10 * d);
+        const int qi = (int)q;
+        c |= (uint32_t)(qi < 0 ? qi + 1024 : (qi > 1023 ? qi - 1024 : qi)) << (10 * d);
     }
     pf[i] = make_float4(o[0], o[1], o[2], (float)h[i]);
     cell[i] = c;
@@ -575,9 +592,10 @@ __global__ void __launch_bounds__(256) k_pack_compact(int64_t n, const double *_
 
 hipError_t pack_compact(hipStream_t st, int64_t n, const double *x, const double *y, const double *z,
                         const double *h, const double *vx, const double *vy, const double *vz, const double *m,
-                        double *grid, float4 *pf, float4 *vf, uint32_t *cell, bool vel_only)
+                        double per, double *grid, float4 *pf, float4 *vf, uint32_t *cell, bool vel_only)
 {
-    k_pack_compact<<<sph_blocks(n, 256), 256, 0, st>>>(n, x, y, z, h, vx, vy, vz, m, grid, pf, vf, cell, vel_only);
+    k_pack_compact<<<sph_blocks(n, 256), 256, 0, st>>>(n, x, y, z, h, vx, vy, vz, m, per, grid, pf, vf, cell,
+                                                       vel_only);
     return hipGetLastError();
 }
 
@@ -593,7 +611,9 @@ hipError_t pack_compact(hipStream_t st, int64_t n, const double *x, const double
 #define LAUNCH_C(K, G, ...)                                                                                        \
     do {                                                                                                           \
         constexpr bool F32 = std::is_same<R, float>::value;                                                        \
-        if (rows.L > 0.0)                                                                                          \
+        if (rows.L > 0.0 && F32 && rows.pf)                                                                        \
+            K<R, G, true, F32><<<sph_blocks(n * G, 256), 256, 0, st>>>(__VA_ARGS__);                                \
+        else if (rows.L > 0.0)                                                                                     \
             K<R, G, true, false><<<sph_blocks(n * G, 256), 256, 0, st>>>(__VA_ARGS__);                              \
         else if (F32 && rows.pf)                                                                                   \
             K<R, G, false, F32><<<sph_blocks(n * G, 256), 256, 0, st>>>(__VA_ARGS__);                               \
