@@ -306,8 +306,8 @@ int build_ngb(sph_ctx *ctx, bool csr, const double *hscale, double per)
 // The neighbour sets plus the hydro passes' packed inputs, (x, y, z, h) and (vx, vy, vz, m)
 // of the sorted particles, packed from the current arrays (call after any of them changes).
 // vel_only: only the velocities changed since the last call (positions, h, m and n as then).
-// compact: also the mixed form's compact records, on the grid of the periodic box or (open
-// boundaries) of the Morton bounding box of the current order (the positions must be those
+// compact: also the mixed form's compact records, fixed point over the periodic box or (open
+// boundaries) over the Morton bounding box of the current order (the positions must be those
 // the order was built from); full = false leaves
 // the FP64 records out (null in r) when no pass of the caller reads them.
 int ngb_rows(sph_ctx *ctx, NgbRows &r, bool vel_only, bool compact, bool full)
@@ -324,12 +324,10 @@ int ngb_rows(sph_ctx *ctx, NgbRows &r, bool vel_only, bool compact, bool full)
     if (compact) {
         CK(ensure(ctx, ctx->pf, (size_t)n * 16));
         CK(ensure(ctx, ctx->vf, (size_t)n * 16));
-        CK(ensure(ctx, ctx->cellc, (size_t)n * 4));
         HIPCK(sph::pack_compact(ctx->st, n, D(x), D(y), D(z), D(h), D(vx), D(vy), D(vz), D(m), ctx->ngb_per, D(box),
-                                P_<float4>(ctx->pf), P_<float4>(ctx->vf), P_<uint32_t>(ctx->cellc), vel_only));
-        r.pf = P_<float4>(ctx->pf);
+                                P_<uint4>(ctx->pf), P_<float4>(ctx->vf), vel_only));
+        r.pf = P_<uint4>(ctx->pf);
         r.vf = P_<float4>(ctx->vf);
-        r.cell = P_<uint32_t>(ctx->cellc);
         r.grid = D(box);
     }
     return SPH_OK;
@@ -431,7 +429,7 @@ extern "C" int sph_ctx_destroy(sph_ctx *ctx)
                   &ctx->hstate, &ctx->hcount, &ctx->hplist, &ctx->hout, &ctx->hin, &ctx->tx, &ctx->ty, &ctx->tz,
                   &ctx->stage, &ctx->kpart, &ctx->kout, &ctx->spread, &ctx->ccidx, &ctx->cccnt,
                   &ctx->cext, &ctx->cnext, &ctx->chr, &ctx->ttab, &ctx->zmet, &ctx->j21, &ctx->ucols, &ctx->uyn,
-                  &ctx->uL, &ctx->uG, &ctx->uflag, &ctx->ucnt, &ctx->hbig, &ctx->p4, &ctx->v4, &ctx->s4, &ctx->pf, &ctx->vf, &ctx->cellc,
+                  &ctx->uL, &ctx->uG, &ctx->uflag, &ctx->ucnt, &ctx->hbig, &ctx->p4, &ctx->v4, &ctx->s4, &ctx->pf, &ctx->vf,
                   &ctx->colg, &ctx->coln, &ctx->tsdt, &ctx->tsdh, &ctx->phi, &ctx->tpart, &ctx->tout};
     for (Buf *b : all)
         if (b->p) (void)hipFree(b->p);

@@ -66,7 +66,7 @@ struct sph_ctx {
     // neighbour lists
     Buf cnt, rcnt0, off, scantmp, idx;  // rcnt0: row lengths before the appended pairs
     Buf p4, v4, s4;                      // packed (x, y, z, h), (vx, vy, vz, m), (rho, P/rho^2, c, f) records
-    Buf pf, vf, cellc;                   // compact FP32 records of the mixed form (NgbRows::pf, vf, cell)
+    Buf pf, vf;                          // compact records of the mixed form (NgbRows::pf, vf)
     // hydro scalars (8 bytes per particle, used as float or double)
     Buf rho, P, c, prho2, divV, curlV, bal, du;
     // h solve scratch

@@ -17,13 +17,11 @@ struct NgbRows {
     double L;  // side of the periodic box the sets were built in (0: open boundaries)
     const double4 *p4;
     const double4 *v4;
-    // Compact FP32 records, read instead of p4/v4 by the mixed form's density, acceleration
-    // and du/dt passes (null: not packed; sph::pack_compact): pf[i] = (x, y, z less the corner
-    // of i's cell; h), vf[i] = (vx, vy, vz, m) and cell[i] = cx | cy << 10 | cz << 20, the
-    // cell's integer coordinates on the grid of spacing grid[7].
-    const float4 *pf = nullptr;
+    // Compact records, read instead of p4/v4 by the mixed form's density, acceleration and
+    // du/dt passes (null: not packed; sph::pack_compact): pf[i] = (x, y, z as 32-bit fixed
+    // point in steps of grid[7], the bits of h as FP32) and vf[i] = (vx, vy, vz, m) as FP32.
+    const uint4 *pf = nullptr;
     const float4 *vf = nullptr;
-    const uint32_t *cell = nullptr;
     const double *grid = nullptr;
 };
 
@@ -49,15 +47,16 @@ namespace sph {
 // out[i] = (a[i], b[i], c[i], d[i]).
 hipError_t pack4(hipStream_t st, int64_t n, const double *a, const double *b, const double *c, const double *d,
                  double4 *out);
-// Compact records of the mixed form (NgbRows::pf, vf, cell) on a grid of 1024 cells per axis.
-// Open boundaries (per = 0): over the box grid[0..5] (lo xyz, hi xyz; the Morton bounding
-// box), spacing cs = the smallest power of two with 1024 cs > the largest extent; periodic box
-// of side per: from the origin, cs = per / 1024, cells modulo 1024.  cs is written to grid[7];
-// cell coordinates floor((x - lo) / cs), offsets (x - lo) - c cs rounded to FP32.
-// vel_only: only vf is rewritten (positions, h and the grid as at the last full call).
+// Compact records of the mixed form (NgbRows::pf, vf): coordinates as 32-bit fixed point,
+// rounded to the nearest step sc.  Open boundaries (per = 0): from the low corner of the box
+// grid[0..5] (lo xyz, hi xyz; the Morton bounding box), sc = 2^(k-32) with 2^k the smallest
+// power of two above the largest extent; periodic box of side per: from the origin,
+// sc = per / 2^32, modulo 2^32 (a pair's 32-bit difference is then its minimum image).
+// sc is written to grid[7].  vel_only: only vf is rewritten (positions, h and the grid as at
+// the last full call).
 hipError_t pack_compact(hipStream_t st, int64_t n, const double *x, const double *y, const double *z,
                         const double *h, const double *vx, const double *vy, const double *vz, const double *m,
-                        double per, double *grid, float4 *pf, float4 *vf, uint32_t *cell, bool vel_only);
+                        double per, double *grid, uint4 *pf, float4 *vf, bool vel_only);
 // s4[i] = (rho, P/rho^2, c, Balsara f or 1 when bal is null): the packed record the
 // acceleration and du/dt passes gather.
 template <typename R>

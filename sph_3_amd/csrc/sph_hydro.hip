@@ -24,46 +24,47 @@
 #define HG_ACC 8
 
 // A particle's pair inputs as a pass reads them (its own and each neighbour's): the FP64
-// records p4/v4 (CMP = false), or the mixed form's compact records pf/vf/cell (CMP = true).
+// records p4/v4 (CMP = false), or the mixed form's compact records pf/vf (CMP = true).
 template <bool CMP> struct Nb;
 template <> struct Nb<false> {
     double4 p, v;
 };
 template <> struct Nb<true> {
-    float4 p, v;
-    uint32_t c;
+    uint4 p;  // 32-bit fixed-point x, y, z and the bits of h (FP32)
+    float4 v;
 };
 
 template <bool CMP>
 __device__ __forceinline__ Nb<CMP> nb_load(const NgbRows &r, int64_t j)
 {
-    if constexpr (CMP) return Nb<true>{r.pf[j], r.vf[j], r.cell[j]};
+    if constexpr (CMP) return Nb<true>{r.pf[j], r.vf[j]};
     else return Nb<false>{r.p4[j], r.v4[j]};
 }
 
+// Difference of two fixed-point coordinates in steps, as FP32 (one rounding).  Periodic box
+// (the period spans 2^32 steps): the wrap-around difference read as signed, i.e. the minimum
+// image.  Open boundaries: the exact difference of the unsigned values whatever its size.
+template <bool PER>
+__device__ __forceinline__ float fx_diff(uint32_t a, uint32_t b)
+{
+    if constexpr (PER) return (float)(int32_t)(a - b);
+    else return a >= b ? (float)(a - b) : -(float)(b - a);
+}
+
 // r_i - r_j in R and hbar_ij = (h_i + h_j) / 2.  FP64 records: the FP64 difference (minimum
-// image when PER) rounded once.  Compact records: (c_i - c_j) cs + (o_i - o_j) per axis in
-// FP32, one fma, exactly antisymmetric in i and j.  Open boundaries: the cell difference
-// times the power-of-two spacing is exact, so the result carries one rounding of the sum plus
-// that of the offsets' difference (<= 2^-25 cs).  Periodic box: the cell difference is taken
-// to the minimum image (|c_i - c_j| <= 512 cells; neighbours lie far inside half a period)
-// and cs = L / 1024 rounded to FP32 adds one more relative rounding.
+// image when PER) rounded once.  Compact records: the difference of the fixed-point
+// coordinates in steps (exact integer, one rounding to FP32) times the step size (a power of
+// two for open boundaries: exact).  Exactly antisymmetric in i and j; the positions
+// themselves carry the fixed point's rounding (half a step).
 template <typename R, bool PER, bool CMP>
-__device__ __forceinline__ void nb_geom(const NgbRows &r, float cs, const Nb<CMP> &a, const Nb<CMP> &b, R &dx,
+__device__ __forceinline__ void nb_geom(const NgbRows &r, float sc, const Nb<CMP> &a, const Nb<CMP> &b, R &dx,
                                         R &dy, R &dz, R &hij)
 {
     if constexpr (CMP) {
-        int cx = (int)(a.c & 1023u) - (int)(b.c & 1023u), cy = (int)((a.c >> 10) & 1023u) - (int)((b.c >> 10) & 1023u),
-            cz = (int)(a.c >> 20) - (int)(b.c >> 20);
-        if constexpr (PER) {  // minimum image in whole cells (1024 cells per period)
-            cx += cx > 512 ? -1024 : (cx < -512 ? 1024 : 0);
-            cy += cy > 512 ? -1024 : (cy < -512 ? 1024 : 0);
-            cz += cz > 512 ? -1024 : (cz < -512 ? 1024 : 0);
-        }
-        dx = (R)__builtin_fmaf((float)cx, cs, a.p.x - b.p.x);
-        dy = (R)__builtin_fmaf((float)cy, cs, a.p.y - b.p.y);
-        dz = (R)__builtin_fmaf((float)cz, cs, a.p.z - b.p.z);
-        hij = (R)(0.5f * (a.p.w + b.p.w));
+        dx = (R)(fx_diff<PER>(a.p.x, b.p.x) * sc);
+        dy = (R)(fx_diff<PER>(a.p.y, b.p.y) * sc);
+        dz = (R)(fx_diff<PER>(a.p.z, b.p.z) * sc);
+        hij = (R)(0.5f * (__uint_as_float(a.p.w) + __uint_as_float(b.p.w)));
     } else {
         dx = (R)wrap_d<PER>(a.p.x - b.p.x, r.L);
         dy = (R)wrap_d<PER>(a.p.y - b.p.y, r.L);
@@ -82,9 +83,17 @@ __device__ __forceinline__ void nb_vel(const Nb<CMP> &a, const Nb<CMP> &b, R &ux
     mj = (R)b.v.w;
 }
 
-// Spacing of the compact records' grid (0 when they are not used).
+// h of a particle's own record.
 template <bool CMP>
-__device__ __forceinline__ float nb_spacing(const NgbRows &r)
+__device__ __forceinline__ double nb_h(const Nb<CMP> &a)
+{
+    if constexpr (CMP) return (double)__uint_as_float(a.p.w);
+    else return a.p.w;
+}
+
+// Step size of the compact records' fixed point (0 when they are not used).
+template <bool CMP>
+__device__ __forceinline__ float nb_scale(const NgbRows &r)
 {
     if constexpr (CMP) return (float)r.grid[7];
     else return 0.0f;
@@ -138,7 +147,7 @@ __global__ void __launch_bounds__(256) k_density(int64_t n, const double *__rest
     bool valid = i < n;
     Kahan<R> acc;
     if (valid) {
-        const float cs = nb_spacing<CMP>(rows);
+        const float cs = nb_scale<CMP>(rows);
         const Nb<CMP> bi = nb_load<CMP>(rows, i);
         int64_t k0, e;
         row_range(rows, i, k0, e);
@@ -231,9 +240,9 @@ __global__ void __launch_bounds__(256) k_density_divcurl(
     Kahan<R> acc, sd, scx, scy, scz;
     R hown = (R)0;
     if (valid) {
-        const float cs = nb_spacing<CMP>(rows);
+        const float cs = nb_scale<CMP>(rows);
         const Nb<CMP> bi = nb_load<CMP>(rows, i);
-        hown = (R)bi.p.w;
+        hown = (R)nb_h<CMP>(bi);
         int64_t k0, e;
         row_range(rows, i, k0, e);
         pair_loop<G, CMP, false, int>(rows, k0, e, l, nullptr, [&](const Nb<CMP> &bj, int) {
@@ -314,7 +323,7 @@ __global__ void __launch_bounds__(256) k_accsph(int64_t n, const R4<R> *__restri
     bool valid = i < n;
     Kahan<R> sx, sy, sz;
     if (valid) {
-        const float cs = nb_spacing<CMP>(rows);
+        const float cs = nb_scale<CMP>(rows);
         const Nb<CMP> bi = nb_load<CMP>(rows, i);
         const R4<R> si = s4[i];
         const R rhoi = si.x, pi_ = si.y, ci = si.z, fi = si.w;
@@ -359,7 +368,7 @@ __global__ void __launch_bounds__(256) k_dudt(int64_t n, const R4<R> *__restrict
     bool valid = i < n;
     Kahan<R> s;
     if (valid) {
-        const float cs = nb_spacing<CMP>(rows);
+        const float cs = nb_scale<CMP>(rows);
         const Nb<CMP> bi = nb_load<CMP>(rows, i);
         const R4<R> si = s4[i];
         const R rhoi = si.x, pi_ = si.y, ci = si.z, fi = si.w;
@@ -542,28 +551,28 @@ hipError_t pack4(hipStream_t st, int64_t n, const double *a, const double *b, co
     return hipGetLastError();
 }
 
-// Compact records (declared in sph_hydro.h).  Open boundaries (per = 0): lo = grid[0..2],
-// the largest extent E = max(grid[3+d] - grid[d]) = f 2^k with f in [0.5, 1) (frexp),
-// cs = 2^(k-10): the smallest power of two with 1024 cs > E, so every coordinate in the box
-// has a cell in [0, 1023] (the clamp only guards positions outside the box).  Periodic box of
-// side per: lo = 0, cs = per / 1024, cells taken modulo 1024 (a coordinate equal to per, which
-// the drift's wrap can round to, lands in cell 0 with an offset of about zero).  Every
-// thread derives cs itself; thread 0 stores it in grid[7] for the passes.
+// Compact records (declared in sph_hydro.h).  Open boundaries (per = 0): lo = grid[0..2], the
+// largest extent E = max(grid[3+d] - grid[d]) = f 2^k with f in [0.5, 1) (frexp), so
+// E < 2^k: the coordinates in the box map to [0, 2^32) in steps of sc = 2^(k-32) (the clamp
+// only guards positions outside the box).  Periodic box of side per: steps of sc = per / 2^32
+// from the origin, modulo 2^32 (a coordinate equal to per, which the drift's wrap can round
+// to, maps to 0).  Rounding to the nearest step.  Every thread derives sc itself; thread 0
+// stores it in grid[7] for the passes.
 __global__ void __launch_bounds__(256) k_pack_compact(int64_t n, const double *__restrict__ x,
                                                       const double *__restrict__ y, const double *__restrict__ z,
                                                       const double *__restrict__ h, const double *__restrict__ vx,
                                                       const double *__restrict__ vy, const double *__restrict__ vz,
                                                       const double *__restrict__ m, double per, double *__restrict__ grid,
-                                                      float4 *__restrict__ pf, float4 *__restrict__ vf,
-                                                      uint32_t *__restrict__ cell, bool vel_only)
+                                                      uint4 *__restrict__ pf, float4 *__restrict__ vf, bool vel_only)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     vf[i] = make_float4((float)vx[i], (float)vy[i], (float)vz[i], (float)m[i]);
     if (vel_only) return;
-    double lo[3] = {0.0, 0.0, 0.0}, cs;
+    double lo[3] = {0.0, 0.0, 0.0}, sc, isc;
     if (per > 0.0) {
-        cs = per / 1024.0;
+        sc = per * 0x1p-32;
+        isc = 0x1p32 / per;
     } else {
         lo[0] = grid[0];
         lo[1] = grid[1];
@@ -571,31 +580,30 @@ __global__ void __launch_bounds__(256) k_pack_compact(int64_t n, const double *_
         const double ext = fmax(fmax(grid[3] - grid[0], grid[4] - grid[1]), grid[5] - grid[2]);
         int k = 0;
         frexp(ext, &k);
-        cs = ldexp(1.0, k - 10);
+        sc = ldexp(1.0, k - 32);
+        isc = ldexp(1.0, 32 - k);
     }
-    if (i == 0) grid[7] = cs;
+    if (i == 0) grid[7] = sc;
     const double p[3] = {x[i], y[i], z[i]};
-    float o[3];
-    uint32_t c = 0;
+    uint32_t q[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        const double r = p[d] - lo[d];
-        double q = floor(r / cs);
-        q = per > 0.0 ? fmin(fmax(q, -1.0), 1024.0) : fmin(fmax(q, 0.0), 1023.0);
-        o[d] = (float)(r - q * cs);
-        const int qi = (int)q;
-        c |= (uint32_t)(qi < 0 ? qi + 1024 : (qi > 1023 ? qi - 1024 : qi)) << (10 * d);
+        double v = rint((p[d] - lo[d]) * isc);
+        if (per > 0.0) {
+            v = fmin(fmax(v, 0.0), 4294967296.0);
+            q[d] = (uint32_t)(uint64_t)v;  // 2^32 wraps to 0
+        } else {
+            q[d] = (uint32_t)fmin(fmax(v, 0.0), 4294967295.0);
+        }
     }
-    pf[i] = make_float4(o[0], o[1], o[2], (float)h[i]);
-    cell[i] = c;
+    pf[i] = make_uint4(q[0], q[1], q[2], __float_as_uint((float)h[i]));
 }
 
 hipError_t pack_compact(hipStream_t st, int64_t n, const double *x, const double *y, const double *z,
                         const double *h, const double *vx, const double *vy, const double *vz, const double *m,
-                        double per, double *grid, float4 *pf, float4 *vf, uint32_t *cell, bool vel_only)
+                        double per, double *grid, uint4 *pf, float4 *vf, bool vel_only)
 {
-    k_pack_compact<<<sph_blocks(n, 256), 256, 0, st>>>(n, x, y, z, h, vx, vy, vz, m, per, grid, pf, vf, cell,
-                                                       vel_only);
+    k_pack_compact<<<sph_blocks(n, 256), 256, 0, st>>>(n, x, y, z, h, vx, vy, vz, m, per, grid, pf, vf, vel_only);
     return hipGetLastError();
 }
 
