@@ -3,10 +3,11 @@
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU baseline
 may import this module; the product path never does.
 
-The network is 4 -> 128 -> 128 -> 128 -> 1 with SiLU activations (the design
+The network is 4 -> 128 -> 128 -> 128 -> 1 with ReLU activations (the design
 point is the reference's offline Keras surrogate,
 Particles_in_BH_Tree_III/CLOUDY_replace_CHIMES/HCooling_Here/ML/prog2.py:24-52,
-4 -> 128 -> 64 -> 1 ReLU trained with Adam on the mean squared error).
+4 -> 128 -> 64 -> 1 ReLU trained with Adam on the mean squared error; the
+derivative of ReLU is taken as 0 at 0, as torch does).
 Parameters are one flat vector: W1[128][4], b1, W2[128][128], b2, W3[128][128],
 b3, W4[128], b4 (weights stored [out][in]).
 
@@ -61,17 +62,12 @@ def bf16(a):
     return (u.astype(np.uint32) << 16).view(np.float32)
 
 
-def _sig(z):
-    return 1.0 / (1.0 + np.exp(-z))
+def relu(z):
+    return np.maximum(z, 0.0)
 
 
-def silu(z):
-    return z * _sig(z)
-
-
-def dsilu(z):
-    s = _sig(z)
-    return s * (1.0 + z * (1.0 - s))
+def drelu(z):
+    return (np.asarray(z) > 0.0).astype(np.float64)
 
 
 def _f32(a):
@@ -84,22 +80,22 @@ def forward(p, x, emulate=False):
     x = np.asarray(x, np.float64)
     if not emulate:
         z1 = x @ b["W1"].T + b["b1"]
-        h1 = silu(z1)
+        h1 = relu(z1)
         z2 = h1 @ b["W2"].T + b["b2"]
-        h2 = silu(z2)
+        h2 = relu(z2)
         z3 = h2 @ b["W3"].T + b["b3"]
-        h3 = silu(z3)
+        h3 = relu(z3)
         y = h3 @ b["W4"] + b["b4"][0]
         return y, dict(x=x, z=(z1, z2, z3), h=(h1, h2, h3), W2=b["W2"], W3=b["W3"], W4=b["W4"])
     x = _f32(x)
     z1 = _f32(x @ b["W1"].T + b["b1"])
-    h1 = bf16(silu(z1)).astype(np.float64)
+    h1 = bf16(relu(z1)).astype(np.float64)
     W2 = bf16(b["W2"]).astype(np.float64)
     W3 = bf16(b["W3"]).astype(np.float64)
     z2 = _f32(_f32(h1 @ W2.T) + b["b2"])
-    h2 = bf16(silu(z2)).astype(np.float64)
+    h2 = bf16(relu(z2)).astype(np.float64)
     z3 = _f32(_f32(h2 @ W3.T) + b["b3"])
-    h3 = bf16(silu(z3)).astype(np.float64)
+    h3 = bf16(relu(z3)).astype(np.float64)
     y = _f32(_f32(h3 @ b["W4"]) + b["b4"][0])
     zb = tuple(bf16(z).astype(np.float64) for z in (z1, z2, z3))
     return y, dict(x=x, z=zb, h=(h1, h2, h3), W2=W2, W3=W3, W4=b["W4"])
@@ -121,13 +117,13 @@ def gradient(p, x, t, emulate=False):
     g = {}
     g["W4"] = dy @ h3
     g["b4"] = np.array([dy.sum()])
-    dz3 = r(dy[:, None] * c["W4"][None, :] * dsilu(z3))
+    dz3 = r(dy[:, None] * c["W4"][None, :] * drelu(z3))
     g["W3"] = dz3.T @ h2
     g["b3"] = dz3.sum(0)
-    dz2 = r((dz3 @ c["W3"]) * dsilu(z2))
+    dz2 = r((dz3 @ c["W3"]) * drelu(z2))
     g["W2"] = dz2.T @ h1
     g["b2"] = dz2.sum(0)
-    dz1 = r((dz2 @ c["W2"]) * dsilu(z1))
+    dz1 = r((dz2 @ c["W2"]) * drelu(z1))
     g["W1"] = dz1.T @ c["x"]
     g["b1"] = dz1.sum(0)
     return pack(g), loss

@@ -190,7 +190,7 @@ def _f32(a, shape=None):
 
 
 class CoolingSurrogate:
-    """Learned log10 Lambda: an MLP 4 -> 128 -> 128 -> 128 -> 1 (SiLU) over
+    """Learned log10 Lambda: an MLP 4 -> 128 -> 128 -> 128 -> 1 (ReLU) over
     (log10 nH, log10 T, log10 Z, log10 J21), BF16 weights/activations with FP32
     accumulation on the matrix cores, trained on the device by its own forward
     and backward kernels on samples of the exact cooling function.

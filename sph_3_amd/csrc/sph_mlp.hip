@@ -1,11 +1,12 @@
 // sph_mlp.hip -- learned surrogate of the per-particle cooling rate: an MLP
-// 4 -> 128 -> 128 -> 128 -> 1 with SiLU activations mapping the normalised
+// 4 -> 128 -> 128 -> 128 -> 1 with ReLU activations mapping the normalised
 // (log10 n_H, log10 T, log10 Z, log10 J21) to the normalised log10 Lambda.
 //
 // Design point: the Keras surrogate of
 // Particles_in_BH_Tree_III/CLOUDY_replace_CHIMES/HCooling_Here/ML/prog2.py:24-52
-// (4 -> 128 -> 64 -> 1, Adam on the mean squared error), which the reference
-// trains and runs offline only.  Here training and inference are device kernels:
+// (4 -> 128 -> 64 -> 1 with ReLU activations, Adam on the mean squared error), which the
+// reference trains and runs offline only.  ReLU costs one maximum per hidden unit, so the
+// two MFMA layers are not held back by activation arithmetic on the VALU.  Here training and inference are device kernels:
 //   * weights and activations BF16, accumulation FP32; the two 128 x 128
 //     layers run on v_mfma_f32_32x32x8_bf16, a workgroup of four wavefronts per
 //     tile of 32 samples (each wavefront owns 32 of the 128 features of a
@@ -18,11 +19,11 @@
 //     result is the table's 4D multilinear interpolation instead.
 //
 // Numerics of one sample (what the CPU oracle emulates):
-//   z1 = W1 x + b1 (FP32),           h1 = bf16(silu(z1))
-//   z2 = bf16(W2) h1 + b2 (FP32 acc), h2 = bf16(silu(z2))   (same for layer 3)
+//   z1 = W1 x + b1 (FP32),           h1 = bf16(relu(z1))
+//   z2 = bf16(W2) h1 + b2 (FP32 acc), h2 = bf16(relu(z2))   (same for layer 3)
 //   y  = W4 . h3 + b4 (FP32)
-//   backward: dz3 = bf16(dy W4 silu'(z3)); dz2 = bf16((dz3 bf16(W3)) silu'(z2));
-//   dz1 = bf16((dz2 bf16(W2)) silu'(z1)); z_l are kept in BF16 for silu'.
+//   backward: dz3 = bf16(dy W4 relu'(z3)); dz2 = bf16((dz3 bf16(W3)) relu'(z2));
+//   dz1 = bf16((dz2 bf16(W2)) relu'(z1)); z_l are kept in BF16 for relu' (its sign).
 //
 // MFMA operand layout (32x32x8, wave64): lane l supplies A[l%32][4(l/32)+i]
 // and B[4(l/32)+i][l%32], i = 0..3; accumulator r holds
@@ -60,13 +61,9 @@ __device__ __forceinline__ unsigned short f2bf(float f)
     return (unsigned short)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float(((unsigned int)b) << 16); }
-__device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
-__device__ __forceinline__ float silu(float z) { return z * sigm(z); }
-__device__ __forceinline__ float dsilu(float z)
-{
-    const float s = sigm(z);
-    return s * (1.0f + z * (1.0f - s));
-}
+// ReLU and its derivative (taken as 0 at z = 0).
+__device__ __forceinline__ float relu(float z) { return fmaxf(z, 0.0f); }
+__device__ __forceinline__ float drelu(float z) { return z > 0.0f ? 1.0f : 0.0f; }
 
 __device__ __forceinline__ bf16x4 as_bf16x4(uint2 v)
 {
@@ -110,8 +107,6 @@ typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 #define IS 64                 // samples per wave
 #define ITILE (IS * LDP)      // activation buffer [64][LDP]
 
-__device__ __forceinline__ float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
-
 __device__ __forceinline__ unsigned int pack2(float a, float b)
 {
     bf16x2 v;
@@ -137,7 +132,7 @@ __device__ __forceinline__ float mlp_forward64(const float xin[4], const float *
             float z = P[O_B1 + f];
 #pragma unroll
             for (int k = 0; k < 4; k++) z = fmaf(P[O_W1 + f * 4 + k], xin[k], z);
-            h[j] = silu_fast(z);
+            h[j] = relu(z);
         }
         uint4 w;
         w.x = pack2(h[0], h[1]);
@@ -181,10 +176,10 @@ __device__ __forceinline__ float mlp_forward64(const float xin[4], const float *
                 if (L == 0) {
 #pragma unroll
                     for (int nt = 0; nt < 2; nt++) {
-                        const float h0 = silu_fast(acc[mt][nt][4 * q] + bias.x);
-                        const float h1 = silu_fast(acc[mt][nt][4 * q + 1] + bias.y);
-                        const float h2 = silu_fast(acc[mt][nt][4 * q + 2] + bias.z);
-                        const float h3 = silu_fast(acc[mt][nt][4 * q + 3] + bias.w);
+                        const float h0 = relu(acc[mt][nt][4 * q] + bias.x);
+                        const float h1 = relu(acc[mt][nt][4 * q + 1] + bias.y);
+                        const float h2 = relu(acc[mt][nt][4 * q + 2] + bias.z);
+                        const float h3 = relu(acc[mt][nt][4 * q + 3] + bias.w);
                         uint2 w;
                         w.x = pack2(h0, h1);
                         w.y = pack2(h2, h3);
@@ -194,10 +189,10 @@ __device__ __forceinline__ float mlp_forward64(const float xin[4], const float *
                     const float4 w4 = *reinterpret_cast<const float4 *>(P + O_W4 + f);
 #pragma unroll
                     for (int nt = 0; nt < 2; nt++) {
-                        y[nt] = fmaf(w4.x, bf16_round(silu_fast(acc[mt][nt][4 * q] + bias.x)), y[nt]);
-                        y[nt] = fmaf(w4.y, bf16_round(silu_fast(acc[mt][nt][4 * q + 1] + bias.y)), y[nt]);
-                        y[nt] = fmaf(w4.z, bf16_round(silu_fast(acc[mt][nt][4 * q + 2] + bias.z)), y[nt]);
-                        y[nt] = fmaf(w4.w, bf16_round(silu_fast(acc[mt][nt][4 * q + 3] + bias.w)), y[nt]);
+                        y[nt] = fmaf(w4.x, bf16_round(relu(acc[mt][nt][4 * q] + bias.x)), y[nt]);
+                        y[nt] = fmaf(w4.y, bf16_round(relu(acc[mt][nt][4 * q + 1] + bias.y)), y[nt]);
+                        y[nt] = fmaf(w4.z, bf16_round(relu(acc[mt][nt][4 * q + 2] + bias.z)), y[nt]);
+                        y[nt] = fmaf(w4.w, bf16_round(relu(acc[mt][nt][4 * q + 3] + bias.w)), y[nt]);
                     }
                 }
             }
@@ -341,7 +336,7 @@ __global__ void __launch_bounds__(64 * TW) k_mlp_train(int64_t n, int64_t first,
         float zz = P[O_B1 + f];
 #pragma unroll
         for (int k = 0; k < 4; k++) zz = fmaf(P[O_W1 + f * 4 + k], xin[k], zz);
-        h1[sl * LDP + f] = f2bf(silu(zz));
+        h1[sl * LDP + f] = f2bf(relu(zz));
         z1[sl * LDP + f] = f2bf(zz);
     }
     __syncthreads();
@@ -360,7 +355,7 @@ __global__ void __launch_bounds__(64 * TW) k_mlp_train(int64_t n, int64_t first,
         for (int r = 0; r < 16; r++) {
             const int row = drow(r, lane);
             const float zz = acc[r] + bias;
-            hout[row * LDP + f] = f2bf(silu(zz));
+            hout[row * LDP + f] = f2bf(relu(zz));
             zout[row * LDP + f] = f2bf(zz);
         }
         __syncthreads();
@@ -402,11 +397,11 @@ __global__ void __launch_bounds__(64 * TW) k_mlp_train(int64_t n, int64_t first,
         unsafeAtomicAdd(&G[O_B4], gb);
     }
     __syncthreads();  // h3 is read above; z3 becomes dz3 below and h3 dz2 later
-    // dz3 = dy W4 silu'(z3), in place over z3
+    // dz3 = dy W4 relu'(z3), in place over z3
     for (int q = 0; q < 16; q++) {
         const int f = fb + 16 * half + q;
         const float zz = bf2f(z3[sl * LDP + f]);
-        z3[sl * LDP + f] = f2bf(dys[sl] * P[O_W4 + f] * dsilu(zz));
+        z3[sl * LDP + f] = f2bf(dys[sl] * P[O_W4 + f] * drelu(zz));
     }
     __syncthreads();
 
@@ -446,7 +441,7 @@ __global__ void __launch_bounds__(64 * TW) k_mlp_train(int64_t n, int64_t first,
             for (int s = 0; s < TS; s++) gb += bf2f(dz[s * LDP + f]);
             unsafeAtomicAdd(&G[obb + f], gb);
         }
-        // dh_prev = dz W (B(o, i) = W[o][i] = Wt[i][o]); dz_prev = dh_prev silu'(z_prev); wavefront
+        // dh_prev = dz W (B(o, i) = W[o][i] = Wt[i][o]); dz_prev = dh_prev relu'(z_prev); wavefront
         // w takes the input features i in fb..fb+31
         unsigned short *nx = L == 0 ? h3 : z3;
         {
@@ -455,7 +450,7 @@ __global__ void __launch_bounds__(64 * TW) k_mlp_train(int64_t n, int64_t first,
 #pragma unroll
             for (int r = 0; r < 16; r++) {
                 const int s = drow(r, lane);
-                nx[s * LDP + i] = f2bf(acc[r] * dsilu(bf2f(zp[s * LDP + i])));
+                nx[s * LDP + i] = f2bf(acc[r] * drelu(bf2f(zp[s * LDP + i])));
             }
         }
         __syncthreads();

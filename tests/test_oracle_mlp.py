@@ -51,9 +51,9 @@ def test_gradient_vs_torch_autograd():
     g, loss = mlp.gradient(p, x, t)
     b = {k: torch.tensor(v.copy(), dtype=torch.float64, requires_grad=True) for k, v in mlp.unpack(p).items()}
     xt = torch.tensor(x)
-    h = torch.nn.functional.silu(xt @ b["W1"].T + b["b1"])
-    h = torch.nn.functional.silu(h @ b["W2"].T + b["b2"])
-    h = torch.nn.functional.silu(h @ b["W3"].T + b["b3"])
+    h = torch.nn.functional.relu(xt @ b["W1"].T + b["b1"])
+    h = torch.nn.functional.relu(h @ b["W2"].T + b["b2"])
+    h = torch.nn.functional.relu(h @ b["W3"].T + b["b3"])
     y = h @ b["W4"] + b["b4"][0]
     lt = torch.mean((y - torch.tensor(t)) ** 2)
     lt.backward()
