@@ -313,7 +313,7 @@ def mlp_section(n_inf, epochs=200):
     lt, _ = tab.lookup(*v.T)
     rms = lambda e: float(np.sqrt(np.mean(e * e)))  # noqa: E731
     return {
-        "workload": f"cooling surrogate 4-128-128-128-1 SiLU, {n_inf} particles uniform in the table box",
+        "workload": f"cooling surrogate 4-128-128-128-1 ReLU, {n_inf} particles uniform in the table box",
         "train": {"samples": n_samples, "epochs": epochs, "seconds": t_train, "final_loss": float(losses[-1]),
                   "tflops": 3 * MLP_FLOP_PER_SAMPLE * n_samples * epochs / t_train / 1e12},
         "ms_gated": tm["ms_gated"], "ms_network": tm["ms_network"], "ms_table_lookup": tm["ms_table"],

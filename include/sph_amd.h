@@ -166,7 +166,7 @@ int sph_cooling_table_lookup(sph_ctx *ctx, int64_t n, const double *lognH, const
                              const double *logJ21, double *logL_out, double *logG_out);
 
 /* ---- learned cooling surrogate ----------------------------------------------
- * An MLP 4 -> 128 -> 128 -> 128 -> 1 (SiLU) from the normalised
+ * An MLP 4 -> 128 -> 128 -> 128 -> 1 (ReLU) from the normalised
  * (log10 nH, log10 T, log10 Z, log10 J21) to the normalised log10 Lambda, BF16
  * weights and activations with FP32 accumulation on the gfx950 matrix cores.
  * Its design point is the reference's offline Keras surrogate

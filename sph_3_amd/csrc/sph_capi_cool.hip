@@ -179,6 +179,7 @@ MLPDev mlp_dev(sph_ctx *ctx)
     d.W3b = P_<unsigned short>(ctx->mW[1]);
     d.W2tb = P_<unsigned short>(ctx->mW[2]);
     d.W3tb = P_<unsigned short>(ctx->mW[3]);
+    d.img = P_<uint4>(ctx->mimg);
     return d;
 }
 
@@ -189,6 +190,7 @@ int mlp_alloc(sph_ctx *ctx)
     Buf *ps[] = {&ctx->mP, &ctx->mG, &ctx->mM, &ctx->mV};
     for (Buf *b : ps) CK(ensure(ctx, *b, (size_t)MLP_NPARAM * 4));
     for (int k = 0; k < 4; k++) CK(ensure(ctx, ctx->mW[k], (size_t)MLP_HIDDEN * MLP_HIDDEN * 2));
+    CK(ensure(ctx, ctx->mimg, (size_t)MLP_IMG_BYTES));
     CK(ensure(ctx, ctx->mcnt, 64));
     return SPH_OK;
 }
@@ -378,6 +380,7 @@ extern "C" int sph_mlp_train_batch(sph_ctx *ctx, int64_t n, const float *x, cons
     if (n < 1 || !x || !t || !(lr > 0.0)) return fail(ctx, SPH_ERR_ARG, "need n >= 1, non-null x, t and lr > 0");
     CK(mlp_grad_on(ctx, n, x, t, loss));
     HIPCK(sph::mlp_adam(ctx->st, mlp_dev(ctx), (float)lr, 0.9f, 0.999f, 1e-8f, ++ctx->mlp_step));
+    HIPCK(sph::mlp_image(ctx->st, mlp_dev(ctx)));
     HIPCK(hipStreamSynchronize(ctx->st));
     ctx->mlp_cal = false;
     return SPH_OK;
@@ -422,6 +425,7 @@ extern "C" int sph_mlp_train(sph_ctx *ctx, int64_t n_samples, int epochs, int ba
             HIPCK(sph::mlp_adam(ctx->st, d, (float)lr_t, 0.9f, 0.999f, 1e-8f, ++ctx->mlp_step));
         }
     }
+    HIPCK(sph::mlp_image(ctx->st, d));
     std::vector<float> l(epochs);
     HIPCK(hipMemcpyAsync(l.data(), ctx->mloss.p, (size_t)epochs * 4, hipMemcpyDeviceToHost, ctx->st));
     HIPCK(hipStreamSynchronize(ctx->st));

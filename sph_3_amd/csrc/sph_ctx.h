@@ -104,7 +104,7 @@ struct sph_ctx {
     Buf tsdt, tsdh;         // dt_i, dh/dt_i (sorted order, FP64)
     sph_cool_params cool_p{};
     // cooling surrogate: parameters, Adam state, BF16 weight copies, samples, error map
-    Buf mP, mG, mM, mV, mW[4], mx, mxn, mt, my, mloss, memap, memap2, memap3, mgate, mcnt, mfl;
+    Buf mP, mG, mM, mV, mW[4], mimg, mx, mxn, mt, my, mloss, memap, memap2, memap3, mgate, mcnt, mfl;
     MLPNorm mnorm{};
     bool mlp_ok = false, mlp_cal = false;
     int mlp_step = 0;

@@ -13,12 +13,17 @@ struct MLPNorm {
     float mu, sigma;
 };
 
+// Inference weights in MFMA operand order (sph_mlp.hip, k_mlp_image): 5,224 records of 16 bytes.
+#define MLP_IMG_BYTES 83584
+
 // Device parameters: FP32 masters, gradient and Adam moments (MLP_NPARAM each,
 // flat layout W1[128][4], b1, W2[128][128], b2, W3[128][128], b3, W4[128], b4),
-// and BF16 copies of W2, W3 (row-major [out][in]) and their transposes.
+// BF16 copies of W2, W3 (row-major [out][in]) and their transposes for training,
+// and the inference image (MLP_IMG_BYTES) built from the masters.
 struct MLPDev {
     float *P, *G, *M, *V;
     unsigned short *W2b, *W3b, *W2tb, *W3tb;
+    uint4 *img;
 };
 
 namespace sph {
@@ -37,10 +42,13 @@ inline int64_t mlp_infer_scratch_ints(int64_t n) { return n + 64; }
 // loss (batch mean) added to *loss.
 hipError_t mlp_train_batch(hipStream_t st, int64_t n, int64_t first, int64_t bsz, int64_t mult, int64_t off,
                            const float *x, const float *t, const MLPDev &d, float *loss);
-// Adam step (bias-corrected, step >= 1) on the masters; refreshes the BF16 copies and clears d.G.
+// Adam step (bias-corrected, step >= 1) on the masters; refreshes the BF16 training copies and
+// clears d.G (the inference image is rebuilt by mlp_image once the training calls are done).
 hipError_t mlp_adam(hipStream_t st, const MLPDev &d, float lr, float b1, float b2, float eps, int step);
-// BF16 copies from the masters.
+// BF16 training copies and the inference image from the masters.
 hipError_t mlp_refresh(hipStream_t st, const MLPDev &d);
+// Inference image from the masters (after the weights changed).
+hipError_t mlp_image(hipStream_t st, const MLPDev &d);
 // Sample coordinates: per > 0 stratified (per points in each cell of the grid nc), else uniform.
 // xraw[n][4] log10 values, xn[n][4] normalised.
 hipError_t mlp_samples(hipStream_t st, int64_t n, uint64_t seed, const MLPNorm &nm, const int *nc, int per,

@@ -94,130 +94,266 @@ __device__ __forceinline__ floatx16 tile_gemm(const unsigned short *A, const uns
 }
 
 // ---------------------------------------------------------------- inference
-// Inference works on 64 samples per wavefront and computes each hidden layer
-// transposed, Z^T = W H^T: the MFMA A operand is a 32 x 16 slice of the
-// row-major BF16 weight (16-byte global loads), the B operand 16 features of
-// 32 samples from the LDS activation rows, and each lane's accumulators hold
-// 4 consecutive features of one sample, so the BF16 activations go back to
-// LDS as 8-byte stores (in place: a layer's reads all precede its writes) and
-// the output layer is a register dot product in the layer-3 epilogue.
+// Inference keeps every activation in registers.  A wavefront takes 64 samples as two
+// 32-sample tiles (nt = 0, 1) and computes each layer transposed, Z^T = W H^T, on
+// v_mfma_f32_32x32x16_bf16: the A operand is a 32 x 16 block of the layer's weights, the B
+// operand 16 inputs of the tile's 32 samples.  Accumulator r of a lane in output block mt
+// holds unit 32 mt + 8 (r/4) + 4 half + r%4 of sample l%32.  After the bias (the first MFMA's C
+// input), ReLU and BF16 rounding, accumulators 8p..8p+7 of block mt ARE the lane's B operand
+// for the next layer's input block kk = 2 mt + p, provided that layer's weight columns are
+// stored in the matching order: B slot i of lane half h in block kk carries hidden unit
+// hperm(16 kk + 8 h + i) = 32 (kk/2) + 16 (kk%2) + 8 (i/4) + 4 h + i%4.  No activation goes
+// through LDS.  The first layer (4 inputs) and the output layer run on the same MFMA with split
+// operands: x = xh + xm + xl and W1 = Wh + Wm + Wl (BF16 each, exact FP32 remainders) give
+// W1 x from the six products whose order is at least 2^-16 (two 16-deep MFMA blocks), within
+// about 2^-24 relative like the FP32 sum, and y = W4h.h3 + W4m.h3 + W4l.h3 (rows 0..2 of the
+// output tile) carries the FP32 W4.  The weights, in operand order (the image, built by
+// k_mlp_image), are copied to LDS once per workgroup; one workgroup of eight wavefronts per CU
+// loops over the tiles.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef short s16x2 __attribute__((ext_vector_type(2)));
 
-#define IS 64                 // samples per wave
-#define ITILE (IS * LDP)      // activation buffer [64][LDP]
+#define IS 64                        // samples per wavefront tile pair
+#define IWAVES 8                     // wavefronts per inference workgroup
+#define IMG_W1 0                     // [mt 4][kb 2][lane 64]
+#define IMG_W2 512                   // [mt 4][kk 8][lane 64]
+#define IMG_W3 (IMG_W2 + 2048)       // [mt 4][kk 8][lane 64]
+#define IMG_W4 (IMG_W3 + 2048)       // [kk 8][lane 64]: rows 0..2 (W4h, W4m, W4l)
+#define IMG_B (IMG_W4 + 512)         // [layer 3][mt 4][half 2][4]: biases in accumulator order
+#define IMG_B4 (IMG_B + 96)          // [half 2][4]: b4 at accumulator 0 of half 0
+#define IMG_U4 (IMG_B4 + 8)          // records of 16 bytes
+static_assert(IMG_U4 * 16 == MLP_IMG_BYTES, "inference image size");
+
+__device__ __forceinline__ int hperm(int k)
+{
+    const int kk = k >> 4, h = (k >> 3) & 1, i = k & 7;
+    return 32 * (kk >> 1) + 16 * (kk & 1) + 8 * (i >> 2) + 4 * h + (i & 3);
+}
+
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ unsigned int pack2(float a, float b)
 {
-    bf16x2 v;
-    v[0] = (__bf16)a;  // v_cvt_pk_bf16_f32: round to nearest even
-    v[1] = (__bf16)b;
-    return __builtin_bit_cast(unsigned int, v);
+    // one v_cvt_pk_bf16_f32 (round to nearest even) for the pair
+    return __builtin_bit_cast(unsigned int, __builtin_convertvector((f32x2){a, b}, bf16x2));
 }
 
-__device__ __forceinline__ float bf16_round(float a) { return (float)(__bf16)a; }
-
-// Normalised output of sample (blockIdx.x*64 + lane) for this lane's input xin.
-__device__ __forceinline__ float mlp_forward64(const float xin[4], const float *__restrict__ P,
-                                               const unsigned short *__restrict__ W2b,
-                                               const unsigned short *__restrict__ W3b, unsigned short *act)
+__device__ __forceinline__ uint4 pack8(const unsigned short v[8])
 {
-    const int lane = threadIdx.x & 63, l32 = lane & 31, half = lane >> 5;
-    // layer 1 on VALU: this lane's sample, all 128 features (weights are wave-uniform)
-    for (int f0 = 0; f0 < HW; f0 += 8) {
-        float h[8];
+    return make_uint4(v[0] | ((unsigned)v[1] << 16), v[2] | ((unsigned)v[3] << 16), v[4] | ((unsigned)v[5] << 16),
+                      v[6] | ((unsigned)v[7] << 16));
+}
+
+// BF16 parts of an FP32 value: w = hi + mid + lo to 24 bits (each remainder exact in FP32).
+__device__ __forceinline__ void split3(float w, unsigned short &hi, unsigned short &mid, unsigned short &lo)
+{
+    hi = f2bf(w);
+    const float r = w - bf2f(hi);
+    mid = f2bf(r);
+    lo = f2bf(r - bf2f(mid));
+}
+
+// One record of the image per thread (the W4 rows from 3 on and the unused slots are zero).
+// Layer 1, output unit f = 32 mt + l%32, against the B slots of mlp_tiles:
+//   block 0, half 0: [Wh, Wh] x [xh, xm];  block 0, half 1: [Wm, Wh] x [xh, xl];
+//   block 1, half 0: [Wm, Wl] x [xm, xh];  block 1, half 1: zero.
+__global__ void __launch_bounds__(256) k_mlp_image(const float *__restrict__ P, uint4 *__restrict__ img)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= IMG_U4) return;
+    if (e < IMG_B) {
+        unsigned short v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (e < IMG_W2) {
+            const int mt = e >> 7, kb = (e >> 6) & 1, lane = e & 63, f = 32 * mt + (lane & 31);
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int f = f0 + j;
-            float z = P[O_B1 + f];
-#pragma unroll
-            for (int k = 0; k < 4; k++) z = fmaf(P[O_W1 + f * 4 + k], xin[k], z);
-            h[j] = relu(z);
-        }
-        uint4 w;
-        w.x = pack2(h[0], h[1]);
-        w.y = pack2(h[2], h[3]);
-        w.z = pack2(h[4], h[5]);
-        w.w = pack2(h[6], h[7]);
-        *reinterpret_cast<uint4 *>(act + lane * LDP + f0) = w;
-    }
-    __syncthreads();
-    float y[2] = {0.0f, 0.0f};
-#pragma unroll
-    for (int L = 0; L < 2; L++) {
-        const unsigned short *W = L == 0 ? W2b : W3b;
-        floatx16 acc[4][2];
-#pragma unroll
-        for (int mt = 0; mt < 4; mt++) acc[mt][0] = acc[mt][1] = floatx16{};
-#pragma unroll 1
-        for (int kk = 0; kk < HW; kk += 16) {
-            bf16x8 bop[2];
-#pragma unroll
-            for (int nt = 0; nt < 2; nt++)
-                bop[nt] = *reinterpret_cast<const bf16x8 *>(act + (nt * 32 + l32) * LDP + kk + 8 * half);
-#pragma unroll
-            for (int mt = 0; mt < 4; mt++) {
-                const bf16x8 aop =
-                    *reinterpret_cast<const bf16x8 *>(W + (size_t)(mt * 32 + l32) * HW + kk + 8 * half);
-#pragma unroll
-                for (int nt = 0; nt < 2; nt++)
-                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aop, bop[nt], acc[mt][nt], 0, 0, 0);
-            }
-        }
-        __syncthreads();
-        const int ob = L == 0 ? O_B2 : O_B3;
-#pragma unroll
-        for (int mt = 0; mt < 4; mt++)
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                // accumulators 4q..4q+3: features f..f+3 of samples nt*32 + l32
-                const int f = mt * 32 + 8 * q + 4 * half;
-                const float4 bias = *reinterpret_cast<const float4 *>(P + ob + f);
-                if (L == 0) {
-#pragma unroll
-                    for (int nt = 0; nt < 2; nt++) {
-                        const float h0 = relu(acc[mt][nt][4 * q] + bias.x);
-                        const float h1 = relu(acc[mt][nt][4 * q + 1] + bias.y);
-                        const float h2 = relu(acc[mt][nt][4 * q + 2] + bias.z);
-                        const float h3 = relu(acc[mt][nt][4 * q + 3] + bias.w);
-                        uint2 w;
-                        w.x = pack2(h0, h1);
-                        w.y = pack2(h2, h3);
-                        *reinterpret_cast<uint2 *>(act + (nt * 32 + l32) * LDP + f) = w;
-                    }
-                } else {
-                    const float4 w4 = *reinterpret_cast<const float4 *>(P + O_W4 + f);
-#pragma unroll
-                    for (int nt = 0; nt < 2; nt++) {
-                        y[nt] = fmaf(w4.x, bf16_round(relu(acc[mt][nt][4 * q] + bias.x)), y[nt]);
-                        y[nt] = fmaf(w4.y, bf16_round(relu(acc[mt][nt][4 * q + 1] + bias.y)), y[nt]);
-                        y[nt] = fmaf(w4.z, bf16_round(relu(acc[mt][nt][4 * q + 2] + bias.z)), y[nt]);
-                        y[nt] = fmaf(w4.w, bf16_round(relu(acc[mt][nt][4 * q + 3] + bias.w)), y[nt]);
-                    }
+            for (int i = 0; i < 4; i++) {
+                unsigned short wh, wm, wl;
+                split3(P[O_W1 + 4 * f + i], wh, wm, wl);
+                if (kb == 0 && lane < 32) {
+                    v[i] = wh;
+                    v[4 + i] = wh;
+                } else if (kb == 0) {
+                    v[i] = wm;
+                    v[4 + i] = wh;
+                } else if (lane < 32) {
+                    v[i] = wm;
+                    v[4 + i] = wl;
                 }
             }
-        __syncthreads();
+        } else if (e < IMG_W4) {  // hidden layers: row 32 mt + l%32, columns in B-slot order
+            const bool l3 = e >= IMG_W3;
+            const int q = e - (l3 ? IMG_W3 : IMG_W2), mt = q >> 9, kk = (q >> 6) & 7, lane = q & 63;
+            const float *W = P + (l3 ? O_W3 : O_W2) + (size_t)(32 * mt + (lane & 31)) * HW;
+#pragma unroll
+            for (int i = 0; i < 8; i++) v[i] = f2bf(W[hperm(16 * kk + 8 * (lane >> 5) + i)]);
+        } else {  // output layer: rows 0, 1, 2 the BF16 parts of W4
+            const int q = e - IMG_W4, kk = q >> 6, lane = q & 63, row = lane & 31;
+            if (row < 3)
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    unsigned short wh, wm, wl;
+                    split3(P[O_W4 + hperm(16 * kk + 8 * (lane >> 5) + i)], wh, wm, wl);
+                    v[i] = row == 0 ? wh : (row == 1 ? wm : wl);
+                }
+        }
+        img[e] = pack8(v);
+    } else {
+        float f[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (e < IMG_B4) {  // accumulator r = 4 j + c of half h in block mt: unit 32 mt + 8 j + 4 h + c
+            const int q = e - IMG_B, L = q >> 5, mt = (q >> 3) & 3, h = (q >> 2) & 1, j = q & 3;
+            const float *b = P + (L == 0 ? O_B1 : (L == 1 ? O_B2 : O_B3));
+#pragma unroll
+            for (int c = 0; c < 4; c++) f[c] = b[32 * mt + 8 * j + 4 * h + c];
+        } else if (e == IMG_B4) {
+            f[0] = P[O_B4];
+        }
+        img[e] = make_uint4(__float_as_uint(f[0]), __float_as_uint(f[1]), __float_as_uint(f[2]),
+                            __float_as_uint(f[3]));
     }
-    // lanes l and l^32 hold the two feature halves of samples l32 and 32 + l32
-    y[0] += __shfl_xor(y[0], 32);
-    y[1] += __shfl_xor(y[1], 32);
-    return (half ? y[1] : y[0]) + P[O_B4];
+}
+
+__device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+// Four consecutive records as one accumulator set (the C input of a block's first MFMA).
+__device__ __forceinline__ floatx16 lds_acc(const uint4 *p)
+{
+    const uint4 a = p[0], b = p[1], c = p[2], d = p[3];
+    floatx16 r;
+    r[0] = __uint_as_float(a.x); r[1] = __uint_as_float(a.y); r[2] = __uint_as_float(a.z); r[3] = __uint_as_float(a.w);
+    r[4] = __uint_as_float(b.x); r[5] = __uint_as_float(b.y); r[6] = __uint_as_float(b.z); r[7] = __uint_as_float(b.w);
+    r[8] = __uint_as_float(c.x); r[9] = __uint_as_float(c.y); r[10] = __uint_as_float(c.z); r[11] = __uint_as_float(c.w);
+    r[12] = __uint_as_float(d.x); r[13] = __uint_as_float(d.y); r[14] = __uint_as_float(d.z); r[15] = __uint_as_float(d.w);
+    return r;
+}
+
+// Accumulators 8p..8p+7 -> ReLU -> BF16 (round to nearest even), as a B operand.  ReLU acts on
+// the packed BF16 pairs: read as signed 16-bit integers the negative BF16 values are exactly the
+// negative integers, so a packed integer maximum with 0 is relu(bf16(z)) = bf16(relu(z)).
+__device__ __forceinline__ bf16x8 relu_bf16(const floatx16 &a, int p)
+{
+    uint4 u;
+    u.x = pack2(a[8 * p + 0], a[8 * p + 1]);
+    u.y = pack2(a[8 * p + 2], a[8 * p + 3]);
+    u.z = pack2(a[8 * p + 4], a[8 * p + 5]);
+    u.w = pack2(a[8 * p + 6], a[8 * p + 7]);
+    const s16x2 z = {0, 0};
+    u.x = __builtin_bit_cast(unsigned int, __builtin_elementwise_max(__builtin_bit_cast(s16x2, u.x), z));
+    u.y = __builtin_bit_cast(unsigned int, __builtin_elementwise_max(__builtin_bit_cast(s16x2, u.y), z));
+    u.z = __builtin_bit_cast(unsigned int, __builtin_elementwise_max(__builtin_bit_cast(s16x2, u.z), z));
+    u.w = __builtin_bit_cast(unsigned int, __builtin_elementwise_max(__builtin_bit_cast(s16x2, u.w), z));
+    return as_bf16x8(u);
+}
+
+// Hidden layer: out[kk][nt] (B operands of the next layer) from in[kk][nt].
+__device__ __forceinline__ void hidden_layer(const uint4 *__restrict__ W, const uint4 *__restrict__ bias,
+                                             const bf16x8 (&in)[8][2], bf16x8 (&out)[8][2], int lane)
+{
+    const int h = lane >> 5;
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+        const floatx16 c = lds_acc(bias + mt * 8 + h * 4);
+        floatx16 acc[2];
+        const bf16x8 a0 = as_bf16x8(W[(mt * 8) * 64 + lane]);
+        acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, in[0][0], c, 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, in[0][1], c, 0, 0, 0);
+#pragma unroll
+        for (int kk = 1; kk < 8; kk++) {
+            const bf16x8 a = as_bf16x8(W[(mt * 8 + kk) * 64 + lane]);
+            acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, in[kk][0], acc[0], 0, 0, 0);
+            acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, in[kk][1], acc[1], 0, 0, 0);
+        }
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++) {
+            out[2 * mt][nt] = relu_bf16(acc[nt], 0);
+            out[2 * mt + 1][nt] = relu_bf16(acc[nt], 1);
+        }
+        // keep the scheduler from hoisting the next blocks' weight reads (registers)
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// Network outputs of the wavefront's 64 samples from the normalised inputs xin[nt] of samples
+// 32 nt + l%32: returns, in every lane l, the output of sample l.
+__device__ __forceinline__ float mlp_tiles(const uint4 *__restrict__ img, const float (&xin)[2][4], int lane)
+{
+    const int h = lane >> 5;
+    bf16x8 b1[2][2];  // [block][tile]
+#pragma unroll
+    for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const float x = xin[nt][i];
+            const __bf16 xh = (__bf16)x;
+            const float r = x - (float)xh;
+            const __bf16 xm = (__bf16)r;
+            const __bf16 xl = (__bf16)(r - (float)xm);
+            b1[0][nt][i] = xh;
+            b1[0][nt][4 + i] = h ? xl : xm;
+            b1[1][nt][i] = h ? (__bf16)0.0f : xm;
+            b1[1][nt][4 + i] = h ? (__bf16)0.0f : xh;
+        }
+    bf16x8 ha[8][2], hb[8][2];
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+        const floatx16 c = lds_acc(img + IMG_B + mt * 8 + h * 4);
+        const bf16x8 a0 = as_bf16x8(img[IMG_W1 + mt * 128 + lane]);
+        const bf16x8 a1 = as_bf16x8(img[IMG_W1 + mt * 128 + 64 + lane]);
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++) {
+            floatx16 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b1[0][nt], c, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1[1][nt], acc, 0, 0, 0);
+            ha[2 * mt][nt] = relu_bf16(acc, 0);
+            ha[2 * mt + 1][nt] = relu_bf16(acc, 1);
+        }
+    }
+    hidden_layer(img + IMG_W2, img + IMG_B + 32, ha, hb, lane);
+    hidden_layer(img + IMG_W3, img + IMG_B + 64, hb, ha, lane);
+    const floatx16 c = lds_acc(img + IMG_B4 + h * 4);
+    floatx16 acc[2];
+    const bf16x8 a0 = as_bf16x8(img[IMG_W4 + lane]);
+    acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, ha[0][0], c, 0, 0, 0);
+    acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, ha[0][1], c, 0, 0, 0);
+#pragma unroll
+    for (int kk = 1; kk < 8; kk++) {
+        const bf16x8 a = as_bf16x8(img[IMG_W4 + kk * 64 + lane]);
+        acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, ha[kk][0], acc[0], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, ha[kk][1], acc[1], 0, 0, 0);
+    }
+    // rows 0..2 (accumulators 0..2 of the lanes l < 32) of tile nt: sample 32 nt + l
+    const float y0 = acc[0][0] + (acc[0][1] + acc[0][2]), y1 = acc[1][0] + (acc[1][1] + acc[1][2]);
+    const float y1s = __shfl(y1, lane & 31);
+    return h ? y1s : y0;
+}
+
+// Copy the image to LDS (whole workgroup).
+__device__ __forceinline__ void load_image(uint4 *img, const uint4 *__restrict__ gimg)
+{
+    for (int k = threadIdx.x; k < IMG_U4; k += blockDim.x) img[k] = gimg[k];
+    __syncthreads();
 }
 
 // Network output for rows of normalised inputs x[n][4].
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
-k_mlp_forward(int64_t n, const float *__restrict__ x, const float *__restrict__ P,
-              const unsigned short *__restrict__ W2b, const unsigned short *__restrict__ W3b, float *__restrict__ y)
+__global__ void __launch_bounds__(64 * IWAVES) k_mlp_forward(int64_t n, const float *__restrict__ x,
+                                                              const uint4 *__restrict__ gimg, float *__restrict__ y)
 {
-    __shared__ __align__(16) unsigned short act[ITILE];
-    const int64_t s = blockIdx.x * (int64_t)IS + threadIdx.x;
-    float xin[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (s < n) {
-        const float4 v = *reinterpret_cast<const float4 *>(x + 4 * s);
-        xin[0] = v.x; xin[1] = v.y; xin[2] = v.z; xin[3] = v.w;
+    __shared__ uint4 img[IMG_U4];
+    load_image(img, gimg);
+    const int lane = threadIdx.x & 63;
+    const int64_t ntile = (n + IS - 1) / IS;
+    for (int64_t tile = (int64_t)blockIdx.x * IWAVES + (threadIdx.x >> 6); tile < ntile;
+         tile += (int64_t)gridDim.x * IWAVES) {
+        const int64_t base = tile * IS;
+        float xin[2][4];
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++) {
+            const int64_t j = base + 32 * nt + (lane & 31);
+            const float4 v = j < n ? *reinterpret_cast<const float4 *>(x + 4 * j) : make_float4(0, 0, 0, 0);
+            xin[nt][0] = v.x; xin[nt][1] = v.y; xin[nt][2] = v.z; xin[nt][3] = v.w;
+        }
+        const float out = mlp_tiles(img, xin, lane);
+        if (base + lane < n) y[base + lane] = out;
     }
-    const float out = mlp_forward64(xin, P, W2b, W3b, act);
-    if (s < n) y[s] = out;
 }
 
 __device__ __forceinline__ int mlp_cell(const float v[4], const MLPNorm &nm, const int nc[4], bool &inside)
@@ -234,47 +370,60 @@ __device__ __forceinline__ int mlp_cell(const float v[4], const MLPNorm &nm, con
     return id;
 }
 
-// Gated inference of samples blockIdx.x*64 + lane from log10 columns x[a*n + i]: the network's
-// value, and the gate (outside the training box, or a cell whose map value exceeds tol: the
-// table instead) evaluated in the same wavefront; the samples that take the table are appended
-// to list (ballot-compacted, one atomic per wavefront) for k_mlp_fallback, so the table's
-// gathers run on full wavefronts of fallback samples only.
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
-k_mlp_infer(int64_t n, const float *__restrict__ x, MLPNorm nm, const float *__restrict__ P,
-            const unsigned short *__restrict__ W2b, const unsigned short *__restrict__ W3b,
+// Gated inference from log10 columns x[a*n + i]: the network's value, and the gate (outside the
+// training box, or a cell whose map value exceeds tol: the table instead) evaluated in the same
+// wavefront; the samples that take the table are appended to list (ballot-compacted, one atomic
+// per wavefront tile) for k_mlp_fallback, so the table's gathers run on full wavefronts of
+// fallback samples only.
+__global__ void __launch_bounds__(64 * IWAVES)
+k_mlp_infer(int64_t n, const float *__restrict__ x, MLPNorm nm, const uint4 *__restrict__ gimg,
             const float *__restrict__ emap, int ec0, int ec1, int ec2, int ec3, float tol, float *__restrict__ yn,
             float *__restrict__ outL, int *__restrict__ flag, int32_t *__restrict__ list,
             unsigned long long *__restrict__ nlist)
 {
-    __shared__ __align__(16) unsigned short act[ITILE];
-    const int64_t s = blockIdx.x * (int64_t)IS + threadIdx.x;
-    const bool valid = s < n;
-    float v[4] = {0.0f, 0.0f, 0.0f, 0.0f}, xin[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (valid)
+    __shared__ uint4 img[IMG_U4];
+    load_image(img, gimg);
+    const int lane = threadIdx.x & 63, h = lane >> 5;
+    float sc[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            v[k] = x[k * n + s];
-            xin[k] = 2.0f * (v[k] - nm.lo[k]) / (nm.hi[k] - nm.lo[k]) - 1.0f;
+    for (int k = 0; k < 4; k++) sc[k] = 2.0f / (nm.hi[k] - nm.lo[k]);
+    const int64_t ntile = (n + IS - 1) / IS;
+    for (int64_t tile = (int64_t)blockIdx.x * IWAVES + (threadIdx.x >> 6); tile < ntile;
+         tile += (int64_t)gridDim.x * IWAVES) {
+        const int64_t base = tile * IS;
+        float v[2][4], xin[2][4];
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++) {
+            const int64_t j = base + 32 * nt + (lane & 31);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                v[nt][k] = j < n ? x[k * n + j] : 0.0f;
+                xin[nt][k] = fmaf(v[nt][k] - nm.lo[k], sc[k], -1.0f);
+            }
         }
-    const float out = mlp_forward64(xin, P, W2b, W3b, act);
-    bool fb = false;
-    if (valid) {
-        const int nc[4] = {ec0, ec1, ec2, ec3};
-        bool inside;
-        const int cell = mlp_cell(v, nm, nc, inside);
-        fb = !inside || !(emap[cell] <= tol);
-        yn[s] = out;
-        outL[s] = fmaf(out, nm.sigma, nm.mu);
-        if (flag) flag[s] = fb ? 1 : 0;
-    }
-    const uint64_t m = __ballot(fb);
-    if (m) {
-        unsigned long long base = 0;
-        if (threadIdx.x == 0) base = atomicAdd(nlist, (unsigned long long)__popcll(m));
-        base = __shfl(base, 0);
-        if (fb)
-            list[base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] =
-                (int32_t)s;
+        const float out = mlp_tiles(img, xin, lane);
+        const int64_t s = base + lane;  // this lane's sample: tile h, row l%32
+        bool fb = false;
+        if (s < n) {
+            const float vs[4] = {h ? v[1][0] : v[0][0], h ? v[1][1] : v[0][1], h ? v[1][2] : v[0][2],
+                                 h ? v[1][3] : v[0][3]};
+            const int nc[4] = {ec0, ec1, ec2, ec3};
+            bool inside;
+            const int cell = mlp_cell(vs, nm, nc, inside);
+            fb = !inside || !(emap[cell] <= tol);
+            yn[s] = out;
+            outL[s] = fmaf(out, nm.sigma, nm.mu);
+            if (flag) flag[s] = fb ? 1 : 0;
+        }
+        const uint64_t m = __ballot(fb);
+        if (m) {
+            unsigned long long lb = 0;
+            if (lane == 0) lb = atomicAdd(nlist, (unsigned long long)__popcll(m));
+            lb = __shfl(lb, 0);
+            if (fb)
+                list[lb + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] =
+                    (int32_t)s;
+        }
     }
 }
 
@@ -591,9 +740,26 @@ __global__ void __launch_bounds__(256) k_gate_map(int64_t ncell, const float *__
 
 namespace sph {
 
+// Persistent inference grid: one workgroup per CU (the image takes half of a CU's LDS and the
+// registers allow two wavefronts per SIMD), no more than the tiles need.
+static unsigned infer_grid(int64_t n)
+{
+    static int cus[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if (cus[dev] <= 0) {
+        int c = 0;
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) c = 256;
+        cus[dev] = c;
+    }
+    const int64_t wgs = ((n + IS - 1) / IS + IWAVES - 1) / IWAVES;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(wgs, cus[dev]));
+}
+
 hipError_t mlp_forward_raw(hipStream_t st, int64_t n, const float *x, const MLPDev &d, float *y)
 {
-    k_mlp_forward<<<(unsigned)((n + IS - 1) / IS), 64, 0, st>>>(n, x, d.P, d.W2b, d.W3b, y);
+    if (n < 1) return hipSuccess;
+    k_mlp_forward<<<infer_grid(n), 64 * IWAVES, 0, st>>>(n, x, d.img, y);
     return hipGetLastError();
 }
 
@@ -605,8 +771,8 @@ hipError_t mlp_infer(hipStream_t st, int64_t n, const float *x, const MLPNorm &n
     unsigned long long *nlist = reinterpret_cast<unsigned long long *>(scratch);  // 64-int header: the count
     int32_t *list = scratch + 64;
     fill<unsigned long long>(st, 1, nlist, 0ull);
-    k_mlp_infer<<<(unsigned)((n + IS - 1) / IS), 64, 0, st>>>(n, x, nm, d.P, d.W2b, d.W3b, emap, ec[0], ec[1], ec[2],
-                                                              ec[3], tol, yn, outL, flag, list, nlist);
+    k_mlp_infer<<<infer_grid(n), 64 * IWAVES, 0, st>>>(n, x, nm, d.img, emap, ec[0], ec[1], ec[2], ec[3], tol, yn,
+                                                       outL, flag, list, nlist);
     const int64_t nb = std::min<int64_t>(sph_blocks(n, 256), 2048);
     k_mlp_fallback<<<(unsigned)nb, 256, 0, st>>>(n, x, ax, tabL, list, nlist, outL, nfall);
     return hipGetLastError();
@@ -631,6 +797,13 @@ hipError_t mlp_adam(hipStream_t st, const MLPDev &d, float lr, float b1, float b
 hipError_t mlp_refresh(hipStream_t st, const MLPDev &d)
 {
     k_refresh_bf16<<<sph_blocks(MLP_NPARAM, 256), 256, 0, st>>>(MLP_NPARAM, d.P, d.W2b, d.W3b, d.W2tb, d.W3tb);
+    const hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : mlp_image(st, d);
+}
+
+hipError_t mlp_image(hipStream_t st, const MLPDev &d)
+{
+    k_mlp_image<<<sph_blocks(IMG_U4, 256), 256, 0, st>>>(d.P, d.img);
     return hipGetLastError();
 }
 
