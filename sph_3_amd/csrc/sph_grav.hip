@@ -20,7 +20,12 @@
 // a pair difference is (s_hi - t_hi) + (s_lo - t_lo), so near pairs keep full
 // FP32 accuracy; sources whose node lies farther from the group box than a
 // quarter of the group's extent skip the lo parts (their offsets dwarf the
-// rounding).  Every particle of a group sees the same list: the counted
+// rounding).  With Plummer or no softening these far sources are staged as
+// (x, y, z, |x|^2 + eps^2) and (m x, m y, m z, m): r^2 = |x_j|^2 + eps^2 +
+// |x_i|^2 - 2 x_i.x_j is three FMAs and an add, and a_i = sum_j r^-3 (m_j x_j)
+// - x_i sum_j r^-3 m_j needs no product by the mass, so a far interaction is
+// ten VALU instructions and a reciprocal square root instead of twelve.
+// Every particle of a group sees the same list: the counted
 // interactions are n_nodes*g + n_parts*g - g per group of g particles.
 #include "sph_grav.h"
 #include "sph_walk.h"
@@ -97,6 +102,52 @@ __device__ __forceinline__ void interact(const Src4<R> &a, const Src4<R> &b, R t
     sz = fma(w, dz, sz);
 }
 
+// Far source in the expanded form (SOFT >= 1): a = (x, y, z, |x|^2 + eps^2), b = m (x, y, z, 1),
+// all relative to the group centre; the target enters as (-2 x_i, -2 y_i, -2 z_i) and |x_i|^2.
+// With the D/4 separation of far sources the rounding of the expanded r^2 stays below about
+// 80 ulp of r^2 (|x_j| <= |x_i| + r, |x_i| <= 0.87 D, r >= D/4).
+template <typename R, bool POT>
+__device__ __forceinline__ void interact_far(const Src4<R> &a, const Src4<R> &b, R m2x, R m2y, R m2z, R t2, R &fx,
+                                             R &fy, R &fz, R &fw, R &sp)
+{
+    const R r2 = fma(m2x, a.x, fma(m2y, a.y, fma(m2z, a.z, a.w))) + t2;
+    const R inv = sph_rsqrt<R>(r2);
+    const R w = inv * inv * inv;
+    fx = fma(w, b.x, fx);
+    fy = fma(w, b.y, fy);
+    fz = fma(w, b.z, fz);
+    fw = fma(w, b.w, fw);
+    if (POT) sp = fma(b.w, inv, sp);
+}
+
+// cnt far sources in the expanded form, two chains; each chain's flush partial
+// sum r^-3 m (x_j - x_i) = sum r^-3 m x_j - x_i sum r^-3 m goes to the Kahan totals of the
+// direct form (k[0..2], k[3..5]), and (POT) k[6]/k[7] += sum m / r per chain.
+template <typename R, bool POT>
+__device__ __forceinline__ void eval_far(int cnt, const Src4<R> *__restrict__ A, const Src4<R> *__restrict__ B, R tx,
+                                         R ty, R tz, Kahan<R> *k)
+{
+    const R m2x = (R)-2 * tx, m2y = (R)-2 * ty, m2z = (R)-2 * tz, t2 = fma(tx, tx, fma(ty, ty, tz * tz));
+    R s[10] = {(R)0, (R)0, (R)0, (R)0, (R)0, (R)0, (R)0, (R)0, (R)0, (R)0};
+    int q = 0;
+    for (; q + 1 < cnt; q += 2) {
+        const Src4<R> a0 = A[q], b0 = B[q], a1 = A[q + 1], b1 = B[q + 1];
+        interact_far<R, POT>(a0, b0, m2x, m2y, m2z, t2, s[0], s[1], s[2], s[3], s[8]);
+        interact_far<R, POT>(a1, b1, m2x, m2y, m2z, t2, s[4], s[5], s[6], s[7], s[9]);
+    }
+    if (q < cnt) interact_far<R, POT>(A[q], B[q], m2x, m2y, m2z, t2, s[0], s[1], s[2], s[3], s[8]);
+    k[0].add(fma(-tx, s[3], s[0]));
+    k[1].add(fma(-ty, s[3], s[1]));
+    k[2].add(fma(-tz, s[3], s[2]));
+    k[3].add(fma(-tx, s[7], s[4]));
+    k[4].add(fma(-ty, s[7], s[5]));
+    k[5].add(fma(-tz, s[7], s[6]));
+    if (POT) {
+        k[6].add(s[8]);
+        k[7].add(s[9]);
+    }
+}
+
 // Evaluate cnt staged sources (LO: with the lo parts of the offsets).  Two
 // independent chains of plain sums (ILP); each chain's partial over this flush
 // (<= GSRC/2 terms) is then added to its Kahan accumulator, so the list total
@@ -130,9 +181,40 @@ __device__ __forceinline__ void eval_sources(int nn, int nfar, const Src4<R> *__
                                              R epsi, R e2, Kahan<R> *k)
 {
     if (nn > 0) eval_run<R, SOFT, true, POT>(nn, A, B, tx, ty, tz, tlx, tly, tlz, epsi, e2, k);
-    if (nfar > 0)
-        eval_run<R, SOFT, false, POT>(nfar, A + (GSRC - nfar), B + (GSRC - nfar), tx, ty, tz, tlx, tly, tlz, epsi,
-                                      e2, k);
+    if (nfar > 0) {
+        if constexpr (SOFT >= 1)
+            eval_far<R, POT>(nfar, A + (GSRC - nfar), B + (GSRC - nfar), tx, ty, tz, k);
+        else
+            eval_run<R, SOFT, false, POT>(nfar, A + (GSRC - nfar), B + (GSRC - nfar), tx, ty, tz, tlx, tly, tlz,
+                                          epsi, e2, k);
+    }
+}
+
+// Stage a source at slot q: near (or spline softening) as hi/lo offsets with the mass and the
+// softening, far with Plummer or no softening in the expanded form of eval_far.
+template <typename R, int SOFT>
+__device__ __forceinline__ void stage_src(Src4<R> *A, Src4<R> *B, int q, bool near, double dx, double dy, double dz,
+                                          double m, double eps, R e2)
+{
+    Src4<R> a, b;
+    if (SOFT >= 1 && !near) {
+        a.x = (R)dx;
+        a.y = (R)dy;
+        a.z = (R)dz;
+        a.w = fma(a.x, a.x, fma(a.y, a.y, fma(a.z, a.z, e2)));
+        b.w = (R)m;
+        b.x = a.x * b.w;
+        b.y = a.y * b.w;
+        b.z = a.z * b.w;
+    } else {
+        split<R>(dx, a.x, b.x);
+        split<R>(dy, a.y, b.y);
+        split<R>(dz, a.z, b.z);
+        a.w = (R)m;
+        b.w = (R)eps;
+    }
+    A[q] = a;
+    B[q] = b;
 }
 
 // POT: phi[i] = G sum_j m_j psi(r_ij) over the same interaction list as the force.
@@ -213,14 +295,7 @@ __global__ void __launch_bounds__(GG) k_gravity_group(const int32_t *__restrict_
         const uint64_t man = wave_ballot(acc && near), maf = ma & ~man;
         if (acc) {
             const int q = near ? nn + (int)__popcll(man & lt) : GSRC - 1 - nfar - (int)__popcll(maf & lt);
-            Src4<R> a, b;
-            split<R>(ncx - gcx, a.x, b.x);
-            split<R>(ncy - gcy, a.y, b.y);
-            split<R>(ncz - gcz, a.z, b.z);
-            a.w = (R)nmass;
-            b.w = (R)neps;
-            A[q] = a;
-            B[q] = b;
+            stage_src<R, SOFT>(A, B, q, near, ncx - gcx, ncy - gcy, ncz - gcz, nmass, neps, e2);
         }
         nn += (int)__popcll(man);
         nfar += (int)__popcll(maf);
@@ -270,14 +345,7 @@ __global__ void __launch_bounds__(GG) k_gravity_group(const int32_t *__restrict_
             if (have) {
                 const int j = bfirst[bsel] + (sidx - bstart[bsel]);
                 const int q = pn ? nn + (int)__popcll(mn & lt) : GSRC - 1 - nfar - (int)__popcll(mf & lt);
-                Src4<R> a, b;
-                split<R>(x[j] - gcx, a.x, b.x);
-                split<R>(y[j] - gcy, a.y, b.y);
-                split<R>(z[j] - gcz, a.z, b.z);
-                a.w = (R)m[j];
-                b.w = eps ? (R)eps[j] : (R)0;
-                A[q] = a;
-                B[q] = b;
+                stage_src<R, SOFT>(A, B, q, pn, x[j] - gcx, y[j] - gcy, z[j] - gcz, m[j], eps ? eps[j] : 0.0, e2);
             }
             nn += (int)__popcll(mn);
             nfar += (int)__popcll(mf);
