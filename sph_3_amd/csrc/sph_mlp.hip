@@ -384,9 +384,6 @@ k_mlp_infer(int64_t n, const float *__restrict__ x, MLPNorm nm, const uint4 *__r
     __shared__ uint4 img[IMG_U4];
     load_image(img, gimg);
     const int lane = threadIdx.x & 63, h = lane >> 5;
-    float sc[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) sc[k] = 2.0f / (nm.hi[k] - nm.lo[k]);
     const int64_t ntile = (n + IS - 1) / IS;
     for (int64_t tile = (int64_t)blockIdx.x * IWAVES + (threadIdx.x >> 6); tile < ntile;
          tile += (int64_t)gridDim.x * IWAVES) {
@@ -398,7 +395,8 @@ k_mlp_infer(int64_t n, const float *__restrict__ x, MLPNorm nm, const uint4 *__r
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 v[nt][k] = j < n ? x[k * n + j] : 0.0f;
-                xin[nt][k] = fmaf(v[nt][k] - nm.lo[k], sc[k], -1.0f);
+                // the normalisation of the training samples (k_mlp_samples), operation for operation
+                xin[nt][k] = 2.0f * (v[nt][k] - nm.lo[k]) / (nm.hi[k] - nm.lo[k]) - 1.0f;
             }
         }
         const float out = mlp_tiles(img, xin, lane);
