@@ -35,7 +35,9 @@ def _rms(a):
     return float(np.sqrt(np.mean(np.square(a, dtype=np.float64))))
 
 
-@pytest.mark.parametrize("n", [1, 100, 4096])
+# n = 1, 65: partial 64-sample tiles; 200,000: more tiles than the persistent grid's wavefronts
+# (one workgroup of eight per CU), so wavefronts loop over several tiles.
+@pytest.mark.parametrize("n", [1, 65, 100, 4096, 200_000])
 def test_forward_vs_oracle(net, n):
     p = mlp.init(1)
     net.set_params(p, NORM)
@@ -48,8 +50,11 @@ def test_forward_vs_oracle(net, n):
     print(f"n={n} rms={s:.3g} max|d_em|={np.max(np.abs(d)):.3g} rms_em={_rms(d):.3g} rms_64={_rms(y - y64):.3g}")
     # the largest deviations are single BF16 rounding flips of a hidden activation where the
     # device's FP32 summation order differs from the oracle's: replaying the emulation with
-    # float32 matrix products instead of FP64 sums gives the same 9e-3 of rms for n = 4096
-    assert np.max(np.abs(d)) <= 1.5e-2 * s
+    # float32 matrix products instead of FP64 sums gives the same 9e-3 of rms for n = 4096;
+    # 200,000 samples reach further into the tail of those flips (1.9e-2 of rms measured),
+    # while a sample-to-output mix-up in the tile loop would show at the level of the rms
+    max_bound = 1.5e-2 if n <= 4096 else 3e-2
+    assert np.max(np.abs(d)) <= max_bound * s
     assert _rms(d) <= 5e-4 * s
     assert _rms(y - y64) <= 3e-2 * s
 
