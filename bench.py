@@ -753,6 +753,7 @@ def main():
     sim, dt = run("mixed", args.steps, args.warmup, d=d, cooling=cooling(mode))
     st = sim.stats()
     g = sim.get()
+    dev_bytes = sim.device_bytes()
     del sim
     ms_step = 1e3 * dt / args.steps
     value = n * args.steps / dt
@@ -774,6 +775,8 @@ def main():
         **extra,
         "cooling_fallback_fraction_last_step": st["cool_fallback"] / n,
         "cooling_floored_fraction_last_step": st["cool_floor"] / n,
+        # every buffer the step's context holds (state, tree, rows, h-solve scratch, tables, surrogate)
+        "device_bytes": dev_bytes, "device_bytes_per_particle": dev_bytes / n,
     }
     if not args.no_fp64_form:
         steps64 = max(2, args.steps // 2)

@@ -57,6 +57,11 @@ int sph_ctx_create(int device, sph_ctx **out);
 int sph_ctx_destroy(sph_ctx *ctx);
 const char *sph_last_error(const sph_ctx *ctx);
 int sph_version(void);
+/* Device memory held by the context (bytes of every buffer it has allocated: the resident
+ * state, the tree, neighbour rows, h-solve scratch, cooling tables, surrogate).  No
+ * counterpart in the reference (its arrays are host numpy or fixed CUDA allocations); for
+ * capacity planning on one GPU. */
+int sph_device_bytes(const sph_ctx *ctx, uint64_t *bytes);
 
 /* ---- per-function entry points (the reference's Python call surface) ---- */
 

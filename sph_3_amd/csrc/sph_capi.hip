@@ -415,32 +415,39 @@ extern "C" int sph_ctx_create(int device, sph_ctx **out)
     return SPH_OK;
 }
 
+namespace {
+// Every device buffer of a context (the one list freed by sph_ctx_destroy and summed by
+// sph_device_bytes).
+std::vector<Buf *> all_bufs(sph_ctx *ctx)
+{
+    std::vector<Buf *> v = {
+        &ctx->x, &ctx->y, &ctx->z, &ctx->vx, &ctx->vy, &ctx->vz, &ctx->m, &ctx->h, &ctx->eps,
+        &ctx->u, &ctx->duprev, &ctx->ax, &ctx->ay, &ctx->az, &ctx->gx, &ctx->gy, &ctx->gz,
+        &ctx->tmpd, &ctx->orig, &ctx->tmpi, &ctx->keys0, &ctx->keys1, &ctx->vals0, &ctx->vals1,
+        &ctx->hist, &ctx->bpart, &ctx->box, &ctx->nodes, &ctx->right, &ctx->parent, &ctx->ncount, &ctx->flags,
+        &ctx->front, &ctx->groups, &ctx->cnt, &ctx->rcnt0, &ctx->off, &ctx->scantmp, &ctx->idx, &ctx->rho,
+        &ctx->P, &ctx->c, &ctx->prho2, &ctx->divV, &ctx->curlV, &ctx->bal, &ctx->du, &ctx->cand, &ctx->hR,
+        &ctx->hsteps, &ctx->hstate, &ctx->hcount, &ctx->hplist, &ctx->hout, &ctx->hin, &ctx->tx, &ctx->ty,
+        &ctx->tz, &ctx->stage, &ctx->kpart, &ctx->kout, &ctx->spread, &ctx->ccidx, &ctx->cccnt, &ctx->cext,
+        &ctx->cnext, &ctx->chr, &ctx->ttab, &ctx->zmet, &ctx->j21, &ctx->ucols, &ctx->uyn, &ctx->uL, &ctx->uG,
+        &ctx->uflag, &ctx->ucnt, &ctx->hbig, &ctx->p4, &ctx->v4, &ctx->s4, &ctx->pf, &ctx->vf, &ctx->colg,
+        &ctx->coln, &ctx->tsdt, &ctx->tsdh, &ctx->phi, &ctx->tpart, &ctx->tout,
+        // cooling and surrogate
+        &ctx->rf1, &ctx->ctabL, &ctx->ctabG, &ctx->cin[0], &ctx->cin[1], &ctx->cin[2], &ctx->cin[3],
+        &ctx->cin[4], &ctx->cin[5], &ctx->cout[0], &ctx->cout[1], &ctx->mP, &ctx->mG, &ctx->mM, &ctx->mV,
+        &ctx->mW[0], &ctx->mW[1], &ctx->mW[2], &ctx->mW[3], &ctx->mimg, &ctx->mx, &ctx->mxn, &ctx->mt, &ctx->my,
+        &ctx->mloss, &ctx->memap, &ctx->memap2, &ctx->memap3, &ctx->mgate, &ctx->mcnt, &ctx->mfl};
+    for (Buf &b : ctx->shadow) v.push_back(&b);
+    return v;
+}
+}  // namespace
+
 extern "C" int sph_ctx_destroy(sph_ctx *ctx)
 {
     if (!ctx) return SPH_ERR_ARG;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->st);
-    Buf *all[] = {&ctx->x, &ctx->y, &ctx->z, &ctx->vx, &ctx->vy, &ctx->vz, &ctx->m, &ctx->h, &ctx->eps,
-                  &ctx->u, &ctx->duprev, &ctx->ax, &ctx->ay, &ctx->az, &ctx->gx, &ctx->gy, &ctx->gz,
-                  &ctx->tmpd, &ctx->orig, &ctx->tmpi, &ctx->keys0, &ctx->keys1, &ctx->vals0, &ctx->vals1,
-                  &ctx->hist, &ctx->bpart, &ctx->box, &ctx->nodes, &ctx->right, &ctx->parent, &ctx->ncount, &ctx->flags, &ctx->front, &ctx->groups,
-                  &ctx->cnt, &ctx->rcnt0, &ctx->off, &ctx->scantmp, &ctx->idx, &ctx->rho, &ctx->P, &ctx->c, &ctx->prho2,
-                  &ctx->divV, &ctx->curlV, &ctx->bal, &ctx->du, &ctx->cand, &ctx->hR, &ctx->hsteps,
-                  &ctx->hstate, &ctx->hcount, &ctx->hplist, &ctx->hout, &ctx->hin, &ctx->tx, &ctx->ty, &ctx->tz,
-                  &ctx->stage, &ctx->kpart, &ctx->kout, &ctx->spread, &ctx->ccidx, &ctx->cccnt,
-                  &ctx->cext, &ctx->cnext, &ctx->chr, &ctx->ttab, &ctx->zmet, &ctx->j21, &ctx->ucols, &ctx->uyn,
-                  &ctx->uL, &ctx->uG, &ctx->uflag, &ctx->ucnt, &ctx->hbig, &ctx->p4, &ctx->v4, &ctx->s4, &ctx->pf, &ctx->vf,
-                  &ctx->colg, &ctx->coln, &ctx->tsdt, &ctx->tsdh, &ctx->phi, &ctx->tpart, &ctx->tout};
-    for (Buf *b : all)
-        if (b->p) (void)hipFree(b->p);
-    for (Buf &b : ctx->shadow)
-        if (b.p) (void)hipFree(b.p);
-    Buf *cool[] = {&ctx->rf1,   &ctx->ctabL,  &ctx->ctabG,  &ctx->cin[0], &ctx->cin[1], &ctx->cin[2], &ctx->cin[3],
-                   &ctx->cin[4], &ctx->cin[5], &ctx->cout[0], &ctx->cout[1], &ctx->mP,     &ctx->mG,     &ctx->mM,
-                   &ctx->mV,     &ctx->mW[0],  &ctx->mW[1],  &ctx->mW[2],  &ctx->mW[3],  &ctx->mx,     &ctx->mxn,
-                   &ctx->mt,     &ctx->my,     &ctx->mloss,  &ctx->memap,  &ctx->memap2, &ctx->memap3,
-                   &ctx->mgate,  &ctx->mcnt,   &ctx->mfl};
-    for (Buf *b : cool)
+    for (Buf *b : all_bufs(ctx))
         if (b->p) (void)hipFree(b->p);
     for (int k = 0; k < 2 * SPH_PH_COUNT; k++) (void)hipEventDestroy(ctx->ev[k]);
     if (ctx->ev_h) (void)hipEventDestroy(ctx->ev_h);
@@ -448,6 +455,15 @@ extern "C" int sph_ctx_destroy(sph_ctx *ctx)
     if (ctx->hpin) (void)hipHostFree(ctx->hpin);
     (void)hipStreamDestroy(ctx->st);
     delete ctx;
+    return SPH_OK;
+}
+
+extern "C" int sph_device_bytes(const sph_ctx *ctx, uint64_t *bytes)
+{
+    if (!ctx || !bytes) return SPH_ERR_ARG;
+    uint64_t total = 0;
+    for (Buf *b : all_bufs(const_cast<sph_ctx *>(ctx))) total += b->p ? (uint64_t)b->bytes : 0;
+    *bytes = total;
     return SPH_OK;
 }
 

@@ -107,6 +107,13 @@ class DeviceSPH:
         c.check(c.lib.sph_state_get_timestep(c.h, ptr(dt), ptr(dh)), "sph_state_get_timestep")
         return dt, dh
 
+    def device_bytes(self) -> int:
+        """Device memory the state's context holds (every buffer it allocated), in bytes."""
+        b = ctypes.c_uint64(0)
+        c = self.ctx
+        c.check(c.lib.sph_device_bytes(c.h, ctypes.byref(b)), "sph_device_bytes")
+        return int(b.value)
+
     def leftover_mass(self) -> float:
         lo = np.zeros(1)
         c = self.ctx

@@ -111,3 +111,39 @@ def test_invalid_particles_are_refused_before_launch(gpu):
     # the context stays usable after a refused call
     rho = X.getDensity(d["pos"], d["m"], d["h0"])
     assert np.all(np.isfinite(rho))
+
+
+def test_device_footprint_query(gpu, monkeypatch):
+    """sph_device_bytes sums every buffer a context holds: on a fresh context it grows
+    in proportion to n, and the device's free memory drops by no more than that sum
+    (plus 64 MiB for the runtime's own objects) while the context fills."""
+    import ctypes
+    import gc
+    import sph_3_amd.step as S
+    from sph_3_amd._lib import Context
+    hip = ctypes.CDLL("libamdhip64.so.7")  # the runtime the library is linked against
+
+    def free_bytes():
+        f = ctypes.c_size_t(0); t = ctypes.c_size_t(0)
+        assert hip.hipDeviceSynchronize() == 0
+        assert hip.hipMemGetInfo(ctypes.byref(f), ctypes.byref(t)) == 0
+        return f.value
+
+    got = {}
+    for n in (50_000, 100_000):
+        d = ic.evrard(n, seed=76)
+        gc.collect()
+        free0 = free_bytes()
+        ctx = Context(0)
+        monkeypatch.setattr(S, "context", lambda device=0: ctx)
+        sim = S.DeviceSPH(d["pos"], d["vel"], d["m"], d["u"], d["h0"])
+        sim.step()
+        b = sim.device_bytes()
+        drop = free0 - free_bytes()
+        print(f"n={n}: {b} bytes ({b / n:.0f} per particle), free memory dropped by {drop}")
+        got[n] = b
+        assert 200 * n < b < 20_000 * n
+        assert drop <= b + (64 << 20)
+        del sim
+        ctx.close()
+    assert 1.6 < got[100_000] / got[50_000] < 2.4
