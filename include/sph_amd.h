@@ -62,6 +62,12 @@ int sph_version(void);
  * counterpart in the reference (its arrays are host numpy or fixed CUDA allocations); for
  * capacity planning on one GPU. */
 int sph_device_bytes(const sph_ctx *ctx, uint64_t *bytes);
+/* Largest size of the smoothing-length solve's candidate distance rows (FP64 r^2 beside each
+ * candidate index: 2 KB per target).  A solve whose rows would exceed it keeps the indices
+ * only and recomputes the distances from the positions: the same bits, 2 KB per particle
+ * less device memory, a slower solve.  The default is a third of the device's memory.  No
+ * counterpart in the reference; for runs near the memory capacity of one GPU. */
+int sph_set_distance_rows_limit(sph_ctx *ctx, uint64_t bytes);
 
 /* ---- per-function entry points (the reference's Python call surface) ---- */
 

@@ -89,6 +89,7 @@ _SIGS = {
     "sph_ctx_destroy": ([_vp], ctypes.c_int),
     "sph_last_error": ([_vp], ctypes.c_char_p),
     "sph_device_bytes": ([_vp, ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
+    "sph_set_distance_rows_limit": ([_vp, ctypes.c_uint64], ctypes.c_int),
     "sph_smoothing_length": ([_vp, _i64, _d, _i64, _d, _d, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                               ctypes.c_int, ctypes.c_int, ctypes.c_double, _d, _i64p], ctypes.c_int),
     "sph_neighbours": ([_vp, _i64, _d, _d, _i64p, _i32p, _i64], ctypes.c_int),

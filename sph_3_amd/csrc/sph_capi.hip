@@ -118,7 +118,7 @@ int capacity(sph_ctx *ctx, int64_t n)
     Buf *dbl[] = {&ctx->x, &ctx->y, &ctx->z, &ctx->vx, &ctx->vy, &ctx->vz, &ctx->m, &ctx->h, &ctx->eps,
                   &ctx->u, &ctx->duprev, &ctx->ax, &ctx->ay, &ctx->az, &ctx->gx, &ctx->gy, &ctx->gz,
                   &ctx->tmpd, &ctx->rho, &ctx->P, &ctx->c, &ctx->prho2, &ctx->divV, &ctx->curlV,
-                  &ctx->bal, &ctx->du, &ctx->hR, &ctx->hout, &ctx->hin, &ctx->tx, &ctx->ty, &ctx->tz};
+                  &ctx->bal, &ctx->du, &ctx->cand, &ctx->hR, &ctx->hout, &ctx->hin, &ctx->tx, &ctx->ty, &ctx->tz};
     for (Buf *b : dbl) CK(ensure(ctx, *b, nd));
     Buf *i32[] = {&ctx->orig, &ctx->tmpi, &ctx->vals0, &ctx->vals1, &ctx->cnt, &ctx->rcnt0, &ctx->hsteps, &ctx->hstate};
     for (Buf *b : i32) CK(ensure(ctx, *b, ni));
@@ -405,6 +405,8 @@ extern "C" int sph_ctx_create(int device, sph_ctx **out)
             return SPH_ERR_HIP;
         }
     ctx->scratch = resolve_scratch(device);  // the device is current here
+    size_t mem_free = 0, mem_total = 0;
+    if (hipMemGetInfo(&mem_free, &mem_total) == hipSuccess) ctx->r2_limit = mem_total / 3;
     if (hipHostMalloc((void **)&ctx->hpin, 128 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_h, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_n, hipEventDisableTiming) != hipSuccess) {
@@ -455,6 +457,13 @@ extern "C" int sph_ctx_destroy(sph_ctx *ctx)
     if (ctx->hpin) (void)hipHostFree(ctx->hpin);
     (void)hipStreamDestroy(ctx->st);
     delete ctx;
+    return SPH_OK;
+}
+
+extern "C" int sph_set_distance_rows_limit(sph_ctx *ctx, uint64_t bytes)
+{
+    if (!ctx) return SPH_ERR_ARG;
+    ctx->r2_limit = bytes;
     return SPH_OK;
 }
 

@@ -14,10 +14,19 @@ namespace sphc {
 // unc (unconverged), rounds_out and work (distance tests, kernel terms).
 int hsolve_params(sph_ctx *ctx, int64_t m, const double *tx, const double *ty, const double *tz, int mode,
                   double param, int n_low, int n_up, int max_steps, double tol, int64_t *unc, int64_t *rounds_out,
-                  int64_t *work, bool keep_indices, sph::HSolveParams &p)
+                  int64_t *work, sph::HSolveParams &p)
 {
-    CK(ensure(ctx, ctx->cand, (size_t)m * SPH_HCAP * 8));
-    if (keep_indices) CK(ensure(ctx, ctx->ccidx, (size_t)m * SPH_HCAP * 4));
+    if (mode != SPH_H_REF_COUNT) CK(ensure(ctx, ctx->ccidx, (size_t)m * SPH_HCAP * 4));
+    // FP64 distances beside the indices while they fit the context's limit; beyond it the
+    // solve recomputes them (the same bits) and the buffer is given back
+    const bool rows_r2 = mode != SPH_H_REF_COUNT && (uint64_t)m * SPH_HCAP * 8 <= ctx->r2_limit;
+    if (rows_r2) {
+        CK(ensure(ctx, ctx->cand, (size_t)m * SPH_HCAP * 8));
+    } else if (ctx->cand.p) {
+        HIPCK(hipFree(ctx->cand.p));
+        ctx->cand.p = nullptr;
+        ctx->cand.bytes = 0;
+    }
     CK(ensure(ctx, ctx->cccnt, (size_t)m * 4));
     p.m = m;
     p.tx = tx; p.ty = ty; p.tz = tz;
@@ -33,7 +42,6 @@ int hsolve_params(sph_ctx *ctx, int64_t m, const double *tx, const double *ty, c
     p.r_fac = 1.06;  // candidate radius 2.12 h_prev: one round while h grows < 6 % per call
     p.h_in = D(hin);
     p.h_out = D(hout);
-    p.cand = D(cand);
     p.R = D(hR);
     p.steps = I32(hsteps);
     p.state = I32(hstate);
@@ -44,7 +52,8 @@ int hsolve_params(sph_ctx *ctx, int64_t m, const double *tx, const double *ty, c
     p.ngroups = ctx->ngroups;
     p.rounds = rounds_out;
     p.work = work;
-    p.cidx = keep_indices ? I32(ccidx) : nullptr;
+    p.cidx = mode != SPH_H_REF_COUNT ? I32(ccidx) : nullptr;
+    p.cand = rows_r2 ? D(cand) : nullptr;
     p.ccnt = I32(cccnt);
     p.host_totals = ctx->hpin;
     p.done = ctx->ev_h;
@@ -65,12 +74,11 @@ int hsolve_params(sph_ctx *ctx, int64_t m, const double *tx, const double *ty, c
 
 int run_hsolve(sph_ctx *ctx, int64_t m, const double *tx, const double *ty, const double *tz, int mode,
                double param, int n_low, int n_up, int max_steps, double tol, int64_t *n_unconv, int64_t *rounds_out,
-               int64_t *work = nullptr, bool keep_indices = false)
+               int64_t *work = nullptr)
 {
     sph::HSolveParams p;
     int64_t unc = 0;
-    CK(hsolve_params(ctx, m, tx, ty, tz, mode, param, n_low, n_up, max_steps, tol, &unc, rounds_out, work,
-                     keep_indices, p));
+    CK(hsolve_params(ctx, m, tx, ty, tz, mode, param, n_low, n_up, max_steps, tol, &unc, rounds_out, work, p));
     int64_t pending = 0;
     if (mode == SPH_H_REF_COUNT) HIPCK(sph::tree_escape(ctx->st, ctx->n, P_<WNode>(ctx->nodes)));  // stackless walk
     HIPCK(sph::hsolve(ctx->st, p, SPH_HROUNDS, &pending));

@@ -127,7 +127,7 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     ctx->hs_work[0] = ctx->hs_work[1] = 0;
     sph::HSolveParams hp;
     CK(hsolve_params(ctx, n, D(x), D(y), D(z), SPH_H_NEWTON, p->n_ngb, 0, 0, 0, p->h_tol, &ctx->hs_unc,
-                     &ctx->stats.h_rounds, ctx->hs_work, true, hp));
+                     &ctx->stats.h_rounds, ctx->hs_work, hp));
     hp.per = p->period;
     HIPCK(sph::hsolve_begin(st, hp));
     ph_end(ctx, SPH_PH_HSOLVE);
@@ -180,7 +180,7 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     const int64_t ext_cap = std::max<int64_t>(4 * n, SPH_SPREAD * 64);
     CK(ensure(ctx, ctx->cext, (size_t)ext_cap * 8));
     CK(ensure(ctx, ctx->cnext, (size_t)SPH_SPREAD_WORDS * 8));
-    CK(ensure(ctx, ctx->chr, (size_t)n * 16));
+    CK(ensure(ctx, ctx->chr, (size_t)n * (hp.cand ? sizeof(double2) : sizeof(sph::CandRec))));
     ph_begin(ctx, SPH_PH_NGB);
     sph::BigRows big;  // targets of the candidate-free h solver: rows by a walk each
     big.list = I32(hbig);
@@ -194,10 +194,10 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
         big.hscale = D(kout) + 7;
     }
     HIPCK(sph::neighbour_rows_from_candidates_begin(
-        st, n, SPH_HCAP, D(cand), I32(ccidx), I32(cccnt), D(h), D(hR), SPH_NGB_STRIDE, I32(cnt), I32(rcnt0), I32(idx),
-        P_<int2>(ctx->cext), ext_cap, P_<unsigned long long>(ctx->cnext),
+        st, n, SPH_HCAP, D(x), D(y), D(z), p->period, hp.cand, I32(ccidx), I32(cccnt), D(h), D(hR), SPH_NGB_STRIDE,
+        I32(cnt), I32(rcnt0), I32(idx), P_<int2>(ctx->cext), ext_cap, P_<unsigned long long>(ctx->cnext),
         P_<unsigned long long>(ctx->spread) + SPR_N * SPH_SPREAD_WORDS, P_<unsigned long long>(ctx->hcount) + 10,
-        ctx->hpin + 8, ctx->ev_n, big, P_<double2>(ctx->chr)));
+        ctx->hpin + 8, ctx->ev_n, big, ctx->chr.p));
     ph_end(ctx, SPH_PH_NGB);
     ctx->ngb_stride = SPH_NGB_STRIDE;
     ctx->ngb_per = p->period;

@@ -52,6 +52,7 @@ struct sph_ctx {
     Buf shadow[SPH_GATHER_MAX];  // targets of the reorder gathers (swapped with the state)
     double m_max = INFINITY;     // largest particle mass of the current masses
     int64_t n_reserve = 0;       // per-particle buffers hold at least this many (injection room)
+    uint64_t r2_limit = 0;       // bytes the h solve's candidate distance rows may take (else indices only)
     int64_t n_placed = 0;        // room of the resident state's buffers (the reserve when it was set)
     bool agn_on = false;
     sph_agn_params agn{};
@@ -73,7 +74,7 @@ struct sph_ctx {
     Buf cand, hR, hsteps, hstate, hcount, hout, hin, tx, ty, tz;
     Buf hplist;  // pending groups and targets of the h-solve rounds after the first (+ two counters)
     Buf ccidx, cccnt, cext, cnext;  // candidate indices, counts, extra pairs and their region counters
-    Buf chr;                        // (h_j, R_j) packed for the rows from candidates
+    Buf chr;                        // (x_j, h_j, R_j) packed for the rows from candidates
     Buf hbig;                       // targets of the candidate-free h solver
     // staging and reductions
     Buf stage, kpart, kout;
@@ -225,7 +226,7 @@ inline sph::ColumnParams column_params(const sph_column_params *cp)
 // sph_capi_api.hip
 int hsolve_params(sph_ctx *ctx, int64_t m, const double *tx, const double *ty, const double *tz, int mode,
                   double param, int n_low, int n_up, int max_steps, double tol, int64_t *unc, int64_t *rounds_out,
-                  int64_t *work, bool keep_indices, sph::HSolveParams &p);
+                  int64_t *work, sph::HSolveParams &p);
 // sph_capi_cool.hip: the cooling update of n particles (counts[0] floored, counts[1] surrogate fallbacks)
 template <typename RHO>
 int cool_run(sph_ctx *ctx, int64_t n, const RHO *rho, const double *u, const double *Z, const double *J21,

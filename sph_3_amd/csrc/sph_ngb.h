@@ -24,9 +24,10 @@ struct HSolveParams {
     double r_fac;                // initial radius = 2 * h_in * r_fac
     const double *h_in;          // start values (device)
     double *h_out;               // result (device)
-    double *cand, *R;            // scratch: cap*m, m
-    int32_t *cidx = nullptr;     // optional: cap*m candidate indices of the final rows (targets must
-                                 //   be the sorted sources; see neighbour_rows_from_candidates)
+    double *R;                   // scratch: m
+    int32_t *cidx = nullptr;     // scratch of modes 0 and 2: cap*m candidate rows (source indices)
+    double *cand = nullptr;      // optional: cap*m squared distances beside them; null: the solve
+                                 //   recomputes them from the positions (the same bits, less memory)
     int32_t *ccnt = nullptr;     // m candidate counts (required by modes 0 and 2)
     int32_t *steps, *state;      // scratch: m, m
     unsigned long long *counters;  // scratch: spread array of 5 counters (5 * SPH_SPREAD_WORDS words)
@@ -91,7 +92,7 @@ hipError_t neighbour_rows(hipStream_t st, int64_t ngroups, const int32_t *groups
                           unsigned long long *totals, int64_t *n_long, int64_t *total,
                           const double *hscale = nullptr, double per = 0.0);
 // Padded neighbour rows (as neighbour_rows) built from the h solve's final candidate rows
-// and indices (targets = the sorted particles, same positions and the solved h) instead of
+// (targets = the sorted particles, same positions and the solved h; per: the solve's box) instead of
 // a tree walk; exact by the containment argument at k_cand_rows.  ext: ext_cap int2 scratch
 // for the pairs found from the other side (SPH_SPREAD regions), next: SPH_SPREAD_WORDS
 // scratch words (one counter per region), cnt0: n int32 scratch (row lengths before the
@@ -101,7 +102,12 @@ hipError_t neighbour_rows(hipStream_t st, int64_t ngroups, const int32_t *groups
 // totals and region counters into host_words (pinned, 2 + SPH_SPREAD words) followed by
 // done; _end waits for done (not the stream) and evaluates them.
 // big: the targets of the candidate-free h solver (their rows are built by a walk each).
-// hr: scratch of n double2 (h_j and R_j per source, packed for one gather per candidate).
+// cand: the solve's squared-distance rows, or null when it kept indices only; rec: scratch of
+// n double2 (h_j, R_j) with cand, else n CandRec (position, h and R per particle, packed so
+// that a candidate's distance and both tests take one 48-byte gather).
+struct __align__(16) CandRec {
+    double x, y, z, h, R, pad;
+};
 struct BigRows {
     const int32_t *list = nullptr;
     int64_t count = 0;
@@ -111,13 +117,15 @@ struct BigRows {
     const double *hscale = nullptr;  // device: node h_max belong to an earlier h (may be null)
     double per = 0.0;                // side of a periodic box (0: open boundaries)
 };
-hipError_t neighbour_rows_from_candidates_begin(hipStream_t st, int64_t n, int cap, const double *cand,
+hipError_t neighbour_rows_from_candidates_begin(hipStream_t st, int64_t n, int cap, const double *x,
+                                               const double *y, const double *z, double per, const double *cand,
                                                const int32_t *cidx, const int32_t *ccnt, const double *h,
-                                               const double *R, int stride, int32_t *cnt, int32_t *cnt0,
-                                               int32_t *idx, int2 *ext, int64_t ext_cap, unsigned long long *next,
+                                               const double *R, int stride,
+                                               int32_t *cnt, int32_t *cnt0, int32_t *idx, int2 *ext,
+                                               int64_t ext_cap, unsigned long long *next,
                                                unsigned long long *spread, unsigned long long *totals,
                                                unsigned long long *host_words, hipEvent_t done,
-                                               const BigRows &big, double2 *hr);
+                                               const BigRows &big, void *rec);
 hipError_t neighbour_rows_from_candidates_end(hipEvent_t done, const unsigned long long *host_words, int64_t ext_cap,
                                              int64_t *n_long, int64_t *total, int64_t *n_extra);
 // *out (device) = max_i h_new[i] / h_old[i], rounded up by 8 ulp: the factor that

@@ -71,9 +71,10 @@ __device__ __forceinline__ void coop_walk(const WNode *__restrict__ nodes, bool 
 // mode 0: Newton-Raphson root of (32/3) sum_j w(r_j/h) - n_ngb (FP64),
 // mode 1: the reference's +-0.3 % count stepping (k_hstep below),
 // mode 2: h = 0.5 * (k+1)-th smallest distance (libsx.py:752-774, k = 50).
-// Candidates (r^2 < R_t^2) of target t go to its row cand[t*cap + c], c < cap,
-// through an 8-entry LDS buffer per lane flushed as whole 64-byte pieces.  The
-// solve then takes the group's targets one at a time with the whole wave.  A target whose
+// Candidates (r^2 < R_t^2) of target t go to its row cidx[t*cap + c], c < cap, as source
+// indices, with their r^2 beside them in cand[t*cap + c] when cand is given; without it the
+// solve and the neighbour rows recompute r^2 from the positions with the walk's own
+// operations (contraction off), so they see the walk's bits.  A target whose
 // candidate set cannot decide its answer grows or shrinks R_t and stays
 // pending (state[t] = 0); the host relaunches until every target is done.
 struct HSolveArgs {
@@ -87,8 +88,8 @@ struct HSolveArgs {
     int n_low, n_up, max_steps;  // mode 1
     int kth;                   // mode 2
     int cap;
-    double *cand;              // m rows of cap doubles
-    int32_t *cidx;             // m rows of cap candidate indices (may be null)
+    int32_t *cidx;             // m rows of cap candidate indices (modes 0 and 2)
+    double *cand;              // null, or m rows of cap squared distances beside them
     int32_t *ccnt;             // candidates per target (written with cidx)
     double *R;                 // per target search radius (in/out)
     double *h;                 // per target h (in: start, out: result)
@@ -154,8 +155,8 @@ __global__ void __launch_bounds__(64) k_hwalk(HSolveArgs a)
     ttx[lane] = xi; tty[lane] = yi; ttz[lane] = zi;
     tR2[lane] = valid ? R2 : -1.0;
     const int cap = a.cap;
-    double *const cbase = a.cand + first * (int64_t)cap;  // rows of the group's targets
-    int32_t *const ibase = a.cidx ? a.cidx + first * (int64_t)cap : nullptr;
+    int32_t *const ibase = a.cidx + first * (int64_t)cap;  // rows of the group's targets
+    double *const cbase = a.cand ? a.cand + first * (int64_t)cap : nullptr;
     int c = 0;  // lane t: candidates of target t so far
     long long ntest = 0;  // algorithmic work: distance tests
     // Candidates: lanes take the staged sources, the group's targets in turn (their
@@ -217,8 +218,8 @@ __global__ void __launch_bounds__(64) k_hwalk(HSolveArgs a)
                             if (pos < (uint32_t)cap) {
                                 // the group's uniform row base + a 32-bit offset (scalar-base addressing)
                                 const uint32_t e = (uint32_t)(tt * cap) + pos;
-                                *(double *)((char *)cbase + e * 8u) = r2;
-                                if (a.cidx) *(int32_t *)((char *)ibase + e * 4u) = pj;
+                                *(int32_t *)((char *)ibase + e * 4u) = pj;
+                                if (cbase) *(double *)((char *)cbase + e * 8u) = r2;
                             }
                         }
                         c = (lane == tt) ? ct + (int)__popcll(hm) : c;
@@ -251,8 +252,10 @@ __global__ void __launch_bounds__(64) k_hwalk(HSolveArgs a)
 
 // Newton (mode 0) or k-th distance (mode 2) solve of the pending targets from their
 // candidate rows, 8 targets per wavefront and 8 lanes per target: lane l of a sub-group
-// holds candidates l, l+8, ... (up to 16 in registers; longer rows are read again from
-// memory in every evaluation) and the sub-group reduces with DPP lane exchanges, which give
+// holds the squared distances of candidates l, l+8, ... (up to 16 in registers; R2: read
+// from the distance rows, else recomputed from the positions as the walk computed them;
+// longer rows are read again in every evaluation) and the sub-group reduces with DPP lane
+// exchanges, which give
 // its 8 lanes bitwise the same sums, so each sub-group's iteration is uniform.  A target
 // whose row overflowed gets R * 0.75, one whose solution lies beyond R/2 a larger R estimated from
 // its weighted count at R/2, and stays pending for another round.
@@ -316,6 +319,7 @@ __device__ __forceinline__ double rcp_refined(double x)
 
 #define HN_SLOTS 16  // candidates per lane held in registers (HN_LANES * HN_SLOTS per target)
 
+template <bool PER, bool R2>
 __global__ void __launch_bounds__(64) k_hnewton(HSolveArgs a)
 {
     constexpr int HL = HN_LANES, HS = HN_SLOTS, HR = HL * HS;
@@ -334,7 +338,19 @@ __global__ void __launch_bounds__(64) k_hnewton(HSolveArgs a)
     const bool solve = valid && !over;
     const int cj = solve ? cjs : 0;
     const double Rj = valid ? a.R[j] : 0.0, hj0 = valid ? a.h[j] : 0.0;
-    const double *rowj = a.cand + (solve ? j : 0) * (int64_t)cap;
+    const int32_t *rowj = a.cidx + (solve ? j : 0) * (int64_t)cap;
+    const double *rowr = R2 ? a.cand + (solve ? j : 0) * (int64_t)cap : nullptr;
+    const double qx = (!R2 && solve) ? a.tx[j] : 0.0, qy = (!R2 && solve) ? a.ty[j] : 0.0,
+                 qz = (!R2 && solve) ? a.tz[j] : 0.0;
+    // r^2 of candidate k: from the row, or with the walk's test operands and order (source
+    // minus target)
+    auto dist2 = [&](int k) {
+        if constexpr (R2) return rowr[k];
+        const int s = rowj[k];
+        const double dx = wrap_d<PER>(a.x[s] - qx, a.per), dy = wrap_d<PER>(a.y[s] - qy, a.per),
+                     dz = wrap_d<PER>(a.z[s] - qz, a.per);
+        return dx * dx + dy * dy + dz * dz;
+    };
     // the most candidates of a target in the wave (uniform): register slots in use, and
     // whether some row continues past them (then read from the row in every evaluation)
     int cw = cj;
@@ -350,8 +366,7 @@ __global__ void __launch_bounds__(64) k_hnewton(HSolveArgs a)
 #pragma unroll
     for (int u = 0; u < HS; u++) {
         const int k = lsub + HL * u;
-        r[u] = INFINITY;
-        if (u < nsl && k < cj) r[u] = rowj[k];
+        r[u] = (u < nsl && k < cj) ? dist2(k) : INFINITY;
     }
     double hnew = hj0, Rnew = Rj;
     int st = 0;
@@ -387,7 +402,7 @@ __global__ void __launch_bounds__(64) k_hnewton(HSolveArgs a)
             for (int u = 0; u < HS; u++)
                 if (u < nsl) term(r[u]);
             if (tail)
-                for (int k = HR + lsub; k < cj; k += HL) term(__builtin_sqrt(rowj[k]));
+                for (int k = HR + lsub; k < cj; k += HL) term(__builtin_sqrt(dist2(k)));
             sl = lane_group_sum<HL>(sl);
             sq = lane_group_sum<HL>(sq);
             f = (32.0 / 3.0) * sl - a.n_ngb;
@@ -466,7 +481,7 @@ __global__ void __launch_bounds__(64) k_hnewton(HSolveArgs a)
                     if (u < nsl && r[u] > cur && r[u] < mn) mn = r[u];
                 if (tail)
                     for (int k = HR + lsub; k < cj; k += HL) {
-                        const double x = rowj[k];
+                        const double x = dist2(k);
                         if (x > cur && x < mn) mn = x;
                     }
                 for (int off = HL / 2; off >= 1; off >>= 1) mn = fmin(mn, __shfl_xor(mn, off, HL));
@@ -475,7 +490,7 @@ __global__ void __launch_bounds__(64) k_hnewton(HSolveArgs a)
                 for (int u = 0; u < HS; u++)
                     if (u < nsl) mult += (r[u] == mn);
                 if (tail)
-                    for (int k = HR + lsub; k < cj; k += HL) mult += (rowj[k] == mn);
+                    for (int k = HR + lsub; k < cj; k += HL) mult += (dist2(k) == mn);
                 for (int off = HL / 2; off >= 1; off >>= 1) mult += __shfl_xor(mult, off, HL);
                 if (taken + mult >= need) { v = mn; break; }
                 taken += mult;
@@ -720,8 +735,8 @@ __global__ void __launch_bounds__(64) k_ngb(const int32_t *__restrict__ groups, 
 // ---------------------------------------------------------------- rows from h-solve candidates
 // After the h solve every target i holds its final candidate row
 // C_i = { j : r_ij^2 < R_i^2 } (FP64 r^2 with the association of the neighbour
-// test; a - b = -(b - a) exactly) and 2 h_i <= R_i exactly (the Newton bracket
-// stops at R_i/2).  Hence for S_i = { j : r_ij^2 < (h_i+h_j)^2 }:
+// test, recomputed here from x_j - x_i as the walk computed it; a - b = -(b - a)
+// exactly) and 2 h_i <= R_i exactly (the Newton bracket stops at R_i/2).  Hence for S_i = { j : r_ij^2 < (h_i+h_j)^2 }:
 //   h_j <= h_i:  (h_i+h_j)^2 <= (2 h_i)^2 <= R_i^2 by monotone rounding, so j in C_i;
 //   h_j >  h_i:  likewise i in C_j.
 // k_cand_rows keeps { j in C_i : r^2 < (h_i+h_j)^2 } in candidate order and
@@ -730,8 +745,9 @@ __global__ void __launch_bounds__(64) k_ngb(const int32_t *__restrict__ groups, 
 // k_row_tail_sort puts each row's appended part in ascending order, so a row's
 // order (and every sum over it) does not depend on the order of the appends.
 // Spread counters: 0 += rows longer than stride, 1 += entries.
-// hr[j] = (h_j, R_j), R_j replaced by -1 for a row of k_hsolve_big's (incomplete: such a j's
-// pairs are found by its own walk, k_big_rows), so one gather gives k_cand_rows both.
+// hr[j] = (h_j, R_j) (with distance rows) or rec[j] = (x_j, y_j, z_j, h_j, R_j) (without),
+// R_j replaced by -1 for a row of k_hsolve_big's (incomplete: such a j's pairs are found by
+// its own walk, k_big_rows), so one gather gives k_cand_rows both tests (and the distance).
 __global__ void __launch_bounds__(256) k_pack_hr(int64_t n, int cap, const double *__restrict__ h,
                                                  const double *__restrict__ R, const int32_t *__restrict__ ccnt,
                                                  double2 *__restrict__ hr)
@@ -740,12 +756,29 @@ __global__ void __launch_bounds__(256) k_pack_hr(int64_t n, int cap, const doubl
     if (j < n) hr[j] = make_double2(h[j], ccnt[j] <= cap ? R[j] : -1.0);
 }
 
+__global__ void __launch_bounds__(256) k_pack_rec(int64_t n, int cap, const double *__restrict__ x,
+                                                  const double *__restrict__ y, const double *__restrict__ z,
+                                                  const double *__restrict__ h, const double *__restrict__ R,
+                                                  const int32_t *__restrict__ ccnt, sph::CandRec *__restrict__ rec)
+{
+    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j < n) {
+        sph::CandRec c;
+        c.x = x[j]; c.y = y[j]; c.z = z[j]; c.h = h[j];
+        c.R = ccnt[j] <= cap ? R[j] : -1.0;
+        c.pad = 0.0;
+        rec[j] = c;
+    }
+}
+
 #define CR_UNROLL 5  // candidates per lane step of k_cand_rows
-template <int G>
+// R2: distances from the rows (cand) and (h_j, R_j) from hr; else recomputed from rec.
+template <int G, bool PER, bool R2>
 __global__ void __launch_bounds__(256) k_cand_rows(int64_t n, int cap, const double *__restrict__ cand,
                                                    const int32_t *__restrict__ cidx, const int32_t *__restrict__ ccnt,
-                                                   const double *__restrict__ h, const double2 *__restrict__ hr,
-                                                   int stride, int32_t *__restrict__ cnt, int32_t *__restrict__ cnt0,
+                                                   const double2 *__restrict__ hr,
+                                                   const sph::CandRec *__restrict__ rec, double per, int stride,
+                                                   int32_t *__restrict__ cnt, int32_t *__restrict__ cnt0,
                                                    int32_t *__restrict__ idx, int2 *__restrict__ ext, int64_t ext_cap,
                                                    unsigned long long *__restrict__ next,
                                                    unsigned long long *__restrict__ spread)
@@ -756,12 +789,18 @@ __global__ void __launch_bounds__(256) k_cand_rows(int64_t n, int cap, const dou
     const bool valid = i < n;
     const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
     const uint64_t gmask = (G == 64 ? ~0ull : ((1ull << G) - 1)) << (lane & ~(G - 1));
-    const double hi = valid ? h[i] : 0.0;
+    double hi = 0.0, xi = 0.0, yi = 0.0, zi = 0.0;
+    if constexpr (R2) {
+        hi = hr[valid ? i : 0].x;
+    } else {
+        const sph::CandRec own = rec[valid ? i : 0];
+        hi = own.h; xi = own.x; yi = own.y; zi = own.z;
+    }
     const int craw = valid ? ccnt[i] : 0;
     const bool incomplete = craw > cap;  // solved without a complete candidate row (k_hsolve_big)
     const int c = incomplete ? 0 : craw;
-    const double *rr = cand + (valid ? i : 0) * (int64_t)cap;
     const int32_t *rj = cidx + (valid ? i : 0) * (int64_t)cap;
+    const double *rr = R2 ? cand + (valid ? i : 0) * (int64_t)cap : nullptr;
     int32_t *row = idx + (valid ? i : 0) * (int64_t)stride;
     int w = 0;
     // one candidate of the row: member test, ballot-compacted append, extra pair
@@ -785,24 +824,46 @@ __global__ void __launch_bounds__(256) k_cand_rows(int64_t n, int cap, const dou
             if (e < (unsigned long long)rcap) ext[reg * rcap + (int64_t)e] = make_int2(j, (int)i);
         }
     };
-    // candidates k0 + l + u G (u < CR_UNROLL) per step: all their indices, distances and
-    // (h_j, R_j) are loaded before any is visited (CR_UNROLL gathers in flight per lane),
-    // visits in row order
-    for (int k0 = 0; k0 < c; k0 += CR_UNROLL * G) {
-        int jv[CR_UNROLL];
-        double rv[CR_UNROLL];
-        double2 hv[CR_UNROLL];
+    // candidates k0 + l + u G (u < CR_UNROLL) per step: all their indices (and distances),
+    // then all their records, are loaded before any is visited (CR_UNROLL gathers in flight
+    // per lane), visits in row order; without distance rows r^2 is recomputed with the walk's
+    // operands and order (x_j - x_i)
+    if constexpr (R2) {
+        for (int k0 = 0; k0 < c; k0 += CR_UNROLL * G) {
+            int jv[CR_UNROLL];
+            double rv[CR_UNROLL];
+            double2 hv[CR_UNROLL];
 #pragma unroll
-        for (int u = 0; u < CR_UNROLL; u++) {
-            const int k = k0 + l + u * G;
-            jv[u] = k < c ? rj[k] : 0;
-            rv[u] = k < c ? rr[k] : 0.0;
+            for (int u = 0; u < CR_UNROLL; u++) {
+                const int k = k0 + l + u * G;
+                jv[u] = k < c ? rj[k] : 0;
+                rv[u] = k < c ? rr[k] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < CR_UNROLL; u++)
+                hv[u] = (k0 + l + u * G < c) ? hr[jv[u]] : make_double2(0.0, 0.0);
+#pragma unroll
+            for (int u = 0; u < CR_UNROLL; u++) visit(k0 + l + u * G < c, jv[u], rv[u], hv[u].x, hv[u].y);
         }
+    } else {
+        for (int k0 = 0; k0 < c; k0 += CR_UNROLL * G) {
+            int jv[CR_UNROLL];
+            sph::CandRec cv[CR_UNROLL];
 #pragma unroll
-        for (int u = 0; u < CR_UNROLL; u++)
-            hv[u] = (k0 + l + u * G < c) ? hr[jv[u]] : make_double2(0.0, 0.0);
+            for (int u = 0; u < CR_UNROLL; u++) {
+                const int k = k0 + l + u * G;
+                jv[u] = k < c ? rj[k] : 0;
+            }
 #pragma unroll
-        for (int u = 0; u < CR_UNROLL; u++) visit(k0 + l + u * G < c, jv[u], rv[u], hv[u].x, hv[u].y);
+            for (int u = 0; u < CR_UNROLL; u++) cv[u] = rec[jv[u]];
+#pragma unroll
+            for (int u = 0; u < CR_UNROLL; u++) {
+                const double dx = wrap_d<PER>(cv[u].x - xi, per), dy = wrap_d<PER>(cv[u].y - yi, per),
+                             dz = wrap_d<PER>(cv[u].z - zi, per);
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                visit(k0 + l + u * G < c, jv[u], r2, cv[u].h, cv[u].R);
+            }
+        }
     }
     // the row and counts of an incomplete candidate row are written by k_big_rows
     const bool own = valid && l == 0 && !incomplete;
@@ -966,20 +1027,35 @@ __global__ void __launch_bounds__(256) k_hratio(int64_t n, const double *__restr
 
 namespace sph {
 
-hipError_t neighbour_rows_from_candidates_begin(hipStream_t st, int64_t n, int cap, const double *cand,
+hipError_t neighbour_rows_from_candidates_begin(hipStream_t st, int64_t n, int cap, const double *x,
+                                               const double *y, const double *z, double per, const double *cand,
                                                const int32_t *cidx, const int32_t *ccnt, const double *h,
-                                               const double *R, int stride, int32_t *cnt, int32_t *cnt0,
-                                               int32_t *idx, int2 *ext, int64_t ext_cap, unsigned long long *next,
+                                               const double *R, int stride,
+                                               int32_t *cnt, int32_t *cnt0, int32_t *idx, int2 *ext,
+                                               int64_t ext_cap, unsigned long long *next,
                                                unsigned long long *spread, unsigned long long *totals,
                                                unsigned long long *host_words, hipEvent_t done,
-                                               const BigRows &big, double2 *hr)
+                                               const BigRows &big, void *rec)
 {
     if (stride < 1 || cap < 1 || ext_cap < SPH_SPREAD) return hipErrorInvalidValue;
     fill<unsigned long long>(st, SPH_SPREAD_WORDS, next, 0ull);
     fill<unsigned long long>(st, 2 * SPH_SPREAD_WORDS, spread, 0ull);
-    k_pack_hr<<<sph_blocks(n, 256), 256, 0, st>>>(n, cap, h, R, ccnt, hr);
-    k_cand_rows<16><<<sph_blocks(n * 16, 256), 256, 0, st>>>(n, cap, cand, cidx, ccnt, h, hr, stride, cnt, cnt0, idx,
-                                                            ext, ext_cap, next, spread);
+    const unsigned nbr = (unsigned)sph_blocks(n * 16, 256);
+    double2 *hr = static_cast<double2 *>(rec);
+    CandRec *cr = static_cast<CandRec *>(rec);
+    if (cand) {
+        k_pack_hr<<<sph_blocks(n, 256), 256, 0, st>>>(n, cap, h, R, ccnt, hr);
+        k_cand_rows<16, false, true><<<nbr, 256, 0, st>>>(n, cap, cand, cidx, ccnt, hr, nullptr, per, stride, cnt,
+                                                          cnt0, idx, ext, ext_cap, next, spread);
+    } else {
+        k_pack_rec<<<sph_blocks(n, 256), 256, 0, st>>>(n, cap, x, y, z, h, R, ccnt, cr);
+        if (per > 0.0)
+            k_cand_rows<16, true, false><<<nbr, 256, 0, st>>>(n, cap, nullptr, cidx, ccnt, nullptr, cr, per, stride,
+                                                              cnt, cnt0, idx, ext, ext_cap, next, spread);
+        else
+            k_cand_rows<16, false, false><<<nbr, 256, 0, st>>>(n, cap, nullptr, cidx, ccnt, nullptr, cr, per, stride,
+                                                               cnt, cnt0, idx, ext, ext_cap, next, spread);
+    }
     if (big.count > 0) {
         const unsigned nb = (unsigned)big.count;
         if (big.per > 0.0)
@@ -1115,7 +1191,13 @@ void hsolve_round(hipStream_t st, const HSolveParams &p, HSolveArgs a, int64_t p
         k_hwalk<true><<<(unsigned)nblk, 64, 0, st>>>(a);
     else
         k_hwalk<false><<<(unsigned)nblk, 64, 0, st>>>(a);
-    k_hnewton<<<(unsigned)((nt + 64 / HN_LANES - 1) / (64 / HN_LANES)), 64, 0, st>>>(a);
+    const unsigned nbn = (unsigned)((nt + 64 / HN_LANES - 1) / (64 / HN_LANES));
+    if (a.cand)
+        k_hnewton<false, true><<<nbn, 64, 0, st>>>(a);  // (distances read from the rows)
+    else if (a.per > 0.0)
+        k_hnewton<true, false><<<nbn, 64, 0, st>>>(a);
+    else
+        k_hnewton<false, false><<<nbn, 64, 0, st>>>(a);
     spread_total(st, 5, p.counters, p.totals);
 }
 

@@ -114,6 +114,14 @@ class DeviceSPH:
         c.check(c.lib.sph_device_bytes(c.h, ctypes.byref(b)), "sph_device_bytes")
         return int(b.value)
 
+    def set_distance_rows_limit(self, nbytes: int):
+        """Largest size of the h solve's FP64 candidate distance rows (2 KB per particle);
+        beyond it the solve keeps candidate indices only and recomputes the distances (the
+        same results, less device memory, a slower solve).  Default: a third of the device's
+        memory."""
+        c = self.ctx
+        c.check(c.lib.sph_set_distance_rows_limit(c.h, int(nbytes)), "sph_set_distance_rows_limit")
+
     def leftover_mass(self) -> float:
         lo = np.zeros(1)
         c = self.ctx
