@@ -240,8 +240,11 @@ int sph_mlp_timing(sph_ctx *ctx, int64_t n, int reps, uint64_t seed, double tol,
  * SPH_COOL_EXACT: FP64 T(u) and rates; SPH_COOL_TABLE: T from a temperature table over
  * (log n_H, log u, log J21) and log Lambda, log Gamma from the 4D table;
  * SPH_COOL_MLP: as TABLE with log Lambda from the gated surrogate (sph_mlp_cooling).
- * The table (and for SPH_COOL_MLP the calibrated surrogate) must be loaded in the context. */
-enum sph_cool_mode { SPH_COOL_EXACT = 0, SPH_COOL_TABLE = 1, SPH_COOL_MLP = 2 };
+ * The table (and for SPH_COOL_MLP the calibrated surrogate) must be loaded in the context.
+ * SPH_COOL_NH (resident step only): doCloudyHCooling itself over the table of
+ * sph_nh_table_set, with the log N_H of the step's own column-density pass (which must be
+ * on) and r = |x|; Z and J21 are not used. */
+enum sph_cool_mode { SPH_COOL_EXACT = 0, SPH_COOL_TABLE = 1, SPH_COOL_MLP = 2, SPH_COOL_NH = 3 };
 typedef struct sph_cool_params {
     double unit_density_cgs; /* UnitDensity_in_cgs */
     double unit_u_cgs;       /* Unit_u_in_cgs */
@@ -258,6 +261,24 @@ typedef struct sph_cool_params {
 int sph_cooling_update(sph_ctx *ctx, int64_t n, const double *rho, const double *u, const double *Z,
                        const double *J21, double dt, const sph_cool_params *cp, double *u_out,
                        int32_t *fallback_out, int64_t *n_floor, int64_t *n_fallback);
+/* ---- cooling from a table over (log n_H, log T, r, log N_H): doCloudyHCooling with
+ *      find_nx0 and interpolate_4d_hypercube (CR/v4t.h:1450-1476, 1681-1765, 1767-1884; the
+ *      table file HCoolMu.bin as CR/v3t.cu:45-122 reads it) ---------------------------------
+ * dims = (n_nH, n_T, n_r, n_NH), each >= 2, at most 2048 grid points together; bounds = the
+ * (nH, T, r, NH) clamp pairs (low, high); grids ascending; heat, cool (log10 rates), mu and
+ * logu (log10 u_cgs) on [nH][T][r][NH], n_nH n_T n_r n_NH floats each.  Copied to the device. */
+int sph_nh_table_set(sph_ctx *ctx, const int32_t *dims, const float *bounds, const float *nH_grid,
+                     const float *T_grid, const float *r_grid, const float *NH_grid, const float *heat,
+                     const float *cool, const float *mu, const float *logu);
+/* u_out[n] (code units) after one step dt: particles (pos, rho, u, log_nh = log10 N_H) in caller
+ * order, typ (may be NULL) leaves particles with typ != 0 unchanged (the reference's Typ[i] == 0
+ * test).  Units, XH and u_floor_cgs come from cp (its mode is not used), gamma is the adiabatic
+ * index; *n_floor (may be NULL) counts floored particles.  FP32 arithmetic in the reference's
+ * order, with its swapped T and r weights in the interpolation; the reference's float log10
+ * and sqrt are taken correctly rounded. */
+int sph_nh_cooling(sph_ctx *ctx, int64_t n, const double *pos, const double *rho, const double *u,
+                   const double *log_nh, const int32_t *typ, double gamma, double dt, const sph_cool_params *cp,
+                   double *u_out, int64_t *n_floor);
 /* T(u_cgs, n_H, J21) by the temperature table of SPH_COOL_TABLE / SPH_COOL_MLP (built for XH
  * on first use); the table's counterpart of convert_u_to_temp_h (photolibs3.py:379). */
 int sph_cooling_temperature(sph_ctx *ctx, int64_t n, const double *u, const double *nH, const double *J21,

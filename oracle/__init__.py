@@ -59,6 +59,8 @@ def lib():
         L.or_cool_rate_from_u.argtypes = [ctypes.c_double] * 5
         L.or_do_cooling.argtypes = [ctypes.c_double] * 7
         L.or_set_period.argtypes = [ctypes.c_double]
+        L.or_nh_cooling.restype = ctypes.c_int64
+        L.or_nh_nx0.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.c_float]
         _LIB = L
     return _LIB
 
@@ -74,6 +76,8 @@ def _p(a):
         return a.ctypes.data_as(_i32p)
     if a.dtype == np.uint64:
         return a.ctypes.data_as(_u64p)
+    if a.dtype == np.float32:
+        return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
     raise TypeError(a.dtype)
 
 
@@ -469,3 +473,30 @@ class periodic:
     def __exit__(self, *exc):
         lib().or_set_period(0.0)
         return False
+
+
+# ---------------------------------------------------------------- cooling from an N_H table
+def nh_nx0(grid, x):
+    """find_nx0 (CR/v4t.h:1450-1476) of float32 grid and value x."""
+    g = np.ascontiguousarray(grid, np.float32)
+    return int(lib().or_nh_nx0(_p(g), len(g), ctypes.c_float(float(np.float32(x)))))
+
+
+def nh_cooling(table, pos, rho, u, log_nh, dt, unit_density, unit_u, unit_time, XH=0.7, u_floor=1.8e11,
+               gamma=5.0 / 3.0, typ=None):
+    """doCloudyHCooling (CR/v4t.h:1767-1884) over a table dict with keys dims (n_nH, n_T, n_r,
+    n_NH), bounds (8), nH, T, r, NH (grids), heat, cool, mu, logu ([nH][T][r][NH]); inputs are
+    rounded to float32 as the reference holds them.  Returns (u_out float32, n_floored)."""
+    dims = np.asarray(table["dims"], np.int32)
+    b = np.asarray(table["bounds"], np.float32)
+    g = np.concatenate([np.asarray(table[k], np.float32).ravel() for k in ("nH", "T", "r", "NH")])
+    tabs = [np.ascontiguousarray(np.asarray(table[k], np.float32).ravel()) for k in ("heat", "cool", "mu", "logu")]
+    n = len(rho)
+    pos = np.ascontiguousarray(np.asarray(pos, np.float64).reshape(n, 3).astype(np.float32))
+    arr = [np.ascontiguousarray(np.asarray(v, np.float64).astype(np.float32)) for v in (rho, u, log_nh)]
+    t = None if typ is None else np.ascontiguousarray(typ, np.int32)
+    f = np.array([unit_density, unit_u, unit_time, XH, u_floor, gamma, dt], np.float64)
+    out = np.empty(n, np.float32)
+    nf = lib().or_nh_cooling(_i64(n), dims.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _p(b), _p(g),
+                             *[_p(x) for x in tabs], _p(pos), *[_p(x) for x in arr], _p(t), _p(f), _p(out))
+    return out, int(nf)

@@ -79,7 +79,7 @@ class AgnParams(ctypes.Structure):
                 ("h_ref_index", ctypes.c_int64)]
 
 
-COOL_MODE = {"exact": 0, "table": 1, "mlp": 2}
+COOL_MODE = {"exact": 0, "table": 1, "mlp": 2, "nh": 3}
 MLP_REF = {"exact": 0, "table": 1}
 
 
@@ -132,6 +132,9 @@ _SIGS = {
                             _i64p, _i64p], ctypes.c_int),
     "sph_cooling_temperature": ([_vp, _i64, _d, _d, _d, ctypes.c_double, _d], ctypes.c_int),
     "sph_state_set_cooling": ([_vp, _d, _d, ctypes.POINTER(CoolParams)], ctypes.c_int),
+    "sph_nh_table_set": ([_vp, _i32p, _f, _f, _f, _f, _f, _f, _f, _f, _f], ctypes.c_int),
+    "sph_nh_cooling": ([_vp, _i64, _d, _d, _d, _d, _i32p, ctypes.c_double, ctypes.c_double,
+                        ctypes.POINTER(CoolParams), _d, _i64p], ctypes.c_int),
     "sph_column_density": ([_vp, _i64, _d, _d, _d, _d, ctypes.POINTER(ColumnParams), ctypes.c_int, _d, _d],
                            ctypes.c_int),
     "sph_state_set_column": ([_vp, ctypes.POINTER(ColumnParams)], ctypes.c_int),

@@ -20,7 +20,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsph_amd.so")
 SOURCES = ["sph_order.hip", "sph_tree.hip", "sph_ngb.hip", "sph_hydro.hip", "sph_grav.hip",
-           "sph_integrate.hip", "sph_cool.hip", "sph_mlp.hip", "sph_agn.hip", "sph_capi.hip",
+           "sph_integrate.hip", "sph_cool.hip", "sph_nhcool.hip", "sph_mlp.hip", "sph_agn.hip", "sph_capi.hip",
            "sph_capi_api.hip", "sph_capi_step.hip", "sph_capi_cool.hip"]
 ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

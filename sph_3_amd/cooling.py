@@ -157,8 +157,10 @@ class CoolingParams:
     """Units and switches of the per-step cooling update (the explicit update of
     CR/v4t.h:1768-1884 doCloudyHCooling; defaults XH = 0.7 and u floor 1.8e11 erg/g
     are that kernel's).  mode: "exact" (FP64 T(u) and rates), "table" (temperature
-    table + 4D rate table) or "mlp" (as "table" with log Lambda from the gated
-    surrogate); tol / ref: the surrogate's gate (CoolingSurrogate.rates)."""
+    table + 4D rate table), "mlp" (as "table" with log Lambda from the gated
+    surrogate) or "nh" (the reference's own table over (log n_H, log T, r, log N_H),
+    ``NHCoolingTable``; resident step only, with the column density on); tol / ref: the
+    surrogate's gate (CoolingSurrogate.rates)."""
     unit_density_cgs: float
     unit_u_cgs: float
     unit_time_s: float
@@ -321,3 +323,74 @@ class CoolingSurrogate:
         c = self.ctx
         c.check(c.lib.sph_mlp_timing(c.h, int(n), int(reps), int(seed), float(tol), ptr(out)), "surrogate timing")
         return dict(ms_gated=out[0], ms_table=out[1], ms_network=out[2], fallback_fraction=out[3])
+
+
+NH_TABLE_KEYS = ("nH", "T", "r", "NH")
+
+
+class NHCoolingTable:
+    """The cooling table of doCloudyHCooling (CR/v4t.h:1767-1884) over (log n_H, log T, r,
+    log N_H): the four ascending grids, their clamp bounds, and log10 heating, log10
+    cooling, mu and log10 u on [nH][T][r][NH], all float32 as HCoolMu.bin holds them
+    (layout as CR/v3t.cu:45-122 reads it: four int32 sizes, eight float32 bounds, the
+    grids, then heat, cool, mu, u)."""
+
+    def __init__(self, grids, bounds, heat, cool, mu, logu):
+        self.grids = [np.ascontiguousarray(g, np.float32).ravel() for g in grids]
+        self.dims = tuple(len(g) for g in self.grids)
+        self.bounds = np.ascontiguousarray(bounds, np.float32).ravel()
+        if len(self.grids) != 4 or self.bounds.shape != (8,):
+            raise ValueError("four grids and eight bounds")
+        self.tables = [np.ascontiguousarray(t, np.float32).reshape(self.dims) for t in (heat, cool, mu, logu)]
+
+    @classmethod
+    def read(cls, path):
+        """A table file in the HCoolMu.bin layout."""
+        with open(path, "rb") as f:
+            dims = np.fromfile(f, np.int32, 4)
+            bounds = np.fromfile(f, np.float32, 8)
+            grids = [np.fromfile(f, np.float32, int(d)) for d in dims]
+            cells = int(np.prod(dims.astype(np.int64)))
+            tabs = [np.fromfile(f, np.float32, cells) for _ in range(4)]
+        if any(len(t) != cells for t in tabs):
+            raise ValueError(f"{path}: truncated table")
+        return cls(grids, bounds, *tabs)
+
+    def write(self, path):
+        with open(path, "wb") as f:
+            np.asarray(self.dims, np.int32).tofile(f)
+            self.bounds.tofile(f)
+            for g in self.grids:
+                g.tofile(f)
+            for t in self.tables:
+                t.tofile(f)
+
+    def as_dict(self):
+        d = dict(zip(NH_TABLE_KEYS, self.grids))
+        d.update(dims=self.dims, bounds=self.bounds, heat=self.tables[0], cool=self.tables[1], mu=self.tables[2],
+                 logu=self.tables[3])
+        return d
+
+    def install(self, ctx: Context):
+        f = ctypes.POINTER(ctypes.c_float)
+        dims = np.asarray(self.dims, np.int32)
+        args = [a.ctypes.data_as(f) for a in (self.bounds, *self.grids, *self.tables)]
+        ctx.check(ctx.lib.sph_nh_table_set(ctx.h, ptr(dims), *args), "sph_nh_table_set")
+
+    def cooling_update(self, pos, rho, u, log_nh, dt, params: CoolingParams, gamma=5.0 / 3.0, typ=None,
+                       device=0):
+        """u after one step dt (code units) of doCloudyHCooling for particles (pos, rho, u,
+        log10 N_H); ``typ`` (optional) leaves particles with typ != 0 unchanged.  Returns
+        (u_out, number floored)."""
+        c = context(device)
+        self.install(c)
+        pos = f64(pos, (-1, 3))
+        n = len(pos)
+        rho, u, lnh = (np.broadcast_to(f64(v), (n,)).copy() for v in (rho, u, log_nh))
+        t = None if typ is None else np.ascontiguousarray(typ, np.int32)
+        out = np.empty(n)
+        nf = np.zeros(1, np.int64)
+        cp = params.c()
+        c.check(c.lib.sph_nh_cooling(c.h, n, ptr(pos), ptr(rho), ptr(u), ptr(lnh), ptr(t), float(gamma), float(dt),
+                                     ctypes.byref(cp), ptr(out), ptr(nf)), "sph_nh_cooling")
+        return out, int(nf[0])

@@ -438,7 +438,7 @@ std::vector<Buf *> all_bufs(sph_ctx *ctx)
         &ctx->rf1, &ctx->ctabL, &ctx->ctabG, &ctx->cin[0], &ctx->cin[1], &ctx->cin[2], &ctx->cin[3],
         &ctx->cin[4], &ctx->cin[5], &ctx->cout[0], &ctx->cout[1], &ctx->mP, &ctx->mG, &ctx->mM, &ctx->mV,
         &ctx->mW[0], &ctx->mW[1], &ctx->mW[2], &ctx->mW[3], &ctx->mimg, &ctx->mx, &ctx->mxn, &ctx->mt, &ctx->my,
-        &ctx->mloss, &ctx->memap, &ctx->memap2, &ctx->memap3, &ctx->mgate, &ctx->mcnt, &ctx->mfl};
+        &ctx->mloss, &ctx->memap, &ctx->memap2, &ctx->memap3, &ctx->mgate, &ctx->mcnt, &ctx->mfl, &ctx->nhtab};
     for (Buf &b : ctx->shadow) v.push_back(&b);
     return v;
 }

@@ -566,11 +566,16 @@ namespace sphc {
 int cool_check(sph_ctx *ctx, const sph_cool_params *cp)
 {
     if (!cp) return fail(ctx, SPH_ERR_ARG, "null cooling parameters");
-    if (cp->mode != SPH_COOL_EXACT && cp->mode != SPH_COOL_TABLE && cp->mode != SPH_COOL_MLP)
-        return fail(ctx, SPH_ERR_ARG, "mode must be SPH_COOL_EXACT, SPH_COOL_TABLE or SPH_COOL_MLP");
+    if (cp->mode != SPH_COOL_EXACT && cp->mode != SPH_COOL_TABLE && cp->mode != SPH_COOL_MLP &&
+        cp->mode != SPH_COOL_NH)
+        return fail(ctx, SPH_ERR_ARG, "mode must be SPH_COOL_EXACT, SPH_COOL_TABLE, SPH_COOL_MLP or SPH_COOL_NH");
     if (!(cp->unit_density_cgs > 0.0) || !(cp->unit_u_cgs > 0.0) || !(cp->unit_time_s > 0.0) ||
         !(cp->XH > 0.0 && cp->XH <= 1.0) || !(cp->u_floor_cgs >= 0.0))
         return fail(ctx, SPH_ERR_ARG, "cooling units must be > 0, 0 < XH <= 1 and u_floor_cgs >= 0");
+    if (cp->mode == SPH_COOL_NH) {
+        if (!ctx->nh_ok) return fail(ctx, SPH_ERR_STATE, "no N_H cooling table (call sph_nh_table_set)");
+        return SPH_OK;
+    }
     if (!ctx->ctab_ok) return fail(ctx, SPH_ERR_STATE, "no cooling table (call sph_cooling_table_build)");
     if (cp->mode == SPH_COOL_MLP) {
         CK(mlp_need(ctx, true, true));
@@ -665,12 +670,133 @@ int cool_run(sph_ctx *ctx, int64_t n, const RHO *rho, const double *u, const dou
     return SPH_OK;
 }
 
+NHTable nh_table(const sph_ctx *ctx)
+{
+    NHTable t;
+    int ng = 0;
+    for (int k = 0; k < 4; k++) {
+        t.dims[k] = ctx->nh_dims[k];
+        ng += ctx->nh_dims[k];
+    }
+    for (int k = 0; k < 8; k++) t.bounds[k] = ctx->nh_bounds[k];
+    const size_t cells = (size_t)t.dims[0] * t.dims[1] * t.dims[2] * t.dims[3];
+    const float *base = static_cast<const float *>(ctx->nhtab.p);
+    const size_t off = ((size_t)ng + 63) / 64 * 64;
+    t.grids = base;
+    t.heat = base + off;
+    t.cool = t.heat + cells;
+    t.mu = t.cool + cells;
+    t.logu = t.mu + cells;
+    return t;
+}
+
+template <typename RHO>
+int nh_run(sph_ctx *ctx, int64_t n, const double *x, const double *y, const double *z, const RHO *rho,
+           const double *u, const double *log_nh, const int32_t *typ, double gamma, double dt,
+           const sph_cool_params &cp, double *u_out, unsigned long long *counts)
+{
+    if (!ctx->nh_ok) return fail(ctx, SPH_ERR_STATE, "no N_H cooling table (call sph_nh_table_set)");
+    hipStream_t st = ctx->st;
+    unsigned long long *spr = P_<unsigned long long>(ctx->spread) + SPR_C * SPH_SPREAD_WORDS;
+    HIPCK(sph::fill<unsigned long long>(st, SPH_SPREAD_WORDS, spr, 0ull));
+    HIPCK(sph::fill<unsigned long long>(st, 1, counts + 1, 0ull));
+    const NHCoolArgs a{(float)cp.unit_density_cgs, (float)cp.unit_u_cgs, (float)cp.unit_time_s, (float)cp.XH,
+                            (float)gamma, (float)dt, cp.u_floor_cgs};
+    HIPCK(sph::nh_cool_update<RHO>(st, n, x, y, z, rho, u, log_nh, typ, nh_table(ctx), a, u_out, spr));
+    HIPCK(sph::spread_total(st, 1, spr, counts));
+    return SPH_OK;
+}
+
+template int nh_run<float>(sph_ctx *, int64_t, const double *, const double *, const double *, const float *,
+                           const double *, const double *, const int32_t *, double, double, const sph_cool_params &,
+                           double *, unsigned long long *);
+template int nh_run<double>(sph_ctx *, int64_t, const double *, const double *, const double *, const double *,
+                            const double *, const double *, const int32_t *, double, double, const sph_cool_params &,
+                            double *, unsigned long long *);
+
 template int cool_run<float>(sph_ctx *, int64_t, const float *, const double *, const double *, const double *, double,
                              const sph_cool_params &, double *, unsigned long long *);
 template int cool_run<double>(sph_ctx *, int64_t, const double *, const double *, const double *, const double *,
                               double, const sph_cool_params &, double *, unsigned long long *);
 
 }  // namespace sphc
+
+extern "C" int sph_nh_table_set(sph_ctx *ctx, const int32_t *dims, const float *bounds, const float *nH_grid,
+                                const float *T_grid, const float *r_grid, const float *NH_grid, const float *heat,
+                                const float *cool, const float *mu, const float *logu)
+{
+    if (!ctx) return SPH_ERR_ARG;
+    if (!dims || !bounds || !nH_grid || !T_grid || !r_grid || !NH_grid || !heat || !cool || !mu || !logu)
+        return fail(ctx, SPH_ERR_ARG, "null argument");
+    int ng = 0;
+    int64_t cells = 1;
+    for (int k = 0; k < 4; k++) {
+        if (dims[k] < 2) return fail(ctx, SPH_ERR_ARG, "every N_H table axis needs at least 2 points");
+        ng += dims[k];
+        cells *= dims[k];
+    }
+    if (ng > SPH_NH_GRID_MAX) return fail(ctx, SPH_ERR_ARG, "N_H table grids exceed 2048 points together");
+    if (cells > ((int64_t)1 << 28)) return fail(ctx, SPH_ERR_ARG, "N_H table too large");
+    const float *grids[4] = {nH_grid, T_grid, r_grid, NH_grid};
+    for (int k = 0; k < 4; k++) {
+        for (int q = 0; q < dims[k]; q++)
+            if (!std::isfinite(grids[k][q]) || (q > 0 && !(grids[k][q] > grids[k][q - 1])))
+                return fail(ctx, SPH_ERR_ARG, "N_H table grids must be finite and strictly ascending");
+        if (!std::isfinite(bounds[2 * k]) || !std::isfinite(bounds[2 * k + 1]))
+            return fail(ctx, SPH_ERR_ARG, "N_H table bounds must be finite");
+    }
+    HIPCK(hipSetDevice(ctx->device));
+    const size_t off = ((size_t)ng + 63) / 64 * 64;
+    CK(ensure(ctx, ctx->nhtab, (off + 4 * (size_t)cells) * sizeof(float)));
+    std::vector<float> g;
+    g.reserve(off);
+    for (int k = 0; k < 4; k++) g.insert(g.end(), grids[k], grids[k] + dims[k]);
+    g.resize(off, 0.0f);
+    float *base = P_<float>(ctx->nhtab);
+    HIPCK(hipMemcpyAsync(base, g.data(), off * sizeof(float), hipMemcpyHostToDevice, ctx->st));
+    const float *tabs[4] = {heat, cool, mu, logu};
+    for (int k = 0; k < 4; k++)
+        HIPCK(hipMemcpyAsync(base + off + k * (size_t)cells, tabs[k], (size_t)cells * sizeof(float),
+                             hipMemcpyHostToDevice, ctx->st));
+    HIPCK(hipStreamSynchronize(ctx->st));
+    for (int k = 0; k < 4; k++) ctx->nh_dims[k] = dims[k];
+    for (int k = 0; k < 8; k++) ctx->nh_bounds[k] = bounds[k];
+    ctx->nh_ok = true;
+    return SPH_OK;
+}
+
+extern "C" int sph_nh_cooling(sph_ctx *ctx, int64_t n, const double *pos, const double *rho, const double *u,
+                              const double *log_nh, const int32_t *typ, double gamma, double dt,
+                              const sph_cool_params *cp, double *u_out, int64_t *n_floor)
+{
+    CK(cool_prep(ctx, n));
+    if (!pos || !rho || !u || !log_nh || !u_out || !cp) return fail(ctx, SPH_ERR_ARG, "null argument");
+    if (!(dt >= 0.0) || !(gamma > 1.0)) return fail(ctx, SPH_ERR_ARG, "dt must be >= 0 and gamma > 1");
+    sph_cool_params c = *cp;
+    c.mode = SPH_COOL_NH;
+    CK(cool_check(ctx, &c));
+    CK(ensure(ctx, ctx->spread, (size_t)SPR_TOTAL * SPH_SPREAD_WORDS * 8));
+    CK(ensure(ctx, ctx->ucnt, 2 * 8));
+    if (typ) CK(ensure(ctx, ctx->uflag, (size_t)n * 4));
+    std::vector<double> xyz((size_t)n * 3);
+    for (int64_t i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) xyz[(size_t)d * n + i] = pos[3 * i + d];
+    for (int d = 0; d < 3; d++)
+        HIPCK(hipMemcpyAsync(ctx->cin[d].p, xyz.data() + (size_t)d * n, sizeof(double) * n, hipMemcpyHostToDevice,
+                             ctx->st));
+    const double *src[3] = {rho, u, log_nh};
+    for (int k = 0; k < 3; k++)
+        HIPCK(hipMemcpyAsync(ctx->cin[3 + k].p, src[k], sizeof(double) * n, hipMemcpyHostToDevice, ctx->st));
+    if (typ) HIPCK(hipMemcpyAsync(ctx->uflag.p, typ, sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->st));
+    CK(nh_run<double>(ctx, n, D(cin[0]), D(cin[1]), D(cin[2]), D(cin[3]), D(cin[4]), D(cin[5]),
+                      typ ? I32(uflag) : nullptr, gamma, dt, c, D(cout[0]), P_<unsigned long long>(ctx->ucnt)));
+    HIPCK(hipMemcpyAsync(u_out, ctx->cout[0].p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->st));
+    unsigned long long cc[2] = {0, 0};
+    HIPCK(hipMemcpyAsync(cc, ctx->ucnt.p, sizeof(cc), hipMemcpyDeviceToHost, ctx->st));
+    HIPCK(hipStreamSynchronize(ctx->st));
+    if (n_floor) *n_floor = (int64_t)cc[0];
+    return SPH_OK;
+}
 
 extern "C" int sph_cooling_update(sph_ctx *ctx, int64_t n, const double *rho, const double *u, const double *Z,
                                   const double *J21, double dt, const sph_cool_params *cp, double *u_out,
@@ -725,6 +851,12 @@ extern "C" int sph_state_set_cooling(sph_ctx *ctx, const double *Z, const double
     HIPCK(hipSetDevice(ctx->device));
     if (!cp) {
         ctx->cool_on = false;
+        return SPH_OK;
+    }
+    if (cp->mode == SPH_COOL_NH) {  // Z and J21 are not used; the step's column density must be on
+        CK(cool_check(ctx, cp));
+        ctx->cool_p = *cp;
+        ctx->cool_on = true;
         return SPH_OK;
     }
     if (!Z || !J21) return fail(ctx, SPH_ERR_ARG, "null argument");

@@ -81,7 +81,7 @@ int reorder_state(sph_ctx *ctx)
     Buf *arr[SPH_GATHER_MAX] = {&ctx->x, &ctx->y, &ctx->z, &ctx->vx, &ctx->vy, &ctx->vz, &ctx->m, &ctx->h,
                                 &ctx->u, &ctx->duprev, &ctx->ax, &ctx->ay, &ctx->az};
     int k = 13;
-    if (ctx->cool_on) {
+    if (cool_columns(ctx)) {
         arr[k++] = &ctx->zmet;
         arr[k++] = &ctx->j21;
     }
@@ -284,8 +284,15 @@ int forces(sph_ctx *ctx, const sph_step_params *p, int first)
     ctx->cool_ran = ctx->cool_on && !first;
     if (ctx->cool_ran) {
         ph_begin(ctx, SPH_PH_COOLING);
-        CK(cool_run<R>(ctx, n, P_<R>(ctx->rho), D(u), D(zmet), D(j21), p->dt, ctx->cool_p, D(u),
-                       P_<unsigned long long>(ctx->hcount) + 12));
+        if (ctx->cool_p.mode == SPH_COOL_NH) {  // log N_H of this evaluation's column pass, r = |x|
+            if (!ctx->col_on)
+                return fail(ctx, SPH_ERR_STATE, "SPH_COOL_NH needs the column density on (sph_state_set_column)");
+            CK(nh_run<R>(ctx, n, D(x), D(y), D(z), P_<R>(ctx->rho), D(u), D(coln), nullptr, p->gamma, p->dt,
+                         ctx->cool_p, D(u), P_<unsigned long long>(ctx->hcount) + 12));
+        } else {
+            CK(cool_run<R>(ctx, n, P_<R>(ctx->rho), D(u), D(zmet), D(j21), p->dt, ctx->cool_p, D(u),
+                           P_<unsigned long long>(ctx->hcount) + 12));
+        }
         ph_end(ctx, SPH_PH_COOLING);
     }
     // per-particle time-step criteria of this evaluation (velocities after the kick)
@@ -365,7 +372,7 @@ int inject_outflow(sph_ctx *ctx, double dt)
     HIPCK(sph::h_of_caller(st, n, I32(orig), D(h), src, hsrc));
     sph::InjectParams ip{a.r_inject, a.v_in, a.m_inject, a.u_inject, a.Z_inject, a.J21_inject, a.seed, ctx->agn_step};
     sph::InjectTarget t{D(x), D(y), D(z), D(vx), D(vy), D(vz), D(m), D(h), D(u), D(duprev), D(ax), D(ay), D(az),
-                        I32(orig), ctx->cool_on ? D(zmet) : nullptr, ctx->cool_on ? D(j21) : nullptr};
+                        I32(orig), cool_columns(ctx) ? D(zmet) : nullptr, cool_columns(ctx) ? D(j21) : nullptr};
     HIPCK(sph::inject_pairs(st, n, pairs, ip, hsrc, t));
     HIPCK(hipStreamSynchronize(st));
     ctx->n = n + 2 * pairs;

@@ -19,6 +19,7 @@
 #include "sph_hydro.h"
 #include "sph_integrate.h"
 #include "sph_mlp.h"
+#include "sph_nhcool.h"
 #include "sph_ngb.h"
 #include "sph_order.h"
 #include "sph_tree.h"
@@ -88,6 +89,10 @@ struct sph_ctx {
     bool rf1_ok = false;
     CoolAxes cax{};
     bool ctab_ok = false;
+    Buf nhtab;                   // N_H cooling table: grids, heat, cool, mu, log u (floats)
+    int nh_dims[4] = {0, 0, 0, 0};
+    float nh_bounds[8] = {};
+    bool nh_ok = false;
     // cooling update: temperature table (built for ttab_XH), per-particle Z and J21 of the
     // resident state (sorted), its parameters, and the surrogate path's columns
     Buf ttab, zmet, j21, ucols, uyn, uL, uG, uflag, ucnt;
@@ -231,5 +236,14 @@ int hsolve_params(sph_ctx *ctx, int64_t m, const double *tx, const double *ty, c
 template <typename RHO>
 int cool_run(sph_ctx *ctx, int64_t n, const RHO *rho, const double *u, const double *Z, const double *J21,
              double dt, const sph_cool_params &cp, double *u_out, unsigned long long *counts);
+// Per-particle Z and J21 travel with the resident state (the cooling modes that read them; the
+// N_H-table mode reads neither and they are not allocated for it).
+inline bool cool_columns(const sph_ctx *ctx) { return ctx->cool_on && ctx->cool_p.mode != SPH_COOL_NH; }
+// sph_capi_cool.hip: the N_H-table cooling update (doCloudyHCooling) of n particles; counts[0]
+// floored, counts[1] = 0
+template <typename RHO>
+int nh_run(sph_ctx *ctx, int64_t n, const double *x, const double *y, const double *z, const RHO *rho,
+           const double *u, const double *log_nh, const int32_t *typ, double gamma, double dt,
+           const sph_cool_params &cp, double *u_out, unsigned long long *counts);
 
 }  // namespace sphc

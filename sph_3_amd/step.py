@@ -147,12 +147,21 @@ class DeviceSPH:
         """Per-particle metallicity Z [Z_sun] and UV amplitude J21 (caller order) and the
         cooling update (``cooling.CoolingParams``) of every later step.  The cooling
         table is built in this state's context; mode "mlp" also loads ``surrogate``
-        (a calibrated ``cooling.CoolingSurrogate``).  ``params=None`` switches it off."""
+        (a calibrated ``cooling.CoolingSurrogate``), mode "nh" the reference's
+        ``cooling.NHCoolingTable`` passed in its place (Z and J21 unused; the column
+        density must be on).  ``params=None`` switches it off."""
         from . import cooling as C
 
         c = self.ctx
         if params is None:
             c.check(c.lib.sph_state_set_cooling(c.h, None, None, None), "sph_state_set_cooling")
+            return
+        if params.mode == "nh":
+            if not isinstance(surrogate, C.NHCoolingTable):
+                raise ValueError('mode "nh" needs its NHCoolingTable (passed as surrogate)')
+            surrogate.install(c)
+            cp = params.c()
+            c.check(c.lib.sph_state_set_cooling(c.h, None, None, ctypes.byref(cp)), "sph_state_set_cooling")
             return
         if params.mode == "mlp":
             if surrogate is None:
