@@ -609,22 +609,36 @@ def sedov_section(args, cfg, run):
 
 def column_section(args, cfg, run, ms_step, kbar, d):
     """Column density N_H from |grad rho| (the CLOUDY driver's grad_rho_norm_ngb and N_H
-    loop) and the per-particle time-step criteria (dt_array_indiv_dt, C_CFL = 0.3) computed
-    in every force evaluation of the 1M Evrard step, units 1e10 M_sun and 10 kpc: each pass's
-    HIP-event time, the column pass's algorithmic bandwidth and the step time with both
-    against the headline step."""
+    loop), the per-particle time-step criteria (dt_array_indiv_dt, C_CFL = 0.3) computed in
+    every force evaluation of the 1M Evrard step, and the driver's cooling update over
+    (n_H, T, r, N_H) fed by that column density (doCloudyHCooling, on a synthetic table of
+    similar size), units 1e10 M_sun and 10 kpc: each pass's HIP-event time, the column pass's
+    algorithmic bandwidth and the step time with all three against the headline step."""
     from sph_3_amd import column as C
 
+    from sph_3_amd import cooling as K
+
     units = C.ColumnParams.from_mass_length(1.989e43, 3.0857e22)
+    # the reference's cooling over (n_H, T, r, N_H) fed by this column density; its table file
+    # is absent, so a smooth stand-in of similar size times the pass
+    nh_table = K.NHCoolingTable.synthetic()
+    nh_cool = K.CoolingParams.from_mass_length(1.989e43, 3.0857e22, mode="nh")
+
+    def setup(s):
+        s.set_column(units)
+        s.set_cooling(None, None, nh_cool, surrogate=nh_table)
+
     cfg.C_CFL = 0.3
     try:
-        sim, dt = run("mixed", args.steps, args.warmup, d=d, cooling=lambda s: s.set_column(units))
+        sim, dt = run("mixed", args.steps, args.warmup, d=d, cooling=setup)
     finally:
         cfg.C_CFL = 0.0
     st = sim.stats()
     n = st["n"]
     ms_col = st["phase_ms"]["column"] / max(1, st["phase_calls"]["column"])
     ms_ts = st["phase_ms"]["timestep"] / max(1, st["phase_calls"]["timestep"])
+    ms_nh = st["phase_ms"]["cooling"] / max(1, st["phase_calls"]["cooling"])
+    nh_floor = st.get("cool_floor")
     lognh, _ = sim.column()
     dt_min = st["dt_min"]
     del sim
@@ -635,7 +649,9 @@ def column_section(args, cfg, run, ms_step, kbar, d):
         "pass_bandwidth": {"GB/s": bpp * n / (ms_col * 1e-3) / 1e9,
                            "frac": bpp * n / (ms_col * 1e-3) / 1e9 / HBM_PEAK_GBS, "bytes_per_particle": bpp},
         "ms_timestep_pass": ms_ts, "dt_min_C_CFL_0.3": dt_min,
-        "ms_per_step_with_both": 1e3 * dt / args.steps, "ms_per_step_without": ms_step,
+        "ms_nh_cooling_pass": ms_nh, "nh_cooling_table": f"synthetic {'x'.join(map(str, nh_table.dims))}",
+        "nh_cooling_floored_fraction_last_step": nh_floor / n,
+        "ms_per_step_with_all": 1e3 * dt / args.steps, "ms_per_step_without": ms_step,
         "log_NH_range_cm2": [float(np.min(lognh)), float(np.median(lognh)), float(np.max(lognh))],
     }
 

@@ -344,6 +344,24 @@ class NHCoolingTable:
         self.tables = [np.ascontiguousarray(t, np.float32).reshape(self.dims) for t in (heat, cool, mu, logu)]
 
     @classmethod
+    def synthetic(cls, gamma=5.0 / 3.0):
+        """A smooth stand-in on axes like the reference run's (log n_H -4..4, log T 2..8, r 0..2,
+        log N_H 18..24), for timing and tests where HCoolMu.bin is absent: mu falls from 1.2 to
+        0.6 across log T = 4, log u is that mu's at each T row, the rates scale as n_H^2 with
+        cooling above heating beyond log T = 4 and below it under, smooth on every axis.  Not a
+        physical table."""
+        kb, mh = 1.3807e-16, 1.673534e-24
+        grids = [np.linspace(-4.0, 4.0, 33), np.linspace(2.0, 8.0, 61), np.linspace(0.0, 2.0, 21),
+                 np.linspace(18.0, 24.0, 25)]
+        nH, T, r, NH = np.meshgrid(*grids, indexing="ij")
+        mu = 0.6 + 0.6 / (1.0 + np.exp((T - 4.0) / 0.3))
+        logu = np.log10(kb * 10.0 ** T / ((gamma - 1.0) * mu * mh))
+        heat = -27.0 + 2.0 * nH + 0.1 * np.sin(1.3 * T) - 0.05 * r + 0.05 * (NH - 21.0)
+        cool = heat + 0.3 * np.tanh((T - 4.0) / 0.5) + 0.05 * np.cos(nH)
+        bounds = [g[i] for g in grids for i in (0, -1)]
+        return cls(grids, bounds, heat, cool, mu, logu)
+
+    @classmethod
     def read(cls, path):
         """A table file in the HCoolMu.bin layout."""
         with open(path, "rb") as f:

@@ -161,3 +161,27 @@ def test_mu_from_the_first_row_at_the_upper_corner():
     inner = (np.abs(fT - 0.5) < 0.45) & (inc > 1e-3 * u) & (inc < 1e3 * u) & (u + inc > 1.01 * 1.8e11 / UNITS["unit_u"])
     assert inner.sum() > 1000
     assert np.mean(np.abs(np.log10(got[inner]) - want[inner]) < 1e-3) > 0.999
+
+
+def test_table_file_layout(tmp_path):
+    """NHCoolingTable writes and reads the layout HCoolMu.bin has for the reference's reader
+    (CR/v3t.cu:57-85): four int32 sizes, eight float32 bounds, the four grids, then heat, cool,
+    mu and u, float32 in [nH][T][r][NH] order."""
+    from sph_3_amd.cooling import NHCoolingTable
+
+    t = NHCoolingTable.synthetic()
+    p = tmp_path / "HCoolMu.bin"
+    t.write(p)
+    raw = p.read_bytes()
+    dims = np.frombuffer(raw[:16], np.int32)
+    assert tuple(dims) == t.dims == (33, 61, 21, 25)
+    f = np.frombuffer(raw[16:], np.float32)
+    assert np.array_equal(f[:8], t.bounds)
+    ng = int(dims.sum())
+    cells = int(np.prod(dims))
+    assert len(f) == 8 + ng + 4 * cells
+    assert np.array_equal(f[8:8 + 33], t.grids[0]) and np.array_equal(f[8 + 33 + 61 + 21:8 + ng], t.grids[3])
+    heat = f[8 + ng:8 + ng + cells].reshape(dims)
+    assert np.array_equal(heat, t.tables[0]) and heat[1, 2, 3, 4] == t.tables[0][1, 2, 3, 4]
+    u = NHCoolingTable.read(p)
+    assert u.dims == t.dims and all(np.array_equal(a, b) for a, b in zip(u.tables, t.tables))
