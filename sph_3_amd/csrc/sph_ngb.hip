@@ -963,25 +963,34 @@ __global__ void __launch_bounds__(64) k_big_rows(const int32_t *__restrict__ lis
     }
 }
 
+// The extras of region r (ext[r rcap + e], e < min(next[r], rcap)) appended to their rows:
+// blocks r, r + SPH_SPREAD, ... stride over region r, so the work follows the pairs recorded
+// rather than the regions' capacity.
 __global__ void __launch_bounds__(256) k_ext_append(int64_t ext_cap, const int2 *__restrict__ ext,
                                                     const unsigned long long *__restrict__ next,
                                                     int32_t *__restrict__ cnt, int32_t *__restrict__ idx, int stride,
                                                     unsigned long long *__restrict__ spread)
 {
-    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int64_t rcap = ext_cap / SPH_SPREAD, reg = e / rcap;
-    bool longrow = false, added = false;
-    if (reg < SPH_SPREAD && (unsigned long long)(e - reg * rcap) < next[reg * SPH_SPREAD_PAD]) {
-        const int2 p = ext[e];
+    const int64_t rcap = ext_cap / SPH_SPREAD;
+    const int reg = blockIdx.x % SPH_SPREAD;
+    const int64_t step = (int64_t)(gridDim.x / SPH_SPREAD) * blockDim.x;
+    const unsigned long long nr = next[reg * SPH_SPREAD_PAD];
+    const int64_t ne = nr < (unsigned long long)rcap ? (int64_t)nr : rcap;
+    unsigned long long nlong = 0, nadd = 0;
+    for (int64_t e = (int64_t)(blockIdx.x / SPH_SPREAD) * blockDim.x + threadIdx.x; e < ne; e += step) {
+        const int2 p = ext[reg * rcap + e];
         const int pos = atomicAdd(&cnt[p.x], 1);
         if (pos < stride) idx[(int64_t)p.x * stride + pos] = p.y;
-        longrow = pos == stride;  // the row just became too long (counted once)
-        added = true;
+        nlong += pos == stride ? 1ull : 0ull;  // the row just became too long (counted once)
+        nadd += 1ull;
     }
-    const uint64_t ml = __ballot(longrow), ma = __ballot(added);
+    for (int off = 32; off >= 1; off >>= 1) {
+        nlong += __shfl_xor(nlong, off);
+        nadd += __shfl_xor(nadd, off);
+    }
     if ((threadIdx.x & 63) == 0) {
-        if (ml) spread_add(spread, 0, (unsigned long long)__popcll(ml));
-        if (ma) spread_add(spread, 1, (unsigned long long)__popcll(ma));
+        if (nlong) spread_add(spread, 0, nlong);
+        if (nadd) spread_add(spread, 1, nadd);
     }
 }
 
@@ -1067,7 +1076,7 @@ hipError_t neighbour_rows_from_candidates_begin(hipStream_t st, int64_t n, int c
                                                  big.hscale, stride, cnt, cnt0, idx, ext, ext_cap, next, spread,
                                                  big.per);
     }
-    k_ext_append<<<sph_blocks(ext_cap, 256), 256, 0, st>>>(ext_cap, ext, next, cnt, idx, stride, spread);
+    k_ext_append<<<SPH_SPREAD * 32, 256, 0, st>>>(ext_cap, ext, next, cnt, idx, stride, spread);
     k_row_tail_sort<<<sph_blocks(n, 256), 256, 0, st>>>(n, cnt0, cnt, idx, stride);
     spread_total(st, 2, spread, totals);
     hipError_t e = hipMemcpyAsync(host_words, totals, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
