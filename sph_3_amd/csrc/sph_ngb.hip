@@ -252,7 +252,7 @@ __global__ void __launch_bounds__(64) k_hwalk(HSolveArgs a)
 
 // Newton (mode 0) or k-th distance (mode 2) solve of the pending targets from their
 // candidate rows, 8 targets per wavefront and 8 lanes per target: lane l of a sub-group
-// holds the squared distances of candidates l, l+8, ... (up to 16 in registers; R2: read
+// holds the squared distances of candidates l, l+8, ... (up to 12 in registers; R2: read
 // from the distance rows, else recomputed from the positions as the walk computed them;
 // longer rows are read again in every evaluation) and the sub-group reduces with DPP lane
 // exchanges, which give
@@ -317,7 +317,7 @@ __device__ __forceinline__ double rcp_refined(double x)
     return fma(y, e, y);
 }
 
-#define HN_SLOTS 16  // candidates per lane held in registers (HN_LANES * HN_SLOTS per target)
+#define HN_SLOTS 12  // candidates per lane held in registers (HN_LANES * HN_SLOTS per target)
 
 template <bool PER, bool R2>
 __global__ void __launch_bounds__(64) k_hnewton(HSolveArgs a)
